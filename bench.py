@@ -1,0 +1,157 @@
+"""LigeSIS commit throughput on one MI355X (Melem/s at mu = 24).
+
+Workload: plain single-polynomial commit at mu = 24, m x n = 2^12 x 2^12, uniform random
+Goldilocks evaluations and A = 32 x (64 * 2^12) uniform Goldilocks from a seed: row NTT/LDE to
+2^12 x 2^13, fused bit-decompose + H = A . B, BLAKE3 Merkle over 2^13 columns. A step is one
+full commit with f already resident in HBM, ending with the 32-byte root on the host.
+
+Prints one JSON line: value = 2^24 / (seconds per commit) / 1e6, the roofline of the dominant
+kernel (timed with HIP events on the library's stream inside the timed region), and the CPU
+baseline (the oracle's single-threaded commit on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+LOG_M = LOG_N = 12
+MU = LOG_M + LOG_N
+HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0
+
+
+def cpu_baseline(seconds_hint: float = 15.0) -> dict:
+    """Oracle commit (single thread, the reference's loop structure) on 2^7 rows x 2^12 columns:
+    the same row length as the workload and 1/32 of its rows. A is then 2 MiB and stays in
+    cache, so the sample runs faster per element than the full workload would (a conservative
+    baseline)."""
+    import oracle
+    from ligesis_amd import synth
+
+    lm, ln = 7, 12
+    inp = synth.Inputs(lm, ln, seed=77, n_fields=4, n_ints=4)
+    pk = oracle.ProvingKey(lm, ln, inp.a, commit_a=False)
+    t0 = time.perf_counter()
+    cd = oracle.CommitData(pk, inp.f)
+    dt = time.perf_counter() - t0
+    del cd
+    return {
+        "value": (1 << (lm + ln)) / dt / 1e6,
+        "unit": "Melem/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle commit of 2^{lm} x 2^{ln} (mu={lm + ln}, the workload's row length and "
+                  f"1/32 of its rows; A fits in cache, so per element this is faster than the full "
+                  f"workload), {dt:.2f} s, one thread",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--hash", type=int, default=0, help="0 auto, 1 MFMA, 2 VALU")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(every box here has one GPU; see DESIGN.md for how the commit would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import ligesis_amd as L
+    from ligesis_amd import synth
+
+    print(f"bench: commit mu={MU} shape 2^{LOG_M} x 2^{LOG_N}", file=sys.stderr, flush=True)
+    inp = synth.Inputs(LOG_M, LOG_N, seed=1, n_fields=4, n_ints=4)
+    ctx = L.Context(0)
+    ctx.set_option("hash", args.hash)
+    pk, _ = L.Ligesis.setup(LOG_M, LOG_N, inp.a, ctx=ctx, commit_only=True)
+    f_dev = torch.from_numpy(inp.f.view(np.int64)).to("cuda:0")
+    torch.cuda.synchronize()
+
+    ctx.set_option("profile", 1)
+    roots = set()
+    for _ in range(args.warmup):
+        r, cd = L.Ligesis.commit(pk, f_dev, device=True)
+        roots.add(r)
+        cd.release()
+    stages = {"ntt_ms": [], "hash_ms": [], "hash_finalize_ms": [], "merkle_ms": [], "commit_ms": []}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r, cd = L.Ligesis.commit(pk, f_dev, device=True)
+        st = ctx.stage_times()
+        for k in stages:
+            stages[k].append(st[k])
+        roots.add(r)
+        cd.release()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    assert len(roots) == 1, "commit is not deterministic"
+    ms = dt / args.steps * 1e3
+    value = (1 << MU) / (ms / 1e3) / 1e6
+    avg = {k: float(np.mean(v)) for k, v in stages.items()}
+
+    m, n = 1 << LOG_M, 1 << LOG_N
+    # algorithmic work per launch: hash = 2^39 int8 MACs = 2^40 ops;
+    # NTT = read f (8 m n bytes) + write F' (16 m n bytes).
+    hash_ops = 2.0 * 256 * (64 * m) * (2 * n)
+    ntt_bytes = 24.0 * m * n
+    hash_tops = hash_ops / (avg["hash_ms"] / 1e3) / 1e12
+    ntt_gbs = ntt_bytes / (avg["ntt_ms"] / 1e3) / 1e9
+    if avg["hash_ms"] >= avg["ntt_ms"]:
+        roof = {"kernel": "hash_mfma_kernel (fused bit-decompose + H = A.B)", "bound": "mfma",
+                "achieved": hash_tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                "frac": hash_tops / INT8_PEAK_TOPS, "traffic": None}
+    else:
+        roof = {"kernel": "ntt_lds_kernel (row LDE)", "bound": "hbm", "achieved": ntt_gbs,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ntt_gbs / HBM_PEAK_GBS,
+                "traffic": None}
+    out = {
+        "metric": "LigeSIS commit Melem/s at mu=24",
+        "value": value,
+        "unit": "Melem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (Goldilocks), int8 MFMA digits",
+        "data": "synthetic (SplitMix64-seeded uniform Goldilocks f and A)",
+        "config": {"workload": "plain single-polynomial commit, mu=24, m x n = 2^12 x 2^12, "
+                               "rho=1/2, eta=64, C=32, BLAKE3 Merkle over 2^13 columns",
+                   "log_m": LOG_M, "log_n": LOG_N, "hash_variant": args.hash},
+        "stages_ms": avg,
+        "roofline": roof,
+        "ntt_roofline": {"bound": "hbm", "achieved": ntt_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ntt_gbs / HBM_PEAK_GBS},
+        "hash_rate_gelem_adds_per_s": (32.0 * 64 * m * 2 * n) / (avg["hash_ms"] / 1e3) / 1e9,
+        "root": next(iter(roots)).hex(),
+    }
+    del pk
+    ctx.close()
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline (oracle, one thread)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline()
+        out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,97 @@
+/*
+ * ligesis_hip.h - C ABI of the MI355X LigeSIS commit / eval-prove library
+ * (ligesis_amd/libligesis_hip.so).
+ *
+ * Drop-in boundary for the reference's LigeSIS API (pcs/src/ligesis.rs:205-756):
+ *   Ligesis::setup(log_m, log_n, rng)      ligesis.rs:239  -> lgs_setup   (A passed in, not sampled)
+ *   Ligesis::commit(&pk, MlPoly f)         ligesis.rs:277  -> lgs_commit, lgs_commit_batch
+ *   Ligesis::prove(&pk, &data, z, oracle)  ligesis.rs:347  -> lgs_prove   (oracle streams passed in)
+ *   CommitData (dropped by the caller)     ligesis.rs:88   -> lgs_commitment_release
+ *   ProvingKey (dropped by the caller)     ligesis.rs:67   -> lgs_key_release
+ * Ligesis::verify (ligesis.rs:569) stays on the CPU with the caller.
+ *
+ * Conventions
+ *  - Every function returns an int status (LGS_OK = 0) except the release functions,
+ *    lgs_last_error and lgs_shutdown. Shape and length violations, which the reference turns
+ *    into assert! panics (ligesis.rs:248, 284, 343, 362), return LGS_ERR_SHAPE / LGS_ERR_ARG;
+ *    the message is in lgs_last_error(ctx).
+ *  - Field elements are canonical little-endian uint64 (< p = 2^64 - 2^32 + 1); an extension
+ *    element of F_p[X]/(X^2 - 7) is two consecutive uint64 (c0, c1).
+ *  - Buffers are caller-owned host memory unless a flag says device memory.
+ *  - One context per host thread; a context owns one HIP stream on one device.
+ */
+#ifndef LIGESIS_HIP_H
+#define LIGESIS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  LGS_OK = 0,
+  LGS_ERR_ARG = 1,    /* bad argument or length (reference: assert_eq! panic) */
+  LGS_ERR_SHAPE = 2,  /* (log_m, log_n) outside the shared-domain window (ligesis.rs:248-251) */
+  LGS_ERR_DEVICE = 3, /* HIP runtime error */
+  LGS_ERR_NOMEM = 4,  /* device allocation failed or the HBM budget is exhausted */
+  LGS_ERR_ORACLE = 5, /* the randomness-oracle stream ran out (reference: index panic) */
+  LGS_ERR_BUFFER = 6, /* output buffer too small; the required size is reported */
+  LGS_ERR_STATE = 7   /* handle cannot do this (e.g. proving with a commit-only key) */
+};
+
+/* lgs_setup flags */
+#define LGS_SETUP_COMMIT_ONLY 1u /* build the hash only: no A-bar commitment, no shape window */
+/* lgs_commit flags */
+#define LGS_INPUT_DEVICE 1u /* f is a device pointer; the commitment borrows it until release */
+
+/* lgs_commitment_read selectors */
+#define LGS_READ_ENCODED 0u /* F' = RS rows, m x 2n uint64 */
+#define LGS_READ_DIGESTS 1u /* H, 32 x 2n uint64 row-major (h_matrix[r][j], ligesis.rs:302) */
+#define LGS_READ_TREE 2u    /* all Merkle layers, (4n - 1) x 32 bytes, leaves first */
+
+typedef struct lgs_ctx lgs_ctx;
+typedef struct lgs_key lgs_key;
+typedef struct lgs_commitment lgs_commitment;
+
+/* Context on HIP device `device` (one stream, default priority). */
+int lgs_init(int device, lgs_ctx** out);
+void lgs_shutdown(lgs_ctx* ctx);
+const char* lgs_last_error(const lgs_ctx* ctx);
+
+/* Tuning knobs that never change results: "hash" (0 auto, 1 MFMA, 2 VALU), "hash_ksplit"
+ * (0 auto), "profile" (1: time every stage with HIP events on the context stream). */
+int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value);
+
+/* Stage times of the last profiled commit, in milliseconds: [0] row NTT, [1] hash main
+ * kernel, [2] hash finalize, [3] Merkle tree, [4] whole commit. Up to `n` values are written. */
+int lgs_stage_times(const lgs_ctx* ctx, double* out_ms, uint32_t n);
+
+/* Setup (ligesis.rs:239): A is the subset-sum matrix, 32 rows x (64 << log_m) canonical
+ * uint64, row-major (subset_sum.rs:23-27). Unless LGS_SETUP_COMMIT_ONLY, the shape must pass
+ * the reference's shared-domain window and A-bar is Basefold-committed on the GPU. */
+int lgs_setup(lgs_ctx* ctx, uint32_t log_m, uint32_t log_n, const uint64_t* a, uint32_t flags,
+              lgs_key** out);
+/* com_a of the key (ligesis.rs:71); LGS_ERR_STATE for a commit-only key. */
+int lgs_key_com_a(const lgs_key* key, uint8_t com_a[32]);
+void lgs_key_release(lgs_key* key);
+
+/* Commit (ligesis.rs:277): f = 2^(log_m + log_n) canonical evaluations, F[i][j] = f[i*n + j].
+ * Writes the 32-byte root and returns a handle that keeps f, F', H and the Merkle tree in HBM
+ * for lgs_prove. */
+int lgs_commit(lgs_ctx* ctx, const lgs_key* key, const uint64_t* f, uint64_t len, uint32_t flags,
+               uint8_t root[32], lgs_commitment** out);
+/* Commit `count` polynomials (each with its own key and length) in one batched pass; roots is
+ * count x 32 bytes. Results are identical to `count` lgs_commit calls. */
+int lgs_commit_batch(lgs_ctx* ctx, uint32_t count, const lgs_key* const* keys,
+                     const uint64_t* const* fs, const uint64_t* lens, uint32_t flags,
+                     uint8_t* roots, lgs_commitment** out);
+void lgs_commitment_release(lgs_commitment* c);
+/* Copy a resident stage (LGS_READ_*) to host memory; cap is in bytes. */
+int lgs_commitment_read(const lgs_commitment* c, uint32_t what, void* out, uint64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIGESIS_HIP_H */

@@ -1,0 +1,263 @@
+"""MI355X LigeSIS: host-side mirror of the reference API over libligesis_hip.so.
+
+The reference API (pcs/src/ligesis.rs:205-756) is::
+
+    Ligesis::setup(log_m, log_n, rng) -> (ProvingKey, VerifyingKey)        ligesis.rs:239
+    Ligesis::commit(&pk, MlPoly f)   -> (LigesisCommit, CommitData)        ligesis.rs:277
+    Ligesis::prove(&pk, &data, z, &mut RandomOracle) -> LigesisProof       ligesis.rs:347
+
+This module keeps those names and argument meanings; the randomness that the reference draws
+inside (`A` in setup, the oracle streams) is passed in explicitly. Shape and length violations
+raise ``LigesisError`` where the reference panics. There is no CPU fallback: every operation
+runs on the GPU through the C ABI in ``include/ligesis_hip.h`` and fails loudly when the
+library or the GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libligesis_hip.so")
+
+ETA = 64
+C = 32
+NUM_COLS_OPENED = 128
+LOG_INTERLEAVE = 6
+
+LGS_OK = 0
+LGS_ERR_ARG = 1
+LGS_ERR_SHAPE = 2
+LGS_ERR_DEVICE = 3
+LGS_ERR_NOMEM = 4
+LGS_ERR_ORACLE = 5
+LGS_ERR_BUFFER = 6
+LGS_ERR_STATE = 7
+
+LGS_SETUP_COMMIT_ONLY = 1
+LGS_INPUT_DEVICE = 1
+LGS_READ_ENCODED, LGS_READ_DIGESTS, LGS_READ_TREE = 0, 1, 2
+
+_lib = None
+
+
+class LigesisError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libligesis_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u32, u64, i64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        L.lgs_init.argtypes = [ctypes.c_int, pp]
+        L.lgs_shutdown.argtypes = [vp]
+        L.lgs_last_error.argtypes = [vp]
+        L.lgs_last_error.restype = ctypes.c_char_p
+        L.lgs_set_option.argtypes = [vp, ctypes.c_char_p, i64]
+        L.lgs_stage_times.argtypes = [vp, vp, u32]
+        L.lgs_setup.argtypes = [vp, u32, u32, vp, u32, pp]
+        L.lgs_key_com_a.argtypes = [vp, vp]
+        L.lgs_key_release.argtypes = [vp]
+        L.lgs_commit.argtypes = [vp, vp, vp, u64, u32, vp, pp]
+        L.lgs_commit_batch.argtypes = [vp, u32, vp, vp, vp, u32, vp, vp]
+        L.lgs_commitment_release.argtypes = [vp]
+        L.lgs_commitment_read.argtypes = [vp, u32, vp, u64]
+        for name in ("lgs_prove",):
+            if hasattr(L, name):
+                getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _u64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
+
+
+class Context:
+    """One HIP device + stream (``lgs_init``)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        rc = lib().lgs_init(device, ctypes.byref(h))
+        if rc != LGS_OK:
+            raise LigesisError(rc, f"lgs_init(device={device}) failed")
+        self.h = h
+
+    def check(self, rc: int):
+        if rc != LGS_OK:
+            raise LigesisError(rc, lib().lgs_last_error(self.h).decode())
+
+    def set_option(self, name: str, value: int):
+        self.check(lib().lgs_set_option(self.h, name.encode(), int(value)))
+
+    def stage_times(self) -> dict:
+        out = np.zeros(5, np.float64)
+        self.check(lib().lgs_stage_times(self.h, _ptr(out), 5))
+        return {"ntt_ms": out[0], "hash_ms": out[1], "hash_finalize_ms": out[2],
+                "merkle_ms": out[3], "commit_ms": out[4]}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lgs_shutdown(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class VerifyingKey:
+    """``VerifyingKey`` (ligesis.rs:77-83)."""
+
+    def __init__(self, com_a: bytes, log_m: int, log_n: int):
+        self.com_a, self.log_m, self.log_n = com_a, log_m, log_n
+        self.width = ETA << log_m
+
+
+class ProvingKey:
+    """``ProvingKey`` (ligesis.rs:67-74): the hash matrix A and the A-bar Basefold state, in HBM."""
+
+    def __init__(self, ctx: Context, log_m: int, log_n: int, a: np.ndarray, commit_only: bool = False):
+        a = _u64(a)
+        if a.size != C * (ETA << log_m):
+            raise LigesisError(LGS_ERR_ARG, f"A must have {C} x {ETA << log_m} entries")
+        h = ctypes.c_void_p()
+        ctx.check(lib().lgs_setup(ctx.h, log_m, log_n, _ptr(a),
+                                  LGS_SETUP_COMMIT_ONLY if commit_only else 0, ctypes.byref(h)))
+        self.ctx, self.h = ctx, h
+        self.log_m, self.log_n = log_m, log_n
+        self.commit_only = commit_only
+
+    def com_a(self) -> bytes:
+        out = np.zeros(32, np.uint8)
+        self.ctx.check(lib().lgs_key_com_a(self.h, _ptr(out)))
+        return out.tobytes()
+
+    def release(self):
+        if getattr(self, "h", None):
+            lib().lgs_key_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.release()
+
+
+class CommitData:
+    """``CommitData`` (ligesis.rs:88-97): f, F', H and the Merkle tree, resident in HBM."""
+
+    def __init__(self, ctx: Context, key: ProvingKey, h: ctypes.c_void_p, root: bytes, keepalive=None):
+        self.ctx, self.key, self.h, self.root = ctx, key, h, root
+        self._keepalive = keepalive
+
+    def _read(self, what: int, shape) -> np.ndarray:
+        out = np.zeros(shape, np.uint64)
+        self.ctx.check(lib().lgs_commitment_read(self.h, what, _ptr(out), out.nbytes))
+        return out
+
+    def encoded(self) -> np.ndarray:
+        m, n = 1 << self.key.log_m, 1 << self.key.log_n
+        return self._read(LGS_READ_ENCODED, (m, 2 * n))
+
+    def digests(self) -> np.ndarray:
+        return self._read(LGS_READ_DIGESTS, (C, 2 << self.key.log_n))
+
+    def tree(self) -> bytes:
+        n2 = 2 << self.key.log_n
+        out = np.zeros((2 * n2 - 1) * 32, np.uint8)
+        self.ctx.check(lib().lgs_commitment_read(self.h, LGS_READ_TREE, _ptr(out), out.nbytes))
+        return out.tobytes()
+
+    def release(self):
+        if getattr(self, "h", None):
+            lib().lgs_commitment_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.release()
+
+
+class RandomOracle:
+    """``RandomOracle`` (utils/src/oracle.rs:10-64) with its pre-drawn streams given explicitly:
+    ``fields`` is (k, 2) uint64 extension elements, ``ints`` is uint64."""
+
+    def __init__(self, fields: np.ndarray, ints: np.ndarray):
+        self.fields = _u64(fields).reshape(-1, 2)
+        self.ints = _u64(ints)
+        self.fields_idx = 0
+        self.ints_idx = 0
+
+    def restart(self):
+        self.fields_idx = self.ints_idx = 0
+
+    def drawn(self):
+        return (self.fields_idx, self.ints_idx)
+
+
+class Ligesis:
+    """The reference's ``Ligesis<Goldilocks, BinomialExtensionField<Goldilocks, 2>>``."""
+
+    @staticmethod
+    def setup(log_m: int, log_n: int, a: np.ndarray, ctx: Context | None = None,
+              commit_only: bool = False):
+        ctx = ctx or default_context()
+        pk = ProvingKey(ctx, log_m, log_n, a, commit_only=commit_only)
+        vk = VerifyingKey(b"" if commit_only else pk.com_a(), log_m, log_n)
+        return pk, vk
+
+    @staticmethod
+    def commit(pk: ProvingKey, f, device: bool = False):
+        """Returns (root bytes, CommitData). With ``device=True``, ``f`` is a torch tensor on the
+        GPU (uint64/int64 storage) that the commitment borrows."""
+        ctx = pk.ctx
+        n = 1 << (pk.log_m + pk.log_n)
+        h = ctypes.c_void_p()
+        root = np.zeros(32, np.uint8)
+        if device:
+            ptr, length, keep = ctypes.c_void_p(f.data_ptr()), f.numel(), f
+            flags = LGS_INPUT_DEVICE
+        else:
+            fa = _u64(f)
+            ptr, length, keep, flags = _ptr(fa), fa.size, None, 0
+        if length != n:
+            raise LigesisError(LGS_ERR_ARG, f"commit: expected {n} evaluations, got {length}")
+        ctx.check(lib().lgs_commit(ctx.h, pk.h, ptr, length, flags, _ptr(root), ctypes.byref(h)))
+        return root.tobytes(), CommitData(ctx, pk, h, root.tobytes(), keep)
+
+    @staticmethod
+    def commit_batch(keys, fs, ctx: Context | None = None):
+        ctx = ctx or (keys[0].ctx if keys else default_context())
+        count = len(keys)
+        arrs = [_u64(f) for f in fs]
+        kp = (ctypes.c_void_p * count)(*[k.h.value for k in keys])
+        fp = (ctypes.c_void_p * count)(*[a.ctypes.data for a in arrs])
+        lens = np.array([a.size for a in arrs], np.uint64)
+        roots = np.zeros((count, 32), np.uint8)
+        outs = (ctypes.c_void_p * count)()
+        ctx.check(lib().lgs_commit_batch(ctx.h, count, kp, fp, _ptr(lens), 0, _ptr(roots), outs))
+        res = []
+        for i in range(count):
+            r = roots[i].tobytes()
+            res.append((r, CommitData(ctx, keys[i], ctypes.c_void_p(outs[i]), r)))
+        return res

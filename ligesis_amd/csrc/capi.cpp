@@ -1,0 +1,276 @@
+// extern "C" boundary of libligesis_hip.so (declared in include/ligesis_hip.h).
+#include "../../include/ligesis_hip.h"
+
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "runtime.hpp"
+
+struct lgs_ctx {
+  lgs::Context c;
+};
+struct lgs_key {
+  lgs::Key k;
+};
+struct lgs_commitment {
+  lgs::Commitment c;
+};
+
+namespace {
+
+template <class F>
+int guarded(lgs::Context* ctx, F&& f) {
+  try {
+    f();
+    return LGS_OK;
+  } catch (const lgs::Error& e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (ctx) ctx->err = "host allocation failed";
+    return LGS_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return LGS_ERR_ARG;
+  }
+}
+
+void finish_profile(lgs::Context& ctx) {
+  if (!ctx.profile) return;
+  for (int i = 0; i < 4; i++) {
+    float ms = 0;
+    lgs::hip_check(hipEventElapsedTime(&ms, ctx.ev[i], ctx.ev[i + 1]), "event time");
+    ctx.stage_ms[i] = ms;
+  }
+  float all = 0;
+  lgs::hip_check(hipEventElapsedTime(&all, ctx.ev[0], ctx.ev[4]), "event time");
+  ctx.stage_ms[4] = all;
+}
+
+void read_root(lgs::Context& ctx, lgs::Commitment* const* cs, uint32_t count) {
+  uint8_t* h = ctx.staging(32 * (size_t)count);
+  for (uint32_t i = 0; i < count; i++) {
+    const lgs::DevTree& t = cs[i]->tree;
+    lgs::hip_check(hipMemcpyAsync(h + 32 * (size_t)i, t.nodes.ptr + (t.node_count() - 1) * 8, 32,
+                                  hipMemcpyDeviceToHost, ctx.stream),
+                   "read root");
+  }
+  lgs::hip_check(hipStreamSynchronize(ctx.stream), "commit sync");
+  for (uint32_t i = 0; i < count; i++) std::memcpy(cs[i]->root, h + 32 * (size_t)i, 32);
+}
+
+void prepare_commitment(lgs::Context& ctx, const lgs::Key& key, const uint64_t* f, uint64_t len,
+                        uint32_t flags, lgs::Commitment& c) {
+  uint64_t need = uint64_t(1) << (key.log_m + key.log_n);
+  if (!f) throw lgs::Error(LGS_ERR_ARG, "commit: null polynomial");
+  if (len != need)
+    throw lgs::Error(LGS_ERR_ARG, "commit: expected 2^(log_m+log_n) = " + std::to_string(need) +
+                                      " evaluations, got " + std::to_string(len));
+  c.ctx = &ctx;
+  c.key = &key;
+  if (flags & LGS_INPUT_DEVICE) {
+    c.f = f;
+  } else {
+    c.f_owned = lgs::DevBuf<uint64_t>(&ctx.pool, len);
+    lgs::hip_check(hipMemcpyAsync(c.f_owned.ptr, f, len * 8, hipMemcpyHostToDevice, ctx.stream),
+                   "upload f");
+    c.f = c.f_owned.ptr;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int lgs_init(int device, lgs_ctx** out) {
+  if (!out) return LGS_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return LGS_ERR_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return LGS_ERR_DEVICE;
+  auto* ctx = new lgs_ctx();
+  ctx->c.device = device;
+  if (hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return LGS_ERR_DEVICE;
+  }
+  for (auto& e : ctx->c.ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete ctx;
+      return LGS_ERR_DEVICE;
+    }
+  *out = ctx;
+  return LGS_OK;
+}
+
+void lgs_shutdown(lgs_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  (void)hipStreamSynchronize(ctx->c.stream);
+  ctx->c.twiddles.clear();
+  ctx->c.pool.trim();
+  for (auto& e : ctx->c.ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
+  (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+}
+
+const char* lgs_last_error(const lgs_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value) {
+  if (!ctx || !name) return LGS_ERR_ARG;
+  std::string n(name);
+  if (n == "hash" && value >= 0 && value <= 2) {
+    ctx->c.hash_variant = (int)value;
+  } else if (n == "hash_ksplit" && value >= 0) {
+    ctx->c.hash_ksplit = (int)value;
+  } else if (n == "profile") {
+    ctx->c.profile = value != 0;
+  } else {
+    ctx->c.err = "unknown option or value: " + n;
+    return LGS_ERR_ARG;
+  }
+  return LGS_OK;
+}
+
+int lgs_stage_times(const lgs_ctx* ctx, double* out_ms, uint32_t n) {
+  if (!ctx || !out_ms) return LGS_ERR_ARG;
+  for (uint32_t i = 0; i < n && i < 8; i++) out_ms[i] = ctx->c.stage_ms[i];
+  return LGS_OK;
+}
+
+int lgs_setup(lgs_ctx* ctx, uint32_t log_m, uint32_t log_n, const uint64_t* a, uint32_t flags,
+              lgs_key** out) {
+  if (!ctx || !out || !a) return LGS_ERR_ARG;
+  *out = nullptr;
+  return guarded(&ctx->c, [&] {
+    bool commit_only = (flags & LGS_SETUP_COMMIT_ONLY) != 0;
+    if (log_m > 20 || log_n > 26 || log_n < 1)
+      throw lgs::Error(LGS_ERR_ARG, "setup: shape out of supported range");
+    if (!commit_only && !lgs::shape_ok(log_m, log_n))
+      throw lgs::Error(LGS_ERR_SHAPE,
+                       "batched LigeSIS needs the {A-bar, B_I, a-bar} var counts within "
+                       "LOG_INTERLEAVE of each other");
+    (void)hipSetDevice(ctx->c.device);
+    auto* k = new lgs_key();
+    k->k.log_m = log_m;
+    k->k.log_n = log_n;
+    try {
+      lgs::setup_key(ctx->c, k->k, a, !commit_only);
+    } catch (...) {
+      delete k;
+      throw;
+    }
+    *out = k;
+  });
+}
+
+int lgs_key_com_a(const lgs_key* key, uint8_t com_a[32]) {
+  if (!key || !com_a) return LGS_ERR_ARG;
+  if (!key->k.has_state) return LGS_ERR_STATE;
+  std::memcpy(com_a, key->k.com_a, 32);
+  return LGS_OK;
+}
+
+void lgs_key_release(lgs_key* key) {
+  if (!key) return;
+  if (key->k.ctx) (void)hipStreamSynchronize(key->k.ctx->stream);
+  delete key;
+}
+
+int lgs_commit(lgs_ctx* ctx, const lgs_key* key, const uint64_t* f, uint64_t len, uint32_t flags,
+               uint8_t root[32], lgs_commitment** out) {
+  if (!ctx || !key || !root || !out) return LGS_ERR_ARG;
+  *out = nullptr;
+  return guarded(&ctx->c, [&] {
+    (void)hipSetDevice(ctx->c.device);
+    auto* c = new lgs_commitment();
+    try {
+      prepare_commitment(ctx->c, key->k, f, len, flags, c->c);
+      lgs::enqueue_commit(ctx->c, key->k, c->c);
+      lgs::Commitment* cp = &c->c;
+      read_root(ctx->c, &cp, 1);
+      finish_profile(ctx->c);
+    } catch (...) {
+      (void)hipStreamSynchronize(ctx->c.stream);
+      delete c;
+      throw;
+    }
+    std::memcpy(root, c->c.root, 32);
+    *out = c;
+  });
+}
+
+int lgs_commit_batch(lgs_ctx* ctx, uint32_t count, const lgs_key* const* keys,
+                     const uint64_t* const* fs, const uint64_t* lens, uint32_t flags,
+                     uint8_t* roots, lgs_commitment** out) {
+  if (!ctx || !keys || !fs || !lens || !roots || !out) return LGS_ERR_ARG;
+  for (uint32_t i = 0; i < count; i++) out[i] = nullptr;
+  std::vector<lgs_commitment*> made;
+  int rc = guarded(&ctx->c, [&] {
+    (void)hipSetDevice(ctx->c.device);
+    std::vector<lgs::Commitment*> cs;
+    bool prof = ctx->c.profile;
+    ctx->c.profile = false;
+    try {
+      for (uint32_t i = 0; i < count; i++) {
+        if (!keys[i]) throw lgs::Error(LGS_ERR_ARG, "commit_batch: null key");
+        auto* c = new lgs_commitment();
+        made.push_back(c);
+        prepare_commitment(ctx->c, keys[i]->k, fs[i], lens[i], flags, c->c);
+        lgs::enqueue_commit(ctx->c, keys[i]->k, c->c);
+        cs.push_back(&c->c);
+      }
+      read_root(ctx->c, cs.data(), count);
+    } catch (...) {
+      ctx->c.profile = prof;
+      throw;
+    }
+    ctx->c.profile = prof;
+  });
+  if (rc != LGS_OK) {
+    (void)hipStreamSynchronize(ctx->c.stream);
+    for (auto* c : made) delete c;
+    return rc;
+  }
+  for (uint32_t i = 0; i < count; i++) {
+    std::memcpy(roots + 32 * (size_t)i, made[i]->c.root, 32);
+    out[i] = made[i];
+  }
+  return LGS_OK;
+}
+
+void lgs_commitment_release(lgs_commitment* c) {
+  if (!c) return;
+  if (c->c.ctx) (void)hipStreamSynchronize(c->c.ctx->stream);
+  delete c;
+}
+
+int lgs_commitment_read(const lgs_commitment* c, uint32_t what, void* out, uint64_t cap) {
+  if (!c || !out) return LGS_ERR_ARG;
+  lgs::Context* ctx = c->c.ctx;
+  return guarded(ctx, [&] {
+    const void* src = nullptr;
+    uint64_t bytes = 0;
+    uint64_t m = uint64_t(1) << c->c.log_m, two_n = uint64_t(2) << c->c.log_n;
+    if (what == LGS_READ_ENCODED) {
+      src = c->c.enc.ptr;
+      bytes = m * two_n * 8;
+    } else if (what == LGS_READ_DIGESTS) {
+      src = c->c.h.ptr;
+      bytes = 32 * two_n * 8;
+    } else if (what == LGS_READ_TREE) {
+      src = c->c.tree.nodes.ptr;
+      bytes = c->c.tree.node_count() * 32;
+    } else {
+      throw lgs::Error(LGS_ERR_ARG, "commitment_read: unknown selector");
+    }
+    if (cap < bytes) throw lgs::Error(LGS_ERR_BUFFER, "commitment_read: need " + std::to_string(bytes) + " bytes");
+    lgs::hip_check(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "read");
+    lgs::hip_check(hipStreamSynchronize(ctx->stream), "read sync");
+  });
+}
+
+}  // extern "C"

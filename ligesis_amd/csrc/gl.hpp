@@ -1,0 +1,96 @@
+// Goldilocks (p = 2^64 - 2^32 + 1) and F_{p^2} = F_p[X]/(X^2 - 7) arithmetic for host and
+// device code. Values are kept canonical (< p) after every operation, which is what the
+// reference serializes (utils/src/merkle.rs:24-29) and compares.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#define LGS_HD __host__ __device__ __forceinline__
+
+namespace lgs {
+
+constexpr uint64_t GL_P = 0xffffffff00000001ull;
+constexpr uint64_t GL_EPS = 0xffffffffull;  // 2^64 mod p
+constexpr uint64_t GL_G32 = 1753635133440165772ull;  // generator of the order-2^32 subgroup
+constexpr uint64_t GL_INV2 = 0x7fffffff80000001ull;  // (p + 1) / 2
+
+LGS_HD uint64_t gl_canon(uint64_t x) { return x >= GL_P ? x - GL_P : x; }
+
+LGS_HD uint64_t gl_add(uint64_t a, uint64_t b) {
+  uint64_t s = a + b;
+  s += (s < a) ? GL_EPS : 0;  // a carry out of 2^64 is worth 2^32 - 1
+  return gl_canon(s);
+}
+
+LGS_HD uint64_t gl_sub(uint64_t a, uint64_t b) {
+  uint64_t d = a - b;
+  return (a < b) ? d - GL_EPS : d;  // a - b + 2^64 - (2^64 - p)
+}
+
+LGS_HD uint64_t gl_neg(uint64_t a) { return a ? GL_P - a : 0; }
+
+// (hi * 2^64 + lo) mod p, using 2^64 = 2^32 - 1 and 2^96 = -1.
+LGS_HD uint64_t gl_reduce(uint64_t lo, uint64_t hi) {
+  uint64_t hh = hi >> 32, hl = hi & GL_EPS;
+  uint64_t t0 = lo - hh;
+  t0 -= (lo < hh) ? GL_EPS : 0;
+  uint64_t t1 = (hl << 32) - hl;
+  uint64_t r = t0 + t1;
+  r += (r < t1) ? GL_EPS : 0;
+  return gl_canon(r);
+}
+
+LGS_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __umul64hi(a, b);
+#else
+  return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+LGS_HD uint64_t gl_mul(uint64_t a, uint64_t b) { return gl_reduce(a * b, mulhi64(a, b)); }
+
+LGS_HD uint64_t gl_pow(uint64_t a, uint64_t e) {
+  uint64_t r = 1;
+  while (e) {
+    if (e & 1) r = gl_mul(r, a);
+    a = gl_mul(a, a);
+    e >>= 1;
+  }
+  return r;
+}
+
+LGS_HD uint64_t gl_inv(uint64_t a) { return gl_pow(a, GL_P - 2); }
+
+// Generator of the order-2^bits subgroup (p3 TwoAdicField::two_adic_generator).
+LGS_HD uint64_t gl_root(int bits) {
+  uint64_t g = GL_G32;
+  for (int i = bits; i < 32; i++) g = gl_mul(g, g);
+  return g;
+}
+
+struct ef2 {
+  uint64_t c0, c1;
+};
+
+LGS_HD ef2 ef_make(uint64_t a, uint64_t b) { return ef2{a, b}; }
+LGS_HD ef2 ef_base(uint64_t a) { return ef2{a, 0}; }
+LGS_HD ef2 ef_add(ef2 a, ef2 b) { return ef2{gl_add(a.c0, b.c0), gl_add(a.c1, b.c1)}; }
+LGS_HD ef2 ef_sub(ef2 a, ef2 b) { return ef2{gl_sub(a.c0, b.c0), gl_sub(a.c1, b.c1)}; }
+LGS_HD ef2 ef_neg(ef2 a) { return ef2{gl_neg(a.c0), gl_neg(a.c1)}; }
+LGS_HD bool ef_eq(ef2 a, ef2 b) { return a.c0 == b.c0 && a.c1 == b.c1; }
+LGS_HD ef2 ef_mul(ef2 a, ef2 b) {
+  uint64_t t = gl_mul(a.c1, b.c1);
+  uint64_t t7 = gl_mul(t, 7);
+  return ef2{gl_add(gl_mul(a.c0, b.c0), t7), gl_add(gl_mul(a.c0, b.c1), gl_mul(a.c1, b.c0))};
+}
+LGS_HD ef2 ef_mul_base(ef2 a, uint64_t b) { return ef2{gl_mul(a.c0, b), gl_mul(a.c1, b)}; }
+LGS_HD ef2 ef_inv(ef2 a) {
+  uint64_t norm = gl_sub(gl_mul(a.c0, a.c0), gl_mul(7, gl_mul(a.c1, a.c1)));
+  uint64_t ni = gl_inv(norm);
+  return ef2{gl_mul(a.c0, ni), gl_mul(gl_neg(a.c1), ni)};
+}
+LGS_HD ef2 ef_one() { return ef2{1, 0}; }
+LGS_HD ef2 ef_zero() { return ef2{0, 0}; }
+
+}  // namespace lgs

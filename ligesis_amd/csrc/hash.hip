@@ -1,0 +1,235 @@
+// Fused bit-decompose + subset-sum hash H = A . B (pcs/src/ligesis.rs:288-302,
+// pcs/src/subset_sum.rs:43-67): H[r][j] = sum over i < W with bit (i mod 64) of F'[i / 64][j]
+// set, of A[r][i], mod p. B (W x 2n bits) is never materialised: its bits are read straight out
+// of the encoded rows F' in registers.
+//
+// Two exact kernels:
+//  * VALU: one lane per column; A rows are wave-uniform (scalar loads from the transposed copy
+//    At[i][r]); 96-bit accumulation without reduction, one reduction per output.
+//  * MFMA: A is split at setup into 8 signed base-256 digits, d_l in [-128, 127] with
+//    sum_l d_l 256^l = A (mod p) (A itself when A <= 0x7F7F7F7F7F7F7F7F, else A - p). Then
+//    S_l = D_l . B is an exact int8 x {0,1} product on v_mfma_i32_16x16x64_i8 with int32 sums
+//    (|S_l| <= 128 W < 2^31), and H = sum_l 256^l S_l mod p. M = 8 digits x 32 rows = 256.
+// Both write per-K-split partials that a finalize kernel sums mod p.
+#include "kernels.hpp"
+
+namespace lgs {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// ---------------- VALU ----------------
+// grid (ncols / 256, ksplit); at = A transposed, W x 32.
+__global__ __launch_bounds__(256) void hash_valu_kernel(const uint64_t* __restrict__ enc,
+                                                        uint64_t ld, const uint64_t* __restrict__ at,
+                                                        uint32_t rows_per_split, uint32_t ncols,
+                                                        uint64_t* __restrict__ part) {
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t split = blockIdx.y;
+  const uint32_t i0 = split * rows_per_split;
+  uint64_t lo[32], hi[32];
+#pragma unroll
+  for (int r = 0; r < 32; r++) lo[r] = hi[r] = 0;
+  const bool live = j < ncols;
+  for (uint32_t i = i0; i < i0 + rows_per_split; i++) {
+    const uint64_t x = live ? enc[(uint64_t)i * ld + j] : 0;
+    const uint64_t* arow = at + (uint64_t)i * 64 * 32;
+    for (int k = 0; k < 64; k++) {
+      const uint64_t mask = 0 - ((x >> k) & 1);
+      const uint64_t* a = arow + k * 32;
+#pragma unroll
+      for (int r = 0; r < 32; r++) {
+        const uint64_t v = a[r];
+        lo[r] += (v & 0xffffffffull) & mask;
+        hi[r] += (v >> 32) & mask;
+      }
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int r = 0; r < 32; r++) {
+    // value = lo + hi * 2^32 (< 2^96)
+    uint64_t sh = hi[r] << 32;
+    uint64_t L = lo[r] + sh;
+    uint64_t H = (hi[r] >> 32) + (L < sh ? 1 : 0);
+    part[((uint64_t)split * 32 + r) * ncols + j] = gl_reduce(L, H);
+  }
+}
+
+__global__ void hash_valu_finalize_kernel(const uint64_t* __restrict__ part, uint32_t ksplit,
+                                          uint32_t ncols, uint64_t* __restrict__ h) {
+  uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (uint64_t)32 * ncols) return;
+  uint64_t acc = 0;
+  for (uint32_t s = 0; s < ksplit; s++) acc = gl_add(acc, part[(uint64_t)s * 32 * ncols + idx]);
+  h[idx] = acc;
+}
+
+// ---------------- MFMA ----------------
+// Digit layout (prepared at setup): for F' row i ("slab", 64 values of the K index), M-tile mt
+// (rows m = 16 mt .. 16 mt + 15, m = 32 l + r for digit l and hash row r), lane L = 16 g + c:
+// 16 bytes at ((i * 16 + mt) * 64 + L) * 16, byte 4 s + t = digit l of A[r][64 i + kb(g) + s + 8 t]
+// with r = 16 (mt & 1) + c, l = mt >> 1, kb(g) = 32 (g >> 1) + 4 (g & 1). The B operand of lane
+// L holds the same (s, t) bits of F'[i][col] shifted right by kb(g), so A and B agree on the
+// K order inside every lane whatever order the instruction assigns to the 16 bytes.
+constexpr int HM_NT = 128;       // columns per workgroup
+constexpr int HM_SPS = 2;        // slabs staged in LDS per step (16 KiB each)
+
+__global__ __launch_bounds__(256) void hash_mfma_kernel(const uint64_t* __restrict__ enc, uint64_t ld,
+                                                        const v4i* __restrict__ adig,
+                                                        uint32_t slabs_per_split, uint32_t ntiles,
+                                                        uint32_t ncols, int32_t* __restrict__ part) {
+  __shared__ v4i lds_a[HM_SPS * 1024];
+  // XCD-aware mapping: consecutive block ids land on different XCDs; give each XCD a
+  // contiguous run of logical blocks so the blocks sharing a K-split share an L2.
+  const uint32_t nb = gridDim.x;
+  const uint32_t per_xcd = nb / 8;
+  const uint32_t bid = blockIdx.x;
+  const uint32_t logical = (nb % 8 == 0) ? (bid % 8) * per_xcd + bid / 8 : bid;
+  const uint32_t split = logical / ntiles;
+  const uint32_t ntile = logical % ntiles;
+
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t wm = wave >> 1, wn = wave & 1;
+  const uint32_t g = lane >> 4, c = lane & 15;
+  const uint32_t kb = 32 * (g >> 1) + 4 * (g & 1);
+  const uint32_t col_base = ntile * HM_NT + wn * 64 + c;
+
+  v4i acc[8][4];
+#pragma unroll
+  for (int a = 0; a < 8; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = v4i{0, 0, 0, 0};
+
+  const uint32_t slab0 = split * slabs_per_split;
+  for (uint32_t s0 = 0; s0 < slabs_per_split; s0 += HM_SPS) {
+    __syncthreads();
+    const v4i* src = adig + (uint64_t)(slab0 + s0) * 1024;
+#pragma unroll
+    for (int q = 0; q < HM_SPS * 4; q++) lds_a[q * 256 + threadIdx.x] = src[q * 256 + threadIdx.x];
+    __syncthreads();
+#pragma unroll
+    for (int ss = 0; ss < HM_SPS; ss++) {
+      const uint64_t i = slab0 + s0 + ss;
+      v4i b[4];
+#pragma unroll
+      for (int nt = 0; nt < 4; nt++) {
+        const uint64_t x = enc[i * ld + col_base + nt * 16];
+        const uint32_t y = (uint32_t)(x >> kb);
+        b[nt] = v4i{(int)(y & 0x01010101u), (int)((y >> 1) & 0x01010101u),
+                    (int)((y >> 2) & 0x01010101u), (int)((y >> 3) & 0x01010101u)};
+      }
+#pragma unroll
+      for (int mt = 0; mt < 8; mt++) {
+        const v4i a = lds_a[ss * 1024 + (wm * 8 + mt) * 64 + lane];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+          acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[nt], acc[mt][nt], 0, 0, 0);
+      }
+    }
+  }
+  // D[row][col] of a 16x16 tile: lane L, register v -> row 4 (L >> 4) + v, col L & 15.
+#pragma unroll
+  for (int mt = 0; mt < 8; mt++) {
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const uint32_t m = (wm * 8 + mt) * 16 + 4 * g + v;
+        const uint32_t col = ntile * HM_NT + wn * 64 + nt * 16 + c;
+        part[((uint64_t)split * 256 + m) * ncols + col] = acc[mt][nt][v];
+      }
+    }
+  }
+}
+
+// H[r][j] = sum_l 256^l (sum_split part[split][32 l + r][j]) mod p.
+__global__ void hash_mfma_finalize_kernel(const int32_t* __restrict__ part, uint32_t ksplit,
+                                          uint32_t ncols, uint64_t* __restrict__ h) {
+  uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (uint64_t)32 * ncols) return;
+  const uint32_t r = (uint32_t)(idx / ncols), j = (uint32_t)(idx % ncols);
+  uint64_t acc = 0;
+#pragma unroll
+  for (int l = 0; l < 8; l++) {
+    int64_t s = 0;
+    for (uint32_t sp = 0; sp < ksplit; sp++) s += part[((uint64_t)sp * 256 + 32 * l + r) * ncols + j];
+    uint64_t sm = s >= 0 ? (uint64_t)s : gl_neg((uint64_t)(-s));
+    acc = gl_add(acc, gl_mul(sm, (uint64_t)1 << (8 * l)));
+  }
+  h[idx] = acc;
+}
+
+// Setup-time layout transforms of A (C = 32 rows x W, row-major, canonical).
+__global__ void a_transpose_kernel(const uint64_t* __restrict__ a, uint64_t W, uint64_t* __restrict__ at) {
+  uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 32 * W) return;
+  uint64_t r = idx / W, i = idx % W;
+  at[i * 32 + r] = a[idx];
+}
+
+__global__ void a_digits_kernel(const uint64_t* __restrict__ a, uint64_t W, int8_t* __restrict__ dig) {
+  // one thread per output byte position (slab, mt, lane, byte)
+  uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 256 * W) return;
+  const uint32_t byte = idx & 15;
+  const uint32_t lane = (idx >> 4) & 63;
+  const uint32_t mt = (idx >> 10) & 15;
+  const uint64_t slab = idx >> 14;
+  const uint32_t g = lane >> 4, c = lane & 15;
+  const uint32_t s = byte >> 2, t = byte & 3;
+  const uint32_t r = 16 * (mt & 1) + c, l = mt >> 1;
+  const uint64_t i = 64 * slab + 32 * (g >> 1) + 4 * (g & 1) + s + 8 * t;
+  uint64_t v = a[(uint64_t)r * W + i];
+  // balanced base-256 digits of v or v - p
+  int64_t x = v <= 0x7F7F7F7F7F7F7F7Full ? (int64_t)v : -(int64_t)(GL_P - v);
+  int d = 0;
+  for (uint32_t q = 0; q <= l; q++) {
+    d = (int)(x & 0xff);
+    if (d >= 128) d -= 256;
+    x = (x - d) >> 8;
+  }
+  dig[idx] = (int8_t)d;
+}
+
+hipError_t launch_a_layouts(const uint64_t* a, uint64_t W, uint64_t* at, int8_t* dig, hipStream_t st) {
+  uint64_t n = 32 * W;
+  hipLaunchKernelGGL(a_transpose_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, a, W, at);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || dig == nullptr) return e;
+  uint64_t nd = 256 * W;
+  hipLaunchKernelGGL(a_digits_kernel, dim3((uint32_t)((nd + 255) / 256)), dim3(256), 0, st, a, W, dig);
+  return hipGetLastError();
+}
+
+hipError_t launch_hash_valu(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
+                            const uint64_t* at, uint32_t ksplit, uint64_t* part, uint64_t* h,
+                            hipStream_t st, hipEvent_t mid) {
+  uint32_t rps = m_rows / ksplit;
+  hipLaunchKernelGGL(hash_valu_kernel, dim3((ncols + 255) / 256, ksplit), dim3(256), 0, st, enc, ld,
+                     at, rps, ncols, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (mid && (e = hipEventRecord(mid, st)) != hipSuccess) return e;
+  uint64_t n = (uint64_t)32 * ncols;
+  hipLaunchKernelGGL(hash_valu_finalize_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
+                     part, ksplit, ncols, h);
+  return hipGetLastError();
+}
+
+hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
+                            const int8_t* dig, uint32_t ksplit, int32_t* part, uint64_t* h,
+                            hipStream_t st, hipEvent_t mid) {
+  uint32_t ntiles = ncols / HM_NT;
+  uint32_t sps = m_rows / ksplit;
+  hipLaunchKernelGGL(hash_mfma_kernel, dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
+                     (const v4i*)dig, sps, ntiles, ncols, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (mid && (e = hipEventRecord(mid, st)) != hipSuccess) return e;
+  uint64_t n = (uint64_t)32 * ncols;
+  hipLaunchKernelGGL(hash_mfma_finalize_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
+                     part, ksplit, ncols, h);
+  return hipGetLastError();
+}
+
+}  // namespace lgs

@@ -1,0 +1,30 @@
+// Host-side launchers of the device kernels (all enqueue on the given stream, no sync).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "gl.hpp"
+
+#define LGS_LDS_LOG 13  // transforms up to 2^13 elements (64 KiB) run wholly in LDS
+
+namespace lgs {
+
+// ntt.hip
+hipError_t launch_twiddles(uint64_t* tw, int logN, hipStream_t st);
+hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_in, uint32_t rows,
+                      int logN, const uint64_t* tw, uint64_t* out, hipStream_t st);
+
+// hash.hip (`mid`, when not null, is recorded between the main kernel and the finalize)
+hipError_t launch_a_layouts(const uint64_t* a, uint64_t W, uint64_t* at, int8_t* dig, hipStream_t st);
+hipError_t launch_hash_valu(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
+                            const uint64_t* at, uint32_t ksplit, uint64_t* part, uint64_t* h,
+                            hipStream_t st, hipEvent_t mid);
+hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
+                            const int8_t* dig, uint32_t ksplit, int32_t* part, uint64_t* h,
+                            hipStream_t st, hipEvent_t mid);
+
+// merkle.hip: tree over nleaves (a power of two) leaves, leaf p = M[r*rs + p*cs], r < rows.
+hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs,
+                         uint64_t nleaves, uint32_t* nodes, hipStream_t st);
+
+}  // namespace lgs

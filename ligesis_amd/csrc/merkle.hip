@@ -1,0 +1,105 @@
+// BLAKE3 column-Merkle trees with the semantics of utils/src/merkle.rs:49-74 (rs_merkle over
+// pre-hashed leaves): leaf p = blake3(serialize(column p)), parent = blake3(left || right).
+//
+// Tree storage: all layers of digests (8 x u32, little-endian words = the 32 digest bytes)
+// concatenated, leaves first: layer k starts at sum_{t<k} L >> t, the root is the last node.
+// Leaf counts are powers of two on this path (2n, Basefold domains, folded levels).
+//
+// One workgroup reduces up to 256 nodes of a layer by up to 8 levels in LDS, so a tree over L
+// leaves takes ceil(log2(L) / 8) launches; the first launch also hashes the leaves.
+#include "blake3_dev.hpp"
+#include "kernels.hpp"
+
+namespace lgs {
+
+// Leaf p = words M[r * rs + p * cs] for r < rows (rows * 8 <= 1024 bytes: a single chunk).
+__device__ __forceinline__ void leaf_digest(const uint64_t* __restrict__ M, uint32_t rows,
+                                            uint64_t rs, uint64_t cs, uint64_t p, uint32_t cv[8]) {
+  b3_init(cv);
+  const uint32_t nblocks = (rows + 7) / 8;
+  for (uint32_t b = 0; b < nblocks; b++) {
+    uint32_t m[16];
+    const uint32_t wcount = (rows - 8 * b) < 8 ? (rows - 8 * b) : 8;
+#pragma unroll
+    for (uint32_t w = 0; w < 8; w++) {
+      uint64_t v = w < wcount ? M[(uint64_t)(8 * b + w) * rs + p * cs] : 0;
+      m[2 * w] = (uint32_t)v;
+      m[2 * w + 1] = (uint32_t)(v >> 32);
+    }
+    uint32_t flags = (b == 0 ? B3_CHUNK_START : 0) | (b + 1 == nblocks ? (B3_CHUNK_END | B3_ROOT) : 0);
+    b3_compress(cv, m, wcount * 8, flags);
+  }
+}
+
+__global__ __launch_bounds__(256) void merkle_reduce_kernel(
+    const uint64_t* __restrict__ M, uint32_t rows, uint64_t rs, uint64_t cs, int from_leaves,
+    uint32_t* __restrict__ nodes, uint64_t child_off, uint64_t child_count, int levels) {
+  __shared__ uint32_t sh[256 * 8];
+  const uint32_t t = threadIdx.x;
+  const uint64_t base = (uint64_t)blockIdx.x * 256;
+  uint32_t local_n = (uint32_t)((child_count - base) < 256 ? (child_count - base) : 256);
+  if (t < local_n) {
+    uint32_t cv[8];
+    if (from_leaves) {
+      leaf_digest(M, rows, rs, cs, base + t, cv);
+#pragma unroll
+      for (int i = 0; i < 8; i++) nodes[(child_off + base + t) * 8 + i] = cv[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; i++) cv[i] = nodes[(child_off + base + t) * 8 + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) sh[t * 8 + i] = cv[i];
+  }
+  uint64_t off = child_off, count = child_count, blk_base = base;
+  for (int lv = 0; lv < levels && local_n > 1; lv++) {
+    __syncthreads();
+    const uint32_t half = local_n / 2;
+    uint32_t out[8];
+    if (t < half) b3_parent(&sh[(2 * t) * 8], &sh[(2 * t + 1) * 8], out);
+    __syncthreads();
+    const uint64_t poff = off + count;
+    blk_base >>= 1;
+    if (t < half) {
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        sh[t * 8 + i] = out[i];
+        nodes[(poff + blk_base + t) * 8 + i] = out[i];
+      }
+    }
+    off = poff;
+    count >>= 1;
+    local_n = half;
+  }
+}
+
+static int ilog2_u64(uint64_t x) {
+  int l = 0;
+  while (((uint64_t)1 << l) < x) l++;
+  return l;
+}
+
+hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs,
+                         uint64_t nleaves, uint32_t* nodes, hipStream_t st) {
+  uint64_t off = 0, count = nleaves;
+  int total_levels = ilog2_u64(nleaves);
+  int done = 0;
+  bool first = true;
+  while (first || count > 1) {
+    int lv = total_levels - done < 8 ? total_levels - done : 8;
+    uint32_t blocks = (uint32_t)((count + 255) / 256);
+    hipLaunchKernelGGL(merkle_reduce_kernel, dim3(blocks), dim3(256), 0, st, M, rows, rs, cs,
+                       first ? 1 : 0, nodes, off, count, lv);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    for (int i = 0; i < lv; i++) {
+      off += count;
+      count >>= 1;
+    }
+    done += lv;
+    first = false;
+  }
+  return hipSuccess;
+}
+
+}  // namespace lgs

@@ -1,0 +1,217 @@
+// Host-side runtime objects behind the C ABI: context (stream, caching device allocator,
+// twiddle tables, scratch), keys and commitments.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace lgs {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw Error(3 /*LGS_ERR_DEVICE*/, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Caching device allocator: freed blocks are kept per size class and reused, so steady-state
+// commits and proofs do not call hipMalloc.
+class DevicePool {
+ public:
+  ~DevicePool() { trim(); }
+  void* alloc(size_t bytes) {
+    size_t cls = size_class(bytes);
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      auto& fl = free_[cls];
+      if (!fl.empty()) {
+        void* p = fl.back();
+        fl.pop_back();
+        return p;
+      }
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, cls) != hipSuccess) {
+      (void)hipGetLastError();
+      trim();
+      if (hipMalloc(&p, cls) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(4 /*LGS_ERR_NOMEM*/, "device allocation of " + std::to_string(cls) + " bytes failed");
+      }
+    }
+    std::lock_guard<std::mutex> g(mu_);
+    sizes_[p] = cls;
+    live_ += cls;
+    return p;
+  }
+  void release(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu_);
+    free_[sizes_.at(p)].push_back(p);
+  }
+  void trim() {
+    std::lock_guard<std::mutex> g(mu_);
+    for (auto& kv : free_)
+      for (void* p : kv.second) {
+        (void)hipFree(p);
+        live_ -= sizes_[p];
+        sizes_.erase(p);
+      }
+    free_.clear();
+  }
+  size_t reserved() const { return live_; }
+  static size_t size_class(size_t b) {
+    if (b <= 256) return 256;
+    size_t c = 256;
+    while (c < b) c <<= 1;  // powers of two up to 2 MiB, then 2 MiB multiples
+    if (c > (size_t(2) << 20)) c = (b + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+    return c;
+  }
+
+ private:
+  std::mutex mu_;
+  std::map<size_t, std::vector<void*>> free_;
+  std::map<void*, size_t> sizes_;
+  size_t live_ = 0;
+};
+
+template <class T>
+struct DevBuf {
+  DevicePool* pool = nullptr;
+  T* ptr = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  DevBuf(DevicePool* p, size_t n) : pool(p), ptr((T*)p->alloc(n * sizeof(T) ? n * sizeof(T) : 1)), count(n) {}
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept { *this = std::move(o); }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      reset();
+      pool = o.pool;
+      ptr = o.ptr;
+      count = o.count;
+      o.ptr = nullptr;
+      o.count = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (ptr && pool) pool->release(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+  size_t bytes() const { return count * sizeof(T); }
+};
+
+// A Merkle tree resident in HBM: all layers, leaves first (see merkle.hip).
+struct DevTree {
+  DevBuf<uint32_t> nodes;
+  uint64_t leaves = 0;
+  uint64_t node_count() const { return 2 * leaves - 1; }
+  uint64_t layer_offset(int k) const {
+    uint64_t off = 0, c = leaves;
+    for (int i = 0; i < k; i++) {
+      off += c;
+      c >>= 1;
+    }
+    return off;
+  }
+};
+
+// Basefold prover state on the GPU (pcs/src/basefold.rs:36-54): interleaved codewords
+// (rows of length D; extension-field polys use two rows per codeword, c0 then c1), their
+// column Merkle tree, and the polynomial itself.
+struct DevBasefold {
+  bool ext = false;
+  uint32_t num_vars = 0, li = 0, code_rate = 1;
+  uint64_t D = 0;
+  DevBuf<uint64_t> codes;  // (ext ? 2 : 1) * 2^li rows x D
+  DevTree tree;
+  DevBuf<uint64_t> poly;  // 2^num_vars base values, or 2^num_vars (c0, c1) pairs
+  const uint64_t* poly_ptr = nullptr;  // poly.ptr, or borrowed storage
+  uint8_t root[32];
+  uint32_t code_rows() const { return (ext ? 2u : 1u) << li; }
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  DevicePool pool;
+  std::map<int, DevBuf<uint64_t>> twiddles;
+  int hash_variant = 0;  // 0 auto, 1 MFMA, 2 VALU
+  int hash_ksplit = 0;
+  bool profile = false;
+  double stage_ms[8] = {0};
+  hipEvent_t ev[8] = {};
+  uint8_t* pinned = nullptr;  // small pinned staging buffer for roots
+  size_t pinned_size = 0;
+
+  const uint64_t* twiddle(int logN);
+  uint8_t* staging(size_t bytes);
+};
+
+struct Key {
+  Context* ctx = nullptr;
+  uint32_t log_m = 0, log_n = 0;
+  uint64_t W = 0;
+  DevBuf<uint64_t> a;    // 32 x W row-major (also A-bar flat, subset_sum.rs:73-79)
+  DevBuf<uint64_t> at;   // W x 32 (VALU hash layout)
+  DevBuf<int8_t> dig;    // MFMA digit layout
+  bool has_state = false;
+  DevBasefold a_state;
+  uint8_t com_a[32] = {0};
+};
+
+struct Commitment {
+  Context* ctx = nullptr;
+  const Key* key = nullptr;
+  uint32_t log_m = 0, log_n = 0;
+  DevBuf<uint64_t> f_owned;
+  const uint64_t* f = nullptr;  // f_owned.ptr or a borrowed device pointer
+  DevBuf<uint64_t> enc;         // m x 2n
+  DevBuf<uint64_t> h;           // 32 x 2n
+  DevTree tree;
+  uint8_t root[32] = {0};
+};
+
+// commit.cpp
+void enqueue_commit(Context& ctx, const Key& key, Commitment& c);
+void enqueue_basefold_commit(Context& ctx, DevBasefold& st, const uint64_t* poly, bool ext,
+                             uint32_t num_vars, uint32_t li, uint32_t code_rate);
+void setup_key(Context& ctx, Key& key, const uint64_t* host_a, bool commit_a);
+
+// Shape helpers restating pcs/src/ligesis.rs:175-203.
+inline void committed_vars(uint32_t lm, uint32_t ln, uint32_t out[3]) {
+  out[0] = 11 + lm;  // A-bar: C * eta * m
+  out[1] = 13 + lm;  // B_I: eta * m * s
+  out[2] = ln;       // a-bar: n
+}
+inline uint32_t max_vars(uint32_t lm, uint32_t ln) {
+  uint32_t v[3];
+  committed_vars(lm, ln, v);
+  return v[0] > v[1] ? (v[0] > v[2] ? v[0] : v[2]) : (v[1] > v[2] ? v[1] : v[2]);
+}
+inline uint32_t min_vars(uint32_t lm, uint32_t ln) {
+  uint32_t v[3];
+  committed_vars(lm, ln, v);
+  return v[0] < v[1] ? (v[0] < v[2] ? v[0] : v[2]) : (v[1] < v[2] ? v[1] : v[2]);
+}
+inline bool shape_ok(uint32_t lm, uint32_t ln) { return max_vars(lm, ln) - min_vars(lm, ln) <= 6; }
+inline uint32_t batch_rounds(uint32_t lm, uint32_t ln) { return max_vars(lm, ln) - 6; }
+inline uint32_t shared_li(uint32_t nv, uint32_t lm, uint32_t ln) { return nv - batch_rounds(lm, ln); }
+
+}  // namespace lgs

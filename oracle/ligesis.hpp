@@ -81,13 +81,27 @@ inline ProvingKey setup(size_t log_m, size_t log_n, const uint64_t* a, bool comm
   return pk;
 }
 
-// subset_sum.rs:43-54
+// subset_sum.rs:43-54. Like p3's Goldilocks addition, the running sums stay in [0, 2^64)
+// without canonicalisation (a carry out of 2^64 folds back as 2^32 - 1); they are made
+// canonical once at the end, which gives the same residues.
+inline uint64_t add_noncanon(uint64_t a, uint64_t b) {
+  uint64_t s = a + b;
+  uint64_t c = (uint64_t)(s < b) * EPS;  // branch-free: carries are data-dependent
+  uint64_t t = s + c;
+  if (t < c) t += EPS;  // second carry (rare)
+  return t;
+}
+
 inline std::vector<uint64_t> hash_column(const ProvingKey& pk, const std::vector<uint8_t>& x) {
-  std::vector<uint64_t> out(C, 0);
+  uint64_t out[C] = {0};
+  const uint64_t* rows[C];
+  for (size_t r = 0; r < C; r++) rows[r] = pk.a[r].data();
   for (size_t i = 0; i < x.size(); i++)
     if (x[i])
-      for (size_t r = 0; r < C; r++) out[r] = fadd(out[r], pk.a[r][i]);
-  return out;
+      for (size_t r = 0; r < C; r++) out[r] = add_noncanon(out[r], rows[r][i]);
+  std::vector<uint64_t> res(C);
+  for (size_t r = 0; r < C; r++) res[r] = canon(out[r]);
+  return res;
 }
 
 inline std::vector<uint8_t> ser_h_col(const std::vector<std::vector<uint64_t>>& h, size_t j) {

@@ -1,0 +1,127 @@
+"""GPU commit parity: F', H and the Merkle root against the oracle, bit for bit.
+
+Restates the reference's commit (pcs/src/ligesis.rs:277-325) checks: the oracle runs the same
+seeded inputs on the CPU; every stage the GPU keeps resident is read back and compared.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from ligesis_amd import Ligesis, LigesisError
+from ligesis_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+# (log_m, log_n): the reference test shape (3, 10), tiny and ragged-split shapes, a shape whose
+# rows exceed one LDS tile (2n = 2^16: global NTT stages), and a larger square.
+SHAPES = [(3, 10), (0, 7), (1, 5), (2, 6), (4, 8), (6, 9), (2, 15), (8, 10)]
+
+
+def _commit_both(gpu_ctx, log_m, log_n, seed, variant):
+    inp = synth.Inputs(log_m, log_n, seed=seed)
+    print(f"commit parity shape=({log_m},{log_n}) seed={seed} hash={variant}", flush=True)
+    gpu_ctx.set_option("hash", variant)
+    pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx, commit_only=True)
+    root, cd = Ligesis.commit(pk, inp.f)
+    opk = oracle.ProvingKey(log_m, log_n, inp.a, commit_a=False)
+    ocd = oracle.CommitData(opk, inp.f, keep_enc=True)
+    return root, cd, ocd
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("variant", [0, 2])
+def test_commit_matches_oracle(gpu_ctx, shape, variant):
+    log_m, log_n = shape
+    root, cd, ocd = _commit_both(gpu_ctx, log_m, log_n, seed=11 + log_m + 3 * log_n, variant=variant)
+    np.testing.assert_array_equal(cd.encoded(), ocd.enc())
+    np.testing.assert_array_equal(cd.digests(), ocd.h())
+    assert root == ocd.root
+    gpu_ctx.set_option("hash", 0)
+
+
+def test_mfma_and_valu_hash_agree(gpu_ctx):
+    inp = synth.Inputs(5, 9, seed=3)
+    pk, _ = Ligesis.setup(5, 9, inp.a, ctx=gpu_ctx, commit_only=True)
+    out = {}
+    for variant in (1, 2):
+        gpu_ctx.set_option("hash", variant)
+        for ks in (0, 1, 2, 4):
+            gpu_ctx.set_option("hash_ksplit", ks)
+            root, cd = Ligesis.commit(pk, inp.f)
+            out[(variant, ks)] = (root, cd.digests())
+    gpu_ctx.set_option("hash", 0)
+    gpu_ctx.set_option("hash_ksplit", 0)
+    ref = out[(2, 1)]
+    for k, v in out.items():
+        assert v[0] == ref[0], k
+        np.testing.assert_array_equal(v[1], ref[1])
+
+
+def test_extreme_field_values(gpu_ctx):
+    """All-(p-1) and all-zero rows and an A of p-1 entries (largest digits / carries)."""
+    log_m, log_n = 2, 7
+    p = synth.P
+    f = np.full(1 << (log_m + log_n), p - 1, np.uint64)
+    f[: 1 << log_n] = 0
+    a = np.full(32 * (64 << log_m), p - 1, np.uint64)
+    a[::3] = 0x7F7F7F7F7F7F7F7F
+    a[1::3] = 0x7F7F7F7F7F7F7F80
+    for variant in (1, 2):
+        gpu_ctx.set_option("hash", variant)
+        pk, _ = Ligesis.setup(log_m, log_n, a, ctx=gpu_ctx, commit_only=True)
+        root, cd = Ligesis.commit(pk, f)
+        opk = oracle.ProvingKey(log_m, log_n, a, commit_a=False)
+        ocd = oracle.CommitData(opk, f, keep_enc=True)
+        np.testing.assert_array_equal(cd.digests(), ocd.h())
+        assert root == ocd.root
+    gpu_ctx.set_option("hash", 0)
+
+
+def test_commit_batch_matches_single(gpu_ctx):
+    shapes = [(3, 10), (2, 6), (5, 5), (4, 9)]
+    keys, fs, expect = [], [], []
+    for i, (lm, ln) in enumerate(shapes):
+        inp = synth.Inputs(lm, ln, seed=100 + i)
+        pk, _ = Ligesis.setup(lm, ln, inp.a, ctx=gpu_ctx, commit_only=True)
+        keys.append(pk)
+        fs.append(inp.f)
+        expect.append(Ligesis.commit(pk, inp.f)[0])
+    res = Ligesis.commit_batch(keys, fs, ctx=gpu_ctx)
+    assert [r for r, _ in res] == expect
+
+
+def test_commit_rejects_bad_length(gpu_ctx):
+    inp = synth.Inputs(2, 6, seed=5)
+    pk, _ = Ligesis.setup(2, 6, inp.a, ctx=gpu_ctx, commit_only=True)
+    with pytest.raises(LigesisError):
+        Ligesis.commit(pk, inp.f[:-1])
+
+
+def test_full_size_square_mu24_chain(gpu_ctx):
+    """mu = 24 at 2^12 x 2^12 (the commit benchmark shape): sampled rows of F' equal the oracle's
+    row encoding, sampled columns of H equal the oracle's subset-sum hash of the bits of F', and
+    the root equals the oracle's Merkle root over the GPU's H. Both hash kernels agree."""
+    log_m = log_n = 12
+    inp = synth.Inputs(log_m, log_n, seed=24, n_fields=4, n_ints=4)
+    print("full-size mu=24 square chain check", flush=True)
+    pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx, commit_only=True)
+    gpu_ctx.set_option("hash", 1)
+    root, cd = Ligesis.commit(pk, inp.f)
+    n, m = 1 << log_n, 1 << log_m
+    enc = cd.encoded()
+    h = cd.digests()
+    rng = np.random.default_rng(0)
+    rows = rng.choice(m, 6, replace=False)
+    f2 = inp.f.reshape(m, n)
+    for i in rows:
+        np.testing.assert_array_equal(enc[i], oracle.rs_encode_rows(f2[i], 1, n)[0])
+    cols = rng.choice(2 * n, 4, replace=False)
+    bits = ((enc[:, cols, None] >> np.arange(64, dtype=np.uint64)) & np.uint64(1)).astype(np.uint8)
+    bits = bits.transpose(1, 0, 2).reshape(len(cols), 64 * m)
+    np.testing.assert_array_equal(h[:, cols], oracle.hash_columns(inp.a.reshape(32, 64 * m), bits))
+    leaves = h.T.copy().view(np.uint8).reshape(2 * n, 256)
+    assert oracle.merkle_root(leaves) == root
+    gpu_ctx.set_option("hash", 2)
+    root2, cd2 = Ligesis.commit(pk, inp.f)
+    assert root2 == root
+    gpu_ctx.set_option("hash", 0)
