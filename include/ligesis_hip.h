@@ -90,6 +90,19 @@ void lgs_commitment_release(lgs_commitment* c);
 /* Copy a resident stage (LGS_READ_*) to host memory; cap is in bytes. */
 int lgs_commitment_read(const lgs_commitment* c, uint32_t what, void* out, uint64_t cap);
 
+/* Eval-prove (ligesis.rs:347): proves f(z) = y for the committed f. z holds z_len =
+ * log_m + log_n extension elements (2 * z_len uint64). The randomness oracle
+ * (utils/src/oracle.rs:10-64) is passed as its two pre-drawn streams: `fields` holds n_fields
+ * extension elements, `ints` n_ints uint64; both are read from the start, as after
+ * RandomOracle::restart. The proof is written in the canonical encoding (field order of
+ * LigesisProof, ligesis.rs:99-119; u64 little-endian lengths, canonical u64 field words) into
+ * `proof` when *proof_len (in: capacity in bytes) suffices; *proof_len is set to the size either
+ * way (LGS_ERR_BUFFER if it did not fit). used[0], used[1] receive the numbers of field and
+ * integer draws, as RandomOracle::drawn() (oracle.rs:57-59). */
+int lgs_prove(lgs_ctx* ctx, const lgs_key* key, const lgs_commitment* c, const uint64_t* z,
+              uint64_t z_len, const uint64_t* fields, uint64_t n_fields, const uint64_t* ints,
+              uint64_t n_ints, uint8_t* proof, uint64_t* proof_len, uint64_t used[2]);
+
 #ifdef __cplusplus
 }
 #endif

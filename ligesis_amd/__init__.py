@@ -71,9 +71,8 @@ def lib():
         L.lgs_commit_batch.argtypes = [vp, u32, vp, vp, vp, u32, vp, vp]
         L.lgs_commitment_release.argtypes = [vp]
         L.lgs_commitment_read.argtypes = [vp, u32, vp, u64]
-        for name in ("lgs_prove",):
-            if hasattr(L, name):
-                getattr(L, name).restype = ctypes.c_int
+        L.lgs_prove.argtypes = [vp, vp, vp, vp, u64, vp, u64, vp, u64, vp,
+                                ctypes.POINTER(ctypes.c_uint64), vp]
         _lib = L
     return _lib
 
@@ -261,3 +260,25 @@ class Ligesis:
             r = roots[i].tobytes()
             res.append((r, CommitData(ctx, keys[i], ctypes.c_void_p(outs[i]), r)))
         return res
+
+    @staticmethod
+    def prove(pk: ProvingKey, data: CommitData, z, oracle: RandomOracle) -> bytes:
+        """Eval-prove (ligesis.rs:347): returns the canonical proof bytes and advances the
+        oracle's cursors by the draws the proof consumed (the reference's RandomOracle is read
+        from its start, i.e. call ``oracle.restart()`` first for a fresh transcript)."""
+        ctx = pk.ctx
+        z = _u64(z).reshape(-1, 2)
+        if oracle.fields_idx or oracle.ints_idx:
+            raise LigesisError(LGS_ERR_ARG, "prove reads the oracle streams from the start; restart() it")
+        used = np.zeros(2, np.uint64)
+        args = (ctx.h, pk.h, data.h, _ptr(z), z.shape[0], _ptr(oracle.fields), oracle.fields.shape[0],
+                _ptr(oracle.ints), oracle.ints.size)
+        out = np.zeros(4 << 20, np.uint8)
+        cap = ctypes.c_uint64(out.size)
+        rc = lib().lgs_prove(*args, _ptr(out), ctypes.byref(cap), _ptr(used))
+        if rc == LGS_ERR_BUFFER:  # larger than the first guess: retry with the reported size
+            out = np.zeros(cap.value, np.uint8)
+            rc = lib().lgs_prove(*args, _ptr(out), ctypes.byref(cap), _ptr(used))
+        ctx.check(rc)
+        oracle.fields_idx, oracle.ints_idx = int(used[0]), int(used[1])
+        return out[: cap.value].tobytes()

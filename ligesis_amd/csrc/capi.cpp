@@ -242,6 +242,33 @@ int lgs_commit_batch(lgs_ctx* ctx, uint32_t count, const lgs_key* const* keys,
   return LGS_OK;
 }
 
+int lgs_prove(lgs_ctx* ctx, const lgs_key* key, const lgs_commitment* c, const uint64_t* z,
+              uint64_t z_len, const uint64_t* fields, uint64_t n_fields, const uint64_t* ints,
+              uint64_t n_ints, uint8_t* proof, uint64_t* proof_len, uint64_t used[2]) {
+  if (!ctx || !key || !c || !z || !proof_len || (n_fields && !fields) || (n_ints && !ints))
+    return LGS_ERR_ARG;
+  return guarded(&ctx->c, [&] {
+    (void)hipSetDevice(ctx->c.device);
+    std::vector<uint8_t> bytes;
+    uint64_t u[2] = {0, 0};
+    try {
+      lgs::prove_impl(ctx->c, key->k, c->c, z, z_len, fields, n_fields, ints, n_ints, bytes, u);
+    } catch (...) {
+      (void)hipStreamSynchronize(ctx->c.stream);
+      throw;
+    }
+    if (used) {
+      used[0] = u[0];
+      used[1] = u[1];
+    }
+    uint64_t cap = *proof_len;
+    *proof_len = bytes.size();
+    if (!proof || cap < bytes.size())
+      throw lgs::Error(LGS_ERR_BUFFER, "prove: proof needs " + std::to_string(bytes.size()) + " bytes");
+    std::memcpy(proof, bytes.data(), bytes.size());
+  });
+}
+
 void lgs_commitment_release(lgs_commitment* c) {
   if (!c) return;
   if (c->c.ctx) (void)hipStreamSynchronize(c->c.ctx->stream);

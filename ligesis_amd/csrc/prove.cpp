@@ -1,0 +1,527 @@
+// Eval-prove on the GPU (pcs/src/ligesis.rs:347-549 with Basefold::batch_prove,
+// pcs/src/basefold.rs:679-856). The transcript is fixed before any launch: the reference's
+// RandomOracle hands out pre-drawn values in order (utils/src/oracle.rs:32-55), so the schedule
+// below assigns every challenge its position in the field / integer streams, the host derives
+// every evaluation point from them, and the device work is enqueued back to back. Proof values
+// are gathered on the device into one staging buffer, copied to the host once, and encoded.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "prove_kernels.hpp"
+#include "runtime.hpp"
+
+namespace lgs {
+
+namespace {
+
+typedef std::vector<ef2> EfV;
+
+EfV rev(const EfV& v) { return EfV(v.rbegin(), v.rend()); }
+EfV cat(const EfV& a, const EfV& b) {
+  EfV r = a;
+  r.insert(r.end(), b.begin(), b.end());
+  return r;
+}
+
+// new_eq on the host for the short points of the eq-compression (<= 64 entries).
+EfV host_eq(const EfV& p) {
+  EfV e{ef_one()};
+  for (size_t k = p.size(); k-- > 0;) {
+    EfV n;
+    ef2 q = ef_sub(ef_one(), p[k]);
+    for (auto& x : e) {
+      n.push_back(ef_mul(q, x));
+      n.push_back(ef_mul(p[k], x));
+    }
+    e.swap(n);
+  }
+  return e;
+}
+
+int ilog2(uint64_t x) {
+  int l = 0;
+  while ((uint64_t(1) << l) < x) l++;
+  return l;
+}
+
+// Nodes of an rs_merkle multiproof (utils/src/merkle.rs:76-78) for sorted distinct leaf
+// positions, as node indices of the layered tree storage, in proof order.
+std::vector<uint64_t> multiproof_nodes(uint64_t leaves, const std::vector<uint64_t>& pos) {
+  std::vector<uint64_t> out, cur = pos;
+  uint64_t off = 0, size = leaves;
+  while (true) {
+    for (size_t a = 0; a < cur.size(); a++) {
+      uint64_t sib = cur[a] ^ 1;
+      bool known = (a + 1 < cur.size() && cur[a + 1] == sib) || (a > 0 && cur[a - 1] == sib);
+      if (!known && sib < size) out.push_back(off + sib);
+    }
+    if (size <= 1) break;
+    std::vector<uint64_t> par;
+    for (uint64_t i : cur)
+      if (par.empty() || par.back() != i / 2) par.push_back(i / 2);
+    cur = par;
+    off += size;
+    size >>= 1;
+  }
+  return out;
+}
+
+std::vector<uint64_t> level_positions(const std::vector<uint64_t>& base, uint64_t half) {
+  std::vector<uint64_t> p;
+  for (uint64_t b : base) {
+    p.push_back(b % half);
+    p.push_back(b % half + half);
+  }
+  std::sort(p.begin(), p.end());
+  p.erase(std::unique(p.begin(), p.end()), p.end());
+  return p;
+}
+
+// Staged gathers: word offsets into several device sources, landing in one staging buffer.
+struct Gather {
+  struct Part {
+    const uint64_t* src;
+    size_t first, count;
+  };
+  std::vector<uint64_t> offs;
+  std::vector<Part> parts;
+  size_t add(const uint64_t* src, const std::vector<uint64_t>& words) {
+    size_t at = offs.size();
+    offs.insert(offs.end(), words.begin(), words.end());
+    parts.push_back({src, at, words.size()});
+    return at;
+  }
+  size_t add_nodes(const DevTree& t, const std::vector<uint64_t>& nodes) {
+    std::vector<uint64_t> w;
+    for (uint64_t nd : nodes)
+      for (int q = 0; q < 4; q++) w.push_back(nd * 4 + q);
+    return add((const uint64_t*)t.nodes.ptr, w);
+  }
+  size_t add_root(const DevTree& t) { return add_nodes(t, {t.node_count() - 1}); }
+};
+
+struct Writer {
+  std::vector<uint8_t> b;
+  void u64(uint64_t v) {
+    for (int i = 0; i < 8; i++) b.push_back((uint8_t)(v >> (8 * i)));
+  }
+  void words(const uint64_t* w, size_t n) {
+    for (size_t i = 0; i < n; i++) u64(w[i]);
+  }
+  void raw(const uint64_t* w, size_t n) {  // n words as raw little-endian bytes (digests)
+    words(w, n);
+  }
+  void ef(const uint64_t* w) { words(w, 2); }
+};
+
+struct Sumcheck {
+  int d = 0;
+  uint32_t nv = 0;
+  size_t rounds_at = 0, finals_at = 0;  // offsets (in ef2) into the results buffer
+};
+
+}  // namespace
+
+int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uint64_t* z, uint64_t z_len,
+                   const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,
+                   std::vector<uint8_t>& proof, uint64_t used[2]) {
+  if (!key.has_state) throw Error(7, "prove: the key was built commit-only (no A-bar commitment)");
+  if (cm.key != &key) throw Error(1, "prove: commitment was made with another key");
+  const uint32_t log_m = key.log_m, log_n = key.log_n;
+  const uint64_t m = uint64_t(1) << log_m, n = uint64_t(1) << log_n, two_n = 2 * n;
+  const uint64_t W = key.W, s = 128;
+  const uint32_t log_w = 6 + log_m, log_s = 7, Lb = log_w + log_s;
+  const uint32_t R = batch_rounds(log_m, log_n);
+  if (z_len != log_m + log_n) throw Error(1, "prove: z must have log_m + log_n entries");
+
+  // ---- transcript schedule (ligesis.rs:390-530, basefold.rs:732, 761, 788) ----
+  size_t fc = 0;
+  auto take = [&](size_t k) {
+    if (fc + k > n_fields) throw Error(5, "prove: randomness oracle field stream exhausted");
+    EfV v(k);
+    for (size_t i = 0; i < k; i++) v[i] = ef2{fields[2 * (fc + i)], fields[2 * (fc + i) + 1]};
+    fc += k;
+    return v;
+  };
+  EfV alpha1 = take(Lb), alpha2 = take(5), ch_bin = take(Lb), r2 = take(log_s);
+  EfV ch_hash = take(log_w), ch_recon = take(log_w), ch_table = take(log_n + 1), ch_rs = take(log_n);
+  EfV gamma_bi = take(1), ch_bibatch = take(Lb), gamma_a = take(1), ch_abatch = take(log_n);
+  EfV coeffs = take(3), ch_batch = take(R);
+  if (n_ints < 4 * s + 100) throw Error(5, "prove: randomness oracle integer stream exhausted");
+  std::vector<uint32_t> idx;
+  {
+    std::vector<uint8_t> seen(two_n, 0);
+    for (uint64_t k = 0; k < 4 * s && idx.size() < s; k++) {
+      uint64_t j = ints[k] % two_n;
+      if (!seen[j]) {
+        seen[j] = 1;
+        idx.push_back((uint32_t)j);
+      }
+    }
+    if (idx.size() != s) throw Error(5, "not enough distinct columns sampled from oracle");
+  }
+  std::vector<uint64_t> sorted(idx.begin(), idx.end());
+  std::sort(sorted.begin(), sorted.end());
+  used[0] = fc;
+  used[1] = 4 * s + 100;
+
+  EfV zz(z_len);
+  for (size_t i = 0; i < z_len; i++) zz[i] = ef2{z[2 * i], z[2 * i + 1]};
+  EfV z2(zz.begin(), zz.begin() + log_n), z1(zz.begin() + log_n, zz.end());
+  EfV r1 = rev(ch_bin), r4 = rev(ch_hash), r5 = rev(ch_recon), r3 = rev(ch_table), r6 = rev(ch_rs);
+  EfV r_bi = rev(ch_bibatch), r_a = rev(ch_abatch);
+  EfV p_a = cat(r4, alpha2), p_bi_hash = cat(r2, r4), p_bi_recon = cat(r2, r5);
+  EfV gbi{ef_one(), gamma_bi[0], ef_mul(gamma_bi[0], gamma_bi[0])}, ga{ef_one(), gamma_a[0]};
+
+  const uint32_t nv_a = 11 + log_m, nv_bi = Lb;
+  const uint32_t li_A = shared_li(nv_a, log_m, log_n), li_bi = shared_li(nv_bi, log_m, log_n),
+                 li_a = shared_li(log_n, log_m, log_n);
+  const EfV* bpoint[3] = {&p_a, &r_bi, &r_a};
+  const uint32_t bli[3] = {li_A, li_bi, li_a};
+  EfV wt[3], eq4[3], zp[3];
+  for (int c = 0; c < 3; c++) {
+    EfV head(bpoint[c]->begin(), bpoint[c]->begin() + bli[c]);
+    eq4[c] = host_eq(head);
+    for (auto& e : eq4[c]) wt[c].push_back(ef_mul(coeffs[c], e));
+    zp[c].assign(bpoint[c]->begin() + bli[c], bpoint[c]->end());
+    if (zp[c].size() != R) throw Error(1, "prove: Basefold point length mismatch");
+  }
+
+  // ---- one constants blob: every point / scalar the kernels read ----
+  EfV blob;
+  auto put = [&](const EfV& v) {
+    size_t at = blob.size();
+    blob.insert(blob.end(), v.begin(), v.end());
+    return at;
+  };
+  size_t o_z1 = put(z1), o_z2 = put(z2), o_alpha1 = put(alpha1), o_alpha2 = put(alpha2);
+  size_t o_chbin = put(ch_bin), o_r2 = put(r2), o_chhash = put(ch_hash), o_chrecon = put(ch_recon);
+  size_t o_chtable = put(ch_table), o_chrs = put(ch_rs), o_r3 = put(r3), o_r1 = put(r1);
+  size_t o_pbh = put(p_bi_hash), o_pbr = put(p_bi_recon), o_r6 = put(r6), o_gbi = put(gbi), o_ga = put(ga);
+  size_t o_chbib = put(ch_bibatch), o_chab = put(ch_abatch), o_chbatch = put(ch_batch);
+  size_t o_wt[3], o_eq4[3], o_zp1[3];
+  for (int c = 0; c < 3; c++) {
+    o_wt[c] = put(wt[c]);
+    o_eq4[c] = put(eq4[c]);
+    o_zp1[c] = put(EfV(zp[c].begin() + (R ? 1 : 0), zp[c].end()));
+  }
+
+  hipStream_t st = ctx.stream;
+  DevicePool* pool = &ctx.pool;
+  DevBuf<ef2> dblob(pool, blob.size());
+  hip_check(hipMemcpyAsync(dblob.ptr, blob.data(), blob.size() * sizeof(ef2), hipMemcpyHostToDevice, st),
+            "upload constants");
+  const ef2* B = dblob.ptr;
+  DevBuf<uint32_t> didx(pool, s);
+  hip_check(hipMemcpyAsync(didx.ptr, idx.data(), s * 4, hipMemcpyHostToDevice, st), "upload idx");
+
+  // results buffer layout
+  Sumcheck sc[7];
+  const int sc_d[7] = {3, 2, 2, 2, 2, 2, 2};
+  const uint32_t sc_nv[7] = {Lb, log_w, log_w, log_n + 1, log_n, Lb, log_n};
+  size_t res_n = 0;
+  for (int i = 0; i < 7; i++) {
+    sc[i].d = sc_d[i];
+    sc[i].nv = sc_nv[i];
+    sc[i].rounds_at = res_n;
+    res_n += (size_t)sc[i].nv * (sc[i].d + 1);
+    sc[i].finals_at = res_n;
+    res_n += sc[i].d;
+  }
+  const size_t o_y = res_n++, o_y4 = res_n++;
+  const size_t o_l0 = res_n;
+  res_n += 3 * (size_t)R;
+  DevBuf<ef2> res(pool, res_n);
+
+  const uint32_t kmax = std::max(std::max(Lb, log_n), log_m + 1);
+  uint64_t eq_scratch_n = (uint64_t(1) << (kmax / 2)) + (uint64_t(1) << (kmax - kmax / 2)) + 64;
+  DevBuf<ef2> eqs(pool, eq_scratch_n);
+  DevBuf<ef2> part(pool, 1024 * 4 + 256);
+
+  auto run_sumcheck = [&](int which, std::vector<ef2*> polys, size_t ch_off) {
+    const Sumcheck& S = sc[which];
+    for (uint32_t t = 0; t < S.nv; t++) {
+      uint64_t half = uint64_t(1) << (S.nv - 1 - t);
+      hip_check(launch_sc_round(S.d, polys.data(), half, B + ch_off + t, part.ptr, st), "sumcheck round");
+      hip_check(launch_reduce_rows(part.ptr, sc_round_blocks(half), S.d + 1,
+                                   res.ptr + S.rounds_at + (size_t)t * (S.d + 1), st),
+                "sumcheck reduce");
+    }
+    for (int j = 0; j < S.d; j++)
+      hip_check(hipMemcpyAsync(res.ptr + S.finals_at + j, polys[j], sizeof(ef2), hipMemcpyDeviceToDevice, st),
+                "sumcheck finals");
+  };
+
+  // ---- a-bar = eq_{z1}^T F, y = a-bar(z2) (ligesis.rs:365-379) ----
+  DevBuf<ef2> eqz1(pool, m), eqz2(pool, n), a_vec(pool, n);
+  hip_check(launch_eq(B + o_z1, log_m, eqs.ptr, nullptr, 0, eqz1.ptr, st), "eq z1");
+  hip_check(launch_eq(B + o_z2, log_n, eqs.ptr, nullptr, 0, eqz2.ptr, st), "eq z2");
+  {
+    uint64_t chunks = std::max<uint64_t>(1, std::min<uint64_t>(m, (uint64_t(1) << 21) / n));
+    while (m % chunks) chunks >>= 1;
+    DevBuf<ef2> wpart(pool, chunks * n);
+    hip_check(launch_wrows(cm.f, (uint32_t)m, n, eqz1.ptr, wpart.ptr, (uint32_t)chunks, a_vec.ptr, st), "a-bar");
+  }
+  hip_check(launch_dot(a_vec.ptr, eqz2.ptr, n, part.ptr, res.ptr + o_y, st), "y");
+  DevBasefold av_state;
+  enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec.ptr, true, log_n, li_a, 1);
+
+  // ---- RS(a-bar) (ligesis.rs:387) ----
+  const int logN = log_n + 1;
+  DevBuf<uint64_t> ra_planar(pool, 2 * two_n);
+  DevBuf<ef2> ra(pool, two_n);
+  hip_check(launch_ntt((const uint64_t*)a_vec.ptr, 1, 2, (uint32_t)n, 2, logN, ctx.twiddle(logN), ra_planar.ptr, st),
+            "RS(a-bar)");
+  hip_check(launch_planar_to_ef(ra_planar.ptr, ra_planar.ptr + two_n, two_n, ra.ptr, st), "RS(a-bar) layout");
+
+  // ---- B_I and its commitment (ligesis.rs:402-416) ----
+  const uint64_t nbi = W * s;
+  DevBuf<uint64_t> bi(pool, nbi);
+  DevBuf<ef2> bi_ef(pool, nbi), bi_m1(pool, nbi), bi_ef2(pool, nbi), big(pool, nbi);
+  hip_check(launch_bits_gather(cm.enc.ptr, two_n, didx.ptr, (uint32_t)s, W, bi.ptr, bi_ef.ptr, bi_m1.ptr, st),
+            "B_I gather");
+  hip_check(hipMemcpyAsync(bi_ef2.ptr, bi_ef.ptr, nbi * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
+  DevBasefold bi_state;
+  enqueue_basefold_commit(ctx, bi_state, bi.ptr, false, nv_bi, li_bi, 1);
+
+  // ---- binary check (ligesis.rs:424-429) ----
+  hip_check(launch_eq(B + o_alpha1, Lb, eqs.ptr, nullptr, 0, big.ptr, st), "eq alpha1");
+  run_sumcheck(0, {bi_ef.ptr, bi_m1.ptr, big.ptr}, o_chbin);
+
+  // ---- hash / recon channels (ligesis.rs:432-485) ----
+  DevBuf<ef2> eqa2(pool, 32), eqr2(pool, s), g1(pool, W), g2(pool, W), g2b(pool, W), vv(pool, W);
+  hip_check(launch_eq(B + o_alpha2, 5, eqs.ptr, nullptr, 0, eqa2.ptr, st), "eq alpha2");
+  hip_check(launch_eq(B + o_r2, log_s, eqs.ptr, nullptr, 0, eqr2.ptr, st), "eq r2");
+  {
+    DevBuf<ef2> wpart(pool, W);
+    hip_check(launch_wrows(key.a.ptr, 32, W, eqa2.ptr, wpart.ptr, 1, g1.ptr, st), "g1");
+  }
+  hip_check(launch_g2(bi.ptr, (uint32_t)s, W, eqr2.ptr, g2.ptr, st), "g2");
+  hip_check(hipMemcpyAsync(g2b.ptr, g2.ptr, W * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
+  DevBuf<ef2> sel(pool, two_n);
+  hip_check(launch_sel(didx.ptr, eqr2.ptr, (uint32_t)s, sel.ptr, two_n, st), "sel");
+  hip_check(launch_dot(sel.ptr, ra.ptr, two_n, part.ptr, res.ptr + o_y4, st), "y4");
+  hip_check(launch_vvec(eqz1.ptr, W, vv.ptr, st), "v");
+  run_sumcheck(1, {g1.ptr, g2.ptr}, o_chhash);
+  run_sumcheck(2, {vv.ptr, g2b.ptr}, o_chrecon);
+
+  // ---- recon table and RS link (ligesis.rs:488-502); y5 = ra(r3) is sc_table's final
+  //      evaluation of ra, the reference's verify compares exactly these (ligesis.rs:681) ----
+  run_sumcheck(3, {sel.ptr, ra.ptr}, o_chtable);
+  DevBuf<ef2> wv(pool, n), a_copy(pool, n);
+  hip_check(launch_rs_weights(B + o_r3, logN, n, ctx.twiddle(logN), wv.ptr, st), "rs weights");
+  hip_check(hipMemcpyAsync(a_copy.ptr, a_vec.ptr, n * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
+  run_sumcheck(4, {a_copy.ptr, wv.ptr}, o_chrs);
+
+  // ---- Stage A batching sumchecks (ligesis.rs:513-522) ----
+  hip_check(launch_eq(B + o_r1, Lb, eqs.ptr, B + o_gbi + 0, 0, big.ptr, st), "combine_eq");
+  hip_check(launch_eq(B + o_pbh, Lb, eqs.ptr, B + o_gbi + 1, 1, big.ptr, st), "combine_eq");
+  hip_check(launch_eq(B + o_pbr, Lb, eqs.ptr, B + o_gbi + 2, 1, big.ptr, st), "combine_eq");
+  run_sumcheck(5, {bi_ef2.ptr, big.ptr}, o_chbib);
+  DevBuf<ef2> g_a(pool, n);
+  hip_check(launch_eq(B + o_z2, log_n, eqs.ptr, B + o_ga + 0, 0, g_a.ptr, st), "combine_eq");
+  hip_check(launch_eq(B + o_r6, log_n, eqs.ptr, B + o_ga + 1, 1, g_a.ptr, st), "combine_eq");
+  hip_check(hipMemcpyAsync(a_copy.ptr, a_vec.ptr, n * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
+  run_sumcheck(6, {a_copy.ptr, g_a.ptr}, o_chab);
+
+  // ---- Stage B: shared-domain batched Basefold open (basefold.rs:679-856) ----
+  const DevBasefold* states[3] = {&key.a_state, &bi_state, &av_state};
+  const uint64_t* polys[3] = {key.a.ptr, bi.ptr, (const uint64_t*)a_vec.ptr};
+  const uint64_t D = key.a_state.D;
+  for (int c = 0; c < 3; c++)
+    if (states[c]->D != D) throw Error(1, "prove: Basefold domains differ");
+  // levels 0..R stored back to back: level k has D >> k entries
+  std::vector<uint64_t> lvl_off(R + 1);
+  uint64_t tot = 0;
+  for (uint32_t k = 0; k <= R; k++) {
+    lvl_off[k] = tot;
+    tot += D >> k;
+  }
+  DevBuf<ef2> levels(pool, tot);
+  BfChannels chans{};
+  chans.count = 3;
+  for (int c = 0; c < 3; c++) {
+    chans.codes[c] = states[c]->codes.ptr;
+    chans.wt[c] = B + o_wt[c];
+    chans.chunks[c] = 1u << bli[c];
+    chans.ext[c] = states[c]->ext ? 1 : 0;
+  }
+  hip_check(launch_bf_combine(chans, D, levels.ptr, st), "batch combine");
+  const uint64_t M = uint64_t(1) << R;
+  DevBuf<ef2> mbuf[3][2], eqr[3][2];
+  for (int c = 0; c < 3; c++) {
+    mbuf[c][0] = DevBuf<ef2>(pool, M);
+    mbuf[c][1] = DevBuf<ef2>(pool, M / 2 + 1);
+    eqr[c][0] = DevBuf<ef2>(pool, M / 2 + 1);
+    eqr[c][1] = DevBuf<ef2>(pool, M / 4 + 1);
+    hip_check(launch_fold_low(polys[c], states[c]->ext ? 1 : 0, (int)bli[c], M, B + o_eq4[c], mbuf[c][0].ptr, st),
+              "batch fold");
+    if (R >= 1)
+      hip_check(launch_eq(B + o_zp1[c], (int)R - 1, eqs.ptr, nullptr, 0, eqr[c][0].ptr, st), "batch eq");
+  }
+  const uint64_t* twD = ctx.twiddle(ilog2(D));
+  for (uint32_t t = 0; t < R; t++) {
+    const uint64_t half = uint64_t(1) << (R - 1 - t);
+    for (int c = 0; c < 3; c++) {
+      ef2* cur = mbuf[c][t & 1].ptr;
+      ef2* nxt = mbuf[c][(t + 1) & 1].ptr;
+      ef2* e_cur = eqr[c][t & 1].ptr;
+      hip_check(launch_bf_round(cur, half, e_cur, B + o_chbatch + t, nxt, part.ptr, st), "batch round");
+      hip_check(launch_reduce_rows(part.ptr, bf_round_blocks(half), 1, res.ptr + o_l0 + (size_t)c * R + t, st),
+                "batch reduce");
+      if (half > 1)
+        hip_check(launch_eq_pairsum(e_cur, half / 2, eqr[c][(t + 1) & 1].ptr, st), "batch eq step");
+    }
+    const uint64_t len = D >> t;
+    hip_check(launch_eq_fold_codeword(levels.ptr + lvl_off[t], len / 2, B + o_chbatch + t,
+                                      ctx.twiddle(ilog2(len)), levels.ptr + lvl_off[t + 1], st),
+              "eq fold codeword");
+  }
+  (void)twD;
+  std::vector<DevTree> ftrees(R > 1 ? R - 1 : 0);
+  for (uint32_t k = 1; k < R; k++) {
+    DevTree& tr = ftrees[k - 1];
+    tr.leaves = D >> k;
+    tr.nodes = DevBuf<uint32_t>(pool, tr.node_count() * 8);
+    hip_check(launch_merkle((const uint64_t*)(levels.ptr + lvl_off[k]), 2, 1, 2, tr.leaves, tr.nodes.ptr, st),
+              "fold tree");
+  }
+
+  // ---- openings: positions are fixed by the integer stream ----
+  std::vector<uint64_t> base;
+  for (uint64_t q = 0; q < 100; q++) base.push_back(ints[4 * s + q] % (D / 2));
+  std::sort(base.begin(), base.end());
+  base.erase(std::unique(base.begin(), base.end()), base.end());
+  std::vector<uint64_t> bpos = level_positions(base, D / 2);
+
+  Gather g;
+  const size_t g_root_h = g.add_root(cm.tree);
+  (void)g_root_h;
+  const size_t g_hopen = g.add_nodes(cm.tree, multiproof_nodes(two_n, sorted));
+  const size_t n_hopen = multiproof_nodes(two_n, sorted).size();
+  std::vector<uint64_t> hw;
+  for (uint64_t j : sorted)
+    for (uint64_t r = 0; r < 32; r++) hw.push_back(r * two_n + j);
+  const size_t g_hcols = g.add(cm.h.ptr, hw);
+  const size_t g_com_av = g.add_root(av_state.tree), g_com_i = g.add_root(bi_state.tree);
+  std::vector<size_t> g_froot(R > 1 ? R - 1 : 0);
+  for (uint32_t k = 1; k < R; k++) g_froot[k - 1] = g.add_root(ftrees[k - 1]);
+  std::vector<uint64_t> fw;
+  for (uint64_t i = 0; i < (D >> R); i++) {
+    fw.push_back(2 * (lvl_off[R] + i));
+    fw.push_back(2 * (lvl_off[R] + i) + 1);
+  }
+  const size_t g_final = g.add((const uint64_t*)levels.ptr, fw);
+  // fold-level openings
+  std::vector<std::vector<uint64_t>> fpos(R > 1 ? R - 1 : 0);
+  std::vector<size_t> g_fproof(R > 1 ? R - 1 : 0), g_fval(R > 1 ? R - 1 : 0), n_fproof(R > 1 ? R - 1 : 0);
+  for (uint32_t k = 1; k < R; k++) {
+    uint64_t len = D >> k;
+    fpos[k - 1] = level_positions(base, len / 2);
+    auto nodes = multiproof_nodes(len, fpos[k - 1]);
+    n_fproof[k - 1] = nodes.size();
+    g_fproof[k - 1] = g.add_nodes(ftrees[k - 1], nodes);
+    std::vector<uint64_t> vw;
+    for (uint64_t p : fpos[k - 1]) {
+      vw.push_back(2 * (lvl_off[k] + p));
+      vw.push_back(2 * (lvl_off[k] + p) + 1);
+    }
+    g_fval[k - 1] = g.add((const uint64_t*)levels.ptr, vw);
+  }
+  // base openings per commitment (columns of the interleaved codewords)
+  auto bnodes_cache = std::vector<uint64_t>();
+  size_t g_bproof[3], g_bval[3], n_bproof = 0;
+  for (int c = 0; c < 3; c++) {
+    auto nodes = multiproof_nodes(D, bpos);
+    n_bproof = nodes.size();
+    g_bproof[c] = g.add_nodes(states[c]->tree, nodes);
+    std::vector<uint64_t> vw;
+    uint32_t rows = states[c]->code_rows();
+    for (uint64_t p : bpos)
+      for (uint32_t rr = 0; rr < rows; rr++) vw.push_back((uint64_t)rr * D + p);
+    g_bval[c] = g.add(states[c]->codes.ptr, vw);
+  }
+
+  // stage: [results (2 * res_n words)] [gathered words]
+  const size_t stage_words = 2 * res_n + g.offs.size();
+  DevBuf<uint64_t> dstage(pool, stage_words);
+  DevBuf<uint64_t> doffs(pool, g.offs.size() + 1);
+  hip_check(hipMemcpyAsync(doffs.ptr, g.offs.data(), g.offs.size() * 8, hipMemcpyHostToDevice, st), "offsets");
+  hip_check(hipMemcpyAsync(dstage.ptr, res.ptr, res_n * sizeof(ef2), hipMemcpyDeviceToDevice, st), "results");
+  for (auto& p : g.parts)
+    hip_check(launch_gather_words(p.src, doffs.ptr + p.first, p.count, dstage.ptr + 2 * res_n + p.first, st),
+              "gather");
+  uint64_t* host = (uint64_t*)ctx.staging(stage_words * 8);
+  hip_check(hipMemcpyAsync(host, dstage.ptr, stage_words * 8, hipMemcpyDeviceToHost, st), "download proof");
+  hip_check(hipStreamSynchronize(st), "prove sync");
+  const uint64_t* RS = host;                 // results as words
+  const uint64_t* GW = host + 2 * res_n;     // gathered words
+
+  // ---- encode (field order of LigesisProof, ligesis.rs:99-119) ----
+  Writer w;
+  w.ef(RS + 2 * o_y);
+  w.raw(GW + g_com_av, 4);
+  w.raw(GW + g_com_i, 4);
+  w.ef(RS + 2 * o_y4);
+  w.ef(RS + 2 * (sc[3].finals_at + 1));  // y5 = ra(r3)
+  w.u64(s);
+  for (uint64_t k = 0; k < s; k++) {
+    w.u64(32);
+    w.words(GW + g_hcols + 32 * k, 32);
+  }
+  w.u64(32 * n_hopen);
+  w.raw(GW + g_hopen, 4 * n_hopen);
+  for (int i = 0; i < 7; i++) {
+    w.u64(sc[i].nv);
+    for (uint32_t t = 0; t < sc[i].nv; t++) {
+      w.u64(sc[i].d + 1);
+      w.words(RS + 2 * (sc[i].rounds_at + (size_t)t * (sc[i].d + 1)), 2 * (sc[i].d + 1));
+    }
+    w.u64(sc[i].d);
+    w.words(RS + 2 * sc[i].finals_at, 2 * sc[i].d);
+  }
+  // BatchProof (basefold.rs:151-162)
+  w.u64(3);
+  for (int c = 0; c < 3; c++) {
+    w.u64(R);
+    w.words(RS + 2 * (o_l0 + (size_t)c * R), 2 * R);
+  }
+  w.u64(R > 1 ? R - 1 : 0);
+  for (uint32_t k = 1; k < R; k++) w.raw(GW + g_froot[k - 1], 4);
+  w.u64(D >> R);
+  w.words(GW + g_final, 2 * (D >> R));
+  w.u64(R > 1 ? R - 1 : 0);
+  for (uint32_t k = 1; k < R; k++) {
+    w.u64(32 * n_fproof[k - 1]);
+    w.raw(GW + g_fproof[k - 1], 4 * n_fproof[k - 1]);
+    w.u64(fpos[k - 1].size());
+    for (size_t q = 0; q < fpos[k - 1].size(); q++) {
+      w.u64(fpos[k - 1][q]);
+      w.u64(1);
+      w.words(GW + g_fval[k - 1] + 2 * q, 2);
+    }
+  }
+  for (int grp = 0; grp < 2; grp++) {  // base_bf (A-bar, B_I), then base_ef (a-bar)
+    int c0 = grp == 0 ? 0 : 2, c1 = grp == 0 ? 2 : 3;
+    w.u64(c1 - c0);
+    for (int c = c0; c < c1; c++) {
+      w.u64(32 * n_bproof);
+      w.raw(GW + g_bproof[c], 4 * n_bproof);
+      w.u64(bpos.size());
+      const uint32_t rows = states[c]->code_rows();
+      const uint32_t elems = states[c]->ext ? rows / 2 : rows;
+      for (size_t q = 0; q < bpos.size(); q++) {
+        w.u64(bpos[q]);
+        w.u64(elems);
+        w.words(GW + g_bval[c] + (size_t)rows * q, rows);
+      }
+    }
+  }
+  w.u64(1);  // code_rate
+  w.u64(ilog2(D));
+  proof.swap(w.b);
+  return (int64_t)proof.size();
+}
+
+}  // namespace lgs

@@ -1,0 +1,422 @@
+// Eval-prove kernels (pcs/src/ligesis.rs:347-549, utils/src/poly.rs, utils/src/sumcheck.rs,
+// pcs/src/basefold.rs:644-856). Every challenge of the transcript is an input known before the
+// first launch (the reference's RandomOracle is a pre-drawn stream, utils/src/oracle.rs:10-64),
+// so every kernel reads its challenges from a device array and the whole prove is enqueued
+// without host round trips. Extension elements are stored interleaved (c0, c1).
+#include "prove_kernels.hpp"
+
+namespace lgs {
+
+__device__ __forceinline__ ef2 ld_ef(const ef2* p, uint64_t i) { return p[i]; }
+__device__ __forceinline__ ef2 ef_one_minus(ef2 a) { return ef2{gl_sub(1, a.c0), gl_neg(a.c1)}; }
+
+// Block-wide sum of W extension values per thread; result valid in thread 0.
+template <int W>
+__device__ __forceinline__ void block_sum(ef2 (&v)[W], ef2* sh /* blockDim x W */) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < W; q++) sh[t * W + q] = v[q];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) {
+#pragma unroll
+      for (int q = 0; q < W; q++) sh[t * W + q] = ef_add(sh[t * W + q], sh[(t + s) * W + q]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+#pragma unroll
+    for (int q = 0; q < W; q++) v[q] = sh[q];
+  }
+}
+
+// ---------------- eq tables (MlPoly::new_eq, poly.rs:44-53) ----------------
+// lo[b] = prod_{t < klo} (bit t of b ? p_t : 1 - p_t), hi likewise over p_{klo..k}.
+__global__ void eq_parts_kernel(const ef2* __restrict__ pt, int k, int klo, ef2* __restrict__ lo,
+                                ef2* __restrict__ hi) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int khi = k - klo;
+  if (b < (1u << klo)) {
+    ef2 acc = ef_one();
+    for (int t = 0; t < klo; t++) acc = ef_mul(acc, ((b >> t) & 1) ? pt[t] : ef_one_minus(pt[t]));
+    lo[b] = acc;
+  }
+  if (b < (1u << khi)) {
+    ef2 acc = ef_one();
+    for (int t = 0; t < khi; t++)
+      acc = ef_mul(acc, ((b >> t) & 1) ? pt[klo + t] : ef_one_minus(pt[klo + t]));
+    hi[b] = acc;
+  }
+}
+
+// out[b] (+)= scale * lo[b mod 2^klo] * hi[b >> klo]
+__global__ void eq_expand_kernel(const ef2* __restrict__ lo, const ef2* __restrict__ hi, int klo,
+                                 uint64_t n, const ef2* __restrict__ scale, int accumulate,
+                                 ef2* __restrict__ out) {
+  const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  ef2 v = ef_mul(lo[b & ((1u << klo) - 1)], hi[b >> klo]);
+  if (scale) v = ef_mul(v, *scale);
+  out[b] = accumulate ? ef_add(out[b], v) : v;
+}
+
+hipError_t launch_eq(const ef2* pt, int k, ef2* scratch /* >= 2^(k/2+1) + 2^(k - k/2) */,
+                     const ef2* scale, int accumulate, ef2* out, hipStream_t st) {
+  const int klo = k / 2, khi = k - klo;
+  ef2* lo = scratch;
+  ef2* hi = scratch + (1u << klo);
+  uint32_t nb = ((1u << (klo > khi ? klo : khi)) + 255) / 256;
+  hipLaunchKernelGGL(eq_parts_kernel, dim3(nb), dim3(256), 0, st, pt, k, klo, lo, hi);
+  uint64_t n = uint64_t(1) << k;
+  hipLaunchKernelGGL(eq_expand_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, lo, hi, klo,
+                     n, scale, accumulate, out);
+  return hipGetLastError();
+}
+
+// ---------------- weighted row sums: out[j] = sum_i w[i] * M[i][j] (M base field) -----------
+// ā = eq_{z1}^T F (ligesis.rs:372-378) and g1 = eq_{alpha2}^T A (ligesis.rs:436-444).
+__global__ void wrows_kernel(const uint64_t* __restrict__ M, uint64_t cols, const ef2* __restrict__ w,
+                             uint32_t rows_per_chunk, ef2* __restrict__ part) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  const uint32_t i0 = blockIdx.y * rows_per_chunk;
+  ef2 acc = ef_zero();
+  for (uint32_t i = i0; i < i0 + rows_per_chunk; i++) acc = ef_add(acc, ef_mul_base(w[i], M[i * cols + j]));
+  part[(uint64_t)blockIdx.y * cols + j] = acc;
+}
+
+__global__ void sum_chunks_kernel(const ef2* __restrict__ part, uint32_t chunks, uint64_t cols,
+                                  ef2* __restrict__ out) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  ef2 acc = ef_zero();
+  for (uint32_t c = 0; c < chunks; c++) acc = ef_add(acc, part[(uint64_t)c * cols + j]);
+  out[j] = acc;
+}
+
+hipError_t launch_wrows(const uint64_t* M, uint32_t rows, uint64_t cols, const ef2* w, ef2* part,
+                        uint32_t chunks, ef2* out, hipStream_t st) {
+  const uint32_t bx = (uint32_t)((cols + 255) / 256);
+  hipLaunchKernelGGL(wrows_kernel, dim3(bx, chunks), dim3(256), 0, st, M, cols, w, rows / chunks, part);
+  hipLaunchKernelGGL(sum_chunks_kernel, dim3(bx), dim3(256), 0, st, part, chunks, cols, out);
+  return hipGetLastError();
+}
+
+// ---------------- dot products: out = sum_i a[i] * b[i] ----------------
+__global__ __launch_bounds__(256) void dot_kernel(const ef2* __restrict__ a, const ef2* __restrict__ b,
+                                                  uint64_t n, ef2* __restrict__ part) {
+  __shared__ ef2 sh[256];
+  ef2 acc[1] = {ef_zero()};
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    acc[0] = ef_add(acc[0], ef_mul(a[i], b[i]));
+  block_sum<1>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// out[q] = sum_{b < nb} part[b * width + q], for q < width (one block)
+__global__ void reduce_rows_kernel(const ef2* __restrict__ part, uint32_t nb, uint32_t width,
+                                   ef2* __restrict__ out) {
+  for (uint32_t q = threadIdx.x; q < width; q += blockDim.x) {
+    ef2 acc = ef_zero();
+    for (uint32_t b = 0; b < nb; b++) acc = ef_add(acc, part[(uint64_t)b * width + q]);
+    out[q] = acc;
+  }
+}
+
+hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part /* >= 256 */, ef2* out,
+                      hipStream_t st) {
+  uint32_t nb = (uint32_t)((n + 255) / 256);
+  if (nb > 256) nb = 256;
+  hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(256), 0, st, a, b, n, part);
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(64), 0, st, part, nb, 1u, out);
+  return hipGetLastError();
+}
+
+// ---------------- LigeSIS-specific vectors ----------------
+// B_I (ligesis.rs:403-410): bi[i * s + k] = bit (i mod 64) of F'[i / 64][idx[k]], as 0/1
+// extension elements; bi_m1 = bi - 1 (ligesis.rs:424-425).
+__global__ void bits_gather_kernel(const uint64_t* __restrict__ enc, uint64_t ld, const uint32_t* __restrict__ idx,
+                                   uint32_t s, uint64_t W, uint64_t* __restrict__ bi,
+                                   ef2* __restrict__ bi_ef, ef2* __restrict__ bi_m1) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= W * s) return;
+  const uint64_t i = t / s;
+  const uint32_t k = (uint32_t)(t % s);
+  const uint64_t bit = (enc[(i >> 6) * ld + idx[k]] >> (i & 63)) & 1;
+  bi[t] = bit;
+  bi_ef[t] = ef2{bit, 0};
+  bi_m1[t] = ef2{bit ? 0 : GL_P - 1, 0};
+}
+
+// g2[i] = sum_k eq_r2[k] * bi[i * s + k] (ligesis.rs:445-455)
+__global__ void g2_kernel(const uint64_t* __restrict__ bi, uint32_t s, uint64_t W, const ef2* __restrict__ eq_r2,
+                          ef2* __restrict__ g2) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W) return;
+  ef2 acc = ef_zero();
+  for (uint32_t k = 0; k < s; k++)
+    if (bi[i * s + k]) acc = ef_add(acc, eq_r2[k]);
+  g2[i] = acc;
+}
+
+// v[i] = eqz1[i / 64] * 2^(i mod 64) (ligesis.rs:474-476)
+__global__ void vvec_kernel(const ef2* __restrict__ eqz1, uint64_t W, ef2* __restrict__ v) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W) return;
+  const uint32_t k = i & 63;
+  const uint64_t p2 = k < 64 ? gl_canon(uint64_t(1) << k) : 0;
+  v[i] = ef_mul_base(eqz1[i >> 6], p2);
+}
+
+// sel[idx[k]] = eq_r2[k] over a zeroed vector (ligesis.rs:488-491)
+__global__ void sel_kernel(const uint32_t* __restrict__ idx, const ef2* __restrict__ eq_r2, uint32_t s,
+                           ef2* __restrict__ sel) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < s) sel[idx[k]] = eq_r2[k];
+}
+
+__global__ void zero_kernel(uint64_t* __restrict__ p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// w[k] = prod_t (1 - r3[t] + r3[t] * g^((2^t k) mod 2n)) (rs_weights, ligesis.rs:553-567);
+// tw[e] = g^e for e < n, and g^(e + n) = -g^e.
+__global__ void rs_weights_kernel(const ef2* __restrict__ r3, int log2n, uint64_t n,
+                                  const uint64_t* __restrict__ tw, ef2* __restrict__ w) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint64_t mask = 2 * n - 1;
+  ef2 prod = ef_one();
+  for (int t = 0; t < log2n; t++) {
+    const uint64_t e = (k << t) & mask;
+    const uint64_t gp = e < n ? tw[e] : gl_neg(tw[e - n]);
+    const ef2 r = r3[t];
+    const ef2 f = ef_add(ef_one_minus(r), ef_mul_base(r, gp));
+    prod = ef_mul(prod, f);
+  }
+  w[k] = prod;
+}
+
+// Planar (c0 row, c1 row) -> interleaved extension vector.
+__global__ void planar_to_ef_kernel(const uint64_t* __restrict__ c0, const uint64_t* __restrict__ c1,
+                                    uint64_t n, ef2* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = ef2{c0[i], c1[i]};
+}
+
+// ---------------- product sumcheck round (sumcheck.rs:37-84) ----------------
+// Round over pairs (lo = p[i], hi = p[i + half]): g(t) = sum_i prod_j ((1 - t) lo_j + t hi_j) for
+// t = 0..D, then p[i] <- lo + r (hi - lo) in place (the high variable is bound first).
+template <int D>
+__global__ __launch_bounds__(256) void sc_round_kernel(ef2* __restrict__ p0, ef2* __restrict__ p1,
+                                                       ef2* __restrict__ p2, uint64_t half,
+                                                       const ef2* __restrict__ r_ptr,
+                                                       ef2* __restrict__ part) {
+  __shared__ ef2 sh[256 * (D + 1)];
+  ef2* ps[3] = {p0, p1, p2};
+  const ef2 r = *r_ptr;
+  ef2 acc[D + 1];
+#pragma unroll
+  for (int q = 0; q <= D; q++) acc[q] = ef_zero();
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += (uint64_t)gridDim.x * blockDim.x) {
+    ef2 lo[D], diff[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+      lo[j] = ps[j][i];
+      const ef2 hi = ps[j][i + half];
+      diff[j] = ef_sub(hi, lo[j]);
+      ps[j][i] = ef_add(lo[j], ef_mul(r, diff[j]));
+    }
+    // evaluations at t = 0..D by stepping x_j(t + 1) = x_j(t) + diff_j
+    ef2 x[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) x[j] = lo[j];
+#pragma unroll
+    for (int q = 0; q <= D; q++) {
+      ef2 prod = x[0];
+#pragma unroll
+      for (int j = 1; j < D; j++) prod = ef_mul(prod, x[j]);
+      acc[q] = ef_add(acc[q], prod);
+#pragma unroll
+      for (int j = 0; j < D; j++) x[j] = ef_add(x[j], diff[j]);
+    }
+  }
+  block_sum<D + 1>(acc, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q <= D; q++) part[(uint64_t)blockIdx.x * (D + 1) + q] = acc[q];
+  }
+}
+
+uint32_t sc_round_blocks(uint64_t half) {
+  uint64_t nb = (half + 255) / 256;
+  return (uint32_t)(nb > 1024 ? 1024 : nb);
+}
+
+hipError_t launch_sc_round(int d, ef2* const* polys, uint64_t half, const ef2* r, ef2* part,
+                           hipStream_t st) {
+  const uint32_t nb = sc_round_blocks(half);
+  if (d == 2)
+    hipLaunchKernelGGL(sc_round_kernel<2>, dim3(nb), dim3(256), 0, st, polys[0], polys[1], nullptr, half, r, part);
+  else if (d == 3)
+    hipLaunchKernelGGL(sc_round_kernel<3>, dim3(nb), dim3(256), 0, st, polys[0], polys[1], polys[2], half, r, part);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_rows(const ef2* part, uint32_t nb, uint32_t width, ef2* out, hipStream_t st) {
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(64), 0, st, part, nb, width, out);
+  return hipGetLastError();
+}
+
+// ---------------- Basefold batch (basefold.rs:648-856) ----------------
+// combined[i] = sum over channels c, chunks j of wt_c[j] * codes_c[j][i], where
+// wt_c[j] = coeffs[c] * eq4_c[j]; an extension channel keeps c0 / c1 in rows 2j / 2j+1.
+__global__ void bf_combine_kernel(BfChannels ch, uint64_t D, ef2* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D) return;
+  ef2 acc = ef_zero();
+  for (int c = 0; c < ch.count; c++) {
+    const uint64_t* codes = ch.codes[c];
+    const ef2* wt = ch.wt[c];
+    for (uint32_t j = 0; j < ch.chunks[c]; j++) {
+      if (ch.ext[c]) {
+        const ef2 v{codes[(2 * j) * D + i], codes[(2 * j + 1) * D + i]};
+        acc = ef_add(acc, ef_mul(wt[j], v));
+      } else {
+        acc = ef_add(acc, ef_mul_base(wt[j], codes[j * D + i]));
+      }
+    }
+  }
+  out[i] = acc;
+}
+
+// m[x] = sum_{b < 2^li} eq4[b] * poly[x * 2^li + b] (MlPoly::fold over the low li variables)
+__global__ void fold_low_kernel(const uint64_t* __restrict__ poly, int ext, int li, uint64_t nout,
+                                const ef2* __restrict__ eq4, ef2* __restrict__ m) {
+  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nout) return;
+  ef2 acc = ef_zero();
+  const uint64_t base = x << li;
+  for (uint32_t b = 0; b < (1u << li); b++) {
+    if (ext) {
+      const ef2 v{poly[2 * (base + b)], poly[2 * (base + b) + 1]};
+      acc = ef_add(acc, ef_mul(eq4[b], v));
+    } else {
+      acc = ef_add(acc, ef_mul_base(eq4[b], poly[base + b]));
+    }
+  }
+  m[x] = acc;
+}
+
+// One reduction round: partial sums of m[2i] * eq_rest[i] (l0, basefold.rs:752-758) and the
+// low-variable fold m'[i] = m[2i] + r (m[2i+1] - m[2i]) into a second buffer.
+__global__ __launch_bounds__(256) void bf_round_kernel(const ef2* __restrict__ m, uint64_t half,
+                                                       const ef2* __restrict__ eq_rest,
+                                                       const ef2* __restrict__ r_ptr,
+                                                       ef2* __restrict__ m_out, ef2* __restrict__ part) {
+  __shared__ ef2 sh[256];
+  const ef2 r = *r_ptr;
+  ef2 acc[1] = {ef_zero()};
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += (uint64_t)gridDim.x * blockDim.x) {
+    const ef2 e = m[2 * i], o = m[2 * i + 1];
+    acc[0] = ef_add(acc[0], ef_mul(e, eq_rest[i]));
+    m_out[i] = ef_add(e, ef_mul(r, ef_sub(o, e)));
+  }
+  block_sum<1>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// eq over point[1..] from eq over point[0..]: out[i] = in[2i] + in[2i+1]
+__global__ void eq_pairsum_kernel(const ef2* __restrict__ in, uint64_t nout, ef2* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nout) out[i] = ef_add(in[2 * i], in[2 * i + 1]);
+}
+
+// eq_fold_codeword (basefold.rs:201-218): out[i] = (1 - r)(x + nx)/2 + r (x - nx) g^-i / 2 with
+// x = v[i], nx = v[i + h]; g^-i = -g^(h - i) for 0 < i < h, tw[e] = g^e (e < h).
+__global__ void eq_fold_codeword_kernel(const ef2* __restrict__ v, uint64_t h, const ef2* __restrict__ r_ptr,
+                                        const uint64_t* __restrict__ tw, ef2* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= h) return;
+  const ef2 r = *r_ptr;
+  const ef2 x = v[i], nx = v[i + h];
+  const uint64_t ginv = i == 0 ? 1 : gl_neg(tw[h - i]);
+  const ef2 pe = ef_mul_base(ef_add(x, nx), GL_INV2);
+  const ef2 po = ef_mul_base(ef_sub(x, nx), gl_mul(ginv, GL_INV2));
+  out[i] = ef_add(ef_mul(ef_one_minus(r), pe), ef_mul(r, po));
+}
+
+// ---------------- gathers for openings ----------------
+__global__ void gather_words_kernel(const uint64_t* __restrict__ src, const uint64_t* __restrict__ off,
+                                    uint64_t count, uint64_t* __restrict__ dst) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) dst[t] = src[off[t]];
+}
+
+#define LGS_GRID(n) dim3((uint32_t)(((n) + 255) / 256)), dim3(256)
+
+hipError_t launch_bits_gather(const uint64_t* enc, uint64_t ld, const uint32_t* idx, uint32_t s, uint64_t W,
+                              uint64_t* bi, ef2* bi_ef, ef2* bi_m1, hipStream_t st) {
+  hipLaunchKernelGGL(bits_gather_kernel, LGS_GRID(W * s), 0, st, enc, ld, idx, s, W, bi, bi_ef, bi_m1);
+  return hipGetLastError();
+}
+hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st) {
+  hipLaunchKernelGGL(g2_kernel, LGS_GRID(W), 0, st, bi, s, W, eq_r2, g2);
+  return hipGetLastError();
+}
+hipError_t launch_vvec(const ef2* eqz1, uint64_t W, ef2* v, hipStream_t st) {
+  hipLaunchKernelGGL(vvec_kernel, LGS_GRID(W), 0, st, eqz1, W, v);
+  return hipGetLastError();
+}
+hipError_t launch_sel(const uint32_t* idx, const ef2* eq_r2, uint32_t s, ef2* sel, uint64_t len, hipStream_t st) {
+  hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, (uint64_t*)sel, 2 * len);
+  hipLaunchKernelGGL(sel_kernel, LGS_GRID(s), 0, st, idx, eq_r2, s, sel);
+  return hipGetLastError();
+}
+hipError_t launch_rs_weights(const ef2* r3, int log2n, uint64_t n, const uint64_t* tw, ef2* w, hipStream_t st) {
+  hipLaunchKernelGGL(rs_weights_kernel, LGS_GRID(n), 0, st, r3, log2n, n, tw, w);
+  return hipGetLastError();
+}
+hipError_t launch_planar_to_ef(const uint64_t* c0, const uint64_t* c1, uint64_t n, ef2* out, hipStream_t st) {
+  hipLaunchKernelGGL(planar_to_ef_kernel, LGS_GRID(n), 0, st, c0, c1, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_bf_combine(const BfChannels& ch, uint64_t D, ef2* out, hipStream_t st) {
+  hipLaunchKernelGGL(bf_combine_kernel, LGS_GRID(D), 0, st, ch, D, out);
+  return hipGetLastError();
+}
+hipError_t launch_fold_low(const uint64_t* poly, int ext, int li, uint64_t nout, const ef2* eq4, ef2* m,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(fold_low_kernel, LGS_GRID(nout), 0, st, poly, ext, li, nout, eq4, m);
+  return hipGetLastError();
+}
+uint32_t bf_round_blocks(uint64_t half) { return sc_round_blocks(half); }
+hipError_t launch_bf_round(const ef2* m, uint64_t half, const ef2* eq_rest, const ef2* r, ef2* m_out, ef2* part,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(bf_round_kernel, dim3(bf_round_blocks(half)), dim3(256), 0, st, m, half, eq_rest, r, m_out, part);
+  return hipGetLastError();
+}
+hipError_t launch_eq_pairsum(const ef2* in, uint64_t nout, ef2* out, hipStream_t st) {
+  hipLaunchKernelGGL(eq_pairsum_kernel, LGS_GRID(nout), 0, st, in, nout, out);
+  return hipGetLastError();
+}
+hipError_t launch_eq_fold_codeword(const ef2* v, uint64_t h, const ef2* r, const uint64_t* tw, ef2* out,
+                                   hipStream_t st) {
+  hipLaunchKernelGGL(eq_fold_codeword_kernel, LGS_GRID(h), 0, st, v, h, r, tw, out);
+  return hipGetLastError();
+}
+hipError_t launch_gather_words(const uint64_t* src, const uint64_t* off, uint64_t count, uint64_t* dst,
+                               hipStream_t st) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_words_kernel, LGS_GRID(count), 0, st, src, off, count, dst);
+  return hipGetLastError();
+}
+hipError_t launch_zero(uint64_t* p, uint64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace lgs

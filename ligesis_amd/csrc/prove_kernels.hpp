@@ -1,0 +1,44 @@
+// Launchers of the eval-prove kernels (prove.hip). All enqueue on `st` without synchronising.
+#pragma once
+#include "kernels.hpp"
+
+namespace lgs {
+
+// Up to three Basefold channels combined in one pass (basefold.rs:732-741).
+struct BfChannels {
+  int count;
+  const uint64_t* codes[3];
+  const ef2* wt[3];  // coeffs[c] * eq4_c[j]
+  uint32_t chunks[3];
+  int ext[3];
+};
+
+hipError_t launch_eq(const ef2* pt, int k, ef2* scratch, const ef2* scale, int accumulate, ef2* out,
+                     hipStream_t st);
+hipError_t launch_wrows(const uint64_t* M, uint32_t rows, uint64_t cols, const ef2* w, ef2* part,
+                        uint32_t chunks, ef2* out, hipStream_t st);
+hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part, ef2* out, hipStream_t st);
+hipError_t launch_bits_gather(const uint64_t* enc, uint64_t ld, const uint32_t* idx, uint32_t s, uint64_t W,
+                              uint64_t* bi, ef2* bi_ef, ef2* bi_m1, hipStream_t st);
+hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st);
+hipError_t launch_vvec(const ef2* eqz1, uint64_t W, ef2* v, hipStream_t st);
+hipError_t launch_sel(const uint32_t* idx, const ef2* eq_r2, uint32_t s, ef2* sel, uint64_t len, hipStream_t st);
+hipError_t launch_rs_weights(const ef2* r3, int log2n, uint64_t n, const uint64_t* tw, ef2* w, hipStream_t st);
+hipError_t launch_planar_to_ef(const uint64_t* c0, const uint64_t* c1, uint64_t n, ef2* out, hipStream_t st);
+uint32_t sc_round_blocks(uint64_t half);
+hipError_t launch_sc_round(int d, ef2* const* polys, uint64_t half, const ef2* r, ef2* part, hipStream_t st);
+hipError_t launch_reduce_rows(const ef2* part, uint32_t nb, uint32_t width, ef2* out, hipStream_t st);
+hipError_t launch_bf_combine(const BfChannels& ch, uint64_t D, ef2* out, hipStream_t st);
+hipError_t launch_fold_low(const uint64_t* poly, int ext, int li, uint64_t nout, const ef2* eq4, ef2* m,
+                           hipStream_t st);
+uint32_t bf_round_blocks(uint64_t half);
+hipError_t launch_bf_round(const ef2* m, uint64_t half, const ef2* eq_rest, const ef2* r, ef2* m_out, ef2* part,
+                           hipStream_t st);
+hipError_t launch_eq_pairsum(const ef2* in, uint64_t nout, ef2* out, hipStream_t st);
+hipError_t launch_eq_fold_codeword(const ef2* v, uint64_t h, const ef2* r, const uint64_t* tw, ef2* out,
+                                   hipStream_t st);
+hipError_t launch_gather_words(const uint64_t* src, const uint64_t* off, uint64_t count, uint64_t* dst,
+                               hipStream_t st);
+hipError_t launch_zero(uint64_t* p, uint64_t n, hipStream_t st);
+
+}  // namespace lgs

@@ -193,6 +193,10 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c);
 void enqueue_basefold_commit(Context& ctx, DevBasefold& st, const uint64_t* poly, bool ext,
                              uint32_t num_vars, uint32_t li, uint32_t code_rate);
 void setup_key(Context& ctx, Key& key, const uint64_t* host_a, bool commit_a);
+// prove.cpp
+int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uint64_t* z, uint64_t z_len,
+                   const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,
+                   std::vector<uint8_t>& proof, uint64_t used[2]);
 
 // Shape helpers restating pcs/src/ligesis.rs:175-203.
 inline void committed_vars(uint32_t lm, uint32_t ln, uint32_t out[3]) {

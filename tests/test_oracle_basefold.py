@@ -1,0 +1,35 @@
+"""Pins the oracle's shared-domain Basefold batch (oracle/basefold.hpp) with the reference's own
+batch tests, restated (pcs/src/basefold.rs:1187-1267): mixed base/extension sizes accepted,
+tampered reduction message / claim / base leaf / final codeword rejected."""
+import pytest
+
+import oracle
+from ligesis_amd import synth
+
+
+def _setup(bf_sizes, ef_sizes, seed):
+    bf = [synth.field_elems(seed + i, 1 << n) for i, n in enumerate(bf_sizes)]
+    ef = [synth.ext_elems(seed + 50 + i, 1 << n) for i, n in enumerate(ef_sizes)]
+    pts = [synth.ext_elems(seed + 100 + i, n) for i, n in enumerate(list(bf_sizes) + list(ef_sizes))]
+    fields = synth.ext_elems(seed + 200, 64)
+    ints = synth.oracle_ints(seed + 300, 256)
+    return bf, ef, pts, fields, ints
+
+
+def test_batch_mixed_sizes_and_fields_round_trip():
+    bf, ef, pts, fields, ints = _setup([12, 10, 9], [11], 1)
+    assert oracle.basefold_batch_roundtrip(bf, ef, pts, rounds=6, code_rate=1, fields=fields, ints=ints)
+
+
+@pytest.mark.parametrize("tamper", [1, 2, 3, 4])
+def test_batch_tampering_rejected(tamper):
+    bf, ef, pts, fields, ints = _setup([12, 10], [11], 2)
+    assert not oracle.basefold_batch_roundtrip(bf, ef, pts, rounds=6, code_rate=1, fields=fields,
+                                               ints=ints, tamper=tamper)
+
+
+def test_batch_extension_only_and_base_only():
+    bf, ef, pts, fields, ints = _setup([], [9, 8], 3)
+    assert oracle.basefold_batch_roundtrip(bf, ef, pts, rounds=5, code_rate=1, fields=fields, ints=ints)
+    bf, ef, pts, fields, ints = _setup([10], [], 4)
+    assert oracle.basefold_batch_roundtrip(bf, ef, pts, rounds=4, code_rate=1, fields=fields, ints=ints)
