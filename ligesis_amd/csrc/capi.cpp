@@ -19,6 +19,23 @@ struct lgs_commitment {
 
 namespace {
 
+void ctx_retain(lgs::Context* c) { c->refs.fetch_add(1); }
+
+// Drops one reference; the last one tears the context down.
+void ctx_unref(lgs::Context* c) {
+  if (!c || c->refs.fetch_sub(1) != 1) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->twiddles.clear();
+  c->pool.trim();
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  c->pinned = nullptr;
+  (void)hipStreamDestroy(c->stream);
+  delete (lgs_ctx*)c->owner;
+}
+
 template <class F>
 int guarded(lgs::Context* ctx, F&& f) {
   try {
@@ -90,6 +107,7 @@ int lgs_init(int device, lgs_ctx** out) {
   if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return LGS_ERR_DEVICE;
   if (hipSetDevice(device) != hipSuccess) return LGS_ERR_DEVICE;
   auto* ctx = new lgs_ctx();
+  ctx->c.owner = ctx;
   ctx->c.device = device;
   if (hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
@@ -106,15 +124,7 @@ int lgs_init(int device, lgs_ctx** out) {
 
 void lgs_shutdown(lgs_ctx* ctx) {
   if (!ctx) return;
-  (void)hipSetDevice(ctx->c.device);
-  (void)hipStreamSynchronize(ctx->c.stream);
-  ctx->c.twiddles.clear();
-  ctx->c.pool.trim();
-  for (auto& e : ctx->c.ev)
-    if (e) (void)hipEventDestroy(e);
-  if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
-  (void)hipStreamDestroy(ctx->c.stream);
-  delete ctx;
+  ctx_unref(&ctx->c);
 }
 
 const char* lgs_last_error(const lgs_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
@@ -160,9 +170,11 @@ int lgs_setup(lgs_ctx* ctx, uint32_t log_m, uint32_t log_n, const uint64_t* a, u
     try {
       lgs::setup_key(ctx->c, k->k, a, !commit_only);
     } catch (...) {
+      (void)hipStreamSynchronize(ctx->c.stream);
       delete k;
       throw;
     }
+    ctx_retain(&ctx->c);
     *out = k;
   });
 }
@@ -176,8 +188,10 @@ int lgs_key_com_a(const lgs_key* key, uint8_t com_a[32]) {
 
 void lgs_key_release(lgs_key* key) {
   if (!key) return;
-  if (key->k.ctx) (void)hipStreamSynchronize(key->k.ctx->stream);
+  lgs::Context* c = key->k.ctx;
+  if (c) (void)hipStreamSynchronize(c->stream);
   delete key;
+  ctx_unref(c);
 }
 
 int lgs_commit(lgs_ctx* ctx, const lgs_key* key, const uint64_t* f, uint64_t len, uint32_t flags,
@@ -199,6 +213,7 @@ int lgs_commit(lgs_ctx* ctx, const lgs_key* key, const uint64_t* f, uint64_t len
       throw;
     }
     std::memcpy(root, c->c.root, 32);
+    ctx_retain(&ctx->c);
     *out = c;
   });
 }
@@ -237,6 +252,7 @@ int lgs_commit_batch(lgs_ctx* ctx, uint32_t count, const lgs_key* const* keys,
   }
   for (uint32_t i = 0; i < count; i++) {
     std::memcpy(roots + 32 * (size_t)i, made[i]->c.root, 32);
+    ctx_retain(&ctx->c);
     out[i] = made[i];
   }
   return LGS_OK;
@@ -271,8 +287,10 @@ int lgs_prove(lgs_ctx* ctx, const lgs_key* key, const lgs_commitment* c, const u
 
 void lgs_commitment_release(lgs_commitment* c) {
   if (!c) return;
-  if (c->c.ctx) (void)hipStreamSynchronize(c->c.ctx->stream);
+  lgs::Context* cx = c->c.ctx;
+  if (cx) (void)hipStreamSynchronize(cx->stream);
   delete c;
+  ctx_unref(cx);
 }
 
 int lgs_commitment_read(const lgs_commitment* c, uint32_t what, void* out, uint64_t cap) {

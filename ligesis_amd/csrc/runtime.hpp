@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -58,7 +59,9 @@ class DevicePool {
   void release(void* p) {
     if (!p) return;
     std::lock_guard<std::mutex> g(mu_);
-    free_[sizes_.at(p)].push_back(p);
+    auto it = sizes_.find(p);
+    if (it == sizes_.end()) return;  // not from this pool: never throw from a destructor path
+    free_[it->second].push_back(p);
   }
   void trim() {
     std::lock_guard<std::mutex> g(mu_);
@@ -147,6 +150,10 @@ struct DevBasefold {
 };
 
 struct Context {
+  // Keys and commitments keep their context alive: lgs_shutdown drops the caller's reference
+  // and the context is torn down when the last key / commitment made on it is released.
+  std::atomic<int> refs{1};
+  void* owner = nullptr;  // the lgs_ctx that embeds this context
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;

@@ -125,3 +125,18 @@ def test_full_size_square_mu24_chain(gpu_ctx):
     root2, cd2 = Ligesis.commit(pk, inp.f)
     assert root2 == root
     gpu_ctx.set_option("hash", 0)
+
+
+def test_handles_outlive_context_shutdown():
+    """Regression: keys and commitments released after lgs_shutdown must not touch freed state
+    (the context is reference-counted by its keys and commitments)."""
+    import ligesis_amd as L
+
+    ctx = L.Context(0)
+    inp = synth.Inputs(2, 6, seed=9)
+    pk, _ = L.Ligesis.setup(2, 6, inp.a, ctx=ctx, commit_only=True)
+    root, cd = L.Ligesis.commit(pk, inp.f)
+    ctx.close()
+    assert cd.digests().shape == (32, 128)  # still usable after the caller's shutdown
+    pk.release()
+    cd.release()
