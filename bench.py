@@ -61,6 +61,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--hash", type=int, default=0, help="0 auto, 1 MFMA, 2 VALU")
+    ap.add_argument("--hash-impl", type=int, default=0, help="MFMA kernel: 0 pipelined, 1 pipelined/1 slab, 2 basic")
+    ap.add_argument("--ksplit", type=int, default=0, help="hash K-split (0 auto)")
+    ap.add_argument("--ntt-impl", type=int, default=0, help="0 radix-8 passes, 1 radix-2 stages")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -77,6 +80,9 @@ def main():
     inp = synth.Inputs(LOG_M, LOG_N, seed=1, n_fields=4, n_ints=4)
     ctx = L.Context(0)
     ctx.set_option("hash", args.hash)
+    ctx.set_option("hash_impl", args.hash_impl)
+    ctx.set_option("hash_ksplit", args.ksplit)
+    ctx.set_option("ntt_impl", args.ntt_impl)
     pk, _ = L.Ligesis.setup(LOG_M, LOG_N, inp.a, ctx=ctx, commit_only=True)
     f_dev = torch.from_numpy(inp.f.view(np.int64)).to("cuda:0")
     torch.cuda.synchronize()
@@ -112,11 +118,11 @@ def main():
     hash_tops = hash_ops / (avg["hash_ms"] / 1e3) / 1e12
     ntt_gbs = ntt_bytes / (avg["ntt_ms"] / 1e3) / 1e9
     if avg["hash_ms"] >= avg["ntt_ms"]:
-        roof = {"kernel": "hash_mfma_kernel (fused bit-decompose + H = A.B)", "bound": "mfma",
+        roof = {"kernel": "hash_mfma_pipe_kernel (fused bit-decompose + H = A.B)", "bound": "mfma",
                 "achieved": hash_tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                 "frac": hash_tops / INT8_PEAK_TOPS, "traffic": None}
     else:
-        roof = {"kernel": "ntt_lds_kernel (row LDE)", "bound": "hbm", "achieved": ntt_gbs,
+        roof = {"kernel": "ntt_r8_kernel (row LDE)", "bound": "hbm", "achieved": ntt_gbs,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ntt_gbs / HBM_PEAK_GBS,
                 "traffic": None}
     out = {
@@ -134,7 +140,8 @@ def main():
         "data": "synthetic (SplitMix64-seeded uniform Goldilocks f and A)",
         "config": {"workload": "plain single-polynomial commit, mu=24, m x n = 2^12 x 2^12, "
                                "rho=1/2, eta=64, C=32, BLAKE3 Merkle over 2^13 columns",
-                   "log_m": LOG_M, "log_n": LOG_N, "hash_variant": args.hash},
+                   "log_m": LOG_M, "log_n": LOG_N, "hash_variant": args.hash,
+                   "hash_impl": args.hash_impl, "hash_ksplit": args.ksplit, "ntt_impl": args.ntt_impl},
         "stages_ms": avg,
         "roofline": roof,
         "ntt_roofline": {"bound": "hbm", "achieved": ntt_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -60,7 +60,9 @@ void lgs_shutdown(lgs_ctx* ctx);
 const char* lgs_last_error(const lgs_ctx* ctx);
 
 /* Tuning knobs that never change results: "hash" (0 auto, 1 MFMA, 2 VALU), "hash_ksplit"
- * (0 auto), "profile" (1: time every stage with HIP events on the context stream). */
+ * (0 auto), "hash_impl" (MFMA kernel: 0 pipelined, 1 pipelined one slab per stage, 2 basic),
+ * "ntt_impl" (0 radix-8 passes, 1 radix-2 stages), "profile" (1: time every stage with HIP
+ * events on the context stream). */
 int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value);
 
 /* Stage times of the last profiled commit, in milliseconds: [0] row NTT, [1] hash main

@@ -101,7 +101,7 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c) {
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[0], ctx.stream), "event");
   // F'[i] = RS(F[i]) (ligesis.rs:289-291)
   hip_check(launch_ntt(c.f, n, 1, (uint32_t)n, (uint32_t)m, logN, ctx.twiddle(logN), c.enc.ptr,
-                       ctx.stream),
+                       ctx.stream, ctx.ntt_impl),
             "row encode");
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[1], ctx.stream), "event");
   // H = A . B with B read bitwise from F' (ligesis.rs:292-302)
@@ -115,7 +115,7 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c) {
     if (ks == 0 || (m % (2 * ks)) != 0) throw Error(1, "hash_ksplit must divide m / 2");
     DevBuf<int32_t> part(&ctx.pool, (size_t)ks * 256 * two_n);
     hip_check(launch_hash_mfma(c.enc.ptr, two_n, (uint32_t)m, (uint32_t)two_n, key.dig.ptr, ks, part.ptr,
-                               c.h.ptr, ctx.stream, ctx.profile ? ctx.ev[2] : nullptr),
+                               c.h.ptr, ctx.stream, ctx.profile ? ctx.ev[2] : nullptr, ctx.hash_impl),
               "hash (MFMA)");
   } else {
     uint64_t nb = (two_n + 255) / 256;

@@ -142,6 +142,105 @@ __global__ __launch_bounds__(256) void hash_mfma_kernel(const uint64_t* __restri
   }
 }
 
+// Pipelined variant: A digits double-buffered in LDS (one barrier per stage of SPS slabs), the
+// next stage's digits and F' words loaded into registers while the current stage computes,
+// and at most 256 registers per lane so two workgroups share a CU.
+template <int SPS>
+__global__ __launch_bounds__(256, 2) void hash_mfma_pipe_kernel(const uint64_t* __restrict__ enc,
+                                                                uint64_t ld, const v4i* __restrict__ adig,
+                                                                uint32_t slabs_per_split, uint32_t ntiles,
+                                                                uint32_t ncols, int32_t* __restrict__ part) {
+  __shared__ v4i lds_a[2][SPS * 1024];
+  const uint32_t nb = gridDim.x;
+  const uint32_t per_xcd = nb / 8;
+  const uint32_t bid = blockIdx.x;
+  const uint32_t logical = (nb % 8 == 0) ? (bid % 8) * per_xcd + bid / 8 : bid;
+  const uint32_t split = logical / ntiles;
+  const uint32_t ntile = logical % ntiles;
+
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t wm = wave >> 1, wn = wave & 1;
+  const uint32_t g = lane >> 4, c = lane & 15;
+  const uint32_t kb = 32 * (g >> 1) + 4 * (g & 1);
+  const uint64_t col_base = (uint64_t)ntile * HM_NT + wn * 64 + c;
+
+  v4i acc[8][4];
+#pragma unroll
+  for (int a = 0; a < 8; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = v4i{0, 0, 0, 0};
+
+  const uint64_t slab0 = (uint64_t)split * slabs_per_split;
+  const uint32_t nstages = slabs_per_split / SPS;
+  v4i pa[SPS * 4];
+  uint64_t px[SPS][4];
+  {
+    const v4i* src = adig + slab0 * 1024;
+#pragma unroll
+    for (int q = 0; q < SPS * 4; q++) pa[q] = src[q * 256 + threadIdx.x];
+#pragma unroll
+    for (int ss = 0; ss < SPS; ss++)
+#pragma unroll
+      for (int nt = 0; nt < 4; nt++) px[ss][nt] = enc[(slab0 + ss) * ld + col_base + nt * 16];
+#pragma unroll
+    for (int q = 0; q < SPS * 4; q++) lds_a[0][q * 256 + threadIdx.x] = pa[q];
+  }
+  __syncthreads();
+  for (uint32_t st = 0; st < nstages; st++) {
+    const uint32_t buf = st & 1;
+    uint64_t cx[SPS][4];
+#pragma unroll
+    for (int ss = 0; ss < SPS; ss++)
+#pragma unroll
+      for (int nt = 0; nt < 4; nt++) cx[ss][nt] = px[ss][nt];
+    const bool more = st + 1 < nstages;
+    if (more) {
+      const uint64_t s1 = slab0 + (uint64_t)(st + 1) * SPS;
+      const v4i* src = adig + s1 * 1024;
+#pragma unroll
+      for (int q = 0; q < SPS * 4; q++) pa[q] = src[q * 256 + threadIdx.x];
+#pragma unroll
+      for (int ss = 0; ss < SPS; ss++)
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) px[ss][nt] = enc[(s1 + ss) * ld + col_base + nt * 16];
+    }
+#pragma unroll
+    for (int ss = 0; ss < SPS; ss++) {
+      v4i b[4];
+#pragma unroll
+      for (int nt = 0; nt < 4; nt++) {
+        const uint32_t y = (uint32_t)(cx[ss][nt] >> kb);
+        b[nt] = v4i{(int)(y & 0x01010101u), (int)((y >> 1) & 0x01010101u),
+                    (int)((y >> 2) & 0x01010101u), (int)((y >> 3) & 0x01010101u)};
+      }
+#pragma unroll
+      for (int mt = 0; mt < 8; mt++) {
+        const v4i a = lds_a[buf][ss * 1024 + (wm * 8 + mt) * 64 + lane];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+          acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[nt], acc[mt][nt], 0, 0, 0);
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < SPS * 4; q++) lds_a[buf ^ 1][q * 256 + threadIdx.x] = pa[q];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int mt = 0; mt < 8; mt++) {
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const uint32_t m = (wm * 8 + mt) * 16 + 4 * g + v;
+        const uint64_t col = (uint64_t)ntile * HM_NT + wn * 64 + nt * 16 + c;
+        part[((uint64_t)split * 256 + m) * ncols + col] = acc[mt][nt][v];
+      }
+    }
+  }
+}
+
 // H[r][j] = sum_l 256^l (sum_split part[split][32 l + r][j]) mod p.
 __global__ void hash_mfma_finalize_kernel(const int32_t* __restrict__ part, uint32_t ksplit,
                                           uint32_t ncols, uint64_t* __restrict__ h) {
@@ -218,11 +317,18 @@ hipError_t launch_hash_valu(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
 
 hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
                             const int8_t* dig, uint32_t ksplit, int32_t* part, uint64_t* h,
-                            hipStream_t st, hipEvent_t mid) {
+                            hipStream_t st, hipEvent_t mid, int impl) {
   uint32_t ntiles = ncols / HM_NT;
   uint32_t sps = m_rows / ksplit;
-  hipLaunchKernelGGL(hash_mfma_kernel, dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
-                     (const v4i*)dig, sps, ntiles, ncols, part);
+  if (impl == 2)
+    hipLaunchKernelGGL(hash_mfma_kernel, dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
+                       (const v4i*)dig, sps, ntiles, ncols, part);
+  else if (impl == 1)
+    hipLaunchKernelGGL(hash_mfma_pipe_kernel<1>, dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
+                       (const v4i*)dig, sps, ntiles, ncols, part);
+  else
+    hipLaunchKernelGGL(hash_mfma_pipe_kernel<2>, dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
+                       (const v4i*)dig, sps, ntiles, ncols, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (mid && (e = hipEventRecord(mid, st)) != hipSuccess) return e;

@@ -11,8 +11,10 @@ namespace lgs {
 
 // ntt.hip
 hipError_t launch_twiddles(uint64_t* tw, int logN, hipStream_t st);
+// impl 0: radix-8 passes with the whole row in LDS when 2^logN <= 2^LGS_LDS_LOG; otherwise (or
+// impl 1) radix-2 stages, the first LGS_LDS_LOG of them in LDS and the rest as global passes.
 hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_in, uint32_t rows,
-                      int logN, const uint64_t* tw, uint64_t* out, hipStream_t st);
+                      int logN, const uint64_t* tw, uint64_t* out, hipStream_t st, int impl = 0);
 
 // hash.hip (`mid`, when not null, is recorded between the main kernel and the finalize)
 hipError_t launch_a_layouts(const uint64_t* a, uint64_t W, uint64_t* at, int8_t* dig, hipStream_t st);
@@ -21,7 +23,7 @@ hipError_t launch_hash_valu(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
                             hipStream_t st, hipEvent_t mid);
 hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
                             const int8_t* dig, uint32_t ksplit, int32_t* part, uint64_t* h,
-                            hipStream_t st, hipEvent_t mid);
+                            hipStream_t st, hipEvent_t mid, int impl);
 
 // merkle.hip: tree over nleaves (a power of two) leaves, leaf p = M[r*rs + p*cs], r < rows.
 hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs,

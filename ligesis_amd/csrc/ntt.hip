@@ -64,6 +64,176 @@ __global__ __launch_bounds__(256) void ntt_lds_kernel(const uint64_t* __restrict
   for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) dst[t] = sm[t];
 }
 
+// ---------------- radix-8 row transform (whole row in LDS) ----------------
+// Roots of unity of order <= 64 are powers of two in Goldilocks: w_64 = 2^39 and
+// w_{2^k} = 2^(39 * 2^(6-k) mod 192) (2^96 = -1, 2^192 = 1). Multiplying by them is a shift and
+// a reduction, so the internal twiddles of an 8-point DFT and every twiddle of the first two
+// radix-8 passes cost no general multiplication.
+
+// x * 2^e mod p for 0 <= e < 192
+__device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x, uint32_t e) {
+  const bool neg = e >= 96;
+  if (neg) e -= 96;
+  uint64_t r;
+  if (e < 64) {
+    r = gl_reduce(x << e, e ? (x >> (64 - e)) : 0);
+  } else {  // x * 2^e = (x * 2^(e - 32)) * 2^32, both factors below 2^64 shifts
+    const uint64_t y = gl_reduce(x << (e - 32), x >> (96 - e));
+    r = gl_reduce(y << 32, y >> 32);
+  }
+  return neg ? gl_neg(r) : r;
+}
+
+// w_N^k for k < N from the half table tw[k] = w^k (k < N/2), w^(k + N/2) = -w^k.
+__device__ __forceinline__ uint64_t tw_full(const uint64_t* __restrict__ tw, uint32_t k, uint32_t half) {
+  return k < half ? tw[k] : gl_neg(tw[k - half]);
+}
+
+// In-place radix-2 DIT network on 8 values held in bit-reversed order; output natural order.
+__device__ __forceinline__ void dft8(uint64_t (&x)[8]) {
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const uint64_t a = x[2 * u], b = x[2 * u + 1];
+    x[2 * u] = gl_add(a, b);
+    x[2 * u + 1] = gl_sub(a, b);
+  }
+#pragma unroll
+  for (int h = 0; h < 8; h += 4) {
+    uint64_t a = x[h], b = x[h + 2];
+    x[h] = gl_add(a, b);
+    x[h + 2] = gl_sub(a, b);
+    a = x[h + 1];
+    b = gl_mul_pow2(x[h + 3], 48);  // w_4
+    x[h + 1] = gl_add(a, b);
+    x[h + 3] = gl_sub(a, b);
+  }
+  {
+    uint64_t a = x[0], b = x[4];
+    x[0] = gl_add(a, b);
+    x[4] = gl_sub(a, b);
+    a = x[1];
+    b = gl_mul_pow2(x[5], 120);  // w_8
+    x[1] = gl_add(a, b);
+    x[5] = gl_sub(a, b);
+    a = x[2];
+    b = gl_mul_pow2(x[6], 48);  // w_8^2
+    x[2] = gl_add(a, b);
+    x[6] = gl_sub(a, b);
+    a = x[3];
+    b = gl_mul_pow2(x[7], 168);  // w_8^3
+    x[3] = gl_add(a, b);
+    x[7] = gl_sub(a, b);
+  }
+}
+
+// Multiply x[t] by c^rev(t) where c = w_{2^(s+lg)}^j and rev is the lg-bit reversal.
+template <int LG>
+__device__ __forceinline__ void pretwiddle(uint64_t (&x)[1 << LG], uint32_t j, int s, int logN,
+                                           const uint64_t* __restrict__ tw) {
+  constexpr int R = 1 << LG;
+  const int lsz = s + LG;  // log2 of this pass's transform size
+  if (lsz <= 6) {
+    const uint32_t e0 = (39u << (6 - lsz)) % 192;
+#pragma unroll
+    for (int t = 1; t < R; t++) {
+      const uint32_t r = __brev((uint32_t)t) >> (32 - LG);
+      x[t] = gl_mul_pow2(x[t], (uint32_t)(((uint64_t)e0 * j * r) % 192));
+    }
+  } else {
+    const uint32_t half = 1u << (logN - 1);
+#pragma unroll
+    for (int t = 1; t < R; t++) {
+      const uint32_t r = __brev((uint32_t)t) >> (32 - LG);
+      x[t] = gl_mul(x[t], tw_full(tw, (j * r) << (logN - lsz), half));
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void ntt_r8_kernel(const uint64_t* __restrict__ in, uint64_t rs,
+                                                     uint64_t es, uint32_t n_in, int logN,
+                                                     const uint64_t* __restrict__ tw,
+                                                     uint64_t* __restrict__ out) {
+  extern __shared__ uint64_t sm[];
+  const uint32_t N = 1u << logN;
+  const uint64_t row = blockIdx.x;
+  const uint64_t* src = in + row * rs;
+  uint64_t* dst = out + (row << logN);
+  int s = 0;
+  if (logN >= 3) {
+    // first pass fused with the load: thread k takes x[k + r N/8] (r = 0..7), which sit at
+    // positions rev3(r) of block rev_{logN-3}(k) of the bit-reversed array
+    const uint32_t E = N / 8;
+    const bool last = logN == 3;
+    for (uint32_t k = threadIdx.x; k < E; k += blockDim.x) {
+      uint64_t x[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        const uint32_t r = __brev((uint32_t)t) >> 29;
+        const uint32_t idx = k + r * E;
+        x[t] = idx < n_in ? src[(uint64_t)idx * es] : 0;
+      }
+      dft8(x);
+      const uint32_t b = logN > 3 ? brev_bits(k, logN - 3) : 0;
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        if (last) dst[8 * b + t] = x[t];
+        else sm[8 * b + t] = x[t];
+      }
+    }
+    s = 3;
+  } else {
+    for (uint32_t k = threadIdx.x; k < N; k += blockDim.x)
+      sm[brev_bits(k, logN)] = k < n_in ? src[(uint64_t)k * es] : 0;
+  }
+  __syncthreads();
+  while (logN - s >= 3) {
+    const bool last = logN - s == 3;
+    const uint32_t M = 1u << s;
+    for (uint32_t gi = threadIdx.x; gi < N / 8; gi += blockDim.x) {
+      const uint32_t j = gi & (M - 1);
+      const uint32_t base = ((gi >> s) << (s + 3)) + j;
+      uint64_t x[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++) x[t] = sm[base + t * M];
+      if (j) pretwiddle<3>(x, j, s, logN, tw);
+      dft8(x);
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        if (last) dst[base + t * M] = x[t];
+        else sm[base + t * M] = x[t];
+      }
+    }
+    if (!last) __syncthreads();
+    s += 3;
+  }
+  if (logN - s == 2) {
+    const uint32_t M = 1u << s;
+    for (uint32_t gi = threadIdx.x; gi < N / 4; gi += blockDim.x) {
+      const uint32_t j = gi & (M - 1);
+      const uint32_t base = ((gi >> s) << (s + 2)) + j;
+      uint64_t x[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) x[t] = sm[base + t * M];
+      if (j) pretwiddle<2>(x, j, s, logN, tw);
+      uint64_t a0 = gl_add(x[0], x[1]), a1 = gl_sub(x[0], x[1]);
+      uint64_t a2 = gl_add(x[2], x[3]), a3 = gl_mul_pow2(gl_sub(x[2], x[3]), 48);
+      dst[base] = gl_add(a0, a2);
+      dst[base + 2 * M] = gl_sub(a0, a2);
+      dst[base + M] = gl_add(a1, a3);
+      dst[base + 3 * M] = gl_sub(a1, a3);
+    }
+  } else if (logN - s == 1) {
+    const uint32_t M = 1u << s;
+    for (uint32_t j = threadIdx.x; j < M; j += blockDim.x) {
+      uint64_t a = sm[j], b = sm[j + M];
+      if (j) b = s + 1 <= 6 ? gl_mul_pow2(b, (uint32_t)(((uint64_t)((39u << (6 - (s + 1))) % 192) * j) % 192))
+                            : gl_mul(b, tw[(uint64_t)j << (logN - s - 1)]);
+      dst[j] = gl_add(a, b);
+      dst[j + M] = gl_sub(a, b);
+    }
+  }
+}
+
 // One global radix-2 stage s (> lt) over every row, in place.
 __global__ __launch_bounds__(256) void ntt_global_stage_kernel(uint64_t* __restrict__ data,
                                                                uint32_t rows, int logN, int s,
@@ -97,8 +267,15 @@ hipError_t launch_twiddles(uint64_t* tw, int logN, hipStream_t st) {
 }
 
 hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_in, uint32_t rows,
-                      int logN, const uint64_t* tw, uint64_t* out, hipStream_t st) {
+                      int logN, const uint64_t* tw, uint64_t* out, hipStream_t st, int impl) {
   if (rows == 0) return hipSuccess;
+  if (impl == 0 && logN >= 1 && logN <= LGS_LDS_LOG) {
+    const uint32_t N = 1u << logN;
+    const uint32_t threads = N / 8 >= 256 ? 256 : 64;
+    hipLaunchKernelGGL(ntt_r8_kernel, dim3(rows), dim3(threads), N * sizeof(uint64_t), st, in, rs, es,
+                       n_in, logN, tw, out);
+    return hipGetLastError();
+  }
   int lt = logN < LGS_LDS_LOG ? logN : LGS_LDS_LOG;
   uint32_t T = 1u << lt;
   uint32_t nblk = 1u << (logN - lt);

@@ -161,6 +161,8 @@ struct Context {
   std::map<int, DevBuf<uint64_t>> twiddles;
   int hash_variant = 0;  // 0 auto, 1 MFMA, 2 VALU
   int hash_ksplit = 0;
+  int hash_impl = 0;  // MFMA kernel: 0 pipelined (2 slabs per stage), 1 pipelined (1), 2 basic
+  int ntt_impl = 0;   // row transform: 0 radix-8 passes, 1 radix-2 stages
   bool profile = false;
   double stage_ms[8] = {0};
   hipEvent_t ev[8] = {};
