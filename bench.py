@@ -64,13 +64,84 @@ def main():
     ap.add_argument("--hash-impl", type=int, default=0, help="MFMA kernel: 0 pipelined, 1 pipelined/1 slab, 2 basic")
     ap.add_argument("--ksplit", type=int, default=0, help="hash K-split (0 auto)")
     ap.add_argument("--ntt-impl", type=int, default=0, help="0 radix-8 passes, 1 radix-2 stages")
+    ap.add_argument("--workload", default="commit", choices=["commit", "prove"],
+                    help="commit: the headline metric; prove: mu=24 eval-prove at 2^8 x 2^16")
+    ap.add_argument("--log-m", type=int, default=8, help="prove workload: log2 rows")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(every box here has one GPU; see DESIGN.md for how the commit would shard)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "prove":
+        bench_prove(args)
+    else:
+        bench_commit(args)
 
+
+def bench_prove(args):
+    """Eval-prove at mu = 24 on a reference-valid shape (2^8 x 2^16 by default; the reference's
+    own bench shape 2^7 x 2^17 with --log-m 7): one step = one full proof (all seven sumchecks
+    and the batched Basefold open) on a resident commitment, ending with the proof bytes on the
+    host."""
+    import ligesis_amd as L
+    from ligesis_amd import synth
+
+    lm = args.log_m
+    ln = MU - lm
+    print(f"bench: prove mu={MU} shape 2^{lm} x 2^{ln}", file=sys.stderr, flush=True)
+    inp = synth.Inputs(lm, ln, seed=5)
+    ctx = L.Context(0)
+    pk, vk = L.Ligesis.setup(lm, ln, inp.a, ctx=ctx)
+    root, cd = L.Ligesis.commit(pk, inp.f)
+    proofs = set()
+    for _ in range(args.warmup):
+        proofs.add(L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints)))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        proofs.add(L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints)))
+    dt = time.perf_counter() - t0
+    assert len(proofs) == 1, "prove is not deterministic"
+    proof = next(iter(proofs))
+    ms = dt / args.steps * 1e3
+    out = {
+        "metric": "LigeSIS eval-prove ms at mu=24",
+        "value": ms,
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 / F_p^2 (Goldilocks)",
+        "data": "synthetic (SplitMix64-seeded f, A, z and oracle streams)",
+        "config": {"workload": f"LigeSIS eval-prove, mu=24, m x n = 2^{lm} x 2^{ln}, s=128, "
+                               "100 queries, proof on a resident commitment", "log_m": lm, "log_n": ln},
+        "proof_bytes": len(proof),
+    }
+    del cd, pk
+    ctx.close()
+    if not args.no_cpu_baseline:
+        import oracle
+
+        slm, sln = 6, 14  # mu = 20 opening shape
+        s_inp = synth.Inputs(slm, sln, seed=6)
+        opk = oracle.ProvingKey(slm, sln, s_inp.a)
+        ocd = oracle.CommitData(opk, s_inp.f)
+        t0 = time.perf_counter()
+        oracle.prove(opk, ocd, s_inp.z, s_inp.fields, s_inp.ints)
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cdt * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": f"oracle prove at mu=20 (2^{slm} x 2^{sln}), one thread, "
+                                         "1/16 of the mu=24 workload's evaluations"}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def bench_commit(args):
     import torch
 
     import ligesis_amd as L
