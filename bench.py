@@ -64,8 +64,10 @@ def main():
     ap.add_argument("--hash-impl", type=int, default=0, help="MFMA kernel: 0 pipelined, 1 pipelined/1 slab, 2 basic")
     ap.add_argument("--ksplit", type=int, default=0, help="hash K-split (0 auto)")
     ap.add_argument("--ntt-impl", type=int, default=0, help="0 radix-8 passes, 1 radix-2 stages")
-    ap.add_argument("--workload", default="commit", choices=["commit", "prove"],
-                    help="commit: the headline metric; prove: mu=24 eval-prove at 2^8 x 2^16")
+    ap.add_argument("--workload", default="commit", choices=["commit", "prove", "batch"],
+                    help="commit: the headline metric; prove: mu=24 eval-prove at 2^8 x 2^16; "
+                         "batch: 64 polynomials with mu uniform in [16, 24] in one batched commit")
+    ap.add_argument("--batch", type=int, default=64, help="batch workload: number of polynomials")
     ap.add_argument("--log-m", type=int, default=8, help="prove workload: log2 rows")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -75,8 +77,92 @@ def main():
         sys.exit(2)
     if args.workload == "prove":
         bench_prove(args)
+    elif args.workload == "batch":
+        bench_batch(args)
     else:
         bench_commit(args)
+
+
+def bench_batch(args):
+    """Batched commit: `--batch` polynomials with mu drawn uniformly from [16, 24] (sizes
+    log-uniform), m = 2^floor(mu/2), n = 2^ceil(mu/2), uniform Goldilocks evaluations resident in
+    HBM. One step = one lgs_commit_batch of all of them (grouped launches: one launch per stage
+    for the whole batch); it is compared with committing the same polynomials one by one."""
+    import torch
+
+    import ligesis_amd as L
+    from ligesis_amd import synth
+
+    rng = np.random.default_rng(2024)
+    mus = rng.integers(16, 25, size=args.batch)
+    shapes = [(int(mu) // 2, int(mu) - int(mu) // 2) for mu in mus]
+    print(f"bench: batched commit of {args.batch} polys, mu in [16,24]", file=sys.stderr, flush=True)
+    ctx = L.Context(0)
+    keys = {}
+    for lm, ln in sorted(set(shapes)):
+        a = synth.field_elems(1000 + 64 * lm + ln, 32 * (64 << lm))
+        keys[(lm, ln)] = L.Ligesis.setup(lm, ln, a, ctx=ctx, commit_only=True)[0]
+    total = sum(1 << (lm + ln) for lm, ln in shapes)
+    pool = synth.field_elems(77, total)
+    fs, off = [], 0
+    for lm, ln in shapes:
+        sz = 1 << (lm + ln)
+        fs.append(torch.from_numpy(pool[off:off + sz].view(np.int64)).to("cuda:0"))
+        off += sz
+    ks = [keys[s] for s in shapes]
+    torch.cuda.synchronize()
+
+    def run_batch():
+        res = L.Ligesis.commit_batch(ks, fs, ctx=ctx, device=True)
+        roots = [r for r, _ in res]
+        for _, cd in res:
+            cd.release()
+        return roots
+
+    def run_single():
+        roots = []
+        for k, f in zip(ks, fs):
+            r, cd = L.Ligesis.commit(k, f, device=True)
+            roots.append(r)
+            cd.release()
+        return roots
+
+    for _ in range(args.warmup):
+        ref = run_single()
+        assert run_batch() == ref, "batched roots differ from per-polynomial roots"
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_batch()
+    dt_b = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_single()
+    dt_s = (time.perf_counter() - t0) / args.steps
+    out = {
+        "metric": "LigeSIS batched commit polys/s (64 polys, mu uniform in [16,24])",
+        "value": args.batch / dt_b,
+        "unit": "polys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt_b * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (Goldilocks), int8 MFMA digits",
+        "data": "synthetic (SplitMix64-seeded uniform Goldilocks f and A per shape)",
+        "config": {"workload": f"batched commit of {args.batch} polynomials, mu uniform in [16,24], "
+                               "m = 2^floor(mu/2), n = 2^ceil(mu/2)",
+                   "mus": [int(x) for x in mus]},
+        "batched_melem_per_s": total / dt_b / 1e6,
+        "per_poly_ms_per_batch": dt_s * 1e3,
+        "per_poly_polys_per_s": args.batch / dt_s,
+        "speedup_vs_per_poly": dt_s / dt_b,
+        "cpu_baseline": None,
+    }
+    del fs, keys, ks
+    ctx.close()
+    print(json.dumps(out), flush=True)
 
 
 def bench_prove(args):

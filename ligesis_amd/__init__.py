@@ -245,20 +245,31 @@ class Ligesis:
         return root.tobytes(), CommitData(ctx, pk, h, root.tobytes(), keep)
 
     @staticmethod
-    def commit_batch(keys, fs, ctx: Context | None = None):
+    def commit_batch(keys, fs, ctx: Context | None = None, device: bool = False):
+        """Commit many polynomials (any mix of shapes) in one batched pass; returns
+        [(root, CommitData)], identical to per-polynomial ``commit``. With ``device=True`` the
+        ``fs`` are GPU tensors that the commitments borrow."""
         ctx = ctx or (keys[0].ctx if keys else default_context())
         count = len(keys)
-        arrs = [_u64(f) for f in fs]
+        if device:
+            ptrs = [f.data_ptr() for f in fs]
+            lens = np.array([f.numel() for f in fs], np.uint64)
+            keep = list(fs)
+        else:
+            arrs = [_u64(f) for f in fs]
+            ptrs = [a.ctypes.data for a in arrs]
+            lens = np.array([a.size for a in arrs], np.uint64)
+            keep = [None] * count
         kp = (ctypes.c_void_p * count)(*[k.h.value for k in keys])
-        fp = (ctypes.c_void_p * count)(*[a.ctypes.data for a in arrs])
-        lens = np.array([a.size for a in arrs], np.uint64)
+        fp = (ctypes.c_void_p * count)(*ptrs)
         roots = np.zeros((count, 32), np.uint8)
         outs = (ctypes.c_void_p * count)()
-        ctx.check(lib().lgs_commit_batch(ctx.h, count, kp, fp, _ptr(lens), 0, _ptr(roots), outs))
+        ctx.check(lib().lgs_commit_batch(ctx.h, count, kp, fp, _ptr(lens),
+                                         LGS_INPUT_DEVICE if device else 0, _ptr(roots), outs))
         res = []
         for i in range(count):
             r = roots[i].tobytes()
-            res.append((r, CommitData(ctx, keys[i], ctypes.c_void_p(outs[i]), r)))
+            res.append((r, CommitData(ctx, keys[i], ctypes.c_void_p(outs[i]), r, keep[i])))
         return res
 
     @staticmethod

@@ -231,23 +231,32 @@ int lgs_commit_batch(lgs_ctx* ctx, uint32_t count, const lgs_key* const* keys,
   int rc = guarded(&ctx->c, [&] {
     (void)hipSetDevice(ctx->c.device);
     std::vector<lgs::Commitment*> cs;
-    bool prof = ctx->c.profile;
-    ctx->c.profile = false;
+    std::vector<std::pair<const lgs::Key*, lgs::Commitment*>> grouped, single;
+    for (uint32_t i = 0; i < count; i++) {
+      if (!keys[i]) throw lgs::Error(LGS_ERR_ARG, "commit_batch: null key");
+      auto* c = new lgs_commitment();
+      made.push_back(c);
+      prepare_commitment(ctx->c, keys[i]->k, fs[i], lens[i], flags, c->c);
+      cs.push_back(&c->c);
+      // the grouped kernels are the MFMA hash and whole-row transforms; other shapes (and a
+      // forced VALU hash) take the per-polynomial path
+      bool group = ctx->c.hash_variant != 2 && lgs::group_eligible(keys[i]->k);
+      (group ? grouped : single).push_back({&keys[i]->k, &c->c});
+    }
+    lgs::GroupHold hold;
+    const bool prof = ctx->c.profile;
+    ctx->c.profile = prof && single.empty();
     try {
-      for (uint32_t i = 0; i < count; i++) {
-        if (!keys[i]) throw lgs::Error(LGS_ERR_ARG, "commit_batch: null key");
-        auto* c = new lgs_commitment();
-        made.push_back(c);
-        prepare_commitment(ctx->c, keys[i]->k, fs[i], lens[i], flags, c->c);
-        lgs::enqueue_commit(ctx->c, keys[i]->k, c->c);
-        cs.push_back(&c->c);
-      }
+      lgs::enqueue_commit_group(ctx->c, grouped, hold);
+      ctx->c.profile = false;
+      for (auto& s : single) lgs::enqueue_commit(ctx->c, *s.first, *s.second);
       read_root(ctx->c, cs.data(), count);
+      ctx->c.profile = prof;
+      if (prof && single.empty() && !grouped.empty()) finish_profile(ctx->c);
     } catch (...) {
       ctx->c.profile = prof;
       throw;
     }
-    ctx->c.profile = prof;
   });
   if (rc != LGS_OK) {
     (void)hipStreamSynchronize(ctx->c.stream);

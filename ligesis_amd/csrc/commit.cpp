@@ -133,4 +133,99 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c) {
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[4], ctx.stream), "event");
 }
 
+// Batched commit: every stage of every eligible commitment runs in one launch per stage (row
+// transforms, MFMA hash tiles, finalize, and one launch per Merkle reduction round), driven by
+// job tables in device memory. Results are identical to enqueue_commit per polynomial.
+bool group_eligible(const Key& key) {
+  const uint64_t m = uint64_t(1) << key.log_m, two_n = uint64_t(2) << key.log_n;
+  return two_n % 128 == 0 && m >= 2 && key.log_n + 1 <= LGS_LDS_LOG;
+}
+
+void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, Commitment*>>& items,
+                          GroupHold& hold) {
+  if (items.empty()) return;
+  std::vector<NttJob> nj;
+  std::vector<HashJob> hj;
+  std::vector<FinalizeJob> fj;
+  uint32_t nblocks = 0, ntiles_total = 0, fblocks = 0;
+  int max_logN = 1;
+  for (auto& it : items) {
+    const Key& key = *it.first;
+    Commitment& c = *it.second;
+    const uint64_t m = uint64_t(1) << key.log_m, n = uint64_t(1) << key.log_n, two_n = 2 * n;
+    const int logN = key.log_n + 1;
+    c.log_m = key.log_m;
+    c.log_n = key.log_n;
+    c.enc = DevBuf<uint64_t>(&ctx.pool, m * two_n);
+    c.h = DevBuf<uint64_t>(&ctx.pool, 32 * two_n);
+    c.tree.leaves = two_n;
+    c.tree.nodes = DevBuf<uint32_t>(&ctx.pool, (size_t)c.tree.node_count() * 8);
+    nj.push_back(NttJob{c.f, n, 1, (uint32_t)n, logN, ctx.twiddle(logN), c.enc.ptr, nblocks});
+    nblocks += (uint32_t)m;
+    max_logN = logN > max_logN ? logN : max_logN;
+    const uint32_t ntiles = (uint32_t)(two_n / 128);
+    uint32_t ks = ctx.hash_ksplit ? (uint32_t)ctx.hash_ksplit : (uint32_t)(m >= 1024 ? m / 512 : 1);
+    if (ks > m / 2) ks = (uint32_t)(m / 2);
+    if (ks == 0 || m % (2 * ks)) throw Error(1, "hash_ksplit must divide m / 2");
+    hold.parts.emplace_back(&ctx.pool, (size_t)ks * 256 * two_n);
+    hj.push_back(HashJob{c.enc.ptr, two_n, key.dig.ptr, (uint32_t)(m / ks), ntiles, (uint32_t)two_n,
+                         hold.parts.back().ptr, ntiles_total});
+    ntiles_total += ntiles * ks;
+    fj.push_back(FinalizeJob{hold.parts.back().ptr, ks, (uint32_t)two_n, c.h.ptr, fblocks});
+    fblocks += (uint32_t)((32 * two_n + 255) / 256);
+  }
+  const int njobs = (int)items.size();
+  auto upload = [&](const void* src, size_t bytes) {
+    hold.tables.emplace_back(&ctx.pool, (bytes + 7) / 8);
+    hip_check(hipMemcpyAsync(hold.tables.back().ptr, src, bytes, hipMemcpyHostToDevice, ctx.stream), "job table");
+    return (const void*)hold.tables.back().ptr;
+  };
+  const NttJob* d_nj = (const NttJob*)upload(nj.data(), nj.size() * sizeof(NttJob));
+  const HashJob* d_hj = (const HashJob*)upload(hj.data(), hj.size() * sizeof(HashJob));
+  const FinalizeJob* d_fj = (const FinalizeJob*)upload(fj.data(), fj.size() * sizeof(FinalizeJob));
+  if (ctx.profile) hip_check(hipEventRecord(ctx.ev[0], ctx.stream), "event");
+  hip_check(launch_ntt_group(d_nj, njobs, nblocks, max_logN, ctx.stream), "batched row encode");
+  if (ctx.profile) hip_check(hipEventRecord(ctx.ev[1], ctx.stream), "event");
+  hip_check(launch_hash_group(d_hj, njobs, ntiles_total, nullptr, 0, ctx.stream), "batched hash");
+  if (ctx.profile) hip_check(hipEventRecord(ctx.ev[2], ctx.stream), "event");
+  hip_check(launch_hash_group(nullptr, njobs, 0, d_fj, fblocks, ctx.stream), "batched finalize");
+  if (ctx.profile) hip_check(hipEventRecord(ctx.ev[3], ctx.stream), "event");
+  // Merkle: rounds of up to 8 levels; every tree takes part until its root is written
+  std::vector<uint64_t> off(items.size(), 0), count(items.size());
+  std::vector<int> left(items.size());
+  for (size_t i = 0; i < items.size(); i++) {
+    count[i] = items[i].second->tree.leaves;
+    int l = 0;
+    while ((uint64_t(1) << l) < count[i]) l++;
+    left[i] = l;
+  }
+  bool first = true;
+  while (true) {
+    std::vector<MerkleJob> mj;
+    uint32_t mblocks = 0;
+    for (size_t i = 0; i < items.size(); i++) {
+      if (!first && left[i] == 0) continue;
+      const int lv = left[i] < 8 ? left[i] : 8;
+      Commitment& c = *items[i].second;
+      mj.push_back(MerkleJob{c.h.ptr, 32, c.tree.leaves, 1, first ? 1 : 0, c.tree.nodes.ptr, off[i], count[i],
+                             lv, mblocks});
+      mblocks += (uint32_t)((count[i] + 255) / 256);
+      for (int q = 0; q < lv; q++) {
+        off[i] += count[i];
+        count[i] >>= 1;
+      }
+      left[i] -= lv;
+    }
+    if (mj.empty()) break;
+    hold.host_merkle.push_back(mj);
+    const MerkleJob* d_mj = (const MerkleJob*)upload(hold.host_merkle.back().data(), mj.size() * sizeof(MerkleJob));
+    hip_check(launch_merkle_group(d_mj, (int)mj.size(), mblocks, ctx.stream), "batched merkle");
+    first = false;
+  }
+  if (ctx.profile) hip_check(hipEventRecord(ctx.ev[4], ctx.stream), "event");
+  hold.host_ntt = std::move(nj);
+  hold.host_hash = std::move(hj);
+  hold.host_fin = std::move(fj);
+}
+
 }  // namespace lgs

@@ -142,22 +142,55 @@ __global__ __launch_bounds__(256) void hash_mfma_kernel(const uint64_t* __restri
   }
 }
 
+__device__ __forceinline__ uint32_t xcd_logical_block() {
+  // consecutive block ids are dealt round-robin to the 8 XCDs; give each XCD a contiguous run
+  // of logical tiles so the tiles sharing a K-split (and its A digits) share an L2
+  const uint32_t nb = gridDim.x, bid = blockIdx.x;
+  return (nb % 8 == 0) ? (bid % 8) * (nb / 8) + bid / 8 : bid;
+}
+
 // Pipelined variant: A digits double-buffered in LDS (one barrier per stage of SPS slabs), the
 // next stage's digits and F' words loaded into registers while the current stage computes,
 // and at most 256 registers per lane so two workgroups share a CU.
+template <int SPS>
+__device__ __forceinline__ void hash_pipe_body(const uint64_t* __restrict__ enc, uint64_t ld,
+                                               const v4i* __restrict__ adig, uint32_t slabs_per_split,
+                                               uint32_t split, uint32_t ntile, uint32_t ncols,
+                                               int32_t* __restrict__ part, v4i (*lds_a)[SPS * 1024]);
+
 template <int SPS>
 __global__ __launch_bounds__(256, 2) void hash_mfma_pipe_kernel(const uint64_t* __restrict__ enc,
                                                                 uint64_t ld, const v4i* __restrict__ adig,
                                                                 uint32_t slabs_per_split, uint32_t ntiles,
                                                                 uint32_t ncols, int32_t* __restrict__ part) {
   __shared__ v4i lds_a[2][SPS * 1024];
-  const uint32_t nb = gridDim.x;
-  const uint32_t per_xcd = nb / 8;
-  const uint32_t bid = blockIdx.x;
-  const uint32_t logical = (nb % 8 == 0) ? (bid % 8) * per_xcd + bid / 8 : bid;
-  const uint32_t split = logical / ntiles;
-  const uint32_t ntile = logical % ntiles;
+  const uint32_t logical = xcd_logical_block();
+  hash_pipe_body<SPS>(enc, ld, adig, slabs_per_split, logical / ntiles, logical % ntiles, ncols, part,
+                      lds_a);
+}
 
+// Tiles of many commitments in one launch (batched commit): logical tile -> job by binary
+// search over the jobs' first tiles.
+__global__ __launch_bounds__(256, 2) void hash_mfma_group_kernel(const HashJob* __restrict__ jobs, int njobs) {
+  __shared__ v4i lds_a[2][2 * 1024];
+  const uint32_t logical = xcd_logical_block();
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].first_block <= logical) lo = mid;
+    else hi = mid - 1;
+  }
+  const HashJob& jb = jobs[lo];
+  const uint32_t local = logical - jb.first_block;
+  hash_pipe_body<2>(jb.enc, jb.ld, (const v4i*)jb.adig, jb.slabs_per_split, local / jb.ntiles,
+                    local % jb.ntiles, jb.ncols, jb.part, lds_a);
+}
+
+template <int SPS>
+__device__ __forceinline__ void hash_pipe_body(const uint64_t* __restrict__ enc, uint64_t ld,
+                                               const v4i* __restrict__ adig, uint32_t slabs_per_split,
+                                               uint32_t split, uint32_t ntile, uint32_t ncols,
+                                               int32_t* __restrict__ part, v4i (*lds_a)[SPS * 1024]) {
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t wm = wave >> 1, wn = wave & 1;
   const uint32_t g = lane >> 4, c = lane & 15;
@@ -335,6 +368,38 @@ hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
   uint64_t n = (uint64_t)32 * ncols;
   hipLaunchKernelGGL(hash_mfma_finalize_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
                      part, ksplit, ncols, h);
+  return hipGetLastError();
+}
+
+__global__ void hash_finalize_group_kernel(const FinalizeJob* __restrict__ jobs, int njobs) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].first_block <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const FinalizeJob& jb = jobs[lo];
+  const uint64_t idx = (uint64_t)(blockIdx.x - jb.first_block) * blockDim.x + threadIdx.x;
+  if (idx >= (uint64_t)32 * jb.ncols) return;
+  const uint32_t r = (uint32_t)(idx / jb.ncols), j = (uint32_t)(idx % jb.ncols);
+  uint64_t acc = 0;
+#pragma unroll
+  for (int l = 0; l < 8; l++) {
+    int64_t s = 0;
+    for (uint32_t sp = 0; sp < jb.ksplit; sp++) s += jb.part[((uint64_t)sp * 256 + 32 * l + r) * jb.ncols + j];
+    uint64_t sm = s >= 0 ? (uint64_t)s : gl_neg((uint64_t)(-s));
+    acc = gl_add(acc, gl_mul(sm, (uint64_t)1 << (8 * l)));
+  }
+  jb.h[idx] = acc;
+}
+
+hipError_t launch_hash_group(const HashJob* jobs, int njobs, uint32_t total_tiles, const FinalizeJob* fjobs,
+                             uint32_t total_fblocks, hipStream_t st) {
+  if (total_tiles) hipLaunchKernelGGL(hash_mfma_group_kernel, dim3(total_tiles), dim3(256), 0, st, jobs, njobs);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (total_fblocks)
+    hipLaunchKernelGGL(hash_finalize_group_kernel, dim3(total_fblocks), dim3(256), 0, st, fjobs, njobs);
   return hipGetLastError();
 }
 

@@ -9,6 +9,42 @@
 
 namespace lgs {
 
+// Job tables of the batched (grouped) launches: block / tile ids [first_block, next job's
+// first_block) belong to a job; jobs are sorted by first_block.
+struct NttJob {
+  const uint64_t* in;
+  uint64_t rs, es;
+  uint32_t n_in;
+  int logN;
+  const uint64_t* tw;
+  uint64_t* out;
+  uint32_t first_block;
+};
+struct HashJob {
+  const uint64_t* enc;
+  uint64_t ld;
+  const void* adig;
+  uint32_t slabs_per_split, ntiles, ncols;
+  int32_t* part;
+  uint32_t first_block;
+};
+struct FinalizeJob {
+  const int32_t* part;
+  uint32_t ksplit, ncols;
+  uint64_t* h;
+  uint32_t first_block;
+};
+struct MerkleJob {
+  const uint64_t* M;
+  uint32_t rows;
+  uint64_t rs, cs;
+  int from_leaves;
+  uint32_t* nodes;
+  uint64_t child_off, child_count;
+  int levels;
+  uint32_t first_block;
+};
+
 // ntt.hip
 hipError_t launch_twiddles(uint64_t* tw, int logN, hipStream_t st);
 // impl 0: radix-8 passes with the whole row in LDS when 2^logN <= 2^LGS_LDS_LOG; otherwise (or
@@ -24,6 +60,12 @@ hipError_t launch_hash_valu(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
 hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
                             const int8_t* dig, uint32_t ksplit, int32_t* part, uint64_t* h,
                             hipStream_t st, hipEvent_t mid, int impl);
+
+// Batched launches over job tables that live in device memory.
+hipError_t launch_ntt_group(const NttJob* jobs, int njobs, uint32_t total_blocks, int max_logN, hipStream_t st);
+hipError_t launch_hash_group(const HashJob* jobs, int njobs, uint32_t total_tiles, const FinalizeJob* fjobs,
+                             uint32_t total_fblocks, hipStream_t st);
+hipError_t launch_merkle_group(const MerkleJob* jobs, int njobs, uint32_t total_blocks, hipStream_t st);
 
 // merkle.hip: tree over nleaves (a power of two) leaves, leaf p = M[r*rs + p*cs], r < rows.
 hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs,

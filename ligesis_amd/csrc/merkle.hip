@@ -31,12 +31,12 @@ __device__ __forceinline__ void leaf_digest(const uint64_t* __restrict__ M, uint
   }
 }
 
-__global__ __launch_bounds__(256) void merkle_reduce_kernel(
-    const uint64_t* __restrict__ M, uint32_t rows, uint64_t rs, uint64_t cs, int from_leaves,
-    uint32_t* __restrict__ nodes, uint64_t child_off, uint64_t child_count, int levels) {
-  __shared__ uint32_t sh[256 * 8];
+__device__ __forceinline__ void merkle_reduce_body(const uint64_t* __restrict__ M, uint32_t rows, uint64_t rs,
+                                                   uint64_t cs, int from_leaves, uint32_t* __restrict__ nodes,
+                                                   uint64_t child_off, uint64_t child_count, int levels,
+                                                   uint32_t blk, uint32_t* sh) {
   const uint32_t t = threadIdx.x;
-  const uint64_t base = (uint64_t)blockIdx.x * 256;
+  const uint64_t base = (uint64_t)blk * 256;
   uint32_t local_n = (uint32_t)((child_count - base) < 256 ? (child_count - base) : 256);
   if (t < local_n) {
     uint32_t cv[8];
@@ -71,6 +71,32 @@ __global__ __launch_bounds__(256) void merkle_reduce_kernel(
     count >>= 1;
     local_n = half;
   }
+}
+
+__global__ __launch_bounds__(256) void merkle_reduce_kernel(
+    const uint64_t* __restrict__ M, uint32_t rows, uint64_t rs, uint64_t cs, int from_leaves,
+    uint32_t* __restrict__ nodes, uint64_t child_off, uint64_t child_count, int levels) {
+  __shared__ uint32_t sh[256 * 8];
+  merkle_reduce_body(M, rows, rs, cs, from_leaves, nodes, child_off, child_count, levels, blockIdx.x, sh);
+}
+
+// One reduction round of many trees in one launch (batched commit).
+__global__ __launch_bounds__(256) void merkle_group_kernel(const MerkleJob* __restrict__ jobs, int njobs) {
+  __shared__ uint32_t sh[256 * 8];
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].first_block <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const MerkleJob& jb = jobs[lo];
+  merkle_reduce_body(jb.M, jb.rows, jb.rs, jb.cs, jb.from_leaves, jb.nodes, jb.child_off, jb.child_count,
+                     jb.levels, blockIdx.x - jb.first_block, sh);
+}
+
+hipError_t launch_merkle_group(const MerkleJob* jobs, int njobs, uint32_t total_blocks, hipStream_t st) {
+  if (total_blocks) hipLaunchKernelGGL(merkle_group_kernel, dim3(total_blocks), dim3(256), 0, st, jobs, njobs);
+  return hipGetLastError();
 }
 
 static int ilog2_u64(uint64_t x) {

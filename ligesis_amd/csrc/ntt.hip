@@ -149,15 +149,11 @@ __device__ __forceinline__ void pretwiddle(uint64_t (&x)[1 << LG], uint32_t j, i
   }
 }
 
-__global__ __launch_bounds__(256) void ntt_r8_kernel(const uint64_t* __restrict__ in, uint64_t rs,
-                                                     uint64_t es, uint32_t n_in, int logN,
-                                                     const uint64_t* __restrict__ tw,
-                                                     uint64_t* __restrict__ out) {
-  extern __shared__ uint64_t sm[];
+// One row: src[k * es] for k < n_in (zero above), transform size 2^logN, result to dst.
+__device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uint64_t es, uint32_t n_in,
+                                           int logN, const uint64_t* __restrict__ tw,
+                                           uint64_t* __restrict__ dst, uint64_t* sm) {
   const uint32_t N = 1u << logN;
-  const uint64_t row = blockIdx.x;
-  const uint64_t* src = in + row * rs;
-  uint64_t* dst = out + (row << logN);
   int s = 0;
   if (logN >= 3) {
     // first pass fused with the load: thread k takes x[k + r N/8] (r = 0..7), which sit at
@@ -232,6 +228,37 @@ __global__ __launch_bounds__(256) void ntt_r8_kernel(const uint64_t* __restrict_
       dst[j + M] = gl_sub(a, b);
     }
   }
+}
+
+__global__ __launch_bounds__(256) void ntt_r8_kernel(const uint64_t* __restrict__ in, uint64_t rs,
+                                                     uint64_t es, uint32_t n_in, int logN,
+                                                     const uint64_t* __restrict__ tw,
+                                                     uint64_t* __restrict__ out) {
+  extern __shared__ uint64_t sm[];
+  const uint64_t row = blockIdx.x;
+  ntt_r8_row(in + row * rs, es, n_in, logN, tw, out + (row << logN), sm);
+}
+
+// Many transforms of different sizes in one launch: block b is row (b - first_block) of the job
+// whose [first_block, next first_block) range holds b.
+__global__ __launch_bounds__(256) void ntt_r8_group_kernel(const NttJob* __restrict__ jobs, int njobs) {
+  extern __shared__ uint64_t sm[];
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].first_block <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const NttJob& jb = jobs[lo];
+  const uint64_t row = blockIdx.x - jb.first_block;
+  ntt_r8_row(jb.in + row * jb.rs, jb.es, jb.n_in, jb.logN, jb.tw, jb.out + (row << jb.logN), sm);
+}
+
+hipError_t launch_ntt_group(const NttJob* jobs, int njobs, uint32_t total_blocks, int max_logN, hipStream_t st) {
+  if (total_blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(ntt_r8_group_kernel, dim3(total_blocks), dim3(256),
+                     (size_t(1) << max_logN) * sizeof(uint64_t), st, jobs, njobs);
+  return hipGetLastError();
 }
 
 // One global radix-2 stage s (> lt) over every row, in place.

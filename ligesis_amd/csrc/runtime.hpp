@@ -197,8 +197,21 @@ struct Commitment {
   uint8_t root[32] = {0};
 };
 
+// Buffers and host job tables of a batched commit, kept until the caller synchronises.
+struct GroupHold {
+  std::vector<DevBuf<int32_t>> parts;
+  std::vector<DevBuf<uint64_t>> tables;
+  std::vector<NttJob> host_ntt;
+  std::vector<HashJob> host_hash;
+  std::vector<FinalizeJob> host_fin;
+  std::vector<std::vector<MerkleJob>> host_merkle;
+};
+
 // commit.cpp
 void enqueue_commit(Context& ctx, const Key& key, Commitment& c);
+bool group_eligible(const Key& key);
+void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, Commitment*>>& items,
+                          GroupHold& hold);
 void enqueue_basefold_commit(Context& ctx, DevBasefold& st, const uint64_t* poly, bool ext,
                              uint32_t num_vars, uint32_t li, uint32_t code_rate);
 void setup_key(Context& ctx, Key& key, const uint64_t* host_a, bool commit_a);

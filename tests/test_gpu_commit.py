@@ -140,3 +140,23 @@ def test_handles_outlive_context_shutdown():
     assert cd.digests().shape == (32, 128)  # still usable after the caller's shutdown
     pk.release()
     cd.release()
+
+
+def test_batched_commit_mixed_shapes_matches_single(gpu_ctx):
+    """Grouped launches over many shapes (including ones that take the per-polynomial path)
+    give the same roots and digests as committing each polynomial alone."""
+    shapes = [(8, 8), (7, 8), (8, 9), (3, 10), (1, 5), (6, 12), (9, 10), (2, 11), (0, 7), (5, 6),
+              (10, 10), (4, 13)]
+    keys, fs, expect = [], [], []
+    for i, (lm, ln) in enumerate(shapes):
+        inp = synth.Inputs(lm, ln, seed=300 + i, n_fields=4, n_ints=4)
+        pk, _ = Ligesis.setup(lm, ln, inp.a, ctx=gpu_ctx, commit_only=True)
+        root, cd = Ligesis.commit(pk, inp.f)
+        keys.append(pk)
+        fs.append(inp.f)
+        expect.append((root, cd.digests()))
+    print("batched commit of", len(shapes), "shapes", flush=True)
+    res = Ligesis.commit_batch(keys, fs, ctx=gpu_ctx)
+    for (r, cd), (er, eh) in zip(res, expect):
+        assert r == er
+        np.testing.assert_array_equal(cd.digests(), eh)
