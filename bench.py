@@ -64,10 +64,11 @@ def main():
     ap.add_argument("--hash-impl", type=int, default=0, help="MFMA kernel: 0 pipelined, 1 pipelined/1 slab, 2 basic")
     ap.add_argument("--ksplit", type=int, default=0, help="hash K-split (0 auto)")
     ap.add_argument("--ntt-impl", type=int, default=0, help="0 radix-8 passes, 1 radix-2 stages")
-    ap.add_argument("--workload", default="commit", choices=["commit", "prove", "batch"],
+    ap.add_argument("--workload", default="commit", choices=["commit", "prove", "batch", "serve"],
                     help="commit: the headline metric; prove: mu=24 eval-prove at 2^8 x 2^16; "
                          "batch: 64 polynomials with mu uniform in [16, 24] in one batched commit")
     ap.add_argument("--batch", type=int, default=64, help="batch workload: number of polynomials")
+    ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
     ap.add_argument("--log-m", type=int, default=8, help="prove workload: log2 rows")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -79,8 +80,127 @@ def main():
         bench_prove(args)
     elif args.workload == "batch":
         bench_batch(args)
+    elif args.workload == "serve":
+        bench_serve(args)
     else:
         bench_commit(args)
+
+
+def bench_serve(args):
+    """Served mixed stream: `--requests` commit-then-open requests, mu uniform in [14, 26] with
+    log_m = floor((mu - 7) / 2) (the reference's shape window), f in pinned host memory, eval
+    points uniform in F_{p^2}^mu. Two loads: saturation (everything queued at t = 0: polys/s) and
+    Poisson arrivals at 70 % of the per-request path's capacity (p50 / p99 latency). Each is run
+    through the serving loop and through the plain per-request path (commit, then prove, one
+    request at a time in arrival order)."""
+    import threading
+
+    import torch
+
+    import ligesis_amd as L
+    from ligesis_amd import synth
+
+    rng = np.random.default_rng(7)
+    nreq = args.requests
+    mus = [int(x) for x in rng.integers(14, 27, size=nreq)]
+    shapes = sorted(set(mus))
+    print(f"bench: served stream of {nreq} requests, mu in [14,26]", file=sys.stderr, flush=True)
+    ctx = L.Context(0)
+    keys, fs = {}, {}
+    for mu in shapes:
+        lm = (mu - 7) // 2
+        ln = mu - lm
+        keys[mu] = L.Ligesis.setup(lm, ln, synth.field_elems(3000 + mu, 32 * (64 << lm)), ctx=ctx)[0]
+        fs[mu] = torch.from_numpy(synth.field_elems(4000 + mu, 1 << mu).view(np.int64)).pin_memory()
+    reqs = []
+    for i, mu in enumerate(mus):
+        reqs.append((mu, synth.ext_elems(5000 + i, mu), synth.ext_elems(6000 + i, 256), synth.oracle_ints(7000 + i, 640)))
+
+    def host_f(mu):
+        return fs[mu].numpy().view(np.uint64)
+
+    def per_request(arrivals):
+        lat = []
+        t0 = time.perf_counter()
+        for (mu, z, fl, it), at in zip(reqs, arrivals):
+            now = time.perf_counter() - t0
+            if now < at:
+                time.sleep(at - now)
+            r, cd = L.Ligesis.commit(keys[mu], host_f(mu))
+            L.Ligesis.prove(keys[mu], cd, z, L.RandomOracle(fl, it))
+            cd.release()
+            lat.append((time.perf_counter() - t0 - at) * 1e3)
+        return lat, time.perf_counter() - t0
+
+    def served(arrivals, workers=3):
+        srv = L.Server(0, hbm_budget=64 << 30, prove_workers=workers, max_batch=16)
+        tickets = [None] * nreq
+        t0 = time.perf_counter()
+
+        def submitter():
+            for i, ((mu, z, fl, it), at) in enumerate(zip(reqs, arrivals)):
+                now = time.perf_counter() - t0
+                if now < at:
+                    time.sleep(at - now)
+                tickets[i] = srv.submit(keys[mu], host_f(mu), z, L.RandomOracle(fl, it))
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat = []
+        for i in range(nreq):
+            while tickets[i] is None:
+                time.sleep(0.0002)
+            lat.append(srv.poll(tickets[i])[2]["total_ms"])
+        th.join()
+        span = time.perf_counter() - t0
+        st = srv.stats()
+        srv.stop()
+        return lat, span, st
+
+    # warm both paths (kernels, twiddles, pools)
+    per_request([0.0] * nreq)
+    served([0.0] * nreq)
+    zero = [0.0] * nreq
+    lat_pr, span_pr = per_request(zero)
+    lat_sv, span_sv, st = served(zero)
+    cap = nreq / span_pr
+    rate = 0.7 * cap
+    arrivals = list(np.cumsum(rng.exponential(1.0 / rate, size=nreq)))
+    lat_pr_p, span_pr_p = per_request(arrivals)
+    lat_sv_p, span_sv_p, st_p = served(arrivals)
+
+    def pct(v, q):
+        return float(np.percentile(np.array(v), q))
+
+    out = {
+        "metric": "LigeSIS served commit+open polys/s and p50/p99 latency (mu uniform in [14,26])",
+        "value": nreq / span_sv,
+        "unit": "polys/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": span_sv * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 / F_p^2 (Goldilocks)",
+        "data": "synthetic (seeded f per shape in pinned host memory, seeded z and oracle streams)",
+        "config": {"workload": f"{nreq} commit-then-open requests, mu uniform in [14,26], "
+                               "log_m = floor((mu-7)/2), HBM budget 64 GiB, 3 prove workers",
+                   "mus": mus},
+        "saturation": {"served_polys_per_s": nreq / span_sv, "per_request_polys_per_s": nreq / span_pr,
+                       "served_p50_ms": pct(lat_sv, 50), "served_p99_ms": pct(lat_sv, 99),
+                       "per_request_p50_ms": pct(lat_pr, 50), "per_request_p99_ms": pct(lat_pr, 99),
+                       "batches": st["batches"]},
+        "poisson": {"rate_per_s": rate, "served_p50_ms": pct(lat_sv_p, 50), "served_p99_ms": pct(lat_sv_p, 99),
+                    "per_request_p50_ms": pct(lat_pr_p, 50), "per_request_p99_ms": pct(lat_pr_p, 99),
+                    "served_polys_per_s": nreq / span_sv_p, "per_request_polys_per_s": nreq / span_pr_p,
+                    "batches": st_p["batches"]},
+        "cpu_baseline": None,
+    }
+    del keys, fs
+    ctx.close()
+    print(json.dumps(out), flush=True)
 
 
 def bench_batch(args):

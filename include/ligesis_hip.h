@@ -37,7 +37,8 @@ enum {
   LGS_ERR_NOMEM = 4,  /* device allocation failed or the HBM budget is exhausted */
   LGS_ERR_ORACLE = 5, /* the randomness-oracle stream ran out (reference: index panic) */
   LGS_ERR_BUFFER = 6, /* output buffer too small; the required size is reported */
-  LGS_ERR_STATE = 7   /* handle cannot do this (e.g. proving with a commit-only key) */
+  LGS_ERR_STATE = 7,  /* handle cannot do this (e.g. proving with a commit-only key) */
+  LGS_PENDING = 8     /* lgs_server_poll: the request is still running */
 };
 
 /* lgs_setup flags */
@@ -104,6 +105,33 @@ int lgs_commitment_read(const lgs_commitment* c, uint32_t what, void* out, uint6
 int lgs_prove(lgs_ctx* ctx, const lgs_key* key, const lgs_commitment* c, const uint64_t* z,
               uint64_t z_len, const uint64_t* fields, uint64_t n_fields, const uint64_t* ints,
               uint64_t n_ints, uint8_t* proof, uint64_t* proof_len, uint64_t used[2]);
+
+/* ---- Serving loop (no counterpart in the single-threaded reference; it drives the calls
+ * above). Requests are commit-then-open jobs (lgs_commit followed by lgs_prove on the resident
+ * commitment) of any shape. One commit worker admits queued requests in arrival order while
+ * their estimated HBM footprint fits `hbm_budget` bytes and commits each admitted group in one
+ * batched pass; `prove_workers` threads prove on their own streams, so proofs overlap each
+ * other and the next commit. Results are identical to the unserved calls. */
+typedef struct lgs_server lgs_server;
+int lgs_server_start(int device, uint64_t hbm_budget, uint32_t prove_workers, uint32_t max_batch,
+                     lgs_server** out);
+/* Queue a request. f (host memory, or device memory with LGS_INPUT_DEVICE) must stay valid
+ * until the request completes; z / fields / ints are copied. The key must not be a commit-only
+ * key. LGS_ERR_NOMEM when the request alone exceeds the budget. */
+int lgs_server_submit(lgs_server* srv, const lgs_key* key, const uint64_t* f, uint64_t len, uint32_t flags,
+                      const uint64_t* z, uint64_t z_len, const uint64_t* fields, uint64_t n_fields,
+                      const uint64_t* ints, uint64_t n_ints, uint64_t* ticket);
+/* Result of a request: LGS_PENDING while running (wait = 0), LGS_ERR_BUFFER with *proof_len set
+ * when `proof` is too small (the result is kept), LGS_OK with root, proof and
+ * times_ms = {submit->committed, committed->proved, submit->proved} (then the ticket is
+ * retired), or the request's own error code. */
+int lgs_server_poll(lgs_server* srv, uint64_t ticket, int wait, uint8_t root[32], uint8_t* proof,
+                    uint64_t* proof_len, double times_ms[3]);
+/* {batches committed, requests committed, HBM bytes reserved now, requests queued}. */
+int lgs_server_stats(lgs_server* srv, uint64_t out[4]);
+/* Cancels queued requests, finishes admitted ones, joins the workers, frees everything. */
+void lgs_server_stop(lgs_server* srv);
+const char* lgs_server_last_error(void);
 
 #ifdef __cplusplus
 }

@@ -293,3 +293,79 @@ class Ligesis:
         ctx.check(rc)
         oracle.fields_idx, oracle.ints_idx = int(used[0]), int(used[1])
         return out[: cap.value].tobytes()
+
+
+LGS_PENDING = 8
+
+
+class Server:
+    """Serving loop (``lgs_server_*``): commit-then-open requests of mixed sizes, continuously
+    batched under an HBM budget; each result is identical to ``commit`` + ``prove`` alone."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, prove_workers: int = 3,
+                 max_batch: int = 16):
+        L = lib()
+        vp, u32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
+        L.lgs_server_start.argtypes = [ctypes.c_int, u64, u32, u32, ctypes.POINTER(ctypes.c_void_p)]
+        L.lgs_server_submit.argtypes = [vp, vp, vp, u64, u32, vp, u64, vp, u64, vp, u64,
+                                        ctypes.POINTER(ctypes.c_uint64)]
+        L.lgs_server_poll.argtypes = [vp, u64, ctypes.c_int, vp, vp, ctypes.POINTER(ctypes.c_uint64), vp]
+        L.lgs_server_stats.argtypes = [vp, vp]
+        L.lgs_server_stop.argtypes = [vp]
+        L.lgs_server_last_error.restype = ctypes.c_char_p
+        h = ctypes.c_void_p()
+        rc = L.lgs_server_start(device, hbm_budget, prove_workers, max_batch, ctypes.byref(h))
+        if rc != LGS_OK:
+            raise LigesisError(rc, L.lgs_server_last_error().decode())
+        self.h = h
+        self._keep = {}
+
+    def submit(self, key: ProvingKey, f, z, oracle: RandomOracle, device: bool = False) -> int:
+        z = _u64(z).reshape(-1, 2)
+        if device:
+            fptr, flen, keep = ctypes.c_void_p(f.data_ptr()), f.numel(), f
+        else:
+            fa = _u64(f)
+            fptr, flen, keep = _ptr(fa), fa.size, fa
+        t = ctypes.c_uint64(0)
+        rc = lib().lgs_server_submit(self.h, key.h, fptr, flen, LGS_INPUT_DEVICE if device else 0, _ptr(z),
+                                     z.shape[0], _ptr(oracle.fields), oracle.fields.shape[0],
+                                     _ptr(oracle.ints), oracle.ints.size, ctypes.byref(t))
+        if rc != LGS_OK:
+            raise LigesisError(rc, lib().lgs_server_last_error().decode())
+        self._keep[t.value] = (keep, key)
+        return t.value
+
+    def poll(self, ticket: int, wait: bool = True):
+        """Returns None while running, else (root, proof bytes, {commit_ms, prove_ms, total_ms})."""
+        root = np.zeros(32, np.uint8)
+        buf = np.zeros(4 << 20, np.uint8)
+        ln = ctypes.c_uint64(buf.size)
+        times = np.zeros(3, np.float64)
+        rc = lib().lgs_server_poll(self.h, ticket, int(wait), _ptr(root), _ptr(buf), ctypes.byref(ln), _ptr(times))
+        if rc == LGS_ERR_BUFFER:
+            buf = np.zeros(ln.value, np.uint8)
+            rc = lib().lgs_server_poll(self.h, ticket, int(wait), _ptr(root), _ptr(buf), ctypes.byref(ln),
+                                       _ptr(times))
+        if rc == LGS_PENDING:
+            return None
+        self._keep.pop(ticket, None)
+        if rc != LGS_OK:
+            raise LigesisError(rc, lib().lgs_server_last_error().decode())
+        return root.tobytes(), buf[: ln.value].tobytes(), {
+            "commit_ms": times[0], "prove_ms": times[1], "total_ms": times[2]}
+
+    def stats(self) -> dict:
+        out = np.zeros(4, np.uint64)
+        lib().lgs_server_stats(self.h, _ptr(out))
+        return {"batches": int(out[0]), "requests": int(out[1]), "reserved_bytes": int(out[2]),
+                "queued": int(out[3])}
+
+    def stop(self):
+        if getattr(self, "h", None):
+            lib().lgs_server_stop(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        self.stop()

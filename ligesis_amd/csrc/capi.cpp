@@ -16,6 +16,9 @@ struct lgs_key {
 struct lgs_commitment {
   lgs::Commitment c;
 };
+struct lgs_server {
+  lgs::Server* s;
+};
 
 namespace {
 
@@ -329,6 +332,65 @@ int lgs_commitment_read(const lgs_commitment* c, uint32_t what, void* out, uint6
     lgs::hip_check(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "read");
     lgs::hip_check(hipStreamSynchronize(ctx->stream), "read sync");
   });
+}
+
+// ---- serving loop ----
+static thread_local std::string g_server_err;
+
+const char* lgs_server_last_error(void) { return g_server_err.c_str(); }
+
+int lgs_server_start(int device, uint64_t hbm_budget, uint32_t prove_workers, uint32_t max_batch,
+                     lgs_server** out) {
+  if (!out || hbm_budget == 0) return LGS_ERR_ARG;
+  *out = nullptr;
+  try {
+    auto* w = new lgs_server();
+    w->s = lgs::server_start(device, hbm_budget, prove_workers, max_batch);
+    *out = w;
+    return LGS_OK;
+  } catch (const lgs::Error& e) {
+    g_server_err = e.what();
+    return e.code;
+  }
+}
+
+int lgs_server_submit(lgs_server* srv, const lgs_key* key, const uint64_t* f, uint64_t len, uint32_t flags,
+                      const uint64_t* z, uint64_t z_len, const uint64_t* fields, uint64_t n_fields,
+                      const uint64_t* ints, uint64_t n_ints, uint64_t* ticket) {
+  if (!srv || !key || !f || !z || !ticket || (n_fields && !fields) || (n_ints && !ints)) return LGS_ERR_ARG;
+  try {
+    *ticket = lgs::server_submit(srv->s, &key->k, f, len, flags, z, z_len, fields, n_fields, ints, n_ints);
+    return LGS_OK;
+  } catch (const lgs::Error& e) {
+    g_server_err = e.what();
+    return e.code;
+  }
+}
+
+int lgs_server_poll(lgs_server* srv, uint64_t ticket, int wait, uint8_t root[32], uint8_t* proof,
+                    uint64_t* proof_len, double times_ms[3]) {
+  if (!srv || !root || !proof_len) return LGS_ERR_ARG;
+  try {
+    std::string err;
+    int rc = lgs::server_poll(srv->s, ticket, wait != 0, root, proof, proof_len, times_ms, err);
+    if (rc != LGS_OK && rc != LGS_PENDING && rc != LGS_ERR_BUFFER) g_server_err = err;
+    return rc;
+  } catch (const lgs::Error& e) {
+    g_server_err = e.what();
+    return e.code;
+  }
+}
+
+int lgs_server_stats(lgs_server* srv, uint64_t out[4]) {
+  if (!srv || !out) return LGS_ERR_ARG;
+  lgs::server_stats(srv->s, out);
+  return LGS_OK;
+}
+
+void lgs_server_stop(lgs_server* srv) {
+  if (!srv) return;
+  lgs::server_stop(srv->s);
+  delete srv;
 }
 
 }  // extern "C"

@@ -84,9 +84,11 @@ __device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x, uint32_t e) {
   return neg ? gl_neg(r) : r;
 }
 
-// w_N^k for k < N from the half table tw[k] = w^k (k < N/2), w^(k + N/2) = -w^k.
-__device__ __forceinline__ uint64_t tw_full(const uint64_t* __restrict__ tw, uint32_t k, uint32_t half) {
-  return k < half ? tw[k] : gl_neg(tw[k - half]);
+// w_N^k for k < N from the half table tw[k] = w^k (k < N/2), w^(k + N/2) = -w^k. The table may
+// belong to a larger transform 2^tsh N (entries taken with stride 2^tsh).
+__device__ __forceinline__ uint64_t tw_full(const uint64_t* __restrict__ tw, uint32_t k, uint32_t half,
+                                            int tsh) {
+  return k < half ? tw[(uint64_t)k << tsh] : gl_neg(tw[(uint64_t)(k - half) << tsh]);
 }
 
 // In-place radix-2 DIT network on 8 values held in bit-reversed order; output natural order.
@@ -129,7 +131,7 @@ __device__ __forceinline__ void dft8(uint64_t (&x)[8]) {
 // Multiply x[t] by c^rev(t) where c = w_{2^(s+lg)}^j and rev is the lg-bit reversal.
 template <int LG>
 __device__ __forceinline__ void pretwiddle(uint64_t (&x)[1 << LG], uint32_t j, int s, int logN,
-                                           const uint64_t* __restrict__ tw) {
+                                           const uint64_t* __restrict__ tw, int tsh) {
   constexpr int R = 1 << LG;
   const int lsz = s + LG;  // log2 of this pass's transform size
   if (lsz <= 6) {
@@ -144,14 +146,15 @@ __device__ __forceinline__ void pretwiddle(uint64_t (&x)[1 << LG], uint32_t j, i
 #pragma unroll
     for (int t = 1; t < R; t++) {
       const uint32_t r = __brev((uint32_t)t) >> (32 - LG);
-      x[t] = gl_mul(x[t], tw_full(tw, (j * r) << (logN - lsz), half));
+      x[t] = gl_mul(x[t], tw_full(tw, (j * r) << (logN - lsz), half, tsh));
     }
   }
 }
 
-// One row: src[k * es] for k < n_in (zero above), transform size 2^logN, result to dst.
+// One row: src[k * es] for k < n_in (zero above), transform size 2^logN, result to dst; tw is
+// the twiddle table of a transform of size 2^(logN + tsh).
 __device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uint64_t es, uint32_t n_in,
-                                           int logN, const uint64_t* __restrict__ tw,
+                                           int logN, const uint64_t* __restrict__ tw, int tsh,
                                            uint64_t* __restrict__ dst, uint64_t* sm) {
   const uint32_t N = 1u << logN;
   int s = 0;
@@ -191,7 +194,7 @@ __device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uin
       uint64_t x[8];
 #pragma unroll
       for (int t = 0; t < 8; t++) x[t] = sm[base + t * M];
-      if (j) pretwiddle<3>(x, j, s, logN, tw);
+      if (j) pretwiddle<3>(x, j, s, logN, tw, tsh);
       dft8(x);
 #pragma unroll
       for (int t = 0; t < 8; t++) {
@@ -210,7 +213,7 @@ __device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uin
       uint64_t x[4];
 #pragma unroll
       for (int t = 0; t < 4; t++) x[t] = sm[base + t * M];
-      if (j) pretwiddle<2>(x, j, s, logN, tw);
+      if (j) pretwiddle<2>(x, j, s, logN, tw, tsh);
       uint64_t a0 = gl_add(x[0], x[1]), a1 = gl_sub(x[0], x[1]);
       uint64_t a2 = gl_add(x[2], x[3]), a3 = gl_mul_pow2(gl_sub(x[2], x[3]), 48);
       dst[base] = gl_add(a0, a2);
@@ -223,7 +226,7 @@ __device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uin
     for (uint32_t j = threadIdx.x; j < M; j += blockDim.x) {
       uint64_t a = sm[j], b = sm[j + M];
       if (j) b = s + 1 <= 6 ? gl_mul_pow2(b, (uint32_t)(((uint64_t)((39u << (6 - (s + 1))) % 192) * j) % 192))
-                            : gl_mul(b, tw[(uint64_t)j << (logN - s - 1)]);
+                            : gl_mul(b, tw[(uint64_t)j << (logN - s - 1 + tsh)]);
       dst[j] = gl_add(a, b);
       dst[j + M] = gl_sub(a, b);
     }
@@ -236,7 +239,26 @@ __global__ __launch_bounds__(256) void ntt_r8_kernel(const uint64_t* __restrict_
                                                      uint64_t* __restrict__ out) {
   extern __shared__ uint64_t sm[];
   const uint64_t row = blockIdx.x;
-  ntt_r8_row(in + row * rs, es, n_in, logN, tw, out + (row << logN), sm);
+  ntt_r8_row(in + row * rs, es, n_in, logN, tw, 0, out + (row << logN), sm);
+}
+
+// Transforms larger than one LDS tile: block (row, blk) computes the size-2^lt transform of the
+// subsequence x[c + k 2^(logN-lt)], c = rev_{logN-lt}(blk), which is what the first lt stages of
+// an in-place radix-2 DIT leave in positions [blk 2^lt, (blk + 1) 2^lt) of the bit-reversed
+// array; the remaining logN - lt stages run as global passes.
+__global__ __launch_bounds__(256) void ntt_r8_blocks_kernel(const uint64_t* __restrict__ in, uint64_t rs,
+                                                            uint64_t es, uint32_t n_in, int logN, int lt,
+                                                            const uint64_t* __restrict__ tw,
+                                                            uint64_t* __restrict__ out) {
+  extern __shared__ uint64_t sm[];
+  const uint32_t nblk = 1u << (logN - lt);
+  const uint64_t row = blockIdx.x / nblk;
+  const uint32_t blk = blockIdx.x % nblk;
+  const uint32_t c = brev_bits(blk, logN - lt);
+  const uint64_t S = uint64_t(1) << (logN - lt);
+  const uint32_t n_sub = n_in > c ? (uint32_t)((n_in - c + S - 1) / S) : 0;
+  ntt_r8_row(in + row * rs + (uint64_t)c * es, es * S, n_sub, lt, tw, logN - lt,
+             out + (row << logN) + ((uint64_t)blk << lt), sm);
 }
 
 // Many transforms of different sizes in one launch: block b is row (b - first_block) of the job
@@ -251,7 +273,7 @@ __global__ __launch_bounds__(256) void ntt_r8_group_kernel(const NttJob* __restr
   }
   const NttJob& jb = jobs[lo];
   const uint64_t row = blockIdx.x - jb.first_block;
-  ntt_r8_row(jb.in + row * jb.rs, jb.es, jb.n_in, jb.logN, jb.tw, jb.out + (row << jb.logN), sm);
+  ntt_r8_row(jb.in + row * jb.rs, jb.es, jb.n_in, jb.logN, jb.tw, 0, jb.out + (row << jb.logN), sm);
 }
 
 hipError_t launch_ntt_group(const NttJob* jobs, int njobs, uint32_t total_blocks, int max_logN, hipStream_t st) {
@@ -307,8 +329,12 @@ hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_i
   uint32_t T = 1u << lt;
   uint32_t nblk = 1u << (logN - lt);
   uint32_t threads = T / 2 < 256 ? (T / 2 < 64 ? 64 : T / 2) : 256;
-  hipLaunchKernelGGL(ntt_lds_kernel, dim3(rows * nblk), dim3(threads), T * sizeof(uint64_t), st,
-                     in, rs, es, n_in, logN, lt, tw, out);
+  if (impl == 0)
+    hipLaunchKernelGGL(ntt_r8_blocks_kernel, dim3(rows * nblk), dim3(256), T * sizeof(uint64_t), st, in, rs,
+                       es, n_in, logN, lt, tw, out);
+  else
+    hipLaunchKernelGGL(ntt_lds_kernel, dim3(rows * nblk), dim3(threads), T * sizeof(uint64_t), st,
+                       in, rs, es, n_in, logN, lt, tw, out);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   uint64_t work = (uint64_t)rows << (logN - 1);

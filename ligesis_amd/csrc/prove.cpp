@@ -243,6 +243,12 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     const Sumcheck& S = sc[which];
     for (uint32_t t = 0; t < S.nv; t++) {
       uint64_t half = uint64_t(1) << (S.nv - 1 - t);
+      if (2 * half <= 1024) {  // the remaining rounds fit one workgroup's LDS
+        hip_check(launch_sc_tail(S.d, polys.data(), (uint32_t)(2 * half), B + ch_off + t,
+                                 res.ptr + S.rounds_at + (size_t)t * (S.d + 1), res.ptr + S.finals_at, st),
+                  "sumcheck tail");
+        return;
+      }
       hip_check(launch_sc_round(S.d, polys.data(), half, B + ch_off + t, part.ptr, st), "sumcheck round");
       hip_check(launch_reduce_rows(part.ptr, sc_round_blocks(half), S.d + 1,
                                    res.ptr + S.rounds_at + (size_t)t * (S.d + 1), st),

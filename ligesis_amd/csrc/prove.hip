@@ -113,14 +113,95 @@ __global__ __launch_bounds__(256) void dot_kernel(const ef2* __restrict__ a, con
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
 }
 
-// out[q] = sum_{b < nb} part[b * width + q], for q < width (one block)
-__global__ void reduce_rows_kernel(const ef2* __restrict__ part, uint32_t nb, uint32_t width,
-                                   ef2* __restrict__ out) {
-  for (uint32_t q = threadIdx.x; q < width; q += blockDim.x) {
-    ef2 acc = ef_zero();
-    for (uint32_t b = 0; b < nb; b++) acc = ef_add(acc, part[(uint64_t)b * width + q]);
-    out[q] = acc;
+// out[q] = sum_{b < nb} part[b * width + q], for q < width <= 4 (one block of 256 threads:
+// strided partial sums, then a tree in LDS)
+__global__ __launch_bounds__(256) void reduce_rows_kernel(const ef2* __restrict__ part, uint32_t nb,
+                                                          uint32_t width, ef2* __restrict__ out) {
+  __shared__ ef2 sh[256 * 4];
+  ef2 acc[4] = {ef_zero(), ef_zero(), ef_zero(), ef_zero()};
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
+    for (uint32_t q = 0; q < width; q++) acc[q] = ef_add(acc[q], part[(uint64_t)b * width + q]);
+  block_sum<4>(acc, sh);
+  if (threadIdx.x == 0)
+    for (uint32_t q = 0; q < width; q++) out[q] = acc[q];
+}
+
+// The last rounds of a product sumcheck in one workgroup: the factors (at most 2 * 512 entries
+// each) are folded in LDS round after round; round polynomials go to `rounds` (D + 1 each) and
+// the final evaluations to `finals`.
+template <int D>
+__global__ __launch_bounds__(256) void sc_tail_kernel(const ef2* __restrict__ p0, const ef2* __restrict__ p1,
+                                                      const ef2* __restrict__ p2, uint32_t len,
+                                                      const ef2* __restrict__ chal, ef2* __restrict__ rounds,
+                                                      ef2* __restrict__ finals) {
+  __shared__ ef2 buf[D][1024];
+  __shared__ ef2 sh[256 * (D + 1)];
+  const ef2* src[3] = {p0, p1, p2};
+#pragma unroll
+  for (int j = 0; j < D; j++)
+    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) buf[j][i] = src[j][i];
+  __syncthreads();
+  uint32_t t = 0;
+  for (uint32_t cur = len; cur > 1; cur >>= 1, t++) {
+    const uint32_t half = cur / 2;
+    const ef2 r = chal[t];
+    ef2 acc[D + 1];
+#pragma unroll
+    for (int q = 0; q <= D; q++) acc[q] = ef_zero();
+    ef2 folded[D][2];
+    for (uint32_t k = 0; k < 2; k++) {
+      const uint32_t i = threadIdx.x + k * blockDim.x;
+      if (i >= half) continue;
+      ef2 lo[D], diff[D], x[D];
+#pragma unroll
+      for (int j = 0; j < D; j++) {
+        lo[j] = buf[j][i];
+        diff[j] = ef_sub(buf[j][i + half], lo[j]);
+        folded[j][k] = ef_add(lo[j], ef_mul(r, diff[j]));
+        x[j] = lo[j];
+      }
+#pragma unroll
+      for (int q = 0; q <= D; q++) {
+        ef2 prod = x[0];
+#pragma unroll
+        for (int j = 1; j < D; j++) prod = ef_mul(prod, x[j]);
+        acc[q] = ef_add(acc[q], prod);
+#pragma unroll
+        for (int j = 0; j < D; j++) x[j] = ef_add(x[j], diff[j]);
+      }
+    }
+    block_sum<D + 1>(acc, sh);  // ends with a barrier: every read of buf is done
+    for (uint32_t k = 0; k < 2; k++) {
+      const uint32_t i = threadIdx.x + k * blockDim.x;
+      if (i < half) {
+#pragma unroll
+        for (int j = 0; j < D; j++) buf[j][i] = folded[j][k];
+      }
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int q = 0; q <= D; q++) rounds[(uint64_t)t * (D + 1) + q] = acc[q];
+    }
+    __syncthreads();
   }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < D; j++) finals[j] = buf[j][0];
+  }
+}
+
+hipError_t launch_sc_tail(int d, ef2* const* polys, uint32_t len, const ef2* chal, ef2* rounds, ef2* finals,
+                          hipStream_t st) {
+  if (len > 1024) return hipErrorInvalidValue;
+  if (d == 2)
+    hipLaunchKernelGGL(sc_tail_kernel<2>, dim3(1), dim3(256), 0, st, polys[0], polys[1], nullptr, len, chal,
+                       rounds, finals);
+  else if (d == 3)
+    hipLaunchKernelGGL(sc_tail_kernel<3>, dim3(1), dim3(256), 0, st, polys[0], polys[1], polys[2], len, chal,
+                       rounds, finals);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part /* >= 256 */, ef2* out,
@@ -128,7 +209,7 @@ hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part /* >= 25
   uint32_t nb = (uint32_t)((n + 255) / 256);
   if (nb > 256) nb = 256;
   hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(256), 0, st, a, b, n, part);
-  hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(64), 0, st, part, nb, 1u, out);
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(256), 0, st, part, nb, 1u, out);
   return hipGetLastError();
 }
 
@@ -267,7 +348,7 @@ hipError_t launch_sc_round(int d, ef2* const* polys, uint64_t half, const ef2* r
 }
 
 hipError_t launch_reduce_rows(const ef2* part, uint32_t nb, uint32_t width, ef2* out, hipStream_t st) {
-  hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(64), 0, st, part, nb, width, out);
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(256), 0, st, part, nb, width, out);
   return hipGetLastError();
 }
 

@@ -28,6 +28,9 @@ hipError_t launch_planar_to_ef(const uint64_t* c0, const uint64_t* c1, uint64_t 
 uint32_t sc_round_blocks(uint64_t half);
 hipError_t launch_sc_round(int d, ef2* const* polys, uint64_t half, const ef2* r, ef2* part, hipStream_t st);
 hipError_t launch_reduce_rows(const ef2* part, uint32_t nb, uint32_t width, ef2* out, hipStream_t st);
+// All rounds of a sumcheck whose factors have len <= 1024 entries left, in one workgroup.
+hipError_t launch_sc_tail(int d, ef2* const* polys, uint32_t len, const ef2* chal, ef2* rounds, ef2* finals,
+                          hipStream_t st);
 hipError_t launch_bf_combine(const BfChannels& ch, uint64_t D, ef2* out, hipStream_t st);
 hipError_t launch_fold_low(const uint64_t* poly, int ext, int li, uint64_t nout, const ef2* eq4, ef2* m,
                            hipStream_t st);

@@ -220,6 +220,18 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
                    const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,
                    std::vector<uint8_t>& proof, uint64_t used[2]);
 
+// server.cpp
+struct Server;
+uint64_t request_footprint(uint32_t log_m, uint32_t log_n, bool host_input);
+Server* server_start(int device, uint64_t budget, uint32_t workers, uint32_t max_batch);
+uint64_t server_submit(Server* s, const Key* key, const uint64_t* f, uint64_t len, uint32_t flags,
+                       const uint64_t* z, uint64_t z_len, const uint64_t* fields, uint64_t nf,
+                       const uint64_t* ints, uint64_t ni);
+int server_poll(Server* s, uint64_t ticket, bool wait, uint8_t root[32], uint8_t* proof, uint64_t* proof_len,
+                double times[3], std::string& err);
+void server_stop(Server* s);
+void server_stats(Server* s, uint64_t out[4]);
+
 // Shape helpers restating pcs/src/ligesis.rs:175-203.
 inline void committed_vars(uint32_t lm, uint32_t ln, uint32_t out[3]) {
   out[0] = 11 + lm;  // A-bar: C * eta * m
