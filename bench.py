@@ -63,13 +63,17 @@ def main():
     ap.add_argument("--hash", type=int, default=0, help="0 auto, 1 MFMA, 2 VALU")
     ap.add_argument("--hash-impl", type=int, default=0, help="MFMA kernel: 0 pipelined, 1 pipelined/1 slab, 2 basic")
     ap.add_argument("--ksplit", type=int, default=0, help="hash K-split (0 auto)")
-    ap.add_argument("--ntt-impl", type=int, default=0, help="0 radix-8 passes, 1 radix-2 stages")
+    ap.add_argument("--ntt-impl", type=int, default=0,
+                    help="0/2/3 radix-8 passes with 256/512/1024 threads per row, 1 radix-2 stages")
     ap.add_argument("--workload", default="commit", choices=["commit", "prove", "batch", "serve"],
                     help="commit: the headline metric; prove: mu=24 eval-prove at 2^8 x 2^16; "
                          "batch: 64 polynomials with mu uniform in [16, 24] in one batched commit")
     ap.add_argument("--batch", type=int, default=64, help="batch workload: number of polynomials")
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
     ap.add_argument("--log-m", type=int, default=8, help="prove workload: log2 rows")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="commit workload: also run a 256 MiB device-to-device copy (a plain copy "
+                         "kernel of known size that scales the profiler's FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -363,6 +367,13 @@ def bench_commit(args):
     pk, _ = L.Ligesis.setup(LOG_M, LOG_N, inp.a, ctx=ctx, commit_only=True)
     f_dev = torch.from_numpy(inp.f.view(np.int64)).to("cuda:0")
     torch.cuda.synchronize()
+    if args.calibrate_copy:
+        src = torch.empty(32 << 20, dtype=torch.int64, device="cuda:0").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     ctx.set_option("profile", 1)
     roots = set()

@@ -61,8 +61,9 @@ void lgs_shutdown(lgs_ctx* ctx);
 const char* lgs_last_error(const lgs_ctx* ctx);
 
 /* Tuning knobs that never change results: "hash" (0 auto, 1 MFMA, 2 VALU), "hash_ksplit"
- * (0 auto), "hash_impl" (MFMA kernel: 0 pipelined, 1 pipelined one slab per stage, 2 basic),
- * "ntt_impl" (0 radix-8 passes, 1 radix-2 stages), "profile" (1: time every stage with HIP
+ * (0 auto), "hash_impl" (MFMA kernel: 0 pipelined, 1 pipelined one slab per stage, 2 basic, 3 pipelined with
+ * 256 x 256 workgroup tiles),
+ * "ntt_impl" (0/2/3 radix-8 passes with 256/512/1024 threads per row, 1 radix-2 stages), "profile" (1: time every stage with HIP
  * events on the context stream). */
 int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value);
 

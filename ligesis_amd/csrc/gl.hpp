@@ -40,15 +40,22 @@ LGS_HD uint64_t gl_reduce(uint64_t lo, uint64_t hi) {
   return gl_canon(r);
 }
 
-LGS_HD uint64_t mulhi64(uint64_t a, uint64_t b) {
-#ifdef __HIP_DEVICE_COMPILE__
-  return __umul64hi(a, b);
-#else
-  return (uint64_t)(((unsigned __int128)a * b) >> 64);
-#endif
+// Full 64 x 64 -> 128-bit product from four 32 x 32 + 64 multiply-adds (v_mad_u64_u32); no
+// partial sum can overflow: (2^32 - 1)^2 + 2 (2^32 - 1) = 2^64 - 1.
+LGS_HD void mul_wide(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  const uint64_t p00 = (uint64_t)a0 * b0;
+  const uint64_t p01 = (uint64_t)a0 * b1 + (p00 >> 32);
+  const uint64_t p10 = (uint64_t)a1 * b0 + (uint32_t)p01;
+  hi = (uint64_t)a1 * b1 + (p01 >> 32) + (p10 >> 32);
+  lo = (p10 << 32) | (uint32_t)p00;
 }
 
-LGS_HD uint64_t gl_mul(uint64_t a, uint64_t b) { return gl_reduce(a * b, mulhi64(a, b)); }
+LGS_HD uint64_t gl_mul(uint64_t a, uint64_t b) {
+  uint64_t lo, hi;
+  mul_wide(a, b, lo, hi);
+  return gl_reduce(lo, hi);
+}
 
 LGS_HD uint64_t gl_pow(uint64_t a, uint64_t e) {
   uint64_t r = 1;

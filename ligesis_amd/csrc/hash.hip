@@ -152,21 +152,23 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
 // Pipelined variant: A digits double-buffered in LDS (one barrier per stage of SPS slabs), the
 // next stage's digits and F' words loaded into registers while the current stage computes,
 // and at most 256 registers per lane so two workgroups share a CU.
-template <int SPS>
+// WN waves along N: a workgroup has 2 x WN waves (128 WN threads) and a tile of 256 rows x
+// 64 WN columns; every wave owns 128 rows x 64 columns.
+template <int SPS, int WN>
 __device__ __forceinline__ void hash_pipe_body(const uint64_t* __restrict__ enc, uint64_t ld,
                                                const v4i* __restrict__ adig, uint32_t slabs_per_split,
                                                uint32_t split, uint32_t ntile, uint32_t ncols,
                                                int32_t* __restrict__ part, v4i (*lds_a)[SPS * 1024]);
 
-template <int SPS>
-__global__ __launch_bounds__(256, 2) void hash_mfma_pipe_kernel(const uint64_t* __restrict__ enc,
-                                                                uint64_t ld, const v4i* __restrict__ adig,
-                                                                uint32_t slabs_per_split, uint32_t ntiles,
-                                                                uint32_t ncols, int32_t* __restrict__ part) {
+template <int SPS, int WN>
+__global__ __launch_bounds__(128 * WN, 2) void hash_mfma_pipe_kernel(const uint64_t* __restrict__ enc,
+                                                                     uint64_t ld, const v4i* __restrict__ adig,
+                                                                     uint32_t slabs_per_split, uint32_t ntiles,
+                                                                     uint32_t ncols, int32_t* __restrict__ part) {
   __shared__ v4i lds_a[2][SPS * 1024];
   const uint32_t logical = xcd_logical_block();
-  hash_pipe_body<SPS>(enc, ld, adig, slabs_per_split, logical / ntiles, logical % ntiles, ncols, part,
-                      lds_a);
+  hash_pipe_body<SPS, WN>(enc, ld, adig, slabs_per_split, logical / ntiles, logical % ntiles, ncols, part,
+                          lds_a);
 }
 
 // Tiles of many commitments in one launch (batched commit): logical tile -> job by binary
@@ -182,20 +184,22 @@ __global__ __launch_bounds__(256, 2) void hash_mfma_group_kernel(const HashJob* 
   }
   const HashJob& jb = jobs[lo];
   const uint32_t local = logical - jb.first_block;
-  hash_pipe_body<2>(jb.enc, jb.ld, (const v4i*)jb.adig, jb.slabs_per_split, local / jb.ntiles,
+  hash_pipe_body<2, 2>(jb.enc, jb.ld, (const v4i*)jb.adig, jb.slabs_per_split, local / jb.ntiles,
                     local % jb.ntiles, jb.ncols, jb.part, lds_a);
 }
 
-template <int SPS>
+template <int SPS, int WN>
 __device__ __forceinline__ void hash_pipe_body(const uint64_t* __restrict__ enc, uint64_t ld,
                                                const v4i* __restrict__ adig, uint32_t slabs_per_split,
                                                uint32_t split, uint32_t ntile, uint32_t ncols,
                                                int32_t* __restrict__ part, v4i (*lds_a)[SPS * 1024]) {
+  constexpr int T = 128 * WN;           // threads
+  constexpr int NA = SPS * 1024 / T;    // 16-byte A words staged per thread per stage
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t wm = wave >> 1, wn = wave & 1;
+  const uint32_t wm = wave / WN, wn = wave % WN;
   const uint32_t g = lane >> 4, c = lane & 15;
   const uint32_t kb = 32 * (g >> 1) + 4 * (g & 1);
-  const uint64_t col_base = (uint64_t)ntile * HM_NT + wn * 64 + c;
+  const uint64_t col_base = (uint64_t)ntile * (64 * WN) + wn * 64 + c;
 
   v4i acc[8][4];
 #pragma unroll
@@ -205,18 +209,18 @@ __device__ __forceinline__ void hash_pipe_body(const uint64_t* __restrict__ enc,
 
   const uint64_t slab0 = (uint64_t)split * slabs_per_split;
   const uint32_t nstages = slabs_per_split / SPS;
-  v4i pa[SPS * 4];
+  v4i pa[NA];
   uint64_t px[SPS][4];
   {
     const v4i* src = adig + slab0 * 1024;
 #pragma unroll
-    for (int q = 0; q < SPS * 4; q++) pa[q] = src[q * 256 + threadIdx.x];
+    for (int q = 0; q < NA; q++) pa[q] = src[q * T + threadIdx.x];
 #pragma unroll
     for (int ss = 0; ss < SPS; ss++)
 #pragma unroll
       for (int nt = 0; nt < 4; nt++) px[ss][nt] = enc[(slab0 + ss) * ld + col_base + nt * 16];
 #pragma unroll
-    for (int q = 0; q < SPS * 4; q++) lds_a[0][q * 256 + threadIdx.x] = pa[q];
+    for (int q = 0; q < NA; q++) lds_a[0][q * T + threadIdx.x] = pa[q];
   }
   __syncthreads();
   for (uint32_t st = 0; st < nstages; st++) {
@@ -231,7 +235,7 @@ __device__ __forceinline__ void hash_pipe_body(const uint64_t* __restrict__ enc,
       const uint64_t s1 = slab0 + (uint64_t)(st + 1) * SPS;
       const v4i* src = adig + s1 * 1024;
 #pragma unroll
-      for (int q = 0; q < SPS * 4; q++) pa[q] = src[q * 256 + threadIdx.x];
+      for (int q = 0; q < NA; q++) pa[q] = src[q * T + threadIdx.x];
 #pragma unroll
       for (int ss = 0; ss < SPS; ss++)
 #pragma unroll
@@ -256,7 +260,7 @@ __device__ __forceinline__ void hash_pipe_body(const uint64_t* __restrict__ enc,
     }
     if (more) {
 #pragma unroll
-      for (int q = 0; q < SPS * 4; q++) lds_a[buf ^ 1][q * 256 + threadIdx.x] = pa[q];
+      for (int q = 0; q < NA; q++) lds_a[buf ^ 1][q * T + threadIdx.x] = pa[q];
     }
     __syncthreads();
   }
@@ -267,7 +271,7 @@ __device__ __forceinline__ void hash_pipe_body(const uint64_t* __restrict__ enc,
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const uint32_t m = (wm * 8 + mt) * 16 + 4 * g + v;
-        const uint64_t col = (uint64_t)ntile * HM_NT + wn * 64 + nt * 16 + c;
+        const uint64_t col = (uint64_t)ntile * (64 * WN) + wn * 64 + nt * 16 + c;
         part[((uint64_t)split * 256 + m) * ncols + col] = acc[mt][nt][v];
       }
     }
@@ -353,14 +357,17 @@ hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
                             hipStream_t st, hipEvent_t mid, int impl) {
   uint32_t ntiles = ncols / HM_NT;
   uint32_t sps = m_rows / ksplit;
-  if (impl == 2)
+  if (impl == 3 && ncols % 256 == 0)
+    hipLaunchKernelGGL((hash_mfma_pipe_kernel<2, 4>), dim3((ncols / 256) * ksplit), dim3(512), 0, st, enc, ld,
+                       (const v4i*)dig, sps, ncols / 256, ncols, part);
+  else if (impl == 2)
     hipLaunchKernelGGL(hash_mfma_kernel, dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
                        (const v4i*)dig, sps, ntiles, ncols, part);
   else if (impl == 1)
-    hipLaunchKernelGGL(hash_mfma_pipe_kernel<1>, dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
+    hipLaunchKernelGGL((hash_mfma_pipe_kernel<1, 2>), dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
                        (const v4i*)dig, sps, ntiles, ncols, part);
   else
-    hipLaunchKernelGGL(hash_mfma_pipe_kernel<2>, dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
+    hipLaunchKernelGGL((hash_mfma_pipe_kernel<2, 2>), dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
                        (const v4i*)dig, sps, ntiles, ncols, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

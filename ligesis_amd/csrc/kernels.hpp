@@ -47,8 +47,9 @@ struct MerkleJob {
 
 // ntt.hip
 hipError_t launch_twiddles(uint64_t* tw, int logN, hipStream_t st);
-// impl 0: radix-8 passes with the whole row in LDS when 2^logN <= 2^LGS_LDS_LOG; otherwise (or
-// impl 1) radix-2 stages, the first LGS_LDS_LOG of them in LDS and the rest as global passes.
+// impl 0 / 2 / 3: radix-8 passes with the whole row in LDS when 2^logN <= 2^LGS_LDS_LOG, with 256 /
+// 512 / 1024 threads per row; larger rows run radix-8 sub-blocks then global radix-2 passes.
+// impl 1: radix-2 stages, the first LGS_LDS_LOG of them in LDS and the rest as global passes.
 hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_in, uint32_t rows,
                       int logN, const uint64_t* tw, uint64_t* out, hipStream_t st, int impl = 0);
 

@@ -233,7 +233,7 @@ __device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uin
   }
 }
 
-__global__ __launch_bounds__(256) void ntt_r8_kernel(const uint64_t* __restrict__ in, uint64_t rs,
+__global__ __launch_bounds__(1024) void ntt_r8_kernel(const uint64_t* __restrict__ in, uint64_t rs,
                                                      uint64_t es, uint32_t n_in, int logN,
                                                      const uint64_t* __restrict__ tw,
                                                      uint64_t* __restrict__ out) {
@@ -318,9 +318,10 @@ hipError_t launch_twiddles(uint64_t* tw, int logN, hipStream_t st) {
 hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_in, uint32_t rows,
                       int logN, const uint64_t* tw, uint64_t* out, hipStream_t st, int impl) {
   if (rows == 0) return hipSuccess;
-  if (impl == 0 && logN >= 1 && logN <= LGS_LDS_LOG) {
+  if (impl != 1 && logN >= 1 && logN <= LGS_LDS_LOG) {
     const uint32_t N = 1u << logN;
-    const uint32_t threads = N / 8 >= 256 ? 256 : 64;
+    const uint32_t want = impl == 2 ? 512 : impl == 3 ? 1024 : 256;
+    const uint32_t threads = N / 8 >= want ? want : (N / 8 >= 256 ? 256 : 64);
     hipLaunchKernelGGL(ntt_r8_kernel, dim3(rows), dim3(threads), N * sizeof(uint64_t), st, in, rs, es,
                        n_in, logN, tw, out);
     return hipGetLastError();
@@ -329,7 +330,7 @@ hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_i
   uint32_t T = 1u << lt;
   uint32_t nblk = 1u << (logN - lt);
   uint32_t threads = T / 2 < 256 ? (T / 2 < 64 ? 64 : T / 2) : 256;
-  if (impl == 0)
+  if (impl != 1)
     hipLaunchKernelGGL(ntt_r8_blocks_kernel, dim3(rows * nblk), dim3(256), T * sizeof(uint64_t), st, in, rs,
                        es, n_in, logN, lt, tw, out);
   else
