@@ -238,21 +238,25 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   uint64_t eq_scratch_n = (uint64_t(1) << (kmax / 2)) + (uint64_t(1) << (kmax - kmax / 2)) + 64;
   DevBuf<ef2> eqs(pool, eq_scratch_n);
   DevBuf<ef2> part(pool, 1024 * 4 + 256);
+  // completion counters of the kernels that reduce their own partial sums; each kernel leaves
+  // its counter at zero again, so one clear serves the whole prove
+  DevBuf<uint64_t> counters(pool, 2);
+  hip_check(launch_zero(counters.ptr, 2, st), "clear counters");
+  unsigned int* counter = (unsigned int*)counters.ptr;
 
   auto run_sumcheck = [&](int which, std::vector<ef2*> polys, size_t ch_off) {
     const Sumcheck& S = sc[which];
     for (uint32_t t = 0; t < S.nv; t++) {
       uint64_t half = uint64_t(1) << (S.nv - 1 - t);
-      if (2 * half <= 1024) {  // the remaining rounds fit one workgroup's LDS
+      if (2 * half <= 128) {  // the remaining rounds run in one wave
         hip_check(launch_sc_tail(S.d, polys.data(), (uint32_t)(2 * half), B + ch_off + t,
                                  res.ptr + S.rounds_at + (size_t)t * (S.d + 1), res.ptr + S.finals_at, st),
                   "sumcheck tail");
         return;
       }
-      hip_check(launch_sc_round(S.d, polys.data(), half, B + ch_off + t, part.ptr, st), "sumcheck round");
-      hip_check(launch_reduce_rows(part.ptr, sc_round_blocks(half), S.d + 1,
-                                   res.ptr + S.rounds_at + (size_t)t * (S.d + 1), st),
-                "sumcheck reduce");
+      hip_check(launch_sc_round(S.d, polys.data(), half, B + ch_off + t, part.ptr, counter,
+                                res.ptr + S.rounds_at + (size_t)t * (S.d + 1), st),
+                "sumcheck round");
     }
     for (int j = 0; j < S.d; j++)
       hip_check(hipMemcpyAsync(res.ptr + S.finals_at + j, polys[j], sizeof(ef2), hipMemcpyDeviceToDevice, st),
@@ -269,7 +273,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     DevBuf<ef2> wpart(pool, chunks * n);
     hip_check(launch_wrows(cm.f, (uint32_t)m, n, eqz1.ptr, wpart.ptr, (uint32_t)chunks, a_vec.ptr, st), "a-bar");
   }
-  hip_check(launch_dot(a_vec.ptr, eqz2.ptr, n, part.ptr, res.ptr + o_y, st), "y");
+  hip_check(launch_dot(a_vec.ptr, eqz2.ptr, n, part.ptr, counter, res.ptr + o_y, st), "y");
   DevBasefold av_state;
   enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec.ptr, true, log_n, li_a, 1);
 
@@ -307,7 +311,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   hip_check(hipMemcpyAsync(g2b.ptr, g2.ptr, W * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
   DevBuf<ef2> sel(pool, two_n);
   hip_check(launch_sel(didx.ptr, eqr2.ptr, (uint32_t)s, sel.ptr, two_n, st), "sel");
-  hip_check(launch_dot(sel.ptr, ra.ptr, two_n, part.ptr, res.ptr + o_y4, st), "y4");
+  hip_check(launch_dot(sel.ptr, ra.ptr, two_n, part.ptr, counter, res.ptr + o_y4, st), "y4");
   hip_check(launch_vvec(eqz1.ptr, W, vv.ptr, st), "v");
   run_sumcheck(1, {g1.ptr, g2.ptr}, o_chhash);
   run_sumcheck(2, {vv.ptr, g2b.ptr}, o_chrecon);
@@ -369,16 +373,17 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   const uint64_t* twD = ctx.twiddle(ilog2(D));
   for (uint32_t t = 0; t < R; t++) {
     const uint64_t half = uint64_t(1) << (R - 1 - t);
+    BfRound br{};
     for (int c = 0; c < 3; c++) {
-      ef2* cur = mbuf[c][t & 1].ptr;
-      ef2* nxt = mbuf[c][(t + 1) & 1].ptr;
-      ef2* e_cur = eqr[c][t & 1].ptr;
-      hip_check(launch_bf_round(cur, half, e_cur, B + o_chbatch + t, nxt, part.ptr, st), "batch round");
-      hip_check(launch_reduce_rows(part.ptr, bf_round_blocks(half), 1, res.ptr + o_l0 + (size_t)c * R + t, st),
-                "batch reduce");
-      if (half > 1)
-        hip_check(launch_eq_pairsum(e_cur, half / 2, eqr[c][(t + 1) & 1].ptr, st), "batch eq step");
+      br.m_in[c] = mbuf[c][t & 1].ptr;
+      br.m_out[c] = mbuf[c][(t + 1) & 1].ptr;
+      br.eq[c] = eqr[c][t & 1].ptr;
+      br.eq_next[c] = eqr[c][(t + 1) & 1].ptr;
+      br.out[c] = res.ptr + o_l0 + (size_t)c * R + t;
     }
+    br.part = part.ptr;
+    br.counter = counter;
+    hip_check(launch_bf_round3(br, half, B + o_chbatch + t, st), "batch round");
     const uint64_t len = D >> t;
     hip_check(launch_eq_fold_codeword(levels.ptr + lvl_off[t], len / 2, B + o_chbatch + t,
                                       ctx.twiddle(ilog2(len)), levels.ptr + lvl_off[t + 1], st),

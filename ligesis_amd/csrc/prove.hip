@@ -30,6 +30,41 @@ __device__ __forceinline__ void block_sum(ef2 (&v)[W], ef2* sh /* blockDim x W *
   }
 }
 
+// Called by every block after its thread 0 stored W partial sums at part[blockIdx.x * W]: the
+// last block to finish (ticket from an atomic counter) adds up all gridDim.x partials into
+// out[0..W) and re-arms the counter for the next launch on the stream. Saves a separate
+// reduction launch per round.
+template <int W>
+__device__ __forceinline__ void last_block_reduce(const ef2* part, unsigned int* counter, ef2* out, ef2* sh) {
+  __shared__ int is_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned int ticket = atomicAdd(counter, 1u);
+    is_last = ticket == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  ef2 acc[W];
+#pragma unroll
+  for (int q = 0; q < W; q++) acc[q] = ef_zero();
+  const volatile uint64_t* pv = (const volatile uint64_t*)part;
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < W; q++) {
+      const uint64_t k = 2 * ((uint64_t)b * W + q);
+      acc[q] = ef_add(acc[q], ef2{pv[k], pv[k + 1]});
+    }
+  }
+  block_sum<W>(acc, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < W; q++) out[q] = acc[q];
+    *counter = 0;
+  }
+}
+
 // ---------------- eq tables (MlPoly::new_eq, poly.rs:44-53) ----------------
 // lo[b] = prod_{t < klo} (bit t of b ? p_t : 1 - p_t), hi likewise over p_{klo..k}.
 __global__ void eq_parts_kernel(const ef2* __restrict__ pt, int k, int klo, ef2* __restrict__ lo,
@@ -104,13 +139,15 @@ hipError_t launch_wrows(const uint64_t* M, uint32_t rows, uint64_t cols, const e
 
 // ---------------- dot products: out = sum_i a[i] * b[i] ----------------
 __global__ __launch_bounds__(256) void dot_kernel(const ef2* __restrict__ a, const ef2* __restrict__ b,
-                                                  uint64_t n, ef2* __restrict__ part) {
+                                                  uint64_t n, ef2* __restrict__ part, unsigned int* counter,
+                                                  ef2* __restrict__ out) {
   __shared__ ef2 sh[256];
   ef2 acc[1] = {ef_zero()};
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     acc[0] = ef_add(acc[0], ef_mul(a[i], b[i]));
   block_sum<1>(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+  last_block_reduce<1>(part, counter, out, sh);
 }
 
 // out[q] = sum_{b < nb} part[b * width + q], for q < width <= 4 (one block of 256 threads:
@@ -129,13 +166,13 @@ __global__ __launch_bounds__(256) void reduce_rows_kernel(const ef2* __restrict_
 // The last rounds of a product sumcheck in one workgroup: the factors (at most 2 * 512 entries
 // each) are folded in LDS round after round; round polynomials go to `rounds` (D + 1 each) and
 // the final evaluations to `finals`.
-template <int D>
+template <int D, int CAP>
 __global__ __launch_bounds__(256) void sc_tail_kernel(const ef2* __restrict__ p0, const ef2* __restrict__ p1,
                                                       const ef2* __restrict__ p2, uint32_t len,
                                                       const ef2* __restrict__ chal, ef2* __restrict__ rounds,
                                                       ef2* __restrict__ finals) {
-  __shared__ ef2 buf[D][1024];
-  __shared__ ef2 sh[256 * (D + 1)];
+  __shared__ ef2 buf[D][CAP];
+  __shared__ ef2 sh[(CAP / 2) * (D + 1)];
   const ef2* src[3] = {p0, p1, p2};
 #pragma unroll
   for (int j = 0; j < D; j++)
@@ -192,24 +229,24 @@ __global__ __launch_bounds__(256) void sc_tail_kernel(const ef2* __restrict__ p0
 
 hipError_t launch_sc_tail(int d, ef2* const* polys, uint32_t len, const ef2* chal, ef2* rounds, ef2* finals,
                           hipStream_t st) {
-  if (len > 1024) return hipErrorInvalidValue;
+  // one wave: every lane owns one pair of the first tail round, LDS tree sums over 64 lanes
+  if (len > 128) return hipErrorInvalidValue;
   if (d == 2)
-    hipLaunchKernelGGL(sc_tail_kernel<2>, dim3(1), dim3(256), 0, st, polys[0], polys[1], nullptr, len, chal,
+    hipLaunchKernelGGL((sc_tail_kernel<2, 128>), dim3(1), dim3(64), 0, st, polys[0], polys[1], nullptr, len, chal,
                        rounds, finals);
   else if (d == 3)
-    hipLaunchKernelGGL(sc_tail_kernel<3>, dim3(1), dim3(256), 0, st, polys[0], polys[1], polys[2], len, chal,
-                       rounds, finals);
+    hipLaunchKernelGGL((sc_tail_kernel<3, 128>), dim3(1), dim3(64), 0, st, polys[0], polys[1], polys[2], len,
+                       chal, rounds, finals);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part /* >= 256 */, ef2* out,
-                      hipStream_t st) {
+hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part /* >= 256 */, unsigned int* counter,
+                      ef2* out, hipStream_t st) {
   uint32_t nb = (uint32_t)((n + 255) / 256);
   if (nb > 256) nb = 256;
-  hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(256), 0, st, a, b, n, part);
-  hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(256), 0, st, part, nb, 1u, out);
+  hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(256), 0, st, a, b, n, part, counter, out);
   return hipGetLastError();
 }
 
@@ -293,7 +330,8 @@ template <int D>
 __global__ __launch_bounds__(256) void sc_round_kernel(ef2* __restrict__ p0, ef2* __restrict__ p1,
                                                        ef2* __restrict__ p2, uint64_t half,
                                                        const ef2* __restrict__ r_ptr,
-                                                       ef2* __restrict__ part) {
+                                                       ef2* __restrict__ part, unsigned int* counter,
+                                                       ef2* __restrict__ out) {
   __shared__ ef2 sh[256 * (D + 1)];
   ef2* ps[3] = {p0, p1, p2};
   const ef2 r = *r_ptr;
@@ -328,6 +366,7 @@ __global__ __launch_bounds__(256) void sc_round_kernel(ef2* __restrict__ p0, ef2
 #pragma unroll
     for (int q = 0; q <= D; q++) part[(uint64_t)blockIdx.x * (D + 1) + q] = acc[q];
   }
+  last_block_reduce<D + 1>(part, counter, out, sh);
 }
 
 uint32_t sc_round_blocks(uint64_t half) {
@@ -336,12 +375,14 @@ uint32_t sc_round_blocks(uint64_t half) {
 }
 
 hipError_t launch_sc_round(int d, ef2* const* polys, uint64_t half, const ef2* r, ef2* part,
-                           hipStream_t st) {
+                           unsigned int* counter, ef2* out, hipStream_t st) {
   const uint32_t nb = sc_round_blocks(half);
   if (d == 2)
-    hipLaunchKernelGGL(sc_round_kernel<2>, dim3(nb), dim3(256), 0, st, polys[0], polys[1], nullptr, half, r, part);
+    hipLaunchKernelGGL(sc_round_kernel<2>, dim3(nb), dim3(256), 0, st, polys[0], polys[1], nullptr, half, r, part,
+                       counter, out);
   else if (d == 3)
-    hipLaunchKernelGGL(sc_round_kernel<3>, dim3(nb), dim3(256), 0, st, polys[0], polys[1], polys[2], half, r, part);
+    hipLaunchKernelGGL(sc_round_kernel<3>, dim3(nb), dim3(256), 0, st, polys[0], polys[1], polys[2], half, r,
+                       part, counter, out);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -408,6 +449,52 @@ __global__ __launch_bounds__(256) void bf_round_kernel(const ef2* __restrict__ m
   }
   block_sum<1>(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// All channels of one reduction round in one launch (blockIdx.y = channel): l0 partials with
+// the last block of each channel reducing them into out[c], the low-variable fold of m, and the
+// eq table of the next round's residual point (pair sums).
+__global__ __launch_bounds__(256) void bf_round3_kernel(BfRound a, uint64_t half, const ef2* __restrict__ r_ptr) {
+  __shared__ ef2 sh[256];
+  const int c = blockIdx.y;
+  const ef2* m = a.m_in[c];
+  const ef2* eq = a.eq[c];
+  ef2* m_out = a.m_out[c];
+  ef2* eq_next = a.eq_next[c];
+  const ef2 r = *r_ptr;
+  ef2 acc[1] = {ef_zero()};
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += (uint64_t)gridDim.x * blockDim.x) {
+    const ef2 e = m[2 * i], o = m[2 * i + 1];
+    acc[0] = ef_add(acc[0], ef_mul(e, eq[i]));
+    m_out[i] = ef_add(e, ef_mul(r, ef_sub(o, e)));
+    if (half > 1 && i < half / 2) eq_next[i] = ef_add(eq[2 * i], eq[2 * i + 1]);
+  }
+  block_sum<1>(acc, sh);
+  ef2* part = a.part + (uint64_t)c * gridDim.x;
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+  // per-channel reduction: the counter of channel c counts that channel's blocks
+  __shared__ int is_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    is_last = atomicAdd(a.counter + c, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  ef2 tot[1] = {ef_zero()};
+  const volatile uint64_t* pv = (const volatile uint64_t*)part;
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) tot[0] = ef_add(tot[0], ef2{pv[2 * b], pv[2 * b + 1]});
+  block_sum<1>(tot, sh);
+  if (threadIdx.x == 0) {
+    a.out[c][0] = tot[0];
+    a.counter[c] = 0;
+  }
+}
+
+hipError_t launch_bf_round3(const BfRound& a, uint64_t half, const ef2* r, hipStream_t st) {
+  hipLaunchKernelGGL(bf_round3_kernel, dim3(sc_round_blocks(half), 3), dim3(256), 0, st, a, half, r);
+  return hipGetLastError();
 }
 
 // eq over point[1..] from eq over point[0..]: out[i] = in[2i] + in[2i+1]

@@ -4,6 +4,18 @@
 
 namespace lgs {
 
+// One Basefold reduction round over its three channels (bf_round3_kernel, basefold.rs:748-775).
+struct BfRound {
+  const ef2* m_in[3];
+  const ef2* eq[3];
+  ef2* m_out[3];
+  ef2* eq_next[3];
+  ef2* out[3];
+  ef2* part;              // 3 x blocks
+  unsigned int* counter;  // 3 zeroed words
+};
+hipError_t launch_bf_round3(const BfRound& a, uint64_t half, const ef2* r, hipStream_t st);
+
 // Up to three Basefold channels combined in one pass (basefold.rs:732-741).
 struct BfChannels {
   int count;
@@ -17,7 +29,10 @@ hipError_t launch_eq(const ef2* pt, int k, ef2* scratch, const ef2* scale, int a
                      hipStream_t st);
 hipError_t launch_wrows(const uint64_t* M, uint32_t rows, uint64_t cols, const ef2* w, ef2* part,
                         uint32_t chunks, ef2* out, hipStream_t st);
-hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part, ef2* out, hipStream_t st);
+// Partial sums are reduced by the last block to finish; `counter` is a zeroed device word that
+// the kernel re-arms, so consecutive launches on one stream may share it.
+hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part, unsigned int* counter, ef2* out,
+                      hipStream_t st);
 hipError_t launch_bits_gather(const uint64_t* enc, uint64_t ld, const uint32_t* idx, uint32_t s, uint64_t W,
                               uint64_t* bi, ef2* bi_ef, ef2* bi_m1, hipStream_t st);
 hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st);
@@ -26,9 +41,10 @@ hipError_t launch_sel(const uint32_t* idx, const ef2* eq_r2, uint32_t s, ef2* se
 hipError_t launch_rs_weights(const ef2* r3, int log2n, uint64_t n, const uint64_t* tw, ef2* w, hipStream_t st);
 hipError_t launch_planar_to_ef(const uint64_t* c0, const uint64_t* c1, uint64_t n, ef2* out, hipStream_t st);
 uint32_t sc_round_blocks(uint64_t half);
-hipError_t launch_sc_round(int d, ef2* const* polys, uint64_t half, const ef2* r, ef2* part, hipStream_t st);
+hipError_t launch_sc_round(int d, ef2* const* polys, uint64_t half, const ef2* r, ef2* part,
+                           unsigned int* counter, ef2* out, hipStream_t st);
 hipError_t launch_reduce_rows(const ef2* part, uint32_t nb, uint32_t width, ef2* out, hipStream_t st);
-// All rounds of a sumcheck whose factors have len <= 1024 entries left, in one workgroup.
+// All rounds of a sumcheck whose factors have len <= 128 entries left, in one wave.
 hipError_t launch_sc_tail(int d, ef2* const* polys, uint32_t len, const ef2* chal, ef2* rounds, ef2* finals,
                           hipStream_t st);
 hipError_t launch_bf_combine(const BfChannels& ch, uint64_t D, ef2* out, hipStream_t st);
