@@ -61,10 +61,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--hash", type=int, default=0, help="0 auto, 1 MFMA, 2 VALU")
-    ap.add_argument("--hash-impl", type=int, default=0, help="MFMA kernel: 0 pipelined, 1 pipelined/1 slab, 2 basic")
+    ap.add_argument("--hash-impl", type=int, default=-1,
+                    help="MFMA kernel: -1 auto, 0 pipelined, 1 pipelined/1 slab, 2 basic, 3 256x256 tiles")
     ap.add_argument("--ksplit", type=int, default=0, help="hash K-split (0 auto)")
-    ap.add_argument("--ntt-impl", type=int, default=0,
-                    help="0/2/3 radix-8 passes with 256/512/1024 threads per row, 1 radix-2 stages")
+    ap.add_argument("--ntt-impl", type=int, default=-1,
+                    help="-1 auto, 0/2/3 radix-8 passes with 256/512/1024 threads per row, 1 radix-2 stages")
+    ap.add_argument("--merkle-lg", type=int, default=0, help="log2 Merkle workgroup width (0 auto, 6..8)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="commit workload: skip the autotuner (schedule options left at auto then "
+                         "take the built-in defaults)")
     ap.add_argument("--workload", default="commit", choices=["commit", "prove", "batch", "serve"],
                     help="commit: the headline metric; prove: mu=24 eval-prove at 2^8 x 2^16; "
                          "batch: 64 polynomials with mu uniform in [16, 24] in one batched commit")
@@ -364,9 +369,16 @@ def bench_commit(args):
     ctx.set_option("hash_impl", args.hash_impl)
     ctx.set_option("hash_ksplit", args.ksplit)
     ctx.set_option("ntt_impl", args.ntt_impl)
+    ctx.set_option("merkle_lg", args.merkle_lg)
     pk, _ = L.Ligesis.setup(LOG_M, LOG_N, inp.a, ctx=ctx, commit_only=True)
     f_dev = torch.from_numpy(inp.f.view(np.int64)).to("cuda:0")
     torch.cuda.synchronize()
+    tuned = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuned = ctx.autotune(pk, reps=5)
+        print(f"bench: autotune {tuned} in {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    schedule = ctx.tuning(LOG_M, LOG_N)
     if args.calibrate_copy:
         src = torch.empty(32 << 20, dtype=torch.int64, device="cuda:0").fill_(1)
         dst = torch.empty_like(src)
@@ -405,8 +417,11 @@ def bench_commit(args):
     ntt_bytes = 24.0 * m * n
     hash_tops = hash_ops / (avg["hash_ms"] / 1e3) / 1e12
     ntt_gbs = ntt_bytes / (avg["ntt_ms"] / 1e3) / 1e9
+    hash_kernel = ("hash_valu_kernel" if schedule["hash_variant"] == 2 else
+                   {0: "hash_mfma_pipe_kernel<2, 2>", 1: "hash_mfma_pipe_kernel<1, 2>", 2: "hash_mfma_kernel",
+                    3: "hash_mfma_pipe_kernel<2, 4>"}[schedule["hash_impl"]])
     if avg["hash_ms"] >= avg["ntt_ms"]:
-        roof = {"kernel": "hash_mfma_pipe_kernel (fused bit-decompose + H = A.B)", "bound": "mfma",
+        roof = {"kernel": f"{hash_kernel} (fused bit-decompose + H = A.B)", "bound": "mfma",
                 "achieved": hash_tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                 "frac": hash_tops / INT8_PEAK_TOPS, "traffic": None}
     else:
@@ -428,8 +443,8 @@ def bench_commit(args):
         "data": "synthetic (SplitMix64-seeded uniform Goldilocks f and A)",
         "config": {"workload": "plain single-polynomial commit, mu=24, m x n = 2^12 x 2^12, "
                                "rho=1/2, eta=64, C=32, BLAKE3 Merkle over 2^13 columns",
-                   "log_m": LOG_M, "log_n": LOG_N, "hash_variant": args.hash,
-                   "hash_impl": args.hash_impl, "hash_ksplit": args.ksplit, "ntt_impl": args.ntt_impl},
+                   "log_m": LOG_M, "log_n": LOG_N, "schedule": schedule,
+                   "autotuned": tuned is not None},
         "stages_ms": avg,
         "roofline": roof,
         "ntt_roofline": {"bound": "hbm", "achieved": ntt_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

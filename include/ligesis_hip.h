@@ -61,15 +61,41 @@ void lgs_shutdown(lgs_ctx* ctx);
 const char* lgs_last_error(const lgs_ctx* ctx);
 
 /* Tuning knobs that never change results: "hash" (0 auto, 1 MFMA, 2 VALU), "hash_ksplit"
- * (0 auto), "hash_impl" (MFMA kernel: 0 pipelined, 1 pipelined one slab per stage, 2 basic, 3 pipelined with
- * 256 x 256 workgroup tiles),
- * "ntt_impl" (0/2/3 radix-8 passes with 256/512/1024 threads per row, 1 radix-2 stages), "profile" (1: time every stage with HIP
- * events on the context stream). */
+ * (0 auto), "hash_impl" (-1 auto; MFMA kernel: 0 pipelined, 1 pipelined one slab per stage, 2 basic,
+ * 3 pipelined with 256 x 256 workgroup tiles), "ntt_impl" (-1 auto; 0/2/3 radix-8 passes with
+ * 256/512/1024 threads per row, 1 radix-2 stages), "merkle_lg" (0 auto; 6..8 = log2 threads per
+ * Merkle workgroup = tree levels per launch), "profile" (1: time every stage with HIP events on the
+ * context stream). An option left at auto takes the autotuned schedule of the commit's shape when
+ * the process tuning table has one (lgs_autotune, lgs_tune_load), else a built-in default. */
 int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value);
 
 /* Stage times of the last profiled commit, in milliseconds: [0] row NTT, [1] hash main
  * kernel, [2] hash finalize, [3] Merkle tree, [4] whole commit. Up to `n` values are written. */
 int lgs_stage_times(const lgs_ctx* ctx, double* out_ms, uint32_t n);
+
+/* Commit launch schedule: {hash_variant, hash_impl, hash_ksplit, ntt_impl, merkle_lg} (the option
+ * names above). No counterpart in the reference, whose schedule is fixed at compile time
+ * (pcs/src/ligesis.rs:53-59 holds its only constants); results never depend on it. */
+#define LGS_TUNE_FIELDS 5
+
+/* Autotuner: times candidate schedules for the key's shape on a synthetic polynomial (each stage
+ * separately: row-transform variant, hash kernel variant and K-split, Merkle workgroup width;
+ * `reps` timed commits per candidate, 0 = 5) and stores the fastest in the process-wide table,
+ * which every context (server workers included) consults. Writes the chosen schedule to `cfg`
+ * and its measured commit time to `ms` (either may be NULL). */
+int lgs_autotune(lgs_ctx* ctx, const lgs_key* key, uint32_t reps, int32_t cfg[LGS_TUNE_FIELDS], double* ms);
+
+/* The schedule a commit of shape 2^log_m x 2^log_n on this context would use now. */
+int lgs_tune_get(lgs_ctx* ctx, uint32_t log_m, uint32_t log_n, int32_t cfg[LGS_TUNE_FIELDS]);
+
+/* Process tuning table as text, one shape per line:
+ * "log_m log_n hash_variant hash_impl hash_ksplit ntt_impl merkle_lg ms", '#' starts a comment.
+ * Load merges into the table and rejects a malformed file as a whole (LGS_ERR_ARG). `count`
+ * (may be NULL) receives the number of entries written / read. Errors are reported on `ctx`,
+ * which may be NULL (the table needs no device). */
+int lgs_tune_save(lgs_ctx* ctx, const char* path, uint64_t* count);
+int lgs_tune_load(lgs_ctx* ctx, const char* path, uint64_t* count);
+void lgs_tune_clear(void);
 
 /* Setup (ligesis.rs:239): A is the subset-sum matrix, 32 rows x (64 << log_m) canonical
  * uint64, row-major (subset_sum.rs:23-27). Unless LGS_SETUP_COMMIT_ONLY, the shape must pass

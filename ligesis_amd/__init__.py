@@ -73,6 +73,11 @@ def lib():
         L.lgs_commitment_read.argtypes = [vp, u32, vp, u64]
         L.lgs_prove.argtypes = [vp, vp, vp, vp, u64, vp, u64, vp, u64, vp,
                                 ctypes.POINTER(ctypes.c_uint64), vp]
+        L.lgs_autotune.argtypes = [vp, vp, u32, vp, ctypes.POINTER(ctypes.c_double)]
+        L.lgs_tune_get.argtypes = [vp, u32, u32, vp]
+        L.lgs_tune_save.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64)]
+        L.lgs_tune_load.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64)]
+        L.lgs_tune_clear.argtypes = []
         _lib = L
     return _lib
 
@@ -108,6 +113,21 @@ class Context:
         return {"ntt_ms": out[0], "hash_ms": out[1], "hash_finalize_ms": out[2],
                 "merkle_ms": out[3], "commit_ms": out[4]}
 
+    def autotune(self, key: "ProvingKey", reps: int = 5) -> dict:
+        """Time the commit schedules of ``key``'s shape and keep the fastest (``lgs_autotune``)."""
+        cfg = np.zeros(TUNE_FIELDS_COUNT, np.int32)
+        ms = ctypes.c_double()
+        self.check(lib().lgs_autotune(self.h, key.h, reps, _ptr(cfg), ctypes.byref(ms)))
+        out = dict(zip(TUNE_FIELDS, (int(v) for v in cfg)))
+        out["ms"] = ms.value
+        return out
+
+    def tuning(self, log_m: int, log_n: int) -> dict:
+        """Schedule a commit of this shape would use now (``lgs_tune_get``)."""
+        cfg = np.zeros(TUNE_FIELDS_COUNT, np.int32)
+        self.check(lib().lgs_tune_get(self.h, log_m, log_n, _ptr(cfg)))
+        return dict(zip(TUNE_FIELDS, (int(v) for v in cfg)))
+
     def close(self):
         if getattr(self, "h", None):
             lib().lgs_shutdown(self.h)
@@ -115,6 +135,32 @@ class Context:
 
     def __del__(self):
         self.close()
+
+
+TUNE_FIELDS = ("hash_variant", "hash_impl", "hash_ksplit", "ntt_impl", "merkle_lg")
+TUNE_FIELDS_COUNT = len(TUNE_FIELDS)
+
+
+def tune_save(path: str) -> int:
+    """Write the process tuning table (``lgs_tune_save``); returns the entry count."""
+    n = ctypes.c_uint64()
+    rc = lib().lgs_tune_save(None, os.fsencode(path), ctypes.byref(n))
+    if rc != LGS_OK:
+        raise LigesisError(rc, f"cannot write tuning table {path}")
+    return n.value
+
+
+def tune_load(path: str) -> int:
+    """Merge a tuning table file into the process table (``lgs_tune_load``)."""
+    n = ctypes.c_uint64()
+    rc = lib().lgs_tune_load(None, os.fsencode(path), ctypes.byref(n))
+    if rc != LGS_OK:
+        raise LigesisError(rc, f"cannot load tuning table {path} (missing or malformed)")
+    return n.value
+
+
+def tune_clear():
+    lib().lgs_tune_clear()
 
 
 _default_ctx = None

@@ -139,10 +139,12 @@ int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value) {
     ctx->c.hash_variant = (int)value;
   } else if (n == "hash_ksplit" && value >= 0) {
     ctx->c.hash_ksplit = (int)value;
-  } else if (n == "hash_impl" && value >= 0 && value <= 3) {
+  } else if (n == "hash_impl" && value >= -1 && value <= 3) {
     ctx->c.hash_impl = (int)value;
-  } else if (n == "ntt_impl" && value >= 0 && value <= 3) {
+  } else if (n == "ntt_impl" && value >= -1 && value <= 3) {
     ctx->c.ntt_impl = (int)value;
+  } else if (n == "merkle_lg" && (value == 0 || (value >= 6 && value <= 8))) {
+    ctx->c.merkle_lg = (int)value;
   } else if (n == "profile") {
     ctx->c.profile = value != 0;
   } else {
@@ -151,6 +153,55 @@ int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value) {
   }
   return LGS_OK;
 }
+
+static void put_tuning(const lgs::Tuning& t, int32_t* cfg) {
+  if (!cfg) return;
+  cfg[0] = t.hash_variant;
+  cfg[1] = t.hash_impl;
+  cfg[2] = t.hash_ksplit;
+  cfg[3] = t.ntt_impl;
+  cfg[4] = t.merkle_lg;
+}
+
+int lgs_autotune(lgs_ctx* ctx, const lgs_key* key, uint32_t reps, int32_t cfg[LGS_TUNE_FIELDS], double* ms) {
+  if (!ctx || !key) return LGS_ERR_ARG;
+  return guarded(&ctx->c, [&] {
+    (void)hipSetDevice(ctx->c.device);
+    lgs::TuneEntry best;
+    try {
+      lgs::autotune(ctx->c, key->k, reps, best);
+    } catch (...) {
+      (void)hipStreamSynchronize(ctx->c.stream);
+      throw;
+    }
+    put_tuning(best.t, cfg);
+    if (ms) *ms = best.ms;
+  });
+}
+
+int lgs_tune_get(lgs_ctx* ctx, uint32_t log_m, uint32_t log_n, int32_t cfg[LGS_TUNE_FIELDS]) {
+  if (!ctx || !cfg) return LGS_ERR_ARG;
+  put_tuning(ctx->c.resolve(log_m, log_n), cfg);
+  return LGS_OK;
+}
+
+int lgs_tune_save(lgs_ctx* ctx, const char* path, uint64_t* count) {
+  if (!path) return LGS_ERR_ARG;
+  return guarded(ctx ? &ctx->c : nullptr, [&] {
+    const size_t k = lgs::tune_save(path);
+    if (count) *count = k;
+  });
+}
+
+int lgs_tune_load(lgs_ctx* ctx, const char* path, uint64_t* count) {
+  if (!path) return LGS_ERR_ARG;
+  return guarded(ctx ? &ctx->c : nullptr, [&] {
+    const size_t k = lgs::tune_load(path);
+    if (count) *count = k;
+  });
+}
+
+void lgs_tune_clear(void) { lgs::tune_clear(); }
 
 int lgs_stage_times(const lgs_ctx* ctx, double* out_ms, uint32_t n) {
   if (!ctx || !out_ms) return LGS_ERR_ARG;

@@ -89,9 +89,30 @@ static uint32_t pow2_at_least(uint64_t x) {
   return p;
 }
 
-void enqueue_commit(Context& ctx, const Key& key, Commitment& c) {
+Tuning builtin_tuning(uint32_t log_m, uint32_t log_n) {
+  (void)log_m;
+  Tuning t;
+  // row transforms of 2^12 and 2^13 points: 512 threads per row measured fastest at m = 2^12
+  t.ntt_impl = log_n + 1 >= 12 ? 2 : 0;
+  return t;
+}
+
+Tuning Context::resolve(uint32_t log_m, uint32_t log_n) const {
+  Tuning t = builtin_tuning(log_m, log_n);
+  TuneEntry e;
+  if (tune_lookup(log_m, log_n, e)) t = e.t;
+  if (hash_variant) t.hash_variant = hash_variant;
+  if (hash_ksplit) t.hash_ksplit = hash_ksplit;
+  if (hash_impl >= 0) t.hash_impl = hash_impl;
+  if (ntt_impl >= 0) t.ntt_impl = ntt_impl;
+  if (merkle_lg) t.merkle_lg = merkle_lg;
+  return t;
+}
+
+void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* force) {
   const uint64_t m = uint64_t(1) << key.log_m, n = uint64_t(1) << key.log_n, two_n = 2 * n;
   const int logN = key.log_n + 1;
+  const Tuning tu = force ? *force : ctx.resolve(key.log_m, key.log_n);
   c.log_m = key.log_m;
   c.log_n = key.log_n;
   c.enc = DevBuf<uint64_t>(&ctx.pool, m * two_n);
@@ -101,25 +122,25 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c) {
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[0], ctx.stream), "event");
   // F'[i] = RS(F[i]) (ligesis.rs:289-291)
   hip_check(launch_ntt(c.f, n, 1, (uint32_t)n, (uint32_t)m, logN, ctx.twiddle(logN), c.enc.ptr,
-                       ctx.stream, ctx.ntt_impl),
+                       ctx.stream, tu.ntt_impl),
             "row encode");
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[1], ctx.stream), "event");
   // H = A . B with B read bitwise from F' (ligesis.rs:292-302)
   bool mfma_ok = (two_n % 128 == 0) && m >= 2;
-  int variant = ctx.hash_variant == 0 ? (mfma_ok ? 1 : 2) : ctx.hash_variant;
+  int variant = tu.hash_variant == 0 ? (mfma_ok ? 1 : 2) : tu.hash_variant;
   if (variant == 1 && !mfma_ok) throw Error(1, "MFMA hash needs 2n % 128 == 0 and m >= 2");
   if (variant == 1) {
     uint64_t ntiles = two_n / 128;
-    uint32_t ks = ctx.hash_ksplit ? (uint32_t)ctx.hash_ksplit : pow2_at_least((512 + ntiles - 1) / ntiles);
+    uint32_t ks = tu.hash_ksplit ? (uint32_t)tu.hash_ksplit : pow2_at_least((512 + ntiles - 1) / ntiles);
     if (ks > m / 2) ks = (uint32_t)(m / 2);
     if (ks == 0 || (m % (2 * ks)) != 0) throw Error(1, "hash_ksplit must divide m / 2");
     DevBuf<int32_t> part(&ctx.pool, (size_t)ks * 256 * two_n);
     hip_check(launch_hash_mfma(c.enc.ptr, two_n, (uint32_t)m, (uint32_t)two_n, key.dig.ptr, ks, part.ptr,
-                               c.h.ptr, ctx.stream, ctx.profile ? ctx.ev[2] : nullptr, ctx.hash_impl),
+                               c.h.ptr, ctx.stream, ctx.profile ? ctx.ev[2] : nullptr, tu.hash_impl),
               "hash (MFMA)");
   } else {
     uint64_t nb = (two_n + 255) / 256;
-    uint32_t ks = ctx.hash_ksplit ? (uint32_t)ctx.hash_ksplit : pow2_at_least((1024 + nb - 1) / nb);
+    uint32_t ks = tu.hash_ksplit ? (uint32_t)tu.hash_ksplit : pow2_at_least((1024 + nb - 1) / nb);
     if (ks > m) ks = (uint32_t)m;
     if (m % ks) throw Error(1, "hash_ksplit must divide m");
     DevBuf<uint64_t> part(&ctx.pool, (size_t)ks * 32 * two_n);
@@ -129,7 +150,7 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c) {
   }
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[3], ctx.stream), "event");
   // column-wise Merkle tree over serialize(H[:, j]) (ligesis.rs:306-313)
-  hip_check(launch_merkle(c.h.ptr, 32, two_n, 1, two_n, c.tree.nodes.ptr, ctx.stream), "merkle");
+  hip_check(launch_merkle(c.h.ptr, 32, two_n, 1, two_n, c.tree.nodes.ptr, ctx.stream, tu.merkle_lg), "merkle");
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[4], ctx.stream), "event");
 }
 
@@ -164,7 +185,8 @@ void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, 
     nblocks += (uint32_t)m;
     max_logN = logN > max_logN ? logN : max_logN;
     const uint32_t ntiles = (uint32_t)(two_n / 128);
-    uint32_t ks = ctx.hash_ksplit ? (uint32_t)ctx.hash_ksplit : (uint32_t)(m >= 1024 ? m / 512 : 1);
+    const Tuning tu = ctx.resolve(key.log_m, key.log_n);
+    uint32_t ks = tu.hash_ksplit ? (uint32_t)tu.hash_ksplit : (uint32_t)(m >= 1024 ? m / 512 : 1);
     if (ks > m / 2) ks = (uint32_t)(m / 2);
     if (ks == 0 || m % (2 * ks)) throw Error(1, "hash_ksplit must divide m / 2");
     hold.parts.emplace_back(&ctx.pool, (size_t)ks * 256 * two_n);

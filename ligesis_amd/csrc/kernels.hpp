@@ -47,6 +47,7 @@ struct MerkleJob {
 
 // ntt.hip
 hipError_t launch_twiddles(uint64_t* tw, int logN, hipStream_t st);
+hipError_t launch_fill_field(uint64_t* p, uint64_t n, uint64_t seed, hipStream_t st);
 // impl 0 / 2 / 3: radix-8 passes with the whole row in LDS when 2^logN <= 2^LGS_LDS_LOG, with 256 /
 // 512 / 1024 threads per row; larger rows run radix-8 sub-blocks then global radix-2 passes.
 // impl 1: radix-2 stages, the first LGS_LDS_LOG of them in LDS and the rest as global passes.
@@ -69,7 +70,8 @@ hipError_t launch_hash_group(const HashJob* jobs, int njobs, uint32_t total_tile
 hipError_t launch_merkle_group(const MerkleJob* jobs, int njobs, uint32_t total_blocks, hipStream_t st);
 
 // merkle.hip: tree over nleaves (a power of two) leaves, leaf p = M[r*rs + p*cs], r < rows.
+// lg_block: log2 threads per workgroup (6..8) = levels reduced per launch.
 hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs,
-                         uint64_t nleaves, uint32_t* nodes, hipStream_t st);
+                         uint64_t nleaves, uint32_t* nodes, hipStream_t st, int lg_block = 8);
 
 }  // namespace lgs

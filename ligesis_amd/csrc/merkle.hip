@@ -5,8 +5,9 @@
 // concatenated, leaves first: layer k starts at sum_{t<k} L >> t, the root is the last node.
 // Leaf counts are powers of two on this path (2n, Basefold domains, folded levels).
 //
-// One workgroup reduces up to 256 nodes of a layer by up to 8 levels in LDS, so a tree over L
-// leaves takes ceil(log2(L) / 8) launches; the first launch also hashes the leaves.
+// One workgroup of 2^g threads (g = 6..8, 8 in grouped launches) reduces up to 2^g nodes of a
+// layer by up to g levels in LDS, so a tree over L leaves takes ceil(log2(L) / g) launches; the
+// first launch also hashes the leaves.
 #include "blake3_dev.hpp"
 #include "kernels.hpp"
 
@@ -35,9 +36,9 @@ __device__ __forceinline__ void merkle_reduce_body(const uint64_t* __restrict__ 
                                                    uint64_t cs, int from_leaves, uint32_t* __restrict__ nodes,
                                                    uint64_t child_off, uint64_t child_count, int levels,
                                                    uint32_t blk, uint32_t* sh) {
-  const uint32_t t = threadIdx.x;
-  const uint64_t base = (uint64_t)blk * 256;
-  uint32_t local_n = (uint32_t)((child_count - base) < 256 ? (child_count - base) : 256);
+  const uint32_t t = threadIdx.x, width = blockDim.x;
+  const uint64_t base = (uint64_t)blk * width;
+  uint32_t local_n = (uint32_t)((child_count - base) < width ? (child_count - base) : width);
   if (t < local_n) {
     uint32_t cv[8];
     if (from_leaves) {
@@ -106,15 +107,17 @@ static int ilog2_u64(uint64_t x) {
 }
 
 hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs,
-                         uint64_t nleaves, uint32_t* nodes, hipStream_t st) {
+                         uint64_t nleaves, uint32_t* nodes, hipStream_t st, int lg_block) {
+  if (lg_block < 6 || lg_block > 8) return hipErrorInvalidValue;
+  const uint32_t width = 1u << lg_block;
   uint64_t off = 0, count = nleaves;
   int total_levels = ilog2_u64(nleaves);
   int done = 0;
   bool first = true;
   while (first || count > 1) {
-    int lv = total_levels - done < 8 ? total_levels - done : 8;
-    uint32_t blocks = (uint32_t)((count + 255) / 256);
-    hipLaunchKernelGGL(merkle_reduce_kernel, dim3(blocks), dim3(256), 0, st, M, rows, rs, cs,
+    int lv = total_levels - done < lg_block ? total_levels - done : lg_block;
+    uint32_t blocks = (uint32_t)((count + width - 1) / width);
+    hipLaunchKernelGGL(merkle_reduce_kernel, dim3(blocks), dim3(width), 0, st, M, rows, rs, cs,
                        first ? 1 : 0, nodes, off, count, lv);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

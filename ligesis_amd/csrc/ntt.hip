@@ -315,6 +315,22 @@ hipError_t launch_twiddles(uint64_t* tw, int logN, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Canonical field elements from SplitMix64 of (seed + index): timing input for the autotuner.
+__global__ void fill_field_kernel(uint64_t* __restrict__ p, uint64_t n, uint64_t seed) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t z = seed + (i + 1) * 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    p[i] = gl_canon(z);
+  }
+}
+
+hipError_t launch_fill_field(uint64_t* p, uint64_t n, uint64_t seed, hipStream_t st) {
+  hipLaunchKernelGGL(fill_field_kernel, dim3(1024), dim3(256), 0, st, p, n, seed);
+  return hipGetLastError();
+}
+
 hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_in, uint32_t rows,
                       int logN, const uint64_t* tw, uint64_t* out, hipStream_t st, int impl) {
   if (rows == 0) return hipSuccess;

@@ -134,6 +134,28 @@ struct DevTree {
   }
 };
 
+// Launch schedule of one commit shape. Every choice yields the same bytes; only speed differs.
+struct Tuning {
+  int hash_variant = 0;  // 1 MFMA, 2 VALU (0: MFMA where eligible)
+  int hash_impl = 0;     // MFMA kernel variant
+  int hash_ksplit = 0;   // slabs of the K (= 64 m) dimension; 0: heuristic
+  int ntt_impl = 0;      // row transform variant
+  int merkle_lg = 8;     // log2 threads per Merkle workgroup
+};
+
+// Process-wide table of measured schedules, keyed by (log_m, log_n) (tune.cpp). Shared by every
+// context, so server workers use what an autotune run on another context found.
+struct TuneEntry {
+  Tuning t;
+  double ms = 0;  // commit time measured with this schedule
+};
+void tune_store(uint32_t log_m, uint32_t log_n, const TuneEntry& e);
+bool tune_lookup(uint32_t log_m, uint32_t log_n, TuneEntry& e);
+void tune_clear();
+size_t tune_save(const char* path);  // entries written
+size_t tune_load(const char* path);  // entries read; throws on a malformed file
+Tuning builtin_tuning(uint32_t log_m, uint32_t log_n);
+
 // Basefold prover state on the GPU (pcs/src/basefold.rs:36-54): interleaved codewords
 // (rows of length D; extension-field polys use two rows per codeword, c0 then c1), their
 // column Merkle tree, and the polynomial itself.
@@ -159,10 +181,13 @@ struct Context {
   std::string err;
   DevicePool pool;
   std::map<int, DevBuf<uint64_t>> twiddles;
+  // Explicit launch-schedule options (lgs_set_option); unset fields fall back to the process
+  // tuning table, then to the built-in defaults (see resolve).
   int hash_variant = 0;  // 0 auto, 1 MFMA, 2 VALU
-  int hash_ksplit = 0;
-  int hash_impl = 0;  // MFMA kernel: 0 pipelined (2 slabs per stage), 1 pipelined (1), 2 basic
-  int ntt_impl = 0;   // row transform: 0 radix-8 passes, 1 radix-2 stages
+  int hash_ksplit = 0;   // 0 auto
+  int hash_impl = -1;    // -1 auto; MFMA kernel: 0 pipelined 2x2 waves, 1 one slab per stage, 2 basic, 3 2x4 waves
+  int ntt_impl = -1;     // -1 auto; row transform: 0/2/3 radix-8 with 256/512/1024 threads, 1 radix-2
+  int merkle_lg = 0;     // 0 auto; log2 threads per Merkle workgroup = levels per launch (6..8)
   bool profile = false;
   double stage_ms[8] = {0};
   hipEvent_t ev[8] = {};
@@ -171,6 +196,8 @@ struct Context {
 
   const uint64_t* twiddle(int logN);
   uint8_t* staging(size_t bytes);
+  // launch schedule of a commit of shape 2^log_m x 2^log_n
+  Tuning resolve(uint32_t log_m, uint32_t log_n) const;
 };
 
 struct Key {
@@ -207,14 +234,17 @@ struct GroupHold {
   std::vector<std::vector<MerkleJob>> host_merkle;
 };
 
-// commit.cpp
-void enqueue_commit(Context& ctx, const Key& key, Commitment& c);
+// commit.cpp; `force` replaces the resolved schedule (autotune candidates)
+void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* force = nullptr);
 bool group_eligible(const Key& key);
 void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, Commitment*>>& items,
                           GroupHold& hold);
 void enqueue_basefold_commit(Context& ctx, DevBasefold& st, const uint64_t* poly, bool ext,
                              uint32_t num_vars, uint32_t li, uint32_t code_rate);
 void setup_key(Context& ctx, Key& key, const uint64_t* host_a, bool commit_a);
+// tune.cpp: times candidate schedules of key's shape on a synthetic polynomial and stores the
+// fastest in the process table; returns the best commit time in ms
+double autotune(Context& ctx, const Key& key, uint32_t reps, TuneEntry& best);
 // prove.cpp
 int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uint64_t* z, uint64_t z_len,
                    const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,

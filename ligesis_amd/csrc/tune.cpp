@@ -1,0 +1,221 @@
+// Autotuner for the commit launch schedule: row-transform variant, hash kernel variant and
+// K-split, and Merkle workgroup width, per (log_m, log_n). The reference fixes its schedule at
+// compile time (pcs/src/ligesis.rs:53-59 holds only the protocol constants); every schedule here
+// produces the same bytes, so the table only trades speed.
+#include <cstdio>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+
+#include "runtime.hpp"
+
+namespace lgs {
+
+namespace {
+std::mutex g_mu;
+std::map<std::pair<uint32_t, uint32_t>, TuneEntry> g_table;
+
+uint32_t pow2_ceil(uint64_t x) {
+  uint32_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+struct ProfileOn {
+  Context& ctx;
+  bool was;
+  explicit ProfileOn(Context& c) : ctx(c), was(c.profile) { c.profile = true; }
+  ~ProfileOn() { ctx.profile = was; }
+};
+
+struct Timing {
+  double ntt = 0, hash = 0, merkle = 0, total = 0;
+};
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, a, b), "tune event time");
+  return ms;
+}
+}  // namespace
+
+void tune_store(uint32_t log_m, uint32_t log_n, const TuneEntry& e) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_table[{log_m, log_n}] = e;
+}
+
+bool tune_lookup(uint32_t log_m, uint32_t log_n, TuneEntry& e) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_table.find({log_m, log_n});
+  if (it == g_table.end()) return false;
+  e = it->second;
+  return true;
+}
+
+void tune_clear() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_table.clear();
+}
+
+static const char* kHeader = "# ligesis_amd commit schedules v1: log_m log_n hash_variant hash_impl hash_ksplit ntt_impl merkle_lg ms";
+
+size_t tune_save(const char* path) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  std::ofstream out(path);
+  if (!out) throw Error(1, std::string("tune: cannot write ") + path);
+  out << kHeader << "\n";
+  for (auto& kv : g_table) {
+    const Tuning& t = kv.second.t;
+    char line[160];
+    std::snprintf(line, sizeof line, "%u %u %d %d %d %d %d %.6f\n", kv.first.first, kv.first.second, t.hash_variant,
+                  t.hash_impl, t.hash_ksplit, t.ntt_impl, t.merkle_lg, kv.second.ms);
+    out << line;
+  }
+  if (!out) throw Error(1, std::string("tune: write failed for ") + path);
+  return g_table.size();
+}
+
+size_t tune_load(const char* path) {
+  std::ifstream in(path);
+  if (!in) throw Error(1, std::string("tune: cannot read ") + path);
+  std::map<std::pair<uint32_t, uint32_t>, TuneEntry> got;
+  std::string line;
+  int lineno = 0;
+  while (std::getline(in, line)) {
+    lineno++;
+    size_t p = line.find_first_not_of(" \t\r");
+    if (p == std::string::npos || line[p] == '#') continue;
+    std::istringstream ss(line);
+    long lm, ln;
+    TuneEntry e;
+    std::string extra;
+    if (!(ss >> lm >> ln >> e.t.hash_variant >> e.t.hash_impl >> e.t.hash_ksplit >> e.t.ntt_impl >> e.t.merkle_lg >>
+          e.ms) ||
+        (ss >> extra))
+      throw Error(1, "tune: malformed line " + std::to_string(lineno) + " in " + path);
+    const Tuning& t = e.t;
+    const bool ok = lm >= 0 && lm <= 20 && ln >= 1 && ln <= 26 && t.hash_variant >= 0 && t.hash_variant <= 2 &&
+                    t.hash_impl >= 0 && t.hash_impl <= 3 && t.hash_ksplit >= 0 &&
+                    (t.hash_ksplit & (t.hash_ksplit - 1)) == 0 && t.ntt_impl >= 0 && t.ntt_impl <= 3 &&
+                    t.merkle_lg >= 6 && t.merkle_lg <= 8;
+    if (!ok) throw Error(1, "tune: value out of range on line " + std::to_string(lineno) + " in " + path);
+    got[{(uint32_t)lm, (uint32_t)ln}] = e;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto& kv : got) g_table[kv.first] = kv.second;
+  return got.size();
+}
+
+double autotune(Context& ctx, const Key& key, uint32_t reps, TuneEntry& best) {
+  if (reps == 0) reps = 5;
+  const uint64_t m = uint64_t(1) << key.log_m, n = uint64_t(1) << key.log_n, two_n = 2 * n;
+  const int logN = key.log_n + 1;
+  DevBuf<uint64_t> f(&ctx.pool, m * n);
+  hip_check(launch_fill_field(f.ptr, m * n, 0x243f6a8885a308d3ull, ctx.stream), "tune input");
+  ProfileOn prof(ctx);
+
+  auto run = [&](const Tuning& t) {
+    Timing acc;
+    for (uint32_t r = 0; r <= reps; r++) {  // r == 0 warms up (twiddles, pool)
+      Commitment c;
+      c.ctx = &ctx;
+      c.key = &key;
+      c.f = f.ptr;
+      enqueue_commit(ctx, key, c, &t);
+      hip_check(hipEventSynchronize(ctx.ev[4]), "tune sync");
+      if (r == 0) continue;
+      acc.ntt += elapsed(ctx.ev[0], ctx.ev[1]);
+      acc.hash += elapsed(ctx.ev[1], ctx.ev[3]);
+      acc.merkle += elapsed(ctx.ev[3], ctx.ev[4]);
+      acc.total += elapsed(ctx.ev[0], ctx.ev[4]);
+    }
+    acc.ntt /= reps;
+    acc.hash /= reps;
+    acc.merkle /= reps;
+    acc.total /= reps;
+    return acc;
+  };
+
+  Tuning cur = builtin_tuning(key.log_m, key.log_n);
+  // stage 1: row transform
+  {
+    std::vector<int> cand = {0, 1};
+    if (logN <= LGS_LDS_LOG) {
+      const uint64_t per_row = (uint64_t(1) << logN) / 8;  // radix-8 butterflies per pass
+      if (per_row >= 512) cand.push_back(2);
+      if (per_row >= 1024) cand.push_back(3);
+    }
+    double bt = 1e30;
+    int bi = cur.ntt_impl;
+    for (int c : cand) {
+      Tuning t = cur;
+      t.ntt_impl = c;
+      const double ms = run(t).ntt;
+      if (ms < bt) {
+        bt = ms;
+        bi = c;
+      }
+    }
+    cur.ntt_impl = bi;
+  }
+  // stage 2: hash (+ finalize)
+  {
+    std::vector<Tuning> cand;
+    const bool mfma_ok = two_n % 128 == 0 && m >= 2;
+    if (mfma_ok) {
+      const uint64_t ntiles = two_n / 128;
+      uint32_t h = pow2_ceil((512 + ntiles - 1) / ntiles);
+      if (h > m / 2) h = (uint32_t)(m / 2);
+      const int impls[3] = {0, 1, 3};
+      for (int impl : impls) {
+        if (impl == 3 && two_n % 256) continue;
+        for (uint32_t ks : {h / 2, h, 2 * h}) {
+          if (ks == 0 || ks > m / 2 || m % (2 * ks)) continue;
+          Tuning t = cur;
+          t.hash_variant = 1;
+          t.hash_impl = impl;
+          t.hash_ksplit = (int)ks;
+          cand.push_back(t);
+        }
+      }
+    }
+    {
+      Tuning t = cur;
+      t.hash_variant = 2;
+      t.hash_impl = 0;
+      t.hash_ksplit = 0;
+      cand.push_back(t);
+    }
+    double bt = 1e30;
+    Tuning bestt = cand.front();
+    for (const Tuning& t : cand) {
+      const double ms = run(t).hash;
+      if (ms < bt) {
+        bt = ms;
+        bestt = t;
+      }
+    }
+    cur = bestt;
+  }
+  // stage 3: Merkle workgroup width
+  {
+    double bt = 1e30;
+    int bl = cur.merkle_lg;
+    for (int lg = 6; lg <= 8; lg++) {
+      Tuning t = cur;
+      t.merkle_lg = lg;
+      const double ms = run(t).merkle;
+      if (ms < bt) {
+        bt = ms;
+        bl = lg;
+      }
+    }
+    cur.merkle_lg = bl;
+  }
+  best.t = cur;
+  best.ms = run(cur).total;
+  tune_store(key.log_m, key.log_n, best);
+  return best.ms;
+}
+
+}  // namespace lgs

@@ -1,0 +1,74 @@
+"""Autotuner on the GPU: the schedule it picks is valid, is what later commits use, survives a
+save / load round trip, and leaves every output byte unchanged (root, F', H against the oracle)."""
+import numpy as np
+import pytest
+
+import ligesis_amd
+import oracle
+from ligesis_amd import Ligesis, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _clean_table():
+    ligesis_amd.tune_clear()
+    yield
+    ligesis_amd.tune_clear()
+
+
+@pytest.mark.parametrize("shape", [(5, 9), (2, 6), (3, 12)])
+def test_autotuned_commit_matches_oracle(gpu_ctx, shape, tmp_path):
+    log_m, log_n = shape
+    print(f"autotune shape={shape}", flush=True)
+    inp = synth.Inputs(log_m, log_n, seed=40 + log_m)
+    pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx, commit_only=True)
+    best = gpu_ctx.autotune(pk, reps=2)
+    assert best["ms"] > 0
+    assert best["hash_variant"] in (1, 2) and 0 <= best["hash_impl"] <= 3
+    assert 0 <= best["ntt_impl"] <= 3 and 6 <= best["merkle_lg"] <= 8
+    cfg = gpu_ctx.tuning(log_m, log_n)
+    assert cfg == {k: best[k] for k in ligesis_amd.TUNE_FIELDS}
+    root, cd = Ligesis.commit(pk, inp.f)
+    opk = oracle.ProvingKey(log_m, log_n, inp.a, commit_a=False)
+    ocd = oracle.CommitData(opk, inp.f, keep_enc=True)
+    assert root == ocd.root
+    np.testing.assert_array_equal(cd.encoded(), ocd.enc())
+    np.testing.assert_array_equal(cd.digests(), ocd.h())
+    # the table round-trips through a file
+    path = str(tmp_path / "tune.txt")
+    assert ligesis_amd.tune_save(path) == 1
+    ligesis_amd.tune_clear()
+    assert ligesis_amd.tune_load(path) == 1
+    assert gpu_ctx.tuning(log_m, log_n) == cfg
+
+
+def test_explicit_option_overrides_table(gpu_ctx):
+    inp = synth.Inputs(4, 8, seed=5)
+    pk, _ = Ligesis.setup(4, 8, inp.a, ctx=gpu_ctx, commit_only=True)
+    gpu_ctx.autotune(pk, reps=1)
+    try:
+        gpu_ctx.set_option("ntt_impl", 1)
+        gpu_ctx.set_option("merkle_lg", 6)
+        cfg = gpu_ctx.tuning(4, 8)
+        assert cfg["ntt_impl"] == 1 and cfg["merkle_lg"] == 6
+        root, _ = Ligesis.commit(pk, inp.f)
+    finally:
+        gpu_ctx.set_option("ntt_impl", -1)
+        gpu_ctx.set_option("merkle_lg", 0)
+    root2, _ = Ligesis.commit(pk, inp.f)
+    assert root == root2
+
+
+@pytest.mark.parametrize("lg", [6, 7, 8])
+def test_merkle_widths_agree(gpu_ctx, lg):
+    """Every Merkle workgroup width gives the oracle's root (2n = 2^11 leaves: 2 launches at 6)."""
+    inp = synth.Inputs(3, 10, seed=9)
+    pk, _ = Ligesis.setup(3, 10, inp.a, ctx=gpu_ctx, commit_only=True)
+    gpu_ctx.set_option("merkle_lg", lg)
+    try:
+        root, _ = Ligesis.commit(pk, inp.f)
+    finally:
+        gpu_ctx.set_option("merkle_lg", 0)
+    opk = oracle.ProvingKey(3, 10, inp.a, commit_a=False)
+    assert root == oracle.CommitData(opk, inp.f).root
