@@ -62,7 +62,8 @@ const char* lgs_last_error(const lgs_ctx* ctx);
 
 /* Tuning knobs that never change results: "hash" (0 auto, 1 MFMA, 2 VALU), "hash_ksplit"
  * (0 auto), "hash_impl" (-1 auto; MFMA kernel: 0 pipelined, 1 pipelined one slab per stage, 2 basic,
- * 3 pipelined with 256 x 256 workgroup tiles), "ntt_impl" (-1 auto; 0/2/3 radix-8 passes with
+ * 3 pipelined with 256 x 256 workgroup tiles, 4 global loads issued 2 slabs ahead, 5 as 4 with
+ * 256 x 256 tiles), "ntt_impl" (-1 auto; 0/2/3 radix-8 passes with
  * 256/512/1024 threads per row, 1 radix-2 stages), "merkle_lg" (0 auto; 6..8 = log2 threads per
  * Merkle workgroup = tree levels per launch), "profile" (1: time every stage with HIP events on the
  * context stream). An option left at auto takes the autotuned schedule of the commit's shape when

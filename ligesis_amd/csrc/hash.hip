@@ -278,6 +278,101 @@ __device__ __forceinline__ void hash_pipe_body(const uint64_t* __restrict__ enc,
   }
 }
 
+// Same tile as hash_pipe_body (one slab per stage), with the global loads issued D stages ahead
+// into a ring of D register slots, so D - 1 stages of MFMA work cover HBM latency instead of
+// one. Slot u of an unrolled group holds stage st0 + u: its B words are expanded first, then the
+// slot is refilled with stage st0 + u + D (clamped to the last slab: those loads are unused).
+// A(st + 1) moves from its slot into the other LDS buffer after the stage's MFMAs.
+// Needs slabs_per_split % D == 0.
+template <int WN, int D>
+__global__ __launch_bounds__(128 * WN, 2) void hash_mfma_ring_kernel(const uint64_t* __restrict__ enc,
+                                                                     uint64_t ld, const v4i* __restrict__ adig,
+                                                                     uint32_t slabs_per_split, uint32_t ntiles,
+                                                                     uint32_t ncols, int32_t* __restrict__ part) {
+  constexpr int T = 128 * WN;
+  constexpr int NA = 1024 / T;
+  __shared__ v4i lds_a[2][1024];
+  const uint32_t logical = xcd_logical_block();
+  const uint32_t split = logical / ntiles, ntile = logical % ntiles;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t wm = wave / WN, wn = wave % WN;
+  const uint32_t g = lane >> 4, c = lane & 15;
+  const uint32_t kb = 32 * (g >> 1) + 4 * (g & 1);
+  const uint64_t col_base = (uint64_t)ntile * (64 * WN) + wn * 64 + c;
+
+  v4i acc[8][4];
+#pragma unroll
+  for (int a = 0; a < 8; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = v4i{0, 0, 0, 0};
+
+  const uint32_t ns = slabs_per_split;
+  const uint64_t slab0 = (uint64_t)split * ns;
+  v4i ra[D][NA];
+  uint64_t rb[D][4];
+#pragma unroll
+  for (int u = 0; u < D; u++) {
+    const uint64_t sl = slab0 + (u < (int)ns ? u : ns - 1);
+#pragma unroll
+    for (int q = 0; q < NA; q++) ra[u][q] = adig[sl * 1024 + q * T + threadIdx.x];
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) rb[u][nt] = enc[sl * ld + col_base + nt * 16];
+  }
+#pragma unroll
+  for (int q = 0; q < NA; q++) lds_a[0][q * T + threadIdx.x] = ra[0][q];
+  __syncthreads();
+
+  for (uint32_t st0 = 0; st0 < ns; st0 += D) {
+#pragma unroll
+    for (int u = 0; u < D; u++) {
+      const uint32_t st = st0 + u;
+      v4i b[4];
+#pragma unroll
+      for (int nt = 0; nt < 4; nt++) {
+        const uint32_t y = (uint32_t)(rb[u][nt] >> kb);
+        b[nt] = v4i{(int)(y & 0x01010101u), (int)((y >> 1) & 0x01010101u), (int)((y >> 2) & 0x01010101u),
+                    (int)((y >> 3) & 0x01010101u)};
+      }
+      {
+        const uint32_t nx = st + D;
+        const uint64_t sl = slab0 + (nx < ns ? nx : ns - 1);
+#pragma unroll
+        for (int q = 0; q < NA; q++) ra[u][q] = adig[sl * 1024 + q * T + threadIdx.x];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) rb[u][nt] = enc[sl * ld + col_base + nt * 16];
+        // keep the refill here: without a scheduling barrier the compiler sinks these loads to
+        // the loop latch, next to their first use, and the ring hides no latency
+        asm volatile("" ::: "memory");
+      }
+      const uint32_t buf = st & 1;
+#pragma unroll
+      for (int mt = 0; mt < 8; mt++) {
+        const v4i a = lds_a[buf][(wm * 8 + mt) * 64 + lane];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+          acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[nt], acc[mt][nt], 0, 0, 0);
+      }
+      if (st + 1 < ns) {
+#pragma unroll
+        for (int q = 0; q < NA; q++) lds_a[buf ^ 1][q * T + threadIdx.x] = ra[(u + 1) % D][q];
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < 8; mt++) {
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const uint32_t m = (wm * 8 + mt) * 16 + 4 * g + v;
+        const uint64_t col = (uint64_t)ntile * (64 * WN) + wn * 64 + nt * 16 + c;
+        part[((uint64_t)split * 256 + m) * ncols + col] = acc[mt][nt][v];
+      }
+    }
+  }
+}
+
 // H[r][j] = sum_l 256^l (sum_split part[split][32 l + r][j]) mod p.
 __global__ void hash_mfma_finalize_kernel(const int32_t* __restrict__ part, uint32_t ksplit,
                                           uint32_t ncols, uint64_t* __restrict__ h) {
@@ -357,7 +452,20 @@ hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
                             hipStream_t st, hipEvent_t mid, int impl) {
   uint32_t ntiles = ncols / HM_NT;
   uint32_t sps = m_rows / ksplit;
-  if (impl == 3 && ncols % 256 == 0)
+  if (impl == 4 || impl == 5) {
+    const int wn = impl == 5 ? 4 : 2;
+    if (sps % 2 == 0 && ncols % (64 * wn) == 0) {
+      if (impl == 4)
+        hipLaunchKernelGGL((hash_mfma_ring_kernel<2, 2>), dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
+                           (const v4i*)dig, sps, ntiles, ncols, part);
+      else
+        hipLaunchKernelGGL((hash_mfma_ring_kernel<4, 2>), dim3((ncols / 256) * ksplit), dim3(512), 0, st, enc, ld,
+                           (const v4i*)dig, sps, ncols / 256, ncols, part);
+    } else {
+      hipLaunchKernelGGL((hash_mfma_pipe_kernel<1, 2>), dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
+                         (const v4i*)dig, sps, ntiles, ncols, part);
+    }
+  } else if (impl == 3 && ncols % 256 == 0)
     hipLaunchKernelGGL((hash_mfma_pipe_kernel<2, 4>), dim3((ncols / 256) * ksplit), dim3(512), 0, st, enc, ld,
                        (const v4i*)dig, sps, ncols / 256, ncols, part);
   else if (impl == 2)

@@ -57,6 +57,33 @@ def test_mfma_and_valu_hash_agree(gpu_ctx):
         np.testing.assert_array_equal(v[1], ref[1])
 
 
+@pytest.mark.parametrize("shape", [(5, 9), (3, 12), (1, 7)])
+def test_every_mfma_kernel_matches_oracle(gpu_ctx, shape):
+    """All MFMA hash kernels (pipelined, basic, 256-wide tiles, load rings) and K-splits give the
+    oracle's H; shapes where a kernel's tiling does not apply take its fallback."""
+    log_m, log_n = shape
+    inp = synth.Inputs(log_m, log_n, seed=21 + log_n)
+    pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx, commit_only=True)
+    ocd = oracle.CommitData(oracle.ProvingKey(log_m, log_n, inp.a, commit_a=False), inp.f, keep_enc=True)
+    want = ocd.h()
+    gpu_ctx.set_option("hash", 1)
+    try:
+        for impl in range(6):
+            for ks in (0, 1, 2, 4):
+                if ks > (1 << log_m) // 2:
+                    continue
+                print(f"hash impl={impl} ks={ks} shape={shape}", flush=True)
+                gpu_ctx.set_option("hash_impl", impl)
+                gpu_ctx.set_option("hash_ksplit", ks)
+                root, cd = Ligesis.commit(pk, inp.f)
+                assert root == ocd.root, (impl, ks)
+                np.testing.assert_array_equal(cd.digests(), want)
+    finally:
+        gpu_ctx.set_option("hash", 0)
+        gpu_ctx.set_option("hash_impl", -1)
+        gpu_ctx.set_option("hash_ksplit", 0)
+
+
 def test_extreme_field_values(gpu_ctx):
     """All-(p-1) and all-zero rows and an A of p-1 entries (largest digits / carries)."""
     log_m, log_n = 2, 7
