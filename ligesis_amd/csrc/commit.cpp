@@ -154,6 +154,57 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[4], ctx.stream), "event");
 }
 
+void enqueue_merkle_trees(Context& ctx, const std::vector<MerkleSource>& trees,
+                          std::vector<std::vector<MerkleJob>>& host_jobs, std::vector<DevBuf<uint64_t>>& tables) {
+  // rounds of up to 8 levels; every tree takes part until its root is written
+  const size_t nt = trees.size();
+  std::vector<uint64_t> off(nt, 0), count(nt);
+  std::vector<int> left(nt);
+  for (size_t i = 0; i < nt; i++) {
+    count[i] = trees[i].leaves;
+    int l = 0;
+    while ((uint64_t(1) << l) < count[i]) l++;
+    left[i] = l;
+  }
+  std::vector<std::vector<MerkleJob>> rounds;
+  std::vector<uint32_t> blocks;
+  for (bool first = true;; first = false) {
+    std::vector<MerkleJob> mj;
+    uint32_t mblocks = 0;
+    for (size_t i = 0; i < nt; i++) {
+      if (!first && left[i] == 0) continue;
+      const int lv = left[i] < 8 ? left[i] : 8;
+      const MerkleSource& t = trees[i];
+      mj.push_back(MerkleJob{t.M, t.rows, t.rs, t.cs, first ? 1 : 0, t.nodes, off[i], count[i], lv, mblocks});
+      mblocks += (uint32_t)((count[i] + 255) / 256);
+      for (int q = 0; q < lv; q++) {
+        off[i] += count[i];
+        count[i] >>= 1;
+      }
+      left[i] -= lv;
+    }
+    if (mj.empty()) break;
+    rounds.push_back(std::move(mj));
+    blocks.push_back(mblocks);
+  }
+  // one job table for all rounds, one upload
+  std::vector<MerkleJob> all;
+  std::vector<size_t> at;
+  for (auto& r : rounds) {
+    at.push_back(all.size());
+    all.insert(all.end(), r.begin(), r.end());
+  }
+  if (all.empty()) return;
+  host_jobs.push_back(std::move(all));
+  const std::vector<MerkleJob>& h = host_jobs.back();
+  tables.emplace_back(&ctx.pool, (h.size() * sizeof(MerkleJob) + 7) / 8);
+  const MerkleJob* d = (const MerkleJob*)tables.back().ptr;
+  hip_check(hipMemcpyAsync((void*)d, h.data(), h.size() * sizeof(MerkleJob), hipMemcpyHostToDevice, ctx.stream),
+            "merkle jobs");
+  for (size_t r = 0; r < rounds.size(); r++)
+    hip_check(launch_merkle_group(d + at[r], (int)rounds[r].size(), blocks[r], ctx.stream), "grouped merkle");
+}
+
 // Batched commit: every stage of every eligible commitment runs in one launch per stage (row
 // transforms, MFMA hash tiles, finalize, and one launch per Merkle reduction round), driven by
 // job tables in device memory. Results are identical to enqueue_commit per polynomial.
@@ -212,38 +263,12 @@ void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, 
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[2], ctx.stream), "event");
   hip_check(launch_hash_group(nullptr, njobs, 0, d_fj, fblocks, ctx.stream), "batched finalize");
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[3], ctx.stream), "event");
-  // Merkle: rounds of up to 8 levels; every tree takes part until its root is written
-  std::vector<uint64_t> off(items.size(), 0), count(items.size());
-  std::vector<int> left(items.size());
-  for (size_t i = 0; i < items.size(); i++) {
-    count[i] = items[i].second->tree.leaves;
-    int l = 0;
-    while ((uint64_t(1) << l) < count[i]) l++;
-    left[i] = l;
+  std::vector<MerkleSource> trees;
+  for (auto& it : items) {
+    Commitment& c = *it.second;
+    trees.push_back(MerkleSource{c.h.ptr, 32, c.tree.leaves, 1, c.tree.leaves, c.tree.nodes.ptr});
   }
-  bool first = true;
-  while (true) {
-    std::vector<MerkleJob> mj;
-    uint32_t mblocks = 0;
-    for (size_t i = 0; i < items.size(); i++) {
-      if (!first && left[i] == 0) continue;
-      const int lv = left[i] < 8 ? left[i] : 8;
-      Commitment& c = *items[i].second;
-      mj.push_back(MerkleJob{c.h.ptr, 32, c.tree.leaves, 1, first ? 1 : 0, c.tree.nodes.ptr, off[i], count[i],
-                             lv, mblocks});
-      mblocks += (uint32_t)((count[i] + 255) / 256);
-      for (int q = 0; q < lv; q++) {
-        off[i] += count[i];
-        count[i] >>= 1;
-      }
-      left[i] -= lv;
-    }
-    if (mj.empty()) break;
-    hold.host_merkle.push_back(mj);
-    const MerkleJob* d_mj = (const MerkleJob*)upload(hold.host_merkle.back().data(), mj.size() * sizeof(MerkleJob));
-    hip_check(launch_merkle_group(d_mj, (int)mj.size(), mblocks, ctx.stream), "batched merkle");
-    first = false;
-  }
+  enqueue_merkle_trees(ctx, trees, hold.host_merkle, hold.tables);
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[4], ctx.stream), "event");
   hold.host_ntt = std::move(nj);
   hold.host_hash = std::move(hj);

@@ -16,15 +16,29 @@ constexpr uint64_t GL_INV2 = 0x7fffffff80000001ull;  // (p + 1) / 2
 
 LGS_HD uint64_t gl_canon(uint64_t x) { return x >= GL_P ? x - GL_P : x; }
 
+// Canonical inputs and outputs. Written as 32-bit carry chains (v_add_co / v_addc_co on the
+// device: six full-rate instructions for an add, five for a subtract, instead of 64-bit adds
+// followed by 64-bit compares to recover the carries).
 LGS_HD uint64_t gl_add(uint64_t a, uint64_t b) {
-  uint64_t s = a + b;
-  s += (s < a) ? GL_EPS : 0;  // a carry out of 2^64 is worth 2^32 - 1
-  return gl_canon(s);
+  // s = a + b mod 2^64 and t = s + (2^32 - 1) mod 2^64 = a + b - p mod 2^64. The true sum is
+  // >= p exactly when either addition carries: a carry out of a + b means a + b >= 2^64 > p,
+  // and then t = a + b - p < p without a second wrap; else s + 2^32 - 1 >= 2^64 iff s >= p.
+  unsigned c0, c1, c2, c3;
+  const unsigned sl = __builtin_addc((unsigned)a, (unsigned)b, 0u, &c0);
+  const unsigned sh = __builtin_addc((unsigned)(a >> 32), (unsigned)(b >> 32), c0, &c1);
+  const unsigned tl = __builtin_addc(sl, 0xffffffffu, 0u, &c2);
+  const unsigned th = __builtin_addc(sh, 0u, c2, &c3);
+  return (c1 | c3) ? (((uint64_t)th << 32) | tl) : (((uint64_t)sh << 32) | sl);
 }
 
 LGS_HD uint64_t gl_sub(uint64_t a, uint64_t b) {
-  uint64_t d = a - b;
-  return (a < b) ? d - GL_EPS : d;  // a - b + 2^64 - (2^64 - p)
+  // a - b, and on a borrow a - b + 2^64 - (2^32 - 1) = a - b + p
+  unsigned b0, b1, b2, b3;
+  const unsigned dl = __builtin_subc((unsigned)a, (unsigned)b, 0u, &b0);
+  const unsigned dh = __builtin_subc((unsigned)(a >> 32), (unsigned)(b >> 32), b0, &b1);
+  const unsigned rl = __builtin_subc(dl, b1 ? 0xffffffffu : 0u, 0u, &b2);
+  const unsigned rh = __builtin_subc(dh, 0u, b2, &b3);
+  return ((uint64_t)rh << 32) | rl;
 }
 
 LGS_HD uint64_t gl_neg(uint64_t a) { return a ? GL_P - a : 0; }

@@ -92,12 +92,15 @@ __device__ __forceinline__ uint64_t tw_full(const uint64_t* __restrict__ tw, uin
 }
 
 // In-place radix-2 DIT network on 8 values held in bit-reversed order; output natural order.
+// ODD_ZERO: x[1], x[3], x[5], x[7] are known to be zero (the upper half of a zero-padded row),
+// so the first level is copies.
+template <bool ODD_ZERO = false>
 __device__ __forceinline__ void dft8(uint64_t (&x)[8]) {
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const uint64_t a = x[2 * u], b = x[2 * u + 1];
-    x[2 * u] = gl_add(a, b);
-    x[2 * u + 1] = gl_sub(a, b);
+    x[2 * u] = ODD_ZERO ? a : gl_add(a, b);
+    x[2 * u + 1] = ODD_ZERO ? a : gl_sub(a, b);
   }
 #pragma unroll
   for (int h = 0; h < 8; h += 4) {
@@ -163,15 +166,21 @@ __device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uin
     // positions rev3(r) of block rev_{logN-3}(k) of the bit-reversed array
     const uint32_t E = N / 8;
     const bool last = logN == 3;
+    // RS encoding pads every row to twice its length: then the odd slots (r = rev3(t) >= 4,
+    // idx >= N / 2) are all zero
+    const bool half_zero = n_in <= N / 2;
     for (uint32_t k = threadIdx.x; k < E; k += blockDim.x) {
       uint64_t x[8];
 #pragma unroll
       for (int t = 0; t < 8; t++) {
         const uint32_t r = __brev((uint32_t)t) >> 29;
         const uint32_t idx = k + r * E;
-        x[t] = idx < n_in ? src[(uint64_t)idx * es] : 0;
+        x[t] = (idx < n_in && !(half_zero && (t & 1))) ? src[(uint64_t)idx * es] : 0;
       }
-      dft8(x);
+      if (half_zero)
+        dft8<true>(x);
+      else
+        dft8(x);
       const uint32_t b = logN > 3 ? brev_bits(k, logN - 3) : 0;
 #pragma unroll
       for (int t = 0; t < 8; t++) {

@@ -244,25 +244,6 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   hip_check(launch_zero(counters.ptr, 2, st), "clear counters");
   unsigned int* counter = (unsigned int*)counters.ptr;
 
-  auto run_sumcheck = [&](int which, std::vector<ef2*> polys, size_t ch_off) {
-    const Sumcheck& S = sc[which];
-    for (uint32_t t = 0; t < S.nv; t++) {
-      uint64_t half = uint64_t(1) << (S.nv - 1 - t);
-      if (2 * half <= 128) {  // the remaining rounds run in one wave
-        hip_check(launch_sc_tail(S.d, polys.data(), (uint32_t)(2 * half), B + ch_off + t,
-                                 res.ptr + S.rounds_at + (size_t)t * (S.d + 1), res.ptr + S.finals_at, st),
-                  "sumcheck tail");
-        return;
-      }
-      hip_check(launch_sc_round(S.d, polys.data(), half, B + ch_off + t, part.ptr, counter,
-                                res.ptr + S.rounds_at + (size_t)t * (S.d + 1), st),
-                "sumcheck round");
-    }
-    for (int j = 0; j < S.d; j++)
-      hip_check(hipMemcpyAsync(res.ptr + S.finals_at + j, polys[j], sizeof(ef2), hipMemcpyDeviceToDevice, st),
-                "sumcheck finals");
-  };
-
   // ---- a-bar = eq_{z1}^T F, y = a-bar(z2) (ligesis.rs:365-379) ----
   DevBuf<ef2> eqz1(pool, m), eqz2(pool, n), a_vec(pool, n);
   hip_check(launch_eq(B + o_z1, log_m, eqs.ptr, nullptr, 0, eqz1.ptr, st), "eq z1");
@@ -288,18 +269,17 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   // ---- B_I and its commitment (ligesis.rs:402-416) ----
   const uint64_t nbi = W * s;
   DevBuf<uint64_t> bi(pool, nbi);
-  DevBuf<ef2> bi_ef(pool, nbi), bi_m1(pool, nbi), bi_ef2(pool, nbi), big(pool, nbi);
+  DevBuf<ef2> bi_ef(pool, nbi), bi_m1(pool, nbi), bi_ef2(pool, nbi), big(pool, nbi), big2(pool, nbi);
   hip_check(launch_bits_gather(cm.enc.ptr, two_n, didx.ptr, (uint32_t)s, W, bi.ptr, bi_ef.ptr, bi_m1.ptr, st),
             "B_I gather");
   hip_check(hipMemcpyAsync(bi_ef2.ptr, bi_ef.ptr, nbi * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
   DevBasefold bi_state;
   enqueue_basefold_commit(ctx, bi_state, bi.ptr, false, nv_bi, li_bi, 1);
 
-  // ---- binary check (ligesis.rs:424-429) ----
+  // ---- input tables of the seven Stage A sumchecks ----
+  // binary check (ligesis.rs:424-429)
   hip_check(launch_eq(B + o_alpha1, Lb, eqs.ptr, nullptr, 0, big.ptr, st), "eq alpha1");
-  run_sumcheck(0, {bi_ef.ptr, bi_m1.ptr, big.ptr}, o_chbin);
-
-  // ---- hash / recon channels (ligesis.rs:432-485) ----
+  // hash / recon channels (ligesis.rs:432-485)
   DevBuf<ef2> eqa2(pool, 32), eqr2(pool, s), g1(pool, W), g2(pool, W), g2b(pool, W), vv(pool, W);
   hip_check(launch_eq(B + o_alpha2, 5, eqs.ptr, nullptr, 0, eqa2.ptr, st), "eq alpha2");
   hip_check(launch_eq(B + o_r2, log_s, eqs.ptr, nullptr, 0, eqr2.ptr, st), "eq r2");
@@ -313,27 +293,92 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   hip_check(launch_sel(didx.ptr, eqr2.ptr, (uint32_t)s, sel.ptr, two_n, st), "sel");
   hip_check(launch_dot(sel.ptr, ra.ptr, two_n, part.ptr, counter, res.ptr + o_y4, st), "y4");
   hip_check(launch_vvec(eqz1.ptr, W, vv.ptr, st), "v");
-  run_sumcheck(1, {g1.ptr, g2.ptr}, o_chhash);
-  run_sumcheck(2, {vv.ptr, g2b.ptr}, o_chrecon);
-
-  // ---- recon table and RS link (ligesis.rs:488-502); y5 = ra(r3) is sc_table's final
-  //      evaluation of ra, the reference's verify compares exactly these (ligesis.rs:681) ----
-  run_sumcheck(3, {sel.ptr, ra.ptr}, o_chtable);
-  DevBuf<ef2> wv(pool, n), a_copy(pool, n);
+  // recon table and RS link (ligesis.rs:488-502); y5 = ra(r3) is sc_table's final evaluation of
+  // ra, which is what the reference's verify compares (ligesis.rs:681)
+  DevBuf<ef2> wv(pool, n), a_copy(pool, n), a_copy2(pool, n);
   hip_check(launch_rs_weights(B + o_r3, logN, n, ctx.twiddle(logN), wv.ptr, st), "rs weights");
   hip_check(hipMemcpyAsync(a_copy.ptr, a_vec.ptr, n * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
-  run_sumcheck(4, {a_copy.ptr, wv.ptr}, o_chrs);
-
-  // ---- Stage A batching sumchecks (ligesis.rs:513-522) ----
-  hip_check(launch_eq(B + o_r1, Lb, eqs.ptr, B + o_gbi + 0, 0, big.ptr, st), "combine_eq");
-  hip_check(launch_eq(B + o_pbh, Lb, eqs.ptr, B + o_gbi + 1, 1, big.ptr, st), "combine_eq");
-  hip_check(launch_eq(B + o_pbr, Lb, eqs.ptr, B + o_gbi + 2, 1, big.ptr, st), "combine_eq");
-  run_sumcheck(5, {bi_ef2.ptr, big.ptr}, o_chbib);
+  // Stage A batching (ligesis.rs:513-522)
+  hip_check(launch_eq(B + o_r1, Lb, eqs.ptr, B + o_gbi + 0, 0, big2.ptr, st), "combine_eq");
+  hip_check(launch_eq(B + o_pbh, Lb, eqs.ptr, B + o_gbi + 1, 1, big2.ptr, st), "combine_eq");
+  hip_check(launch_eq(B + o_pbr, Lb, eqs.ptr, B + o_gbi + 2, 1, big2.ptr, st), "combine_eq");
   DevBuf<ef2> g_a(pool, n);
   hip_check(launch_eq(B + o_z2, log_n, eqs.ptr, B + o_ga + 0, 0, g_a.ptr, st), "combine_eq");
   hip_check(launch_eq(B + o_r6, log_n, eqs.ptr, B + o_ga + 1, 1, g_a.ptr, st), "combine_eq");
-  hip_check(hipMemcpyAsync(a_copy.ptr, a_vec.ptr, n * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
-  run_sumcheck(6, {a_copy.ptr, g_a.ptr}, o_chab);
+  hip_check(hipMemcpyAsync(a_copy2.ptr, a_vec.ptr, n * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
+
+  // ---- the seven sumchecks, in lockstep ----
+  // Every challenge is fixed before the first round (the oracle streams are inputs), so the
+  // sumchecks only depend on their own tables: one grouped launch per round step covers every
+  // sumcheck with rounds left, aligned so that all reach their last 7 rounds (<= 128 entries)
+  // together; one more launch runs those tails, one wave per sumcheck. Each round folds its
+  // tables in place, so every sumcheck owns its tables (big2, a_copy2).
+  struct ScRun {
+    ef2* p[3];
+    size_t ch_off;
+  };
+  const ScRun runs[7] = {{{bi_ef.ptr, bi_m1.ptr, big.ptr}, o_chbin}, {{g1.ptr, g2.ptr, nullptr}, o_chhash},
+                         {{vv.ptr, g2b.ptr, nullptr}, o_chrecon},     {{sel.ptr, ra.ptr, nullptr}, o_chtable},
+                         {{a_copy.ptr, wv.ptr, nullptr}, o_chrs},     {{bi_ef2.ptr, big2.ptr, nullptr}, o_chbib},
+                         {{a_copy2.ptr, g_a.ptr, nullptr}, o_chab}};
+  uint32_t n_big[7], steps = 0;
+  for (int i = 0; i < 7; i++) {
+    n_big[i] = sc[i].nv > 7 ? sc[i].nv - 7 : 0;  // rounds with more than 128 entries
+    steps = std::max(steps, n_big[i]);
+  }
+  DevBuf<ef2> scpart(pool, (size_t)7 * 1024 * 4);
+  DevBuf<uint64_t> sccount(pool, 4);  // 7 counters
+  hip_check(launch_zero(sccount.ptr, 4, st), "clear counters");
+  std::vector<ScRoundJob> rjobs;
+  std::vector<uint32_t> step_first(steps + 1), step_blocks(steps);
+  for (uint32_t tau = 0; tau < steps; tau++) {
+    step_first[tau] = (uint32_t)rjobs.size();
+    uint32_t blocks = 0;
+    size_t poff = 0;
+    for (int i = 0; i < 7; i++) {
+      if (tau + n_big[i] < steps) continue;  // starts later
+      const uint32_t t = tau + n_big[i] - steps;
+      const Sumcheck& S = sc[i];
+      ScRoundJob j{};
+      for (int q = 0; q < 3; q++) j.p[q] = runs[i].p[q];
+      j.half = uint64_t(1) << (S.nv - 1 - t);
+      j.r = B + runs[i].ch_off + t;
+      j.nblk = sc_round_blocks(j.half);
+      j.part = scpart.ptr + poff;
+      poff += (size_t)j.nblk * (S.d + 1);
+      j.counter = (unsigned int*)sccount.ptr + i;
+      j.out = res.ptr + S.rounds_at + (size_t)t * (S.d + 1);
+      j.d = (uint32_t)S.d;
+      j.first_block = blocks;
+      blocks += j.nblk;
+      rjobs.push_back(j);
+    }
+    step_blocks[tau] = blocks;
+  }
+  step_first[steps] = (uint32_t)rjobs.size();
+  std::vector<ScTailJob> tjobs(7);
+  for (int i = 0; i < 7; i++) {
+    const Sumcheck& S = sc[i];
+    const uint32_t t0 = n_big[i];
+    for (int q = 0; q < 3; q++) tjobs[i].p[q] = runs[i].p[q];
+    tjobs[i].len = 1u << (S.nv - t0);
+    tjobs[i].chal = B + runs[i].ch_off + t0;
+    tjobs[i].rounds = res.ptr + S.rounds_at + (size_t)t0 * (S.d + 1);
+    tjobs[i].finals = res.ptr + S.finals_at;
+    tjobs[i].d = (uint32_t)S.d;
+  }
+  DevBuf<uint8_t> scjobs(pool, rjobs.size() * sizeof(ScRoundJob) + tjobs.size() * sizeof(ScTailJob));
+  if (!rjobs.empty())
+    hip_check(hipMemcpyAsync(scjobs.ptr, rjobs.data(), rjobs.size() * sizeof(ScRoundJob), hipMemcpyHostToDevice, st),
+              "sumcheck jobs");
+  ScTailJob* d_tjobs = (ScTailJob*)(scjobs.ptr + rjobs.size() * sizeof(ScRoundJob));
+  hip_check(hipMemcpyAsync(d_tjobs, tjobs.data(), tjobs.size() * sizeof(ScTailJob), hipMemcpyHostToDevice, st),
+            "sumcheck tail jobs");
+  for (uint32_t tau = 0; tau < steps; tau++)
+    hip_check(launch_sc_round_group((const ScRoundJob*)scjobs.ptr + step_first[tau],
+                                    (int)(step_first[tau + 1] - step_first[tau]), step_blocks[tau], st),
+              "sumcheck round step");
+  hip_check(launch_sc_tail_group(d_tjobs, 7, st), "sumcheck tails");
 
   // ---- Stage B: shared-domain batched Basefold open (basefold.rs:679-856) ----
   const DevBasefold* states[3] = {&key.a_state, &bi_state, &av_state};
@@ -347,6 +392,43 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   for (uint32_t k = 0; k <= R; k++) {
     lvl_off[k] = tot;
     tot += D >> k;
+  }
+
+  // Opening positions are fixed by the integer stream, so every index the proof opens is
+  // computed here on the host while the device works through the sumchecks.
+  std::vector<uint64_t> base;
+  for (uint64_t q = 0; q < 100; q++) base.push_back(ints[4 * s + q] % (D / 2));
+  std::sort(base.begin(), base.end());
+  base.erase(std::unique(base.begin(), base.end()), base.end());
+  const std::vector<uint64_t> bpos = level_positions(base, D / 2);
+  const std::vector<uint64_t> hnodes = multiproof_nodes(two_n, sorted);
+  const size_t n_hopen = hnodes.size();
+  std::vector<uint64_t> hw;
+  for (uint64_t j : sorted)
+    for (uint64_t r = 0; r < 32; r++) hw.push_back(r * two_n + j);
+  std::vector<uint64_t> fw;
+  for (uint64_t i = 0; i < (D >> R); i++) {
+    fw.push_back(2 * (lvl_off[R] + i));
+    fw.push_back(2 * (lvl_off[R] + i) + 1);
+  }
+  const size_t nf = R > 1 ? R - 1 : 0;
+  std::vector<std::vector<uint64_t>> fpos(nf), fnodes(nf), fvw(nf);
+  for (uint32_t k = 1; k < R; k++) {
+    const uint64_t len = D >> k;
+    fpos[k - 1] = level_positions(base, len / 2);
+    fnodes[k - 1] = multiproof_nodes(len, fpos[k - 1]);
+    for (uint64_t p : fpos[k - 1]) {
+      fvw[k - 1].push_back(2 * (lvl_off[k] + p));
+      fvw[k - 1].push_back(2 * (lvl_off[k] + p) + 1);
+    }
+  }
+  const std::vector<uint64_t> bnodes = multiproof_nodes(D, bpos);
+  const size_t n_bproof = bnodes.size();
+  std::vector<uint64_t> bvw[3];
+  for (int c = 0; c < 3; c++) {
+    const uint32_t rows = states[c]->code_rows();
+    for (uint64_t p : bpos)
+      for (uint32_t rr = 0; rr < rows; rr++) bvw[c].push_back((uint64_t)rr * D + p);
   }
   DevBuf<ef2> levels(pool, tot);
   BfChannels chans{};
@@ -390,79 +472,54 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
               "eq fold codeword");
   }
   (void)twD;
+  // trees over the folded levels 1..R-1 (leaf = one F_p^2 value), built together
   std::vector<DevTree> ftrees(R > 1 ? R - 1 : 0);
+  std::vector<MerkleSource> fsrc;
   for (uint32_t k = 1; k < R; k++) {
     DevTree& tr = ftrees[k - 1];
     tr.leaves = D >> k;
     tr.nodes = DevBuf<uint32_t>(pool, tr.node_count() * 8);
-    hip_check(launch_merkle((const uint64_t*)(levels.ptr + lvl_off[k]), 2, 1, 2, tr.leaves, tr.nodes.ptr, st),
-              "fold tree");
+    fsrc.push_back(MerkleSource{(const uint64_t*)(levels.ptr + lvl_off[k]), 2, 1, 2, tr.leaves, tr.nodes.ptr});
   }
+  std::vector<std::vector<MerkleJob>> fjobs;
+  std::vector<DevBuf<uint64_t>> ftables;
+  enqueue_merkle_trees(ctx, fsrc, fjobs, ftables);
 
-  // ---- openings: positions are fixed by the integer stream ----
-  std::vector<uint64_t> base;
-  for (uint64_t q = 0; q < 100; q++) base.push_back(ints[4 * s + q] % (D / 2));
-  std::sort(base.begin(), base.end());
-  base.erase(std::unique(base.begin(), base.end()), base.end());
-  std::vector<uint64_t> bpos = level_positions(base, D / 2);
-
+  // ---- openings ----
   Gather g;
-  const size_t g_root_h = g.add_root(cm.tree);
-  (void)g_root_h;
-  const size_t g_hopen = g.add_nodes(cm.tree, multiproof_nodes(two_n, sorted));
-  const size_t n_hopen = multiproof_nodes(two_n, sorted).size();
-  std::vector<uint64_t> hw;
-  for (uint64_t j : sorted)
-    for (uint64_t r = 0; r < 32; r++) hw.push_back(r * two_n + j);
+  const size_t g_hopen = g.add_nodes(cm.tree, hnodes);
   const size_t g_hcols = g.add(cm.h.ptr, hw);
   const size_t g_com_av = g.add_root(av_state.tree), g_com_i = g.add_root(bi_state.tree);
-  std::vector<size_t> g_froot(R > 1 ? R - 1 : 0);
+  std::vector<size_t> g_froot(nf), g_fproof(nf), g_fval(nf);
   for (uint32_t k = 1; k < R; k++) g_froot[k - 1] = g.add_root(ftrees[k - 1]);
-  std::vector<uint64_t> fw;
-  for (uint64_t i = 0; i < (D >> R); i++) {
-    fw.push_back(2 * (lvl_off[R] + i));
-    fw.push_back(2 * (lvl_off[R] + i) + 1);
-  }
   const size_t g_final = g.add((const uint64_t*)levels.ptr, fw);
-  // fold-level openings
-  std::vector<std::vector<uint64_t>> fpos(R > 1 ? R - 1 : 0);
-  std::vector<size_t> g_fproof(R > 1 ? R - 1 : 0), g_fval(R > 1 ? R - 1 : 0), n_fproof(R > 1 ? R - 1 : 0);
   for (uint32_t k = 1; k < R; k++) {
-    uint64_t len = D >> k;
-    fpos[k - 1] = level_positions(base, len / 2);
-    auto nodes = multiproof_nodes(len, fpos[k - 1]);
-    n_fproof[k - 1] = nodes.size();
-    g_fproof[k - 1] = g.add_nodes(ftrees[k - 1], nodes);
-    std::vector<uint64_t> vw;
-    for (uint64_t p : fpos[k - 1]) {
-      vw.push_back(2 * (lvl_off[k] + p));
-      vw.push_back(2 * (lvl_off[k] + p) + 1);
-    }
-    g_fval[k - 1] = g.add((const uint64_t*)levels.ptr, vw);
+    g_fproof[k - 1] = g.add_nodes(ftrees[k - 1], fnodes[k - 1]);
+    g_fval[k - 1] = g.add((const uint64_t*)levels.ptr, fvw[k - 1]);
   }
-  // base openings per commitment (columns of the interleaved codewords)
-  auto bnodes_cache = std::vector<uint64_t>();
-  size_t g_bproof[3], g_bval[3], n_bproof = 0;
+  size_t g_bproof[3], g_bval[3];
   for (int c = 0; c < 3; c++) {
-    auto nodes = multiproof_nodes(D, bpos);
-    n_bproof = nodes.size();
-    g_bproof[c] = g.add_nodes(states[c]->tree, nodes);
-    std::vector<uint64_t> vw;
-    uint32_t rows = states[c]->code_rows();
-    for (uint64_t p : bpos)
-      for (uint32_t rr = 0; rr < rows; rr++) vw.push_back((uint64_t)rr * D + p);
-    g_bval[c] = g.add(states[c]->codes.ptr, vw);
+    g_bproof[c] = g.add_nodes(states[c]->tree, bnodes);
+    g_bval[c] = g.add(states[c]->codes.ptr, bvw[c]);
   }
 
-  // stage: [results (2 * res_n words)] [gathered words]
+  // stage: [results (2 * res_n words)] [gathered words], gathered by one launch over a table of
+  // (source, first word) parts followed by the word offsets
   const size_t stage_words = 2 * res_n + g.offs.size();
   DevBuf<uint64_t> dstage(pool, stage_words);
-  DevBuf<uint64_t> doffs(pool, g.offs.size() + 1);
-  hip_check(hipMemcpyAsync(doffs.ptr, g.offs.data(), g.offs.size() * 8, hipMemcpyHostToDevice, st), "offsets");
+  std::vector<uint64_t> gtab;
+  gtab.reserve(2 * g.parts.size() + g.offs.size());
+  for (auto& p : g.parts) {
+    gtab.push_back((uint64_t)(uintptr_t)p.src);
+    gtab.push_back(p.first);
+  }
+  gtab.insert(gtab.end(), g.offs.begin(), g.offs.end());
+  DevBuf<uint64_t> dgtab(pool, gtab.size());
+  hip_check(hipMemcpyAsync(dgtab.ptr, gtab.data(), gtab.size() * 8, hipMemcpyHostToDevice, st), "gather table");
   hip_check(hipMemcpyAsync(dstage.ptr, res.ptr, res_n * sizeof(ef2), hipMemcpyDeviceToDevice, st), "results");
-  for (auto& p : g.parts)
-    hip_check(launch_gather_words(p.src, doffs.ptr + p.first, p.count, dstage.ptr + 2 * res_n + p.first, st),
-              "gather");
+  hip_check(launch_gather_parts((const GatherPart*)dgtab.ptr, (uint32_t)g.parts.size(),
+                                dgtab.ptr + 2 * g.parts.size(), g.offs.size(), dstage.ptr + 2 * res_n, st),
+            "gather");
   uint64_t* host = (uint64_t*)ctx.staging(stage_words * 8);
   hip_check(hipMemcpyAsync(host, dstage.ptr, stage_words * 8, hipMemcpyDeviceToHost, st), "download proof");
   hip_check(hipStreamSynchronize(st), "prove sync");
@@ -504,8 +561,8 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   w.words(GW + g_final, 2 * (D >> R));
   w.u64(R > 1 ? R - 1 : 0);
   for (uint32_t k = 1; k < R; k++) {
-    w.u64(32 * n_fproof[k - 1]);
-    w.raw(GW + g_fproof[k - 1], 4 * n_fproof[k - 1]);
+    w.u64(32 * fnodes[k - 1].size());
+    w.raw(GW + g_fproof[k - 1], 4 * fnodes[k - 1].size());
     w.u64(fpos[k - 1].size());
     for (size_t q = 0; q < fpos[k - 1].size(); q++) {
       w.u64(fpos[k - 1][q]);

@@ -30,18 +30,19 @@ __device__ __forceinline__ void block_sum(ef2 (&v)[W], ef2* sh /* blockDim x W *
   }
 }
 
-// Called by every block after its thread 0 stored W partial sums at part[blockIdx.x * W]: the
-// last block to finish (ticket from an atomic counter) adds up all gridDim.x partials into
-// out[0..W) and re-arms the counter for the next launch on the stream. Saves a separate
-// reduction launch per round.
+// Called by every one of the nblk blocks of a reduction after its thread 0 stored W partial
+// sums at part[blk * W]: the last block to finish (ticket from an atomic counter) adds up all
+// nblk partials into out[0..W) and re-arms the counter for the next launch on the stream. Saves
+// a separate reduction launch per round.
 template <int W>
-__device__ __forceinline__ void last_block_reduce(const ef2* part, unsigned int* counter, ef2* out, ef2* sh) {
+__device__ __forceinline__ void last_block_reduce(const ef2* part, unsigned int* counter, ef2* out, ef2* sh,
+                                                  uint32_t nblk) {
   __shared__ int is_last;
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
     const unsigned int ticket = atomicAdd(counter, 1u);
-    is_last = ticket == gridDim.x - 1;
+    is_last = ticket == nblk - 1;
   }
   __syncthreads();
   if (!is_last) return;
@@ -50,7 +51,7 @@ __device__ __forceinline__ void last_block_reduce(const ef2* part, unsigned int*
 #pragma unroll
   for (int q = 0; q < W; q++) acc[q] = ef_zero();
   const volatile uint64_t* pv = (const volatile uint64_t*)part;
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+  for (uint32_t b = threadIdx.x; b < nblk; b += blockDim.x) {
 #pragma unroll
     for (int q = 0; q < W; q++) {
       const uint64_t k = 2 * ((uint64_t)b * W + q);
@@ -147,32 +148,20 @@ __global__ __launch_bounds__(256) void dot_kernel(const ef2* __restrict__ a, con
     acc[0] = ef_add(acc[0], ef_mul(a[i], b[i]));
   block_sum<1>(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
-  last_block_reduce<1>(part, counter, out, sh);
+  last_block_reduce<1>(part, counter, out, sh, gridDim.x);
 }
 
 // out[q] = sum_{b < nb} part[b * width + q], for q < width <= 4 (one block of 256 threads:
 // strided partial sums, then a tree in LDS)
-__global__ __launch_bounds__(256) void reduce_rows_kernel(const ef2* __restrict__ part, uint32_t nb,
-                                                          uint32_t width, ef2* __restrict__ out) {
-  __shared__ ef2 sh[256 * 4];
-  ef2 acc[4] = {ef_zero(), ef_zero(), ef_zero(), ef_zero()};
-  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
-    for (uint32_t q = 0; q < width; q++) acc[q] = ef_add(acc[q], part[(uint64_t)b * width + q]);
-  block_sum<4>(acc, sh);
-  if (threadIdx.x == 0)
-    for (uint32_t q = 0; q < width; q++) out[q] = acc[q];
-}
 
 // The last rounds of a product sumcheck in one workgroup: the factors (at most 2 * 512 entries
 // each) are folded in LDS round after round; round polynomials go to `rounds` (D + 1 each) and
 // the final evaluations to `finals`.
 template <int D, int CAP>
-__global__ __launch_bounds__(256) void sc_tail_kernel(const ef2* __restrict__ p0, const ef2* __restrict__ p1,
-                                                      const ef2* __restrict__ p2, uint32_t len,
-                                                      const ef2* __restrict__ chal, ef2* __restrict__ rounds,
-                                                      ef2* __restrict__ finals) {
-  __shared__ ef2 buf[D][CAP];
-  __shared__ ef2 sh[(CAP / 2) * (D + 1)];
+__device__ __forceinline__ void sc_tail_body(const ef2* __restrict__ p0, const ef2* __restrict__ p1,
+                                             const ef2* __restrict__ p2, uint32_t len,
+                                             const ef2* __restrict__ chal, ef2* __restrict__ rounds,
+                                             ef2* __restrict__ finals, ef2 (*buf)[CAP], ef2* sh) {
   const ef2* src[3] = {p0, p1, p2};
 #pragma unroll
   for (int j = 0; j < D; j++)
@@ -227,20 +216,22 @@ __global__ __launch_bounds__(256) void sc_tail_kernel(const ef2* __restrict__ p0
   }
 }
 
-hipError_t launch_sc_tail(int d, ef2* const* polys, uint32_t len, const ef2* chal, ef2* rounds, ef2* finals,
-                          hipStream_t st) {
-  // one wave: every lane owns one pair of the first tail round, LDS tree sums over 64 lanes
-  if (len > 128) return hipErrorInvalidValue;
-  if (d == 2)
-    hipLaunchKernelGGL((sc_tail_kernel<2, 128>), dim3(1), dim3(64), 0, st, polys[0], polys[1], nullptr, len, chal,
-                       rounds, finals);
-  else if (d == 3)
-    hipLaunchKernelGGL((sc_tail_kernel<3, 128>), dim3(1), dim3(64), 0, st, polys[0], polys[1], polys[2], len,
-                       chal, rounds, finals);
+// The tails of several sumchecks, one wave each (blockIdx.x = job).
+__global__ __launch_bounds__(64) void sc_tail_group_kernel(const ScTailJob* __restrict__ jobs) {
+  __shared__ ef2 buf[3][128];
+  __shared__ ef2 sh[64 * 4];
+  const ScTailJob& j = jobs[blockIdx.x];
+  if (j.d == 3)
+    sc_tail_body<3, 128>(j.p[0], j.p[1], j.p[2], j.len, j.chal, j.rounds, j.finals, buf, sh);
   else
-    return hipErrorInvalidValue;
+    sc_tail_body<2, 128>(j.p[0], j.p[1], nullptr, j.len, j.chal, j.rounds, j.finals, buf, sh);
+}
+
+hipError_t launch_sc_tail_group(const ScTailJob* jobs, uint32_t njobs, hipStream_t st) {
+  if (njobs) hipLaunchKernelGGL(sc_tail_group_kernel, dim3(njobs), dim3(64), 0, st, jobs);
   return hipGetLastError();
 }
+
 
 hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part /* >= 256 */, unsigned int* counter,
                       ef2* out, hipStream_t st) {
@@ -327,18 +318,16 @@ __global__ void planar_to_ef_kernel(const uint64_t* __restrict__ c0, const uint6
 // Round over pairs (lo = p[i], hi = p[i + half]): g(t) = sum_i prod_j ((1 - t) lo_j + t hi_j) for
 // t = 0..D, then p[i] <- lo + r (hi - lo) in place (the high variable is bound first).
 template <int D>
-__global__ __launch_bounds__(256) void sc_round_kernel(ef2* __restrict__ p0, ef2* __restrict__ p1,
-                                                       ef2* __restrict__ p2, uint64_t half,
-                                                       const ef2* __restrict__ r_ptr,
-                                                       ef2* __restrict__ part, unsigned int* counter,
-                                                       ef2* __restrict__ out) {
-  __shared__ ef2 sh[256 * (D + 1)];
+__device__ __forceinline__ void sc_round_body(ef2* __restrict__ p0, ef2* __restrict__ p1, ef2* __restrict__ p2,
+                                              uint64_t half, const ef2* __restrict__ r_ptr, ef2* __restrict__ part,
+                                              unsigned int* counter, ef2* __restrict__ out, uint32_t blk,
+                                              uint32_t nblk, ef2* sh) {
   ef2* ps[3] = {p0, p1, p2};
   const ef2 r = *r_ptr;
   ef2 acc[D + 1];
 #pragma unroll
   for (int q = 0; q <= D; q++) acc[q] = ef_zero();
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += (uint64_t)gridDim.x * blockDim.x) {
+  for (uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x; i < half; i += (uint64_t)nblk * blockDim.x) {
     ef2 lo[D], diff[D];
 #pragma unroll
     for (int j = 0; j < D; j++) {
@@ -364,9 +353,32 @@ __global__ __launch_bounds__(256) void sc_round_kernel(ef2* __restrict__ p0, ef2
   block_sum<D + 1>(acc, sh);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q <= D; q++) part[(uint64_t)blockIdx.x * (D + 1) + q] = acc[q];
+    for (int q = 0; q <= D; q++) part[(uint64_t)blk * (D + 1) + q] = acc[q];
   }
-  last_block_reduce<D + 1>(part, counter, out, sh);
+  last_block_reduce<D + 1>(part, counter, out, sh, nblk);
+}
+
+// One round of several independent sumchecks in one launch (sumchecks run in lockstep): block
+// -> job by binary search over the jobs' first blocks.
+__global__ __launch_bounds__(256) void sc_round_group_kernel(const ScRoundJob* __restrict__ jobs, int njobs) {
+  __shared__ ef2 sh[256 * 4];
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].first_block <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const ScRoundJob& j = jobs[lo];
+  const uint32_t blk = blockIdx.x - j.first_block;
+  if (j.d == 3)
+    sc_round_body<3>(j.p[0], j.p[1], j.p[2], j.half, j.r, j.part, j.counter, j.out, blk, j.nblk, sh);
+  else
+    sc_round_body<2>(j.p[0], j.p[1], nullptr, j.half, j.r, j.part, j.counter, j.out, blk, j.nblk, sh);
+}
+
+hipError_t launch_sc_round_group(const ScRoundJob* jobs, int njobs, uint32_t total_blocks, hipStream_t st) {
+  if (total_blocks) hipLaunchKernelGGL(sc_round_group_kernel, dim3(total_blocks), dim3(256), 0, st, jobs, njobs);
+  return hipGetLastError();
 }
 
 uint32_t sc_round_blocks(uint64_t half) {
@@ -374,24 +386,7 @@ uint32_t sc_round_blocks(uint64_t half) {
   return (uint32_t)(nb > 1024 ? 1024 : nb);
 }
 
-hipError_t launch_sc_round(int d, ef2* const* polys, uint64_t half, const ef2* r, ef2* part,
-                           unsigned int* counter, ef2* out, hipStream_t st) {
-  const uint32_t nb = sc_round_blocks(half);
-  if (d == 2)
-    hipLaunchKernelGGL(sc_round_kernel<2>, dim3(nb), dim3(256), 0, st, polys[0], polys[1], nullptr, half, r, part,
-                       counter, out);
-  else if (d == 3)
-    hipLaunchKernelGGL(sc_round_kernel<3>, dim3(nb), dim3(256), 0, st, polys[0], polys[1], polys[2], half, r,
-                       part, counter, out);
-  else
-    return hipErrorInvalidValue;
-  return hipGetLastError();
-}
 
-hipError_t launch_reduce_rows(const ef2* part, uint32_t nb, uint32_t width, ef2* out, hipStream_t st) {
-  hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(256), 0, st, part, nb, width, out);
-  return hipGetLastError();
-}
 
 // ---------------- Basefold batch (basefold.rs:648-856) ----------------
 // combined[i] = sum over channels c, chunks j of wt_c[j] * codes_c[j][i], where
@@ -435,21 +430,6 @@ __global__ void fold_low_kernel(const uint64_t* __restrict__ poly, int ext, int 
 
 // One reduction round: partial sums of m[2i] * eq_rest[i] (l0, basefold.rs:752-758) and the
 // low-variable fold m'[i] = m[2i] + r (m[2i+1] - m[2i]) into a second buffer.
-__global__ __launch_bounds__(256) void bf_round_kernel(const ef2* __restrict__ m, uint64_t half,
-                                                       const ef2* __restrict__ eq_rest,
-                                                       const ef2* __restrict__ r_ptr,
-                                                       ef2* __restrict__ m_out, ef2* __restrict__ part) {
-  __shared__ ef2 sh[256];
-  const ef2 r = *r_ptr;
-  ef2 acc[1] = {ef_zero()};
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += (uint64_t)gridDim.x * blockDim.x) {
-    const ef2 e = m[2 * i], o = m[2 * i + 1];
-    acc[0] = ef_add(acc[0], ef_mul(e, eq_rest[i]));
-    m_out[i] = ef_add(e, ef_mul(r, ef_sub(o, e)));
-  }
-  block_sum<1>(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
-}
 
 // All channels of one reduction round in one launch (blockIdx.y = channel): l0 partials with
 // the last block of each channel reducing them into out[c], the low-variable fold of m, and the
@@ -497,11 +477,6 @@ hipError_t launch_bf_round3(const BfRound& a, uint64_t half, const ef2* r, hipSt
   return hipGetLastError();
 }
 
-// eq over point[1..] from eq over point[0..]: out[i] = in[2i] + in[2i+1]
-__global__ void eq_pairsum_kernel(const ef2* __restrict__ in, uint64_t nout, ef2* __restrict__ out) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nout) out[i] = ef_add(in[2 * i], in[2 * i + 1]);
-}
 
 // eq_fold_codeword (basefold.rs:201-218): out[i] = (1 - r)(x + nx)/2 + r (x - nx) g^-i / 2 with
 // x = v[i], nx = v[i + h]; g^-i = -g^(h - i) for 0 < i < h, tw[e] = g^e (e < h).
@@ -518,10 +493,17 @@ __global__ void eq_fold_codeword_kernel(const ef2* __restrict__ v, uint64_t h, c
 }
 
 // ---------------- gathers for openings ----------------
-__global__ void gather_words_kernel(const uint64_t* __restrict__ src, const uint64_t* __restrict__ off,
-                                    uint64_t count, uint64_t* __restrict__ dst) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < count) dst[t] = src[off[t]];
+__global__ void gather_parts_kernel(const GatherPart* __restrict__ parts, uint32_t nparts,
+                                    const uint64_t* __restrict__ off, uint64_t total, uint64_t* __restrict__ dst) {
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = nparts - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (parts[mid].first <= w) lo = mid;
+      else hi = mid - 1;
+    }
+    dst[w] = parts[lo].src[off[w]];
+  }
 }
 
 #define LGS_GRID(n) dim3((uint32_t)(((n) + 255) / 256)), dim3(256)
@@ -561,25 +543,17 @@ hipError_t launch_fold_low(const uint64_t* poly, int ext, int li, uint64_t nout,
   hipLaunchKernelGGL(fold_low_kernel, LGS_GRID(nout), 0, st, poly, ext, li, nout, eq4, m);
   return hipGetLastError();
 }
-uint32_t bf_round_blocks(uint64_t half) { return sc_round_blocks(half); }
-hipError_t launch_bf_round(const ef2* m, uint64_t half, const ef2* eq_rest, const ef2* r, ef2* m_out, ef2* part,
-                           hipStream_t st) {
-  hipLaunchKernelGGL(bf_round_kernel, dim3(bf_round_blocks(half)), dim3(256), 0, st, m, half, eq_rest, r, m_out, part);
-  return hipGetLastError();
-}
-hipError_t launch_eq_pairsum(const ef2* in, uint64_t nout, ef2* out, hipStream_t st) {
-  hipLaunchKernelGGL(eq_pairsum_kernel, LGS_GRID(nout), 0, st, in, nout, out);
-  return hipGetLastError();
-}
 hipError_t launch_eq_fold_codeword(const ef2* v, uint64_t h, const ef2* r, const uint64_t* tw, ef2* out,
                                    hipStream_t st) {
   hipLaunchKernelGGL(eq_fold_codeword_kernel, LGS_GRID(h), 0, st, v, h, r, tw, out);
   return hipGetLastError();
 }
-hipError_t launch_gather_words(const uint64_t* src, const uint64_t* off, uint64_t count, uint64_t* dst,
-                               hipStream_t st) {
-  if (count == 0) return hipSuccess;
-  hipLaunchKernelGGL(gather_words_kernel, LGS_GRID(count), 0, st, src, off, count, dst);
+hipError_t launch_gather_parts(const GatherPart* parts, uint32_t nparts, const uint64_t* off, uint64_t total,
+                               uint64_t* dst, hipStream_t st) {
+  if (total == 0 || nparts == 0) return hipSuccess;
+  uint64_t blocks = (total + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(gather_parts_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, parts, nparts, off, total, dst);
   return hipGetLastError();
 }
 hipError_t launch_zero(uint64_t* p, uint64_t n, hipStream_t st) {

@@ -4,6 +4,29 @@
 
 namespace lgs {
 
+// One round of one sumcheck inside a grouped launch (sc_round_group_kernel): blocks
+// [first_block, first_block + nblk) of the launch. p[2] is unused when d == 2.
+struct ScRoundJob {
+  ef2* p[3];
+  uint64_t half;
+  const ef2* r;
+  ef2* part;              // nblk x (d + 1) partial sums
+  unsigned int* counter;  // zeroed; the job's last block re-arms it
+  ef2* out;               // d + 1 round evaluations
+  uint32_t d, first_block, nblk, pad;
+};
+hipError_t launch_sc_round_group(const ScRoundJob* jobs, int njobs, uint32_t total_blocks, hipStream_t st);
+
+// The remaining rounds (len <= 128 entries) of one sumcheck, one wave per job.
+struct ScTailJob {
+  const ef2* p[3];
+  const ef2* chal;
+  ef2* rounds;
+  ef2* finals;
+  uint32_t d, len;
+};
+hipError_t launch_sc_tail_group(const ScTailJob* jobs, uint32_t njobs, hipStream_t st);
+
 // One Basefold reduction round over its three channels (bf_round3_kernel, basefold.rs:748-775).
 struct BfRound {
   const ef2* m_in[3];
@@ -41,23 +64,18 @@ hipError_t launch_sel(const uint32_t* idx, const ef2* eq_r2, uint32_t s, ef2* se
 hipError_t launch_rs_weights(const ef2* r3, int log2n, uint64_t n, const uint64_t* tw, ef2* w, hipStream_t st);
 hipError_t launch_planar_to_ef(const uint64_t* c0, const uint64_t* c1, uint64_t n, ef2* out, hipStream_t st);
 uint32_t sc_round_blocks(uint64_t half);
-hipError_t launch_sc_round(int d, ef2* const* polys, uint64_t half, const ef2* r, ef2* part,
-                           unsigned int* counter, ef2* out, hipStream_t st);
-hipError_t launch_reduce_rows(const ef2* part, uint32_t nb, uint32_t width, ef2* out, hipStream_t st);
-// All rounds of a sumcheck whose factors have len <= 128 entries left, in one wave.
-hipError_t launch_sc_tail(int d, ef2* const* polys, uint32_t len, const ef2* chal, ef2* rounds, ef2* finals,
-                          hipStream_t st);
 hipError_t launch_bf_combine(const BfChannels& ch, uint64_t D, ef2* out, hipStream_t st);
 hipError_t launch_fold_low(const uint64_t* poly, int ext, int li, uint64_t nout, const ef2* eq4, ef2* m,
                            hipStream_t st);
-uint32_t bf_round_blocks(uint64_t half);
-hipError_t launch_bf_round(const ef2* m, uint64_t half, const ef2* eq_rest, const ef2* r, ef2* m_out, ef2* part,
-                           hipStream_t st);
-hipError_t launch_eq_pairsum(const ef2* in, uint64_t nout, ef2* out, hipStream_t st);
 hipError_t launch_eq_fold_codeword(const ef2* v, uint64_t h, const ef2* r, const uint64_t* tw, ef2* out,
                                    hipStream_t st);
-hipError_t launch_gather_words(const uint64_t* src, const uint64_t* off, uint64_t count, uint64_t* dst,
-                               hipStream_t st);
+// dst[w] = parts[p].src[off[w]] for the part p whose [first, next first) range holds w.
+struct GatherPart {
+  const uint64_t* src;
+  uint64_t first;
+};
+hipError_t launch_gather_parts(const GatherPart* parts, uint32_t nparts, const uint64_t* off, uint64_t total,
+                               uint64_t* dst, hipStream_t st);
 hipError_t launch_zero(uint64_t* p, uint64_t n, hipStream_t st);
 
 }  // namespace lgs

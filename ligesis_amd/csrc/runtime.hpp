@@ -240,6 +240,17 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
 bool group_eligible(const Key& key);
 void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, Commitment*>>& items,
                           GroupHold& hold);
+// Merkle trees over many sources (leaf p = rows words M[r * rs + p * cs]), one launch per
+// reduction round of up to 8 levels across all trees; the job tables stay in host_jobs / tables
+// until the caller synchronises.
+struct MerkleSource {
+  const uint64_t* M;
+  uint32_t rows;
+  uint64_t rs, cs, leaves;
+  uint32_t* nodes;
+};
+void enqueue_merkle_trees(Context& ctx, const std::vector<MerkleSource>& trees,
+                          std::vector<std::vector<MerkleJob>>& host_jobs, std::vector<DevBuf<uint64_t>>& tables);
 void enqueue_basefold_commit(Context& ctx, DevBasefold& st, const uint64_t* poly, bool ext,
                              uint32_t num_vars, uint32_t li, uint32_t code_rate);
 void setup_key(Context& ctx, Key& key, const uint64_t* host_a, bool commit_a);

@@ -137,6 +137,12 @@ double autotune(Context& ctx, const Key& key, uint32_t reps, TuneEntry& best) {
   };
 
   Tuning cur = builtin_tuning(key.log_m, key.log_n);
+  // The device raises its clocks over the first ~100 ms of load; candidates timed before that
+  // would look slower than later ones. Keep it busy for at least 200 ms first.
+  {
+    double busy = 0;
+    for (int i = 0; i < 2000 && busy < 200.0; i++) busy += run(cur).total * (reps + 1);
+  }
   // stage 1: row transform
   {
     std::vector<int> cand = {0, 1};
