@@ -29,6 +29,7 @@ void ctx_unref(lgs::Context* c) {
   if (!c || c->refs.fetch_sub(1) != 1) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   c->twiddles.clear();
   c->pool.trim();
   for (auto& e : c->ev)
@@ -36,6 +37,7 @@ void ctx_unref(lgs::Context* c) {
   if (c->pinned) (void)hipHostFree(c->pinned);
   c->pinned = nullptr;
   (void)hipStreamDestroy(c->stream);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   delete (lgs_ctx*)c->owner;
 }
 
@@ -337,7 +339,9 @@ int lgs_prove(lgs_ctx* ctx, const lgs_key* key, const lgs_commitment* c, const u
     try {
       lgs::prove_impl(ctx->c, key->k, c->c, z, z_len, fields, n_fields, ints, n_ints, bytes, u);
     } catch (...) {
+      // no buffer released by the failed prove may be reused before both streams drain
       (void)hipStreamSynchronize(ctx->c.stream);
+      if (ctx->c.stream2) (void)hipStreamSynchronize(ctx->c.stream2);
       throw;
     }
     if (used) {

@@ -22,6 +22,11 @@ const uint64_t* Context::twiddle(int logN) {
   return p;
 }
 
+hipStream_t Context::side_stream() {
+  if (!stream2) hip_check(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking), "side stream");
+  return stream2;
+}
+
 uint8_t* Context::staging(size_t bytes) {
   if (bytes > pinned_size) {
     if (pinned) (void)hipHostFree(pinned);
@@ -155,7 +160,8 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
 }
 
 void enqueue_merkle_trees(Context& ctx, const std::vector<MerkleSource>& trees,
-                          std::vector<std::vector<MerkleJob>>& host_jobs, std::vector<DevBuf<uint64_t>>& tables) {
+                          std::vector<std::vector<MerkleJob>>& host_jobs, std::vector<DevBuf<uint64_t>>& tables,
+                          hipStream_t st) {
   // rounds of up to 8 levels; every tree takes part until its root is written
   const size_t nt = trees.size();
   std::vector<uint64_t> off(nt, 0), count(nt);
@@ -199,10 +205,10 @@ void enqueue_merkle_trees(Context& ctx, const std::vector<MerkleSource>& trees,
   const std::vector<MerkleJob>& h = host_jobs.back();
   tables.emplace_back(&ctx.pool, (h.size() * sizeof(MerkleJob) + 7) / 8);
   const MerkleJob* d = (const MerkleJob*)tables.back().ptr;
-  hip_check(hipMemcpyAsync((void*)d, h.data(), h.size() * sizeof(MerkleJob), hipMemcpyHostToDevice, ctx.stream),
+  hip_check(hipMemcpyAsync((void*)d, h.data(), h.size() * sizeof(MerkleJob), hipMemcpyHostToDevice, st),
             "merkle jobs");
   for (size_t r = 0; r < rounds.size(); r++)
-    hip_check(launch_merkle_group(d + at[r], (int)rounds[r].size(), blocks[r], ctx.stream), "grouped merkle");
+    hip_check(launch_merkle_group(d + at[r], (int)rounds[r].size(), blocks[r], st), "grouped merkle");
 }
 
 // Batched commit: every stage of every eligible commitment runs in one launch per stage (row
@@ -268,7 +274,7 @@ void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, 
     Commitment& c = *it.second;
     trees.push_back(MerkleSource{c.h.ptr, 32, c.tree.leaves, 1, c.tree.leaves, c.tree.nodes.ptr});
   }
-  enqueue_merkle_trees(ctx, trees, hold.host_merkle, hold.tables);
+  enqueue_merkle_trees(ctx, trees, hold.host_merkle, hold.tables, ctx.stream);
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[4], ctx.stream), "event");
   hold.host_ntt = std::move(nj);
   hold.host_hash = std::move(hj);

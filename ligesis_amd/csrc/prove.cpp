@@ -259,12 +259,22 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec.ptr, true, log_n, li_a, 1);
 
   // ---- RS(a-bar) (ligesis.rs:387) ----
+  // With log_interleave 0 the Basefold codeword of a-bar just computed is this same transform
+  // (both planes zero-padded to 2n and transformed over the order-2n subgroup), so reuse it.
   const int logN = log_n + 1;
-  DevBuf<uint64_t> ra_planar(pool, 2 * two_n);
+  DevBuf<uint64_t> ra_planar;
+  const uint64_t* ra_src;
+  if (av_state.li == 0 && av_state.D == two_n) {
+    ra_src = av_state.codes.ptr;
+  } else {
+    ra_planar = DevBuf<uint64_t>(pool, 2 * two_n);
+    hip_check(launch_ntt((const uint64_t*)a_vec.ptr, 1, 2, (uint32_t)n, 2, logN, ctx.twiddle(logN), ra_planar.ptr,
+                         st),
+              "RS(a-bar)");
+    ra_src = ra_planar.ptr;
+  }
   DevBuf<ef2> ra(pool, two_n);
-  hip_check(launch_ntt((const uint64_t*)a_vec.ptr, 1, 2, (uint32_t)n, 2, logN, ctx.twiddle(logN), ra_planar.ptr, st),
-            "RS(a-bar)");
-  hip_check(launch_planar_to_ef(ra_planar.ptr, ra_planar.ptr + two_n, two_n, ra.ptr, st), "RS(a-bar) layout");
+  hip_check(launch_planar_to_ef(ra_src, ra_src + two_n, two_n, ra.ptr, st), "RS(a-bar) layout");
 
   // ---- B_I and its commitment (ligesis.rs:402-416) ----
   const uint64_t nbi = W * s;
@@ -275,6 +285,86 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   hip_check(hipMemcpyAsync(bi_ef2.ptr, bi_ef.ptr, nbi * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
   DevBasefold bi_state;
   enqueue_basefold_commit(ctx, bi_state, bi.ptr, false, nv_bi, li_bi, 1);
+
+  // ---- Stage B: shared-domain batched Basefold open (basefold.rs:679-856), on a second stream ----
+  // It reads only the three Basefold commitments, the committed polynomials and fixed
+  // challenges, so it runs beside Stage A: fork here, join before the openings are gathered.
+  // Its scratch (eq tables, partial sums, counters) is its own.
+  const DevBasefold* states[3] = {&key.a_state, &bi_state, &av_state};
+  const uint64_t* polys[3] = {key.a.ptr, bi.ptr, (const uint64_t*)a_vec.ptr};
+  const uint64_t D = key.a_state.D;
+  for (int c = 0; c < 3; c++)
+    if (states[c]->D != D) throw Error(1, "prove: Basefold domains differ");
+  // levels 0..R stored back to back: level k has D >> k entries
+  std::vector<uint64_t> lvl_off(R + 1);
+  uint64_t tot = 0;
+  for (uint32_t k = 0; k <= R; k++) {
+    lvl_off[k] = tot;
+    tot += D >> k;
+  }
+
+  for (uint32_t t = 0; t < R; t++) (void)ctx.twiddle(ilog2(D >> t));  // tables made on st, before the fork
+  hipStream_t sb = ctx.side_stream();
+  hip_check(hipEventRecord(ctx.ev[5], st), "fork");
+  hip_check(hipStreamWaitEvent(sb, ctx.ev[5], 0), "fork");
+  DevBuf<ef2> eqs_b(pool, eq_scratch_n), part_b(pool, 3 * 1024 + 64);
+  DevBuf<uint64_t> counters_b(pool, 2);
+  hip_check(launch_zero(counters_b.ptr, 2, sb), "clear counters");
+  DevBuf<ef2> levels(pool, tot);
+  BfChannels chans{};
+  chans.count = 3;
+  for (int c = 0; c < 3; c++) {
+    chans.codes[c] = states[c]->codes.ptr;
+    chans.wt[c] = B + o_wt[c];
+    chans.chunks[c] = 1u << bli[c];
+    chans.ext[c] = states[c]->ext ? 1 : 0;
+  }
+  hip_check(launch_bf_combine(chans, D, levels.ptr, sb), "batch combine");
+  const uint64_t M = uint64_t(1) << R;
+  DevBuf<ef2> mbuf[3][2], eqr[3][2];
+  for (int c = 0; c < 3; c++) {
+    mbuf[c][0] = DevBuf<ef2>(pool, M);
+    mbuf[c][1] = DevBuf<ef2>(pool, M / 2 + 1);
+    eqr[c][0] = DevBuf<ef2>(pool, M / 2 + 1);
+    eqr[c][1] = DevBuf<ef2>(pool, M / 4 + 1);
+    hip_check(launch_fold_low(polys[c], states[c]->ext ? 1 : 0, (int)bli[c], M, B + o_eq4[c], mbuf[c][0].ptr, sb),
+              "batch fold");
+    if (R >= 1)
+      hip_check(launch_eq(B + o_zp1[c], (int)R - 1, eqs_b.ptr, nullptr, 0, eqr[c][0].ptr, sb), "batch eq");
+  }
+  for (uint32_t t = 0; t < R; t++) {
+    const uint64_t half = uint64_t(1) << (R - 1 - t);
+    BfRound br{};
+    for (int c = 0; c < 3; c++) {
+      br.m_in[c] = mbuf[c][t & 1].ptr;
+      br.m_out[c] = mbuf[c][(t + 1) & 1].ptr;
+      br.eq[c] = eqr[c][t & 1].ptr;
+      br.eq_next[c] = eqr[c][(t + 1) & 1].ptr;
+      br.out[c] = res.ptr + o_l0 + (size_t)c * R + t;
+    }
+    br.part = part_b.ptr;
+    br.counter = (unsigned int*)counters_b.ptr;
+    hip_check(launch_bf_round3(br, half, B + o_chbatch + t, sb), "batch round");
+    const uint64_t len = D >> t;
+    hip_check(launch_eq_fold_codeword(levels.ptr + lvl_off[t], len / 2, B + o_chbatch + t,
+                                      ctx.twiddle(ilog2(len)), levels.ptr + lvl_off[t + 1], sb),
+              "eq fold codeword");
+  }
+  // trees over the folded levels 1..R-1 (leaf = one F_p^2 value), built together
+  std::vector<DevTree> ftrees(R > 1 ? R - 1 : 0);
+  std::vector<MerkleSource> fsrc;
+  for (uint32_t k = 1; k < R; k++) {
+    DevTree& tr = ftrees[k - 1];
+    tr.leaves = D >> k;
+    tr.nodes = DevBuf<uint32_t>(pool, tr.node_count() * 8);
+    fsrc.push_back(MerkleSource{(const uint64_t*)(levels.ptr + lvl_off[k]), 2, 1, 2, tr.leaves, tr.nodes.ptr});
+  }
+  std::vector<std::vector<MerkleJob>> fjobs;
+  std::vector<DevBuf<uint64_t>> ftables;
+  enqueue_merkle_trees(ctx, fsrc, fjobs, ftables, sb);
+  hip_check(hipEventRecord(ctx.ev[6], sb), "join");
+
+
 
   // ---- input tables of the seven Stage A sumchecks ----
   // binary check (ligesis.rs:424-429)
@@ -380,22 +470,8 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
               "sumcheck round step");
   hip_check(launch_sc_tail_group(d_tjobs, 7, st), "sumcheck tails");
 
-  // ---- Stage B: shared-domain batched Basefold open (basefold.rs:679-856) ----
-  const DevBasefold* states[3] = {&key.a_state, &bi_state, &av_state};
-  const uint64_t* polys[3] = {key.a.ptr, bi.ptr, (const uint64_t*)a_vec.ptr};
-  const uint64_t D = key.a_state.D;
-  for (int c = 0; c < 3; c++)
-    if (states[c]->D != D) throw Error(1, "prove: Basefold domains differ");
-  // levels 0..R stored back to back: level k has D >> k entries
-  std::vector<uint64_t> lvl_off(R + 1);
-  uint64_t tot = 0;
-  for (uint32_t k = 0; k <= R; k++) {
-    lvl_off[k] = tot;
-    tot += D >> k;
-  }
-
   // Opening positions are fixed by the integer stream, so every index the proof opens is
-  // computed here on the host while the device works through the sumchecks.
+  // computed here on the host while the device works through both stages.
   std::vector<uint64_t> base;
   for (uint64_t q = 0; q < 100; q++) base.push_back(ints[4 * s + q] % (D / 2));
   std::sort(base.begin(), base.end());
@@ -430,62 +506,8 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     for (uint64_t p : bpos)
       for (uint32_t rr = 0; rr < rows; rr++) bvw[c].push_back((uint64_t)rr * D + p);
   }
-  DevBuf<ef2> levels(pool, tot);
-  BfChannels chans{};
-  chans.count = 3;
-  for (int c = 0; c < 3; c++) {
-    chans.codes[c] = states[c]->codes.ptr;
-    chans.wt[c] = B + o_wt[c];
-    chans.chunks[c] = 1u << bli[c];
-    chans.ext[c] = states[c]->ext ? 1 : 0;
-  }
-  hip_check(launch_bf_combine(chans, D, levels.ptr, st), "batch combine");
-  const uint64_t M = uint64_t(1) << R;
-  DevBuf<ef2> mbuf[3][2], eqr[3][2];
-  for (int c = 0; c < 3; c++) {
-    mbuf[c][0] = DevBuf<ef2>(pool, M);
-    mbuf[c][1] = DevBuf<ef2>(pool, M / 2 + 1);
-    eqr[c][0] = DevBuf<ef2>(pool, M / 2 + 1);
-    eqr[c][1] = DevBuf<ef2>(pool, M / 4 + 1);
-    hip_check(launch_fold_low(polys[c], states[c]->ext ? 1 : 0, (int)bli[c], M, B + o_eq4[c], mbuf[c][0].ptr, st),
-              "batch fold");
-    if (R >= 1)
-      hip_check(launch_eq(B + o_zp1[c], (int)R - 1, eqs.ptr, nullptr, 0, eqr[c][0].ptr, st), "batch eq");
-  }
-  const uint64_t* twD = ctx.twiddle(ilog2(D));
-  for (uint32_t t = 0; t < R; t++) {
-    const uint64_t half = uint64_t(1) << (R - 1 - t);
-    BfRound br{};
-    for (int c = 0; c < 3; c++) {
-      br.m_in[c] = mbuf[c][t & 1].ptr;
-      br.m_out[c] = mbuf[c][(t + 1) & 1].ptr;
-      br.eq[c] = eqr[c][t & 1].ptr;
-      br.eq_next[c] = eqr[c][(t + 1) & 1].ptr;
-      br.out[c] = res.ptr + o_l0 + (size_t)c * R + t;
-    }
-    br.part = part.ptr;
-    br.counter = counter;
-    hip_check(launch_bf_round3(br, half, B + o_chbatch + t, st), "batch round");
-    const uint64_t len = D >> t;
-    hip_check(launch_eq_fold_codeword(levels.ptr + lvl_off[t], len / 2, B + o_chbatch + t,
-                                      ctx.twiddle(ilog2(len)), levels.ptr + lvl_off[t + 1], st),
-              "eq fold codeword");
-  }
-  (void)twD;
-  // trees over the folded levels 1..R-1 (leaf = one F_p^2 value), built together
-  std::vector<DevTree> ftrees(R > 1 ? R - 1 : 0);
-  std::vector<MerkleSource> fsrc;
-  for (uint32_t k = 1; k < R; k++) {
-    DevTree& tr = ftrees[k - 1];
-    tr.leaves = D >> k;
-    tr.nodes = DevBuf<uint32_t>(pool, tr.node_count() * 8);
-    fsrc.push_back(MerkleSource{(const uint64_t*)(levels.ptr + lvl_off[k]), 2, 1, 2, tr.leaves, tr.nodes.ptr});
-  }
-  std::vector<std::vector<MerkleJob>> fjobs;
-  std::vector<DevBuf<uint64_t>> ftables;
-  enqueue_merkle_trees(ctx, fsrc, fjobs, ftables);
-
   // ---- openings ----
+  hip_check(hipStreamWaitEvent(st, ctx.ev[6], 0), "join");
   Gather g;
   const size_t g_hopen = g.add_nodes(cm.tree, hnodes);
   const size_t g_hcols = g.add(cm.h.ptr, hw);

@@ -178,6 +178,7 @@ struct Context {
   void* owner = nullptr;  // the lgs_ctx that embeds this context
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // second stream (default priority) for work that overlaps `stream`
   std::string err;
   DevicePool pool;
   std::map<int, DevBuf<uint64_t>> twiddles;
@@ -197,6 +198,7 @@ struct Context {
 
   const uint64_t* twiddle(int logN);
   uint8_t* staging(size_t bytes);
+  hipStream_t side_stream();  // stream2, created on first use
   // launch schedule of a commit of shape 2^log_m x 2^log_n
   Tuning resolve(uint32_t log_m, uint32_t log_n) const;
 };
@@ -250,7 +252,8 @@ struct MerkleSource {
   uint32_t* nodes;
 };
 void enqueue_merkle_trees(Context& ctx, const std::vector<MerkleSource>& trees,
-                          std::vector<std::vector<MerkleJob>>& host_jobs, std::vector<DevBuf<uint64_t>>& tables);
+                          std::vector<std::vector<MerkleJob>>& host_jobs, std::vector<DevBuf<uint64_t>>& tables,
+                          hipStream_t st);
 void enqueue_basefold_commit(Context& ctx, DevBasefold& st, const uint64_t* poly, bool ext,
                              uint32_t num_vars, uint32_t li, uint32_t code_rate);
 void setup_key(Context& ctx, Key& key, const uint64_t* host_a, bool commit_a);

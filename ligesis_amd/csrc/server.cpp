@@ -132,8 +132,9 @@ void Server::commit_loop() {
                   "read root");
       }
       hip_check(hipStreamSynchronize(ctx.stream), "commit sync");
-    } catch (const Error& e) {
-      rc = e.code;
+    } catch (const std::exception& e) {
+      const Error* le = dynamic_cast<const Error*>(&e);
+      rc = le ? le->code : (dynamic_cast<const std::bad_alloc*>(&e) ? 4 : 1);
       msg = e.what();
       (void)hipStreamSynchronize(ctx.stream);
     }
@@ -177,10 +178,12 @@ void Server::prove_loop(Context* ctx) {
       uint64_t used[2];
       prove_impl(*ctx, *r->key, *r->cm, r->z.data(), r->z.size() / 2, r->fields.data(), r->fields.size() / 2,
                  r->ints.data(), r->ints.size(), proof, used);
-    } catch (const Error& e) {
-      rc = e.code;
+    } catch (const std::exception& e) {
+      const Error* le = dynamic_cast<const Error*>(&e);
+      rc = le ? le->code : (dynamic_cast<const std::bad_alloc*>(&e) ? 4 : 1);
       msg = e.what();
       (void)hipStreamSynchronize(ctx->stream);
+      if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     }
     r->cm.reset();  // f, F', H and the tree go back to the commit context's pool
     std::lock_guard<std::mutex> lk(mu);
@@ -286,6 +289,7 @@ void server_stop(Server* s) {
   s->all.clear();
   for (auto& c : s->ctxs) {
     (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     c->twiddles.clear();
     c->pool.trim();
     for (auto& e : c->ev)
@@ -293,6 +297,7 @@ void server_stop(Server* s) {
     if (c->pinned) (void)hipHostFree(c->pinned);
     c->pinned = nullptr;
     (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
   }
   delete s;
 }
