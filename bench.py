@@ -320,6 +320,12 @@ def bench_prove(args):
     assert len(proofs) == 1, "prove is not deterministic"
     proof = next(iter(proofs))
     ms = dt / args.steps * 1e3
+    # one more proof with host-side timing: how long issuing the work takes vs the whole proof
+    ctx.set_option("profile", 1)
+    L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints))
+    raw = np.zeros(7, np.float64)
+    ctx.check(L.lib().lgs_stage_times(ctx.h, L._ptr(raw), 7))
+    ctx.set_option("profile", 0)
     out = {
         "metric": "LigeSIS eval-prove ms at mu=24",
         "value": ms,
@@ -335,6 +341,8 @@ def bench_prove(args):
         "data": "synthetic (SplitMix64-seeded f, A, z and oracle streams)",
         "config": {"workload": f"LigeSIS eval-prove, mu=24, m x n = 2^{lm} x 2^{ln}, s=128, "
                                "100 queries, proof on a resident commitment", "log_m": lm, "log_n": ln},
+        "host_issue_ms": float(raw[5]),
+        "device_bound_ms": float(raw[6]),
         "proof_bytes": len(proof),
     }
     del cd, pk

@@ -330,12 +330,17 @@ class Ligesis:
         used = np.zeros(2, np.uint64)
         args = (ctx.h, pk.h, data.h, _ptr(z), z.shape[0], _ptr(oracle.fields), oracle.fields.shape[0],
                 _ptr(oracle.ints), oracle.ints.size)
-        out = np.zeros(4 << 20, np.uint8)
+        # proofs of one key have one size: keep the buffer the last proof needed
+        out = getattr(pk, "_proof_buf", None)
+        if out is None:
+            out = np.empty(4 << 20, np.uint8)
         cap = ctypes.c_uint64(out.size)
         rc = lib().lgs_prove(*args, _ptr(out), ctypes.byref(cap), _ptr(used))
         if rc == LGS_ERR_BUFFER:  # larger than the first guess: retry with the reported size
-            out = np.zeros(cap.value, np.uint8)
+            out = np.empty(cap.value, np.uint8)
             rc = lib().lgs_prove(*args, _ptr(out), ctypes.byref(cap), _ptr(used))
+        if rc == LGS_OK:
+            pk._proof_buf = out
         ctx.check(rc)
         oracle.fields_idx, oracle.ints_idx = int(used[0]), int(used[1])
         return out[: cap.value].tobytes()

@@ -5,6 +5,7 @@
 // every evaluation point from them, and the device work is enqueued back to back. Proof values
 // are gathered on the device into one staging buffer, copied to the host once, and encoded.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <vector>
 
@@ -101,13 +102,16 @@ struct Gather {
   size_t add_root(const DevTree& t) { return add_nodes(t, {t.node_count() - 1}); }
 };
 
+static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "proof encoding copies host words as little-endian");
+
 struct Writer {
   std::vector<uint8_t> b;
-  void u64(uint64_t v) {
-    for (int i = 0; i < 8; i++) b.push_back((uint8_t)(v >> (8 * i)));
-  }
+  // little-endian words; the host is little-endian (x86-64), so words are copied as they are
+  void u64(uint64_t v) { words(&v, 1); }
   void words(const uint64_t* w, size_t n) {
-    for (size_t i = 0; i < n; i++) u64(w[i]);
+    const size_t at = b.size();
+    b.resize(at + 8 * n);
+    std::memcpy(b.data() + at, w, 8 * n);
   }
   void raw(const uint64_t* w, size_t n) {  // n words as raw little-endian bytes (digests)
     words(w, n);
@@ -128,6 +132,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
                    std::vector<uint8_t>& proof, uint64_t used[2]) {
   if (!key.has_state) throw Error(7, "prove: the key was built commit-only (no A-bar commitment)");
   if (cm.key != &key) throw Error(1, "prove: commitment was made with another key");
+  const auto t_start = std::chrono::steady_clock::now();
   const uint32_t log_m = key.log_m, log_n = key.log_n;
   const uint64_t m = uint64_t(1) << log_m, n = uint64_t(1) << log_n, two_n = 2 * n;
   const uint64_t W = key.W, s = 128;
@@ -234,26 +239,108 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   res_n += 3 * (size_t)R;
   DevBuf<ef2> res(pool, res_n);
 
-  const uint32_t kmax = std::max(std::max(Lb, log_n), log_m + 1);
-  uint64_t eq_scratch_n = (uint64_t(1) << (kmax / 2)) + (uint64_t(1) << (kmax - kmax / 2)) + 64;
-  DevBuf<ef2> eqs(pool, eq_scratch_n);
   DevBuf<ef2> part(pool, 1024 * 4 + 256);
   // completion counters of the kernels that reduce their own partial sums; each kernel leaves
   // its counter at zero again, so one clear serves the whole prove
   DevBuf<uint64_t> counters(pool, 2);
   hip_check(launch_zero(counters.ptr, 2, st), "clear counters");
   unsigned int* counter = (unsigned int*)counters.ptr;
+  // Every buffer below lives until the end of the prove: the second stream runs beside this
+  // one, so no buffer may go back to the pool (and be handed out again) while work is queued.
 
-  // ---- a-bar = eq_{z1}^T F, y = a-bar(z2) (ligesis.rs:365-379) ----
-  DevBuf<ef2> eqz1(pool, m), eqz2(pool, n), a_vec(pool, n);
-  hip_check(launch_eq(B + o_z1, log_m, eqs.ptr, nullptr, 0, eqz1.ptr, st), "eq z1");
-  hip_check(launch_eq(B + o_z2, log_n, eqs.ptr, nullptr, 0, eqz2.ptr, st), "eq z2");
-  {
-    uint64_t chunks = std::max<uint64_t>(1, std::min<uint64_t>(m, (uint64_t(1) << 21) / n));
-    while (m % chunks) chunks >>= 1;
-    DevBuf<ef2> wpart(pool, chunks * n);
-    hip_check(launch_wrows(cm.f, (uint32_t)m, n, eqz1.ptr, wpart.ptr, (uint32_t)chunks, a_vec.ptr, st), "a-bar");
+  // ---- every eq table of the proof, in two launches (MlPoly::new_eq, poly.rs:44-53) ----
+  const uint64_t nbi = W * s;
+  const uint64_t M = uint64_t(1) << R;
+  DevBuf<ef2> eqz1(pool, m), eqz2(pool, n), big(pool, nbi), big2(pool, nbi), eqa2(pool, 32), eqr2(pool, s),
+      g_a(pool, n);
+  DevBuf<ef2> mbuf[3][2], eqr[3][2];
+  for (int c = 0; c < 3; c++) {
+    mbuf[c][0] = DevBuf<ef2>(pool, M);
+    mbuf[c][1] = DevBuf<ef2>(pool, M / 2 + 1);
+    eqr[c][0] = DevBuf<ef2>(pool, M / 2 + 1);
+    eqr[c][1] = DevBuf<ef2>(pool, M / 4 + 1);
   }
+  std::vector<EqTerm> eq_terms;
+  std::vector<EqTable> eq_tabs;
+  uint64_t eq_half_words = 0;
+  {
+    // a table: out (2^k entries) = sum over (point, scale) terms of scale * eq(point, .)
+    struct Want {
+      ef2* out;
+      uint32_t k;
+      std::vector<std::pair<size_t, const ef2*>> terms;  // blob offset of the point, scale
+    };
+    std::vector<Want> want = {
+        {eqz1.ptr, log_m, {{o_z1, nullptr}}},
+        {eqz2.ptr, log_n, {{o_z2, nullptr}}},
+        {big.ptr, Lb, {{o_alpha1, nullptr}}},                       // binary check (ligesis.rs:424-429)
+        {eqa2.ptr, 5, {{o_alpha2, nullptr}}},                       // hash channel (ligesis.rs:436-444)
+        {eqr2.ptr, log_s, {{o_r2, nullptr}}},
+        {big2.ptr, Lb, {{o_r1, B + o_gbi}, {o_pbh, B + o_gbi + 1}, {o_pbr, B + o_gbi + 2}}},  // ligesis.rs:513-517
+        {g_a.ptr, log_n, {{o_z2, B + o_ga}, {o_r6, B + o_ga + 1}}},                          // ligesis.rs:518-522
+    };
+    for (int c = 0; c < 3; c++)
+      if (R >= 1) want.push_back({eqr[c][0].ptr, R - 1, {{o_zp1[c], nullptr}}});  // Basefold rounds
+    for (auto& w : want) {
+      const uint32_t klo = w.k / 2, khi = w.k - klo;
+      EqTable tb{};
+      tb.out = w.out;
+      tb.n = uint64_t(1) << w.k;
+      tb.klo = klo;
+      tb.nterms = (uint32_t)w.terms.size();
+      for (size_t q = 0; q < w.terms.size(); q++) {
+        EqTerm t{};
+        t.pt = B + w.terms[q].first;
+        t.scale = w.terms[q].second;
+        t.k = w.k;
+        t.klo = klo;
+        t.lo = (ef2*)(uintptr_t)eq_half_words;  // offsets into the half-table scratch for now
+        eq_half_words += uint64_t(1) << klo;
+        t.hi = (ef2*)(uintptr_t)eq_half_words;
+        eq_half_words += uint64_t(1) << khi;
+        eq_terms.push_back(t);
+      }
+      eq_tabs.push_back(tb);
+    }
+  }
+  DevBuf<ef2> eq_half(pool, eq_half_words);
+  uint32_t eq_term_blocks = 0, eq_tab_blocks = 0;
+  {
+    size_t ti = 0;
+    for (auto& tb : eq_tabs) {
+      for (uint32_t q = 0; q < tb.nterms; q++, ti++) {
+        EqTerm& t = eq_terms[ti];
+        t.lo = eq_half.ptr + (uintptr_t)t.lo;
+        t.hi = eq_half.ptr + (uintptr_t)t.hi;
+        t.first_block = eq_term_blocks;
+        eq_term_blocks += (uint32_t)(((uint64_t(1) << std::max(t.klo, t.k - t.klo)) + 255) / 256);
+        tb.lo[q] = t.lo;
+        tb.hi[q] = t.hi;
+      }
+      tb.first_block = eq_tab_blocks;
+      eq_tab_blocks += (uint32_t)((tb.n + 255) / 256);
+    }
+  }
+  DevBuf<uint8_t> eq_jobs(pool, eq_terms.size() * sizeof(EqTerm) + eq_tabs.size() * sizeof(EqTable));
+  hip_check(hipMemcpyAsync(eq_jobs.ptr, eq_terms.data(), eq_terms.size() * sizeof(EqTerm), hipMemcpyHostToDevice, st),
+            "eq terms");
+  hip_check(hipMemcpyAsync(eq_jobs.ptr + eq_terms.size() * sizeof(EqTerm), eq_tabs.data(),
+                           eq_tabs.size() * sizeof(EqTable), hipMemcpyHostToDevice, st),
+            "eq tables");
+  hip_check(launch_eq_group((const EqTerm*)eq_jobs.ptr, (int)eq_terms.size(), eq_term_blocks,
+                            (const EqTable*)(eq_jobs.ptr + eq_terms.size() * sizeof(EqTerm)), (int)eq_tabs.size(),
+                            eq_tab_blocks, st),
+            "eq tables");
+
+  // ---- a-bar = eq_{z1}^T F, y = a-bar(z2) (ligesis.rs:365-379); two copies for the sumchecks
+  //      that fold a-bar in place ----
+  DevBuf<ef2> a_vec(pool, n), a_copy(pool, n), a_copy2(pool, n);
+  uint64_t chunks = std::max<uint64_t>(1, std::min<uint64_t>(m, (uint64_t(1) << 21) / n));
+  while (m % chunks) chunks >>= 1;
+  DevBuf<ef2> wpart(pool, chunks * n), wpart_g1(pool, W);
+  hip_check(launch_wrows(cm.f, (uint32_t)m, n, eqz1.ptr, wpart.ptr, (uint32_t)chunks, a_vec.ptr, st, a_copy.ptr,
+                         a_copy2.ptr),
+            "a-bar");
   hip_check(launch_dot(a_vec.ptr, eqz2.ptr, n, part.ptr, counter, res.ptr + o_y, st), "y");
   DevBasefold av_state;
   enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec.ptr, true, log_n, li_a, 1);
@@ -277,125 +364,39 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   hip_check(launch_planar_to_ef(ra_src, ra_src + two_n, two_n, ra.ptr, st), "RS(a-bar) layout");
 
   // ---- B_I and its commitment (ligesis.rs:402-416) ----
-  const uint64_t nbi = W * s;
   DevBuf<uint64_t> bi(pool, nbi);
-  DevBuf<ef2> bi_ef(pool, nbi), bi_m1(pool, nbi), bi_ef2(pool, nbi), big(pool, nbi), big2(pool, nbi);
-  hip_check(launch_bits_gather(cm.enc.ptr, two_n, didx.ptr, (uint32_t)s, W, bi.ptr, bi_ef.ptr, bi_m1.ptr, st),
+  DevBuf<ef2> bi_ef(pool, nbi), bi_m1(pool, nbi), bi_ef2(pool, nbi);
+  hip_check(launch_bits_gather(cm.enc.ptr, two_n, didx.ptr, (uint32_t)s, W, bi.ptr, bi_ef.ptr, bi_m1.ptr, st,
+                               bi_ef2.ptr),
             "B_I gather");
-  hip_check(hipMemcpyAsync(bi_ef2.ptr, bi_ef.ptr, nbi * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
   DevBasefold bi_state;
   enqueue_basefold_commit(ctx, bi_state, bi.ptr, false, nv_bi, li_bi, 1);
 
-  // ---- Stage B: shared-domain batched Basefold open (basefold.rs:679-856), on a second stream ----
-  // It reads only the three Basefold commitments, the committed polynomials and fixed
-  // challenges, so it runs beside Stage A: fork here, join before the openings are gathered.
-  // Its scratch (eq tables, partial sums, counters) is its own.
+  // Stage B (below) reads only the three Basefold commitments, the committed polynomials, its eq
+  // tables and fixed challenges: it forks here onto the second stream and joins before the
+  // openings are gathered.
   const DevBasefold* states[3] = {&key.a_state, &bi_state, &av_state};
   const uint64_t* polys[3] = {key.a.ptr, bi.ptr, (const uint64_t*)a_vec.ptr};
   const uint64_t D = key.a_state.D;
   for (int c = 0; c < 3; c++)
     if (states[c]->D != D) throw Error(1, "prove: Basefold domains differ");
-  // levels 0..R stored back to back: level k has D >> k entries
-  std::vector<uint64_t> lvl_off(R + 1);
-  uint64_t tot = 0;
-  for (uint32_t k = 0; k <= R; k++) {
-    lvl_off[k] = tot;
-    tot += D >> k;
-  }
-
   for (uint32_t t = 0; t < R; t++) (void)ctx.twiddle(ilog2(D >> t));  // tables made on st, before the fork
   hipStream_t sb = ctx.side_stream();
   hip_check(hipEventRecord(ctx.ev[5], st), "fork");
-  hip_check(hipStreamWaitEvent(sb, ctx.ev[5], 0), "fork");
-  DevBuf<ef2> eqs_b(pool, eq_scratch_n), part_b(pool, 3 * 1024 + 64);
-  DevBuf<uint64_t> counters_b(pool, 2);
-  hip_check(launch_zero(counters_b.ptr, 2, sb), "clear counters");
-  DevBuf<ef2> levels(pool, tot);
-  BfChannels chans{};
-  chans.count = 3;
-  for (int c = 0; c < 3; c++) {
-    chans.codes[c] = states[c]->codes.ptr;
-    chans.wt[c] = B + o_wt[c];
-    chans.chunks[c] = 1u << bli[c];
-    chans.ext[c] = states[c]->ext ? 1 : 0;
-  }
-  hip_check(launch_bf_combine(chans, D, levels.ptr, sb), "batch combine");
-  const uint64_t M = uint64_t(1) << R;
-  DevBuf<ef2> mbuf[3][2], eqr[3][2];
-  for (int c = 0; c < 3; c++) {
-    mbuf[c][0] = DevBuf<ef2>(pool, M);
-    mbuf[c][1] = DevBuf<ef2>(pool, M / 2 + 1);
-    eqr[c][0] = DevBuf<ef2>(pool, M / 2 + 1);
-    eqr[c][1] = DevBuf<ef2>(pool, M / 4 + 1);
-    hip_check(launch_fold_low(polys[c], states[c]->ext ? 1 : 0, (int)bli[c], M, B + o_eq4[c], mbuf[c][0].ptr, sb),
-              "batch fold");
-    if (R >= 1)
-      hip_check(launch_eq(B + o_zp1[c], (int)R - 1, eqs_b.ptr, nullptr, 0, eqr[c][0].ptr, sb), "batch eq");
-  }
-  for (uint32_t t = 0; t < R; t++) {
-    const uint64_t half = uint64_t(1) << (R - 1 - t);
-    BfRound br{};
-    for (int c = 0; c < 3; c++) {
-      br.m_in[c] = mbuf[c][t & 1].ptr;
-      br.m_out[c] = mbuf[c][(t + 1) & 1].ptr;
-      br.eq[c] = eqr[c][t & 1].ptr;
-      br.eq_next[c] = eqr[c][(t + 1) & 1].ptr;
-      br.out[c] = res.ptr + o_l0 + (size_t)c * R + t;
-    }
-    br.part = part_b.ptr;
-    br.counter = (unsigned int*)counters_b.ptr;
-    hip_check(launch_bf_round3(br, half, B + o_chbatch + t, sb), "batch round");
-    const uint64_t len = D >> t;
-    hip_check(launch_eq_fold_codeword(levels.ptr + lvl_off[t], len / 2, B + o_chbatch + t,
-                                      ctx.twiddle(ilog2(len)), levels.ptr + lvl_off[t + 1], sb),
-              "eq fold codeword");
-  }
-  // trees over the folded levels 1..R-1 (leaf = one F_p^2 value), built together
-  std::vector<DevTree> ftrees(R > 1 ? R - 1 : 0);
-  std::vector<MerkleSource> fsrc;
-  for (uint32_t k = 1; k < R; k++) {
-    DevTree& tr = ftrees[k - 1];
-    tr.leaves = D >> k;
-    tr.nodes = DevBuf<uint32_t>(pool, tr.node_count() * 8);
-    fsrc.push_back(MerkleSource{(const uint64_t*)(levels.ptr + lvl_off[k]), 2, 1, 2, tr.leaves, tr.nodes.ptr});
-  }
-  std::vector<std::vector<MerkleJob>> fjobs;
-  std::vector<DevBuf<uint64_t>> ftables;
-  enqueue_merkle_trees(ctx, fsrc, fjobs, ftables, sb);
-  hip_check(hipEventRecord(ctx.ev[6], sb), "join");
-
-
 
   // ---- input tables of the seven Stage A sumchecks ----
-  // binary check (ligesis.rs:424-429)
-  hip_check(launch_eq(B + o_alpha1, Lb, eqs.ptr, nullptr, 0, big.ptr, st), "eq alpha1");
   // hash / recon channels (ligesis.rs:432-485)
-  DevBuf<ef2> eqa2(pool, 32), eqr2(pool, s), g1(pool, W), g2(pool, W), g2b(pool, W), vv(pool, W);
-  hip_check(launch_eq(B + o_alpha2, 5, eqs.ptr, nullptr, 0, eqa2.ptr, st), "eq alpha2");
-  hip_check(launch_eq(B + o_r2, log_s, eqs.ptr, nullptr, 0, eqr2.ptr, st), "eq r2");
-  {
-    DevBuf<ef2> wpart(pool, W);
-    hip_check(launch_wrows(key.a.ptr, 32, W, eqa2.ptr, wpart.ptr, 1, g1.ptr, st), "g1");
-  }
-  hip_check(launch_g2(bi.ptr, (uint32_t)s, W, eqr2.ptr, g2.ptr, st), "g2");
-  hip_check(hipMemcpyAsync(g2b.ptr, g2.ptr, W * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
+  DevBuf<ef2> g1(pool, W), g2(pool, W), g2b(pool, W), vv(pool, W);
+  hip_check(launch_wrows(key.a.ptr, 32, W, eqa2.ptr, wpart_g1.ptr, 1, g1.ptr, st), "g1");
+  hip_check(launch_g2(bi.ptr, (uint32_t)s, W, eqr2.ptr, g2.ptr, st, g2b.ptr), "g2");
   DevBuf<ef2> sel(pool, two_n);
   hip_check(launch_sel(didx.ptr, eqr2.ptr, (uint32_t)s, sel.ptr, two_n, st), "sel");
   hip_check(launch_dot(sel.ptr, ra.ptr, two_n, part.ptr, counter, res.ptr + o_y4, st), "y4");
   hip_check(launch_vvec(eqz1.ptr, W, vv.ptr, st), "v");
   // recon table and RS link (ligesis.rs:488-502); y5 = ra(r3) is sc_table's final evaluation of
   // ra, which is what the reference's verify compares (ligesis.rs:681)
-  DevBuf<ef2> wv(pool, n), a_copy(pool, n), a_copy2(pool, n);
+  DevBuf<ef2> wv(pool, n);
   hip_check(launch_rs_weights(B + o_r3, logN, n, ctx.twiddle(logN), wv.ptr, st), "rs weights");
-  hip_check(hipMemcpyAsync(a_copy.ptr, a_vec.ptr, n * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
-  // Stage A batching (ligesis.rs:513-522)
-  hip_check(launch_eq(B + o_r1, Lb, eqs.ptr, B + o_gbi + 0, 0, big2.ptr, st), "combine_eq");
-  hip_check(launch_eq(B + o_pbh, Lb, eqs.ptr, B + o_gbi + 1, 1, big2.ptr, st), "combine_eq");
-  hip_check(launch_eq(B + o_pbr, Lb, eqs.ptr, B + o_gbi + 2, 1, big2.ptr, st), "combine_eq");
-  DevBuf<ef2> g_a(pool, n);
-  hip_check(launch_eq(B + o_z2, log_n, eqs.ptr, B + o_ga + 0, 0, g_a.ptr, st), "combine_eq");
-  hip_check(launch_eq(B + o_r6, log_n, eqs.ptr, B + o_ga + 1, 1, g_a.ptr, st), "combine_eq");
-  hip_check(hipMemcpyAsync(a_copy2.ptr, a_vec.ptr, n * sizeof(ef2), hipMemcpyDeviceToDevice, st), "copy");
 
   // ---- the seven sumchecks, in lockstep ----
   // Every challenge is fixed before the first round (the oracle streams are inputs), so the
@@ -469,6 +470,67 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
                                     (int)(step_first[tau + 1] - step_first[tau]), step_blocks[tau], st),
               "sumcheck round step");
   hip_check(launch_sc_tail_group(d_tjobs, 7, st), "sumcheck tails");
+
+  // ---- Stage B: shared-domain batched Basefold open (basefold.rs:679-856), second stream ----
+  // Issued after Stage A so that the host queues the longer stage first; its scratch (partial
+  // sums, counters) is its own.
+  hip_check(hipStreamWaitEvent(sb, ctx.ev[5], 0), "fork");
+  // levels 0..R stored back to back: level k has D >> k entries
+  std::vector<uint64_t> lvl_off(R + 1);
+  uint64_t tot = 0;
+  for (uint32_t k = 0; k <= R; k++) {
+    lvl_off[k] = tot;
+    tot += D >> k;
+  }
+  DevBuf<ef2> part_b(pool, 3 * 1024 + 64);
+  DevBuf<uint64_t> counters_b(pool, 2);
+  hip_check(launch_zero(counters_b.ptr, 2, sb), "clear counters");
+  DevBuf<ef2> levels(pool, tot);
+  BfChannels chans{};
+  chans.count = 3;
+  for (int c = 0; c < 3; c++) {
+    chans.codes[c] = states[c]->codes.ptr;
+    chans.wt[c] = B + o_wt[c];
+    chans.chunks[c] = 1u << bli[c];
+    chans.ext[c] = states[c]->ext ? 1 : 0;
+  }
+  hip_check(launch_bf_combine(chans, D, levels.ptr, sb), "batch combine");
+  for (int c = 0; c < 3; c++)
+    hip_check(launch_fold_low(polys[c], states[c]->ext ? 1 : 0, (int)bli[c], M, B + o_eq4[c], mbuf[c][0].ptr, sb),
+              "batch fold");
+  for (uint32_t t = 0; t < R; t++) {
+    const uint64_t half = uint64_t(1) << (R - 1 - t);
+    BfRound br{};
+    for (int c = 0; c < 3; c++) {
+      br.m_in[c] = mbuf[c][t & 1].ptr;
+      br.m_out[c] = mbuf[c][(t + 1) & 1].ptr;
+      br.eq[c] = eqr[c][t & 1].ptr;
+      br.eq_next[c] = eqr[c][(t + 1) & 1].ptr;
+      br.out[c] = res.ptr + o_l0 + (size_t)c * R + t;
+    }
+    br.part = part_b.ptr;
+    br.counter = (unsigned int*)counters_b.ptr;
+    const uint64_t len = D >> t;
+    br.cw_in = levels.ptr + lvl_off[t];
+    br.cw_half = len / 2;
+    br.cw_tw = ctx.twiddle(ilog2(len));
+    br.cw_out = levels.ptr + lvl_off[t + 1];
+    hip_check(launch_bf_round3(br, half, B + o_chbatch + t, sb), "batch round");
+  }
+  // trees over the folded levels 1..R-1 (leaf = one F_p^2 value), built together
+  std::vector<DevTree> ftrees(R > 1 ? R - 1 : 0);
+  std::vector<MerkleSource> fsrc;
+  for (uint32_t k = 1; k < R; k++) {
+    DevTree& tr = ftrees[k - 1];
+    tr.leaves = D >> k;
+    tr.nodes = DevBuf<uint32_t>(pool, tr.node_count() * 8);
+    fsrc.push_back(MerkleSource{(const uint64_t*)(levels.ptr + lvl_off[k]), 2, 1, 2, tr.leaves, tr.nodes.ptr});
+  }
+  std::vector<std::vector<MerkleJob>> fjobs;
+  std::vector<DevBuf<uint64_t>> ftables;
+  enqueue_merkle_trees(ctx, fsrc, fjobs, ftables, sb);
+  hip_check(hipEventRecord(ctx.ev[6], sb), "join");
+
 
   // Opening positions are fixed by the integer stream, so every index the proof opens is
   // computed here on the host while the device works through both stages.
@@ -544,12 +606,18 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
             "gather");
   uint64_t* host = (uint64_t*)ctx.staging(stage_words * 8);
   hip_check(hipMemcpyAsync(host, dstage.ptr, stage_words * 8, hipMemcpyDeviceToHost, st), "download proof");
+  const auto t_enqueued = std::chrono::steady_clock::now();
   hip_check(hipStreamSynchronize(st), "prove sync");
+  if (ctx.profile) {  // host time spent issuing work, and the whole device-bound part
+    ctx.stage_ms[5] = std::chrono::duration<double, std::milli>(t_enqueued - t_start).count();
+    ctx.stage_ms[6] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  }
   const uint64_t* RS = host;                 // results as words
   const uint64_t* GW = host + 2 * res_n;     // gathered words
 
   // ---- encode (field order of LigesisProof, ligesis.rs:99-119) ----
   Writer w;
+  w.b.reserve(8 * (stage_words + 4096));
   w.ef(RS + 2 * o_y);
   w.raw(GW + g_com_av, 4);
   w.raw(GW + g_com_i, 4);

@@ -67,45 +67,51 @@ __device__ __forceinline__ void last_block_reduce(const ef2* part, unsigned int*
 }
 
 // ---------------- eq tables (MlPoly::new_eq, poly.rs:44-53) ----------------
-// lo[b] = prod_{t < klo} (bit t of b ? p_t : 1 - p_t), hi likewise over p_{klo..k}.
-__global__ void eq_parts_kernel(const ef2* __restrict__ pt, int k, int klo, ef2* __restrict__ lo,
-                                ef2* __restrict__ hi) {
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  const int khi = k - klo;
+// Many eq tables in two launches. Parts: one job per (point, half) pair of every term; expand:
+// out[b] = sum_t scale_t * lo_t[b mod 2^klo] * hi_t[b >> klo] over the table's terms (a table
+// with several terms is a random linear combination of eq tables, ligesis.rs:513-522).
+__global__ void eq_parts_group_kernel(const EqTerm* __restrict__ terms, int nterms) {
+  int lo = 0, hi = nterms - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (terms[mid].first_block <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const EqTerm& t = terms[lo];
+  const uint32_t b = (blockIdx.x - t.first_block) * blockDim.x + threadIdx.x;
+  const int klo = (int)t.klo, khi = (int)(t.k - t.klo);
   if (b < (1u << klo)) {
     ef2 acc = ef_one();
-    for (int t = 0; t < klo; t++) acc = ef_mul(acc, ((b >> t) & 1) ? pt[t] : ef_one_minus(pt[t]));
-    lo[b] = acc;
+    for (int q = 0; q < klo; q++) acc = ef_mul(acc, ((b >> q) & 1) ? t.pt[q] : ef_one_minus(t.pt[q]));
+    t.lo[b] = acc;
   }
   if (b < (1u << khi)) {
-    ef2 acc = ef_one();
-    for (int t = 0; t < khi; t++)
-      acc = ef_mul(acc, ((b >> t) & 1) ? pt[klo + t] : ef_one_minus(pt[klo + t]));
-    hi[b] = acc;
+    ef2 acc = t.scale ? *t.scale : ef_one();
+    for (int q = 0; q < khi; q++) acc = ef_mul(acc, ((b >> q) & 1) ? t.pt[klo + q] : ef_one_minus(t.pt[klo + q]));
+    t.hi[b] = acc;
   }
 }
 
-// out[b] (+)= scale * lo[b mod 2^klo] * hi[b >> klo]
-__global__ void eq_expand_kernel(const ef2* __restrict__ lo, const ef2* __restrict__ hi, int klo,
-                                 uint64_t n, const ef2* __restrict__ scale, int accumulate,
-                                 ef2* __restrict__ out) {
-  const uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n) return;
-  ef2 v = ef_mul(lo[b & ((1u << klo) - 1)], hi[b >> klo]);
-  if (scale) v = ef_mul(v, *scale);
-  out[b] = accumulate ? ef_add(out[b], v) : v;
+__global__ void eq_expand_group_kernel(const EqTable* __restrict__ tabs, int ntabs) {
+  int lo = 0, hi = ntabs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (tabs[mid].first_block <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const EqTable& t = tabs[lo];
+  const uint64_t b = (uint64_t)(blockIdx.x - t.first_block) * blockDim.x + threadIdx.x;
+  if (b >= t.n) return;
+  const uint64_t bl = b & ((uint64_t(1) << t.klo) - 1), bh = b >> t.klo;
+  ef2 v = ef_mul(t.lo[0][bl], t.hi[0][bh]);
+  for (uint32_t q = 1; q < t.nterms; q++) v = ef_add(v, ef_mul(t.lo[q][bl], t.hi[q][bh]));
+  t.out[b] = v;
 }
 
-hipError_t launch_eq(const ef2* pt, int k, ef2* scratch /* >= 2^(k/2+1) + 2^(k - k/2) */,
-                     const ef2* scale, int accumulate, ef2* out, hipStream_t st) {
-  const int klo = k / 2, khi = k - klo;
-  ef2* lo = scratch;
-  ef2* hi = scratch + (1u << klo);
-  uint32_t nb = ((1u << (klo > khi ? klo : khi)) + 255) / 256;
-  hipLaunchKernelGGL(eq_parts_kernel, dim3(nb), dim3(256), 0, st, pt, k, klo, lo, hi);
-  uint64_t n = uint64_t(1) << k;
-  hipLaunchKernelGGL(eq_expand_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, lo, hi, klo,
-                     n, scale, accumulate, out);
+hipError_t launch_eq_group(const EqTerm* terms, int nterms, uint32_t term_blocks, const EqTable* tabs, int ntabs,
+                           uint32_t tab_blocks, hipStream_t st) {
+  if (term_blocks) hipLaunchKernelGGL(eq_parts_group_kernel, dim3(term_blocks), dim3(256), 0, st, terms, nterms);
+  if (tab_blocks) hipLaunchKernelGGL(eq_expand_group_kernel, dim3(tab_blocks), dim3(256), 0, st, tabs, ntabs);
   return hipGetLastError();
 }
 
@@ -121,20 +127,23 @@ __global__ void wrows_kernel(const uint64_t* __restrict__ M, uint64_t cols, cons
   part[(uint64_t)blockIdx.y * cols + j] = acc;
 }
 
+// out (and the optional copies out2, out3, which later kernels fold in place) = sum of chunks
 __global__ void sum_chunks_kernel(const ef2* __restrict__ part, uint32_t chunks, uint64_t cols,
-                                  ef2* __restrict__ out) {
+                                  ef2* __restrict__ out, ef2* __restrict__ out2, ef2* __restrict__ out3) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= cols) return;
   ef2 acc = ef_zero();
   for (uint32_t c = 0; c < chunks; c++) acc = ef_add(acc, part[(uint64_t)c * cols + j]);
   out[j] = acc;
+  if (out2) out2[j] = acc;
+  if (out3) out3[j] = acc;
 }
 
 hipError_t launch_wrows(const uint64_t* M, uint32_t rows, uint64_t cols, const ef2* w, ef2* part,
-                        uint32_t chunks, ef2* out, hipStream_t st) {
+                        uint32_t chunks, ef2* out, hipStream_t st, ef2* out2, ef2* out3) {
   const uint32_t bx = (uint32_t)((cols + 255) / 256);
   hipLaunchKernelGGL(wrows_kernel, dim3(bx, chunks), dim3(256), 0, st, M, cols, w, rows / chunks, part);
-  hipLaunchKernelGGL(sum_chunks_kernel, dim3(bx), dim3(256), 0, st, part, chunks, cols, out);
+  hipLaunchKernelGGL(sum_chunks_kernel, dim3(bx), dim3(256), 0, st, part, chunks, cols, out, out2, out3);
   return hipGetLastError();
 }
 
@@ -246,7 +255,7 @@ hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part /* >= 25
 // extension elements; bi_m1 = bi - 1 (ligesis.rs:424-425).
 __global__ void bits_gather_kernel(const uint64_t* __restrict__ enc, uint64_t ld, const uint32_t* __restrict__ idx,
                                    uint32_t s, uint64_t W, uint64_t* __restrict__ bi,
-                                   ef2* __restrict__ bi_ef, ef2* __restrict__ bi_m1) {
+                                   ef2* __restrict__ bi_ef, ef2* __restrict__ bi_m1, ef2* __restrict__ bi_ef2) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= W * s) return;
   const uint64_t i = t / s;
@@ -255,17 +264,19 @@ __global__ void bits_gather_kernel(const uint64_t* __restrict__ enc, uint64_t ld
   bi[t] = bit;
   bi_ef[t] = ef2{bit, 0};
   bi_m1[t] = ef2{bit ? 0 : GL_P - 1, 0};
+  if (bi_ef2) bi_ef2[t] = ef2{bit, 0};
 }
 
 // g2[i] = sum_k eq_r2[k] * bi[i * s + k] (ligesis.rs:445-455)
 __global__ void g2_kernel(const uint64_t* __restrict__ bi, uint32_t s, uint64_t W, const ef2* __restrict__ eq_r2,
-                          ef2* __restrict__ g2) {
+                          ef2* __restrict__ g2, ef2* __restrict__ g2b) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= W) return;
   ef2 acc = ef_zero();
   for (uint32_t k = 0; k < s; k++)
     if (bi[i * s + k]) acc = ef_add(acc, eq_r2[k]);
   g2[i] = acc;
+  if (g2b) g2b[i] = acc;
 }
 
 // v[i] = eqz1[i / 64] * 2^(i mod 64) (ligesis.rs:474-476)
@@ -431,12 +442,32 @@ __global__ void fold_low_kernel(const uint64_t* __restrict__ poly, int ext, int 
 // One reduction round: partial sums of m[2i] * eq_rest[i] (l0, basefold.rs:752-758) and the
 // low-variable fold m'[i] = m[2i] + r (m[2i+1] - m[2i]) into a second buffer.
 
+// Fold of a codeword over the subgroup of order 2h at challenge r: with x = v[i], nx = v[i + h]
+// (the values at g^i and -g^i), out[i] = (1 - r) (x + nx) / 2 + r (x - nx) / (2 g^i).
+__device__ __forceinline__ void eq_fold_codeword_at(const ef2* __restrict__ v, uint64_t h, ef2 r,
+                                                    const uint64_t* __restrict__ tw, ef2* __restrict__ out,
+                                                    uint64_t i) {
+  const ef2 x = v[i], nx = v[i + h];
+  const uint64_t ginv = i == 0 ? 1 : gl_neg(tw[h - i]);
+  const ef2 pe = ef_mul_base(ef_add(x, nx), GL_INV2);
+  const ef2 po = ef_mul_base(ef_sub(x, nx), gl_mul(ginv, GL_INV2));
+  out[i] = ef_add(ef_mul(ef_one_minus(r), pe), ef_mul(r, po));
+}
+
+
 // All channels of one reduction round in one launch (blockIdx.y = channel): l0 partials with
 // the last block of each channel reducing them into out[c], the low-variable fold of m, and the
 // eq table of the next round's residual point (pair sums).
 __global__ __launch_bounds__(256) void bf_round3_kernel(BfRound a, uint64_t half, const ef2* __restrict__ r_ptr) {
   __shared__ ef2 sh[256];
   const int c = blockIdx.y;
+  if (c == 3) {  // codeword fold, grid-stride over its half length
+    const ef2 r = *r_ptr;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.cw_half;
+         i += (uint64_t)gridDim.x * blockDim.x)
+      eq_fold_codeword_at(a.cw_in, a.cw_half, r, a.cw_tw, a.cw_out, i);
+    return;
+  }
   const ef2* m = a.m_in[c];
   const ef2* eq = a.eq[c];
   ef2* m_out = a.m_out[c];
@@ -473,24 +504,19 @@ __global__ __launch_bounds__(256) void bf_round3_kernel(BfRound a, uint64_t half
 }
 
 hipError_t launch_bf_round3(const BfRound& a, uint64_t half, const ef2* r, hipStream_t st) {
-  hipLaunchKernelGGL(bf_round3_kernel, dim3(sc_round_blocks(half), 3), dim3(256), 0, st, a, half, r);
+  // blocks of the three channels; the codeword fold (y = 3) grid-strides over the same count,
+  // at least 64 blocks while it has work for them
+  uint32_t nb = sc_round_blocks(half);
+  const uint64_t fold_blocks = (a.cw_half + 255) / 256;
+  if (nb < 64 && fold_blocks > nb) nb = (uint32_t)(fold_blocks < 64 ? fold_blocks : 64);
+  hipLaunchKernelGGL(bf_round3_kernel, dim3(nb, a.cw_out ? 4 : 3), dim3(256), 0, st, a, half, r);
   return hipGetLastError();
 }
 
 
 // eq_fold_codeword (basefold.rs:201-218): out[i] = (1 - r)(x + nx)/2 + r (x - nx) g^-i / 2 with
 // x = v[i], nx = v[i + h]; g^-i = -g^(h - i) for 0 < i < h, tw[e] = g^e (e < h).
-__global__ void eq_fold_codeword_kernel(const ef2* __restrict__ v, uint64_t h, const ef2* __restrict__ r_ptr,
-                                        const uint64_t* __restrict__ tw, ef2* __restrict__ out) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= h) return;
-  const ef2 r = *r_ptr;
-  const ef2 x = v[i], nx = v[i + h];
-  const uint64_t ginv = i == 0 ? 1 : gl_neg(tw[h - i]);
-  const ef2 pe = ef_mul_base(ef_add(x, nx), GL_INV2);
-  const ef2 po = ef_mul_base(ef_sub(x, nx), gl_mul(ginv, GL_INV2));
-  out[i] = ef_add(ef_mul(ef_one_minus(r), pe), ef_mul(r, po));
-}
+
 
 // ---------------- gathers for openings ----------------
 __global__ void gather_parts_kernel(const GatherPart* __restrict__ parts, uint32_t nparts,
@@ -509,12 +535,13 @@ __global__ void gather_parts_kernel(const GatherPart* __restrict__ parts, uint32
 #define LGS_GRID(n) dim3((uint32_t)(((n) + 255) / 256)), dim3(256)
 
 hipError_t launch_bits_gather(const uint64_t* enc, uint64_t ld, const uint32_t* idx, uint32_t s, uint64_t W,
-                              uint64_t* bi, ef2* bi_ef, ef2* bi_m1, hipStream_t st) {
-  hipLaunchKernelGGL(bits_gather_kernel, LGS_GRID(W * s), 0, st, enc, ld, idx, s, W, bi, bi_ef, bi_m1);
+                              uint64_t* bi, ef2* bi_ef, ef2* bi_m1, hipStream_t st, ef2* bi_ef2) {
+  hipLaunchKernelGGL(bits_gather_kernel, LGS_GRID(W * s), 0, st, enc, ld, idx, s, W, bi, bi_ef, bi_m1, bi_ef2);
   return hipGetLastError();
 }
-hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st) {
-  hipLaunchKernelGGL(g2_kernel, LGS_GRID(W), 0, st, bi, s, W, eq_r2, g2);
+hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st,
+                     ef2* g2b) {
+  hipLaunchKernelGGL(g2_kernel, LGS_GRID(W), 0, st, bi, s, W, eq_r2, g2, g2b);
   return hipGetLastError();
 }
 hipError_t launch_vvec(const ef2* eqz1, uint64_t W, ef2* v, hipStream_t st) {
@@ -541,11 +568,6 @@ hipError_t launch_bf_combine(const BfChannels& ch, uint64_t D, ef2* out, hipStre
 hipError_t launch_fold_low(const uint64_t* poly, int ext, int li, uint64_t nout, const ef2* eq4, ef2* m,
                            hipStream_t st) {
   hipLaunchKernelGGL(fold_low_kernel, LGS_GRID(nout), 0, st, poly, ext, li, nout, eq4, m);
-  return hipGetLastError();
-}
-hipError_t launch_eq_fold_codeword(const ef2* v, uint64_t h, const ef2* r, const uint64_t* tw, ef2* out,
-                                   hipStream_t st) {
-  hipLaunchKernelGGL(eq_fold_codeword_kernel, LGS_GRID(h), 0, st, v, h, r, tw, out);
   return hipGetLastError();
 }
 hipError_t launch_gather_parts(const GatherPart* parts, uint32_t nparts, const uint64_t* off, uint64_t total,

@@ -36,6 +36,11 @@ struct BfRound {
   ef2* out[3];
   ef2* part;              // 3 x blocks
   unsigned int* counter;  // 3 zeroed words
+  // the same round's fold of the combined codeword (eq_fold_codeword), in the same launch
+  const ef2* cw_in;
+  uint64_t cw_half;
+  const uint64_t* cw_tw;
+  ef2* cw_out;
 };
 hipError_t launch_bf_round3(const BfRound& a, uint64_t half, const ef2* r, hipStream_t st);
 
@@ -48,17 +53,34 @@ struct BfChannels {
   int ext[3];
 };
 
-hipError_t launch_eq(const ef2* pt, int k, ef2* scratch, const ef2* scale, int accumulate, ef2* out,
-                     hipStream_t st);
+// Half tables of one eq term: lo over pt[0..klo), hi over pt[klo..k) times *scale (if set).
+struct EqTerm {
+  const ef2* pt;
+  const ef2* scale;
+  ef2* lo;
+  ef2* hi;
+  uint32_t k, klo, first_block, pad;
+};
+// out[b] = sum over nterms of lo[q][b mod 2^klo] * hi[q][b >> klo], b < n.
+struct EqTable {
+  ef2* out;
+  uint64_t n;
+  const ef2* lo[3];
+  const ef2* hi[3];
+  uint32_t klo, nterms, first_block, pad;
+};
+hipError_t launch_eq_group(const EqTerm* terms, int nterms, uint32_t term_blocks, const EqTable* tabs, int ntabs,
+                           uint32_t tab_blocks, hipStream_t st);
 hipError_t launch_wrows(const uint64_t* M, uint32_t rows, uint64_t cols, const ef2* w, ef2* part,
-                        uint32_t chunks, ef2* out, hipStream_t st);
+                        uint32_t chunks, ef2* out, hipStream_t st, ef2* out2 = nullptr, ef2* out3 = nullptr);
 // Partial sums are reduced by the last block to finish; `counter` is a zeroed device word that
 // the kernel re-arms, so consecutive launches on one stream may share it.
 hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part, unsigned int* counter, ef2* out,
                       hipStream_t st);
 hipError_t launch_bits_gather(const uint64_t* enc, uint64_t ld, const uint32_t* idx, uint32_t s, uint64_t W,
-                              uint64_t* bi, ef2* bi_ef, ef2* bi_m1, hipStream_t st);
-hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st);
+                              uint64_t* bi, ef2* bi_ef, ef2* bi_m1, hipStream_t st, ef2* bi_ef2 = nullptr);
+hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st,
+                     ef2* g2b = nullptr);
 hipError_t launch_vvec(const ef2* eqz1, uint64_t W, ef2* v, hipStream_t st);
 hipError_t launch_sel(const uint32_t* idx, const ef2* eq_r2, uint32_t s, ef2* sel, uint64_t len, hipStream_t st);
 hipError_t launch_rs_weights(const ef2* r3, int log2n, uint64_t n, const uint64_t* tw, ef2* w, hipStream_t st);
@@ -67,8 +89,6 @@ uint32_t sc_round_blocks(uint64_t half);
 hipError_t launch_bf_combine(const BfChannels& ch, uint64_t D, ef2* out, hipStream_t st);
 hipError_t launch_fold_low(const uint64_t* poly, int ext, int li, uint64_t nout, const ef2* eq4, ef2* m,
                            hipStream_t st);
-hipError_t launch_eq_fold_codeword(const ef2* v, uint64_t h, const ef2* r, const uint64_t* tw, ef2* out,
-                                   hipStream_t st);
 // dst[w] = parts[p].src[off[w]] for the part p whose [first, next first) range holds w.
 struct GatherPart {
   const uint64_t* src;
