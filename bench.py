@@ -28,15 +28,43 @@ HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0
 
 
+def pmc_traffic(kernel: str, tag: str):
+    """HBM bytes per launch of `kernel` under schedule `tag`, from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE and WRITE_SIZE runs scaled by a copy kernel of known
+    size; see the file for the commands). None when that schedule was not profiled."""
+    path = os.path.join(ROOT, "profiles", "commit_mu24_2e12x2e12.md")
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0] == kernel and cells[1] == tag:
+            try:
+                return int(cells[4])
+            except ValueError:
+                return None
+    return None
+
+
+def settle(step, ms: float = 300.0):
+    """Run the workload untimed for `ms` milliseconds before the warmup steps: the device raises
+    its clocks over the first ~100 ms of load, and a timed region started from idle would
+    measure that ramp instead of the kernels."""
+    t0 = time.perf_counter()
+    while (time.perf_counter() - t0) * 1e3 < ms:
+        step()
+
+
 def cpu_baseline(seconds_hint: float = 15.0) -> dict:
-    """Oracle commit (single thread, the reference's loop structure) on 2^7 rows x 2^12 columns:
-    the same row length as the workload and 1/32 of its rows. A is then 2 MiB and stays in
-    cache, so the sample runs faster per element than the full workload would (a conservative
-    baseline)."""
+    """Oracle commit (single thread, the reference's loop structure) on 2^9 rows x 2^12 columns:
+    the same row length as the workload and 1/8 of its rows (about 15 s of CPU work). A is then
+    8 MiB, so the sample runs at least as fast per element as the full workload would (a
+    conservative baseline)."""
     import oracle
     from ligesis_amd import synth
 
-    lm, ln = 7, 12
+    lm, ln = 9, 12
     inp = synth.Inputs(lm, ln, seed=77, n_fields=4, n_ints=4)
     pk = oracle.ProvingKey(lm, ln, inp.a, commit_a=False)
     t0 = time.perf_counter()
@@ -49,8 +77,7 @@ def cpu_baseline(seconds_hint: float = 15.0) -> dict:
         "cores": 1,
         "kind": "port",
         "sample": f"oracle commit of 2^{lm} x 2^{ln} (mu={lm + ln}, the workload's row length and "
-                  f"1/32 of its rows; A fits in cache, so per element this is faster than the full "
-                  f"workload), {dt:.2f} s, one thread",
+                  f"1/{1 << (LOG_M - lm)} of its rows), {dt:.2f} s, one thread",
     }
 
 
@@ -68,6 +95,8 @@ def main():
     ap.add_argument("--ntt-impl", type=int, default=-1,
                     help="-1 auto, 0/2/3 radix-8 passes with 256/512/1024 threads per row, 1 radix-2 stages")
     ap.add_argument("--merkle-lg", type=int, default=0, help="log2 Merkle workgroup width (0 auto, 6..8)")
+    ap.add_argument("--pipeline", type=int, default=0,
+                    help="row chunks whose transform overlaps the hash (0 auto, 1 off, 2, 4, 8)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="commit workload: skip the autotuner (schedule options left at auto then "
                          "take the built-in defaults)")
@@ -122,6 +151,8 @@ def bench_serve(args):
         ln = mu - lm
         keys[mu] = L.Ligesis.setup(lm, ln, synth.field_elems(3000 + mu, 32 * (64 << lm)), ctx=ctx)[0]
         fs[mu] = torch.from_numpy(synth.field_elems(4000 + mu, 1 << mu).view(np.int64)).pin_memory()
+    for mu in shapes:  # commit schedules for every shape in the stream (process-wide table)
+        ctx.autotune(keys[mu], reps=3)
     reqs = []
     for i, mu in enumerate(mus):
         reqs.append((mu, synth.ext_elems(5000 + i, mu), synth.ext_elems(6000 + i, 256), synth.oracle_ints(7000 + i, 640)))
@@ -257,6 +288,7 @@ def bench_batch(args):
             cd.release()
         return roots
 
+    settle(run_batch)
     for _ in range(args.warmup):
         ref = run_single()
         assert run_batch() == ref, "batched roots differ from per-polynomial roots"
@@ -310,13 +342,15 @@ def bench_prove(args):
     ctx = L.Context(0)
     pk, vk = L.Ligesis.setup(lm, ln, inp.a, ctx=ctx)
     root, cd = L.Ligesis.commit(pk, inp.f)
-    proofs = set()
+    settle(lambda: L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints)))
+    proofs = []
     for _ in range(args.warmup):
-        proofs.add(L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints)))
+        proofs.append(L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints)))
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        proofs.add(L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints)))
+        proofs.append(L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints)))
     dt = time.perf_counter() - t0
+    proofs = set(proofs)  # hashing 0.6 MB proofs stays outside the timed loop
     assert len(proofs) == 1, "prove is not deterministic"
     proof = next(iter(proofs))
     ms = dt / args.steps * 1e3
@@ -379,6 +413,7 @@ def bench_commit(args):
     ctx.set_option("hash_ksplit", args.ksplit)
     ctx.set_option("ntt_impl", args.ntt_impl)
     ctx.set_option("merkle_lg", args.merkle_lg)
+    ctx.set_option("pipeline", args.pipeline)
     pk, _ = L.Ligesis.setup(LOG_M, LOG_N, inp.a, ctx=ctx, commit_only=True)
     f_dev = torch.from_numpy(inp.f.view(np.int64)).to("cuda:0")
     torch.cuda.synchronize()
@@ -396,12 +431,17 @@ def bench_commit(args):
         torch.cuda.synchronize()
         del src, dst
 
+    def one_commit():
+        r, cd = L.Ligesis.commit(pk, f_dev, device=True)
+        cd.release()
+        return r
+
+    if tuned is None:  # the autotuner's own warm-up did this otherwise
+        settle(one_commit)
     ctx.set_option("profile", 1)
     roots = set()
     for _ in range(args.warmup):
-        r, cd = L.Ligesis.commit(pk, f_dev, device=True)
-        roots.add(r)
-        cd.release()
+        roots.add(one_commit())
     stages = {"ntt_ms": [], "hash_ms": [], "hash_finalize_ms": [], "merkle_ms": [], "commit_ms": []}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -430,14 +470,16 @@ def bench_commit(args):
                    {0: "hash_mfma_pipe_kernel<2, 2>", 1: "hash_mfma_pipe_kernel<1, 2>", 2: "hash_mfma_kernel",
                     3: "hash_mfma_pipe_kernel<2, 4>", 4: "hash_mfma_ring_kernel<2, 2>",
                     5: "hash_mfma_ring_kernel<4, 2>"}[schedule["hash_impl"]])
+    hash_tag = f"hash_impl={schedule['hash_impl']} ksplit={schedule['hash_ksplit']} pipeline={schedule['pipeline']}"
+    ntt_tag = f"ntt_impl={schedule['ntt_impl']}"
     if avg["hash_ms"] >= avg["ntt_ms"]:
         roof = {"kernel": f"{hash_kernel} (fused bit-decompose + H = A.B)", "bound": "mfma",
                 "achieved": hash_tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
-                "frac": hash_tops / INT8_PEAK_TOPS, "traffic": None}
+                "frac": hash_tops / INT8_PEAK_TOPS, "traffic": pmc_traffic(hash_kernel, hash_tag)}
     else:
         roof = {"kernel": "ntt_r8_kernel (row LDE)", "bound": "hbm", "achieved": ntt_gbs,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ntt_gbs / HBM_PEAK_GBS,
-                "traffic": None}
+                "traffic": pmc_traffic("ntt_r8_kernel", ntt_tag)}
     out = {
         "metric": "LigeSIS commit Melem/s at mu=24",
         "value": value,
@@ -458,7 +500,8 @@ def bench_commit(args):
         "stages_ms": avg,
         "roofline": roof,
         "ntt_roofline": {"bound": "hbm", "achieved": ntt_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": ntt_gbs / HBM_PEAK_GBS},
+                         "frac": ntt_gbs / HBM_PEAK_GBS, "traffic": pmc_traffic("ntt_r8_kernel", ntt_tag)},
+        "merkle_gbs": (32.0 * 8 * 2 * n) / (avg["merkle_ms"] / 1e3) / 1e9,
         "hash_rate_gelem_adds_per_s": (32.0 * 64 * m * 2 * n) / (avg["hash_ms"] / 1e3) / 1e9,
         "root": next(iter(roots)).hex(),
     }

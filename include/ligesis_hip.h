@@ -65,8 +65,10 @@ const char* lgs_last_error(const lgs_ctx* ctx);
  * 3 pipelined with 256 x 256 workgroup tiles, 4 global loads issued 2 slabs ahead, 5 as 4 with
  * 256 x 256 tiles), "ntt_impl" (-1 auto; 0/2/3 radix-8 passes with
  * 256/512/1024 threads per row, 1 radix-2 stages), "merkle_lg" (0 auto; 6..8 = log2 threads per
- * Merkle workgroup = tree levels per launch), "profile" (1: time every stage with HIP events on the
- * context stream). An option left at auto takes the autotuned schedule of the commit's shape when
+ * Merkle workgroup = tree levels per launch), "pipeline" (0 auto; 1 off; 2 / 4 / 8: the rows are
+ * encoded in that many chunks and each chunk's K-splits are hashed on a second stream while the
+ * next chunk is encoded), "profile" (1: time every stage with HIP events; pipelined stages report
+ * the summed durations of their launches). An option left at auto takes the autotuned schedule of the commit's shape when
  * the process tuning table has one (lgs_autotune, lgs_tune_load), else a built-in default. */
 int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value);
 
@@ -74,13 +76,14 @@ int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value);
  * kernel, [2] hash finalize, [3] Merkle tree, [4] whole commit. Up to `n` values are written. */
 int lgs_stage_times(const lgs_ctx* ctx, double* out_ms, uint32_t n);
 
-/* Commit launch schedule: {hash_variant, hash_impl, hash_ksplit, ntt_impl, merkle_lg} (the option
- * names above). No counterpart in the reference, whose schedule is fixed at compile time
- * (pcs/src/ligesis.rs:53-59 holds its only constants); results never depend on it. */
-#define LGS_TUNE_FIELDS 5
+/* Commit launch schedule: {hash_variant, hash_impl, hash_ksplit, ntt_impl, merkle_lg, pipeline}
+ * (the option names above). No counterpart in the reference, whose schedule is fixed at compile
+ * time (pcs/src/ligesis.rs:53-59 holds its only constants); results never depend on it. */
+#define LGS_TUNE_FIELDS 6
 
 /* Autotuner: times candidate schedules for the key's shape on a synthetic polynomial (each stage
- * separately: row-transform variant, hash kernel variant and K-split, Merkle workgroup width;
+ * separately: row-transform variant, hash kernel variant and K-split, Merkle workgroup width,
+ * then transform / hash overlap on the whole commit;
  * `reps` timed commits per candidate, 0 = 5) and stores the fastest in the process-wide table,
  * which every context (server workers included) consults. Writes the chosen schedule to `cfg`
  * and its measured commit time to `ms` (either may be NULL). */
@@ -90,7 +93,8 @@ int lgs_autotune(lgs_ctx* ctx, const lgs_key* key, uint32_t reps, int32_t cfg[LG
 int lgs_tune_get(lgs_ctx* ctx, uint32_t log_m, uint32_t log_n, int32_t cfg[LGS_TUNE_FIELDS]);
 
 /* Process tuning table as text, one shape per line:
- * "log_m log_n hash_variant hash_impl hash_ksplit ntt_impl merkle_lg ms", '#' starts a comment.
+ * "log_m log_n hash_variant hash_impl hash_ksplit ntt_impl merkle_lg pipeline ms", '#' starts a
+ * comment.
  * Load merges into the table and rejects a malformed file as a whole (LGS_ERR_ARG). `count`
  * (may be NULL) receives the number of entries written / read. Errors are reported on `ctx`,
  * which may be NULL (the table needs no device). */

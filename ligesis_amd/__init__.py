@@ -137,7 +137,7 @@ class Context:
         self.close()
 
 
-TUNE_FIELDS = ("hash_variant", "hash_impl", "hash_ksplit", "ntt_impl", "merkle_lg")
+TUNE_FIELDS = ("hash_variant", "hash_impl", "hash_ksplit", "ntt_impl", "merkle_lg", "pipeline")
 TUNE_FIELDS_COUNT = len(TUNE_FIELDS)
 
 

@@ -34,6 +34,10 @@ void ctx_unref(lgs::Context* c) {
   c->pool.trim();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->pipe_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->tev)
+    if (e) (void)hipEventDestroy(e);
   if (c->pinned) (void)hipHostFree(c->pinned);
   c->pinned = nullptr;
   (void)hipStreamDestroy(c->stream);
@@ -64,6 +68,18 @@ void finish_profile(lgs::Context& ctx) {
     float ms = 0;
     lgs::hip_check(hipEventElapsedTime(&ms, ctx.ev[i], ctx.ev[i + 1]), "event time");
     ctx.stage_ms[i] = ms;
+  }
+  if (ctx.prof_chunks > 0) {  // pipelined: sum the launches of each kind (they overlap in time)
+    double ntt = 0, hash = 0;
+    for (int q = 0; q < ctx.prof_chunks; q++) {
+      float a = 0, b = 0;
+      lgs::hip_check(hipEventElapsedTime(&a, ctx.tev[4 * q], ctx.tev[4 * q + 1]), "event time");
+      lgs::hip_check(hipEventElapsedTime(&b, ctx.tev[4 * q + 2], ctx.tev[4 * q + 3]), "event time");
+      ntt += a;
+      hash += b;
+    }
+    ctx.stage_ms[0] = ntt;
+    ctx.stage_ms[1] = hash;
   }
   float all = 0;
   lgs::hip_check(hipEventElapsedTime(&all, ctx.ev[0], ctx.ev[4]), "event time");
@@ -147,6 +163,8 @@ int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value) {
     ctx->c.ntt_impl = (int)value;
   } else if (n == "merkle_lg" && (value == 0 || (value >= 6 && value <= 8))) {
     ctx->c.merkle_lg = (int)value;
+  } else if (n == "pipeline" && (value == 0 || value == 1 || value == 2 || value == 4 || value == 8)) {
+    ctx->c.pipeline = (int)value;
   } else if (n == "profile") {
     ctx->c.profile = value != 0;
   } else {
@@ -163,6 +181,7 @@ static void put_tuning(const lgs::Tuning& t, int32_t* cfg) {
   cfg[2] = t.hash_ksplit;
   cfg[3] = t.ntt_impl;
   cfg[4] = t.merkle_lg;
+  cfg[5] = t.pipeline;
 }
 
 int lgs_autotune(lgs_ctx* ctx, const lgs_key* key, uint32_t reps, int32_t cfg[LGS_TUNE_FIELDS], double* ms) {
@@ -269,6 +288,7 @@ int lgs_commit(lgs_ctx* ctx, const lgs_key* key, const uint64_t* f, uint64_t len
       finish_profile(ctx->c);
     } catch (...) {
       (void)hipStreamSynchronize(ctx->c.stream);
+      if (ctx->c.stream2) (void)hipStreamSynchronize(ctx->c.stream2);
       delete c;
       throw;
     }
@@ -316,6 +336,7 @@ int lgs_commit_batch(lgs_ctx* ctx, uint32_t count, const lgs_key* const* keys,
   });
   if (rc != LGS_OK) {
     (void)hipStreamSynchronize(ctx->c.stream);
+    if (ctx->c.stream2) (void)hipStreamSynchronize(ctx->c.stream2);
     for (auto* c : made) delete c;
     return rc;
   }

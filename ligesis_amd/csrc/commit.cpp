@@ -27,6 +27,16 @@ hipStream_t Context::side_stream() {
   return stream2;
 }
 
+hipEvent_t Context::pipe_event(int i) {
+  if (!pipe_ev[i]) hip_check(hipEventCreateWithFlags(&pipe_ev[i], hipEventDisableTiming), "pipeline event");
+  return pipe_ev[i];
+}
+
+hipEvent_t Context::timing_event(int i) {
+  if (!tev[i]) hip_check(hipEventCreate(&tev[i]), "timing event");
+  return tev[i];
+}
+
 uint8_t* Context::staging(size_t bytes) {
   if (bytes > pinned_size) {
     if (pinned) (void)hipHostFree(pinned);
@@ -111,6 +121,7 @@ Tuning Context::resolve(uint32_t log_m, uint32_t log_n) const {
   if (hash_impl >= 0) t.hash_impl = hash_impl;
   if (ntt_impl >= 0) t.ntt_impl = ntt_impl;
   if (merkle_lg) t.merkle_lg = merkle_lg;
+  if (pipeline) t.pipeline = pipeline;
   return t;
 }
 
@@ -124,21 +135,64 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
   c.h = DevBuf<uint64_t>(&ctx.pool, 32 * two_n);
   c.tree.leaves = two_n;
   c.tree.nodes = DevBuf<uint32_t>(&ctx.pool, (size_t)c.tree.node_count() * 8);
+  const uint64_t* tw = ctx.twiddle(logN);
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[0], ctx.stream), "event");
-  // F'[i] = RS(F[i]) (ligesis.rs:289-291)
-  hip_check(launch_ntt(c.f, n, 1, (uint32_t)n, (uint32_t)m, logN, ctx.twiddle(logN), c.enc.ptr,
-                       ctx.stream, tu.ntt_impl),
-            "row encode");
-  if (ctx.profile) hip_check(hipEventRecord(ctx.ev[1], ctx.stream), "event");
   // H = A . B with B read bitwise from F' (ligesis.rs:292-302)
   bool mfma_ok = (two_n % 128 == 0) && m >= 2;
   int variant = tu.hash_variant == 0 ? (mfma_ok ? 1 : 2) : tu.hash_variant;
   if (variant == 1 && !mfma_ok) throw Error(1, "MFMA hash needs 2n % 128 == 0 and m >= 2");
+  uint32_t ks = 0;
   if (variant == 1) {
     uint64_t ntiles = two_n / 128;
-    uint32_t ks = tu.hash_ksplit ? (uint32_t)tu.hash_ksplit : pow2_at_least((512 + ntiles - 1) / ntiles);
+    ks = tu.hash_ksplit ? (uint32_t)tu.hash_ksplit : pow2_at_least((512 + ntiles - 1) / ntiles);
     if (ks > m / 2) ks = (uint32_t)(m / 2);
     if (ks == 0 || (m % (2 * ks)) != 0) throw Error(1, "hash_ksplit must divide m / 2");
+  }
+  const uint32_t chunks = (uint32_t)tu.pipeline;
+  if (variant == 1 && chunks > 1 && ks % chunks == 0 && chunks <= 8) {
+    // Pipelined: row chunk q (the rows of K-splits [q ks/C, (q+1) ks/C)) is encoded on the
+    // context stream, then hashed on the second stream while chunk q + 1 is encoded, so the
+    // VALU-bound transform and the MFMA-bound hash share the device.
+    hipStream_t sh = ctx.side_stream();
+    const uint64_t rows = m / chunks;
+    const uint32_t kc = ks / chunks;
+    DevBuf<int32_t> part(&ctx.pool, (size_t)ks * 256 * two_n);
+    auto mark = [&](int i, hipStream_t s) {
+      if (ctx.profile) hip_check(hipEventRecord(ctx.timing_event(i), s), "event");
+    };
+    for (uint32_t q = 0; q < chunks; q++) {
+      mark(4 * q, ctx.stream);
+      hip_check(launch_ntt(c.f + q * rows * n, n, 1, (uint32_t)n, (uint32_t)rows, logN, tw,
+                           c.enc.ptr + q * rows * two_n, ctx.stream, tu.ntt_impl),
+                "row encode");
+      mark(4 * q + 1, ctx.stream);
+      hip_check(hipEventRecord(ctx.pipe_event(q), ctx.stream), "pipeline event");
+      hip_check(hipStreamWaitEvent(sh, ctx.pipe_event(q), 0), "pipeline wait");
+      mark(4 * q + 2, sh);
+      hip_check(launch_hash_mfma_splits(c.enc.ptr, two_n, (uint32_t)m, (uint32_t)two_n, key.dig.ptr, ks, q * kc, kc,
+                                        part.ptr, sh, tu.hash_impl),
+                "hash (MFMA)");
+      mark(4 * q + 3, sh);
+    }
+    ctx.prof_chunks = ctx.profile ? (int)chunks : 0;
+    if (ctx.profile) {
+      hip_check(hipEventRecord(ctx.ev[1], ctx.stream), "event");  // every chunk encoded
+      hip_check(hipEventRecord(ctx.ev[2], sh), "event");          // every split hashed
+    }
+    hip_check(launch_hash_mfma_finalize(part.ptr, ks, (uint32_t)two_n, c.h.ptr, sh), "hash finalize");
+    if (ctx.profile) hip_check(hipEventRecord(ctx.ev[3], sh), "event");
+    hip_check(hipEventRecord(ctx.pipe_event(9), sh), "pipeline join");
+    hip_check(hipStreamWaitEvent(ctx.stream, ctx.pipe_event(9), 0), "pipeline join");
+    hip_check(launch_merkle(c.h.ptr, 32, two_n, 1, two_n, c.tree.nodes.ptr, ctx.stream, tu.merkle_lg), "merkle");
+    if (ctx.profile) hip_check(hipEventRecord(ctx.ev[4], ctx.stream), "event");
+    return;
+  }
+  ctx.prof_chunks = 0;
+  // F'[i] = RS(F[i]) (ligesis.rs:289-291)
+  hip_check(launch_ntt(c.f, n, 1, (uint32_t)n, (uint32_t)m, logN, tw, c.enc.ptr, ctx.stream, tu.ntt_impl),
+            "row encode");
+  if (ctx.profile) hip_check(hipEventRecord(ctx.ev[1], ctx.stream), "event");
+  if (variant == 1) {
     DevBuf<int32_t> part(&ctx.pool, (size_t)ks * 256 * two_n);
     hip_check(launch_hash_mfma(c.enc.ptr, two_n, (uint32_t)m, (uint32_t)two_n, key.dig.ptr, ks, part.ptr,
                                c.h.ptr, ctx.stream, ctx.profile ? ctx.ev[2] : nullptr, tu.hash_impl),

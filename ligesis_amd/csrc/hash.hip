@@ -77,7 +77,8 @@ constexpr int HM_SPS = 2;        // slabs staged in LDS per step (16 KiB each)
 __global__ __launch_bounds__(256) void hash_mfma_kernel(const uint64_t* __restrict__ enc, uint64_t ld,
                                                         const v4i* __restrict__ adig,
                                                         uint32_t slabs_per_split, uint32_t ntiles,
-                                                        uint32_t ncols, int32_t* __restrict__ part) {
+                                                        uint32_t ncols, int32_t* __restrict__ part,
+                                                        uint32_t split0) {
   __shared__ v4i lds_a[HM_SPS * 1024];
   // XCD-aware mapping: consecutive block ids land on different XCDs; give each XCD a
   // contiguous run of logical blocks so the blocks sharing a K-split share an L2.
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(256) void hash_mfma_kernel(const uint64_t* __restri
   const uint32_t per_xcd = nb / 8;
   const uint32_t bid = blockIdx.x;
   const uint32_t logical = (nb % 8 == 0) ? (bid % 8) * per_xcd + bid / 8 : bid;
-  const uint32_t split = logical / ntiles;
+  const uint32_t split = split0 + logical / ntiles;
   const uint32_t ntile = logical % ntiles;
 
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -164,10 +165,11 @@ template <int SPS, int WN>
 __global__ __launch_bounds__(128 * WN, 2) void hash_mfma_pipe_kernel(const uint64_t* __restrict__ enc,
                                                                      uint64_t ld, const v4i* __restrict__ adig,
                                                                      uint32_t slabs_per_split, uint32_t ntiles,
-                                                                     uint32_t ncols, int32_t* __restrict__ part) {
+                                                                     uint32_t ncols, int32_t* __restrict__ part,
+                                                                     uint32_t split0) {
   __shared__ v4i lds_a[2][SPS * 1024];
   const uint32_t logical = xcd_logical_block();
-  hash_pipe_body<SPS, WN>(enc, ld, adig, slabs_per_split, logical / ntiles, logical % ntiles, ncols, part,
+  hash_pipe_body<SPS, WN>(enc, ld, adig, slabs_per_split, split0 + logical / ntiles, logical % ntiles, ncols, part,
                           lds_a);
 }
 
@@ -288,12 +290,13 @@ template <int WN, int D>
 __global__ __launch_bounds__(128 * WN, 2) void hash_mfma_ring_kernel(const uint64_t* __restrict__ enc,
                                                                      uint64_t ld, const v4i* __restrict__ adig,
                                                                      uint32_t slabs_per_split, uint32_t ntiles,
-                                                                     uint32_t ncols, int32_t* __restrict__ part) {
+                                                                     uint32_t ncols, int32_t* __restrict__ part,
+                                                                     uint32_t split0) {
   constexpr int T = 128 * WN;
   constexpr int NA = 1024 / T;
   __shared__ v4i lds_a[2][1024];
   const uint32_t logical = xcd_logical_block();
-  const uint32_t split = logical / ntiles, ntile = logical % ntiles;
+  const uint32_t split = split0 + logical / ntiles, ntile = logical % ntiles;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t wm = wave / WN, wn = wave % WN;
   const uint32_t g = lane >> 4, c = lane & 15;
@@ -447,43 +450,56 @@ hipError_t launch_hash_valu(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
   return hipGetLastError();
 }
 
-hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
-                            const int8_t* dig, uint32_t ksplit, int32_t* part, uint64_t* h,
-                            hipStream_t st, hipEvent_t mid, int impl) {
-  uint32_t ntiles = ncols / HM_NT;
-  uint32_t sps = m_rows / ksplit;
+hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
+                                  const int8_t* dig, uint32_t ksplit, uint32_t split0, uint32_t nsplits,
+                                  int32_t* part, hipStream_t st, int impl) {
+  const uint32_t ntiles = ncols / HM_NT;
+  const uint32_t sps = m_rows / ksplit;
+  const v4i* ad = (const v4i*)dig;
   if (impl == 4 || impl == 5) {
     const int wn = impl == 5 ? 4 : 2;
     if (sps % 2 == 0 && ncols % (64 * wn) == 0) {
       if (impl == 4)
-        hipLaunchKernelGGL((hash_mfma_ring_kernel<2, 2>), dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
-                           (const v4i*)dig, sps, ntiles, ncols, part);
+        hipLaunchKernelGGL((hash_mfma_ring_kernel<2, 2>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps,
+                           ntiles, ncols, part, split0);
       else
-        hipLaunchKernelGGL((hash_mfma_ring_kernel<4, 2>), dim3((ncols / 256) * ksplit), dim3(512), 0, st, enc, ld,
-                           (const v4i*)dig, sps, ncols / 256, ncols, part);
+        hipLaunchKernelGGL((hash_mfma_ring_kernel<4, 2>), dim3((ncols / 256) * nsplits), dim3(512), 0, st, enc, ld,
+                           ad, sps, ncols / 256, ncols, part, split0);
     } else {
-      hipLaunchKernelGGL((hash_mfma_pipe_kernel<1, 2>), dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
-                         (const v4i*)dig, sps, ntiles, ncols, part);
+      hipLaunchKernelGGL((hash_mfma_pipe_kernel<1, 2>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps,
+                         ntiles, ncols, part, split0);
     }
-  } else if (impl == 3 && ncols % 256 == 0)
-    hipLaunchKernelGGL((hash_mfma_pipe_kernel<2, 4>), dim3((ncols / 256) * ksplit), dim3(512), 0, st, enc, ld,
-                       (const v4i*)dig, sps, ncols / 256, ncols, part);
-  else if (impl == 2)
-    hipLaunchKernelGGL(hash_mfma_kernel, dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
-                       (const v4i*)dig, sps, ntiles, ncols, part);
-  else if (impl == 1)
-    hipLaunchKernelGGL((hash_mfma_pipe_kernel<1, 2>), dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
-                       (const v4i*)dig, sps, ntiles, ncols, part);
-  else
-    hipLaunchKernelGGL((hash_mfma_pipe_kernel<2, 2>), dim3(ntiles * ksplit), dim3(256), 0, st, enc, ld,
-                       (const v4i*)dig, sps, ntiles, ncols, part);
-  hipError_t e = hipGetLastError();
+  } else if (impl == 3 && ncols % 256 == 0) {
+    hipLaunchKernelGGL((hash_mfma_pipe_kernel<2, 4>), dim3((ncols / 256) * nsplits), dim3(512), 0, st, enc, ld, ad,
+                       sps, ncols / 256, ncols, part, split0);
+  } else if (impl == 2) {
+    hipLaunchKernelGGL(hash_mfma_kernel, dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps, ntiles, ncols,
+                       part, split0);
+  } else if (impl == 1) {
+    hipLaunchKernelGGL((hash_mfma_pipe_kernel<1, 2>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps,
+                       ntiles, ncols, part, split0);
+  } else {
+    hipLaunchKernelGGL((hash_mfma_pipe_kernel<2, 2>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps,
+                       ntiles, ncols, part, split0);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_hash_mfma_finalize(const int32_t* part, uint32_t ksplit, uint32_t ncols, uint64_t* h,
+                                     hipStream_t st) {
+  const uint64_t n = (uint64_t)32 * ncols;
+  hipLaunchKernelGGL(hash_mfma_finalize_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, part, ksplit,
+                     ncols, h);
+  return hipGetLastError();
+}
+
+hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
+                            const int8_t* dig, uint32_t ksplit, int32_t* part, uint64_t* h,
+                            hipStream_t st, hipEvent_t mid, int impl) {
+  hipError_t e = launch_hash_mfma_splits(enc, ld, m_rows, ncols, dig, ksplit, 0, ksplit, part, st, impl);
   if (e != hipSuccess) return e;
   if (mid && (e = hipEventRecord(mid, st)) != hipSuccess) return e;
-  uint64_t n = (uint64_t)32 * ncols;
-  hipLaunchKernelGGL(hash_mfma_finalize_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
-                     part, ksplit, ncols, h);
-  return hipGetLastError();
+  return launch_hash_mfma_finalize(part, ksplit, ncols, h, st);
 }
 
 __global__ void hash_finalize_group_kernel(const FinalizeJob* __restrict__ jobs, int njobs) {

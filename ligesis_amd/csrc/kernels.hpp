@@ -62,6 +62,13 @@ hipError_t launch_hash_valu(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
 hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
                             const int8_t* dig, uint32_t ksplit, int32_t* part, uint64_t* h,
                             hipStream_t st, hipEvent_t mid, int impl);
+// The main MFMA kernel for K-splits [split0, split0 + nsplits) only (partials of those splits),
+// and the finalize over all ksplit partials: a commit may issue its splits in several launches.
+hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
+                                  const int8_t* dig, uint32_t ksplit, uint32_t split0, uint32_t nsplits,
+                                  int32_t* part, hipStream_t st, int impl);
+hipError_t launch_hash_mfma_finalize(const int32_t* part, uint32_t ksplit, uint32_t ncols, uint64_t* h,
+                                     hipStream_t st);
 
 // Batched launches over job tables that live in device memory.
 hipError_t launch_ntt_group(const NttJob* jobs, int njobs, uint32_t total_blocks, int max_logN, hipStream_t st);

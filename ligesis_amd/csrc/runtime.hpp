@@ -141,6 +141,7 @@ struct Tuning {
   int hash_ksplit = 0;   // slabs of the K (= 64 m) dimension; 0: heuristic
   int ntt_impl = 0;      // row transform variant
   int merkle_lg = 8;     // log2 threads per Merkle workgroup
+  int pipeline = 1;      // row chunks whose transform overlaps the hash of the previous chunk (1: off)
 };
 
 // Process-wide table of measured schedules, keyed by (log_m, log_n) (tune.cpp). Shared by every
@@ -190,6 +191,7 @@ struct Context {
                          // 3 2x4 waves, 4 loads issued 2 slabs ahead, 5 as 4 with 2x4 waves
   int ntt_impl = -1;     // -1 auto; row transform: 0/2/3 radix-8 with 256/512/1024 threads, 1 radix-2
   int merkle_lg = 0;     // 0 auto; log2 threads per Merkle workgroup = levels per launch (6..8)
+  int pipeline = 0;      // 0 auto; 1 off, 2 / 4 / 8 row chunks (transform on stream, hash on stream2)
   bool profile = false;
   double stage_ms[8] = {0};
   hipEvent_t ev[8] = {};
@@ -199,6 +201,13 @@ struct Context {
   const uint64_t* twiddle(int logN);
   uint8_t* staging(size_t bytes);
   hipStream_t side_stream();  // stream2, created on first use
+  hipEvent_t pipe_ev[10] = {};  // cross-stream events of a pipelined commit, created on first use
+  hipEvent_t pipe_event(int i);
+  // profiled pipelined commit: events around chunk q's transform (4q, 4q + 1) and hash (4q + 2,
+  // 4q + 3); prof_chunks = number of chunks of the last profiled commit (0: not pipelined)
+  hipEvent_t tev[32] = {};
+  int prof_chunks = 0;
+  hipEvent_t timing_event(int i);
   // launch schedule of a commit of shape 2^log_m x 2^log_n
   Tuning resolve(uint32_t log_m, uint32_t log_n) const;
 };

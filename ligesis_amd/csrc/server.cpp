@@ -137,6 +137,7 @@ void Server::commit_loop() {
       rc = le ? le->code : (dynamic_cast<const std::bad_alloc*>(&e) ? 4 : 1);
       msg = e.what();
       (void)hipStreamSynchronize(ctx.stream);
+      if (ctx.stream2) (void)hipStreamSynchronize(ctx.stream2);
     }
     std::lock_guard<std::mutex> lk(mu);
     batches++;
@@ -293,6 +294,10 @@ void server_stop(Server* s) {
     c->twiddles.clear();
     c->pool.trim();
     for (auto& e : c->ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->pipe_ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->tev)
       if (e) (void)hipEventDestroy(e);
     if (c->pinned) (void)hipHostFree(c->pinned);
     c->pinned = nullptr;

@@ -26,7 +26,7 @@ def test_autotuned_commit_matches_oracle(gpu_ctx, shape, tmp_path):
     best = gpu_ctx.autotune(pk, reps=2)
     assert best["ms"] > 0
     assert best["hash_variant"] in (1, 2) and 0 <= best["hash_impl"] <= 3
-    assert 0 <= best["ntt_impl"] <= 3 and 6 <= best["merkle_lg"] <= 8
+    assert 0 <= best["ntt_impl"] <= 3 and 6 <= best["merkle_lg"] <= 8 and best["pipeline"] in (1, 2, 4, 8)
     cfg = gpu_ctx.tuning(log_m, log_n)
     assert cfg == {k: best[k] for k in ligesis_amd.TUNE_FIELDS}
     root, cd = Ligesis.commit(pk, inp.f)
@@ -72,3 +72,26 @@ def test_merkle_widths_agree(gpu_ctx, lg):
         gpu_ctx.set_option("merkle_lg", 0)
     opk = oracle.ProvingKey(3, 10, inp.a, commit_a=False)
     assert root == oracle.CommitData(opk, inp.f).root
+
+
+@pytest.mark.parametrize("shape,pipeline", [((3, 12), 2), ((4, 9), 4), ((5, 9), 8), ((2, 7), 2)])
+def test_pipelined_commit_matches_oracle(gpu_ctx, shape, pipeline):
+    """Row chunks encoded on one stream and hashed on another give the oracle's F', H and root;
+    (2, 7) has too few K-splits for two chunks and takes the plain path."""
+    log_m, log_n = shape
+    print(f"pipelined commit shape={shape} chunks={pipeline}", flush=True)
+    inp = synth.Inputs(log_m, log_n, seed=70 + log_m)
+    pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx, commit_only=True)
+    gpu_ctx.set_option("pipeline", pipeline)
+    gpu_ctx.set_option("profile", 1)
+    try:
+        root, cd = Ligesis.commit(pk, inp.f)
+        st = gpu_ctx.stage_times()
+    finally:
+        gpu_ctx.set_option("pipeline", 0)
+        gpu_ctx.set_option("profile", 0)
+    ocd = oracle.CommitData(oracle.ProvingKey(log_m, log_n, inp.a, commit_a=False), inp.f, keep_enc=True)
+    assert root == ocd.root
+    np.testing.assert_array_equal(cd.encoded(), ocd.enc())
+    np.testing.assert_array_equal(cd.digests(), ocd.h())
+    assert all(v >= 0 for v in st.values()) and st["commit_ms"] > 0

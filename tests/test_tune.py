@@ -21,35 +21,37 @@ def _entries(path):
 def test_load_save_roundtrip(tmp_path):
     src = tmp_path / "in.txt"
     src.write_text("# comment line\n"
-                   "12 12 1 0 8 2 7 0.925\n"
+                   "12 12 1 0 8 2 7 2 0.925\n"
                    "\n"
-                   "  10 14 2 0 0 0 8 1.5\n")
+                   "  10 14 2 0 0 0 8 1 1.5\n")
     assert ligesis_amd.tune_load(str(src)) == 2
     out = tmp_path / "out.txt"
     assert ligesis_amd.tune_save(str(out)) == 2
     got = _entries(out)
-    assert [g[:7] for g in got] == [["10", "14", "2", "0", "0", "0", "8"], ["12", "12", "1", "0", "8", "2", "7"]]
-    assert float(got[1][7]) == pytest.approx(0.925)
+    assert [g[:8] for g in got] == [["10", "14", "2", "0", "0", "0", "8", "1"],
+                                    ["12", "12", "1", "0", "8", "2", "7", "2"]]
+    assert float(got[1][8]) == pytest.approx(0.925)
     # a second load merges by shape: the newer entry replaces the older one
     upd = tmp_path / "upd.txt"
-    upd.write_text("12 12 1 3 16 0 6 0.9\n")
+    upd.write_text("12 12 1 3 16 0 6 4 0.9\n")
     assert ligesis_amd.tune_load(str(upd)) == 1
     ligesis_amd.tune_save(str(out))
-    assert [g[:7] for g in _entries(out)][1] == ["12", "12", "1", "3", "16", "0", "6"]
+    assert [g[:8] for g in _entries(out)][1] == ["12", "12", "1", "3", "16", "0", "6", "4"]
 
 
 @pytest.mark.parametrize("line", [
-    "12 12 1 0 8 2 7",                # missing field
-    "12 12 1 0 8 2 7 0.9 extra",      # trailing field
-    "12 12 1 6 8 2 7 0.9",            # hash_impl out of range
-    "12 12 1 0 6 2 7 0.9",            # ksplit not a power of two
-    "12 12 1 0 8 2 9 0.9",            # merkle_lg out of range
-    "21 12 1 0 8 2 7 0.9",            # log_m out of range
-    "12 12 x 0 8 2 7 0.9",            # not a number
+    "12 12 1 0 8 2 7 1",              # missing field
+    "12 12 1 0 8 2 7 1 0.9 extra",    # trailing field
+    "12 12 1 6 8 2 7 1 0.9",          # hash_impl out of range
+    "12 12 1 0 6 2 7 1 0.9",          # ksplit not a power of two
+    "12 12 1 0 8 2 9 1 0.9",          # merkle_lg out of range
+    "12 12 1 0 8 2 7 3 0.9",          # pipeline not 1, 2, 4 or 8
+    "21 12 1 0 8 2 7 1 0.9",          # log_m out of range
+    "12 12 x 0 8 2 7 1 0.9",          # not a number
 ])
 def test_malformed_file_rejected_whole(tmp_path, line):
     p = tmp_path / "bad.txt"
-    p.write_text("10 10 1 0 4 0 8 0.1\n" + line + "\n")
+    p.write_text("10 10 1 0 4 0 8 1 0.1\n" + line + "\n")
     with pytest.raises(LigesisError) as ei:
         ligesis_amd.tune_load(str(p))
     assert ei.value.code == ligesis_amd.LGS_ERR_ARG
