@@ -107,8 +107,9 @@ static uint32_t pow2_at_least(uint64_t x) {
 Tuning builtin_tuning(uint32_t log_m, uint32_t log_n) {
   (void)log_m;
   Tuning t;
-  // row transforms of 2^12 and 2^13 points: 512 threads per row measured fastest at m = 2^12
-  t.ntt_impl = log_n + 1 >= 12 ? 2 : 0;
+  // one thread per radix-8 butterfly of a pass measured fastest at m = 2^12 (1024 threads for
+  // rows of 2^13 points, 512 for 2^12)
+  t.ntt_impl = log_n + 1 >= 13 ? 3 : log_n + 1 == 12 ? 2 : 0;
   return t;
 }
 

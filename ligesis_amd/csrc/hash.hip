@@ -376,6 +376,92 @@ __global__ __launch_bounds__(128 * WN, 2) void hash_mfma_ring_kernel(const uint6
   }
 }
 
+// Smaller wave tiles for more waves per SIMD: every wave owns MT x NT 16 x 16 tiles (MT * 16
+// rows of the 256, NT * 16 columns), so its accumulators take MT * NT * 4 registers; a workgroup
+// has (16 / MT) x WN waves covering all 256 rows and WN * NT * 16 columns. With MT = NT = 4 a wave
+// needs 64 accumulator registers and four waves fit on a SIMD, which hides the per-stage latency
+// (loads, LDS staging, barrier) that bounds the 128 x 64 tiles at two waves per SIMD.
+template <int MT, int NT, int WN, int MINB>
+__global__ __launch_bounds__(64 * (16 / MT) * WN, MINB) void hash_mfma_tile_kernel(
+    const uint64_t* __restrict__ enc, uint64_t ld, const v4i* __restrict__ adig, uint32_t slabs_per_split,
+    uint32_t ntiles, uint32_t ncols, int32_t* __restrict__ part, uint32_t split0) {
+  constexpr int WM = 16 / MT;
+  constexpr int T = 64 * WM * WN;
+  constexpr int NA = 1024 / T;  // 16-byte A words staged per thread per slab
+  static_assert(NA >= 1 && 1024 % T == 0, "A staging needs T | 1024");
+  __shared__ v4i lds_a[2][1024];
+  const uint32_t logical = xcd_logical_block();
+  const uint32_t split = split0 + logical / ntiles, ntile = logical % ntiles;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t wm = wave / WN, wn = wave % WN;
+  const uint32_t g = lane >> 4, c = lane & 15;
+  const uint32_t kb = 32 * (g >> 1) + 4 * (g & 1);
+  const uint64_t col_base = (uint64_t)ntile * (WN * NT * 16) + wn * (NT * 16) + c;
+
+  v4i acc[MT][NT];
+#pragma unroll
+  for (int a = 0; a < MT; a++)
+#pragma unroll
+    for (int b = 0; b < NT; b++) acc[a][b] = v4i{0, 0, 0, 0};
+
+  const uint32_t ns = slabs_per_split;
+  const uint64_t slab0 = (uint64_t)split * ns;
+  v4i pa[NA];
+  uint64_t px[NT];
+#pragma unroll
+  for (int q = 0; q < NA; q++) pa[q] = adig[slab0 * 1024 + q * T + threadIdx.x];
+#pragma unroll
+  for (int nt = 0; nt < NT; nt++) px[nt] = enc[slab0 * ld + col_base + nt * 16];
+#pragma unroll
+  for (int q = 0; q < NA; q++) lds_a[0][q * T + threadIdx.x] = pa[q];
+  __syncthreads();
+  for (uint32_t st = 0; st < ns; st++) {
+    const uint32_t buf = st & 1;
+    uint64_t cx[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) cx[nt] = px[nt];
+    const bool more = st + 1 < ns;
+    if (more) {
+      const uint64_t s1 = slab0 + st + 1;
+#pragma unroll
+      for (int q = 0; q < NA; q++) pa[q] = adig[s1 * 1024 + q * T + threadIdx.x];
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) px[nt] = enc[s1 * ld + col_base + nt * 16];
+    }
+    v4i b[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+      const uint32_t y = (uint32_t)(cx[nt] >> kb);
+      b[nt] = v4i{(int)(y & 0x01010101u), (int)((y >> 1) & 0x01010101u), (int)((y >> 2) & 0x01010101u),
+                  (int)((y >> 3) & 0x01010101u)};
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+      const v4i a = lds_a[buf][(wm * MT + mt) * 64 + lane];
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++)
+        acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[nt], acc[mt][nt], 0, 0, 0);
+    }
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < NA; q++) lds_a[buf ^ 1][q * T + threadIdx.x] = pa[q];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++) {
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const uint32_t m = (wm * MT + mt) * 16 + 4 * g + v;
+        const uint64_t col = (uint64_t)ntile * (WN * NT * 16) + wn * (NT * 16) + nt * 16 + c;
+        part[((uint64_t)split * 256 + m) * ncols + col] = acc[mt][nt][v];
+      }
+    }
+  }
+}
+
 // H[r][j] = sum_l 256^l (sum_split part[split][32 l + r][j]) mod p.
 __global__ void hash_mfma_finalize_kernel(const int32_t* __restrict__ part, uint32_t ksplit,
                                           uint32_t ncols, uint64_t* __restrict__ h) {
@@ -469,6 +555,12 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
       hipLaunchKernelGGL((hash_mfma_pipe_kernel<1, 2>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps,
                          ntiles, ncols, part, split0);
     }
+  } else if (impl == 6 && ncols % 128 == 0) {  // 64 x 64 wave tiles, 8 waves, 256 x 128 workgroup tile
+    hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 4, 2, 2>), dim3((ncols / 128) * nsplits), dim3(512), 0, st, enc, ld,
+                       ad, sps, ncols / 128, ncols, part, split0);
+  } else if (impl == 7 && ncols % 256 == 0) {  // 64 x 64 wave tiles, 16 waves, 256 x 256 workgroup tile
+    hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 4, 4, 1>), dim3((ncols / 256) * nsplits), dim3(1024), 0, st, enc,
+                       ld, ad, sps, ncols / 256, ncols, part, split0);
   } else if (impl == 3 && ncols % 256 == 0) {
     hipLaunchKernelGGL((hash_mfma_pipe_kernel<2, 4>), dim3((ncols / 256) * nsplits), dim3(512), 0, st, enc, ld, ad,
                        sps, ncols / 256, ncols, part, split0);

@@ -67,8 +67,8 @@ __global__ __launch_bounds__(256) void ntt_lds_kernel(const uint64_t* __restrict
 // ---------------- radix-8 row transform (whole row in LDS) ----------------
 // Roots of unity of order <= 64 are powers of two in Goldilocks: w_64 = 2^39 and
 // w_{2^k} = 2^(39 * 2^(6-k) mod 192) (2^96 = -1, 2^192 = 1). Multiplying by them is a shift and
-// a reduction, so the internal twiddles of an 8-point DFT and every twiddle of the first two
-// radix-8 passes cost no general multiplication.
+// a reduction, so the internal twiddles of an 8-point DFT (fixed exponents) cost no general
+// multiplication.
 
 // x * 2^e mod p for 0 <= e < 192
 __device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x, uint32_t e) {
@@ -131,28 +131,28 @@ __device__ __forceinline__ void dft8(uint64_t (&x)[8]) {
   }
 }
 
-// Multiply x[t] by c^rev(t) where c = w_{2^(s+lg)}^j and rev is the lg-bit reversal.
+// Multiply x[t] by c^rev(t) where c = w_{2^(s+lg)}^j and rev is the lg-bit reversal. Always a
+// table twiddle and a full multiply: the roots of order <= 64 are powers of two, but their
+// exponent depends on j, so a shift-based product would branch differently in every lane.
 template <int LG>
 __device__ __forceinline__ void pretwiddle(uint64_t (&x)[1 << LG], uint32_t j, int s, int logN,
                                            const uint64_t* __restrict__ tw, int tsh) {
   constexpr int R = 1 << LG;
   const int lsz = s + LG;  // log2 of this pass's transform size
-  if (lsz <= 6) {
-    const uint32_t e0 = (39u << (6 - lsz)) % 192;
+  const uint32_t half = 1u << (logN - 1);
 #pragma unroll
-    for (int t = 1; t < R; t++) {
-      const uint32_t r = __brev((uint32_t)t) >> (32 - LG);
-      x[t] = gl_mul_pow2(x[t], (uint32_t)(((uint64_t)e0 * j * r) % 192));
-    }
-  } else {
-    const uint32_t half = 1u << (logN - 1);
-#pragma unroll
-    for (int t = 1; t < R; t++) {
-      const uint32_t r = __brev((uint32_t)t) >> (32 - LG);
-      x[t] = gl_mul(x[t], tw_full(tw, (j * r) << (logN - lsz), half, tsh));
-    }
+  for (int t = 1; t < R; t++) {
+    const uint32_t r = __brev((uint32_t)t) >> (32 - LG);
+    x[t] = gl_mul(x[t], tw_full(tw, (j * r) << (logN - lsz), half, tsh));
   }
 }
+
+// LDS word of row position w: the low 5 bits XOR the top 5 bits of the row index (sh = logN - 5;
+// sh = 31 leaves w alone). The first pass scatters to bit-reversed blocks, where 32 consecutive
+// lanes differ only in those top bits and would otherwise all hit one bank; runs of 32
+// consecutive positions (every later pass) stay a permutation of one bank row.
+__device__ __forceinline__ uint32_t lds_pos(uint32_t w, int sh) { return w ^ ((w >> sh) & 31u); }
+__host__ __device__ constexpr size_t ntt_lds_bytes(uint32_t N) { return (size_t)N * sizeof(uint64_t); }
 
 // One row: src[k * es] for k < n_in (zero above), transform size 2^logN, result to dst; tw is
 // the twiddle table of a transform of size 2^(logN + tsh).
@@ -160,6 +160,7 @@ __device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uin
                                            int logN, const uint64_t* __restrict__ tw, int tsh,
                                            uint64_t* __restrict__ dst, uint64_t* sm) {
   const uint32_t N = 1u << logN;
+  const int sh = logN >= 10 ? logN - 5 : 31;
   int s = 0;
   if (logN >= 3) {
     // first pass fused with the load: thread k takes x[k + r N/8] (r = 0..7), which sit at
@@ -185,13 +186,13 @@ __device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uin
 #pragma unroll
       for (int t = 0; t < 8; t++) {
         if (last) dst[8 * b + t] = x[t];
-        else sm[8 * b + t] = x[t];
+        else sm[lds_pos(8 * b + t, sh)] = x[t];
       }
     }
     s = 3;
   } else {
     for (uint32_t k = threadIdx.x; k < N; k += blockDim.x)
-      sm[brev_bits(k, logN)] = k < n_in ? src[(uint64_t)k * es] : 0;
+      sm[lds_pos(brev_bits(k, logN), sh)] = k < n_in ? src[(uint64_t)k * es] : 0;
   }
   __syncthreads();
   while (logN - s >= 3) {
@@ -202,13 +203,13 @@ __device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uin
       const uint32_t base = ((gi >> s) << (s + 3)) + j;
       uint64_t x[8];
 #pragma unroll
-      for (int t = 0; t < 8; t++) x[t] = sm[base + t * M];
+      for (int t = 0; t < 8; t++) x[t] = sm[lds_pos(base + t * M, sh)];
       if (j) pretwiddle<3>(x, j, s, logN, tw, tsh);
       dft8(x);
 #pragma unroll
       for (int t = 0; t < 8; t++) {
         if (last) dst[base + t * M] = x[t];
-        else sm[base + t * M] = x[t];
+        else sm[lds_pos(base + t * M, sh)] = x[t];
       }
     }
     if (!last) __syncthreads();
@@ -221,7 +222,7 @@ __device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uin
       const uint32_t base = ((gi >> s) << (s + 2)) + j;
       uint64_t x[4];
 #pragma unroll
-      for (int t = 0; t < 4; t++) x[t] = sm[base + t * M];
+      for (int t = 0; t < 4; t++) x[t] = sm[lds_pos(base + t * M, sh)];
       if (j) pretwiddle<2>(x, j, s, logN, tw, tsh);
       uint64_t a0 = gl_add(x[0], x[1]), a1 = gl_sub(x[0], x[1]);
       uint64_t a2 = gl_add(x[2], x[3]), a3 = gl_mul_pow2(gl_sub(x[2], x[3]), 48);
@@ -233,9 +234,8 @@ __device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uin
   } else if (logN - s == 1) {
     const uint32_t M = 1u << s;
     for (uint32_t j = threadIdx.x; j < M; j += blockDim.x) {
-      uint64_t a = sm[j], b = sm[j + M];
-      if (j) b = s + 1 <= 6 ? gl_mul_pow2(b, (uint32_t)(((uint64_t)((39u << (6 - (s + 1))) % 192) * j) % 192))
-                            : gl_mul(b, tw[(uint64_t)j << (logN - s - 1 + tsh)]);
+      uint64_t a = sm[lds_pos(j, sh)], b = sm[lds_pos(j + M, sh)];
+      if (j) b = gl_mul(b, tw[(uint64_t)j << (logN - s - 1 + tsh)]);
       dst[j] = gl_add(a, b);
       dst[j + M] = gl_sub(a, b);
     }
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(256) void ntt_r8_group_kernel(const NttJob* __restr
 hipError_t launch_ntt_group(const NttJob* jobs, int njobs, uint32_t total_blocks, int max_logN, hipStream_t st) {
   if (total_blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(ntt_r8_group_kernel, dim3(total_blocks), dim3(256),
-                     (size_t(1) << max_logN) * sizeof(uint64_t), st, jobs, njobs);
+                     ntt_lds_bytes(1u << max_logN), st, jobs, njobs);
   return hipGetLastError();
 }
 
@@ -347,7 +347,7 @@ hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_i
     const uint32_t N = 1u << logN;
     const uint32_t want = impl == 2 ? 512 : impl == 3 ? 1024 : 256;
     const uint32_t threads = N / 8 >= want ? want : (N / 8 >= 256 ? 256 : 64);
-    hipLaunchKernelGGL(ntt_r8_kernel, dim3(rows), dim3(threads), N * sizeof(uint64_t), st, in, rs, es,
+    hipLaunchKernelGGL(ntt_r8_kernel, dim3(rows), dim3(threads), ntt_lds_bytes(N), st, in, rs, es,
                        n_in, logN, tw, out);
     return hipGetLastError();
   }
@@ -356,7 +356,7 @@ hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_i
   uint32_t nblk = 1u << (logN - lt);
   uint32_t threads = T / 2 < 256 ? (T / 2 < 64 ? 64 : T / 2) : 256;
   if (impl != 1)
-    hipLaunchKernelGGL(ntt_r8_blocks_kernel, dim3(rows * nblk), dim3(256), T * sizeof(uint64_t), st, in, rs,
+    hipLaunchKernelGGL(ntt_r8_blocks_kernel, dim3(rows * nblk), dim3(256), ntt_lds_bytes(T), st, in, rs,
                        es, n_in, logN, lt, tw, out);
   else
     hipLaunchKernelGGL(ntt_lds_kernel, dim3(rows * nblk), dim3(threads), T * sizeof(uint64_t), st,

@@ -188,7 +188,8 @@ struct Context {
   int hash_variant = 0;  // 0 auto, 1 MFMA, 2 VALU
   int hash_ksplit = 0;   // 0 auto
   int hash_impl = -1;    // -1 auto; MFMA kernel: 0 pipelined 2x2 waves, 1 one slab per stage, 2 basic,
-                         // 3 2x4 waves, 4 loads issued 2 slabs ahead, 5 as 4 with 2x4 waves
+                         // 3 2x4 waves, 4 loads issued 2 slabs ahead, 5 as 4 with 2x4 waves,
+                         // 6 / 7 64x64 wave tiles in 8 / 16-wave workgroups
   int ntt_impl = -1;     // -1 auto; row transform: 0/2/3 radix-8 with 256/512/1024 threads, 1 radix-2
   int merkle_lg = 0;     // 0 auto; log2 threads per Merkle workgroup = levels per launch (6..8)
   int pipeline = 0;      // 0 auto; 1 off, 2 / 4 / 8 row chunks (transform on stream, hash on stream2)
