@@ -49,7 +49,9 @@ uint8_t* Context::staging(size_t bytes) {
 }
 
 void enqueue_basefold_commit(Context& ctx, DevBasefold& st, const uint64_t* poly, bool ext,
-                             uint32_t num_vars, uint32_t li, uint32_t code_rate) {
+                             uint32_t num_vars, uint32_t li, uint32_t code_rate, hipStream_t s,
+                             hipEvent_t encoded) {
+  if (!s) s = ctx.stream;
   if (li > num_vars || li > 6) throw Error(1, "basefold: log_interleave out of range");
   st.ext = ext;
   st.num_vars = num_vars;
@@ -63,13 +65,12 @@ void enqueue_basefold_commit(Context& ctx, DevBasefold& st, const uint64_t* poly
   st.poly_ptr = poly;
   // chunk j of the poly (basefold.rs:346, MlPoly::split) is poly[j :: 2^li]; an extension
   // element is two words, so word row rho = 2 j + c walks with stride 2^(li+1).
-  hip_check(launch_ntt(poly, 1, rows, (uint32_t)len_c, rows, logD, ctx.twiddle(logD), st.codes.ptr,
-                       ctx.stream),
+  hip_check(launch_ntt(poly, 1, rows, (uint32_t)len_c, rows, logD, ctx.twiddle(logD), st.codes.ptr, s),
             "basefold encode");
+  if (encoded) hip_check(hipEventRecord(encoded, s), "basefold encoded");
   st.tree.leaves = st.D;
   st.tree.nodes = DevBuf<uint32_t>(&ctx.pool, (size_t)st.tree.node_count() * 8);
-  hip_check(launch_merkle(st.codes.ptr, rows, st.D, 1, st.D, st.tree.nodes.ptr, ctx.stream),
-            "basefold merkle");
+  hip_check(launch_merkle(st.codes.ptr, rows, st.D, 1, st.D, st.tree.nodes.ptr, s), "basefold merkle");
 }
 
 void setup_key(Context& ctx, Key& key, const uint64_t* host_a, bool commit_a) {

@@ -342,16 +342,40 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
                          a_copy2.ptr),
             "a-bar");
   hip_check(launch_dot(a_vec.ptr, eqz2.ptr, n, part.ptr, counter, res.ptr + o_y, st), "y");
-  DevBasefold av_state;
-  enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec.ptr, true, log_n, li_a, 1);
 
-  // ---- RS(a-bar) (ligesis.rs:387) ----
-  // With log_interleave 0 the Basefold codeword of a-bar just computed is this same transform
-  // (both planes zero-padded to 2n and transformed over the order-2n subgroup), so reuse it.
+  // ---- B_I (ligesis.rs:402-410) ----
+  DevBuf<uint64_t> bi(pool, nbi);
+  DevBuf<ef2> bi_ef(pool, nbi), bi_m1(pool, nbi), bi_ef2(pool, nbi);
+  hip_check(launch_bits_gather(cm.enc.ptr, two_n, didx.ptr, (uint32_t)s, W, bi.ptr, bi_ef.ptr, bi_m1.ptr, st,
+                               bi_ef2.ptr),
+            "B_I gather");
+
+  // ---- the Basefold commitments of a-bar and B_I (ligesis.rs:384, 411-416), second stream ----
+  // Only Stage B reads them (and the proof their roots), so they are built on the second stream,
+  // which forks here; Stage B (below) follows them there and joins before the openings are
+  // gathered. Every twiddle table either stream needs is made on st before the fork.
+  const uint64_t D = key.a_state.D;
   const int logN = log_n + 1;
+  for (uint32_t t = 0; t < R; t++) (void)ctx.twiddle(ilog2(D >> t));
+  (void)ctx.twiddle(ilog2(D));
+  (void)ctx.twiddle(logN);
+  hipStream_t sb = ctx.side_stream();
+  hip_check(hipEventRecord(ctx.ev[5], st), "fork");
+  hip_check(hipStreamWaitEvent(sb, ctx.ev[5], 0), "fork");
+  DevBasefold av_state, bi_state;
+  // RS(a-bar) (ligesis.rs:387): with log_interleave 0 a-bar's Basefold codeword is this same
+  // transform (both planes zero-padded to 2n over the order-2n subgroup), so st waits for it
+  const bool ra_shared = li_a == 0 && (uint64_t(1) << (log_n - li_a + 1)) == two_n;
+  enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec.ptr, true, log_n, li_a, 1, sb,
+                          ra_shared ? ctx.ev[7] : nullptr);
+  enqueue_basefold_commit(ctx, bi_state, bi.ptr, false, nv_bi, li_bi, 1, sb);
+  const DevBasefold* states[3] = {&key.a_state, &bi_state, &av_state};
+  const uint64_t* polys[3] = {key.a.ptr, bi.ptr, (const uint64_t*)a_vec.ptr};
+  for (int c = 0; c < 3; c++)
+    if (states[c]->D != D) throw Error(1, "prove: Basefold domains differ");
   DevBuf<uint64_t> ra_planar;
   const uint64_t* ra_src;
-  if (av_state.li == 0 && av_state.D == two_n) {
+  if (ra_shared) {
     ra_src = av_state.codes.ptr;
   } else {
     ra_planar = DevBuf<uint64_t>(pool, 2 * two_n);
@@ -361,28 +385,6 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     ra_src = ra_planar.ptr;
   }
   DevBuf<ef2> ra(pool, two_n);
-  hip_check(launch_planar_to_ef(ra_src, ra_src + two_n, two_n, ra.ptr, st), "RS(a-bar) layout");
-
-  // ---- B_I and its commitment (ligesis.rs:402-416) ----
-  DevBuf<uint64_t> bi(pool, nbi);
-  DevBuf<ef2> bi_ef(pool, nbi), bi_m1(pool, nbi), bi_ef2(pool, nbi);
-  hip_check(launch_bits_gather(cm.enc.ptr, two_n, didx.ptr, (uint32_t)s, W, bi.ptr, bi_ef.ptr, bi_m1.ptr, st,
-                               bi_ef2.ptr),
-            "B_I gather");
-  DevBasefold bi_state;
-  enqueue_basefold_commit(ctx, bi_state, bi.ptr, false, nv_bi, li_bi, 1);
-
-  // Stage B (below) reads only the three Basefold commitments, the committed polynomials, its eq
-  // tables and fixed challenges: it forks here onto the second stream and joins before the
-  // openings are gathered.
-  const DevBasefold* states[3] = {&key.a_state, &bi_state, &av_state};
-  const uint64_t* polys[3] = {key.a.ptr, bi.ptr, (const uint64_t*)a_vec.ptr};
-  const uint64_t D = key.a_state.D;
-  for (int c = 0; c < 3; c++)
-    if (states[c]->D != D) throw Error(1, "prove: Basefold domains differ");
-  for (uint32_t t = 0; t < R; t++) (void)ctx.twiddle(ilog2(D >> t));  // tables made on st, before the fork
-  hipStream_t sb = ctx.side_stream();
-  hip_check(hipEventRecord(ctx.ev[5], st), "fork");
 
   // ---- input tables of the seven Stage A sumchecks ----
   // hash / recon channels (ligesis.rs:432-485)
@@ -391,12 +393,15 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   hip_check(launch_g2(bi.ptr, (uint32_t)s, W, eqr2.ptr, g2.ptr, st, g2b.ptr), "g2");
   DevBuf<ef2> sel(pool, two_n);
   hip_check(launch_sel(didx.ptr, eqr2.ptr, (uint32_t)s, sel.ptr, two_n, st), "sel");
-  hip_check(launch_dot(sel.ptr, ra.ptr, two_n, part.ptr, counter, res.ptr + o_y4, st), "y4");
   hip_check(launch_vvec(eqz1.ptr, W, vv.ptr, st), "v");
   // recon table and RS link (ligesis.rs:488-502); y5 = ra(r3) is sc_table's final evaluation of
   // ra, which is what the reference's verify compares (ligesis.rs:681)
   DevBuf<ef2> wv(pool, n);
   hip_check(launch_rs_weights(B + o_r3, logN, n, ctx.twiddle(logN), wv.ptr, st), "rs weights");
+  // last, since it may wait for the second stream's encoding of a-bar
+  if (ra_shared) hip_check(hipStreamWaitEvent(st, ctx.ev[7], 0), "a-bar codeword");
+  hip_check(launch_planar_to_ef(ra_src, ra_src + two_n, two_n, ra.ptr, st), "RS(a-bar) layout");
+  hip_check(launch_dot(sel.ptr, ra.ptr, two_n, part.ptr, counter, res.ptr + o_y4, st), "y4");
 
   // ---- the seven sumchecks, in lockstep ----
   // Every challenge is fixed before the first round (the oracle streams are inputs), so the
@@ -474,7 +479,6 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   // ---- Stage B: shared-domain batched Basefold open (basefold.rs:679-856), second stream ----
   // Issued after Stage A so that the host queues the longer stage first; its scratch (partial
   // sums, counters) is its own.
-  hip_check(hipStreamWaitEvent(sb, ctx.ev[5], 0), "fork");
   // levels 0..R stored back to back: level k has D >> k entries
   std::vector<uint64_t> lvl_off(R + 1);
   uint64_t tot = 0;

@@ -264,8 +264,12 @@ struct MerkleSource {
 void enqueue_merkle_trees(Context& ctx, const std::vector<MerkleSource>& trees,
                           std::vector<std::vector<MerkleJob>>& host_jobs, std::vector<DevBuf<uint64_t>>& tables,
                           hipStream_t st);
+// On stream `s` (default: the context stream); `encoded`, if set, is recorded once the codewords
+// are written (before the Merkle tree). The twiddle table of the codeword length must already
+// exist when `s` is not the context stream.
 void enqueue_basefold_commit(Context& ctx, DevBasefold& st, const uint64_t* poly, bool ext,
-                             uint32_t num_vars, uint32_t li, uint32_t code_rate);
+                             uint32_t num_vars, uint32_t li, uint32_t code_rate, hipStream_t s = nullptr,
+                             hipEvent_t encoded = nullptr);
 void setup_key(Context& ctx, Key& key, const uint64_t* host_a, bool commit_a);
 // tune.cpp: times candidate schedules of key's shape on a synthetic polynomial and stores the
 // fastest in the process table; returns the best commit time in ms
