@@ -357,8 +357,8 @@ def bench_prove(args):
     # one more proof with host-side timing: how long issuing the work takes vs the whole proof
     ctx.set_option("profile", 1)
     L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints))
-    raw = np.zeros(7, np.float64)
-    ctx.check(L.lib().lgs_stage_times(ctx.h, L._ptr(raw), 7))
+    raw = np.zeros(12, np.float64)
+    ctx.check(L.lib().lgs_stage_times(ctx.h, L._ptr(raw), 12))
     ctx.set_option("profile", 0)
     out = {
         "metric": "LigeSIS eval-prove ms at mu=24",
@@ -377,6 +377,9 @@ def bench_prove(args):
                                "100 queries, proof on a resident commitment", "log_m": lm, "log_n": ln},
         "host_issue_ms": float(raw[5]),
         "device_bound_ms": float(raw[6]),
+        "device_phases_ms": {"sumcheck_start": float(raw[7]), "sumcheck_end": float(raw[8]),
+                             "basefold_open_end": float(raw[9]), "gathered": float(raw[10]),
+                             "downloaded": float(raw[11])},
         "proof_bytes": len(proof),
     }
     del cd, pk

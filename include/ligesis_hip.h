@@ -61,19 +61,25 @@ void lgs_shutdown(lgs_ctx* ctx);
 const char* lgs_last_error(const lgs_ctx* ctx);
 
 /* Tuning knobs that never change results: "hash" (0 auto, 1 MFMA, 2 VALU), "hash_ksplit"
- * (0 auto), "hash_impl" (-1 auto; MFMA kernel: 0 pipelined, 1 pipelined one slab per stage, 2 basic,
- * 3 pipelined with 256 x 256 workgroup tiles, 4 global loads issued 2 slabs ahead, 5 as 4 with
- * 256 x 256 tiles, 6 / 7 64 x 64 wave tiles in 256 x 128 / 256 x 256 workgroup tiles), "ntt_impl" (-1 auto; 0/2/3 radix-8 passes with
- * 256/512/1024 threads per row, 1 radix-2 stages), "merkle_lg" (0 auto; 6..8 = log2 threads per
- * Merkle workgroup = tree levels per launch), "pipeline" (0 auto; 1 off; 2 / 4 / 8: the rows are
- * encoded in that many chunks and each chunk's K-splits are hashed on a second stream while the
- * next chunk is encoded), "profile" (1: time every stage with HIP events; pipelined stages report
- * the summed durations of their launches). An option left at auto takes the autotuned schedule of the commit's shape when
- * the process tuning table has one (lgs_autotune, lgs_tune_load), else a built-in default. */
+ * (0 auto), "hash_impl" (-1 auto; MFMA kernel: 0 pipelined, 1 pipelined one slab per stage,
+ * 2 basic, 3 pipelined with 256 x 256 workgroup tiles, 4 global loads issued 2 slabs ahead, 5 as
+ * 4 with 256 x 256 tiles, 6 / 7 64 x 64 wave tiles in 256 x 128 / 256 x 256 workgroup tiles),
+ * "ntt_impl" (-1 auto; 0/2/3 radix-8 passes with 256/512/1024 threads per row, 1 radix-2
+ * stages), "merkle_lg" (0 auto; 6..8 = log2 threads per Merkle workgroup = tree levels per
+ * launch), "pipeline" (0 auto; 1 off; 2 / 4 / 8: the rows are encoded in that many chunks and
+ * each chunk's K-splits are hashed on a second stream while the next chunk is encoded),
+ * "profile" (1: time every stage with HIP events; pipelined stages report the summed durations
+ * of their launches). An option left at auto takes the autotuned schedule of the commit's shape
+ * when the process tuning table has one (lgs_autotune, lgs_tune_load), else a built-in
+ * default. */
 int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value);
 
-/* Stage times of the last profiled commit, in milliseconds: [0] row NTT, [1] hash main
- * kernel, [2] hash finalize, [3] Merkle tree, [4] whole commit. Up to `n` values are written. */
+/* Stage times of the last profiled call, in milliseconds. Commit: [0] row NTT, [1] hash main
+ * kernel, [2] hash finalize, [3] Merkle tree, [4] whole commit. Prove: [5] host time to issue
+ * the work, [6] host time to the proof's download; device timestamps from the first upload:
+ * [7] first sumcheck round starts, [8] last sumcheck launch ends, [9] the Basefold open on the
+ * second stream ends, [10] openings gathered, [11] proof downloaded. Up to `n` (<= 12) values
+ * are written. */
 int lgs_stage_times(const lgs_ctx* ctx, double* out_ms, uint32_t n);
 
 /* Commit launch schedule: {hash_variant, hash_impl, hash_ksplit, ntt_impl, merkle_lg, pipeline}

@@ -38,8 +38,7 @@ void ctx_unref(lgs::Context* c) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->tev)
     if (e) (void)hipEventDestroy(e);
-  if (c->pinned) (void)hipHostFree(c->pinned);
-  c->pinned = nullptr;
+  c->free_host();
   (void)hipStreamDestroy(c->stream);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   delete (lgs_ctx*)c->owner;
@@ -226,7 +225,7 @@ void lgs_tune_clear(void) { lgs::tune_clear(); }
 
 int lgs_stage_times(const lgs_ctx* ctx, double* out_ms, uint32_t n) {
   if (!ctx || !out_ms) return LGS_ERR_ARG;
-  for (uint32_t i = 0; i < n && i < 8; i++) out_ms[i] = ctx->c.stage_ms[i];
+  for (uint32_t i = 0; i < n && i < 12; i++) out_ms[i] = ctx->c.stage_ms[i];
   return LGS_OK;
 }
 

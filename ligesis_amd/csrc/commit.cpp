@@ -48,6 +48,45 @@ uint8_t* Context::staging(size_t bytes) {
   return pinned;
 }
 
+void Context::upload_reset() {
+  size_t total = 0;
+  for (auto& c : up_chunks) total += c.second;
+  if (up_chunks.size() > 1) {  // merge into one chunk of the combined size
+    for (auto& c : up_chunks) (void)hipHostFree(c.first);
+    up_chunks.clear();
+    uint8_t* p = nullptr;
+    hip_check(hipHostMalloc((void**)&p, total, hipHostMallocDefault), "hipHostMalloc");
+    up_chunks.emplace_back(p, total);
+  }
+  up_used = 0;
+}
+
+void Context::upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return;
+  size_t at = (up_used + 255) & ~size_t(255);
+  if (up_chunks.empty() || at + bytes > up_chunks.back().second) {  // new chunk; earlier ones stay
+    size_t sz = std::max<size_t>(bytes, size_t(1) << 20);
+    if (!up_chunks.empty()) sz = std::max(sz, 2 * up_chunks.back().second);
+    uint8_t* p = nullptr;
+    hip_check(hipHostMalloc((void**)&p, sz, hipHostMallocDefault), "hipHostMalloc");
+    up_chunks.emplace_back(p, sz);
+    at = 0;
+  }
+  uint8_t* h = up_chunks.back().first + at;
+  std::memcpy(h, src, bytes);
+  up_used = at + bytes;
+  hip_check(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s), "upload");
+}
+
+void Context::free_host() {
+  if (pinned) (void)hipHostFree(pinned);
+  pinned = nullptr;
+  pinned_size = 0;
+  for (auto& c : up_chunks) (void)hipHostFree(c.first);
+  up_chunks.clear();
+  up_used = 0;
+}
+
 void enqueue_basefold_commit(Context& ctx, DevBasefold& st, const uint64_t* poly, bool ext,
                              uint32_t num_vars, uint32_t li, uint32_t code_rate, hipStream_t s,
                              hipEvent_t encoded) {
@@ -261,8 +300,7 @@ void enqueue_merkle_trees(Context& ctx, const std::vector<MerkleSource>& trees,
   const std::vector<MerkleJob>& h = host_jobs.back();
   tables.emplace_back(&ctx.pool, (h.size() * sizeof(MerkleJob) + 7) / 8);
   const MerkleJob* d = (const MerkleJob*)tables.back().ptr;
-  hip_check(hipMemcpyAsync((void*)d, h.data(), h.size() * sizeof(MerkleJob), hipMemcpyHostToDevice, st),
-            "merkle jobs");
+  ctx.upload((void*)d, h.data(), h.size() * sizeof(MerkleJob), st);
   for (size_t r = 0; r < rounds.size(); r++)
     hip_check(launch_merkle_group(d + at[r], (int)rounds[r].size(), blocks[r], st), "grouped merkle");
 }
@@ -278,6 +316,7 @@ bool group_eligible(const Key& key) {
 void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, Commitment*>>& items,
                           GroupHold& hold) {
   if (items.empty()) return;
+  ctx.upload_reset();  // every earlier call on this context has been synchronized
   std::vector<NttJob> nj;
   std::vector<HashJob> hj;
   std::vector<FinalizeJob> fj;
@@ -312,7 +351,7 @@ void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, 
   const int njobs = (int)items.size();
   auto upload = [&](const void* src, size_t bytes) {
     hold.tables.emplace_back(&ctx.pool, (bytes + 7) / 8);
-    hip_check(hipMemcpyAsync(hold.tables.back().ptr, src, bytes, hipMemcpyHostToDevice, ctx.stream), "job table");
+    ctx.upload(hold.tables.back().ptr, src, bytes, ctx.stream);
     return (const void*)hold.tables.back().ptr;
   };
   const NttJob* d_nj = (const NttJob*)upload(nj.data(), nj.size() * sizeof(NttJob));

@@ -255,7 +255,7 @@ __global__ __launch_bounds__(1024) void ntt_r8_kernel(const uint64_t* __restrict
 // subsequence x[c + k 2^(logN-lt)], c = rev_{logN-lt}(blk), which is what the first lt stages of
 // an in-place radix-2 DIT leave in positions [blk 2^lt, (blk + 1) 2^lt) of the bit-reversed
 // array; the remaining logN - lt stages run as global passes.
-__global__ __launch_bounds__(256) void ntt_r8_blocks_kernel(const uint64_t* __restrict__ in, uint64_t rs,
+__global__ __launch_bounds__(1024) void ntt_r8_blocks_kernel(const uint64_t* __restrict__ in, uint64_t rs,
                                                             uint64_t es, uint32_t n_in, int logN, int lt,
                                                             const uint64_t* __restrict__ tw,
                                                             uint64_t* __restrict__ out) {
@@ -290,6 +290,57 @@ hipError_t launch_ntt_group(const NttJob* jobs, int njobs, uint32_t total_blocks
   hipLaunchKernelGGL(ntt_r8_group_kernel, dim3(total_blocks), dim3(256),
                      ntt_lds_bytes(1u << max_logN), st, jobs, njobs);
   return hipGetLastError();
+}
+
+// In-place radix-2 DIT network on 2^K values in bit-reversed order (natural-order output); the
+// twiddles are roots of order <= 16, i.e. powers of two with exponents fixed at compile time.
+template <int K>
+__device__ __forceinline__ void dft_br(uint64_t (&x)[1 << K]) {
+#pragma unroll
+  for (int s = 1; s <= K; s++) {
+    const int half = 1 << (s - 1);
+    const uint32_t es = (39u << (6 - s)) % 192;  // w_{2^s} = 2^es
+#pragma unroll
+    for (int g = 0; g < (1 << K); g += 2 * half) {
+#pragma unroll
+      for (int jj = 0; jj < half; jj++) {
+        const uint64_t u = x[g + jj];
+        uint64_t v = x[g + jj + half];
+        if (jj) v = gl_mul_pow2(v, (es * (uint32_t)jj) % 192);
+        x[g + jj] = gl_add(u, v);
+        x[g + jj + half] = gl_sub(u, v);
+      }
+    }
+  }
+}
+
+// The last K stages of a transform of 2^logN points in one pass. Before it, the in-place DIT
+// state holds, in block b of T = 2^(logN - K) positions, the T-point DFT Y_c of the subsequence
+// x[c + k 2^K] with c = rev_K(b); the result is X[j + T q] = sum_c w_N^(c j) w_S^(c q) Y_c[j]
+// (S = 2^K): for every j an S-point DFT of twiddled values, read and written at stride T by one
+// thread (consecutive threads, consecutive j: coalesced).
+template <int K>
+__global__ __launch_bounds__(256) void ntt_tail_kernel(uint64_t* __restrict__ data, uint32_t rows, int logN,
+                                                       const uint64_t* __restrict__ tw) {
+  constexpr int S = 1 << K;
+  const int lt = logN - K;
+  const uint64_t T = uint64_t(1) << lt;
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (uint64_t)rows * T) return;
+  const uint64_t row = gid >> lt, j = gid & (T - 1);
+  uint64_t* d = data + (row << logN);
+  const uint32_t half = 1u << (logN - 1);
+  uint64_t x[S];
+#pragma unroll
+  for (int t = 0; t < S; t++) x[t] = d[(uint64_t)t * T + j];
+#pragma unroll
+  for (int t = 1; t < S; t++) {
+    const uint32_t c = __brev((uint32_t)t) >> (32 - K);
+    x[t] = gl_mul(x[t], tw_full(tw, (uint32_t)(c * j), half, 0));
+  }
+  dft_br<K>(x);
+#pragma unroll
+  for (int q = 0; q < S; q++) d[(uint64_t)q * T + j] = x[q];
 }
 
 // One global radix-2 stage s (> lt) over every row, in place.
@@ -355,23 +406,37 @@ hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_i
   uint32_t T = 1u << lt;
   uint32_t nblk = 1u << (logN - lt);
   uint32_t threads = T / 2 < 256 ? (T / 2 < 64 ? 64 : T / 2) : 256;
-  if (impl != 1)
-    hipLaunchKernelGGL(ntt_r8_blocks_kernel, dim3(rows * nblk), dim3(256), ntt_lds_bytes(T), st, in, rs,
+  // the last (up to 4) stages in one tail pass instead of one global pass each
+  const int kt = impl == 1 ? 0 : (logN - lt < 4 ? logN - lt : 4);
+  if (impl != 1) {
+    // few workgroups (few rows): a thread per radix-8 butterfly shortens each block's transform
+    const uint32_t bt = (uint64_t)rows * nblk < 512 ? 1024 : 256;
+    hipLaunchKernelGGL(ntt_r8_blocks_kernel, dim3(rows * nblk), dim3(bt), ntt_lds_bytes(T), st, in, rs,
                        es, n_in, logN, lt, tw, out);
-  else
+  } else {
     hipLaunchKernelGGL(ntt_lds_kernel, dim3(rows * nblk), dim3(threads), T * sizeof(uint64_t), st,
                        in, rs, es, n_in, logN, lt, tw, out);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   uint64_t work = (uint64_t)rows << (logN - 1);
   uint32_t blocks = (uint32_t)((work + 255) / 256);
-  for (int s = lt + 1; s <= logN; s++) {
+  for (int s = lt + 1; s <= logN - kt; s++) {
     hipLaunchKernelGGL(ntt_global_stage_kernel, dim3(blocks), dim3(256), 0, st, out, rows, logN, s,
                        tw);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  return hipSuccess;
+  if (kt > 0) {
+    const uint64_t cols = (uint64_t)rows << (logN - kt);
+    const dim3 g((uint32_t)((cols + 255) / 256));
+    if (kt == 1) hipLaunchKernelGGL(ntt_tail_kernel<1>, g, dim3(256), 0, st, out, rows, logN, tw);
+    else if (kt == 2) hipLaunchKernelGGL(ntt_tail_kernel<2>, g, dim3(256), 0, st, out, rows, logN, tw);
+    else if (kt == 3) hipLaunchKernelGGL(ntt_tail_kernel<3>, g, dim3(256), 0, st, out, rows, logN, tw);
+    else hipLaunchKernelGGL(ntt_tail_kernel<4>, g, dim3(256), 0, st, out, rows, logN, tw);
+    e = hipGetLastError();
+  }
+  return e;
 }
 
 }  // namespace lgs

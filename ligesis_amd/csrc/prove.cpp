@@ -215,11 +215,16 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   hipStream_t st = ctx.stream;
   DevicePool* pool = &ctx.pool;
   DevBuf<ef2> dblob(pool, blob.size());
-  hip_check(hipMemcpyAsync(dblob.ptr, blob.data(), blob.size() * sizeof(ef2), hipMemcpyHostToDevice, st),
-            "upload constants");
+  ctx.upload_reset();  // every earlier call on this context has been synchronized
+  // profile mode: device timestamps of the phases (stage_ms[7..11], from the first upload)
+  auto mark = [&](int i, hipStream_t s) {
+    if (ctx.profile) hip_check(hipEventRecord(ctx.timing_event(32 + i), s), "profile event");
+  };
+  mark(0, st);
+  ctx.upload(dblob.ptr, blob.data(), blob.size() * sizeof(ef2), st);
   const ef2* B = dblob.ptr;
   DevBuf<uint32_t> didx(pool, s);
-  hip_check(hipMemcpyAsync(didx.ptr, idx.data(), s * 4, hipMemcpyHostToDevice, st), "upload idx");
+  ctx.upload(didx.ptr, idx.data(), s * 4, st);
 
   // results buffer layout
   Sumcheck sc[7];
@@ -239,12 +244,8 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   res_n += 3 * (size_t)R;
   DevBuf<ef2> res(pool, res_n);
 
-  DevBuf<ef2> part(pool, 1024 * 4 + 256);
-  // completion counters of the kernels that reduce their own partial sums; each kernel leaves
-  // its counter at zero again, so one clear serves the whole prove
-  DevBuf<uint64_t> counters(pool, 2);
-  hip_check(launch_zero(counters.ptr, 2, st), "clear counters");
-  unsigned int* counter = (unsigned int*)counters.ptr;
+  // partial sums of the two dot products (y, y4); the first sumcheck launch reduces them
+  DevBuf<ef2> part(pool, 512);
   // Every buffer below lives until the end of the prove: the second stream runs beside this
   // one, so no buffer may go back to the pool (and be handed out again) while work is queued.
 
@@ -322,11 +323,8 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     }
   }
   DevBuf<uint8_t> eq_jobs(pool, eq_terms.size() * sizeof(EqTerm) + eq_tabs.size() * sizeof(EqTable));
-  hip_check(hipMemcpyAsync(eq_jobs.ptr, eq_terms.data(), eq_terms.size() * sizeof(EqTerm), hipMemcpyHostToDevice, st),
-            "eq terms");
-  hip_check(hipMemcpyAsync(eq_jobs.ptr + eq_terms.size() * sizeof(EqTerm), eq_tabs.data(),
-                           eq_tabs.size() * sizeof(EqTable), hipMemcpyHostToDevice, st),
-            "eq tables");
+  ctx.upload(eq_jobs.ptr, eq_terms.data(), eq_terms.size() * sizeof(EqTerm), st);
+  ctx.upload(eq_jobs.ptr + eq_terms.size() * sizeof(EqTerm), eq_tabs.data(), eq_tabs.size() * sizeof(EqTable), st);
   hip_check(launch_eq_group((const EqTerm*)eq_jobs.ptr, (int)eq_terms.size(), eq_term_blocks,
                             (const EqTable*)(eq_jobs.ptr + eq_terms.size() * sizeof(EqTerm)), (int)eq_tabs.size(),
                             eq_tab_blocks, st),
@@ -341,14 +339,14 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   hip_check(launch_wrows(cm.f, (uint32_t)m, n, eqz1.ptr, wpart.ptr, (uint32_t)chunks, a_vec.ptr, st, a_copy.ptr,
                          a_copy2.ptr),
             "a-bar");
-  hip_check(launch_dot(a_vec.ptr, eqz2.ptr, n, part.ptr, counter, res.ptr + o_y, st), "y");
+  hip_check(launch_dot(a_vec.ptr, eqz2.ptr, n, part.ptr, st), "y");
 
   // ---- B_I (ligesis.rs:402-410) ----
+  // The two sumchecks over B_I read these 0/1 words in their first round, which writes the folded
+  // extension-field tables (half the length) that their later rounds work on.
   DevBuf<uint64_t> bi(pool, nbi);
-  DevBuf<ef2> bi_ef(pool, nbi), bi_m1(pool, nbi), bi_ef2(pool, nbi);
-  hip_check(launch_bits_gather(cm.enc.ptr, two_n, didx.ptr, (uint32_t)s, W, bi.ptr, bi_ef.ptr, bi_m1.ptr, st,
-                               bi_ef2.ptr),
-            "B_I gather");
+  DevBuf<ef2> bi_ef(pool, nbi / 2), bi_m1(pool, nbi / 2), bi_ef2(pool, nbi / 2);
+  hip_check(launch_bits_gather(cm.enc.ptr, two_n, didx.ptr, (uint32_t)s, W, bi.ptr, st), "B_I gather");
 
   // ---- the Basefold commitments of a-bar and B_I (ligesis.rs:384, 411-416), second stream ----
   // Only Stage B reads them (and the proof their roots), so they are built on the second stream,
@@ -401,7 +399,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   // last, since it may wait for the second stream's encoding of a-bar
   if (ra_shared) hip_check(hipStreamWaitEvent(st, ctx.ev[7], 0), "a-bar codeword");
   hip_check(launch_planar_to_ef(ra_src, ra_src + two_n, two_n, ra.ptr, st), "RS(a-bar) layout");
-  hip_check(launch_dot(sel.ptr, ra.ptr, two_n, part.ptr, counter, res.ptr + o_y4, st), "y4");
+  hip_check(launch_dot(sel.ptr, ra.ptr, two_n, part.ptr + 256, st), "y4");
 
   // ---- the seven sumchecks, in lockstep ----
   // Every challenge is fixed before the first round (the oracle streams are inputs), so the
@@ -422,15 +420,21 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     n_big[i] = sc[i].nv > 7 ? sc[i].nv - 7 : 0;  // rounds with more than 128 entries
     steps = std::max(steps, n_big[i]);
   }
-  DevBuf<ef2> scpart(pool, (size_t)7 * 1024 * 4);
-  DevBuf<uint64_t> sccount(pool, 4);  // 7 counters
-  hip_check(launch_zero(sccount.ptr, 4, st), "clear counters");
+  if (n_big[0] == 0 || n_big[5] == 0) throw Error(2, "prove: B_I has fewer than 2^8 entries");
+  // Round partial sums alternate between two halves of scpart: the launch of step tau writes
+  // its half while its extra blocks reduce the other half, written by step tau - 1 (the dot
+  // products' partials for step 0); the tail launch reduces the last step's.
+  const size_t part_half = (size_t)7 * 1024 * 4;
+  DevBuf<ef2> scpart(pool, 2 * part_half);
   std::vector<ScRoundJob> rjobs;
-  std::vector<uint32_t> step_first(steps + 1), step_blocks(steps);
+  std::vector<RedJob> reds = {RedJob{part.ptr, res.ptr + o_y, dot_blocks(n), 1},
+                              RedJob{part.ptr + 256, res.ptr + o_y4, dot_blocks(two_n), 1}};
+  std::vector<uint32_t> step_first(steps + 1), step_blocks(steps), red_first(steps + 2);
   for (uint32_t tau = 0; tau < steps; tau++) {
     step_first[tau] = (uint32_t)rjobs.size();
+    red_first[tau + 1] = (uint32_t)reds.size();
     uint32_t blocks = 0;
-    size_t poff = 0;
+    size_t poff = (tau & 1) * part_half;
     for (int i = 0; i < 7; i++) {
       if (tau + n_big[i] < steps) continue;  // starts later
       const uint32_t t = tau + n_big[i] - steps;
@@ -442,9 +446,9 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
       j.nblk = sc_round_blocks(j.half);
       j.part = scpart.ptr + poff;
       poff += (size_t)j.nblk * (S.d + 1);
-      j.counter = (unsigned int*)sccount.ptr + i;
-      j.out = res.ptr + S.rounds_at + (size_t)t * (S.d + 1);
+      reds.push_back(RedJob{j.part, res.ptr + S.rounds_at + (size_t)t * (S.d + 1), j.nblk, (uint32_t)S.d + 1});
       j.d = (uint32_t)S.d;
+      if (t == 0 && (i == 0 || i == 5)) j.bits = bi.ptr;  // B_I's sumchecks start from the 0/1 words
       j.first_block = blocks;
       blocks += j.nblk;
       rjobs.push_back(j);
@@ -452,6 +456,9 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     step_blocks[tau] = blocks;
   }
   step_first[steps] = (uint32_t)rjobs.size();
+  red_first[0] = 0;
+  red_first[steps + 1] = (uint32_t)reds.size();
+  // launch tau (tau = steps: the tail) runs reductions [red_first[tau], red_first[tau + 1])
   std::vector<ScTailJob> tjobs(7);
   for (int i = 0; i < 7; i++) {
     const Sumcheck& S = sc[i];
@@ -463,22 +470,26 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     tjobs[i].finals = res.ptr + S.finals_at;
     tjobs[i].d = (uint32_t)S.d;
   }
-  DevBuf<uint8_t> scjobs(pool, rjobs.size() * sizeof(ScRoundJob) + tjobs.size() * sizeof(ScTailJob));
-  if (!rjobs.empty())
-    hip_check(hipMemcpyAsync(scjobs.ptr, rjobs.data(), rjobs.size() * sizeof(ScRoundJob), hipMemcpyHostToDevice, st),
-              "sumcheck jobs");
-  ScTailJob* d_tjobs = (ScTailJob*)(scjobs.ptr + rjobs.size() * sizeof(ScRoundJob));
-  hip_check(hipMemcpyAsync(d_tjobs, tjobs.data(), tjobs.size() * sizeof(ScTailJob), hipMemcpyHostToDevice, st),
-            "sumcheck tail jobs");
+  const size_t rj_bytes = rjobs.size() * sizeof(ScRoundJob), tj_bytes = tjobs.size() * sizeof(ScTailJob);
+  DevBuf<uint8_t> scjobs(pool, rj_bytes + tj_bytes + reds.size() * sizeof(RedJob));
+  if (!rjobs.empty()) ctx.upload(scjobs.ptr, rjobs.data(), rj_bytes, st);
+  ScTailJob* d_tjobs = (ScTailJob*)(scjobs.ptr + rj_bytes);
+  ctx.upload(d_tjobs, tjobs.data(), tj_bytes, st);
+  const RedJob* d_reds = (const RedJob*)(scjobs.ptr + rj_bytes + tj_bytes);
+  ctx.upload((void*)d_reds, reds.data(), reds.size() * sizeof(RedJob), st);
+  mark(1, st);
   for (uint32_t tau = 0; tau < steps; tau++)
     hip_check(launch_sc_round_group((const ScRoundJob*)scjobs.ptr + step_first[tau],
-                                    (int)(step_first[tau + 1] - step_first[tau]), step_blocks[tau], st),
+                                    (int)(step_first[tau + 1] - step_first[tau]), step_blocks[tau],
+                                    d_reds + red_first[tau], red_first[tau + 1] - red_first[tau], st),
               "sumcheck round step");
-  hip_check(launch_sc_tail_group(d_tjobs, 7, st), "sumcheck tails");
+  hip_check(launch_sc_tail_group(d_tjobs, 7, d_reds + red_first[steps], red_first[steps + 1] - red_first[steps], st),
+            "sumcheck tails");
+  mark(2, st);
 
   // ---- Stage B: shared-domain batched Basefold open (basefold.rs:679-856), second stream ----
   // Issued after Stage A so that the host queues the longer stage first; its scratch (partial
-  // sums, counters) is its own.
+  // sums) is its own.
   // levels 0..R stored back to back: level k has D >> k entries
   std::vector<uint64_t> lvl_off(R + 1);
   uint64_t tot = 0;
@@ -486,9 +497,8 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     lvl_off[k] = tot;
     tot += D >> k;
   }
-  DevBuf<ef2> part_b(pool, 3 * 1024 + 64);
-  DevBuf<uint64_t> counters_b(pool, 2);
-  hip_check(launch_zero(counters_b.ptr, 2, sb), "clear counters");
+  // l0 partial sums, alternating halves by round parity (each round reduces the previous one's)
+  DevBuf<ef2> part_b(pool, 2 * 3 * 1024);
   DevBuf<ef2> levels(pool, tot);
   BfChannels chans{};
   chans.count = 3;
@@ -502,6 +512,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   for (int c = 0; c < 3; c++)
     hip_check(launch_fold_low(polys[c], states[c]->ext ? 1 : 0, (int)bli[c], M, B + o_eq4[c], mbuf[c][0].ptr, sb),
               "batch fold");
+  RedJobs last{};
   for (uint32_t t = 0; t < R; t++) {
     const uint64_t half = uint64_t(1) << (R - 1 - t);
     BfRound br{};
@@ -510,17 +521,21 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
       br.m_out[c] = mbuf[c][(t + 1) & 1].ptr;
       br.eq[c] = eqr[c][t & 1].ptr;
       br.eq_next[c] = eqr[c][(t + 1) & 1].ptr;
-      br.out[c] = res.ptr + o_l0 + (size_t)c * R + t;
+      br.prev[c] = last.j[c];  // zero (no reduction) in round 0
     }
-    br.part = part_b.ptr;
-    br.counter = (unsigned int*)counters_b.ptr;
+    br.part = part_b.ptr + (size_t)(t & 1) * 3 * 1024;
     const uint64_t len = D >> t;
     br.cw_in = levels.ptr + lvl_off[t];
     br.cw_half = len / 2;
     br.cw_tw = ctx.twiddle(ilog2(len));
     br.cw_out = levels.ptr + lvl_off[t + 1];
     hip_check(launch_bf_round3(br, half, B + o_chbatch + t, sb), "batch round");
+    const uint32_t nb = bf_round3_blocks(half, br.cw_half);
+    for (int c = 0; c < 3; c++)
+      last.j[c] = RedJob{br.part + (size_t)c * nb, res.ptr + o_l0 + (size_t)c * R + t, nb, 1};
+    last.n = 3;
   }
+  hip_check(launch_reduce_jobs(last, sb), "batch round sums");
   // trees over the folded levels 1..R-1 (leaf = one F_p^2 value), built together
   std::vector<DevTree> ftrees(R > 1 ? R - 1 : 0);
   std::vector<MerkleSource> fsrc;
@@ -533,6 +548,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   std::vector<std::vector<MerkleJob>> fjobs;
   std::vector<DevBuf<uint64_t>> ftables;
   enqueue_merkle_trees(ctx, fsrc, fjobs, ftables, sb);
+  mark(3, sb);
   hip_check(hipEventRecord(ctx.ev[6], sb), "join");
 
 
@@ -603,18 +619,25 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   }
   gtab.insert(gtab.end(), g.offs.begin(), g.offs.end());
   DevBuf<uint64_t> dgtab(pool, gtab.size());
-  hip_check(hipMemcpyAsync(dgtab.ptr, gtab.data(), gtab.size() * 8, hipMemcpyHostToDevice, st), "gather table");
+  ctx.upload(dgtab.ptr, gtab.data(), gtab.size() * 8, st);
   hip_check(hipMemcpyAsync(dstage.ptr, res.ptr, res_n * sizeof(ef2), hipMemcpyDeviceToDevice, st), "results");
   hip_check(launch_gather_parts((const GatherPart*)dgtab.ptr, (uint32_t)g.parts.size(),
                                 dgtab.ptr + 2 * g.parts.size(), g.offs.size(), dstage.ptr + 2 * res_n, st),
             "gather");
+  mark(4, st);
   uint64_t* host = (uint64_t*)ctx.staging(stage_words * 8);
   hip_check(hipMemcpyAsync(host, dstage.ptr, stage_words * 8, hipMemcpyDeviceToHost, st), "download proof");
+  mark(5, st);
   const auto t_enqueued = std::chrono::steady_clock::now();
   hip_check(hipStreamSynchronize(st), "prove sync");
   if (ctx.profile) {  // host time spent issuing work, and the whole device-bound part
     ctx.stage_ms[5] = std::chrono::duration<double, std::milli>(t_enqueued - t_start).count();
     ctx.stage_ms[6] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    float ms = 0;
+    for (int i = 1; i <= 5; i++) {
+      hip_check(hipEventElapsedTime(&ms, ctx.timing_event(32), ctx.timing_event(32 + i)), "profile time");
+      ctx.stage_ms[6 + i] = ms;
+    }
   }
   const uint64_t* RS = host;                 // results as words
   const uint64_t* GW = host + 2 * res_n;     // gathered words

@@ -30,40 +30,25 @@ __device__ __forceinline__ void block_sum(ef2 (&v)[W], ef2* sh /* blockDim x W *
   }
 }
 
-// Called by every one of the nblk blocks of a reduction after its thread 0 stored W partial
-// sums at part[blk * W]: the last block to finish (ticket from an atomic counter) adds up all
-// nblk partials into out[0..W) and re-arms the counter for the next launch on the stream. Saves
-// a separate reduction launch per round.
-template <int W>
-__device__ __forceinline__ void last_block_reduce(const ef2* part, unsigned int* counter, ef2* out, ef2* sh,
-                                                  uint32_t nblk) {
-  __shared__ int is_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned int ticket = atomicAdd(counter, 1u);
-    is_last = ticket == nblk - 1;
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  ef2 acc[W];
-#pragma unroll
-  for (int q = 0; q < W; q++) acc[q] = ef_zero();
-  const volatile uint64_t* pv = (const volatile uint64_t*)part;
-  for (uint32_t b = threadIdx.x; b < nblk; b += blockDim.x) {
-#pragma unroll
-    for (int q = 0; q < W; q++) {
-      const uint64_t k = 2 * ((uint64_t)b * W + q);
-      acc[q] = ef_add(acc[q], ef2{pv[k], pv[k + 1]});
-    }
-  }
-  block_sum<W>(acc, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < W; q++) out[q] = acc[q];
-    *counter = 0;
-  }
+// One deferred reduction (RedJob) by one block: strided partial sums, then a tree in LDS (sh
+// holds blockDim x 4 values).
+__device__ __forceinline__ void red_job_body(const RedJob& r, ef2* sh) {
+  ef2 acc[4] = {ef_zero(), ef_zero(), ef_zero(), ef_zero()};
+  for (uint32_t b = threadIdx.x; b < r.nblk; b += blockDim.x)
+    for (uint32_t q = 0; q < r.w; q++) acc[q] = ef_add(acc[q], r.part[(uint64_t)b * r.w + q]);
+  block_sum<4>(acc, sh);
+  if (threadIdx.x == 0)
+    for (uint32_t q = 0; q < r.w; q++) r.out[q] = acc[q];
+}
+
+__global__ __launch_bounds__(256) void reduce_jobs_kernel(RedJobs r) {
+  __shared__ ef2 sh[256 * 4];
+  red_job_body(r.j[blockIdx.x], sh);
+}
+
+hipError_t launch_reduce_jobs(const RedJobs& r, hipStream_t st) {
+  if (r.n > 0) hipLaunchKernelGGL(reduce_jobs_kernel, dim3(r.n), dim3(256), 0, st, r);
+  return hipGetLastError();
 }
 
 // ---------------- eq tables (MlPoly::new_eq, poly.rs:44-53) ----------------
@@ -149,15 +134,13 @@ hipError_t launch_wrows(const uint64_t* M, uint32_t rows, uint64_t cols, const e
 
 // ---------------- dot products: out = sum_i a[i] * b[i] ----------------
 __global__ __launch_bounds__(256) void dot_kernel(const ef2* __restrict__ a, const ef2* __restrict__ b,
-                                                  uint64_t n, ef2* __restrict__ part, unsigned int* counter,
-                                                  ef2* __restrict__ out) {
+                                                  uint64_t n, ef2* __restrict__ part) {
   __shared__ ef2 sh[256];
   ef2 acc[1] = {ef_zero()};
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     acc[0] = ef_add(acc[0], ef_mul(a[i], b[i]));
   block_sum<1>(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
-  last_block_reduce<1>(part, counter, out, sh, gridDim.x);
 }
 
 // out[q] = sum_{b < nb} part[b * width + q], for q < width <= 4 (one block of 256 threads:
@@ -226,9 +209,14 @@ __device__ __forceinline__ void sc_tail_body(const ef2* __restrict__ p0, const e
 }
 
 // The tails of several sumchecks, one wave each (blockIdx.x = job).
-__global__ __launch_bounds__(64) void sc_tail_group_kernel(const ScTailJob* __restrict__ jobs) {
+__global__ __launch_bounds__(64) void sc_tail_group_kernel(const ScTailJob* __restrict__ jobs, uint32_t njobs,
+                                                           const RedJob* __restrict__ reds) {
   __shared__ ef2 buf[3][128];
   __shared__ ef2 sh[64 * 4];
+  if (blockIdx.x >= njobs) {
+    red_job_body(reds[blockIdx.x - njobs], sh);
+    return;
+  }
   const ScTailJob& j = jobs[blockIdx.x];
   if (j.d == 3)
     sc_tail_body<3, 128>(j.p[0], j.p[1], j.p[2], j.len, j.chal, j.rounds, j.finals, buf, sh);
@@ -236,17 +224,21 @@ __global__ __launch_bounds__(64) void sc_tail_group_kernel(const ScTailJob* __re
     sc_tail_body<2, 128>(j.p[0], j.p[1], nullptr, j.len, j.chal, j.rounds, j.finals, buf, sh);
 }
 
-hipError_t launch_sc_tail_group(const ScTailJob* jobs, uint32_t njobs, hipStream_t st) {
-  if (njobs) hipLaunchKernelGGL(sc_tail_group_kernel, dim3(njobs), dim3(64), 0, st, jobs);
+hipError_t launch_sc_tail_group(const ScTailJob* jobs, uint32_t njobs, const RedJob* reds, uint32_t nreds,
+                                hipStream_t st) {
+  if (njobs + nreds)
+    hipLaunchKernelGGL(sc_tail_group_kernel, dim3(njobs + nreds), dim3(64), 0, st, jobs, njobs, reds);
   return hipGetLastError();
 }
 
 
-hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part /* >= 256 */, unsigned int* counter,
-                      ef2* out, hipStream_t st) {
-  uint32_t nb = (uint32_t)((n + 255) / 256);
-  if (nb > 256) nb = 256;
-  hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(256), 0, st, a, b, n, part, counter, out);
+uint32_t dot_blocks(uint64_t n) {
+  const uint64_t nb = (n + 255) / 256;
+  return (uint32_t)(nb > 256 ? 256 : nb == 0 ? 1 : nb);
+}
+
+hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part, hipStream_t st) {
+  hipLaunchKernelGGL(dot_kernel, dim3(dot_blocks(n)), dim3(256), 0, st, a, b, n, part);
   return hipGetLastError();
 }
 
@@ -254,17 +246,12 @@ hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part /* >= 25
 // B_I (ligesis.rs:403-410): bi[i * s + k] = bit (i mod 64) of F'[i / 64][idx[k]], as 0/1
 // extension elements; bi_m1 = bi - 1 (ligesis.rs:424-425).
 __global__ void bits_gather_kernel(const uint64_t* __restrict__ enc, uint64_t ld, const uint32_t* __restrict__ idx,
-                                   uint32_t s, uint64_t W, uint64_t* __restrict__ bi,
-                                   ef2* __restrict__ bi_ef, ef2* __restrict__ bi_m1, ef2* __restrict__ bi_ef2) {
+                                   uint32_t s, uint64_t W, uint64_t* __restrict__ bi) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= W * s) return;
   const uint64_t i = t / s;
   const uint32_t k = (uint32_t)(t % s);
-  const uint64_t bit = (enc[(i >> 6) * ld + idx[k]] >> (i & 63)) & 1;
-  bi[t] = bit;
-  bi_ef[t] = ef2{bit, 0};
-  bi_m1[t] = ef2{bit ? 0 : GL_P - 1, 0};
-  if (bi_ef2) bi_ef2[t] = ef2{bit, 0};
+  bi[t] = (enc[(i >> 6) * ld + idx[k]] >> (i & 63)) & 1;
 }
 
 // g2[i] = sum_k eq_r2[k] * bi[i * s + k] (ligesis.rs:445-455)
@@ -331,8 +318,7 @@ __global__ void planar_to_ef_kernel(const uint64_t* __restrict__ c0, const uint6
 template <int D>
 __device__ __forceinline__ void sc_round_body(ef2* __restrict__ p0, ef2* __restrict__ p1, ef2* __restrict__ p2,
                                               uint64_t half, const ef2* __restrict__ r_ptr, ef2* __restrict__ part,
-                                              unsigned int* counter, ef2* __restrict__ out, uint32_t blk,
-                                              uint32_t nblk, ef2* sh) {
+                                              uint32_t blk, uint32_t nblk, ef2* sh) {
   ef2* ps[3] = {p0, p1, p2};
   const ef2 r = *r_ptr;
   ef2 acc[D + 1];
@@ -366,13 +352,93 @@ __device__ __forceinline__ void sc_round_body(ef2* __restrict__ p0, ef2* __restr
 #pragma unroll
     for (int q = 0; q <= D; q++) part[(uint64_t)blk * (D + 1) + q] = acc[q];
   }
-  last_block_reduce<D + 1>(part, counter, out, sh, nblk);
+}
+
+// First round of the binary-check sumcheck sum_x eq(x) b(x) (b(x) - 1) (ligesis.rs:405-409)
+// with b read as 0/1 words. Per pair (lo, hi) of the top variable, b(t) = lo + t (hi - lo):
+// b(t) (b(t) - 1) vanishes when lo = hi, and is t (t - 1) otherwise, so the round values are
+// s(0) = s(1) = 0, s(2) = 2 sum (2 E_hi - E_lo), s(3) = 6 sum (3 E_hi - 2 E_lo) over the pairs
+// with lo != hi -- the same field values the general product rule gives, from additions only.
+// The folded bit lo + r (hi - lo) is one of 0, 1, r, 1 - r.
+__device__ __forceinline__ void sc_bits3_body(const uint64_t* __restrict__ bits, ef2* __restrict__ b_out,
+                                              ef2* __restrict__ m1_out, ef2* __restrict__ e, uint64_t half,
+                                              const ef2* __restrict__ r_ptr, ef2* __restrict__ part, uint32_t blk,
+                                              uint32_t nblk, ef2* sh) {
+  const ef2 r = *r_ptr, one = ef_one();
+  const ef2 omr = ef_sub(one, r), rm1 = ef_sub(r, one), nr = ef_neg(r), m1 = ef_neg(one);
+  ef2 L = ef_zero(), H = ef_zero();
+  for (uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x; i < half; i += (uint64_t)nblk * blockDim.x) {
+    const uint64_t blo = bits[i], bhi = bits[i + half];
+    const ef2 elo = e[i], ehi = e[i + half];
+    e[i] = ef_add(elo, ef_mul(r, ef_sub(ehi, elo)));
+    ef2 bf, bm;
+    if (blo == bhi) {
+      bf = blo ? one : ef_zero();
+      bm = blo ? ef_zero() : m1;
+    } else {
+      bf = bhi ? r : omr;
+      bm = bhi ? rm1 : nr;
+      L = ef_add(L, elo);
+      H = ef_add(H, ehi);
+    }
+    b_out[i] = bf;
+    m1_out[i] = bm;
+  }
+  // 2 (2H - L) and 6 (3H - 2L), linear in the block's sums, so partials add up
+  const ef2 h2 = ef_add(H, H), l2 = ef_add(L, L);
+  const ef2 s2 = ef_sub(h2, L);
+  const ef2 s3 = ef_sub(ef_add(h2, H), l2);
+  const ef2 s3x2 = ef_add(s3, s3);
+  ef2 acc[4] = {ef_zero(), ef_zero(), ef_add(s2, s2), ef_add(s3x2, ef_add(s3x2, s3x2))};
+  block_sum<4>(acc, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) part[(uint64_t)blk * 4 + q] = acc[q];
+  }
+}
+
+// First round of sum_x b(x) G(x) with b read as 0/1 words (the batched bit opening,
+// ligesis.rs:513-522): s(0) = sum_{lo = 1} G_lo, s(1) = sum_{hi = 1} G_hi and
+// s(2) = sum (2 hi - lo)(2 G_hi - G_lo) = 4 s(1) - 2 C + s(0) with
+// C = sum_{hi = 1} G_lo + sum_{lo = 1} G_hi.
+__device__ __forceinline__ void sc_bits2_body(const uint64_t* __restrict__ bits, ef2* __restrict__ b_out,
+                                              ef2* __restrict__ g, uint64_t half, const ef2* __restrict__ r_ptr,
+                                              ef2* __restrict__ part, uint32_t blk, uint32_t nblk, ef2* sh) {
+  const ef2 r = *r_ptr, one = ef_one();
+  const ef2 omr = ef_sub(one, r);
+  ef2 A0 = ef_zero(), A1 = ef_zero(), C = ef_zero();
+  for (uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x; i < half; i += (uint64_t)nblk * blockDim.x) {
+    const uint64_t blo = bits[i], bhi = bits[i + half];
+    const ef2 glo = g[i], ghi = g[i + half];
+    g[i] = ef_add(glo, ef_mul(r, ef_sub(ghi, glo)));
+    if (blo) {
+      A0 = ef_add(A0, glo);
+      C = ef_add(C, ghi);
+    }
+    if (bhi) {
+      A1 = ef_add(A1, ghi);
+      C = ef_add(C, glo);
+    }
+    b_out[i] = blo == bhi ? (blo ? one : ef_zero()) : (bhi ? r : omr);
+  }
+  const ef2 a1x2 = ef_add(A1, A1), cx2 = ef_add(C, C);
+  ef2 acc[3] = {A0, A1, ef_add(ef_sub(ef_add(a1x2, a1x2), cx2), A0)};
+  block_sum<3>(acc, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) part[(uint64_t)blk * 3 + q] = acc[q];
+  }
 }
 
 // One round of several independent sumchecks in one launch (sumchecks run in lockstep): block
 // -> job by binary search over the jobs' first blocks.
-__global__ __launch_bounds__(256) void sc_round_group_kernel(const ScRoundJob* __restrict__ jobs, int njobs) {
+__global__ __launch_bounds__(256) void sc_round_group_kernel(const ScRoundJob* __restrict__ jobs, int njobs,
+                                                             uint32_t round_blocks, const RedJob* __restrict__ reds) {
   __shared__ ef2 sh[256 * 4];
+  if (blockIdx.x >= round_blocks) {
+    red_job_body(reds[blockIdx.x - round_blocks], sh);
+    return;
+  }
   int lo = 0, hi = njobs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) / 2;
@@ -381,14 +447,22 @@ __global__ __launch_bounds__(256) void sc_round_group_kernel(const ScRoundJob* _
   }
   const ScRoundJob& j = jobs[lo];
   const uint32_t blk = blockIdx.x - j.first_block;
-  if (j.d == 3)
-    sc_round_body<3>(j.p[0], j.p[1], j.p[2], j.half, j.r, j.part, j.counter, j.out, blk, j.nblk, sh);
+  if (j.bits) {
+    if (j.d == 3)
+      sc_bits3_body(j.bits, j.p[0], j.p[1], j.p[2], j.half, j.r, j.part, blk, j.nblk, sh);
+    else
+      sc_bits2_body(j.bits, j.p[0], j.p[1], j.half, j.r, j.part, blk, j.nblk, sh);
+  } else if (j.d == 3)
+    sc_round_body<3>(j.p[0], j.p[1], j.p[2], j.half, j.r, j.part, blk, j.nblk, sh);
   else
-    sc_round_body<2>(j.p[0], j.p[1], nullptr, j.half, j.r, j.part, j.counter, j.out, blk, j.nblk, sh);
+    sc_round_body<2>(j.p[0], j.p[1], nullptr, j.half, j.r, j.part, blk, j.nblk, sh);
 }
 
-hipError_t launch_sc_round_group(const ScRoundJob* jobs, int njobs, uint32_t total_blocks, hipStream_t st) {
-  if (total_blocks) hipLaunchKernelGGL(sc_round_group_kernel, dim3(total_blocks), dim3(256), 0, st, jobs, njobs);
+hipError_t launch_sc_round_group(const ScRoundJob* jobs, int njobs, uint32_t total_blocks, const RedJob* reds,
+                                 uint32_t nreds, hipStream_t st) {
+  if (total_blocks + nreds)
+    hipLaunchKernelGGL(sc_round_group_kernel, dim3(total_blocks + nreds), dim3(256), 0, st, jobs, njobs,
+                       total_blocks, reds);
   return hipGetLastError();
 }
 
@@ -459,9 +533,14 @@ __device__ __forceinline__ void eq_fold_codeword_at(const ef2* __restrict__ v, u
 // the last block of each channel reducing them into out[c], the low-variable fold of m, and the
 // eq table of the next round's residual point (pair sums).
 __global__ __launch_bounds__(256) void bf_round3_kernel(BfRound a, uint64_t half, const ef2* __restrict__ r_ptr) {
-  __shared__ ef2 sh[256];
+  __shared__ ef2 sh[256 * 4];
   const int c = blockIdx.y;
+  if (c == 4) {  // the previous round's l0 reductions
+    if (blockIdx.x < 3 && a.prev[blockIdx.x].nblk) red_job_body(a.prev[blockIdx.x], sh);
+    return;
+  }
   if (c == 3) {  // codeword fold, grid-stride over its half length
+    if (!a.cw_out) return;
     const ef2 r = *r_ptr;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.cw_half;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -481,35 +560,21 @@ __global__ __launch_bounds__(256) void bf_round3_kernel(BfRound a, uint64_t half
     if (half > 1 && i < half / 2) eq_next[i] = ef_add(eq[2 * i], eq[2 * i + 1]);
   }
   block_sum<1>(acc, sh);
-  ef2* part = a.part + (uint64_t)c * gridDim.x;
-  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
-  // per-channel reduction: the counter of channel c counts that channel's blocks
-  __shared__ int is_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    is_last = atomicAdd(a.counter + c, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  ef2 tot[1] = {ef_zero()};
-  const volatile uint64_t* pv = (const volatile uint64_t*)part;
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) tot[0] = ef_add(tot[0], ef2{pv[2 * b], pv[2 * b + 1]});
-  block_sum<1>(tot, sh);
-  if (threadIdx.x == 0) {
-    a.out[c][0] = tot[0];
-    a.counter[c] = 0;
-  }
+  if (threadIdx.x == 0) a.part[(uint64_t)c * gridDim.x + blockIdx.x] = acc[0];
+}
+
+uint32_t bf_round3_blocks(uint64_t half, uint64_t cw_half) {
+  // blocks of the three channels; the codeword fold (y = 3) grid-strides over the same count,
+  // at least 64 blocks while it has work for them; at least 3 for the reductions (y = 4)
+  uint32_t nb = sc_round_blocks(half);
+  const uint64_t fold_blocks = (cw_half + 255) / 256;
+  if (nb < 64 && fold_blocks > nb) nb = (uint32_t)(fold_blocks < 64 ? fold_blocks : 64);
+  return nb < 3 ? 3 : nb;
 }
 
 hipError_t launch_bf_round3(const BfRound& a, uint64_t half, const ef2* r, hipStream_t st) {
-  // blocks of the three channels; the codeword fold (y = 3) grid-strides over the same count,
-  // at least 64 blocks while it has work for them
-  uint32_t nb = sc_round_blocks(half);
-  const uint64_t fold_blocks = (a.cw_half + 255) / 256;
-  if (nb < 64 && fold_blocks > nb) nb = (uint32_t)(fold_blocks < 64 ? fold_blocks : 64);
-  hipLaunchKernelGGL(bf_round3_kernel, dim3(nb, a.cw_out ? 4 : 3), dim3(256), 0, st, a, half, r);
+  hipLaunchKernelGGL(bf_round3_kernel, dim3(bf_round3_blocks(half, a.cw_out ? a.cw_half : 0), 5), dim3(256), 0,
+                     st, a, half, r);
   return hipGetLastError();
 }
 
@@ -535,8 +600,8 @@ __global__ void gather_parts_kernel(const GatherPart* __restrict__ parts, uint32
 #define LGS_GRID(n) dim3((uint32_t)(((n) + 255) / 256)), dim3(256)
 
 hipError_t launch_bits_gather(const uint64_t* enc, uint64_t ld, const uint32_t* idx, uint32_t s, uint64_t W,
-                              uint64_t* bi, ef2* bi_ef, ef2* bi_m1, hipStream_t st, ef2* bi_ef2) {
-  hipLaunchKernelGGL(bits_gather_kernel, LGS_GRID(W * s), 0, st, enc, ld, idx, s, W, bi, bi_ef, bi_m1, bi_ef2);
+                              uint64_t* bi, hipStream_t st) {
+  hipLaunchKernelGGL(bits_gather_kernel, LGS_GRID(W * s), 0, st, enc, ld, idx, s, W, bi);
   return hipGetLastError();
 }
 hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st,

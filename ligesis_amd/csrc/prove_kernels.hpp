@@ -4,18 +4,39 @@
 
 namespace lgs {
 
+// Deferred reduction of per-block partial sums: out[q] = sum_{b < nblk} part[b * w + q] for
+// q < w <= 4. Kernels that produce partials leave them in memory; a block of a later launch on
+// the same stream adds them up, after the producing launch has completed, so no cross-workgroup
+// fence or completion counter is needed (an agent-scope fence writes back and invalidates the
+// L2 of the workgroup's XCD).
+struct RedJob {
+  const ef2* part;
+  ef2* out;
+  uint32_t nblk, w;
+};
+// Up to four reductions in one small launch (one block each).
+struct RedJobs {
+  RedJob j[4];
+  int n;
+};
+hipError_t launch_reduce_jobs(const RedJobs& r, hipStream_t st);
+
 // One round of one sumcheck inside a grouped launch (sc_round_group_kernel): blocks
 // [first_block, first_block + nblk) of the launch. p[2] is unused when d == 2.
 struct ScRoundJob {
   ef2* p[3];
   uint64_t half;
   const ef2* r;
-  ef2* part;              // nblk x (d + 1) partial sums
-  unsigned int* counter;  // zeroed; the job's last block re-arms it
-  ef2* out;               // d + 1 round evaluations
+  ef2* part;  // nblk x (d + 1) partial sums, reduced into the round's d + 1 values later
+  // bits: first round of a sumcheck whose first table is 0/1 valued, read from these words
+  // (2 half entries); the folded bit table goes to p[0] (and its copy minus one to p[1] when
+  // d = 3), the last table p[d - 1] folds in place. Null for an ordinary round.
+  const uint64_t* bits;
   uint32_t d, first_block, nblk, pad;
 };
-hipError_t launch_sc_round_group(const ScRoundJob* jobs, int njobs, uint32_t total_blocks, hipStream_t st);
+// Blocks past total_blocks run the reductions `reds` (one block each).
+hipError_t launch_sc_round_group(const ScRoundJob* jobs, int njobs, uint32_t total_blocks, const RedJob* reds,
+                                 uint32_t nreds, hipStream_t st);
 
 // The remaining rounds (len <= 128 entries) of one sumcheck, one wave per job.
 struct ScTailJob {
@@ -25,7 +46,9 @@ struct ScTailJob {
   ef2* finals;
   uint32_t d, len;
 };
-hipError_t launch_sc_tail_group(const ScTailJob* jobs, uint32_t njobs, hipStream_t st);
+// Blocks past njobs run the reductions `reds`.
+hipError_t launch_sc_tail_group(const ScTailJob* jobs, uint32_t njobs, const RedJob* reds, uint32_t nreds,
+                                hipStream_t st);
 
 // One Basefold reduction round over its three channels (bf_round3_kernel, basefold.rs:748-775).
 struct BfRound {
@@ -33,15 +56,17 @@ struct BfRound {
   const ef2* eq[3];
   ef2* m_out[3];
   ef2* eq_next[3];
-  ef2* out[3];
-  ef2* part;              // 3 x blocks
-  unsigned int* counter;  // 3 zeroed words
+  ef2* part;  // 3 x blocks partial sums of l0 (channel-major)
+  RedJob prev[3];  // the previous round's reductions, run by blocks (c, 4) of this launch
   // the same round's fold of the combined codeword (eq_fold_codeword), in the same launch
   const ef2* cw_in;
   uint64_t cw_half;
   const uint64_t* cw_tw;
   ef2* cw_out;
 };
+// Grid: bf_round3_blocks(half, cw_half) x 5 (rows 0-2 channels, 3 codeword fold, 4 the previous
+// round's reductions); the partials of this round are that many per channel.
+uint32_t bf_round3_blocks(uint64_t half, uint64_t cw_half);
 hipError_t launch_bf_round3(const BfRound& a, uint64_t half, const ef2* r, hipStream_t st);
 
 // Up to three Basefold channels combined in one pass (basefold.rs:732-741).
@@ -75,10 +100,12 @@ hipError_t launch_wrows(const uint64_t* M, uint32_t rows, uint64_t cols, const e
                         uint32_t chunks, ef2* out, hipStream_t st, ef2* out2 = nullptr, ef2* out3 = nullptr);
 // Partial sums are reduced by the last block to finish; `counter` is a zeroed device word that
 // the kernel re-arms, so consecutive launches on one stream may share it.
-hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part, unsigned int* counter, ef2* out,
-                      hipStream_t st);
+// Partial sums of sum_i a[i] b[i], one per block (dot_blocks(n) blocks), for a RedJob.
+uint32_t dot_blocks(uint64_t n);
+hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part, hipStream_t st);
+// B_I as 0/1 words: bi[i * s + k] = bit i of column idx[k] of the encoded rows.
 hipError_t launch_bits_gather(const uint64_t* enc, uint64_t ld, const uint32_t* idx, uint32_t s, uint64_t W,
-                              uint64_t* bi, ef2* bi_ef, ef2* bi_m1, hipStream_t st, ef2* bi_ef2 = nullptr);
+                              uint64_t* bi, hipStream_t st);
 hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st,
                      ef2* g2b = nullptr);
 hipError_t launch_vvec(const ef2* eqz1, uint64_t W, ef2* v, hipStream_t st);

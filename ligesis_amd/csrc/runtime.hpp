@@ -194,10 +194,19 @@ struct Context {
   int merkle_lg = 0;     // 0 auto; log2 threads per Merkle workgroup = levels per launch (6..8)
   int pipeline = 0;      // 0 auto; 1 off, 2 / 4 / 8 row chunks (transform on stream, hash on stream2)
   bool profile = false;
-  double stage_ms[8] = {0};
+  double stage_ms[12] = {0};
   hipEvent_t ev[8] = {};
   uint8_t* pinned = nullptr;  // small pinned staging buffer for roots
   size_t pinned_size = 0;
+  // Pinned host arena for the job tables and constants one prove or batched commit uploads: a
+  // copy from pageable memory starts only once its stream has drained, which holds the issuing
+  // thread behind the device. Bump-allocated in chunks; upload_reset() may run only when no copy
+  // from the arena is pending (after the streams that read it were synchronized).
+  std::vector<std::pair<uint8_t*, size_t>> up_chunks;
+  size_t up_used = 0;  // bytes used in the last chunk
+  void upload_reset();
+  void upload(void* dst, const void* src, size_t bytes, hipStream_t s);
+  void free_host();  // pinned staging and arena
 
   const uint64_t* twiddle(int logN);
   uint8_t* staging(size_t bytes);
@@ -206,7 +215,7 @@ struct Context {
   hipEvent_t pipe_event(int i);
   // profiled pipelined commit: events around chunk q's transform (4q, 4q + 1) and hash (4q + 2,
   // 4q + 3); prof_chunks = number of chunks of the last profiled commit (0: not pipelined)
-  hipEvent_t tev[32] = {};
+  hipEvent_t tev[40] = {};  // [32..37]: phases of a profiled prove
   int prof_chunks = 0;
   hipEvent_t timing_event(int i);
   // launch schedule of a commit of shape 2^log_m x 2^log_n

@@ -299,8 +299,7 @@ void server_stop(Server* s) {
       if (e) (void)hipEventDestroy(e);
     for (auto& e : c->tev)
       if (e) (void)hipEventDestroy(e);
-    if (c->pinned) (void)hipHostFree(c->pinned);
-    c->pinned = nullptr;
+    c->free_host();
     (void)hipStreamDestroy(c->stream);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
   }

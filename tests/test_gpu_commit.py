@@ -39,6 +39,25 @@ def test_commit_matches_oracle(gpu_ctx, shape, variant):
     gpu_ctx.set_option("hash", 0)
 
 
+@pytest.mark.parametrize("shape", [(1, 17), (0, 19)])
+def test_long_rows_match_oracle(gpu_ctx, shape):
+    """Rows of 2^18 and 2^20 points: LDS blocks of 2^13, then one and three global radix-2 stages
+    before the four-stage tail pass (ntt_impl 0), or radix-2 stages throughout (ntt_impl 1)."""
+    log_m, log_n = shape
+    inp = synth.Inputs(log_m, log_n, seed=40 + log_n)
+    pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx, commit_only=True)
+    ocd = oracle.CommitData(oracle.ProvingKey(log_m, log_n, inp.a, commit_a=False), inp.f, keep_enc=True)
+    try:
+        for impl in (0, 1):
+            print(f"long-row commit shape={shape} ntt_impl={impl}", flush=True)
+            gpu_ctx.set_option("ntt_impl", impl)
+            root, cd = Ligesis.commit(pk, inp.f)
+            np.testing.assert_array_equal(cd.encoded(), ocd.enc())
+            assert root == ocd.root
+    finally:
+        gpu_ctx.set_option("ntt_impl", -1)
+
+
 def test_mfma_and_valu_hash_agree(gpu_ctx):
     inp = synth.Inputs(5, 9, seed=3)
     pk, _ = Ligesis.setup(5, 9, inp.a, ctx=gpu_ctx, commit_only=True)
