@@ -354,7 +354,7 @@ int lgs_prove(lgs_ctx* ctx, const lgs_key* key, const lgs_commitment* c, const u
     return LGS_ERR_ARG;
   return guarded(&ctx->c, [&] {
     (void)hipSetDevice(ctx->c.device);
-    std::vector<uint8_t> bytes;
+    std::vector<uint8_t>& bytes = ctx->c.proof_buf;  // reused: no allocation per proof
     uint64_t u[2] = {0, 0};
     try {
       lgs::prove_impl(ctx->c, key->k, c->c, z, z_len, fields, n_fields, ints, n_ints, bytes, u);

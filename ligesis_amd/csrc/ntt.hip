@@ -292,55 +292,50 @@ hipError_t launch_ntt_group(const NttJob* jobs, int njobs, uint32_t total_blocks
   return hipGetLastError();
 }
 
-// In-place radix-2 DIT network on 2^K values in bit-reversed order (natural-order output); the
-// twiddles are roots of order <= 16, i.e. powers of two with exponents fixed at compile time.
-template <int K>
-__device__ __forceinline__ void dft_br(uint64_t (&x)[1 << K]) {
-#pragma unroll
-  for (int s = 1; s <= K; s++) {
-    const int half = 1 << (s - 1);
-    const uint32_t es = (39u << (6 - s)) % 192;  // w_{2^s} = 2^es
-#pragma unroll
-    for (int g = 0; g < (1 << K); g += 2 * half) {
-#pragma unroll
-      for (int jj = 0; jj < half; jj++) {
-        const uint64_t u = x[g + jj];
-        uint64_t v = x[g + jj + half];
-        if (jj) v = gl_mul_pow2(v, (es * (uint32_t)jj) % 192);
-        x[g + jj] = gl_add(u, v);
-        x[g + jj + half] = gl_sub(u, v);
-      }
-    }
-  }
-}
-
-// The last K stages of a transform of 2^logN points in one pass. Before it, the in-place DIT
-// state holds, in block b of T = 2^(logN - K) positions, the T-point DFT Y_c of the subsequence
-// x[c + k 2^K] with c = rev_K(b); the result is X[j + T q] = sum_c w_N^(c j) w_S^(c q) Y_c[j]
-// (S = 2^K): for every j an S-point DFT of twiddled values, read and written at stride T by one
-// thread (consecutive threads, consecutive j: coalesced).
-template <int K>
-__global__ __launch_bounds__(256) void ntt_tail_kernel(uint64_t* __restrict__ data, uint32_t rows, int logN,
-                                                       const uint64_t* __restrict__ tw) {
-  constexpr int S = 1 << K;
-  const int lt = logN - K;
-  const uint64_t T = uint64_t(1) << lt;
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (uint64_t)rows * T) return;
-  const uint64_t row = gid >> lt, j = gid & (T - 1);
+// The last K (<= 7) stages of a transform of 2^logN points. The first logN - K stages of the
+// in-place DIT left, in block b of T = 2^(logN - K) positions, the T-point DFT Y_c (c = rev_K(b))
+// of the subsequence x[c + k 2^K]; the result is X[j + T q] = sum_c w_S^(c q) (w_N^(c j) Y_c[j])
+// with S = 2^K: for every j an S-point DFT of twiddled values. A workgroup takes kColJ
+// consecutive j of one row: it loads the kColJ x S values (runs of kColJ words), twiddles them,
+// keeps each j's sequence in LDS in bit-reversed order, runs K radix-2 DIT stages there and
+// stores the natural-order result at stride T (in place: it reads and writes the same words).
+constexpr uint32_t kColJ = 8;
+__global__ __launch_bounds__(256) void ntt_cols_kernel(uint64_t* __restrict__ data, int logN, int K,
+                                                      const uint64_t* __restrict__ tw) {
+  __shared__ uint64_t sm[kColJ * 128];
+  const uint32_t S = 1u << K;
+  const uint64_t T = uint64_t(1) << (logN - K);
+  const uint32_t jblocks = (uint32_t)(T / kColJ);
+  const uint64_t row = blockIdx.x / jblocks;
+  const uint64_t j0 = (uint64_t)(blockIdx.x % jblocks) * kColJ;
   uint64_t* d = data + (row << logN);
   const uint32_t half = 1u << (logN - 1);
-  uint64_t x[S];
-#pragma unroll
-  for (int t = 0; t < S; t++) x[t] = d[(uint64_t)t * T + j];
-#pragma unroll
-  for (int t = 1; t < S; t++) {
-    const uint32_t c = __brev((uint32_t)t) >> (32 - K);
-    x[t] = gl_mul(x[t], tw_full(tw, (uint32_t)(c * j), half, 0));
+  const uint32_t n = kColJ * S;
+  for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+    const uint32_t t = e / kColJ, jj = e % kColJ;
+    uint64_t v = d[(uint64_t)t * T + j0 + jj];
+    const uint64_t cj = (uint64_t)brev_bits(t, K) * (j0 + jj);  // < 2^logN
+    if (cj) v = gl_mul(v, tw_full(tw, (uint32_t)cj, half, 0));
+    sm[jj * S + t] = v;
   }
-  dft_br<K>(x);
-#pragma unroll
-  for (int q = 0; q < S; q++) d[(uint64_t)q * T + j] = x[q];
+  __syncthreads();
+  for (int s = 1; s <= K; s++) {
+    const uint32_t hs = 1u << (s - 1);
+    for (uint32_t b = threadIdx.x; b < n / 2; b += blockDim.x) {
+      const uint32_t jj = b >> (K - 1), w = b & (S / 2 - 1);
+      const uint32_t k = w & (hs - 1), at = jj * S + ((w >> (s - 1)) << s) + k;
+      const uint64_t u = sm[at];
+      uint64_t v = sm[at + hs];
+      if (k) v = gl_mul(v, tw[(uint64_t)k << (logN - s)]);  // w_{2^s}^k
+      sm[at] = gl_add(u, v);
+      sm[at + hs] = gl_sub(u, v);
+    }
+    __syncthreads();
+  }
+  for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+    const uint32_t q = e / kColJ, jj = e % kColJ;
+    d[(uint64_t)q * T + j0 + jj] = sm[jj * S + q];
+  }
 }
 
 // One global radix-2 stage s (> lt) over every row, in place.
@@ -402,41 +397,41 @@ hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_i
                        n_in, logN, tw, out);
     return hipGetLastError();
   }
-  int lt = logN < LGS_LDS_LOG ? logN : LGS_LDS_LOG;
-  uint32_t T = 1u << lt;
-  uint32_t nblk = 1u << (logN - lt);
-  uint32_t threads = T / 2 < 256 ? (T / 2 < 64 ? 64 : T / 2) : 256;
-  // the last (up to 4) stages in one tail pass instead of one global pass each
-  const int kt = impl == 1 ? 0 : (logN - lt < 4 ? logN - lt : 4);
-  if (impl != 1) {
-    // few workgroups (few rows): a thread per radix-8 butterfly shortens each block's transform
-    const uint32_t bt = (uint64_t)rows * nblk < 512 ? 1024 : 256;
-    hipLaunchKernelGGL(ntt_r8_blocks_kernel, dim3(rows * nblk), dim3(bt), ntt_lds_bytes(T), st, in, rs,
-                       es, n_in, logN, lt, tw, out);
-  } else {
-    hipLaunchKernelGGL(ntt_lds_kernel, dim3(rows * nblk), dim3(threads), T * sizeof(uint64_t), st,
-                       in, rs, es, n_in, logN, lt, tw, out);
+  if (impl == 1) {  // radix-2: LDS stages, then one global pass per remaining stage
+    const int lt = logN < LGS_LDS_LOG ? logN : LGS_LDS_LOG;
+    const uint32_t T = 1u << lt, nblk = 1u << (logN - lt);
+    const uint32_t threads = T / 2 < 256 ? (T / 2 < 64 ? 64 : T / 2) : 256;
+    hipLaunchKernelGGL(ntt_lds_kernel, dim3(rows * nblk), dim3(threads), T * sizeof(uint64_t), st, in, rs, es,
+                       n_in, logN, lt, tw, out);
+    hipError_t e = hipGetLastError();
+    const uint32_t blocks = (uint32_t)((((uint64_t)rows << (logN - 1)) + 255) / 256);
+    for (int s = lt + 1; s <= logN && e == hipSuccess; s++) {
+      hipLaunchKernelGGL(ntt_global_stage_kernel, dim3(blocks), dim3(256), 0, st, out, rows, logN, s, tw);
+      e = hipGetLastError();
+    }
+    return e;
   }
+  // Longer than one LDS tile: 2^lt-point transforms of strided subsequences in LDS (lt = 10 while
+  // that leaves at most 7 stages, so that few rows still give many workgroups), global radix-2
+  // stages while more than 7 remain, then the last K <= 7 stages in one column pass.
+  const int K = logN - 10 < 7 ? logN - 10 : 7;
+  const int lt = logN - K > LGS_LDS_LOG ? LGS_LDS_LOG : logN - K;
+  const uint32_t T = 1u << lt, nblk = 1u << (logN - lt);
+  uint32_t bt = T / 8;  // one radix-8 butterfly per thread and pass
+  if (bt > 256 && (uint64_t)rows * nblk >= 512) bt = 256;
+  hipLaunchKernelGGL(ntt_r8_blocks_kernel, dim3(rows * nblk), dim3(bt), ntt_lds_bytes(T), st, in, rs, es, n_in,
+                     logN, lt, tw, out);
   hipError_t e = hipGetLastError();
+  const uint32_t blocks = (uint32_t)((((uint64_t)rows << (logN - 1)) + 255) / 256);
+  for (int s = lt + 1; s <= logN - K && e == hipSuccess; s++) {
+    hipLaunchKernelGGL(ntt_global_stage_kernel, dim3(blocks), dim3(256), 0, st, out, rows, logN, s, tw);
+    e = hipGetLastError();
+  }
   if (e != hipSuccess) return e;
-  uint64_t work = (uint64_t)rows << (logN - 1);
-  uint32_t blocks = (uint32_t)((work + 255) / 256);
-  for (int s = lt + 1; s <= logN - kt; s++) {
-    hipLaunchKernelGGL(ntt_global_stage_kernel, dim3(blocks), dim3(256), 0, st, out, rows, logN, s,
-                       tw);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  if (kt > 0) {
-    const uint64_t cols = (uint64_t)rows << (logN - kt);
-    const dim3 g((uint32_t)((cols + 255) / 256));
-    if (kt == 1) hipLaunchKernelGGL(ntt_tail_kernel<1>, g, dim3(256), 0, st, out, rows, logN, tw);
-    else if (kt == 2) hipLaunchKernelGGL(ntt_tail_kernel<2>, g, dim3(256), 0, st, out, rows, logN, tw);
-    else if (kt == 3) hipLaunchKernelGGL(ntt_tail_kernel<3>, g, dim3(256), 0, st, out, rows, logN, tw);
-    else hipLaunchKernelGGL(ntt_tail_kernel<4>, g, dim3(256), 0, st, out, rows, logN, tw);
-    e = hipGetLastError();
-  }
-  return e;
+  hipLaunchKernelGGL(ntt_cols_kernel, dim3((uint32_t)(((uint64_t)rows << (logN - K)) / kColJ)), dim3(256), 0, st, out,
+                     logN, K, tw);
+  return hipGetLastError();
 }
+
 
 }  // namespace lgs

@@ -104,14 +104,18 @@ struct Gather {
 
 static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "proof encoding copies host words as little-endian");
 
+// Appends into a caller-held vector whose size serves as capacity (a reused vector is written
+// without reallocating or zero-filling); `n` bytes are valid.
 struct Writer {
-  std::vector<uint8_t> b;
+  std::vector<uint8_t>& b;
+  size_t n = 0;
+  explicit Writer(std::vector<uint8_t>& out) : b(out) {}
   // little-endian words; the host is little-endian (x86-64), so words are copied as they are
   void u64(uint64_t v) { words(&v, 1); }
-  void words(const uint64_t* w, size_t n) {
-    const size_t at = b.size();
-    b.resize(at + 8 * n);
-    std::memcpy(b.data() + at, w, 8 * n);
+  void words(const uint64_t* w, size_t k) {
+    if (n + 8 * k > b.size()) b.resize(std::max(n + 8 * k, 2 * b.size()));
+    std::memcpy(b.data() + n, w, 8 * k);
+    n += 8 * k;
   }
   void raw(const uint64_t* w, size_t n) {  // n words as raw little-endian bytes (digests)
     words(w, n);
@@ -643,8 +647,8 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   const uint64_t* GW = host + 2 * res_n;     // gathered words
 
   // ---- encode (field order of LigesisProof, ligesis.rs:99-119) ----
-  Writer w;
-  w.b.reserve(8 * (stage_words + 4096));
+  Writer w(proof);
+  if (proof.size() < 8 * (stage_words + 4096)) proof.resize(8 * (stage_words + 4096));
   w.ef(RS + 2 * o_y);
   w.raw(GW + g_com_av, 4);
   w.raw(GW + g_com_i, 4);
@@ -705,8 +709,8 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   }
   w.u64(1);  // code_rate
   w.u64(ilog2(D));
-  proof.swap(w.b);
-  return (int64_t)proof.size();
+  proof.resize(w.n);
+  return (int64_t)w.n;
 }
 
 }  // namespace lgs

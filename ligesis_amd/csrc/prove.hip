@@ -254,16 +254,31 @@ __global__ void bits_gather_kernel(const uint64_t* __restrict__ enc, uint64_t ld
   bi[t] = (enc[(i >> 6) * ld + idx[k]] >> (i & 63)) & 1;
 }
 
-// g2[i] = sum_k eq_r2[k] * bi[i * s + k] (ligesis.rs:445-455)
-__global__ void g2_kernel(const uint64_t* __restrict__ bi, uint32_t s, uint64_t W, const ef2* __restrict__ eq_r2,
-                          ef2* __restrict__ g2, ef2* __restrict__ g2b) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= W) return;
+// g2[i] = sum_k eq_r2[k] * bi[i * s + k] (ligesis.rs:445-455): one wave per i, its lanes
+// reading consecutive words of the row (coalesced), then a butterfly sum across the wave.
+__device__ __forceinline__ ef2 wave_sum(ef2 v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const ef2 o{(uint64_t)__shfl_xor((unsigned long long)v.c0, off), (uint64_t)__shfl_xor((unsigned long long)v.c1, off)};
+    v = ef_add(v, o);
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(256) void g2_kernel(const uint64_t* __restrict__ bi, uint32_t s, uint64_t W,
+                                                 const ef2* __restrict__ eq_r2, ef2* __restrict__ g2,
+                                                 ef2* __restrict__ g2b) {
+  const uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63;
+  if (i >= W) return;  // whole waves leave together
   ef2 acc = ef_zero();
-  for (uint32_t k = 0; k < s; k++)
+  for (uint32_t k = lane; k < s; k += 64)
     if (bi[i * s + k]) acc = ef_add(acc, eq_r2[k]);
-  g2[i] = acc;
-  if (g2b) g2b[i] = acc;
+  acc = wave_sum(acc);
+  if (lane == 0) {
+    g2[i] = acc;
+    if (g2b) g2b[i] = acc;
+  }
 }
 
 // v[i] = eqz1[i / 64] * 2^(i mod 64) (ligesis.rs:474-476)
@@ -606,7 +621,7 @@ hipError_t launch_bits_gather(const uint64_t* enc, uint64_t ld, const uint32_t* 
 }
 hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st,
                      ef2* g2b) {
-  hipLaunchKernelGGL(g2_kernel, LGS_GRID(W), 0, st, bi, s, W, eq_r2, g2, g2b);
+  hipLaunchKernelGGL(g2_kernel, LGS_GRID(W * 64), 0, st, bi, s, W, eq_r2, g2, g2b);
   return hipGetLastError();
 }
 hipError_t launch_vvec(const ef2* eqz1, uint64_t W, ef2* v, hipStream_t st) {

@@ -207,6 +207,7 @@ struct Context {
   void upload_reset();
   void upload(void* dst, const void* src, size_t bytes, hipStream_t s);
   void free_host();  // pinned staging and arena
+  std::vector<uint8_t> proof_buf;  // encoded proof of the last lgs_prove
 
   const uint64_t* twiddle(int logN);
   uint8_t* staging(size_t bytes);
