@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="batch workload: number of polynomials")
     ap.add_argument("--requests", type=int, default=48, help="serve workload: number of requests")
     ap.add_argument("--log-m", type=int, default=8, help="prove workload: log2 rows")
+    ap.add_argument("--prove-graph", type=int, default=0,
+                    help="prove workload: replay repeated proofs as a captured graph (1) or not (0)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="commit workload: also run a 256 MiB device-to-device copy (a plain copy "
                          "kernel of known size that scales the profiler's FETCH_SIZE / WRITE_SIZE)")
@@ -340,6 +342,7 @@ def bench_prove(args):
     print(f"bench: prove mu={MU} shape 2^{lm} x 2^{ln}", file=sys.stderr, flush=True)
     inp = synth.Inputs(lm, ln, seed=5)
     ctx = L.Context(0)
+    ctx.set_option("prove_graph", args.prove_graph)
     pk, vk = L.Ligesis.setup(lm, ln, inp.a, ctx=ctx)
     root, cd = L.Ligesis.commit(pk, inp.f)
     settle(lambda: L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints)))
@@ -357,8 +360,8 @@ def bench_prove(args):
     # one more proof with host-side timing: how long issuing the work takes vs the whole proof
     ctx.set_option("profile", 1)
     L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints))
-    raw = np.zeros(12, np.float64)
-    ctx.check(L.lib().lgs_stage_times(ctx.h, L._ptr(raw), 12))
+    raw = np.zeros(28, np.float64)
+    ctx.check(L.lib().lgs_stage_times(ctx.h, L._ptr(raw), 28))
     ctx.set_option("profile", 0)
     out = {
         "metric": "LigeSIS eval-prove ms at mu=24",
@@ -379,7 +382,15 @@ def bench_prove(args):
         "device_bound_ms": float(raw[6]),
         "device_phases_ms": {"sumcheck_start": float(raw[7]), "sumcheck_end": float(raw[8]),
                              "basefold_open_end": float(raw[9]), "gathered": float(raw[10]),
-                             "downloaded": float(raw[11])},
+                             "downloaded": float(raw[11]), "side_start": float(raw[12]),
+                             "bi_committed": float(raw[13]), "abar_committed": float(raw[14]),
+                             "combined": float(raw[15]), "tail_rounds_start": float(raw[16])},
+        "host_issued_ms": {"first_upload": float(raw[17]), "sumcheck_start": float(raw[18]),
+                           "sumcheck_end": float(raw[19]), "basefold_open_end": float(raw[20]),
+                           "gathered": float(raw[21]), "downloaded": float(raw[22]),
+                           "side_start": float(raw[23]), "bi_committed": float(raw[24]),
+                           "abar_committed": float(raw[25]), "combined": float(raw[26]),
+                           "tail_rounds_start": float(raw[27])},
         "proof_bytes": len(proof),
     }
     del cd, pk

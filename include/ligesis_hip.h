@@ -68,18 +68,23 @@ const char* lgs_last_error(const lgs_ctx* ctx);
  * stages), "merkle_lg" (0 auto; 6..8 = log2 threads per Merkle workgroup = tree levels per
  * launch), "pipeline" (0 auto; 1 off; 2 / 4 / 8: the rows are encoded in that many chunks and
  * each chunk's K-splits are hashed on a second stream while the next chunk is encoded),
- * "profile" (1: time every stage with HIP events; pipelined stages report the summed durations
- * of their launches). An option left at auto takes the autotuned schedule of the commit's shape
- * when the process tuning table has one (lgs_autotune, lgs_tune_load), else a built-in
- * default. */
+ * "prove_graph" (0 default; 1: the second proof of a commitment on its own context captures
+ * its launches into a HIP graph, kept with the commitment, and later proofs replay it with their
+ * own constants), "profile" (1: time every stage with HIP events; pipelined stages report the
+ * summed durations of their launches; proofs are then issued without a graph). An option left
+ * at auto takes the autotuned schedule of the commit's shape when the process tuning table has
+ * one (lgs_autotune, lgs_tune_load), else a built-in default. */
 int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value);
 
 /* Stage times of the last profiled call, in milliseconds. Commit: [0] row NTT, [1] hash main
  * kernel, [2] hash finalize, [3] Merkle tree, [4] whole commit. Prove: [5] host time to issue
  * the work, [6] host time to the proof's download; device timestamps from the first upload:
  * [7] first sumcheck round starts, [8] last sumcheck launch ends, [9] the Basefold open on the
- * second stream ends, [10] openings gathered, [11] proof downloaded. Up to `n` (<= 12) values
- * are written. */
+ * second stream ends, [10] openings gathered, [11] proof downloaded, and on the second stream
+ * [12] its work starts, [13] B_I's commitment done, [14] a-bar's commitment done, [15] the
+ * batched codeword combined, [16] the single-workgroup rounds start; [17 + k] the host time
+ * (from the call's start) at which the work before device timestamp [7 + k] had been issued
+ * ([17] for the first upload). Up to `n` (<= 28) values are written. */
 int lgs_stage_times(const lgs_ctx* ctx, double* out_ms, uint32_t n);
 
 /* Commit launch schedule: {hash_variant, hash_impl, hash_ksplit, ntt_impl, merkle_lg, pipeline}

@@ -164,6 +164,8 @@ int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value) {
     ctx->c.merkle_lg = (int)value;
   } else if (n == "pipeline" && (value == 0 || value == 1 || value == 2 || value == 4 || value == 8)) {
     ctx->c.pipeline = (int)value;
+  } else if (n == "prove_graph" && (value == 0 || value == 1)) {
+    ctx->c.prove_graph = value != 0;
   } else if (n == "profile") {
     ctx->c.profile = value != 0;
   } else {
@@ -225,7 +227,7 @@ void lgs_tune_clear(void) { lgs::tune_clear(); }
 
 int lgs_stage_times(const lgs_ctx* ctx, double* out_ms, uint32_t n) {
   if (!ctx || !out_ms) return LGS_ERR_ARG;
-  for (uint32_t i = 0; i < n && i < 12; i++) out_ms[i] = ctx->c.stage_ms[i];
+  for (uint32_t i = 0; i < n && i < 28; i++) out_ms[i] = ctx->c.stage_ms[i];
   return LGS_OK;
 }
 

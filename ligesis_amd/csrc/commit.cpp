@@ -48,43 +48,48 @@ uint8_t* Context::staging(size_t bytes) {
   return pinned;
 }
 
-void Context::upload_reset() {
+void PinnedArena::reset() {
   size_t total = 0;
-  for (auto& c : up_chunks) total += c.second;
-  if (up_chunks.size() > 1) {  // merge into one chunk of the combined size
-    for (auto& c : up_chunks) (void)hipHostFree(c.first);
-    up_chunks.clear();
+  for (auto& c : chunks_) total += c.second;
+  if (chunks_.size() > 1) {  // merge into one chunk of the combined size
+    release();
     uint8_t* p = nullptr;
     hip_check(hipHostMalloc((void**)&p, total, hipHostMallocDefault), "hipHostMalloc");
-    up_chunks.emplace_back(p, total);
+    chunks_.emplace_back(p, total);
   }
-  up_used = 0;
+  used_ = 0;
 }
 
-void Context::upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
-  if (bytes == 0) return;
-  size_t at = (up_used + 255) & ~size_t(255);
-  if (up_chunks.empty() || at + bytes > up_chunks.back().second) {  // new chunk; earlier ones stay
+void* PinnedArena::upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return nullptr;
+  size_t at = (used_ + 255) & ~size_t(255);
+  if (chunks_.empty() || at + bytes > chunks_.back().second) {  // new chunk; earlier ones stay
     size_t sz = std::max<size_t>(bytes, size_t(1) << 20);
-    if (!up_chunks.empty()) sz = std::max(sz, 2 * up_chunks.back().second);
+    if (!chunks_.empty()) sz = std::max(sz, 2 * chunks_.back().second);
     uint8_t* p = nullptr;
     hip_check(hipHostMalloc((void**)&p, sz, hipHostMallocDefault), "hipHostMalloc");
-    up_chunks.emplace_back(p, sz);
+    chunks_.emplace_back(p, sz);
     at = 0;
   }
-  uint8_t* h = up_chunks.back().first + at;
+  uint8_t* h = chunks_.back().first + at;
   std::memcpy(h, src, bytes);
-  up_used = at + bytes;
+  used_ = at + bytes;
   hip_check(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s), "upload");
+  return h;
+}
+
+void PinnedArena::release() {
+  for (auto& c : chunks_) (void)hipHostFree(c.first);
+  chunks_.clear();
+  used_ = 0;
 }
 
 void Context::free_host() {
   if (pinned) (void)hipHostFree(pinned);
   pinned = nullptr;
   pinned_size = 0;
-  for (auto& c : up_chunks) (void)hipHostFree(c.first);
-  up_chunks.clear();
-  up_used = 0;
+  arena.release();
+  prove_arena.release();
 }
 
 void enqueue_basefold_commit(Context& ctx, DevBasefold& st, const uint64_t* poly, bool ext,
@@ -256,7 +261,7 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
 
 void enqueue_merkle_trees(Context& ctx, const std::vector<MerkleSource>& trees,
                           std::vector<std::vector<MerkleJob>>& host_jobs, std::vector<DevBuf<uint64_t>>& tables,
-                          hipStream_t st) {
+                          hipStream_t st, PinnedArena* arena) {
   // rounds of up to 8 levels; every tree takes part until its root is written
   const size_t nt = trees.size();
   std::vector<uint64_t> off(nt, 0), count(nt);
@@ -300,7 +305,7 @@ void enqueue_merkle_trees(Context& ctx, const std::vector<MerkleSource>& trees,
   const std::vector<MerkleJob>& h = host_jobs.back();
   tables.emplace_back(&ctx.pool, (h.size() * sizeof(MerkleJob) + 7) / 8);
   const MerkleJob* d = (const MerkleJob*)tables.back().ptr;
-  ctx.upload((void*)d, h.data(), h.size() * sizeof(MerkleJob), st);
+  (arena ? arena : &ctx.arena)->upload((void*)d, h.data(), h.size() * sizeof(MerkleJob), st);
   for (size_t r = 0; r < rounds.size(); r++)
     hip_check(launch_merkle_group(d + at[r], (int)rounds[r].size(), blocks[r], st), "grouped merkle");
 }

@@ -91,6 +91,34 @@ LGS_HD uint64_t gl_root(int bits) {
   return g;
 }
 
+// Sum of up to 2^32 products x y of canonical values, kept as the integer lo + 2^64 hi +
+// 2^128 top and reduced once at the end (2^128 = -2^32 mod p): a multiply-add costs one wide
+// product and a few adds instead of a reduction.
+struct WideAcc {
+  uint64_t lo, hi;
+  uint32_t top;
+};
+LGS_HD WideAcc wide_zero() { return WideAcc{0, 0, 0}; }
+LGS_HD void wide_mul_add(WideAcc& a, uint64_t x, uint64_t y) {
+  uint64_t pl, ph;
+  mul_wide(x, y, pl, ph);
+  a.lo += pl;
+  // ph <= (p - 1)^2 / 2^64 < 2^64 - 2^33 + 2, so ph + carry does not wrap
+  const uint64_t t = ph + (a.lo < pl ? 1 : 0);
+  a.hi += t;
+  a.top += a.hi < t ? 1u : 0u;
+}
+LGS_HD void wide_add(WideAcc& a, uint64_t x) {
+  a.lo += x;
+  if (a.lo < x) {
+    a.hi += 1;
+    a.top += a.hi == 0 ? 1u : 0u;
+  }
+}
+LGS_HD uint64_t wide_reduce(const WideAcc& a) {
+  return gl_sub(gl_reduce(a.lo, a.hi), gl_canon((uint64_t)a.top << 32));
+}
+
 struct ef2 {
   uint64_t c0, c1;
 };

@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <deque>
+#include <memory>
 #include <vector>
 
 #include "prove_kernels.hpp"
@@ -131,6 +133,37 @@ struct Sumcheck {
 
 }  // namespace
 
+// The device side of one proof up to the openings: every scratch buffer, the Basefold states of
+// B_I and a-bar, the folded levels' trees and job tables, and the pinned staging of every upload
+// (blob_host / idx_host: the per-proof constants and column indices). A plan stays with its
+// commitment after a proof on the commitment's context; the next proof captures its launches
+// into a graph, and later ones refill the two staging slots and replay the graph.
+struct ProvePlan {
+  DevicePool* pool = nullptr;
+  std::deque<DevBuf<uint8_t>> bufs;
+  PinnedArena arena;
+  void* blob_host = nullptr;
+  void* idx_host = nullptr;
+  DevBasefold av_state, bi_state;
+  std::vector<DevTree> ftrees;
+  std::vector<std::vector<MerkleJob>> fjobs;
+  std::vector<DevBuf<uint64_t>> ftables;
+  ef2* res = nullptr;     // round polynomials, final evaluations, y, y4, l0 (layout in prove_impl)
+  ef2* levels = nullptr;  // the batched codeword and its folds, level after level
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  template <class T>
+  T* get(size_t n) {
+    bufs.emplace_back(pool, n * sizeof(T));
+    return (T*)bufs.back().ptr;
+  }
+  ~ProvePlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
 int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uint64_t* z, uint64_t z_len,
                    const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,
                    std::vector<uint8_t>& proof, uint64_t used[2]) {
@@ -216,20 +249,11 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     o_zp1[c] = put(EfV(zp[c].begin() + (R ? 1 : 0), zp[c].end()));
   }
 
-  hipStream_t st = ctx.stream;
-  DevicePool* pool = &ctx.pool;
-  DevBuf<ef2> dblob(pool, blob.size());
-  ctx.upload_reset();  // every earlier call on this context has been synchronized
-  // profile mode: device timestamps of the phases (stage_ms[7..11], from the first upload)
-  auto mark = [&](int i, hipStream_t s) {
-    if (ctx.profile) hip_check(hipEventRecord(ctx.timing_event(32 + i), s), "profile event");
-  };
-  mark(0, st);
-  ctx.upload(dblob.ptr, blob.data(), blob.size() * sizeof(ef2), st);
-  const ef2* B = dblob.ptr;
-  DevBuf<uint32_t> didx(pool, s);
-  ctx.upload(didx.ptr, idx.data(), s * 4, st);
-
+  // shapes and layouts (the same for every prove of this key)
+  const uint64_t D = key.a_state.D;  // shared Basefold domain
+  const int logN = log_n + 1;
+  const uint64_t nbi = W * s;  // entries of B_I
+  const uint64_t M = uint64_t(1) << R;
   // results buffer layout
   Sumcheck sc[7];
   const int sc_d[7] = {3, 2, 2, 2, 2, 2, 2};
@@ -246,27 +270,90 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   const size_t o_y = res_n++, o_y4 = res_n++;
   const size_t o_l0 = res_n;
   res_n += 3 * (size_t)R;
-  DevBuf<ef2> res(pool, res_n);
+  // levels 0..R stored back to back: level k has D >> k entries
+  std::vector<uint64_t> lvl_off(R + 1);
+  uint64_t tot = 0;
+  for (uint32_t k = 0; k <= R; k++) {
+    lvl_off[k] = tot;
+    tot += D >> k;
+  }
+
+  hipStream_t st = ctx.stream;
+  hipStream_t sb = ctx.side_stream();
+  DevicePool* pool = &ctx.pool;
+  ctx.upload_reset();  // every earlier call on this context has been synchronized
+  // profile mode: device timestamps of the phases (stage_ms[7..16], from the first upload)
+  auto mark = [&](int i, hipStream_t s) {
+    if (!ctx.profile) return;
+    hip_check(hipEventRecord(ctx.timing_event(32 + i), s), "profile event");
+    // host time at which the work up to this mark was issued
+    ctx.stage_ms[17 + i] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  };
+
+  // ---- every device step of the proof up to the openings, on st and sb, into plan P's
+  //      buffers; the blob and idx are uploaded from P's staging slots ----
+  auto issue = [&](ProvePlan& P) {
+  DevBasefold& av_state = P.av_state;
+  DevBasefold& bi_state = P.bi_state;
+  mark(0, st);
+  ef2* dblob = P.get<ef2>(blob.size());
+  P.blob_host = P.arena.upload(dblob, blob.data(), blob.size() * sizeof(ef2), st);
+  const ef2* B = dblob;
+  uint32_t* didx = P.get<uint32_t>(s);
+  P.idx_host = P.arena.upload(didx, idx.data(), s * 4, st);
+
+  // ---- the second stream: B_I and its Basefold commitment, then a-bar's, then Stage B ----
+  // B_I (ligesis.rs:402-410) needs only F' and the column indices, and its commitment
+  // (ligesis.rs:411-416) is the longest piece of the second stream, so the stream forks here;
+  // only Stage B reads the commitments (and the proof their roots), and Stage B joins st before
+  // the openings are gathered. Every twiddle table either stream needs is made on st first.
+  for (uint32_t t = 0; t < R; t++) (void)ctx.twiddle(ilog2(D >> t));
+  (void)ctx.twiddle(ilog2(D));
+  (void)ctx.twiddle(logN);
+  hip_check(hipEventRecord(ctx.ev[5], st), "fork");
+  hip_check(hipStreamWaitEvent(sb, ctx.ev[5], 0), "fork");
+  mark(6, sb);
+  // The two sumchecks over B_I read these 0/1 words in their first round, which writes the folded
+  // extension-field tables (half the length) that their later rounds work on; st waits for them
+  // (ev[8]) before its first use.
+  uint64_t* bi = P.get<uint64_t>(nbi);
+  uint8_t* bi8 = P.get<uint8_t>(nbi);
+  ef2* bi_ef = P.get<ef2>(nbi / 2);
+  ef2* bi_ef2 = P.get<ef2>(nbi / 2);
+  hip_check(launch_bits_gather(cm.enc.ptr, two_n, didx, (uint32_t)s, W, bi, bi8, sb), "B_I gather");
+  hip_check(hipEventRecord(ctx.ev[8], sb), "B_I ready");
+  enqueue_basefold_commit(ctx, bi_state, bi, false, nv_bi, li_bi, 1, sb);
+  mark(7, sb);
+
+  ef2* res = P.get<ef2>(res_n);
+  P.res = res;
 
   // partial sums of the two dot products (y, y4); the first sumcheck launch reduces them
-  DevBuf<ef2> part(pool, 512);
+  ef2* part = P.get<ef2>(512);
   // Every buffer below lives until the end of the prove: the second stream runs beside this
   // one, so no buffer may go back to the pool (and be handed out again) while work is queued.
 
   // ---- every eq table of the proof, in two launches (MlPoly::new_eq, poly.rs:44-53) ----
-  const uint64_t nbi = W * s;
-  const uint64_t M = uint64_t(1) << R;
-  DevBuf<ef2> eqz1(pool, m), eqz2(pool, n), big(pool, nbi), big2(pool, nbi), eqa2(pool, 32), eqr2(pool, s),
-      g_a(pool, n);
-  DevBuf<ef2> mbuf[3][2], eqr[3][2];
+  // big and big2 (2^Lb entries) are never expanded: the first round of their sumchecks evaluates
+  // them from their half tables and stores the folded halves here
+  ef2* eqz1 = P.get<ef2>(m);
+  ef2* eqz2 = P.get<ef2>(n);
+  ef2* big = P.get<ef2>(nbi / 2);
+  ef2* big2 = P.get<ef2>(nbi / 2);
+  ef2* eqa2 = P.get<ef2>(32);
+  ef2* eqr2 = P.get<ef2>(s);
+  ef2* g_a = P.get<ef2>(n);
+  ef2* mbuf[3][2];
+  ef2* eqr[3][2];
   for (int c = 0; c < 3; c++) {
-    mbuf[c][0] = DevBuf<ef2>(pool, M);
-    mbuf[c][1] = DevBuf<ef2>(pool, M / 2 + 1);
-    eqr[c][0] = DevBuf<ef2>(pool, M / 2 + 1);
-    eqr[c][1] = DevBuf<ef2>(pool, M / 4 + 1);
+    mbuf[c][0] = P.get<ef2>(M);
+    mbuf[c][1] = P.get<ef2>(M / 2 + 1);
+    eqr[c][0] = P.get<ef2>(M / 2 + 1);
+    eqr[c][1] = P.get<ef2>(M / 4 + 1);
   }
   std::vector<EqTerm> eq_terms;
-  std::vector<EqTable> eq_tabs;
+  std::vector<EqTable> all_tabs, eq_tabs, eq_fact;  // all in `want` order; expanded; factored
+  std::vector<bool> factored;
   uint64_t eq_half_words = 0;
   {
     // a table: out (2^k entries) = sum over (point, scale) terms of scale * eq(point, .)
@@ -274,18 +361,19 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
       ef2* out;
       uint32_t k;
       std::vector<std::pair<size_t, const ef2*>> terms;  // blob offset of the point, scale
+      bool factored;
     };
     std::vector<Want> want = {
-        {eqz1.ptr, log_m, {{o_z1, nullptr}}},
-        {eqz2.ptr, log_n, {{o_z2, nullptr}}},
-        {big.ptr, Lb, {{o_alpha1, nullptr}}},                       // binary check (ligesis.rs:424-429)
-        {eqa2.ptr, 5, {{o_alpha2, nullptr}}},                       // hash channel (ligesis.rs:436-444)
-        {eqr2.ptr, log_s, {{o_r2, nullptr}}},
-        {big2.ptr, Lb, {{o_r1, B + o_gbi}, {o_pbh, B + o_gbi + 1}, {o_pbr, B + o_gbi + 2}}},  // ligesis.rs:513-517
-        {g_a.ptr, log_n, {{o_z2, B + o_ga}, {o_r6, B + o_ga + 1}}},                          // ligesis.rs:518-522
+        {eqz1, log_m, {{o_z1, nullptr}}, false},
+        {eqz2, log_n, {{o_z2, nullptr}}, false},
+        {big, Lb, {{o_alpha1, nullptr}}, true},                 // binary check (ligesis.rs:424-429)
+        {eqa2, 5, {{o_alpha2, nullptr}}, false},                // hash channel (ligesis.rs:436-444)
+        {eqr2, log_s, {{o_r2, nullptr}}, false},
+        {big2, Lb, {{o_r1, B + o_gbi}, {o_pbh, B + o_gbi + 1}, {o_pbr, B + o_gbi + 2}}, true},  // 513-517
+        {g_a, log_n, {{o_z2, B + o_ga}, {o_r6, B + o_ga + 1}}, false},                         // 518-522
     };
     for (int c = 0; c < 3; c++)
-      if (R >= 1) want.push_back({eqr[c][0].ptr, R - 1, {{o_zp1[c], nullptr}}});  // Basefold rounds
+      if (R >= 1) want.push_back({eqr[c][0], R - 1, {{o_zp1[c], nullptr}}, false});  // Basefold rounds
     for (auto& w : want) {
       const uint32_t klo = w.k / 2, khi = w.k - klo;
       EqTable tb{};
@@ -305,134 +393,139 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
         eq_half_words += uint64_t(1) << khi;
         eq_terms.push_back(t);
       }
-      eq_tabs.push_back(tb);
+      all_tabs.push_back(tb);
+      factored.push_back(w.factored);
     }
   }
-  DevBuf<ef2> eq_half(pool, eq_half_words);
+  ef2* eq_half = P.get<ef2>(eq_half_words);
   uint32_t eq_term_blocks = 0, eq_tab_blocks = 0;
   {
-    size_t ti = 0;
-    for (auto& tb : eq_tabs) {
+    size_t ti = 0;  // terms were appended table by table in `want` order
+    for (size_t a = 0; a < all_tabs.size(); a++) {
+      EqTable& tb = all_tabs[a];
       for (uint32_t q = 0; q < tb.nterms; q++, ti++) {
         EqTerm& t = eq_terms[ti];
-        t.lo = eq_half.ptr + (uintptr_t)t.lo;
-        t.hi = eq_half.ptr + (uintptr_t)t.hi;
+        t.lo = eq_half + (uintptr_t)t.lo;
+        t.hi = eq_half + (uintptr_t)t.hi;
         t.first_block = eq_term_blocks;
         eq_term_blocks += (uint32_t)(((uint64_t(1) << std::max(t.klo, t.k - t.klo)) + 255) / 256);
         tb.lo[q] = t.lo;
         tb.hi[q] = t.hi;
       }
+      if (factored[a]) {  // big and big2, in that order
+        eq_fact.push_back(tb);
+        continue;
+      }
       tb.first_block = eq_tab_blocks;
       eq_tab_blocks += (uint32_t)((tb.n + 255) / 256);
+      eq_tabs.push_back(tb);
     }
   }
-  DevBuf<uint8_t> eq_jobs(pool, eq_terms.size() * sizeof(EqTerm) + eq_tabs.size() * sizeof(EqTable));
-  ctx.upload(eq_jobs.ptr, eq_terms.data(), eq_terms.size() * sizeof(EqTerm), st);
-  ctx.upload(eq_jobs.ptr + eq_terms.size() * sizeof(EqTerm), eq_tabs.data(), eq_tabs.size() * sizeof(EqTable), st);
-  hip_check(launch_eq_group((const EqTerm*)eq_jobs.ptr, (int)eq_terms.size(), eq_term_blocks,
-                            (const EqTable*)(eq_jobs.ptr + eq_terms.size() * sizeof(EqTerm)), (int)eq_tabs.size(),
+  const size_t terms_bytes = eq_terms.size() * sizeof(EqTerm), tabs_bytes = eq_tabs.size() * sizeof(EqTable);
+  uint8_t* eq_jobs = P.get<uint8_t>(terms_bytes + tabs_bytes + eq_fact.size() * sizeof(EqTable));
+  P.arena.upload(eq_jobs, eq_terms.data(), terms_bytes, st);
+  P.arena.upload(eq_jobs + terms_bytes, eq_tabs.data(), tabs_bytes, st);
+  P.arena.upload(eq_jobs + terms_bytes + tabs_bytes, eq_fact.data(), eq_fact.size() * sizeof(EqTable), st);
+  const EqTable* d_eq_fact = (const EqTable*)(eq_jobs + terms_bytes + tabs_bytes);  // [0] big, [1] big2
+  hip_check(launch_eq_group((const EqTerm*)eq_jobs, (int)eq_terms.size(), eq_term_blocks,
+                            (const EqTable*)(eq_jobs + eq_terms.size() * sizeof(EqTerm)), (int)eq_tabs.size(),
                             eq_tab_blocks, st),
             "eq tables");
 
   // ---- a-bar = eq_{z1}^T F, y = a-bar(z2) (ligesis.rs:365-379); two copies for the sumchecks
   //      that fold a-bar in place ----
-  DevBuf<ef2> a_vec(pool, n), a_copy(pool, n), a_copy2(pool, n);
+  ef2* a_vec = P.get<ef2>(n);
+  ef2* a_copy = P.get<ef2>(n);
+  ef2* a_copy2 = P.get<ef2>(n);
   uint64_t chunks = std::max<uint64_t>(1, std::min<uint64_t>(m, (uint64_t(1) << 21) / n));
   while (m % chunks) chunks >>= 1;
-  DevBuf<ef2> wpart(pool, chunks * n), wpart_g1(pool, W);
-  hip_check(launch_wrows(cm.f, (uint32_t)m, n, eqz1.ptr, wpart.ptr, (uint32_t)chunks, a_vec.ptr, st, a_copy.ptr,
-                         a_copy2.ptr),
+  ef2* wpart = P.get<ef2>(chunks * n);
+  ef2* wpart_g1 = P.get<ef2>(W);
+  hip_check(launch_wrows(cm.f, (uint32_t)m, n, eqz1, wpart, (uint32_t)chunks, a_vec, st, a_copy,
+                         a_copy2),
             "a-bar");
-  hip_check(launch_dot(a_vec.ptr, eqz2.ptr, n, part.ptr, st), "y");
+  hip_check(launch_dot(a_vec, eqz2, n, part, st), "y");
 
-  // ---- B_I (ligesis.rs:402-410) ----
-  // The two sumchecks over B_I read these 0/1 words in their first round, which writes the folded
-  // extension-field tables (half the length) that their later rounds work on.
-  DevBuf<uint64_t> bi(pool, nbi);
-  DevBuf<ef2> bi_ef(pool, nbi / 2), bi_m1(pool, nbi / 2), bi_ef2(pool, nbi / 2);
-  hip_check(launch_bits_gather(cm.enc.ptr, two_n, didx.ptr, (uint32_t)s, W, bi.ptr, st), "B_I gather");
-
-  // ---- the Basefold commitments of a-bar and B_I (ligesis.rs:384, 411-416), second stream ----
-  // Only Stage B reads them (and the proof their roots), so they are built on the second stream,
-  // which forks here; Stage B (below) follows them there and joins before the openings are
-  // gathered. Every twiddle table either stream needs is made on st before the fork.
-  const uint64_t D = key.a_state.D;
-  const int logN = log_n + 1;
-  for (uint32_t t = 0; t < R; t++) (void)ctx.twiddle(ilog2(D >> t));
-  (void)ctx.twiddle(ilog2(D));
-  (void)ctx.twiddle(logN);
-  hipStream_t sb = ctx.side_stream();
-  hip_check(hipEventRecord(ctx.ev[5], st), "fork");
-  hip_check(hipStreamWaitEvent(sb, ctx.ev[5], 0), "fork");
-  DevBasefold av_state, bi_state;
+  // ---- the Basefold commitment of a-bar (ligesis.rs:384) on the second stream, once a-bar is
+  //      there; then Stage B ----
+  hip_check(hipEventRecord(ctx.ev[9], st), "a-bar ready");
+  hip_check(hipStreamWaitEvent(sb, ctx.ev[9], 0), "a-bar ready");
   // RS(a-bar) (ligesis.rs:387): with log_interleave 0 a-bar's Basefold codeword is this same
   // transform (both planes zero-padded to 2n over the order-2n subgroup), so st waits for it
   const bool ra_shared = li_a == 0 && (uint64_t(1) << (log_n - li_a + 1)) == two_n;
-  enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec.ptr, true, log_n, li_a, 1, sb,
+  enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec, true, log_n, li_a, 1, sb,
                           ra_shared ? ctx.ev[7] : nullptr);
-  enqueue_basefold_commit(ctx, bi_state, bi.ptr, false, nv_bi, li_bi, 1, sb);
+  mark(8, sb);
   const DevBasefold* states[3] = {&key.a_state, &bi_state, &av_state};
-  const uint64_t* polys[3] = {key.a.ptr, bi.ptr, (const uint64_t*)a_vec.ptr};
+  const uint64_t* polys[3] = {key.a.ptr, bi, (const uint64_t*)a_vec};
   for (int c = 0; c < 3; c++)
     if (states[c]->D != D) throw Error(1, "prove: Basefold domains differ");
-  DevBuf<uint64_t> ra_planar;
+
+
+  uint64_t* ra_planar = nullptr;
   const uint64_t* ra_src;
   if (ra_shared) {
     ra_src = av_state.codes.ptr;
   } else {
-    ra_planar = DevBuf<uint64_t>(pool, 2 * two_n);
-    hip_check(launch_ntt((const uint64_t*)a_vec.ptr, 1, 2, (uint32_t)n, 2, logN, ctx.twiddle(logN), ra_planar.ptr,
+    ra_planar = P.get<uint64_t>(2 * two_n);
+    hip_check(launch_ntt((const uint64_t*)a_vec, 1, 2, (uint32_t)n, 2, logN, ctx.twiddle(logN), ra_planar,
                          st),
               "RS(a-bar)");
-    ra_src = ra_planar.ptr;
+    ra_src = ra_planar;
   }
-  DevBuf<ef2> ra(pool, two_n);
+  ef2* ra = P.get<ef2>(two_n);
 
   // ---- input tables of the seven Stage A sumchecks ----
   // hash / recon channels (ligesis.rs:432-485)
-  DevBuf<ef2> g1(pool, W), g2(pool, W), g2b(pool, W), vv(pool, W);
-  hip_check(launch_wrows(key.a.ptr, 32, W, eqa2.ptr, wpart_g1.ptr, 1, g1.ptr, st), "g1");
-  hip_check(launch_g2(bi.ptr, (uint32_t)s, W, eqr2.ptr, g2.ptr, st, g2b.ptr), "g2");
-  DevBuf<ef2> sel(pool, two_n);
-  hip_check(launch_sel(didx.ptr, eqr2.ptr, (uint32_t)s, sel.ptr, two_n, st), "sel");
-  hip_check(launch_vvec(eqz1.ptr, W, vv.ptr, st), "v");
+  ef2* g1 = P.get<ef2>(W);
+  ef2* g2 = P.get<ef2>(W);
+  ef2* g2b = P.get<ef2>(W);
+  ef2* vv = P.get<ef2>(W);
+  hip_check(launch_wrows(key.a.ptr, 32, W, eqa2, wpart_g1, 1, g1, st), "g1");
+  hip_check(hipStreamWaitEvent(st, ctx.ev[8], 0), "B_I ready");
+  hip_check(launch_g2(bi, (uint32_t)s, W, eqr2, g2, st, g2b), "g2");
+  ef2* sel = P.get<ef2>(two_n);
+  hip_check(launch_sel(didx, eqr2, (uint32_t)s, sel, two_n, st), "sel");
+  hip_check(launch_vvec(eqz1, W, vv, st), "v");
   // recon table and RS link (ligesis.rs:488-502); y5 = ra(r3) is sc_table's final evaluation of
   // ra, which is what the reference's verify compares (ligesis.rs:681)
-  DevBuf<ef2> wv(pool, n);
-  hip_check(launch_rs_weights(B + o_r3, logN, n, ctx.twiddle(logN), wv.ptr, st), "rs weights");
+  ef2* wv = P.get<ef2>(n);
+  hip_check(launch_rs_weights(B + o_r3, logN, n, ctx.twiddle(logN), wv, st), "rs weights");
   // last, since it may wait for the second stream's encoding of a-bar
   if (ra_shared) hip_check(hipStreamWaitEvent(st, ctx.ev[7], 0), "a-bar codeword");
-  hip_check(launch_planar_to_ef(ra_src, ra_src + two_n, two_n, ra.ptr, st), "RS(a-bar) layout");
-  hip_check(launch_dot(sel.ptr, ra.ptr, two_n, part.ptr + 256, st), "y4");
+  hip_check(launch_planar_to_ef(ra_src, ra_src + two_n, two_n, ra, st), "RS(a-bar) layout");
+  hip_check(launch_dot(sel, ra, two_n, part + 256, st), "y4");
 
   // ---- the seven sumchecks, in lockstep ----
   // Every challenge is fixed before the first round (the oracle streams are inputs), so the
-  // sumchecks only depend on their own tables: one grouped launch per round step covers every
-  // sumcheck with rounds left, aligned so that all reach their last 7 rounds (<= 128 entries)
-  // together; one more launch runs those tails, one wave per sumcheck. Each round folds its
+  // sumchecks only depend on their own tables: grouped launches cover every sumcheck with
+  // rounds left, aligned so that all reach their last rounds (<= 2^kScTailLog entries)
+  // together; one more launch runs those tails, one workgroup per sumcheck. Each round folds its
   // tables in place, so every sumcheck owns its tables (big2, a_copy2).
   struct ScRun {
     ef2* p[3];
     size_t ch_off;
   };
-  const ScRun runs[7] = {{{bi_ef.ptr, bi_m1.ptr, big.ptr}, o_chbin}, {{g1.ptr, g2.ptr, nullptr}, o_chhash},
-                         {{vv.ptr, g2b.ptr, nullptr}, o_chrecon},     {{sel.ptr, ra.ptr, nullptr}, o_chtable},
-                         {{a_copy.ptr, wv.ptr, nullptr}, o_chrs},     {{bi_ef2.ptr, big2.ptr, nullptr}, o_chbib},
-                         {{a_copy2.ptr, g_a.ptr, nullptr}, o_chab}};
+  const ScRun runs[7] = {{{bi_ef, nullptr, big}, o_chbin},   {{g1, g2, nullptr}, o_chhash},
+                         {{vv, g2b, nullptr}, o_chrecon},     {{sel, ra, nullptr}, o_chtable},
+                         {{a_copy, wv, nullptr}, o_chrs},     {{bi_ef2, big2, nullptr}, o_chbib},
+                         {{a_copy2, g_a, nullptr}, o_chab}};
+  // One grouped launch per round step; the first round of B_I's two sumchecks reads the 0/1
+  // words alone (sc_bits*_body).
   uint32_t n_big[7], steps = 0;
   for (int i = 0; i < 7; i++) {
-    n_big[i] = sc[i].nv > 7 ? sc[i].nv - 7 : 0;  // rounds with more than 128 entries
+    n_big[i] = sc[i].nv > kScTailLog ? sc[i].nv - kScTailLog : 0;  // rounds before the tail
     steps = std::max(steps, n_big[i]);
   }
-  if (n_big[0] == 0 || n_big[5] == 0) throw Error(2, "prove: B_I has fewer than 2^8 entries");
-  // Round partial sums alternate between two halves of scpart: the launch of step tau writes
-  // its half while its extra blocks reduce the other half, written by step tau - 1 (the dot
-  // products' partials for step 0); the tail launch reduces the last step's.
+  if (n_big[0] == 0 || n_big[5] == 0) throw Error(2, "prove: B_I has fewer than 2^11 entries");
+  // Partial sums alternate between two halves of scpart: the launch of step tau writes its half
+  // while its extra blocks reduce the other half, written by step tau - 1 (the dot products'
+  // partials for step 0); the tail launch reduces the last step's.
   const size_t part_half = (size_t)7 * 1024 * 4;
-  DevBuf<ef2> scpart(pool, 2 * part_half);
+  ef2* scpart = P.get<ef2>(2 * part_half);
   std::vector<ScRoundJob> rjobs;
-  std::vector<RedJob> reds = {RedJob{part.ptr, res.ptr + o_y, dot_blocks(n), 1},
-                              RedJob{part.ptr + 256, res.ptr + o_y4, dot_blocks(two_n), 1}};
+  std::vector<RedJob> reds = {RedJob{part, res + o_y, dot_blocks(n), 1},
+                              RedJob{part + 256, res + o_y4, dot_blocks(two_n), 1}};
   std::vector<uint32_t> step_first(steps + 1), step_blocks(steps), red_first(steps + 2);
   for (uint32_t tau = 0; tau < steps; tau++) {
     step_first[tau] = (uint32_t)rjobs.size();
@@ -447,12 +540,16 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
       for (int q = 0; q < 3; q++) j.p[q] = runs[i].p[q];
       j.half = uint64_t(1) << (S.nv - 1 - t);
       j.r = B + runs[i].ch_off + t;
+      if (t == 0 && (i == 0 || i == 5)) {  // B_I's sumchecks start from the 0/1 bytes
+        j.bits = bi8;
+        j.eqt = d_eq_fact + (i == 0 ? 0 : 1);
+      }
+      j.m1 = i == 0;  // the binary check's second table is B_I - 1
       j.nblk = sc_round_blocks(j.half);
-      j.part = scpart.ptr + poff;
+      j.part = scpart + poff;
       poff += (size_t)j.nblk * (S.d + 1);
-      reds.push_back(RedJob{j.part, res.ptr + S.rounds_at + (size_t)t * (S.d + 1), j.nblk, (uint32_t)S.d + 1});
+      reds.push_back(RedJob{j.part, res + S.rounds_at + (size_t)t * (S.d + 1), j.nblk, (uint32_t)S.d + 1});
       j.d = (uint32_t)S.d;
-      if (t == 0 && (i == 0 || i == 5)) j.bits = bi.ptr;  // B_I's sumchecks start from the 0/1 words
       j.first_block = blocks;
       blocks += j.nblk;
       rjobs.push_back(j);
@@ -470,20 +567,21 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     for (int q = 0; q < 3; q++) tjobs[i].p[q] = runs[i].p[q];
     tjobs[i].len = 1u << (S.nv - t0);
     tjobs[i].chal = B + runs[i].ch_off + t0;
-    tjobs[i].rounds = res.ptr + S.rounds_at + (size_t)t0 * (S.d + 1);
-    tjobs[i].finals = res.ptr + S.finals_at;
+    tjobs[i].rounds = res + S.rounds_at + (size_t)t0 * (S.d + 1);
+    tjobs[i].finals = res + S.finals_at;
     tjobs[i].d = (uint32_t)S.d;
+    tjobs[i].m1 = i == 0;
   }
   const size_t rj_bytes = rjobs.size() * sizeof(ScRoundJob), tj_bytes = tjobs.size() * sizeof(ScTailJob);
-  DevBuf<uint8_t> scjobs(pool, rj_bytes + tj_bytes + reds.size() * sizeof(RedJob));
-  if (!rjobs.empty()) ctx.upload(scjobs.ptr, rjobs.data(), rj_bytes, st);
-  ScTailJob* d_tjobs = (ScTailJob*)(scjobs.ptr + rj_bytes);
-  ctx.upload(d_tjobs, tjobs.data(), tj_bytes, st);
-  const RedJob* d_reds = (const RedJob*)(scjobs.ptr + rj_bytes + tj_bytes);
-  ctx.upload((void*)d_reds, reds.data(), reds.size() * sizeof(RedJob), st);
+  uint8_t* scjobs = P.get<uint8_t>(rj_bytes + tj_bytes + reds.size() * sizeof(RedJob));
+  if (!rjobs.empty()) P.arena.upload(scjobs, rjobs.data(), rj_bytes, st);
+  ScTailJob* d_tjobs = (ScTailJob*)(scjobs + rj_bytes);
+  P.arena.upload(d_tjobs, tjobs.data(), tj_bytes, st);
+  const RedJob* d_reds = (const RedJob*)(scjobs + rj_bytes + tj_bytes);
+  P.arena.upload((void*)d_reds, reds.data(), reds.size() * sizeof(RedJob), st);
   mark(1, st);
   for (uint32_t tau = 0; tau < steps; tau++)
-    hip_check(launch_sc_round_group((const ScRoundJob*)scjobs.ptr + step_first[tau],
+    hip_check(launch_sc_round_group((const ScRoundJob*)scjobs + step_first[tau],
                                     (int)(step_first[tau + 1] - step_first[tau]), step_blocks[tau],
                                     d_reds + red_first[tau], red_first[tau + 1] - red_first[tau], st),
               "sumcheck round step");
@@ -491,19 +589,16 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
             "sumcheck tails");
   mark(2, st);
 
+
+
   // ---- Stage B: shared-domain batched Basefold open (basefold.rs:679-856), second stream ----
-  // Issued after Stage A so that the host queues the longer stage first; its scratch (partial
-  // sums) is its own.
-  // levels 0..R stored back to back: level k has D >> k entries
-  std::vector<uint64_t> lvl_off(R + 1);
-  uint64_t tot = 0;
-  for (uint32_t k = 0; k <= R; k++) {
-    lvl_off[k] = tot;
-    tot += D >> k;
-  }
+  // Queued right behind the commitments it opens, before Stage A's work on st, so that the
+  // second stream has its whole share as early as possible; its scratch (partial sums) is its
+  // own.
   // l0 partial sums, alternating halves by round parity (each round reduces the previous one's)
-  DevBuf<ef2> part_b(pool, 2 * 3 * 1024);
-  DevBuf<ef2> levels(pool, tot);
+  ef2* part_b = P.get<ef2>(2 * 3 * 1024);
+  ef2* levels = P.get<ef2>(tot);
+  P.levels = levels;
   BfChannels chans{};
   chans.count = 3;
   for (int c = 0; c < 3; c++) {
@@ -512,49 +607,118 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     chans.chunks[c] = 1u << bli[c];
     chans.ext[c] = states[c]->ext ? 1 : 0;
   }
-  hip_check(launch_bf_combine(chans, D, levels.ptr, sb), "batch combine");
+  hip_check(launch_bf_combine(chans, D, levels, sb), "batch combine");
+  mark(9, sb);
   for (int c = 0; c < 3; c++)
-    hip_check(launch_fold_low(polys[c], states[c]->ext ? 1 : 0, (int)bli[c], M, B + o_eq4[c], mbuf[c][0].ptr, sb),
+    hip_check(launch_fold_low(polys[c], states[c]->ext ? 1 : 0, (int)bli[c], M, B + o_eq4[c], mbuf[c][0], sb),
               "batch fold");
+  // rounds whose level has more than 2^kBfTailLog entries on the grid, the rest in one workgroup
+  const uint32_t logD = (uint32_t)ilog2(D);
+  const uint32_t t0 = logD > (uint32_t)kBfTailLog ? std::min<uint32_t>(R, logD - kBfTailLog) : 0;
   RedJobs last{};
-  for (uint32_t t = 0; t < R; t++) {
+  for (uint32_t t = 0; t < t0; t++) {
     const uint64_t half = uint64_t(1) << (R - 1 - t);
     BfRound br{};
     for (int c = 0; c < 3; c++) {
-      br.m_in[c] = mbuf[c][t & 1].ptr;
-      br.m_out[c] = mbuf[c][(t + 1) & 1].ptr;
-      br.eq[c] = eqr[c][t & 1].ptr;
-      br.eq_next[c] = eqr[c][(t + 1) & 1].ptr;
+      br.m_in[c] = mbuf[c][t & 1];
+      br.m_out[c] = mbuf[c][(t + 1) & 1];
+      br.eq[c] = eqr[c][t & 1];
+      br.eq_next[c] = eqr[c][(t + 1) & 1];
       br.prev[c] = last.j[c];  // zero (no reduction) in round 0
     }
-    br.part = part_b.ptr + (size_t)(t & 1) * 3 * 1024;
+    br.part = part_b + (size_t)(t & 1) * 3 * 1024;
     const uint64_t len = D >> t;
-    br.cw_in = levels.ptr + lvl_off[t];
+    br.cw_in = levels + lvl_off[t];
     br.cw_half = len / 2;
     br.cw_tw = ctx.twiddle(ilog2(len));
-    br.cw_out = levels.ptr + lvl_off[t + 1];
+    br.cw_out = levels + lvl_off[t + 1];
     hip_check(launch_bf_round3(br, half, B + o_chbatch + t, sb), "batch round");
     const uint32_t nb = bf_round3_blocks(half, br.cw_half);
     for (int c = 0; c < 3; c++)
-      last.j[c] = RedJob{br.part + (size_t)c * nb, res.ptr + o_l0 + (size_t)c * R + t, nb, 1};
+      last.j[c] = RedJob{br.part + (size_t)c * nb, res + o_l0 + (size_t)c * R + t, nb, 1};
     last.n = 3;
   }
-  hip_check(launch_reduce_jobs(last, sb), "batch round sums");
+  mark(10, sb);
+  if (t0 < R) {
+    BfTail bt{};
+    for (int c = 0; c < 3; c++) {
+      bt.m_in[c] = mbuf[c][t0 & 1];
+      bt.eq[c] = eqr[c][t0 & 1];
+      bt.prev[c] = last.j[c];
+    }
+    bt.cw_in = levels + lvl_off[t0];
+    bt.lvl_out = levels + lvl_off[t0 + 1];
+    bt.l0 = res + o_l0;
+    bt.chal = B + o_chbatch;
+    bt.tw = ctx.twiddle((int)logD);
+    bt.R = R;
+    bt.t0 = t0;
+    bt.logD = logD;
+    hip_check(launch_bf_tail(bt, sb), "batch tail rounds");
+  } else {
+    hip_check(launch_reduce_jobs(last, sb), "batch round sums");
+  }
   // trees over the folded levels 1..R-1 (leaf = one F_p^2 value), built together
-  std::vector<DevTree> ftrees(R > 1 ? R - 1 : 0);
+  std::vector<DevTree>& ftrees = P.ftrees;
+  ftrees.resize(R > 1 ? R - 1 : 0);
   std::vector<MerkleSource> fsrc;
   for (uint32_t k = 1; k < R; k++) {
     DevTree& tr = ftrees[k - 1];
     tr.leaves = D >> k;
     tr.nodes = DevBuf<uint32_t>(pool, tr.node_count() * 8);
-    fsrc.push_back(MerkleSource{(const uint64_t*)(levels.ptr + lvl_off[k]), 2, 1, 2, tr.leaves, tr.nodes.ptr});
+    fsrc.push_back(MerkleSource{(const uint64_t*)(levels + lvl_off[k]), 2, 1, 2, tr.leaves, tr.nodes.ptr});
   }
-  std::vector<std::vector<MerkleJob>> fjobs;
-  std::vector<DevBuf<uint64_t>> ftables;
-  enqueue_merkle_trees(ctx, fsrc, fjobs, ftables, sb);
+  std::vector<std::vector<MerkleJob>>& fjobs = P.fjobs;
+  std::vector<DevBuf<uint64_t>>& ftables = P.ftables;
+  enqueue_merkle_trees(ctx, fsrc, fjobs, ftables, sb, &P.arena);
   mark(3, sb);
   hip_check(hipEventRecord(ctx.ev[6], sb), "join");
+  hip_check(hipStreamWaitEvent(st, ctx.ev[6], 0), "join");
+  };
 
+  // With the option prove_graph (proofs on the commitment's own context), the plan stays with
+  // the commitment, the second proof captures its launches into a graph, and later proofs
+  // refill the blob and idx slots that the graph's upload nodes read and replay it. Otherwise
+  // the steps are issued one by one into a plan that lives for this call.
+  const bool own_ctx = cm.ctx == &ctx, graphs = own_ctx && ctx.prove_graph && !ctx.profile;
+  std::shared_ptr<ProvePlan> plan = own_ctx ? cm.plan : nullptr;
+  if (graphs && plan && plan->exec) {
+    std::memcpy(plan->blob_host, blob.data(), blob.size() * sizeof(ef2));
+    std::memcpy(plan->idx_host, idx.data(), s * 4);
+    hip_check(hipGraphLaunch(plan->exec, st), "prove graph launch");
+  } else {
+    auto np = std::make_shared<ProvePlan>();
+    np->pool = pool;
+    // staging memory: the last kept plan's, else the context's spare (pinning new host memory
+    // on every proof would cost more than the proof's uploads)
+    np->arena.take(plan ? plan->arena : ctx.prove_arena);
+    if (plan) {  // the last plan's buffers go back to the pool (its proof has completed)
+      if (own_ctx) cm.plan.reset();
+      plan.reset();
+    }
+    const bool capture = graphs && cm.prove_calls >= 1;
+    if (capture) hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "prove capture");
+    try {
+      issue(*np);
+    } catch (...) {
+      if (capture) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+      }
+      throw;
+    }
+    if (capture) {
+      hip_check(hipStreamEndCapture(st, &np->graph), "prove capture end");
+      hip_check(hipGraphInstantiate(&np->exec, np->graph, nullptr, nullptr, 0), "prove graph instantiate");
+      hip_check(hipGraphLaunch(np->exec, st), "prove graph launch");
+    }
+    plan = np;
+    if (graphs) cm.plan = np;
+  }
+  if (own_ctx) cm.prove_calls++;
+  ProvePlan& P = *plan;
+  const DevBasefold* states[3] = {&key.a_state, &P.bi_state, &P.av_state};
 
   // Opening positions are fixed by the integer stream, so every index the proof opens is
   // computed here on the host while the device works through both stages.
@@ -593,17 +757,16 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
       for (uint32_t rr = 0; rr < rows; rr++) bvw[c].push_back((uint64_t)rr * D + p);
   }
   // ---- openings ----
-  hip_check(hipStreamWaitEvent(st, ctx.ev[6], 0), "join");
   Gather g;
   const size_t g_hopen = g.add_nodes(cm.tree, hnodes);
   const size_t g_hcols = g.add(cm.h.ptr, hw);
-  const size_t g_com_av = g.add_root(av_state.tree), g_com_i = g.add_root(bi_state.tree);
+  const size_t g_com_av = g.add_root(P.av_state.tree), g_com_i = g.add_root(P.bi_state.tree);
   std::vector<size_t> g_froot(nf), g_fproof(nf), g_fval(nf);
-  for (uint32_t k = 1; k < R; k++) g_froot[k - 1] = g.add_root(ftrees[k - 1]);
-  const size_t g_final = g.add((const uint64_t*)levels.ptr, fw);
+  for (uint32_t k = 1; k < R; k++) g_froot[k - 1] = g.add_root(P.ftrees[k - 1]);
+  const size_t g_final = g.add((const uint64_t*)P.levels, fw);
   for (uint32_t k = 1; k < R; k++) {
-    g_fproof[k - 1] = g.add_nodes(ftrees[k - 1], fnodes[k - 1]);
-    g_fval[k - 1] = g.add((const uint64_t*)levels.ptr, fvw[k - 1]);
+    g_fproof[k - 1] = g.add_nodes(P.ftrees[k - 1], fnodes[k - 1]);
+    g_fval[k - 1] = g.add((const uint64_t*)P.levels, fvw[k - 1]);
   }
   size_t g_bproof[3], g_bval[3];
   for (int c = 0; c < 3; c++) {
@@ -624,7 +787,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   gtab.insert(gtab.end(), g.offs.begin(), g.offs.end());
   DevBuf<uint64_t> dgtab(pool, gtab.size());
   ctx.upload(dgtab.ptr, gtab.data(), gtab.size() * 8, st);
-  hip_check(hipMemcpyAsync(dstage.ptr, res.ptr, res_n * sizeof(ef2), hipMemcpyDeviceToDevice, st), "results");
+  hip_check(hipMemcpyAsync(dstage.ptr, P.res, res_n * sizeof(ef2), hipMemcpyDeviceToDevice, st), "results");
   hip_check(launch_gather_parts((const GatherPart*)dgtab.ptr, (uint32_t)g.parts.size(),
                                 dgtab.ptr + 2 * g.parts.size(), g.offs.size(), dstage.ptr + 2 * res_n, st),
             "gather");
@@ -634,11 +797,12 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   mark(5, st);
   const auto t_enqueued = std::chrono::steady_clock::now();
   hip_check(hipStreamSynchronize(st), "prove sync");
+  if (!cm.plan || cm.plan.get() != &P) ctx.prove_arena.take(P.arena);  // a plan of this call only
   if (ctx.profile) {  // host time spent issuing work, and the whole device-bound part
     ctx.stage_ms[5] = std::chrono::duration<double, std::milli>(t_enqueued - t_start).count();
     ctx.stage_ms[6] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     float ms = 0;
-    for (int i = 1; i <= 5; i++) {
+    for (int i = 1; i <= 10; i++) {
       hip_check(hipEventElapsedTime(&ms, ctx.timing_event(32), ctx.timing_event(32 + i)), "profile time");
       ctx.stage_ms[6 + i] = ms;
     }

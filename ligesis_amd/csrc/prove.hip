@@ -10,35 +10,58 @@ namespace lgs {
 __device__ __forceinline__ ef2 ld_ef(const ef2* p, uint64_t i) { return p[i]; }
 __device__ __forceinline__ ef2 ef_one_minus(ef2 a) { return ef2{gl_sub(1, a.c0), gl_neg(a.c1)}; }
 
-// Block-wide sum of W extension values per thread; result valid in thread 0.
-template <int W>
-__device__ __forceinline__ void block_sum(ef2 (&v)[W], ef2* sh /* blockDim x W */) {
-  const int t = threadIdx.x;
+// Sum over the 64 lanes of a wave (butterfly: every lane ends with the total).
+__device__ __forceinline__ ef2 wave_sum(ef2 v) {
 #pragma unroll
-  for (int q = 0; q < W; q++) sh[t * W + q] = v[q];
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (t < s) {
-#pragma unroll
-      for (int q = 0; q < W; q++) sh[t * W + q] = ef_add(sh[t * W + q], sh[(t + s) * W + q]);
-    }
-    __syncthreads();
+  for (int off = 32; off > 0; off >>= 1) {
+    const ef2 o{(uint64_t)__shfl_xor((unsigned long long)v.c0, off), (uint64_t)__shfl_xor((unsigned long long)v.c1, off)};
+    v = ef_add(v, o);
   }
-  if (t == 0) {
+  return v;
+}
+
+// Block-wide sum of W extension values per thread (blockDim a multiple of 64, sh at least
+// blockDim / 64 x W values); result valid in thread 0. Contains barriers before and after the
+// exchange through sh, so every thread's earlier LDS reads are done when it returns.
+template <int W>
+__device__ __forceinline__ void block_sum(ef2 (&v)[W], ef2* sh) {
 #pragma unroll
-    for (int q = 0; q < W; q++) v[q] = sh[q];
+  for (int q = 0; q < W; q++) v[q] = wave_sum(v[q]);
+  const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < W; q++) sh[wave * W + q] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < W; q++) {
+      ef2 t = sh[q];
+      for (uint32_t w = 1; w < nw; w++) t = ef_add(t, sh[w * W + q]);
+      v[q] = t;
+    }
   }
 }
 
-// One deferred reduction (RedJob) by one block: strided partial sums, then a tree in LDS (sh
-// holds blockDim x 4 values).
+// One deferred reduction (RedJob, w <= 16 values) by one block: strided partial sums, then a
+// block sum per group of four values.
 __device__ __forceinline__ void red_job_body(const RedJob& r, ef2* sh) {
-  ef2 acc[4] = {ef_zero(), ef_zero(), ef_zero(), ef_zero()};
-  for (uint32_t b = threadIdx.x; b < r.nblk; b += blockDim.x)
-    for (uint32_t q = 0; q < r.w; q++) acc[q] = ef_add(acc[q], r.part[(uint64_t)b * r.w + q]);
-  block_sum<4>(acc, sh);
-  if (threadIdx.x == 0)
-    for (uint32_t q = 0; q < r.w; q++) r.out[q] = acc[q];
+  for (uint32_t q0 = 0; q0 < r.w; q0 += 4) {
+    const uint32_t wq = r.w - q0 < 4 ? r.w - q0 : 4;
+    ef2 acc[4] = {ef_zero(), ef_zero(), ef_zero(), ef_zero()};
+    for (uint32_t b = threadIdx.x; b < r.nblk; b += blockDim.x) {
+#pragma unroll
+      for (uint32_t q = 0; q < 4; q++)
+        if (q < wq) acc[q] = ef_add(acc[q], r.part[(uint64_t)b * r.w + q0 + q]);
+    }
+    block_sum<4>(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (uint32_t q = 0; q < 4; q++)
+        if (q < wq) r.out[q0 + q] = acc[q];
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void reduce_jobs_kernel(RedJobs r) {
@@ -93,6 +116,14 @@ __global__ void eq_expand_group_kernel(const EqTable* __restrict__ tabs, int nta
   t.out[b] = v;
 }
 
+// Entry x of an eq table from its half tables (what eq_expand_group_kernel would store).
+__device__ __forceinline__ ef2 eq_table_at(const EqTable& t, uint64_t x) {
+  const uint64_t bl = x & ((uint64_t(1) << t.klo) - 1), bh = x >> t.klo;
+  ef2 v = ef_mul(t.lo[0][bl], t.hi[0][bh]);
+  for (uint32_t q = 1; q < t.nterms; q++) v = ef_add(v, ef_mul(t.lo[q][bl], t.hi[q][bh]));
+  return v;
+}
+
 hipError_t launch_eq_group(const EqTerm* terms, int nterms, uint32_t term_blocks, const EqTable* tabs, int ntabs,
                            uint32_t tab_blocks, hipStream_t st) {
   if (term_blocks) hipLaunchKernelGGL(eq_parts_group_kernel, dim3(term_blocks), dim3(256), 0, st, terms, nterms);
@@ -106,10 +137,27 @@ __global__ void wrows_kernel(const uint64_t* __restrict__ M, uint64_t cols, cons
                              uint32_t rows_per_chunk, ef2* __restrict__ part) {
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= cols) return;
-  const uint32_t i0 = blockIdx.y * rows_per_chunk;
-  ef2 acc = ef_zero();
-  for (uint32_t i = i0; i < i0 + rows_per_chunk; i++) acc = ef_add(acc, ef_mul_base(w[i], M[i * cols + j]));
-  part[(uint64_t)blockIdx.y * cols + j] = acc;
+  const uint32_t i0 = blockIdx.y * rows_per_chunk, i1 = i0 + rows_per_chunk;
+  // unreduced sums of the two components, four rows' loads in flight at a time
+  WideAcc a0 = wide_zero(), a1 = wide_zero();
+  uint32_t i = i0;
+  for (; i + 4 <= i1; i += 4) {
+    uint64_t x[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) x[u] = M[(uint64_t)(i + u) * cols + j];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const ef2 wi = w[i + u];
+      wide_mul_add(a0, wi.c0, x[u]);
+      wide_mul_add(a1, wi.c1, x[u]);
+    }
+  }
+  for (; i < i1; i++) {
+    const uint64_t x = M[(uint64_t)i * cols + j];
+    wide_mul_add(a0, w[i].c0, x);
+    wide_mul_add(a1, w[i].c1, x);
+  }
+  part[(uint64_t)blockIdx.y * cols + j] = ef2{wide_reduce(a0), wide_reduce(a1)};
 }
 
 // out (and the optional copies out2, out3, which later kernels fold in place) = sum of chunks
@@ -143,91 +191,107 @@ __global__ __launch_bounds__(256) void dot_kernel(const ef2* __restrict__ a, con
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
 }
 
-// out[q] = sum_{b < nb} part[b * width + q], for q < width <= 4 (one block of 256 threads:
-// strided partial sums, then a tree in LDS)
-
-// The last rounds of a product sumcheck in one workgroup: the factors (at most 2 * 512 entries
-// each) are folded in LDS round after round; round polynomials go to `rounds` (D + 1 each) and
-// the final evaluations to `finals`.
-template <int D, int CAP>
-__device__ __forceinline__ void sc_tail_body(const ef2* __restrict__ p0, const ef2* __restrict__ p1,
-                                             const ef2* __restrict__ p2, uint32_t len,
-                                             const ef2* __restrict__ chal, ef2* __restrict__ rounds,
-                                             ef2* __restrict__ finals, ef2 (*buf)[CAP], ef2* sh) {
-  const ef2* src[3] = {p0, p1, p2};
+// The last rounds (at most 2^kScTailLog entries) of one product sumcheck in one workgroup of 1024
+// threads, the tables in LDS. Four threads share a pair (lo, hi): thread q evaluates the
+// product at t = q (q <= D) and folds table q (q < D); the folded values are written back in
+// place after a barrier. The round sums do not feed later rounds (every challenge is an input),
+// so each wave keeps per-round partials (a sum over its lanes of one q) and the workgroup adds
+// them up once, after the last round.
+constexpr int kTailThreads = 1024;
+template <int D>
+__device__ __forceinline__ void sc_tail_body(const ScTailJob& jb, ef2 (*tab)[1 << kScTailLog],
+                                             ef2 (*wpart)[kTailThreads / 64][4]) {
+  const uint32_t len = jb.len, tid = threadIdx.x, q = tid & 3, wave = tid >> 6, lane = tid & 63;
+  const bool m1 = jb.m1 != 0;  // table 1 = table 0 - 1, not stored
 #pragma unroll
-  for (int j = 0; j < D; j++)
-    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) buf[j][i] = src[j][i];
+  for (int j = 0; j < D; j++) {
+    if (m1 && j == 1) continue;
+    for (uint32_t i = tid; i < len; i += blockDim.x) tab[j][i] = jb.p[j][i];
+  }
   __syncthreads();
-  uint32_t t = 0;
-  for (uint32_t cur = len; cur > 1; cur >>= 1, t++) {
+  uint32_t k = 0;
+  for (uint32_t cur = len; cur > 1; cur >>= 1, k++) {
     const uint32_t half = cur / 2;
-    const ef2 r = chal[t];
-    ef2 acc[D + 1];
+    const ef2 r = jb.chal[k];
+    ef2 acc = ef_zero();
+    ef2 fv[(4 << kScTailLog) / 2 / kTailThreads];  // folded values held across the barrier
 #pragma unroll
-    for (int q = 0; q <= D; q++) acc[q] = ef_zero();
-    ef2 folded[D][2];
-    for (uint32_t k = 0; k < 2; k++) {
-      const uint32_t i = threadIdx.x + k * blockDim.x;
-      if (i >= half) continue;
-      ef2 lo[D], diff[D], x[D];
+    for (uint32_t it = 0; it < (4u << kScTailLog) / 2 / kTailThreads; it++) {
+      const uint32_t e = tid + it * kTailThreads, i = e >> 2;
+      if (i >= half) break;
+      ef2 y[D];
 #pragma unroll
       for (int j = 0; j < D; j++) {
-        lo[j] = buf[j][i];
-        diff[j] = ef_sub(buf[j][i + half], lo[j]);
-        folded[j][k] = ef_add(lo[j], ef_mul(r, diff[j]));
-        x[j] = lo[j];
+        if (m1 && j == 1) {
+          y[1] = ef_sub(y[0], ef_one());
+          continue;
+        }
+        const ef2 lo = tab[j][i], hi = tab[j][i + half];
+        const ef2 diff = ef_sub(hi, lo);
+        // y_j = lo + q diff
+        ef2 v = q == 0 ? lo : hi;
+        if (q >= 2) v = ef_add(v, diff);
+        if (q == 3) v = ef_add(v, diff);
+        y[j] = v;
+        if ((int)q == j) fv[it] = ef_add(lo, ef_mul(r, diff));
       }
+      if ((int)q <= D) {
+        ef2 prod = y[0];
 #pragma unroll
-      for (int q = 0; q <= D; q++) {
-        ef2 prod = x[0];
-#pragma unroll
-        for (int j = 1; j < D; j++) prod = ef_mul(prod, x[j]);
-        acc[q] = ef_add(acc[q], prod);
-#pragma unroll
-        for (int j = 0; j < D; j++) x[j] = ef_add(x[j], diff[j]);
-      }
-    }
-    block_sum<D + 1>(acc, sh);  // ends with a barrier: every read of buf is done
-    for (uint32_t k = 0; k < 2; k++) {
-      const uint32_t i = threadIdx.x + k * blockDim.x;
-      if (i < half) {
-#pragma unroll
-        for (int j = 0; j < D; j++) buf[j][i] = folded[j][k];
+        for (int j = 1; j < D; j++) prod = ef_mul(prod, y[j]);
+        acc = ef_add(acc, prod);
       }
     }
-    if (threadIdx.x == 0) {
+    __syncthreads();  // every read of this round's tables is done
 #pragma unroll
-      for (int q = 0; q <= D; q++) rounds[(uint64_t)t * (D + 1) + q] = acc[q];
+    for (uint32_t it = 0; it < (4u << kScTailLog) / 2 / kTailThreads; it++) {
+      const uint32_t e = tid + it * kTailThreads, i = e >> 2;
+      if (i >= half) break;
+      if ((int)q < D && !(m1 && q == 1)) tab[q][i] = fv[it];
     }
-    __syncthreads();
+    // lanes with equal q (stride 4) summed: offsets 32, 16, 8, 4
+#pragma unroll
+    for (int off = 32; off >= 4; off >>= 1) {
+      const ef2 o{(uint64_t)__shfl_xor((unsigned long long)acc.c0, off),
+                  (uint64_t)__shfl_xor((unsigned long long)acc.c1, off)};
+      acc = ef_add(acc, o);
+    }
+    if (lane < 4) wpart[k][wave][lane] = acc;
+    __syncthreads();  // folded tables and partials visible
   }
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int j = 0; j < D; j++) finals[j] = buf[j][0];
+  // round k, value q: sum over the waves
+  const uint32_t nw = blockDim.x >> 6;
+  if (tid < k * (D + 1)) {
+    const uint32_t kk = tid / (D + 1), qq = tid % (D + 1);
+    ef2 t = ef_zero();
+    for (uint32_t w = 0; w < nw; w++) t = ef_add(t, wpart[kk][w][qq]);
+    jb.rounds[tid] = t;
   }
+  if (tid < (uint32_t)D) jb.finals[tid] = (m1 && tid == 1) ? ef_sub(tab[0][0], ef_one()) : tab[tid][0];
 }
 
-// The tails of several sumchecks, one wave each (blockIdx.x = job).
-__global__ __launch_bounds__(64) void sc_tail_group_kernel(const ScTailJob* __restrict__ jobs, uint32_t njobs,
-                                                           const RedJob* __restrict__ reds) {
-  __shared__ ef2 buf[3][128];
-  __shared__ ef2 sh[64 * 4];
+// The tails of several sumchecks, one workgroup each (blockIdx.x = job); blocks past njobs run
+// the deferred reductions `reds`.
+__global__ __launch_bounds__(kTailThreads) void sc_tail_group_kernel(const ScTailJob* __restrict__ jobs,
+                                                                     uint32_t njobs,
+                                                                     const RedJob* __restrict__ reds) {
+  __shared__ ef2 tab[3][1 << kScTailLog];
+  __shared__ ef2 wpart[kScTailLog][kTailThreads / 64][4];
   if (blockIdx.x >= njobs) {
-    red_job_body(reds[blockIdx.x - njobs], sh);
+    red_job_body(reds[blockIdx.x - njobs], &wpart[0][0][0]);
     return;
   }
   const ScTailJob& j = jobs[blockIdx.x];
   if (j.d == 3)
-    sc_tail_body<3, 128>(j.p[0], j.p[1], j.p[2], j.len, j.chal, j.rounds, j.finals, buf, sh);
+    sc_tail_body<3>(j, tab, wpart);
   else
-    sc_tail_body<2, 128>(j.p[0], j.p[1], nullptr, j.len, j.chal, j.rounds, j.finals, buf, sh);
+    sc_tail_body<2>(j, tab, wpart);
 }
 
 hipError_t launch_sc_tail_group(const ScTailJob* jobs, uint32_t njobs, const RedJob* reds, uint32_t nreds,
                                 hipStream_t st) {
   if (njobs + nreds)
-    hipLaunchKernelGGL(sc_tail_group_kernel, dim3(njobs + nreds), dim3(64), 0, st, jobs, njobs, reds);
+    hipLaunchKernelGGL(sc_tail_group_kernel, dim3(njobs + nreds), dim3(kTailThreads), 0, st, jobs, njobs, reds);
   return hipGetLastError();
 }
 
@@ -246,24 +310,18 @@ hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part, hipStre
 // B_I (ligesis.rs:403-410): bi[i * s + k] = bit (i mod 64) of F'[i / 64][idx[k]], as 0/1
 // extension elements; bi_m1 = bi - 1 (ligesis.rs:424-425).
 __global__ void bits_gather_kernel(const uint64_t* __restrict__ enc, uint64_t ld, const uint32_t* __restrict__ idx,
-                                   uint32_t s, uint64_t W, uint64_t* __restrict__ bi) {
+                                   uint32_t s, uint64_t W, uint64_t* __restrict__ bi, uint8_t* __restrict__ bi8) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= W * s) return;
   const uint64_t i = t / s;
   const uint32_t k = (uint32_t)(t % s);
-  bi[t] = (enc[(i >> 6) * ld + idx[k]] >> (i & 63)) & 1;
+  const uint64_t bit = (enc[(i >> 6) * ld + idx[k]] >> (i & 63)) & 1;
+  bi[t] = bit;
+  bi8[t] = (uint8_t)bit;
 }
 
 // g2[i] = sum_k eq_r2[k] * bi[i * s + k] (ligesis.rs:445-455): one wave per i, its lanes
 // reading consecutive words of the row (coalesced), then a butterfly sum across the wave.
-__device__ __forceinline__ ef2 wave_sum(ef2 v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const ef2 o{(uint64_t)__shfl_xor((unsigned long long)v.c0, off), (uint64_t)__shfl_xor((unsigned long long)v.c1, off)};
-    v = ef_add(v, o);
-  }
-  return v;
-}
 
 __global__ __launch_bounds__(256) void g2_kernel(const uint64_t* __restrict__ bi, uint32_t s, uint64_t W,
                                                  const ef2* __restrict__ eq_r2, ef2* __restrict__ g2,
@@ -330,7 +388,18 @@ __global__ void planar_to_ef_kernel(const uint64_t* __restrict__ c0, const uint6
 // ---------------- product sumcheck round (sumcheck.rs:37-84) ----------------
 // Round over pairs (lo = p[i], hi = p[i + half]): g(t) = sum_i prod_j ((1 - t) lo_j + t hi_j) for
 // t = 0..D, then p[i] <- lo + r (hi - lo) in place (the high variable is bound first).
-template <int D>
+// R consecutive rounds of one product sumcheck in one pass (R <= 3). With the table length
+// L = 2 half and stride = L / 2^R, thread i owns the entries i + g stride (g < 2^R) of every
+// table: each round pairs g with g + 2^(R-1-k) among the entries it still holds, adds its
+// products at t = 0..D to that round's sums, and folds with the round's challenge; after R
+// rounds entry i of the folded table (length L / 2^R) is written back in place. No entry is
+// shared between threads, so the rounds need no synchronisation.
+// One round of one product sumcheck over its pairs (lo, hi) = (p[i], p[i + half]): the
+// products at t = 0..D go to the block's partial sums, and every table folds in place with the
+// round's challenge, p[i] = lo + r (hi - lo).
+// M1: the second table is the first minus one and is not stored (its pair and fold follow from
+// the first table's).
+template <int D, bool M1>
 __device__ __forceinline__ void sc_round_body(ef2* __restrict__ p0, ef2* __restrict__ p1, ef2* __restrict__ p2,
                                               uint64_t half, const ef2* __restrict__ r_ptr, ef2* __restrict__ part,
                                               uint32_t blk, uint32_t nblk, ef2* sh) {
@@ -340,26 +409,29 @@ __device__ __forceinline__ void sc_round_body(ef2* __restrict__ p0, ef2* __restr
 #pragma unroll
   for (int q = 0; q <= D; q++) acc[q] = ef_zero();
   for (uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x; i < half; i += (uint64_t)nblk * blockDim.x) {
-    ef2 lo[D], diff[D];
+    ef2 y[D], diff[D];
 #pragma unroll
     for (int j = 0; j < D; j++) {
-      lo[j] = ps[j][i];
-      const ef2 hi = ps[j][i + half];
-      diff[j] = ef_sub(hi, lo[j]);
-      ps[j][i] = ef_add(lo[j], ef_mul(r, diff[j]));
+      if (M1 && j == 1) {
+        y[1] = ef_sub(y[0], ef_one());
+        diff[1] = diff[0];
+        continue;
+      }
+      y[j] = ps[j][i];
+      diff[j] = ef_sub(ps[j][i + half], y[j]);
+      ps[j][i] = ef_add(y[j], ef_mul(r, diff[j]));
     }
-    // evaluations at t = 0..D by stepping x_j(t + 1) = x_j(t) + diff_j
-    ef2 x[D];
-#pragma unroll
-    for (int j = 0; j < D; j++) x[j] = lo[j];
+    // products at t = 0..D, stepping each factor by its difference
 #pragma unroll
     for (int q = 0; q <= D; q++) {
-      ef2 prod = x[0];
+      ef2 prod = y[0];
 #pragma unroll
-      for (int j = 1; j < D; j++) prod = ef_mul(prod, x[j]);
+      for (int j = 1; j < D; j++) prod = ef_mul(prod, y[j]);
       acc[q] = ef_add(acc[q], prod);
+      if (q < D) {
 #pragma unroll
-      for (int j = 0; j < D; j++) x[j] = ef_add(x[j], diff[j]);
+        for (int j = 0; j < D; j++) y[j] = ef_add(y[j], diff[j]);
+      }
     }
   }
   block_sum<D + 1>(acc, sh);
@@ -374,30 +446,29 @@ __device__ __forceinline__ void sc_round_body(ef2* __restrict__ p0, ef2* __restr
 // b(t) (b(t) - 1) vanishes when lo = hi, and is t (t - 1) otherwise, so the round values are
 // s(0) = s(1) = 0, s(2) = 2 sum (2 E_hi - E_lo), s(3) = 6 sum (3 E_hi - 2 E_lo) over the pairs
 // with lo != hi -- the same field values the general product rule gives, from additions only.
-// The folded bit lo + r (hi - lo) is one of 0, 1, r, 1 - r.
-__device__ __forceinline__ void sc_bits3_body(const uint64_t* __restrict__ bits, ef2* __restrict__ b_out,
-                                              ef2* __restrict__ m1_out, ef2* __restrict__ e, uint64_t half,
+// The folded bit lo + r (hi - lo) is one of 0, 1, r, 1 - r (the b - 1 table is implicit: the
+// later rounds use b - 1 from b).
+__device__ __forceinline__ void sc_bits3_body(const uint8_t* __restrict__ bits, ef2* __restrict__ b_out,
+                                              const EqTable* __restrict__ eqt, ef2* __restrict__ e, uint64_t half,
                                               const ef2* __restrict__ r_ptr, ef2* __restrict__ part, uint32_t blk,
                                               uint32_t nblk, ef2* sh) {
   const ef2 r = *r_ptr, one = ef_one();
-  const ef2 omr = ef_sub(one, r), rm1 = ef_sub(r, one), nr = ef_neg(r), m1 = ef_neg(one);
+  const ef2 omr = ef_sub(one, r);
+  const EqTable T = *eqt;
   ef2 L = ef_zero(), H = ef_zero();
   for (uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x; i < half; i += (uint64_t)nblk * blockDim.x) {
-    const uint64_t blo = bits[i], bhi = bits[i + half];
-    const ef2 elo = e[i], ehi = e[i + half];
+    const uint32_t blo = bits[i], bhi = bits[i + half];
+    const ef2 elo = eq_table_at(T, i), ehi = eq_table_at(T, i + half);
     e[i] = ef_add(elo, ef_mul(r, ef_sub(ehi, elo)));
-    ef2 bf, bm;
+    ef2 bf;
     if (blo == bhi) {
       bf = blo ? one : ef_zero();
-      bm = blo ? ef_zero() : m1;
     } else {
       bf = bhi ? r : omr;
-      bm = bhi ? rm1 : nr;
       L = ef_add(L, elo);
       H = ef_add(H, ehi);
     }
     b_out[i] = bf;
-    m1_out[i] = bm;
   }
   // 2 (2H - L) and 6 (3H - 2L), linear in the block's sums, so partials add up
   const ef2 h2 = ef_add(H, H), l2 = ef_add(L, L);
@@ -416,15 +487,17 @@ __device__ __forceinline__ void sc_bits3_body(const uint64_t* __restrict__ bits,
 // ligesis.rs:513-522): s(0) = sum_{lo = 1} G_lo, s(1) = sum_{hi = 1} G_hi and
 // s(2) = sum (2 hi - lo)(2 G_hi - G_lo) = 4 s(1) - 2 C + s(0) with
 // C = sum_{hi = 1} G_lo + sum_{lo = 1} G_hi.
-__device__ __forceinline__ void sc_bits2_body(const uint64_t* __restrict__ bits, ef2* __restrict__ b_out,
-                                              ef2* __restrict__ g, uint64_t half, const ef2* __restrict__ r_ptr,
-                                              ef2* __restrict__ part, uint32_t blk, uint32_t nblk, ef2* sh) {
+__device__ __forceinline__ void sc_bits2_body(const uint8_t* __restrict__ bits, ef2* __restrict__ b_out,
+                                              const EqTable* __restrict__ eqt, ef2* __restrict__ g, uint64_t half,
+                                              const ef2* __restrict__ r_ptr, ef2* __restrict__ part, uint32_t blk,
+                                              uint32_t nblk, ef2* sh) {
   const ef2 r = *r_ptr, one = ef_one();
   const ef2 omr = ef_sub(one, r);
+  const EqTable T = *eqt;
   ef2 A0 = ef_zero(), A1 = ef_zero(), C = ef_zero();
   for (uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x; i < half; i += (uint64_t)nblk * blockDim.x) {
-    const uint64_t blo = bits[i], bhi = bits[i + half];
-    const ef2 glo = g[i], ghi = g[i + half];
+    const uint32_t blo = bits[i], bhi = bits[i + half];
+    const ef2 glo = eq_table_at(T, i), ghi = eq_table_at(T, i + half);
     g[i] = ef_add(glo, ef_mul(r, ef_sub(ghi, glo)));
     if (blo) {
       A0 = ef_add(A0, glo);
@@ -464,13 +537,15 @@ __global__ __launch_bounds__(256) void sc_round_group_kernel(const ScRoundJob* _
   const uint32_t blk = blockIdx.x - j.first_block;
   if (j.bits) {
     if (j.d == 3)
-      sc_bits3_body(j.bits, j.p[0], j.p[1], j.p[2], j.half, j.r, j.part, blk, j.nblk, sh);
+      sc_bits3_body(j.bits, j.p[0], j.eqt, j.p[2], j.half, j.r, j.part, blk, j.nblk, sh);
     else
-      sc_bits2_body(j.bits, j.p[0], j.p[1], j.half, j.r, j.part, blk, j.nblk, sh);
-  } else if (j.d == 3)
-    sc_round_body<3>(j.p[0], j.p[1], j.p[2], j.half, j.r, j.part, blk, j.nblk, sh);
-  else
-    sc_round_body<2>(j.p[0], j.p[1], nullptr, j.half, j.r, j.part, blk, j.nblk, sh);
+      sc_bits2_body(j.bits, j.p[0], j.eqt, j.p[1], j.half, j.r, j.part, blk, j.nblk, sh);
+  } else if (j.d == 3) {
+    if (j.m1) sc_round_body<3, true>(j.p[0], nullptr, j.p[2], j.half, j.r, j.part, blk, j.nblk, sh);
+    else sc_round_body<3, false>(j.p[0], j.p[1], j.p[2], j.half, j.r, j.part, blk, j.nblk, sh);
+  } else {
+    sc_round_body<2, false>(j.p[0], j.p[1], nullptr, j.half, j.r, j.part, blk, j.nblk, sh);
+  }
 }
 
 hipError_t launch_sc_round_group(const ScRoundJob* jobs, int njobs, uint32_t total_blocks, const RedJob* reds,
@@ -488,44 +563,67 @@ uint32_t sc_round_blocks(uint64_t half) {
 
 
 
+
 // ---------------- Basefold batch (basefold.rs:648-856) ----------------
 // combined[i] = sum over channels c, chunks j of wt_c[j] * codes_c[j][i], where
 // wt_c[j] = coeffs[c] * eq4_c[j]; an extension channel keeps c0 / c1 in rows 2j / 2j+1.
-__global__ void bf_combine_kernel(BfChannels ch, uint64_t D, ef2* __restrict__ out) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= D) return;
-  ef2 acc = ef_zero();
-  for (int c = 0; c < ch.count; c++) {
-    const uint64_t* codes = ch.codes[c];
-    const ef2* wt = ch.wt[c];
-    for (uint32_t j = 0; j < ch.chunks[c]; j++) {
-      if (ch.ext[c]) {
-        const ef2 v{codes[(2 * j) * D + i], codes[(2 * j + 1) * D + i]};
-        acc = ef_add(acc, ef_mul(wt[j], v));
-      } else {
-        acc = ef_add(acc, ef_mul_base(wt[j], codes[j * D + i]));
+// A workgroup takes 64 consecutive positions i; its four waves split the rows (row j to wave
+// j mod 4), accumulate unreduced, and add their four sums through LDS.
+__global__ __launch_bounds__(256) void bf_combine_kernel(BfChannels ch, uint64_t D, ef2* __restrict__ out) {
+  __shared__ ef2 sh[3][64];
+  const uint32_t lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
+  const uint64_t i = (uint64_t)blockIdx.x * 64 + lane;
+  WideAcc a0 = wide_zero(), a1 = wide_zero();
+  if (i < D) {
+    for (int c = 0; c < ch.count; c++) {
+      const uint64_t* codes = ch.codes[c];
+      const ef2* wt = ch.wt[c];
+      for (uint32_t j = sub; j < ch.chunks[c]; j += 4) {
+        if (ch.ext[c]) {
+          const ef2 e = ef_mul(wt[j], ef2{codes[(2 * j) * D + i], codes[(2 * j + 1) * D + i]});
+          wide_add(a0, e.c0);
+          wide_add(a1, e.c1);
+        } else {
+          const uint64_t x = codes[j * D + i];
+          wide_mul_add(a0, wt[j].c0, x);
+          wide_mul_add(a1, wt[j].c1, x);
+        }
       }
     }
   }
-  out[i] = acc;
+  const ef2 r{wide_reduce(a0), wide_reduce(a1)};
+  if (sub) sh[sub - 1][lane] = r;
+  __syncthreads();
+  if (sub == 0 && i < D) out[i] = ef_add(ef_add(r, sh[0][lane]), ef_add(sh[1][lane], sh[2][lane]));
 }
 
-// m[x] = sum_{b < 2^li} eq4[b] * poly[x * 2^li + b] (MlPoly::fold over the low li variables)
-__global__ void fold_low_kernel(const uint64_t* __restrict__ poly, int ext, int li, uint64_t nout,
-                                const ef2* __restrict__ eq4, ef2* __restrict__ m) {
-  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= nout) return;
-  ef2 acc = ef_zero();
-  const uint64_t base = x << li;
-  for (uint32_t b = 0; b < (1u << li); b++) {
-    if (ext) {
-      const ef2 v{poly[2 * (base + b)], poly[2 * (base + b) + 1]};
-      acc = ef_add(acc, ef_mul(eq4[b], v));
-    } else {
-      acc = ef_add(acc, ef_mul_base(eq4[b], poly[base + b]));
+// m[x] = sum_{b < 2^li} eq4[b] * poly[x * 2^li + b] (MlPoly::fold over the low li variables); a
+// workgroup takes 64 consecutive x, its four waves split b (b mod 4) as in bf_combine_kernel.
+__global__ __launch_bounds__(256) void fold_low_kernel(const uint64_t* __restrict__ poly, int ext, int li,
+                                                       uint64_t nout, const ef2* __restrict__ eq4,
+                                                       ef2* __restrict__ m) {
+  __shared__ ef2 sh[3][64];
+  const uint32_t lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
+  const uint64_t x = (uint64_t)blockIdx.x * 64 + lane;
+  WideAcc a0 = wide_zero(), a1 = wide_zero();
+  if (x < nout) {
+    const uint64_t base = x << li;
+    for (uint32_t b = sub; b < (1u << li); b += 4) {
+      if (ext) {
+        const ef2 e = ef_mul(eq4[b], ef2{poly[2 * (base + b)], poly[2 * (base + b) + 1]});
+        wide_add(a0, e.c0);
+        wide_add(a1, e.c1);
+      } else {
+        const uint64_t v = poly[base + b];
+        wide_mul_add(a0, eq4[b].c0, v);
+        wide_mul_add(a1, eq4[b].c1, v);
+      }
     }
   }
-  m[x] = acc;
+  const ef2 r{wide_reduce(a0), wide_reduce(a1)};
+  if (sub) sh[sub - 1][lane] = r;
+  __syncthreads();
+  if (sub == 0 && x < nout) m[x] = ef_add(ef_add(r, sh[0][lane]), ef_add(sh[1][lane], sh[2][lane]));
 }
 
 // One reduction round: partial sums of m[2i] * eq_rest[i] (l0, basefold.rs:752-758) and the
@@ -578,6 +676,81 @@ __global__ __launch_bounds__(256) void bf_round3_kernel(BfRound a, uint64_t half
   if (threadIdx.x == 0) a.part[(uint64_t)c * gridDim.x + blockIdx.x] = acc[0];
 }
 
+// The last Basefold reduction rounds t0..R-1 (level t0 has at most 2^kBfTailLog entries) in
+// one workgroup of 512 threads, with the codeword, the three channels' m and eq tables in LDS.
+// Per round: thread i folds codeword pair (i, i + len/2) (written back in place and to the
+// level's place in global memory, for the Merkle trees and openings), and for i < len/4 adds
+// m_c[2i] eq_c[i] to channel c's sum and folds m_c and eq_c; reads and writes of m and eq are
+// separated by a barrier. The round sums (outputs only) stay per wave until the end.
+__global__ __launch_bounds__(512) void bf_tail_kernel(BfTail a) {
+  __shared__ ef2 cw[1 << kBfTailLog];
+  __shared__ ef2 m[3][1 << (kBfTailLog - 1)];
+  __shared__ ef2 eq[3][1 << (kBfTailLog - 2)];
+  __shared__ ef2 wp[kBfTailLog][8][3];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  for (int c = 0; c < 3; c++) {  // the last grid round's sums
+    if (a.prev[c].nblk) red_job_body(a.prev[c], &wp[0][0][0]);
+    __syncthreads();
+  }
+  const uint32_t len0 = 1u << (a.logD - a.t0);
+  for (uint32_t i = tid; i < len0; i += blockDim.x) cw[i] = a.cw_in[i];
+  for (int c = 0; c < 3; c++) {
+    for (uint32_t i = tid; i < len0 / 2; i += blockDim.x) m[c][i] = a.m_in[c][i];
+    for (uint32_t i = tid; i < len0 / 4; i += blockDim.x) eq[c][i] = a.eq[c][i];
+  }
+  __syncthreads();
+  ef2* lvl = a.lvl_out;
+  for (uint32_t t = a.t0; t < a.R; t++) {
+    const uint32_t k = t - a.t0, len = len0 >> k, h = len / 2, half = len / 4;
+    const uint32_t lsh = t;  // log2(D / len): the order-len twiddles are every 2^t-th of tw
+    const ef2 r = a.chal[t];
+    ef2 cwv = ef_zero(), mv[3], ev[3], acc[3];
+    if (tid < h) {  // eq_fold_codeword_at over the subgroup of order len
+      const ef2 x = cw[tid], nx = cw[tid + h];
+      const uint64_t ginv = tid == 0 ? 1 : gl_neg(a.tw[(uint64_t)(h - tid) << lsh]);
+      const ef2 pe = ef_mul_base(ef_add(x, nx), GL_INV2);
+      const ef2 po = ef_mul_base(ef_sub(x, nx), gl_mul(ginv, GL_INV2));
+      cwv = ef_add(ef_mul(ef_one_minus(r), pe), ef_mul(r, po));
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      acc[c] = ef_zero();
+      if (tid < half) {
+        const ef2 e = m[c][2 * tid], o = m[c][2 * tid + 1];
+        acc[c] = ef_mul(e, eq[c][tid]);
+        mv[c] = ef_add(e, ef_mul(r, ef_sub(o, e)));
+        if (tid < half / 2) ev[c] = ef_add(eq[c][2 * tid], eq[c][2 * tid + 1]);
+      }
+    }
+    __syncthreads();  // every read of this round's tables is done
+    if (tid < h) {
+      cw[tid] = cwv;
+      lvl[tid] = cwv;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      if (tid < half) m[c][tid] = mv[c];
+      if (tid < half / 2) eq[c][tid] = ev[c];
+      acc[c] = wave_sum(acc[c]);
+      if (lane == 0) wp[k][wave][c] = acc[c];
+    }
+    lvl += h;
+    __syncthreads();
+  }
+  const uint32_t rounds = a.R - a.t0, nw = blockDim.x >> 6;
+  if (tid < 3 * rounds) {
+    const uint32_t k = tid / 3, c = tid % 3;
+    ef2 t = ef_zero();
+    for (uint32_t w = 0; w < nw; w++) t = ef_add(t, wp[k][w][c]);
+    a.l0[(uint64_t)c * a.R + a.t0 + k] = t;
+  }
+}
+
+hipError_t launch_bf_tail(const BfTail& a, hipStream_t st) {
+  if (a.t0 < a.R) hipLaunchKernelGGL(bf_tail_kernel, dim3(1), dim3(512), 0, st, a);
+  return hipGetLastError();
+}
+
 uint32_t bf_round3_blocks(uint64_t half, uint64_t cw_half) {
   // blocks of the three channels; the codeword fold (y = 3) grid-strides over the same count,
   // at least 64 blocks while it has work for them; at least 3 for the reductions (y = 4)
@@ -615,8 +788,8 @@ __global__ void gather_parts_kernel(const GatherPart* __restrict__ parts, uint32
 #define LGS_GRID(n) dim3((uint32_t)(((n) + 255) / 256)), dim3(256)
 
 hipError_t launch_bits_gather(const uint64_t* enc, uint64_t ld, const uint32_t* idx, uint32_t s, uint64_t W,
-                              uint64_t* bi, hipStream_t st) {
-  hipLaunchKernelGGL(bits_gather_kernel, LGS_GRID(W * s), 0, st, enc, ld, idx, s, W, bi);
+                              uint64_t* bi, uint8_t* bi8, hipStream_t st) {
+  hipLaunchKernelGGL(bits_gather_kernel, LGS_GRID(W * s), 0, st, enc, ld, idx, s, W, bi, bi8);
   return hipGetLastError();
 }
 hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st,
@@ -642,12 +815,13 @@ hipError_t launch_planar_to_ef(const uint64_t* c0, const uint64_t* c1, uint64_t 
   return hipGetLastError();
 }
 hipError_t launch_bf_combine(const BfChannels& ch, uint64_t D, ef2* out, hipStream_t st) {
-  hipLaunchKernelGGL(bf_combine_kernel, LGS_GRID(D), 0, st, ch, D, out);
+  hipLaunchKernelGGL(bf_combine_kernel, dim3((uint32_t)((D + 63) / 64)), dim3(256), 0, st, ch, D, out);
   return hipGetLastError();
 }
 hipError_t launch_fold_low(const uint64_t* poly, int ext, int li, uint64_t nout, const ef2* eq4, ef2* m,
                            hipStream_t st) {
-  hipLaunchKernelGGL(fold_low_kernel, LGS_GRID(nout), 0, st, poly, ext, li, nout, eq4, m);
+  hipLaunchKernelGGL(fold_low_kernel, dim3((uint32_t)((nout + 63) / 64)), dim3(256), 0, st, poly, ext, li, nout,
+                     eq4, m);
   return hipGetLastError();
 }
 hipError_t launch_gather_parts(const GatherPart* parts, uint32_t nparts, const uint64_t* off, uint64_t total,

@@ -5,7 +5,7 @@
 namespace lgs {
 
 // Deferred reduction of per-block partial sums: out[q] = sum_{b < nblk} part[b * w + q] for
-// q < w <= 4. Kernels that produce partials leave them in memory; a block of a later launch on
+// q < w <= 16. Kernels that produce partials leave them in memory; a block of a later launch on
 // the same stream adds them up, after the producing launch has completed, so no cross-workgroup
 // fence or completion counter is needed (an agent-scope fence writes back and invalidates the
 // L2 of the workgroup's XCD).
@@ -21,6 +21,8 @@ struct RedJobs {
 };
 hipError_t launch_reduce_jobs(const RedJobs& r, hipStream_t st);
 
+struct EqTable;
+
 // One round of one sumcheck inside a grouped launch (sc_round_group_kernel): blocks
 // [first_block, first_block + nblk) of the launch. p[2] is unused when d == 2.
 struct ScRoundJob {
@@ -28,23 +30,27 @@ struct ScRoundJob {
   uint64_t half;
   const ef2* r;
   ef2* part;  // nblk x (d + 1) partial sums, reduced into the round's d + 1 values later
-  // bits: first round of a sumcheck whose first table is 0/1 valued, read from these words
-  // (2 half entries); the folded bit table goes to p[0] (and its copy minus one to p[1] when
-  // d = 3), the last table p[d - 1] folds in place. Null for an ordinary round.
-  const uint64_t* bits;
-  uint32_t d, first_block, nblk, pad;
+  // bits: first round of a sumcheck whose first table is 0/1 valued, read from these bytes
+  // (2 half entries), and whose last table is the eq table eqt, evaluated from its factors
+  // rather than read; the folded bit table goes to p[0], the folded eq table to p[d - 1]. Null
+  // for an ordinary round.
+  const uint8_t* bits;
+  const EqTable* eqt;
+  // m1: d = 3 and the second table is the first minus one (not stored: p[1] unused)
+  uint32_t d, first_block, nblk, m1;
 };
 // Blocks past total_blocks run the reductions `reds` (one block each).
 hipError_t launch_sc_round_group(const ScRoundJob* jobs, int njobs, uint32_t total_blocks, const RedJob* reds,
                                  uint32_t nreds, hipStream_t st);
 
-// The remaining rounds (len <= 128 entries) of one sumcheck, one wave per job.
+// The remaining rounds (len <= 2^kScTailLog entries) of one sumcheck, one workgroup per job.
+constexpr int kScTailLog = 10;
 struct ScTailJob {
   const ef2* p[3];
   const ef2* chal;
   ef2* rounds;
   ef2* finals;
-  uint32_t d, len;
+  uint32_t d, len, m1, pad;  // m1 as in ScRoundJob
 };
 // Blocks past njobs run the reductions `reds`.
 hipError_t launch_sc_tail_group(const ScTailJob* jobs, uint32_t njobs, const RedJob* reds, uint32_t nreds,
@@ -68,6 +74,22 @@ struct BfRound {
 // round's reductions); the partials of this round are that many per channel.
 uint32_t bf_round3_blocks(uint64_t half, uint64_t cw_half);
 hipError_t launch_bf_round3(const BfRound& a, uint64_t half, const ef2* r, hipStream_t st);
+
+// Rounds t0..R-1 of the Basefold reduction in one workgroup (bf_tail_kernel); level t0 has
+// D >> t0 <= 2^kBfTailLog entries.
+constexpr int kBfTailLog = 10;
+struct BfTail {
+  const ef2* m_in[3];  // round t0's m (D >> (t0 + 1) entries) and eq (D >> (t0 + 2)) per channel
+  const ef2* eq[3];
+  RedJob prev[3];       // sums of the last grid round (nblk 0: none)
+  const ef2* cw_in;     // level t0
+  ef2* lvl_out;         // level t0 + 1; each later level follows the previous one
+  ef2* l0;              // round t's sum of channel c goes to l0[c * R + t]
+  const ef2* chal;      // round t's challenge at chal[t]
+  const uint64_t* tw;   // tw[k] = g^k (k < D / 2), g of order D
+  uint32_t R, t0, logD, pad;
+};
+hipError_t launch_bf_tail(const BfTail& a, hipStream_t st);
 
 // Up to three Basefold channels combined in one pass (basefold.rs:732-741).
 struct BfChannels {
@@ -103,9 +125,10 @@ hipError_t launch_wrows(const uint64_t* M, uint32_t rows, uint64_t cols, const e
 // Partial sums of sum_i a[i] b[i], one per block (dot_blocks(n) blocks), for a RedJob.
 uint32_t dot_blocks(uint64_t n);
 hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part, hipStream_t st);
-// B_I as 0/1 words: bi[i * s + k] = bit i of column idx[k] of the encoded rows.
+// B_I as 0/1 words and bytes: bi[i * s + k] = bi8[i * s + k] = bit i of column idx[k] of the
+// encoded rows.
 hipError_t launch_bits_gather(const uint64_t* enc, uint64_t ld, const uint32_t* idx, uint32_t s, uint64_t W,
-                              uint64_t* bi, hipStream_t st);
+                              uint64_t* bi, uint8_t* bi8, hipStream_t st);
 hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st,
                      ef2* g2b = nullptr);
 hipError_t launch_vvec(const ef2* eqz1, uint64_t W, ef2* v, hipStream_t st);

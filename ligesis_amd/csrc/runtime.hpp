@@ -172,6 +172,31 @@ struct DevBasefold {
   uint32_t code_rows() const { return (ext ? 2u : 1u) << li; }
 };
 
+// Pinned host staging for uploads: a copy from pageable memory starts only once its stream has
+// drained, which holds the issuing thread behind the device. Bump-allocated in chunks (256-byte
+// aligned); reset() may run only when no copy from the arena is pending (after the streams that
+// read it were synchronized) and merges the chunks into one.
+class PinnedArena {
+ public:
+  PinnedArena() = default;
+  PinnedArena(const PinnedArena&) = delete;
+  PinnedArena& operator=(const PinnedArena&) = delete;
+  ~PinnedArena() { release(); }
+  void reset();
+  // copies src into the arena and enqueues the copy to dst; returns the staged host copy
+  void* upload(void* dst, const void* src, size_t bytes, hipStream_t s);
+  void release();
+  void take(PinnedArena& o) {  // o's memory (o must have no pending copy), reset
+    release();
+    chunks_.swap(o.chunks_);
+    reset();
+  }
+
+ private:
+  std::vector<std::pair<uint8_t*, size_t>> chunks_;
+  size_t used_ = 0;  // bytes used in the last chunk
+};
+
 struct Context {
   // Keys and commitments keep their context alive: lgs_shutdown drops the caller's reference
   // and the context is torn down when the last key / commitment made on it is released.
@@ -193,19 +218,19 @@ struct Context {
   int ntt_impl = -1;     // -1 auto; row transform: 0/2/3 radix-8 with 256/512/1024 threads, 1 radix-2
   int merkle_lg = 0;     // 0 auto; log2 threads per Merkle workgroup = levels per launch (6..8)
   int pipeline = 0;      // 0 auto; 1 off, 2 / 4 / 8 row chunks (transform on stream, hash on stream2)
+  // replay repeated proofs of a commitment as a captured graph (off by default: on this ROCm the
+  // graph ran the two streams' steps in capture order, slower than issuing them directly)
+  bool prove_graph = false;
   bool profile = false;
-  double stage_ms[12] = {0};
-  hipEvent_t ev[8] = {};
+  double stage_ms[28] = {0};
+  hipEvent_t ev[10] = {};
   uint8_t* pinned = nullptr;  // small pinned staging buffer for roots
   size_t pinned_size = 0;
-  // Pinned host arena for the job tables and constants one prove or batched commit uploads: a
-  // copy from pageable memory starts only once its stream has drained, which holds the issuing
-  // thread behind the device. Bump-allocated in chunks; upload_reset() may run only when no copy
-  // from the arena is pending (after the streams that read it were synchronized).
-  std::vector<std::pair<uint8_t*, size_t>> up_chunks;
-  size_t up_used = 0;  // bytes used in the last chunk
-  void upload_reset();
-  void upload(void* dst, const void* src, size_t bytes, hipStream_t s);
+  // Pinned staging of the job tables and constants a batched commit or a prove uploads.
+  PinnedArena arena;
+  PinnedArena prove_arena;  // staging a proof borrows for its uploads (see prove.cpp)
+  void upload_reset() { arena.reset(); }
+  void upload(void* dst, const void* src, size_t bytes, hipStream_t s) { arena.upload(dst, src, bytes, s); }
   void free_host();  // pinned staging and arena
   std::vector<uint8_t> proof_buf;  // encoded proof of the last lgs_prove
 
@@ -216,7 +241,7 @@ struct Context {
   hipEvent_t pipe_event(int i);
   // profiled pipelined commit: events around chunk q's transform (4q, 4q + 1) and hash (4q + 2,
   // 4q + 3); prof_chunks = number of chunks of the last profiled commit (0: not pipelined)
-  hipEvent_t tev[40] = {};  // [32..37]: phases of a profiled prove
+  hipEvent_t tev[48] = {};  // [32..42]: phases of a profiled prove
   int prof_chunks = 0;
   hipEvent_t timing_event(int i);
   // launch schedule of a commit of shape 2^log_m x 2^log_n
@@ -235,6 +260,8 @@ struct Key {
   uint8_t com_a[32] = {0};
 };
 
+struct ProvePlan;  // prove.cpp
+
 struct Commitment {
   Context* ctx = nullptr;
   const Key* key = nullptr;
@@ -245,6 +272,9 @@ struct Commitment {
   DevBuf<uint64_t> h;           // 32 x 2n
   DevTree tree;
   uint8_t root[32] = {0};
+  // device state of the last proof on ctx, replayed as a graph by later proofs (see prove.cpp)
+  mutable std::shared_ptr<ProvePlan> plan;
+  mutable uint32_t prove_calls = 0;
 };
 
 // Buffers and host job tables of a batched commit, kept until the caller synchronises.
@@ -273,7 +303,7 @@ struct MerkleSource {
 };
 void enqueue_merkle_trees(Context& ctx, const std::vector<MerkleSource>& trees,
                           std::vector<std::vector<MerkleJob>>& host_jobs, std::vector<DevBuf<uint64_t>>& tables,
-                          hipStream_t st);
+                          hipStream_t st, PinnedArena* arena = nullptr);
 // On stream `s` (default: the context stream); `encoded`, if set, is recorded once the codewords
 // are written (before the Merkle tree). The twiddle table of the codeword length must already
 // exist when `s` is not the context stream.

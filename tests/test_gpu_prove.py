@@ -85,6 +85,35 @@ def test_prove_is_repeatable_and_commitment_reusable(gpu_ctx):
     assert oracle.verify(1, 9, vk.com_a, root, z2, p2, inp.fields, inp.ints)
 
 
+@pytest.mark.parametrize("shape", [(3, 10), (4, 13)])
+def test_repeated_proofs_replay_a_graph(gpu_ctx, shape):
+    """The second proof of a commitment captures its launches into a graph and later ones
+    replay it with new constants: each of five proofs (own point and oracle streams) equals the
+    oracle's, with other commits and proofs on the context between the replays, and again with
+    the graph switched off."""
+    log_m, log_n = shape
+    inp = synth.Inputs(log_m, log_n, seed=91 + log_n)
+    pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx)
+    _, cd = Ligesis.commit(pk, inp.f)
+    opk = oracle.ProvingKey(log_m, log_n, inp.a)
+    ocd = oracle.CommitData(opk, inp.f)
+    other = synth.Inputs(2, 9, seed=17)
+    opk2, _ = Ligesis.setup(2, 9, other.a, ctx=gpu_ctx)
+    try:
+        for k, graph in enumerate([1, 1, 1, 1, 1, 0]):
+            gpu_ctx.set_option("prove_graph", graph)
+            q = synth.Inputs(log_m, log_n, seed=500 + k, with_a=False)
+            print(f"prove {k} shape={shape} graph={graph}", flush=True)
+            proof = Ligesis.prove(pk, cd, q.z, RandomOracle(q.fields, q.ints))
+            want, _ = oracle.prove(opk, ocd, q.z, q.fields, q.ints)
+            assert proof == want, k
+            # other work on the context between the proofs
+            _, cd2 = Ligesis.commit(opk2, other.f)
+            Ligesis.prove(opk2, cd2, other.z, RandomOracle(other.fields, other.ints))
+    finally:
+        gpu_ctx.set_option("prove_graph", 0)
+
+
 def test_prove_errors(gpu_ctx):
     inp = synth.Inputs(1, 8, seed=3)
     pk, _ = Ligesis.setup(1, 8, inp.a, ctx=gpu_ctx)
