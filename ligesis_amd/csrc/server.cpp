@@ -1,11 +1,12 @@
 // Serving loop: commit-then-open requests of mixed sizes, continuously batched.
 //
 // One commit worker admits queued requests in arrival order while their HBM footprint fits the
-// budget, commits every admitted request in one batched pass (grouped launches where the shape
-// allows), and hands the resident commitments (f, F', H, Merkle tree) to P prove workers. Each
-// worker proves on its own context and stream, so proofs of one batch run concurrently with each
-// other and with the commit of the next batch. Memory is returned to the budget when a proof is
-// done; requests that arrive meanwhile join the next batch as soon as they fit.
+// budget, uploads their inputs on its own stream while the previous batch is still committing,
+// commits every admitted request in one batched pass (grouped launches where the shape allows),
+// and hands the resident commitments (f, F', H, Merkle tree) to P prove workers. Each worker
+// proves on its own context and stream, so proofs of one batch run concurrently with each other
+// and with the uploads and commits of the next batches. Memory is returned to the budget when a
+// proof is done; requests that arrive meanwhile join the next batch as soon as they fit.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -68,96 +69,173 @@ struct Server {
   std::vector<std::thread> threads;
   uint64_t batches = 0, batched_requests = 0;
 
+  hipStream_t upload_stream = nullptr;  // host-to-device copies of request inputs
+
+  struct Batch;
+  std::unique_ptr<Batch> start_batch(std::vector<std::shared_ptr<Request>>& batch);
+  void commit_batch(Batch& b);
+  void finish_batch(Batch& b);
   void commit_loop();
   void prove_loop(Context* ctx);
 };
 
-void Server::commit_loop() {
+// A batch between admission and hand-over: its inputs go to the device on the upload stream,
+// its commits run on the commit context's stream after them.
+struct Server::Batch {
+  std::vector<std::shared_ptr<Request>> ok;
+  GroupHold hold;
+  hipEvent_t uploaded = nullptr, done = nullptr;
+  int rc = 0;
+  std::string msg;
+  ~Batch() {
+    if (uploaded) (void)hipEventDestroy(uploaded);
+    if (done) (void)hipEventDestroy(done);
+  }
+};
+
+// Uploads of one admitted batch: a commitment per request, f copied on the upload stream.
+std::unique_ptr<Server::Batch> Server::start_batch(std::vector<std::shared_ptr<Request>>& batch) {
   Context& ctx = *ctxs[0];
+  std::unique_ptr<Batch> b(new Batch());
+  for (auto& r : batch) {
+    try {
+      r->cm.reset(new Commitment());
+      Commitment& c = *r->cm;
+      c.ctx = &ctx;
+      c.key = r->key;
+      if (r->flags & 1u) {
+        c.f = r->f;
+      } else {
+        c.f_owned = DevBuf<uint64_t>(&ctx.pool, r->len);
+        hip_check(hipMemcpyAsync(c.f_owned.ptr, r->f, r->len * 8, hipMemcpyHostToDevice, upload_stream),
+                  "upload f");
+        c.f = c.f_owned.ptr;
+      }
+      b->ok.push_back(r);
+    } catch (const std::exception& e) {
+      r->cm.reset();
+      std::lock_guard<std::mutex> lk(mu);
+      r->status = 4;
+      r->err = e.what();
+      reserved -= r->footprint;
+      cv_done.notify_all();
+    }
+  }
+  hip_check(hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming), "batch event");
+  hip_check(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "batch event");
+  hip_check(hipEventRecord(b->uploaded, upload_stream), "uploaded");
+  return b;
+}
+
+// Commits of a batch whose uploads are enqueued: grouped launches where the shape allows, then
+// the roots to the host.
+void Server::commit_batch(Batch& b) {
+  Context& ctx = *ctxs[0];
+  try {
+    hip_check(hipStreamWaitEvent(ctx.stream, b.uploaded, 0), "wait uploads");
+    std::vector<std::pair<const Key*, Commitment*>> grouped, single;
+    for (auto& r : b.ok) (group_eligible(*r->key) ? grouped : single).push_back({r->key, r->cm.get()});
+    enqueue_commit_group(ctx, grouped, b.hold);
+    for (auto& s : single) enqueue_commit(ctx, *s.first, *s.second);
+    for (auto& r : b.ok) {
+      const DevTree& t = r->cm->tree;
+      hip_check(hipMemcpyAsync(r->root, t.nodes.ptr + (t.node_count() - 1) * 8, 32, hipMemcpyDeviceToHost,
+                               ctx.stream),
+                "read root");
+    }
+    hip_check(hipEventRecord(b.done, ctx.stream), "committed");
+  } catch (const std::exception& e) {
+    const Error* le = dynamic_cast<const Error*>(&e);
+    b.rc = le ? le->code : (dynamic_cast<const std::bad_alloc*>(&e) ? 4 : 1);
+    b.msg = e.what();
+  }
+}
+
+// Waits for a batch's commits and hands its commitments to the prove workers.
+void Server::finish_batch(Batch& b) {
+  Context& ctx = *ctxs[0];
+  if (!b.rc) {
+    const hipError_t e = hipEventSynchronize(b.done);
+    if (e != hipSuccess) {
+      b.rc = 3;
+      b.msg = std::string("commit sync: ") + hipGetErrorString(e);
+    }
+  }
+  if (b.rc) {
+    (void)hipStreamSynchronize(upload_stream);
+    (void)hipStreamSynchronize(ctx.stream);
+    if (ctx.stream2) (void)hipStreamSynchronize(ctx.stream2);
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  batches++;
+  batched_requests += b.ok.size();
+  const double t = now_ms();
+  for (auto& r : b.ok) {
+    if (b.rc) {
+      r->status = b.rc;
+      r->err = b.msg;
+      r->cm.reset();
+      reserved -= r->footprint;
+      continue;
+    }
+    std::memcpy(r->cm->root, r->root, 32);
+    r->t_committed = t;
+    committed.push_back(r);
+  }
+  cv_prove.notify_all();
+  cv_done.notify_all();
+  cv_commit.notify_all();
+}
+
+// Admission and commits, pipelined: the uploads of a newly admitted batch are enqueued before the
+// batch in flight is waited for, so they overlap its commit kernels (and the proofs running on
+// the workers' streams); its commits follow once the batch in flight is handed over.
+void Server::commit_loop() {
   (void)hipSetDevice(device);
+  std::unique_ptr<Batch> inflight;
   while (true) {
     std::vector<std::shared_ptr<Request>> batch;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_commit.wait(lk, [&] {
-        return stopping || (!pending.empty() && reserved + pending.front()->footprint <= budget);
-      });
-      if (stopping) {  // queued requests were cancelled by stop; in-flight ones finish
+      auto admissible = [&] { return !pending.empty() && reserved + pending.front()->footprint <= budget; };
+      if (!inflight) cv_commit.wait(lk, [&] { return stopping || admissible(); });
+      if (stopping && !inflight) {  // queued requests were cancelled by stop; admitted ones finish
         commit_running = false;
         cv_prove.notify_all();
         return;
       }
-      while (!pending.empty() && batch.size() < max_batch && reserved + pending.front()->footprint <= budget) {
+      while (!stopping && admissible() && batch.size() < max_batch) {
         reserved += pending.front()->footprint;
         batch.push_back(pending.front());
         pending.pop_front();
       }
     }
-    std::vector<std::pair<const Key*, Commitment*>> grouped, single;
-    std::vector<std::shared_ptr<Request>> ok;
-    for (auto& r : batch) {
-      try {
-        r->cm.reset(new Commitment());
-        Commitment& c = *r->cm;
-        c.ctx = &ctx;
-        c.key = r->key;
-        if (r->flags & 1u) {
-          c.f = r->f;
-        } else {
-          c.f_owned = DevBuf<uint64_t>(&ctx.pool, r->len);
-          hip_check(hipMemcpyAsync(c.f_owned.ptr, r->f, r->len * 8, hipMemcpyHostToDevice, ctx.stream), "upload f");
-          c.f = c.f_owned.ptr;
-        }
-        (group_eligible(*r->key) ? grouped : single).push_back({r->key, &c});
-        ok.push_back(r);
-      } catch (const std::exception& e) {
+    if (batch.empty()) {  // nothing more fits now: hand over the batch in flight
+      finish_batch(*inflight);
+      inflight.reset();
+      continue;
+    }
+    std::unique_ptr<Batch> next;
+    try {
+      next = start_batch(batch);
+    } catch (const std::exception& e) {  // event creation failed: fail the batch's requests
+      std::lock_guard<std::mutex> lk(mu);
+      for (auto& r : batch) {
+        if (r->status >= 0) continue;
         r->cm.reset();
-        std::lock_guard<std::mutex> lk(mu);
-        r->status = 4;
+        r->status = 3;
         r->err = e.what();
         reserved -= r->footprint;
-        cv_done.notify_all();
       }
+      cv_done.notify_all();
+      continue;
     }
-    int rc = 0;
-    std::string msg;
-    try {
-      GroupHold hold;
-      enqueue_commit_group(ctx, grouped, hold);
-      for (auto& s : single) enqueue_commit(ctx, *s.first, *s.second);
-      for (auto& r : ok) {
-        const DevTree& t = r->cm->tree;
-        hip_check(hipMemcpyAsync(r->root, t.nodes.ptr + (t.node_count() - 1) * 8, 32, hipMemcpyDeviceToHost,
-                                 ctx.stream),
-                  "read root");
-      }
-      hip_check(hipStreamSynchronize(ctx.stream), "commit sync");
-    } catch (const std::exception& e) {
-      const Error* le = dynamic_cast<const Error*>(&e);
-      rc = le ? le->code : (dynamic_cast<const std::bad_alloc*>(&e) ? 4 : 1);
-      msg = e.what();
-      (void)hipStreamSynchronize(ctx.stream);
-      if (ctx.stream2) (void)hipStreamSynchronize(ctx.stream2);
+    if (inflight) {
+      finish_batch(*inflight);
+      inflight.reset();
     }
-    std::lock_guard<std::mutex> lk(mu);
-    batches++;
-    batched_requests += ok.size();
-    const double t = now_ms();
-    for (auto& r : ok) {
-      if (rc) {
-        r->status = rc;
-        r->err = msg;
-        r->cm.reset();
-        reserved -= r->footprint;
-        continue;
-      }
-      std::memcpy(r->cm->root, r->root, 32);
-      r->t_committed = t;
-      committed.push_back(r);
-    }
-    cv_prove.notify_all();
-    cv_done.notify_all();
-    cv_commit.notify_all();
+    commit_batch(*next);
+    inflight = std::move(next);
   }
 }
 
@@ -212,6 +290,7 @@ Server* server_start(int device, uint64_t budget, uint32_t workers, uint32_t max
     for (auto& e : c->ev) hip_check(hipEventCreate(&e), "event");
     s->ctxs.push_back(std::move(c));
   }
+  hip_check(hipStreamCreateWithFlags(&s->upload_stream, hipStreamNonBlocking), "stream");
   s->threads.emplace_back(&Server::commit_loop, s.get());
   for (uint32_t i = 1; i <= workers; i++) s->threads.emplace_back(&Server::prove_loop, s.get(), s->ctxs[i].get());
   return s.release();
@@ -288,6 +367,10 @@ void server_stop(Server* s) {
   for (auto& t : s->threads) t.join();
   for (auto& kv : s->all) kv.second->cm.reset();
   s->all.clear();
+  if (s->upload_stream) {
+    (void)hipStreamSynchronize(s->upload_stream);
+    (void)hipStreamDestroy(s->upload_stream);
+  }
   for (auto& c : s->ctxs) {
     (void)hipStreamSynchronize(c->stream);
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
