@@ -70,16 +70,28 @@ __global__ __launch_bounds__(256) void ntt_lds_kernel(const uint64_t* __restrict
 // a reduction, so the internal twiddles of an 8-point DFT (fixed exponents) cost no general
 // multiplication.
 
-// x * 2^e mod p for 0 <= e < 192
+// (hi 2^64 + lo) mod p for hi < 2^32: lo + hi (2^32 - 1), one carry fix, then canonical.
+__device__ __forceinline__ uint64_t gl_reduce_small(uint64_t lo, uint64_t hi) {
+  const uint64_t t = (hi << 32) - hi;
+  uint64_t r = lo + t;
+  r += (r < t) ? GL_EPS : 0;  // a wrap is 2^64 = 2^32 - 1; no second wrap (t < 2^64 - 2^33)
+  return gl_canon(r);
+}
+
+// x * 2^e mod p for 0 <= e < 192 (e is a compile-time constant where this is inlined)
 __device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x, uint32_t e) {
   const bool neg = e >= 96;
   if (neg) e -= 96;
   uint64_t r;
-  if (e < 64) {
-    r = gl_reduce(x << e, e ? (x >> (64 - e)) : 0);
+  if (e == 0) {
+    r = x;
+  } else if (e <= 32) {
+    r = gl_reduce_small(x << e, x >> (64 - e));
+  } else if (e < 64) {
+    r = gl_reduce(x << e, x >> (64 - e));
   } else {  // x * 2^e = (x * 2^(e - 32)) * 2^32, both factors below 2^64 shifts
     const uint64_t y = gl_reduce(x << (e - 32), x >> (96 - e));
-    r = gl_reduce(y << 32, y >> 32);
+    r = gl_reduce_small(y << 32, y >> 32);
   }
   return neg ? gl_neg(r) : r;
 }
@@ -88,7 +100,8 @@ __device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x, uint32_t e) {
 // belong to a larger transform 2^tsh N (entries taken with stride 2^tsh).
 __device__ __forceinline__ uint64_t tw_full(const uint64_t* __restrict__ tw, uint32_t k, uint32_t half,
                                             int tsh) {
-  return k < half ? tw[(uint64_t)k << tsh] : gl_neg(tw[(uint64_t)(k - half) << tsh]);
+  const uint64_t v = tw[(uint64_t)(k & (half - 1)) << tsh];  // half is a power of two
+  return (k & half) ? gl_neg(v) : v;
 }
 
 // In-place radix-2 DIT network on 8 values held in bit-reversed order; output natural order.
@@ -147,11 +160,15 @@ __device__ __forceinline__ void pretwiddle(uint64_t (&x)[1 << LG], uint32_t j, i
   }
 }
 
-// LDS word of row position w: the low 5 bits XOR the top 5 bits of the row index (sh = logN - 5;
-// sh = 31 leaves w alone). The first pass scatters to bit-reversed blocks, where 32 consecutive
-// lanes differ only in those top bits and would otherwise all hit one bank; runs of 32
-// consecutive positions (every later pass) stay a permutation of one bank row.
-__device__ __forceinline__ uint32_t lds_pos(uint32_t w, int sh) { return w ^ ((w >> sh) & 31u); }
+// LDS word of row position w: the low 5 bits XOR the top 5 bits of the row index (sh = logN - 5)
+// and bits 3-4 also XOR bits 6-7 (sh = 31 leaves w alone). The first pass scatters to
+// bit-reversed blocks, where 32 consecutive lanes differ only in the top bits; the pass over
+// 8-point groups (stride 8) has lanes 8 apart in bits 6-7; both would otherwise hit few banks.
+// Only bits 0-4 change, as a function of bits >= 5, so this is a permutation, and runs of 32
+// consecutive positions (the other passes) stay a permutation of one bank row.
+__device__ __forceinline__ uint32_t lds_pos(uint32_t w, int sh) {
+  return sh == 31 ? w : w ^ ((w >> sh) & 31u) ^ (((w >> 6) & 3u) << 3);
+}
 __host__ __device__ constexpr size_t ntt_lds_bytes(uint32_t N) { return (size_t)N * sizeof(uint64_t); }
 
 // One row: src[k * es] for k < n_in (zero above), transform size 2^logN, result to dst; tw is
@@ -204,7 +221,7 @@ __device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uin
       uint64_t x[8];
 #pragma unroll
       for (int t = 0; t < 8; t++) x[t] = sm[lds_pos(base + t * M, sh)];
-      if (j) pretwiddle<3>(x, j, s, logN, tw, tsh);
+      pretwiddle<3>(x, j, s, logN, tw, tsh);  // j = 0: factors of one (no divergent branch)
       dft8(x);
 #pragma unroll
       for (int t = 0; t < 8; t++) {
