@@ -179,7 +179,38 @@ __device__ __forceinline__ void ntt_r8_row(const uint64_t* __restrict__ src, uin
   const uint32_t N = 1u << logN;
   const int sh = logN >= 10 ? logN - 5 : 31;
   int s = 0;
-  if (logN >= 3) {
+  if (n_in <= N / 2 && logN >= 7 && (logN - 1) % 3 == 0) {
+    // Zero upper half (RS padding): DIT stage 1 pairs a[2i] = x[rev(2i)] with a[2i + 1], a value
+    // of the zero half, so it leaves a[2i] = a[2i + 1] = x[rev(2i)]. The first pass then does
+    // stages 2-4: group (g, j) holds positions 16 g + 2 t + j (t < 8), whose stage-1 values are
+    // x[rev(16 g + 2 t)] = x[k + rev3(t) N / 16] with k = rev_{logN-4}(g) -- consecutive threads
+    // load consecutive words -- pretwiddled by w_16^(j rev3(t)), a power of two. Stages 5..logN
+    // are the radix-8 passes from s = 4, the last one writing dst: one pass fewer than
+    // starting from stage 1, and no radix-2 pass.
+    const uint32_t E = N / 16;
+    for (uint32_t gi = threadIdx.x; gi < N / 8; gi += blockDim.x) {
+      const uint32_t k = gi & (E - 1), j = gi / E;  // j uniform per half of the threads
+      uint64_t x[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        const uint32_t u = __brev((uint32_t)t) >> 29;
+        const uint32_t idx = k + u * E;
+        x[t] = idx < n_in ? src[(uint64_t)idx * es] : 0;
+      }
+      if (j) {
+#pragma unroll
+        for (int t = 1; t < 8; t++) {
+          const uint32_t u = __brev((uint32_t)t) >> 29;
+          x[t] = gl_mul_pow2(x[t], (156u * u) % 192u);  // w_16 = 2^156
+        }
+      }
+      dft8(x);
+      const uint32_t g = brev_bits(k, logN - 4);
+#pragma unroll
+      for (int t = 0; t < 8; t++) sm[lds_pos(16 * g + 2 * t + j, sh)] = x[t];
+    }
+    s = 4;
+  } else if (logN >= 3) {
     // first pass fused with the load: thread k takes x[k + r N/8] (r = 0..7), which sit at
     // positions rev3(r) of block rev_{logN-3}(k) of the bit-reversed array
     const uint32_t E = N / 8;
