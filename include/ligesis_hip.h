@@ -143,7 +143,7 @@ int lgs_commitment_read(const lgs_commitment* c, uint32_t what, void* out, uint6
  * RandomOracle::restart. The proof is written in the canonical encoding (field order of
  * LigesisProof, ligesis.rs:99-119; u64 little-endian lengths, canonical u64 field words) into
  * `proof` when *proof_len (in: capacity in bytes) suffices; *proof_len is set to the size either
- * way (LGS_ERR_BUFFER if it did not fit). used[0], used[1] receive the numbers of field and
+ * way (LGS_ERR_BUFFER if it did not fit; the buffer's contents are then unspecified). used[0], used[1] receive the numbers of field and
  * integer draws, as RandomOracle::drawn() (oracle.rs:57-59). */
 int lgs_prove(lgs_ctx* ctx, const lgs_key* key, const lgs_commitment* c, const uint64_t* z,
               uint64_t z_len, const uint64_t* fields, uint64_t n_fields, const uint64_t* ints,

@@ -356,10 +356,12 @@ int lgs_prove(lgs_ctx* ctx, const lgs_key* key, const lgs_commitment* c, const u
     return LGS_ERR_ARG;
   return guarded(&ctx->c, [&] {
     (void)hipSetDevice(ctx->c.device);
-    std::vector<uint8_t>& bytes = ctx->c.proof_buf;  // reused: no allocation per proof
-    uint64_t u[2] = {0, 0};
+    // encoded straight into the caller's buffer when it is large enough
+    const uint64_t cap = proof ? *proof_len : 0;
+    uint64_t u[2] = {0, 0}, size = 0;
     try {
-      lgs::prove_impl(ctx->c, key->k, c->c, z, z_len, fields, n_fields, ints, n_ints, bytes, u);
+      size = (uint64_t)lgs::prove_impl(ctx->c, key->k, c->c, z, z_len, fields, n_fields, ints, n_ints,
+                                       lgs::ProofOut{nullptr, proof, cap}, u);
     } catch (...) {
       // no buffer released by the failed prove may be reused before both streams drain
       (void)hipStreamSynchronize(ctx->c.stream);
@@ -370,11 +372,8 @@ int lgs_prove(lgs_ctx* ctx, const lgs_key* key, const lgs_commitment* c, const u
       used[0] = u[0];
       used[1] = u[1];
     }
-    uint64_t cap = *proof_len;
-    *proof_len = bytes.size();
-    if (!proof || cap < bytes.size())
-      throw lgs::Error(LGS_ERR_BUFFER, "prove: proof needs " + std::to_string(bytes.size()) + " bytes");
-    std::memcpy(proof, bytes.data(), bytes.size());
+    *proof_len = size;
+    if (cap < size) throw lgs::Error(LGS_ERR_BUFFER, "prove: proof needs " + std::to_string(size) + " bytes");
   });
 }
 

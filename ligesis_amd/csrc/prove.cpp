@@ -81,43 +81,57 @@ std::vector<uint64_t> level_positions(const std::vector<uint64_t>& base, uint64_
   return p;
 }
 
-// Staged gathers: word offsets into several device sources, landing in one staging buffer.
+static_assert(sizeof(GatherPart) % 8 == 0, "gather table is uploaded as words");
+
+// Staged gathers from several device sources into one staging buffer, described by positions
+// (GatherPart): a part is a list of positions, each giving `rows` runs of `width` words.
 struct Gather {
-  struct Part {
-    const uint64_t* src;
-    size_t first, count;
-  };
-  std::vector<uint64_t> offs;
-  std::vector<Part> parts;
-  size_t add(const uint64_t* src, const std::vector<uint64_t>& words) {
-    size_t at = offs.size();
-    offs.insert(offs.end(), words.begin(), words.end());
-    parts.push_back({src, at, words.size()});
+  std::vector<GatherPart> parts;
+  std::vector<uint32_t> pos;
+  uint64_t total = 0;
+  // returns the part's first word in the staging buffer
+  template <class V>
+  uint64_t add(const uint64_t* src, const V& p, uint64_t pmul, uint32_t width, uint32_t rows = 1,
+               uint64_t rstride = 0) {
+    const uint64_t at = total;
+    if (p.empty()) return at;
+    parts.push_back(GatherPart{src, at, pmul, rstride, (uint32_t)pos.size(), rows, width, 0});
+    for (auto v : p) {
+      if ((uint64_t)v > UINT32_MAX) throw Error(1, "prove: opening position out of range");
+      pos.push_back((uint32_t)v);
+    }
+    total += (uint64_t)p.size() * rows * width;
     return at;
   }
-  size_t add_nodes(const DevTree& t, const std::vector<uint64_t>& nodes) {
-    std::vector<uint64_t> w;
-    for (uint64_t nd : nodes)
-      for (int q = 0; q < 4; q++) w.push_back(nd * 4 + q);
-    return add((const uint64_t*)t.nodes.ptr, w);
+  // 4-word digests of tree nodes (node indices of the layered storage)
+  uint64_t add_nodes(const DevTree& t, const std::vector<uint64_t>& nodes) {
+    return add((const uint64_t*)t.nodes.ptr, nodes, 4, 4);
   }
-  size_t add_root(const DevTree& t) { return add_nodes(t, {t.node_count() - 1}); }
+  uint64_t add_root(const DevTree& t) { return add_nodes(t, {t.node_count() - 1}); }
+  uint64_t add_range(const uint64_t* src, uint64_t words) {  // words consecutive words of src
+    return add(src, std::vector<uint32_t>{0}, 0, (uint32_t)words);
+  }
 };
 
 static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "proof encoding copies host words as little-endian");
 
-// Appends into a caller-held vector whose size serves as capacity (a reused vector is written
-// without reallocating or zero-filling); `n` bytes are valid.
+// Appends little-endian words to a ProofOut; past a fixed buffer's end it only counts, so the
+// final n is the proof's size either way.
 struct Writer {
-  std::vector<uint8_t>& b;
+  ProofOut o;
   size_t n = 0;
-  explicit Writer(std::vector<uint8_t>& out) : b(out) {}
-  // little-endian words; the host is little-endian (x86-64), so words are copied as they are
+  explicit Writer(ProofOut out) : o(out) {}
+  // the host is little-endian (x86-64), so words are copied as they are
   void u64(uint64_t v) { words(&v, 1); }
   void words(const uint64_t* w, size_t k) {
-    if (n + 8 * k > b.size()) b.resize(std::max(n + 8 * k, 2 * b.size()));
-    std::memcpy(b.data() + n, w, 8 * k);
-    n += 8 * k;
+    const size_t need = n + 8 * k;
+    if (o.vec) {
+      if (need > o.vec->size()) o.vec->resize(std::max(need, 2 * o.vec->size()));
+      std::memcpy(o.vec->data() + n, w, 8 * k);
+    } else if (need <= o.cap) {
+      std::memcpy(o.dst + n, w, 8 * k);
+    }
+    n = need;
   }
   void raw(const uint64_t* w, size_t n) {  // n words as raw little-endian bytes (digests)
     words(w, n);
@@ -166,7 +180,7 @@ struct ProvePlan {
 
 int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uint64_t* z, uint64_t z_len,
                    const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,
-                   std::vector<uint8_t>& proof, uint64_t used[2]) {
+                   ProofOut out, uint64_t used[2]) {
   if (!key.has_state) throw Error(7, "prove: the key was built commit-only (no A-bar commitment)");
   if (cm.key != &key) throw Error(1, "prove: commitment was made with another key");
   const auto t_start = std::chrono::steady_clock::now();
@@ -328,8 +342,8 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   ef2* res = P.get<ef2>(res_n);
   P.res = res;
 
-  // partial sums of the two dot products (y, y4); the first sumcheck launch reduces them
-  ef2* part = P.get<ef2>(512);
+  // partial sums of the dot product y; the first sumcheck launch reduces them
+  ef2* part = P.get<ef2>(256);
   // Every buffer below lives until the end of the prove: the second stream runs beside this
   // one, so no buffer may go back to the pool (and be handed out again) while work is queued.
 
@@ -482,19 +496,35 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   ef2* g2b = P.get<ef2>(W);
   ef2* vv = P.get<ef2>(W);
   hip_check(launch_wrows(key.a.ptr, 32, W, eqa2, wpart_g1, 1, g1, st), "g1");
-  hip_check(hipStreamWaitEvent(st, ctx.ev[8], 0), "B_I ready");
-  hip_check(launch_g2(bi, (uint32_t)s, W, eqr2, g2, st, g2b), "g2");
   ef2* sel = P.get<ef2>(two_n);
-  hip_check(launch_sel(didx, eqr2, (uint32_t)s, sel, two_n, st), "sel");
-  hip_check(launch_vvec(eqz1, W, vv, st), "v");
   // recon table and RS link (ligesis.rs:488-502); y5 = ra(r3) is sc_table's final evaluation of
   // ra, which is what the reference's verify compares (ligesis.rs:681)
   ef2* wv = P.get<ef2>(n);
-  hip_check(launch_rs_weights(B + o_r3, logN, n, ctx.twiddle(logN), wv, st), "rs weights");
-  // last, since it may wait for the second stream's encoding of a-bar
+  // g2 reads B_I from the second stream; RS(a-bar) may be its codeword of a-bar
+  hip_check(hipStreamWaitEvent(st, ctx.ev[8], 0), "B_I ready");
   if (ra_shared) hip_check(hipStreamWaitEvent(st, ctx.ev[7], 0), "a-bar codeword");
-  hip_check(launch_planar_to_ef(ra_src, ra_src + two_n, two_n, ra, st), "RS(a-bar) layout");
-  hip_check(launch_dot(sel, ra, two_n, part + 256, st), "y4");
+  PrepJobs pj{};
+  pj.bi = bi;
+  pj.s = (uint32_t)s;
+  pj.W = W;
+  pj.eq_r2 = eqr2;
+  pj.g2 = g2;
+  pj.g2b = g2b;
+  pj.sel = sel;
+  pj.sel_n = two_n;
+  pj.eqz1 = eqz1;
+  pj.vv = vv;
+  pj.r3 = B + o_r3;
+  pj.log2n = logN;
+  pj.n = n;
+  pj.tw = ctx.twiddle(logN);
+  pj.wv = wv;
+  pj.c0 = ra_src;
+  pj.c1 = ra_src + two_n;
+  pj.pn = two_n;
+  pj.ra = ra;
+  hip_check(launch_prep_group(pj, st), "sumcheck inputs");
+  hip_check(launch_sel_y4(didx, eqr2, (uint32_t)s, ra, sel, res + o_y4, st), "sel, y4");
 
   // ---- the seven sumchecks, in lockstep ----
   // Every challenge is fixed before the first round (the oracle streams are inputs), so the
@@ -524,8 +554,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   const size_t part_half = (size_t)7 * 1024 * 4;
   ef2* scpart = P.get<ef2>(2 * part_half);
   std::vector<ScRoundJob> rjobs;
-  std::vector<RedJob> reds = {RedJob{part, res + o_y, dot_blocks(n), 1},
-                              RedJob{part + 256, res + o_y4, dot_blocks(two_n), 1}};
+  std::vector<RedJob> reds = {RedJob{part, res + o_y, dot_blocks(n), 1}};
   std::vector<uint32_t> step_first(steps + 1), step_blocks(steps), red_first(steps + 2);
   for (uint32_t tau = 0; tau < steps; tau++) {
     step_first[tau] = (uint32_t)rjobs.size();
@@ -729,67 +758,47 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   const std::vector<uint64_t> bpos = level_positions(base, D / 2);
   const std::vector<uint64_t> hnodes = multiproof_nodes(two_n, sorted);
   const size_t n_hopen = hnodes.size();
-  std::vector<uint64_t> hw;
-  for (uint64_t j : sorted)
-    for (uint64_t r = 0; r < 32; r++) hw.push_back(r * two_n + j);
-  std::vector<uint64_t> fw;
-  for (uint64_t i = 0; i < (D >> R); i++) {
-    fw.push_back(2 * (lvl_off[R] + i));
-    fw.push_back(2 * (lvl_off[R] + i) + 1);
-  }
   const size_t nf = R > 1 ? R - 1 : 0;
-  std::vector<std::vector<uint64_t>> fpos(nf), fnodes(nf), fvw(nf);
+  std::vector<std::vector<uint64_t>> fpos(nf), fnodes(nf);
   for (uint32_t k = 1; k < R; k++) {
     const uint64_t len = D >> k;
     fpos[k - 1] = level_positions(base, len / 2);
     fnodes[k - 1] = multiproof_nodes(len, fpos[k - 1]);
-    for (uint64_t p : fpos[k - 1]) {
-      fvw[k - 1].push_back(2 * (lvl_off[k] + p));
-      fvw[k - 1].push_back(2 * (lvl_off[k] + p) + 1);
-    }
   }
   const std::vector<uint64_t> bnodes = multiproof_nodes(D, bpos);
   const size_t n_bproof = bnodes.size();
-  std::vector<uint64_t> bvw[3];
-  for (int c = 0; c < 3; c++) {
-    const uint32_t rows = states[c]->code_rows();
-    for (uint64_t p : bpos)
-      for (uint32_t rr = 0; rr < rows; rr++) bvw[c].push_back((uint64_t)rr * D + p);
-  }
-  // ---- openings ----
+  // ---- openings: the results first, then every opened word, in one gather ----
+  const uint64_t* lv = (const uint64_t*)P.levels;  // level k's entries from word 2 lvl_off[k]
   Gather g;
-  const size_t g_hopen = g.add_nodes(cm.tree, hnodes);
-  const size_t g_hcols = g.add(cm.h.ptr, hw);
-  const size_t g_com_av = g.add_root(P.av_state.tree), g_com_i = g.add_root(P.bi_state.tree);
-  std::vector<size_t> g_froot(nf), g_fproof(nf), g_fval(nf);
+  const uint64_t g_res = g.add_range((const uint64_t*)P.res, 2 * res_n);
+  const uint64_t g_hopen = g.add_nodes(cm.tree, hnodes);
+  const uint64_t g_hcols = g.add(cm.h.ptr, sorted, 1, 1, 32, two_n);  // H[r][j], r inner
+  const uint64_t g_com_av = g.add_root(P.av_state.tree), g_com_i = g.add_root(P.bi_state.tree);
+  std::vector<uint64_t> g_froot(nf), g_fproof(nf), g_fval(nf);
   for (uint32_t k = 1; k < R; k++) g_froot[k - 1] = g.add_root(P.ftrees[k - 1]);
-  const size_t g_final = g.add((const uint64_t*)P.levels, fw);
+  const uint64_t g_final = g.add_range(lv + 2 * lvl_off[R], 2 * (D >> R));
   for (uint32_t k = 1; k < R; k++) {
     g_fproof[k - 1] = g.add_nodes(P.ftrees[k - 1], fnodes[k - 1]);
-    g_fval[k - 1] = g.add((const uint64_t*)P.levels, fvw[k - 1]);
+    g_fval[k - 1] = g.add(lv + 2 * lvl_off[k], fpos[k - 1], 2, 2);
   }
-  size_t g_bproof[3], g_bval[3];
+  uint64_t g_bproof[3], g_bval[3];
   for (int c = 0; c < 3; c++) {
     g_bproof[c] = g.add_nodes(states[c]->tree, bnodes);
-    g_bval[c] = g.add(states[c]->codes.ptr, bvw[c]);
+    g_bval[c] = g.add(states[c]->codes.ptr, bpos, 1, 1, states[c]->code_rows(), D);  // rows inner
   }
 
-  // stage: [results (2 * res_n words)] [gathered words], gathered by one launch over a table of
-  // (source, first word) parts followed by the word offsets
-  const size_t stage_words = 2 * res_n + g.offs.size();
+  // one upload: the part table, then the positions
+  const size_t stage_words = g.total;
+  const size_t tab_words = g.parts.size() * (sizeof(GatherPart) / 8), pos_words = (g.pos.size() + 1) / 2;
+  g.pos.resize(2 * pos_words);
+  std::vector<uint64_t> gtab(tab_words + pos_words);
+  std::memcpy(gtab.data(), g.parts.data(), tab_words * 8);
+  std::memcpy(gtab.data() + tab_words, g.pos.data(), pos_words * 8);
   DevBuf<uint64_t> dstage(pool, stage_words);
-  std::vector<uint64_t> gtab;
-  gtab.reserve(2 * g.parts.size() + g.offs.size());
-  for (auto& p : g.parts) {
-    gtab.push_back((uint64_t)(uintptr_t)p.src);
-    gtab.push_back(p.first);
-  }
-  gtab.insert(gtab.end(), g.offs.begin(), g.offs.end());
   DevBuf<uint64_t> dgtab(pool, gtab.size());
   ctx.upload(dgtab.ptr, gtab.data(), gtab.size() * 8, st);
-  hip_check(hipMemcpyAsync(dstage.ptr, P.res, res_n * sizeof(ef2), hipMemcpyDeviceToDevice, st), "results");
   hip_check(launch_gather_parts((const GatherPart*)dgtab.ptr, (uint32_t)g.parts.size(),
-                                dgtab.ptr + 2 * g.parts.size(), g.offs.size(), dstage.ptr + 2 * res_n, st),
+                                (const uint32_t*)(dgtab.ptr + tab_words), stage_words, dstage.ptr, st),
             "gather");
   mark(4, st);
   uint64_t* host = (uint64_t*)ctx.staging(stage_words * 8);
@@ -807,12 +816,12 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
       ctx.stage_ms[6 + i] = ms;
     }
   }
-  const uint64_t* RS = host;                 // results as words
-  const uint64_t* GW = host + 2 * res_n;     // gathered words
+  const uint64_t* RS = host + g_res;  // results as words
+  const uint64_t* GW = host;          // gathered words
 
   // ---- encode (field order of LigesisProof, ligesis.rs:99-119) ----
-  Writer w(proof);
-  if (proof.size() < 8 * (stage_words + 4096)) proof.resize(8 * (stage_words + 4096));
+  if (out.vec && out.vec->size() < 8 * (stage_words + 4096)) out.vec->resize(8 * (stage_words + 4096));
+  Writer w(out);
   w.ef(RS + 2 * o_y);
   w.raw(GW + g_com_av, 4);
   w.raw(GW + g_com_i, 4);
@@ -873,7 +882,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   }
   w.u64(1);  // code_rate
   w.u64(ilog2(D));
-  proof.resize(w.n);
+  if (out.vec) out.vec->resize(w.n);
   return (int64_t)w.n;
 }
 

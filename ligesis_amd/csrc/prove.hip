@@ -323,10 +323,10 @@ __global__ void bits_gather_kernel(const uint64_t* __restrict__ enc, uint64_t ld
 // g2[i] = sum_k eq_r2[k] * bi[i * s + k] (ligesis.rs:445-455): one wave per i, its lanes
 // reading consecutive words of the row (coalesced), then a butterfly sum across the wave.
 
-__global__ __launch_bounds__(256) void g2_kernel(const uint64_t* __restrict__ bi, uint32_t s, uint64_t W,
-                                                 const ef2* __restrict__ eq_r2, ef2* __restrict__ g2,
-                                                 ef2* __restrict__ g2b) {
-  const uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+__device__ __forceinline__ void g2_body(const uint64_t* __restrict__ bi, uint32_t s, uint64_t W,
+                                        const ef2* __restrict__ eq_r2, ef2* __restrict__ g2, ef2* __restrict__ g2b,
+                                        uint32_t blk) {
+  const uint64_t i = ((uint64_t)blk * blockDim.x + threadIdx.x) >> 6;
   const uint32_t lane = threadIdx.x & 63;
   if (i >= W) return;  // whole waves leave together
   ef2 acc = ef_zero();
@@ -340,31 +340,20 @@ __global__ __launch_bounds__(256) void g2_kernel(const uint64_t* __restrict__ bi
 }
 
 // v[i] = eqz1[i / 64] * 2^(i mod 64) (ligesis.rs:474-476)
-__global__ void vvec_kernel(const ef2* __restrict__ eqz1, uint64_t W, ef2* __restrict__ v) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void vvec_body(const ef2* __restrict__ eqz1, uint64_t W, ef2* __restrict__ v,
+                                          uint32_t blk) {
+  const uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x;
   if (i >= W) return;
   const uint32_t k = i & 63;
-  const uint64_t p2 = k < 64 ? gl_canon(uint64_t(1) << k) : 0;
-  v[i] = ef_mul_base(eqz1[i >> 6], p2);
-}
-
-// sel[idx[k]] = eq_r2[k] over a zeroed vector (ligesis.rs:488-491)
-__global__ void sel_kernel(const uint32_t* __restrict__ idx, const ef2* __restrict__ eq_r2, uint32_t s,
-                           ef2* __restrict__ sel) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < s) sel[idx[k]] = eq_r2[k];
-}
-
-__global__ void zero_kernel(uint64_t* __restrict__ p, uint64_t n) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    p[i] = 0;
+  v[i] = ef_mul_base(eqz1[i >> 6], gl_canon(uint64_t(1) << k));
 }
 
 // w[k] = prod_t (1 - r3[t] + r3[t] * g^((2^t k) mod 2n)) (rs_weights, ligesis.rs:553-567);
 // tw[e] = g^e for e < n, and g^(e + n) = -g^e.
-__global__ void rs_weights_kernel(const ef2* __restrict__ r3, int log2n, uint64_t n,
-                                  const uint64_t* __restrict__ tw, ef2* __restrict__ w) {
-  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void rs_weights_body(const ef2* __restrict__ r3, int log2n, uint64_t n,
+                                                const uint64_t* __restrict__ tw, ef2* __restrict__ w,
+                                                uint32_t blk) {
+  const uint64_t k = (uint64_t)blk * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const uint64_t mask = 2 * n - 1;
   ef2 prod = ef_one();
@@ -378,25 +367,46 @@ __global__ void rs_weights_kernel(const ef2* __restrict__ r3, int log2n, uint64_
   w[k] = prod;
 }
 
-// Planar (c0 row, c1 row) -> interleaved extension vector.
-__global__ void planar_to_ef_kernel(const uint64_t* __restrict__ c0, const uint64_t* __restrict__ c1,
-                                    uint64_t n, ef2* __restrict__ out) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = ef2{c0[i], c1[i]};
+// The elementwise inputs of the Stage A sumchecks in one launch, block ranges in order: g2 (and
+// its copy), the zeroed sel vector, v, the RS weights, RS(a-bar) interleaved from its planes.
+__global__ __launch_bounds__(256) void prep_group_kernel(PrepJobs a) {
+  uint32_t b = blockIdx.x;
+  if (b < a.b_g2) return g2_body(a.bi, a.s, a.W, a.eq_r2, a.g2, a.g2b, b);
+  b -= a.b_g2;
+  if (b < a.b_zero) {
+    for (uint64_t i = (uint64_t)b * blockDim.x + threadIdx.x; i < a.sel_n; i += (uint64_t)a.b_zero * blockDim.x)
+      a.sel[i] = ef_zero();
+    return;
+  }
+  b -= a.b_zero;
+  if (b < a.b_vv) return vvec_body(a.eqz1, a.W, a.vv, b);
+  b -= a.b_vv;
+  if (b < a.b_rs) return rs_weights_body(a.r3, a.log2n, a.n, a.tw, a.wv, b);
+  b -= a.b_rs;
+  const uint64_t i = (uint64_t)b * blockDim.x + threadIdx.x;
+  if (i < a.pn) a.ra[i] = ef2{a.c0[i], a.c1[i]};
+}
+
+// sel[idx[k]] = eq_r2[k] (ligesis.rs:488-491) and y4 = sum_k eq_r2[k] ra[idx[k]] (ligesis.rs:456),
+// the only nonzero terms of <sel, ra>; one workgroup.
+__global__ __launch_bounds__(256) void sel_y4_kernel(const uint32_t* __restrict__ idx, const ef2* __restrict__ eq_r2,
+                                                     uint32_t s, const ef2* __restrict__ ra, ef2* __restrict__ sel,
+                                                     ef2* __restrict__ y4) {
+  __shared__ ef2 sh[4];
+  ef2 acc[1] = {ef_zero()};
+  for (uint32_t k = threadIdx.x; k < s; k += blockDim.x) {
+    sel[idx[k]] = eq_r2[k];
+    acc[0] = ef_add(acc[0], ef_mul(eq_r2[k], ra[idx[k]]));
+  }
+  block_sum<1>(acc, sh);
+  if (threadIdx.x == 0) *y4 = acc[0];
 }
 
 // ---------------- product sumcheck round (sumcheck.rs:37-84) ----------------
-// Round over pairs (lo = p[i], hi = p[i + half]): g(t) = sum_i prod_j ((1 - t) lo_j + t hi_j) for
-// t = 0..D, then p[i] <- lo + r (hi - lo) in place (the high variable is bound first).
-// R consecutive rounds of one product sumcheck in one pass (R <= 3). With the table length
-// L = 2 half and stride = L / 2^R, thread i owns the entries i + g stride (g < 2^R) of every
-// table: each round pairs g with g + 2^(R-1-k) among the entries it still holds, adds its
-// products at t = 0..D to that round's sums, and folds with the round's challenge; after R
-// rounds entry i of the folded table (length L / 2^R) is written back in place. No entry is
-// shared between threads, so the rounds need no synchronisation.
-// One round of one product sumcheck over its pairs (lo, hi) = (p[i], p[i + half]): the
-// products at t = 0..D go to the block's partial sums, and every table folds in place with the
-// round's challenge, p[i] = lo + r (hi - lo).
+// One round of one product sumcheck over its pairs (lo, hi) = (p[i], p[i + half]):
+// g(t) = sum_i prod_j ((1 - t) lo_j + t hi_j) for t = 0..D goes to the block's partial sums, and
+// every table folds in place with the round's challenge, p[i] = lo + r (hi - lo) (the high
+// variable is bound first).
 // M1: the second table is the first minus one and is not stored (its pair and fold follow from
 // the first table's).
 template <int D, bool M1>
@@ -773,7 +783,7 @@ hipError_t launch_bf_round3(const BfRound& a, uint64_t half, const ef2* r, hipSt
 
 // ---------------- gathers for openings ----------------
 __global__ void gather_parts_kernel(const GatherPart* __restrict__ parts, uint32_t nparts,
-                                    const uint64_t* __restrict__ off, uint64_t total, uint64_t* __restrict__ dst) {
+                                    const uint32_t* __restrict__ pos, uint64_t total, uint64_t* __restrict__ dst) {
   for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t lo = 0, hi = nparts - 1;
     while (lo < hi) {
@@ -781,7 +791,10 @@ __global__ void gather_parts_kernel(const GatherPart* __restrict__ parts, uint32
       if (parts[mid].first <= w) lo = mid;
       else hi = mid - 1;
     }
-    dst[w] = parts[lo].src[off[w]];
+    const GatherPart& g = parts[lo];
+    const uint64_t t = w - g.first, run = (uint64_t)g.rows * g.width;
+    const uint64_t k = t / run, rem = t - k * run, rr = rem / g.width, q = rem - rr * g.width;
+    dst[w] = g.src[(uint64_t)pos[g.pos_at + k] * g.pmul + rr * g.rstride + q];
   }
 }
 
@@ -790,28 +803,6 @@ __global__ void gather_parts_kernel(const GatherPart* __restrict__ parts, uint32
 hipError_t launch_bits_gather(const uint64_t* enc, uint64_t ld, const uint32_t* idx, uint32_t s, uint64_t W,
                               uint64_t* bi, uint8_t* bi8, hipStream_t st) {
   hipLaunchKernelGGL(bits_gather_kernel, LGS_GRID(W * s), 0, st, enc, ld, idx, s, W, bi, bi8);
-  return hipGetLastError();
-}
-hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st,
-                     ef2* g2b) {
-  hipLaunchKernelGGL(g2_kernel, LGS_GRID(W * 64), 0, st, bi, s, W, eq_r2, g2, g2b);
-  return hipGetLastError();
-}
-hipError_t launch_vvec(const ef2* eqz1, uint64_t W, ef2* v, hipStream_t st) {
-  hipLaunchKernelGGL(vvec_kernel, LGS_GRID(W), 0, st, eqz1, W, v);
-  return hipGetLastError();
-}
-hipError_t launch_sel(const uint32_t* idx, const ef2* eq_r2, uint32_t s, ef2* sel, uint64_t len, hipStream_t st) {
-  hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, (uint64_t*)sel, 2 * len);
-  hipLaunchKernelGGL(sel_kernel, LGS_GRID(s), 0, st, idx, eq_r2, s, sel);
-  return hipGetLastError();
-}
-hipError_t launch_rs_weights(const ef2* r3, int log2n, uint64_t n, const uint64_t* tw, ef2* w, hipStream_t st) {
-  hipLaunchKernelGGL(rs_weights_kernel, LGS_GRID(n), 0, st, r3, log2n, n, tw, w);
-  return hipGetLastError();
-}
-hipError_t launch_planar_to_ef(const uint64_t* c0, const uint64_t* c1, uint64_t n, ef2* out, hipStream_t st) {
-  hipLaunchKernelGGL(planar_to_ef_kernel, LGS_GRID(n), 0, st, c0, c1, n, out);
   return hipGetLastError();
 }
 hipError_t launch_bf_combine(const BfChannels& ch, uint64_t D, ef2* out, hipStream_t st) {
@@ -824,16 +815,26 @@ hipError_t launch_fold_low(const uint64_t* poly, int ext, int li, uint64_t nout,
                      eq4, m);
   return hipGetLastError();
 }
-hipError_t launch_gather_parts(const GatherPart* parts, uint32_t nparts, const uint64_t* off, uint64_t total,
+hipError_t launch_prep_group(PrepJobs a, hipStream_t st) {
+  a.b_g2 = (uint32_t)((a.W * 64 + 255) / 256);
+  a.b_zero = 256;
+  a.b_vv = (uint32_t)((a.W + 255) / 256);
+  a.b_rs = (uint32_t)((a.n + 255) / 256);
+  const uint32_t b_pl = (uint32_t)((a.pn + 255) / 256);
+  hipLaunchKernelGGL(prep_group_kernel, dim3(a.b_g2 + a.b_zero + a.b_vv + a.b_rs + b_pl), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+hipError_t launch_sel_y4(const uint32_t* idx, const ef2* eq_r2, uint32_t s, const ef2* ra, ef2* sel, ef2* y4,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(sel_y4_kernel, dim3(1), dim3(256), 0, st, idx, eq_r2, s, ra, sel, y4);
+  return hipGetLastError();
+}
+hipError_t launch_gather_parts(const GatherPart* parts, uint32_t nparts, const uint32_t* pos, uint64_t total,
                                uint64_t* dst, hipStream_t st) {
   if (total == 0 || nparts == 0) return hipSuccess;
   uint64_t blocks = (total + 255) / 256;
   if (blocks > 1024) blocks = 1024;
-  hipLaunchKernelGGL(gather_parts_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, parts, nparts, off, total, dst);
-  return hipGetLastError();
-}
-hipError_t launch_zero(uint64_t* p, uint64_t n, hipStream_t st) {
-  hipLaunchKernelGGL(zero_kernel, dim3(256), dim3(256), 0, st, p, n);
+  hipLaunchKernelGGL(gather_parts_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, parts, nparts, pos, total, dst);
   return hipGetLastError();
 }
 

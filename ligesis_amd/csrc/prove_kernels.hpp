@@ -129,23 +129,50 @@ hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part, hipStre
 // encoded rows.
 hipError_t launch_bits_gather(const uint64_t* enc, uint64_t ld, const uint32_t* idx, uint32_t s, uint64_t W,
                               uint64_t* bi, uint8_t* bi8, hipStream_t st);
-hipError_t launch_g2(const uint64_t* bi, uint32_t s, uint64_t W, const ef2* eq_r2, ef2* g2, hipStream_t st,
-                     ef2* g2b = nullptr);
-hipError_t launch_vvec(const ef2* eqz1, uint64_t W, ef2* v, hipStream_t st);
-hipError_t launch_sel(const uint32_t* idx, const ef2* eq_r2, uint32_t s, ef2* sel, uint64_t len, hipStream_t st);
-hipError_t launch_rs_weights(const ef2* r3, int log2n, uint64_t n, const uint64_t* tw, ef2* w, hipStream_t st);
-hipError_t launch_planar_to_ef(const uint64_t* c0, const uint64_t* c1, uint64_t n, ef2* out, hipStream_t st);
+// Elementwise inputs of the Stage A sumchecks in one launch (block ranges filled in by the
+// launcher): g2[i] = sum_k eq_r2[k] bi[i s + k] (and its copy g2b), sel zeroed (sel_n entries),
+// v[i] = eqz1[i / 64] 2^(i mod 64), the RS weights w (n entries, twiddles of order 2n), and
+// RS(a-bar) interleaved from planes c0, c1 (pn entries; pn = 0 skips).
+struct PrepJobs {
+  const uint64_t* bi;
+  uint32_t s;
+  uint64_t W;
+  const ef2* eq_r2;
+  ef2* g2;
+  ef2* g2b;
+  ef2* sel;
+  uint64_t sel_n;
+  const ef2* eqz1;
+  ef2* vv;
+  const ef2* r3;
+  int log2n;
+  uint64_t n;
+  const uint64_t* tw;
+  ef2* wv;
+  const uint64_t* c0;
+  const uint64_t* c1;
+  uint64_t pn;
+  ef2* ra;
+  uint32_t b_g2, b_zero, b_vv, b_rs;
+};
+hipError_t launch_prep_group(PrepJobs a, hipStream_t st);
+// sel[idx[k]] = eq_r2[k] into the zeroed sel, and y4 = sum_k eq_r2[k] ra[idx[k]].
+hipError_t launch_sel_y4(const uint32_t* idx, const ef2* eq_r2, uint32_t s, const ef2* ra, ef2* sel, ef2* y4,
+                         hipStream_t st);
 uint32_t sc_round_blocks(uint64_t half);
 hipError_t launch_bf_combine(const BfChannels& ch, uint64_t D, ef2* out, hipStream_t st);
 hipError_t launch_fold_low(const uint64_t* poly, int ext, int li, uint64_t nout, const ef2* eq4, ef2* m,
                            hipStream_t st);
-// dst[w] = parts[p].src[off[w]] for the part p whose [first, next first) range holds w.
+// One part of a staged gather: for each listed position k (in order), `rows` runs of `width`
+// consecutive words, run rr starting at word pos[k] * pmul + rr * rstride of src. Part words
+// land at dst[first ...]; parts are sorted by first and tile [0, total).
 struct GatherPart {
   const uint64_t* src;
   uint64_t first;
+  uint64_t pmul, rstride;
+  uint32_t pos_at, rows, width, pad;
 };
-hipError_t launch_gather_parts(const GatherPart* parts, uint32_t nparts, const uint64_t* off, uint64_t total,
+hipError_t launch_gather_parts(const GatherPart* parts, uint32_t nparts, const uint32_t* pos, uint64_t total,
                                uint64_t* dst, hipStream_t st);
-hipError_t launch_zero(uint64_t* p, uint64_t n, hipStream_t st);
 
 }  // namespace lgs

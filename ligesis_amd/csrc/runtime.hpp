@@ -232,7 +232,6 @@ struct Context {
   void upload_reset() { arena.reset(); }
   void upload(void* dst, const void* src, size_t bytes, hipStream_t s) { arena.upload(dst, src, bytes, s); }
   void free_host();  // pinned staging and arena
-  std::vector<uint8_t> proof_buf;  // encoded proof of the last lgs_prove
 
   const uint64_t* twiddle(int logN);
   uint8_t* staging(size_t bytes);
@@ -315,9 +314,17 @@ void setup_key(Context& ctx, Key& key, const uint64_t* host_a, bool commit_a);
 // fastest in the process table; returns the best commit time in ms
 double autotune(Context& ctx, const Key& key, uint32_t reps, TuneEntry& best);
 // prove.cpp
+// Where a proof's bytes go: a vector (resized to the proof), or a caller's buffer of cap bytes,
+// written only when the whole proof fits (the returned size says whether it did).
+struct ProofOut {
+  std::vector<uint8_t>* vec = nullptr;
+  uint8_t* dst = nullptr;
+  uint64_t cap = 0;
+};
+// returns the proof's size in bytes
 int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uint64_t* z, uint64_t z_len,
                    const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,
-                   std::vector<uint8_t>& proof, uint64_t used[2]);
+                   ProofOut out, uint64_t used[2]);
 
 // server.cpp
 struct Server;

@@ -256,7 +256,7 @@ void Server::prove_loop(Context* ctx) {
     try {
       uint64_t used[2];
       prove_impl(*ctx, *r->key, *r->cm, r->z.data(), r->z.size() / 2, r->fields.data(), r->fields.size() / 2,
-                 r->ints.data(), r->ints.size(), proof, used);
+                 r->ints.data(), r->ints.size(), ProofOut{&proof}, used);
     } catch (const std::exception& e) {
       const Error* le = dynamic_cast<const Error*>(&e);
       rc = le ? le->code : (dynamic_cast<const std::bad_alloc*>(&e) ? 4 : 1);
