@@ -63,7 +63,10 @@ const char* lgs_last_error(const lgs_ctx* ctx);
 /* Tuning knobs that never change results: "hash" (0 auto, 1 MFMA, 2 VALU), "hash_ksplit"
  * (0 auto), "hash_impl" (-1 auto; MFMA kernel: 0 pipelined, 1 pipelined one slab per stage,
  * 2 basic, 3 pipelined with 256 x 256 workgroup tiles, 4 global loads issued 2 slabs ahead, 5 as
- * 4 with 256 x 256 tiles, 6 / 7 64 x 64 wave tiles in 256 x 128 / 256 x 256 workgroup tiles),
+ * 4 with 256 x 256 tiles, 6 / 7 64 x 64 wave tiles in 256 x 128 / 256 x 256 workgroup tiles,
+ * 8 / 9 64 x 128 wave tiles in 256 x 128 / 256 x 256 workgroup tiles, 10 as 0 with each A
+ * fragment read from LDS one product group ahead, 11 / 12 A digits and F' words copied straight
+ * into an LDS ring of 3 / 4 slabs),
  * "ntt_impl" (-1 auto; 0/2/3 radix-8 passes with 256/512/1024 threads per row, 1 radix-2
  * stages), "merkle_lg" (0 auto; 6..8 = log2 threads per Merkle workgroup = tree levels per
  * launch), "pipeline" (0 auto; 1 off; 2 / 4 / 8: the rows are encoded in that many chunks and

@@ -28,8 +28,7 @@ void ctx_retain(lgs::Context* c) { c->refs.fetch_add(1); }
 void ctx_unref(lgs::Context* c) {
   if (!c || c->refs.fetch_sub(1) != 1) return;
   (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
-  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  c->drain();
   c->twiddles.clear();
   c->pool.trim();
   for (auto& e : c->ev)
@@ -39,8 +38,7 @@ void ctx_unref(lgs::Context* c) {
   for (auto& e : c->tev)
     if (e) (void)hipEventDestroy(e);
   c->free_host();
-  (void)hipStreamDestroy(c->stream);
-  if (c->stream2) (void)hipStreamDestroy(c->stream2);
+  c->destroy_streams();
   delete (lgs_ctx*)c->owner;
 }
 
@@ -156,7 +154,7 @@ int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value) {
     ctx->c.hash_variant = (int)value;
   } else if (n == "hash_ksplit" && value >= 0) {
     ctx->c.hash_ksplit = (int)value;
-  } else if (n == "hash_impl" && value >= -1 && value <= 7) {
+  } else if (n == "hash_impl" && value >= -1 && value <= 12) {
     ctx->c.hash_impl = (int)value;
   } else if (n == "ntt_impl" && value >= -1 && value <= 3) {
     ctx->c.ntt_impl = (int)value;
@@ -193,7 +191,7 @@ int lgs_autotune(lgs_ctx* ctx, const lgs_key* key, uint32_t reps, int32_t cfg[LG
     try {
       lgs::autotune(ctx->c, key->k, reps, best);
     } catch (...) {
-      (void)hipStreamSynchronize(ctx->c.stream);
+      ctx->c.drain();
       throw;
     }
     put_tuning(best.t, cfg);
@@ -250,7 +248,7 @@ int lgs_setup(lgs_ctx* ctx, uint32_t log_m, uint32_t log_n, const uint64_t* a, u
     try {
       lgs::setup_key(ctx->c, k->k, a, !commit_only);
     } catch (...) {
-      (void)hipStreamSynchronize(ctx->c.stream);
+      ctx->c.drain();
       delete k;
       throw;
     }
@@ -288,8 +286,7 @@ int lgs_commit(lgs_ctx* ctx, const lgs_key* key, const uint64_t* f, uint64_t len
       read_root(ctx->c, &cp, 1);
       finish_profile(ctx->c);
     } catch (...) {
-      (void)hipStreamSynchronize(ctx->c.stream);
-      if (ctx->c.stream2) (void)hipStreamSynchronize(ctx->c.stream2);
+      ctx->c.drain();
       delete c;
       throw;
     }
@@ -336,8 +333,7 @@ int lgs_commit_batch(lgs_ctx* ctx, uint32_t count, const lgs_key* const* keys,
     }
   });
   if (rc != LGS_OK) {
-    (void)hipStreamSynchronize(ctx->c.stream);
-    if (ctx->c.stream2) (void)hipStreamSynchronize(ctx->c.stream2);
+    ctx->c.drain();
     for (auto* c : made) delete c;
     return rc;
   }
@@ -363,9 +359,8 @@ int lgs_prove(lgs_ctx* ctx, const lgs_key* key, const lgs_commitment* c, const u
       size = (uint64_t)lgs::prove_impl(ctx->c, key->k, c->c, z, z_len, fields, n_fields, ints, n_ints,
                                        lgs::ProofOut{nullptr, proof, cap}, u);
     } catch (...) {
-      // no buffer released by the failed prove may be reused before both streams drain
-      (void)hipStreamSynchronize(ctx->c.stream);
-      if (ctx->c.stream2) (void)hipStreamSynchronize(ctx->c.stream2);
+      // no buffer released by the failed prove may be reused before every stream drains
+      ctx->c.drain();
       throw;
     }
     if (used) {

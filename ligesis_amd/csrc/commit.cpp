@@ -27,6 +27,24 @@ hipStream_t Context::side_stream() {
   return stream2;
 }
 
+hipStream_t Context::third_stream() {
+  if (!stream3) hip_check(hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking), "third stream");
+  return stream3;
+}
+
+void Context::drain() {
+  for (hipStream_t s : {stream, stream2, stream3})
+    if (s) (void)hipStreamSynchronize(s);
+}
+
+void Context::destroy_streams() {
+  for (hipStream_t* s : {&stream, &stream2, &stream3})
+    if (*s) {
+      (void)hipStreamDestroy(*s);
+      *s = nullptr;
+    }
+}
+
 hipEvent_t Context::pipe_event(int i) {
   if (!pipe_ev[i]) hip_check(hipEventCreateWithFlags(&pipe_ev[i], hipEventDisableTiming), "pipeline event");
   return pipe_ev[i];

@@ -155,13 +155,13 @@ __device__ __forceinline__ uint32_t xcd_logical_block() {
 // and at most 256 registers per lane so two workgroups share a CU.
 // WN waves along N: a workgroup has 2 x WN waves (128 WN threads) and a tile of 256 rows x
 // 64 WN columns; every wave owns 128 rows x 64 columns.
-template <int SPS, int WN>
+template <int SPS, int WN, bool PF = false>
 __device__ __forceinline__ void hash_pipe_body(const uint64_t* __restrict__ enc, uint64_t ld,
                                                const v4i* __restrict__ adig, uint32_t slabs_per_split,
                                                uint32_t split, uint32_t ntile, uint32_t ncols,
                                                int32_t* __restrict__ part, v4i (*lds_a)[SPS * 1024]);
 
-template <int SPS, int WN>
+template <int SPS, int WN, bool PF = false>
 __global__ __launch_bounds__(128 * WN, 2) void hash_mfma_pipe_kernel(const uint64_t* __restrict__ enc,
                                                                      uint64_t ld, const v4i* __restrict__ adig,
                                                                      uint32_t slabs_per_split, uint32_t ntiles,
@@ -169,28 +169,11 @@ __global__ __launch_bounds__(128 * WN, 2) void hash_mfma_pipe_kernel(const uint6
                                                                      uint32_t split0) {
   __shared__ v4i lds_a[2][SPS * 1024];
   const uint32_t logical = xcd_logical_block();
-  hash_pipe_body<SPS, WN>(enc, ld, adig, slabs_per_split, split0 + logical / ntiles, logical % ntiles, ncols, part,
-                          lds_a);
+  hash_pipe_body<SPS, WN, PF>(enc, ld, adig, slabs_per_split, split0 + logical / ntiles, logical % ntiles, ncols,
+                              part, lds_a);
 }
 
-// Tiles of many commitments in one launch (batched commit): logical tile -> job by binary
-// search over the jobs' first tiles.
-__global__ __launch_bounds__(256, 2) void hash_mfma_group_kernel(const HashJob* __restrict__ jobs, int njobs) {
-  __shared__ v4i lds_a[2][2 * 1024];
-  const uint32_t logical = xcd_logical_block();
-  int lo = 0, hi = njobs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) / 2;
-    if (jobs[mid].first_block <= logical) lo = mid;
-    else hi = mid - 1;
-  }
-  const HashJob& jb = jobs[lo];
-  const uint32_t local = logical - jb.first_block;
-  hash_pipe_body<2, 2>(jb.enc, jb.ld, (const v4i*)jb.adig, jb.slabs_per_split, local / jb.ntiles,
-                    local % jb.ntiles, jb.ncols, jb.part, lds_a);
-}
-
-template <int SPS, int WN>
+template <int SPS, int WN, bool PF>
 __device__ __forceinline__ void hash_pipe_body(const uint64_t* __restrict__ enc, uint64_t ld,
                                                const v4i* __restrict__ adig, uint32_t slabs_per_split,
                                                uint32_t split, uint32_t ntile, uint32_t ncols,
@@ -252,12 +235,36 @@ __device__ __forceinline__ void hash_pipe_body(const uint64_t* __restrict__ enc,
         b[nt] = v4i{(int)(y & 0x01010101u), (int)((y >> 1) & 0x01010101u),
                     (int)((y >> 2) & 0x01010101u), (int)((y >> 3) & 0x01010101u)};
       }
+      if (PF) {
+        // A fragment mt + 1 is read while the four products of fragment mt issue, so the LDS
+        // latency hides behind MFMA work; the schedule is pinned (one LDS read, then four MFMAs)
+        v4i a = lds_a[buf][ss * 1024 + (wm * 8) * 64 + lane];
 #pragma unroll
-      for (int mt = 0; mt < 8; mt++) {
-        const v4i a = lds_a[buf][ss * 1024 + (wm * 8 + mt) * 64 + lane];
+        for (int mt = 0; mt < 8; mt++) {
+          v4i an = a;
+          if (mt < 7) an = lds_a[buf][ss * 1024 + (wm * 8 + mt + 1) * 64 + lane];
 #pragma unroll
-        for (int nt = 0; nt < 4; nt++)
-          acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[nt], acc[mt][nt], 0, 0, 0);
+          for (int nt = 0; nt < 4; nt++)
+            acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[nt], acc[mt][nt], 0, 0, 0);
+          a = an;
+        }
+        // groups in issue order: reads of fragments 0 and 1, then (4 MFMAs, the next read) x 6,
+        // then the last 8 MFMAs
+        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+#pragma unroll
+        for (int mt = 0; mt < 6; mt++) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+      } else {
+#pragma unroll
+        for (int mt = 0; mt < 8; mt++) {
+          const v4i a = lds_a[buf][ss * 1024 + (wm * 8 + mt) * 64 + lane];
+#pragma unroll
+          for (int nt = 0; nt < 4; nt++)
+            acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[nt], acc[mt][nt], 0, 0, 0);
+        }
       }
     }
     if (more) {
@@ -395,7 +402,6 @@ __global__ __launch_bounds__(64 * (16 / MT) * WN, MINB) void hash_mfma_tile_kern
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t wm = wave / WN, wn = wave % WN;
   const uint32_t g = lane >> 4, c = lane & 15;
-  const uint32_t kb = 32 * (g >> 1) + 4 * (g & 1);
   const uint64_t col_base = (uint64_t)ntile * (WN * NT * 16) + wn * (NT * 16) + c;
 
   v4i acc[MT][NT];
@@ -406,18 +412,22 @@ __global__ __launch_bounds__(64 * (16 / MT) * WN, MINB) void hash_mfma_tile_kern
 
   const uint32_t ns = slabs_per_split;
   const uint64_t slab0 = (uint64_t)split * ns;
+  // a lane uses bits kb .. kb + 27 of its F' word, all inside 32-bit half g >> 1: it loads that
+  // half only, and y = half >> (kb mod 32)
+  const uint32_t* enc32 = (const uint32_t*)enc + (g >> 1);
+  const uint32_t ksh = 4 * (g & 1);
   v4i pa[NA];
-  uint64_t px[NT];
+  uint32_t px[NT];
 #pragma unroll
   for (int q = 0; q < NA; q++) pa[q] = adig[slab0 * 1024 + q * T + threadIdx.x];
 #pragma unroll
-  for (int nt = 0; nt < NT; nt++) px[nt] = enc[slab0 * ld + col_base + nt * 16];
+  for (int nt = 0; nt < NT; nt++) px[nt] = enc32[2 * (slab0 * ld + col_base + nt * 16)];
 #pragma unroll
   for (int q = 0; q < NA; q++) lds_a[0][q * T + threadIdx.x] = pa[q];
   __syncthreads();
   for (uint32_t st = 0; st < ns; st++) {
     const uint32_t buf = st & 1;
-    uint64_t cx[NT];
+    uint32_t cx[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; nt++) cx[nt] = px[nt];
     const bool more = st + 1 < ns;
@@ -426,12 +436,12 @@ __global__ __launch_bounds__(64 * (16 / MT) * WN, MINB) void hash_mfma_tile_kern
 #pragma unroll
       for (int q = 0; q < NA; q++) pa[q] = adig[s1 * 1024 + q * T + threadIdx.x];
 #pragma unroll
-      for (int nt = 0; nt < NT; nt++) px[nt] = enc[s1 * ld + col_base + nt * 16];
+      for (int nt = 0; nt < NT; nt++) px[nt] = enc32[2 * (s1 * ld + col_base + nt * 16)];
     }
     v4i b[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; nt++) {
-      const uint32_t y = (uint32_t)(cx[nt] >> kb);
+      const uint32_t y = cx[nt] >> ksh;
       b[nt] = v4i{(int)(y & 0x01010101u), (int)((y >> 1) & 0x01010101u), (int)((y >> 2) & 0x01010101u),
                   (int)((y >> 3) & 0x01010101u)};
     }
@@ -460,6 +470,176 @@ __global__ __launch_bounds__(64 * (16 / MT) * WN, MINB) void hash_mfma_tile_kern
       }
     }
   }
+}
+
+// Asynchronous-copy variant of the 256 x 128 tile (2 x 2 waves of 128 x 64): each slab's A digits
+// (16 KiB) and F' words (128 columns, 1 KiB) go from global memory straight into an LDS ring of NB
+// slots (global_load_lds, no staging registers), NB - 1 slabs ahead of the slab being multiplied.
+// Every wave issues the same five copies per slab (four of A, one of F'), so "this wave's copies
+// of slab s are done" is vmcnt <= 5 (NB - 2) once the copies of the later slabs are issued; a
+// barrier then makes every wave's copies visible. The copy into a slot is issued after the
+// barrier that follows the last reads of its previous slab. Near the end the copies repeat the
+// last slab (unused), so the count stays uniform.
+#define LGS_AS1 __attribute__((address_space(1)))
+#define LGS_AS3 __attribute__((address_space(3)))
+constexpr int vmcnt_imm(int n) { return (n & 0xF) | (((n >> 4) & 3) << 14) | (7 << 4) | (0xF << 8); }
+
+// One slot of the ring: separate LDS arrays per slot, so the compiler can tell a slot's reads from
+// the copies in flight into the other slots (a shared array with a dynamic slot index makes it
+// wait for every copy before each read).
+struct DmaSlot {
+  v4i* a;
+  uint32_t* b;
+};
+
+struct DmaTile {
+  const uint64_t* enc;
+  uint64_t ld;
+  const v4i* adig;
+  uint64_t slab0, col0;
+  uint32_t ns, tid, lane, wave, wm, wn, g, c, ksh;
+};
+
+__device__ __forceinline__ void dma_copy_slab(const DmaTile& t, uint32_t sl, DmaSlot d) {
+  const v4i* src = t.adig + (t.slab0 + sl) * 1024;
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    __builtin_amdgcn_global_load_lds((const LGS_AS1 void*)(src + q * 256 + t.tid),
+                                     (LGS_AS3 void*)(d.a + q * 256 + t.wave * 64), 16, 0, 0);
+  const uint32_t* bs = (const uint32_t*)(t.enc + (t.slab0 + sl) * t.ld + t.col0) + t.tid;
+  __builtin_amdgcn_global_load_lds((const LGS_AS1 void*)bs, (LGS_AS3 void*)(d.b + t.wave * 64), 4, 0, 0);
+}
+
+// Slab sl from slot cur; the copy of slab sl + NB - 1 goes to slot nxt.
+template <int NB>
+__device__ __forceinline__ void dma_step(const DmaTile& t, uint32_t sl, DmaSlot cur, DmaSlot nxt, v4i (&acc)[8][4]) {
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(5 * (NB - 2)));
+  asm volatile("s_barrier" ::: "memory");
+  const uint32_t nx = sl + NB - 1;
+  dma_copy_slab(t, nx < t.ns ? nx : t.ns - 1, nxt);
+  v4i b[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) {
+    const uint32_t y = cur.b[2 * (t.wn * 64 + nt * 16 + t.c) + (t.g >> 1)] >> t.ksh;
+    b[nt] = v4i{(int)(y & 0x01010101u), (int)((y >> 1) & 0x01010101u), (int)((y >> 2) & 0x01010101u),
+                (int)((y >> 3) & 0x01010101u)};
+  }
+  const v4i* A = cur.a + (t.wm * 8) * 64 + t.lane;
+  v4i a = A[0];
+#pragma unroll
+  for (int mt = 0; mt < 8; mt++) {
+    v4i an = a;
+    if (mt < 7) an = A[(mt + 1) * 64];
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+      acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[nt], acc[mt][nt], 0, 0, 0);
+    a = an;
+  }
+  // issue order: the B reads (two paired reads) and the first two A reads, then (four MFMAs,
+  // the next A read) x 6, the last eight
+  __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+#pragma unroll
+  for (int mt = 0; mt < 6; mt++) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+  }
+  __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+}
+
+// One 256 x 128 tile over slabs [split * ns, (split + 1) * ns) with a ring of NB slots (NB = 4
+// needs ns % 4 == 0).
+template <int NB>
+__device__ __forceinline__ void hash_dma_body(const uint64_t* __restrict__ enc, uint64_t ld,
+                                              const v4i* __restrict__ adig, uint32_t ns, uint32_t split,
+                                              uint32_t ntile, uint32_t ncols, int32_t* __restrict__ part,
+                                              DmaSlot s0, DmaSlot s1, DmaSlot s2, DmaSlot s3) {
+  static_assert(NB == 3 || NB == 4, "ring of 3 or 4 slabs");
+  DmaTile t;
+  t.enc = enc;
+  t.ld = ld;
+  t.adig = adig;
+  t.ns = ns;
+  t.tid = threadIdx.x;
+  t.lane = t.tid & 63;
+  t.wave = t.tid >> 6;
+  t.wm = t.wave >> 1;
+  t.wn = t.wave & 1;
+  t.g = t.lane >> 4;
+  t.c = t.lane & 15;
+  t.ksh = 4 * (t.g & 1);
+  t.col0 = (uint64_t)ntile * 128;
+  t.slab0 = (uint64_t)split * ns;
+
+  v4i acc[8][4];
+#pragma unroll
+  for (int a = 0; a < 8; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = v4i{0, 0, 0, 0};
+
+  dma_copy_slab(t, 0, s0);
+  dma_copy_slab(t, 1 < ns ? 1 : ns - 1, s1);
+  if (NB == 4) dma_copy_slab(t, 2 < ns ? 2 : ns - 1, s2);
+  for (uint32_t sl = 0; sl < ns; sl += NB) {
+    if (NB == 3) {
+      dma_step<3>(t, sl, s0, s2, acc);
+      if (sl + 1 < ns) dma_step<3>(t, sl + 1, s1, s0, acc);
+      if (sl + 2 < ns) dma_step<3>(t, sl + 2, s2, s1, acc);
+    } else {  // one straight-line body
+      dma_step<4>(t, sl, s0, s3, acc);
+      dma_step<4>(t, sl + 1, s1, s0, acc);
+      dma_step<4>(t, sl + 2, s2, s1, acc);
+      dma_step<4>(t, sl + 3, s3, s2, acc);
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // no copy into LDS outlives the workgroup
+#pragma unroll
+  for (int mt = 0; mt < 8; mt++) {
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const uint32_t m = (t.wm * 8 + mt) * 16 + 4 * t.g + v;
+        const uint64_t col = t.col0 + t.wn * 64 + nt * 16 + t.c;
+        part[((uint64_t)split * 256 + m) * ncols + col] = acc[mt][nt][v];
+      }
+    }
+  }
+}
+
+template <int NB>
+__global__ __launch_bounds__(256, 2) void hash_mfma_dma_kernel(const uint64_t* __restrict__ enc, uint64_t ld,
+                                                               const v4i* __restrict__ adig, uint32_t ns,
+                                                               uint32_t ntiles, uint32_t ncols,
+                                                               int32_t* __restrict__ part, uint32_t split0) {
+  __shared__ v4i la0[1024], la1[1024], la2[1024], la3[NB == 4 ? 1024 : 1];
+  __shared__ uint32_t lb0[256], lb1[256], lb2[256], lb3[NB == 4 ? 256 : 1];
+  const uint32_t logical = xcd_logical_block();
+  hash_dma_body<NB>(enc, ld, adig, ns, split0 + logical / ntiles, logical % ntiles, ncols, part, DmaSlot{la0, lb0},
+                    DmaSlot{la1, lb1}, DmaSlot{la2, lb2}, DmaSlot{la3, lb3});
+}
+
+// Tiles of many commitments in one launch (batched commit): logical tile -> job by binary
+// search over the jobs' first tiles; a job whose split has a multiple of 4 slabs takes the 4-slot
+// ring.
+__global__ __launch_bounds__(256, 2) void hash_mfma_dma_group_kernel(const HashJob* __restrict__ jobs, int njobs) {
+  __shared__ v4i la0[1024], la1[1024], la2[1024], la3[1024];
+  __shared__ uint32_t lb0[256], lb1[256], lb2[256], lb3[256];
+  const uint32_t logical = xcd_logical_block();
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].first_block <= logical) lo = mid;
+    else hi = mid - 1;
+  }
+  const HashJob& jb = jobs[lo];
+  const uint32_t local = logical - jb.first_block;
+  const DmaSlot s0{la0, lb0}, s1{la1, lb1}, s2{la2, lb2}, s3{la3, lb3};
+  if (jb.slabs_per_split % 4 == 0)
+    hash_dma_body<4>(jb.enc, jb.ld, (const v4i*)jb.adig, jb.slabs_per_split, local / jb.ntiles, local % jb.ntiles,
+                     jb.ncols, jb.part, s0, s1, s2, s3);
+  else
+    hash_dma_body<3>(jb.enc, jb.ld, (const v4i*)jb.adig, jb.slabs_per_split, local / jb.ntiles, local % jb.ntiles,
+                     jb.ncols, jb.part, s0, s1, s2, s3);
 }
 
 // H[r][j] = sum_l 256^l (sum_split part[split][32 l + r][j]) mod p.
@@ -558,6 +738,22 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
   } else if (impl == 6 && ncols % 128 == 0) {  // 64 x 64 wave tiles, 8 waves, 256 x 128 workgroup tile
     hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 4, 2, 2>), dim3((ncols / 128) * nsplits), dim3(512), 0, st, enc, ld,
                        ad, sps, ncols / 128, ncols, part, split0);
+  } else if (impl == 11 || impl == 12) {  // asynchronous copies into an LDS ring of 3 / 4 slabs
+    if (impl == 11 || sps % 4)
+      hipLaunchKernelGGL((hash_mfma_dma_kernel<3>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps,
+                         ntiles, ncols, part, split0);
+    else
+      hipLaunchKernelGGL((hash_mfma_dma_kernel<4>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps,
+                         ntiles, ncols, part, split0);
+  } else if (impl == 10) {  // as 0, A fragments read one product group ahead
+    hipLaunchKernelGGL((hash_mfma_pipe_kernel<2, 2, true>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad,
+                       sps, ntiles, ncols, part, split0);
+  } else if (impl == 8 && ncols % 128 == 0) {  // 64 x 128 wave tiles, 4 waves, 256 x 128 workgroup tile
+    hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 8, 1, 2>), dim3((ncols / 128) * nsplits), dim3(256), 0, st, enc, ld,
+                       ad, sps, ncols / 128, ncols, part, split0);
+  } else if (impl == 9 && ncols % 256 == 0) {  // 64 x 128 wave tiles, 8 waves, 256 x 256 workgroup tile
+    hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 8, 2, 1>), dim3((ncols / 256) * nsplits), dim3(512), 0, st, enc, ld,
+                       ad, sps, ncols / 256, ncols, part, split0);
   } else if (impl == 7 && ncols % 256 == 0) {  // 64 x 64 wave tiles, 16 waves, 256 x 256 workgroup tile
     hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 4, 4, 1>), dim3((ncols / 256) * nsplits), dim3(1024), 0, st, enc,
                        ld, ad, sps, ncols / 256, ncols, part, split0);
@@ -618,7 +814,7 @@ __global__ void hash_finalize_group_kernel(const FinalizeJob* __restrict__ jobs,
 
 hipError_t launch_hash_group(const HashJob* jobs, int njobs, uint32_t total_tiles, const FinalizeJob* fjobs,
                              uint32_t total_fblocks, hipStream_t st) {
-  if (total_tiles) hipLaunchKernelGGL(hash_mfma_group_kernel, dim3(total_tiles), dim3(256), 0, st, jobs, njobs);
+  if (total_tiles) hipLaunchKernelGGL(hash_mfma_dma_group_kernel, dim3(total_tiles), dim3(256), 0, st, jobs, njobs);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (total_fblocks)

@@ -460,33 +460,27 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
             "a-bar");
   hip_check(launch_dot(a_vec, eqz2, n, part, st), "y");
 
-  // ---- the Basefold commitment of a-bar (ligesis.rs:384) on the second stream, once a-bar is
-  //      there; then Stage B ----
+  // ---- the Basefold commitment of a-bar (ligesis.rs:384) on a third stream, once a-bar is
+  //      there, so that it runs beside B_I's; Stage B on the second stream waits for it ----
   hip_check(hipEventRecord(ctx.ev[9], st), "a-bar ready");
-  hip_check(hipStreamWaitEvent(sb, ctx.ev[9], 0), "a-bar ready");
-  // RS(a-bar) (ligesis.rs:387): with log_interleave 0 a-bar's Basefold codeword is this same
-  // transform (both planes zero-padded to 2n over the order-2n subgroup), so st waits for it
-  const bool ra_shared = li_a == 0 && (uint64_t(1) << (log_n - li_a + 1)) == two_n;
-  enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec, true, log_n, li_a, 1, sb,
-                          ra_shared ? ctx.ev[7] : nullptr);
-  mark(8, sb);
+  hipStream_t s3 = ctx.third_stream();
+  hip_check(hipStreamWaitEvent(s3, ctx.ev[9], 0), "a-bar ready");
+  enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec, true, log_n, li_a, 1, s3);
+  mark(8, s3);
+  hip_check(hipEventRecord(ctx.ev[7], s3), "a-bar committed");
+  hip_check(hipStreamWaitEvent(sb, ctx.ev[7], 0), "a-bar committed");
   const DevBasefold* states[3] = {&key.a_state, &bi_state, &av_state};
   const uint64_t* polys[3] = {key.a.ptr, bi, (const uint64_t*)a_vec};
   for (int c = 0; c < 3; c++)
     if (states[c]->D != D) throw Error(1, "prove: Basefold domains differ");
 
-
-  uint64_t* ra_planar = nullptr;
-  const uint64_t* ra_src;
-  if (ra_shared) {
-    ra_src = av_state.codes.ptr;
-  } else {
-    ra_planar = P.get<uint64_t>(2 * two_n);
-    hip_check(launch_ntt((const uint64_t*)a_vec, 1, 2, (uint32_t)n, 2, logN, ctx.twiddle(logN), ra_planar,
-                         st),
-              "RS(a-bar)");
-    ra_src = ra_planar;
-  }
+  // RS(a-bar) (ligesis.rs:387), both planes zero-padded to 2n, transformed here rather than
+  // taken from a-bar's Basefold codeword (the same transform when log_interleave is 0), so that
+  // the sumchecks need not wait for the other streams
+  uint64_t* ra_planar = P.get<uint64_t>(2 * two_n);
+  hip_check(launch_ntt((const uint64_t*)a_vec, 1, 2, (uint32_t)n, 2, logN, ctx.twiddle(logN), ra_planar, st),
+            "RS(a-bar)");
+  const uint64_t* ra_src = ra_planar;
   ef2* ra = P.get<ef2>(two_n);
 
   // ---- input tables of the seven Stage A sumchecks ----
@@ -500,9 +494,8 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   // recon table and RS link (ligesis.rs:488-502); y5 = ra(r3) is sc_table's final evaluation of
   // ra, which is what the reference's verify compares (ligesis.rs:681)
   ef2* wv = P.get<ef2>(n);
-  // g2 reads B_I from the second stream; RS(a-bar) may be its codeword of a-bar
+  // g2 reads B_I from the second stream
   hip_check(hipStreamWaitEvent(st, ctx.ev[8], 0), "B_I ready");
-  if (ra_shared) hip_check(hipStreamWaitEvent(st, ctx.ev[7], 0), "a-bar codeword");
   PrepJobs pj{};
   pj.bi = bi;
   pj.s = (uint32_t)s;

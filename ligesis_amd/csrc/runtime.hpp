@@ -137,7 +137,7 @@ struct DevTree {
 // Launch schedule of one commit shape. Every choice yields the same bytes; only speed differs.
 struct Tuning {
   int hash_variant = 0;  // 1 MFMA, 2 VALU (0: MFMA where eligible)
-  int hash_impl = 0;     // MFMA kernel variant
+  int hash_impl = 12;    // MFMA kernel variant (12: asynchronous copies into a 4-slab LDS ring)
   int hash_ksplit = 0;   // slabs of the K (= 64 m) dimension; 0: heuristic
   int ntt_impl = 0;      // row transform variant
   int merkle_lg = 8;     // log2 threads per Merkle workgroup
@@ -205,6 +205,7 @@ struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // second stream (default priority) for work that overlaps `stream`
+  hipStream_t stream3 = nullptr;  // third stream (default priority), for a prove's a-bar commitment
   std::string err;
   DevicePool pool;
   std::map<int, DevBuf<uint64_t>> twiddles;
@@ -214,7 +215,9 @@ struct Context {
   int hash_ksplit = 0;   // 0 auto
   int hash_impl = -1;    // -1 auto; MFMA kernel: 0 pipelined 2x2 waves, 1 one slab per stage, 2 basic,
                          // 3 2x4 waves, 4 loads issued 2 slabs ahead, 5 as 4 with 2x4 waves,
-                         // 6 / 7 64x64 wave tiles in 8 / 16-wave workgroups
+                         // 6 / 7 64x64 wave tiles in 8 / 16-wave workgroups,
+                         // 8 / 9 64x128 wave tiles in 4 / 8-wave workgroups, 10 as 0 with A
+                         // fragments read ahead, 11 / 12 asynchronous copies into an LDS ring
   int ntt_impl = -1;     // -1 auto; row transform: 0/2/3 radix-8 with 256/512/1024 threads, 1 radix-2
   int merkle_lg = 0;     // 0 auto; log2 threads per Merkle workgroup = levels per launch (6..8)
   int pipeline = 0;      // 0 auto; 1 off, 2 / 4 / 8 row chunks (transform on stream, hash on stream2)
@@ -235,7 +238,10 @@ struct Context {
 
   const uint64_t* twiddle(int logN);
   uint8_t* staging(size_t bytes);
-  hipStream_t side_stream();  // stream2, created on first use
+  hipStream_t side_stream();   // stream2, created on first use
+  hipStream_t third_stream();  // stream3, created on first use
+  void drain();                // waits for every stream, ignoring errors (error and teardown paths)
+  void destroy_streams();
   hipEvent_t pipe_ev[10] = {};  // cross-stream events of a pipelined commit, created on first use
   hipEvent_t pipe_event(int i);
   // profiled pipelined commit: events around chunk q's transform (4q, 4q + 1) and hash (4q + 2,

@@ -163,8 +163,7 @@ void Server::finish_batch(Batch& b) {
   }
   if (b.rc) {
     (void)hipStreamSynchronize(upload_stream);
-    (void)hipStreamSynchronize(ctx.stream);
-    if (ctx.stream2) (void)hipStreamSynchronize(ctx.stream2);
+    ctx.drain();
   }
   std::lock_guard<std::mutex> lk(mu);
   batches++;
@@ -261,8 +260,7 @@ void Server::prove_loop(Context* ctx) {
       const Error* le = dynamic_cast<const Error*>(&e);
       rc = le ? le->code : (dynamic_cast<const std::bad_alloc*>(&e) ? 4 : 1);
       msg = e.what();
-      (void)hipStreamSynchronize(ctx->stream);
-      if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+      ctx->drain();
     }
     r->cm.reset();  // f, F', H and the tree go back to the commit context's pool
     std::lock_guard<std::mutex> lk(mu);
@@ -372,8 +370,7 @@ void server_stop(Server* s) {
     (void)hipStreamDestroy(s->upload_stream);
   }
   for (auto& c : s->ctxs) {
-    (void)hipStreamSynchronize(c->stream);
-    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    c->drain();
     c->twiddles.clear();
     c->pool.trim();
     for (auto& e : c->ev)
@@ -383,8 +380,7 @@ void server_stop(Server* s) {
     for (auto& e : c->tev)
       if (e) (void)hipEventDestroy(e);
     c->free_host();
-    (void)hipStreamDestroy(c->stream);
-    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    c->destroy_streams();
   }
   delete s;
 }

@@ -87,7 +87,7 @@ def test_every_mfma_kernel_matches_oracle(gpu_ctx, shape):
     want = ocd.h()
     gpu_ctx.set_option("hash", 1)
     try:
-        for impl in range(8):
+        for impl in range(13):
             for ks in (0, 1, 2, 4):
                 if ks > (1 << log_m) // 2:
                     continue
