@@ -524,26 +524,36 @@ __device__ __forceinline__ void dma_step(const DmaTile& t, uint32_t sl, DmaSlot 
     b[nt] = v4i{(int)(y & 0x01010101u), (int)((y >> 1) & 0x01010101u), (int)((y >> 2) & 0x01010101u),
                 (int)((y >> 3) & 0x01010101u)};
   }
+  // A fragments in pairs: the pair after next is read while the current pair's eight products
+  // issue (with copies into LDS in flight the compiler's LDS waits are all-or-nothing, so a read
+  // is waited for one product group after it is issued)
   const v4i* A = cur.a + (t.wm * 8) * 64 + t.lane;
-  v4i a = A[0];
+  v4i a0 = A[0], a1 = A[64];
 #pragma unroll
-  for (int mt = 0; mt < 8; mt++) {
-    v4i an = a;
-    if (mt < 7) an = A[(mt + 1) * 64];
+  for (int k = 0; k < 4; k++) {
+    v4i n0 = a0, n1 = a1;
+    if (k < 3) {
+      n0 = A[(2 * k + 2) * 64];
+      n1 = A[(2 * k + 3) * 64];
+    }
 #pragma unroll
     for (int nt = 0; nt < 4; nt++)
-      acc[mt][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[nt], acc[mt][nt], 0, 0, 0);
-    a = an;
-  }
-  // issue order: the B reads (two paired reads) and the first two A reads, then (four MFMAs,
-  // the next A read) x 6, the last eight
-  __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+      acc[2 * k][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, b[nt], acc[2 * k][nt], 0, 0, 0);
 #pragma unroll
-  for (int mt = 0; mt < 6; mt++) {
-    __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
-    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    for (int nt = 0; nt < 4; nt++)
+      acc[2 * k + 1][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, b[nt], acc[2 * k + 1][nt], 0, 0, 0);
+    a0 = n0;
+    a1 = n1;
   }
+  // issue order: the B reads (two paired reads) and A pairs 0 and 1; pair 0's products; pair 2's
+  // reads; pair 1's products; one product of pair 2 (after the wait for pair 2, so pair 3's reads
+  // are not waited for with it); pair 3's reads; the remaining products
+  __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
   __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+  __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+  __builtin_amdgcn_sched_group_barrier(0x008, 9, 0);
+  __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+  __builtin_amdgcn_sched_group_barrier(0x008, 15, 0);
 }
 
 // One 256 x 128 tile over slabs [split * ns, (split + 1) * ns) with a ring of NB slots (NB = 4
