@@ -117,7 +117,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(every box here has one GPU; see DESIGN.md for how the commit would shard)",
+              "(it measures the one-GPU share of the work; the commit shards by polynomial, one "
+              "process per GPU, with no cross-GPU traffic on the hot path)",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "prove":
@@ -495,7 +496,7 @@ def bench_commit(args):
     avg = {k: float(np.mean(v)) for k, v in stages.items()}
 
     m, n = 1 << LOG_M, 1 << LOG_N
-    # algorithmic work per launch: hash = 2^39 int8 MACs = 2^40 ops;
+    # algorithmic work per launch: hash = 2 * 256 * (64 m) * (2 n) int8 ops (2^40 at m = n = 2^12);
     # NTT = read f (8 m n bytes) + write F' (16 m n bytes).
     hash_ops = 2.0 * 256 * (64 * m) * (2 * n)
     ntt_bytes = 24.0 * m * n

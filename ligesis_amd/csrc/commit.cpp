@@ -243,7 +243,7 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
       hip_check(hipEventRecord(ctx.ev[1], ctx.stream), "event");  // every chunk encoded
       hip_check(hipEventRecord(ctx.ev[2], sh), "event");          // every split hashed
     }
-    hip_check(launch_hash_mfma_finalize(part.ptr, ks, (uint32_t)two_n, c.h.ptr, sh), "hash finalize");
+    hip_check(launch_hash_mfma_finalize(part.ptr, ks, (uint32_t)two_n, c.h.ptr, sh, tu.hash_impl), "hash finalize");
     if (ctx.profile) hip_check(hipEventRecord(ctx.ev[3], sh), "event");
     hip_check(hipEventRecord(ctx.pipe_event(9), sh), "pipeline join");
     hip_check(hipStreamWaitEvent(ctx.stream, ctx.pipe_event(9), 0), "pipeline join");

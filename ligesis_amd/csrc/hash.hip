@@ -483,6 +483,7 @@ __global__ __launch_bounds__(64 * (16 / MT) * WN, MINB) void hash_mfma_tile_kern
 #define LGS_AS1 __attribute__((address_space(1)))
 #define LGS_AS3 __attribute__((address_space(3)))
 constexpr int vmcnt_imm(int n) { return (n & 0xF) | (((n >> 4) & 3) << 14) | (7 << 4) | (0xF << 8); }
+constexpr int kLgkmZero = 0xC07F;  // lgkmcnt(0), vmcnt and expcnt unconstrained
 
 // One slot of the ring: separate LDS arrays per slot, so the compiler can tell a slot's reads from
 // the copies in flight into the other slots (a shared array with a dynamic slot index makes it
@@ -561,7 +562,7 @@ __device__ __forceinline__ void dma_step(const DmaTile& t, uint32_t sl, DmaSlot 
 template <int NB>
 __device__ __forceinline__ void hash_dma_body(const uint64_t* __restrict__ enc, uint64_t ld,
                                               const v4i* __restrict__ adig, uint32_t ns, uint32_t split,
-                                              uint32_t ntile, uint32_t ncols, int32_t* __restrict__ part,
+                                              uint32_t ntile, uint32_t ncols, uint64_t* __restrict__ part,
                                               DmaSlot s0, DmaSlot s1, DmaSlot s2, DmaSlot s3) {
   static_assert(NB == 3 || NB == 4, "ring of 3 or 4 slabs");
   DmaTile t;
@@ -601,18 +602,51 @@ __device__ __forceinline__ void hash_dma_body(const uint64_t* __restrict__ enc, 
       dma_step<4>(t, sl + 3, s3, s2, acc);
     }
   }
+  // Epilogue: the split's contribution to H, reduced mod p. Row m = 32 l + r of the tile is digit
+  // plane l of hash row r; a lane holds, for r = 16 h + 4 g + v and column nt * 16 + c, the
+  // sums of planes l = 4 wm + k (k < 4) in acc[2 k + h][nt][v]. Each wave folds its four planes
+  // exactly in 64 bits (|sum_k 256^k S| < 2^55); the upper planes' fold (wm = 1) moves through
+  // LDS (ring slots 0 and 1, free once every copy and read has finished) to the lower planes'
+  // wave, which adds 2^32 times it mod p and writes one word per (split, row, column).
   __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // no copy into LDS outlives the workgroup
+  __builtin_amdgcn_s_waitcnt(kLgkmZero);
+  asm volatile("s_barrier" ::: "memory");
+  uint64_t* xch = (uint64_t*)(t.wn ? s1.a : s0.a);  // 32 rows x 64 columns
+  // fold first, so the accumulators die here: f[(h * 4 + nt) * 4 + v]
+  uint64_t f[32];
 #pragma unroll
-  for (int mt = 0; mt < 8; mt++) {
+  for (int h = 0; h < 2; h++)
 #pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
+    for (int nt = 0; nt < 4; nt++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const uint32_t m = (t.wm * 8 + mt) * 16 + 4 * t.g + v;
-        const uint64_t col = t.col0 + t.wn * 64 + nt * 16 + t.c;
-        part[((uint64_t)split * 256 + m) * ncols + col] = acc[mt][nt][v];
+        int64_t x = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) x += (int64_t)acc[2 * k + h][nt][v] * ((int64_t)1 << (8 * k));
+        f[(h * 4 + nt) * 4 + v] = x >= 0 ? (uint64_t)x : GL_P - (uint64_t)(-x);
       }
-    }
+  if (t.wm == 1) {
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) xch[(16 * h + 4 * t.g + v) * 64 + nt * 16 + t.c] = f[(h * 4 + nt) * 4 + v];
+  }
+  __builtin_amdgcn_s_waitcnt(kLgkmZero);
+  asm volatile("s_barrier" ::: "memory");
+  if (t.wm == 0) {
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const uint32_t r = 16 * h + 4 * t.g + v;
+          const uint64_t hi = xch[r * 64 + nt * 16 + t.c];
+          const uint64_t col = t.col0 + t.wn * 64 + nt * 16 + t.c;
+          part[((uint64_t)split * 32 + r) * ncols + col] = gl_add(f[(h * 4 + nt) * 4 + v], gl_mul(hi, uint64_t(1) << 32));
+        }
   }
 }
 
@@ -624,7 +658,8 @@ __global__ __launch_bounds__(256, 2) void hash_mfma_dma_kernel(const uint64_t* _
   __shared__ v4i la0[1024], la1[1024], la2[1024], la3[NB == 4 ? 1024 : 1];
   __shared__ uint32_t lb0[256], lb1[256], lb2[256], lb3[NB == 4 ? 256 : 1];
   const uint32_t logical = xcd_logical_block();
-  hash_dma_body<NB>(enc, ld, adig, ns, split0 + logical / ntiles, logical % ntiles, ncols, part, DmaSlot{la0, lb0},
+  hash_dma_body<NB>(enc, ld, adig, ns, split0 + logical / ntiles, logical % ntiles, ncols, (uint64_t*)part,
+                    DmaSlot{la0, lb0},
                     DmaSlot{la1, lb1}, DmaSlot{la2, lb2}, DmaSlot{la3, lb3});
 }
 
@@ -646,10 +681,10 @@ __global__ __launch_bounds__(256, 2) void hash_mfma_dma_group_kernel(const HashJ
   const DmaSlot s0{la0, lb0}, s1{la1, lb1}, s2{la2, lb2}, s3{la3, lb3};
   if (jb.slabs_per_split % 4 == 0)
     hash_dma_body<4>(jb.enc, jb.ld, (const v4i*)jb.adig, jb.slabs_per_split, local / jb.ntiles, local % jb.ntiles,
-                     jb.ncols, jb.part, s0, s1, s2, s3);
+                     jb.ncols, (uint64_t*)jb.part, s0, s1, s2, s3);
   else
     hash_dma_body<3>(jb.enc, jb.ld, (const v4i*)jb.adig, jb.slabs_per_split, local / jb.ntiles, local % jb.ntiles,
-                     jb.ncols, jb.part, s0, s1, s2, s3);
+                     jb.ncols, (uint64_t*)jb.part, s0, s1, s2, s3);
 }
 
 // H[r][j] = sum_l 256^l (sum_split part[split][32 l + r][j]) mod p.
@@ -783,11 +818,17 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
   return hipGetLastError();
 }
 
+bool hash_impl_reduces(int impl) { return impl == 11 || impl == 12; }
+
 hipError_t launch_hash_mfma_finalize(const int32_t* part, uint32_t ksplit, uint32_t ncols, uint64_t* h,
-                                     hipStream_t st) {
+                                     hipStream_t st, int impl) {
   const uint64_t n = (uint64_t)32 * ncols;
-  hipLaunchKernelGGL(hash_mfma_finalize_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, part, ksplit,
-                     ncols, h);
+  if (hash_impl_reduces(impl))  // one word per (split, row, column), already mod p
+    hipLaunchKernelGGL(hash_valu_finalize_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
+                       (const uint64_t*)part, ksplit, ncols, h);
+  else
+    hipLaunchKernelGGL(hash_mfma_finalize_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, part, ksplit,
+                       ncols, h);
   return hipGetLastError();
 }
 
@@ -797,7 +838,7 @@ hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
   hipError_t e = launch_hash_mfma_splits(enc, ld, m_rows, ncols, dig, ksplit, 0, ksplit, part, st, impl);
   if (e != hipSuccess) return e;
   if (mid && (e = hipEventRecord(mid, st)) != hipSuccess) return e;
-  return launch_hash_mfma_finalize(part, ksplit, ncols, h, st);
+  return launch_hash_mfma_finalize(part, ksplit, ncols, h, st, impl);
 }
 
 __global__ void hash_finalize_group_kernel(const FinalizeJob* __restrict__ jobs, int njobs) {
@@ -810,15 +851,10 @@ __global__ void hash_finalize_group_kernel(const FinalizeJob* __restrict__ jobs,
   const FinalizeJob& jb = jobs[lo];
   const uint64_t idx = (uint64_t)(blockIdx.x - jb.first_block) * blockDim.x + threadIdx.x;
   if (idx >= (uint64_t)32 * jb.ncols) return;
-  const uint32_t r = (uint32_t)(idx / jb.ncols), j = (uint32_t)(idx % jb.ncols);
+  // the batched hash kernel leaves one reduced word per (split, row, column)
+  const uint64_t* part = (const uint64_t*)jb.part;
   uint64_t acc = 0;
-#pragma unroll
-  for (int l = 0; l < 8; l++) {
-    int64_t s = 0;
-    for (uint32_t sp = 0; sp < jb.ksplit; sp++) s += jb.part[((uint64_t)sp * 256 + 32 * l + r) * jb.ncols + j];
-    uint64_t sm = s >= 0 ? (uint64_t)s : gl_neg((uint64_t)(-s));
-    acc = gl_add(acc, gl_mul(sm, (uint64_t)1 << (8 * l)));
-  }
+  for (uint32_t sp = 0; sp < jb.ksplit; sp++) acc = gl_add(acc, part[(uint64_t)sp * 32 * jb.ncols + idx]);
   jb.h[idx] = acc;
 }
 

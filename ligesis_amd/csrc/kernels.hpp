@@ -67,8 +67,11 @@ hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
 hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
                                   const int8_t* dig, uint32_t ksplit, uint32_t split0, uint32_t nsplits,
                                   int32_t* part, hipStream_t st, int impl);
+// Kernels 11 and 12 (and the batched hash) leave one word per (split, row, column), reduced mod p,
+// in the first quarter of the same buffer; the others 8 digit-plane int32 sums.
+bool hash_impl_reduces(int impl);
 hipError_t launch_hash_mfma_finalize(const int32_t* part, uint32_t ksplit, uint32_t ncols, uint64_t* h,
-                                     hipStream_t st);
+                                     hipStream_t st, int impl);
 
 // Batched launches over job tables that live in device memory.
 hipError_t launch_ntt_group(const NttJob* jobs, int njobs, uint32_t total_blocks, int max_logN, hipStream_t st);
