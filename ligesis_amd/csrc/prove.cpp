@@ -310,11 +310,19 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   DevBasefold& av_state = P.av_state;
   DevBasefold& bi_state = P.bi_state;
   mark(0, st);
-  ef2* dblob = P.get<ef2>(blob.size());
-  P.blob_host = P.arena.upload(dblob, blob.data(), blob.size() * sizeof(ef2), st);
-  const ef2* B = dblob;
-  uint32_t* didx = P.get<uint32_t>(s);
-  P.idx_host = P.arena.upload(didx, idx.data(), s * 4, st);
+  // the constants blob and the column indices, in one upload
+  const size_t blob_bytes = blob.size() * sizeof(ef2);
+  uint8_t* dconst = P.get<uint8_t>(blob_bytes + s * 4);
+  {
+    std::vector<uint8_t> h(blob_bytes + s * 4);
+    std::memcpy(h.data(), blob.data(), blob_bytes);
+    std::memcpy(h.data() + blob_bytes, idx.data(), s * 4);
+    uint8_t* hp = (uint8_t*)P.arena.upload(dconst, h.data(), h.size(), st);
+    P.blob_host = hp;
+    P.idx_host = hp + blob_bytes;
+  }
+  const ef2* B = (const ef2*)dconst;
+  uint32_t* didx = (uint32_t*)(dconst + blob_bytes);
 
   // ---- the second stream: B_I and its Basefold commitment, then a-bar's, then Stage B ----
   // B_I (ligesis.rs:402-410) needs only F' and the column indices, and its commitment
@@ -436,10 +444,15 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     }
   }
   const size_t terms_bytes = eq_terms.size() * sizeof(EqTerm), tabs_bytes = eq_tabs.size() * sizeof(EqTable);
-  uint8_t* eq_jobs = P.get<uint8_t>(terms_bytes + tabs_bytes + eq_fact.size() * sizeof(EqTable));
-  P.arena.upload(eq_jobs, eq_terms.data(), terms_bytes, st);
-  P.arena.upload(eq_jobs + terms_bytes, eq_tabs.data(), tabs_bytes, st);
-  P.arena.upload(eq_jobs + terms_bytes + tabs_bytes, eq_fact.data(), eq_fact.size() * sizeof(EqTable), st);
+  const size_t fact_bytes = eq_fact.size() * sizeof(EqTable);
+  uint8_t* eq_jobs = P.get<uint8_t>(terms_bytes + tabs_bytes + fact_bytes);
+  {  // one upload: terms, tables, factored tables
+    std::vector<uint8_t> h(terms_bytes + tabs_bytes + fact_bytes);
+    std::memcpy(h.data(), eq_terms.data(), terms_bytes);
+    std::memcpy(h.data() + terms_bytes, eq_tabs.data(), tabs_bytes);
+    std::memcpy(h.data() + terms_bytes + tabs_bytes, eq_fact.data(), fact_bytes);
+    P.arena.upload(eq_jobs, h.data(), h.size(), st);
+  }
   const EqTable* d_eq_fact = (const EqTable*)(eq_jobs + terms_bytes + tabs_bytes);  // [0] big, [1] big2
   hip_check(launch_eq_group((const EqTerm*)eq_jobs, (int)eq_terms.size(), eq_term_blocks,
                             (const EqTable*)(eq_jobs + eq_terms.size() * sizeof(EqTerm)), (int)eq_tabs.size(),
@@ -594,21 +607,27 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     tjobs[i].d = (uint32_t)S.d;
     tjobs[i].m1 = i == 0;
   }
-  const size_t rj_bytes = rjobs.size() * sizeof(ScRoundJob), tj_bytes = tjobs.size() * sizeof(ScTailJob);
-  uint8_t* scjobs = P.get<uint8_t>(rj_bytes + tj_bytes + reds.size() * sizeof(RedJob));
-  if (!rjobs.empty()) P.arena.upload(scjobs, rjobs.data(), rj_bytes, st);
-  ScTailJob* d_tjobs = (ScTailJob*)(scjobs + rj_bytes);
-  P.arena.upload(d_tjobs, tjobs.data(), tj_bytes, st);
-  const RedJob* d_reds = (const RedJob*)(scjobs + rj_bytes + tj_bytes);
-  P.arena.upload((void*)d_reds, reds.data(), reds.size() * sizeof(RedJob), st);
+  // the step tables go to the launches as kernel arguments: no uploads on st before the rounds
+  auto fill_reds = [&](RedJob* out, uint32_t& nr, uint32_t tau) {
+    nr = red_first[tau + 1] - red_first[tau];
+    if (nr > (uint32_t)kScMaxReds) throw Error(1, "prove: too many reductions in one sumcheck launch");
+    for (uint32_t q = 0; q < nr; q++) out[q] = reds[red_first[tau] + q];
+  };
   mark(1, st);
-  for (uint32_t tau = 0; tau < steps; tau++)
-    hip_check(launch_sc_round_group((const ScRoundJob*)scjobs + step_first[tau],
-                                    (int)(step_first[tau + 1] - step_first[tau]), step_blocks[tau],
-                                    d_reds + red_first[tau], red_first[tau + 1] - red_first[tau], st),
-              "sumcheck round step");
-  hip_check(launch_sc_tail_group(d_tjobs, 7, d_reds + red_first[steps], red_first[steps + 1] - red_first[steps], st),
-            "sumcheck tails");
+  for (uint32_t tau = 0; tau < steps; tau++) {
+    ScStep step{};
+    step.nj = step_first[tau + 1] - step_first[tau];
+    if (step.nj > (uint32_t)kScMaxJobs) throw Error(1, "prove: too many sumchecks in one launch");
+    for (uint32_t q = 0; q < step.nj; q++) step.j[q] = rjobs[step_first[tau] + q];
+    step.round_blocks = step_blocks[tau];
+    fill_reds(step.r, step.nr, tau);
+    hip_check(launch_sc_round_group(step, st), "sumcheck round step");
+  }
+  ScTail tail{};
+  tail.nj = 7;
+  for (int i = 0; i < 7; i++) tail.j[i] = tjobs[i];
+  fill_reds(tail.r, tail.nr, steps);
+  hip_check(launch_sc_tail_group(tail, st), "sumcheck tails");
   mark(2, st);
 
 

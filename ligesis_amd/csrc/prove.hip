@@ -272,26 +272,22 @@ __device__ __forceinline__ void sc_tail_body(const ScTailJob& jb, ef2 (*tab)[1 <
 
 // The tails of several sumchecks, one workgroup each (blockIdx.x = job); blocks past njobs run
 // the deferred reductions `reds`.
-__global__ __launch_bounds__(kTailThreads) void sc_tail_group_kernel(const ScTailJob* __restrict__ jobs,
-                                                                     uint32_t njobs,
-                                                                     const RedJob* __restrict__ reds) {
+__global__ __launch_bounds__(kTailThreads) void sc_tail_group_kernel(ScTail a) {
   __shared__ ef2 tab[3][1 << kScTailLog];
   __shared__ ef2 wpart[kScTailLog][kTailThreads / 64][4];
-  if (blockIdx.x >= njobs) {
-    red_job_body(reds[blockIdx.x - njobs], &wpart[0][0][0]);
+  if (blockIdx.x >= a.nj) {
+    red_job_body(a.r[blockIdx.x - a.nj], &wpart[0][0][0]);
     return;
   }
-  const ScTailJob& j = jobs[blockIdx.x];
+  const ScTailJob& j = a.j[blockIdx.x];
   if (j.d == 3)
     sc_tail_body<3>(j, tab, wpart);
   else
     sc_tail_body<2>(j, tab, wpart);
 }
 
-hipError_t launch_sc_tail_group(const ScTailJob* jobs, uint32_t njobs, const RedJob* reds, uint32_t nreds,
-                                hipStream_t st) {
-  if (njobs + nreds)
-    hipLaunchKernelGGL(sc_tail_group_kernel, dim3(njobs + nreds), dim3(kTailThreads), 0, st, jobs, njobs, reds);
+hipError_t launch_sc_tail_group(const ScTail& tail, hipStream_t st) {
+  if (tail.nj + tail.nr) hipLaunchKernelGGL(sc_tail_group_kernel, dim3(tail.nj + tail.nr), dim3(kTailThreads), 0, st, tail);
   return hipGetLastError();
 }
 
@@ -530,20 +526,19 @@ __device__ __forceinline__ void sc_bits2_body(const uint8_t* __restrict__ bits, 
 
 // One round of several independent sumchecks in one launch (sumchecks run in lockstep): block
 // -> job by binary search over the jobs' first blocks.
-__global__ __launch_bounds__(256) void sc_round_group_kernel(const ScRoundJob* __restrict__ jobs, int njobs,
-                                                             uint32_t round_blocks, const RedJob* __restrict__ reds) {
+__global__ __launch_bounds__(256) void sc_round_group_kernel(ScStep a) {
   __shared__ ef2 sh[256 * 4];
-  if (blockIdx.x >= round_blocks) {
-    red_job_body(reds[blockIdx.x - round_blocks], sh);
+  if (blockIdx.x >= a.round_blocks) {
+    red_job_body(a.r[blockIdx.x - a.round_blocks], sh);
     return;
   }
-  int lo = 0, hi = njobs - 1;
+  int lo = 0, hi = (int)a.nj - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) / 2;
-    if (jobs[mid].first_block <= blockIdx.x) lo = mid;
+    if (a.j[mid].first_block <= blockIdx.x) lo = mid;
     else hi = mid - 1;
   }
-  const ScRoundJob& j = jobs[lo];
+  const ScRoundJob& j = a.j[lo];
   const uint32_t blk = blockIdx.x - j.first_block;
   if (j.bits) {
     if (j.d == 3)
@@ -558,11 +553,9 @@ __global__ __launch_bounds__(256) void sc_round_group_kernel(const ScRoundJob* _
   }
 }
 
-hipError_t launch_sc_round_group(const ScRoundJob* jobs, int njobs, uint32_t total_blocks, const RedJob* reds,
-                                 uint32_t nreds, hipStream_t st) {
-  if (total_blocks + nreds)
-    hipLaunchKernelGGL(sc_round_group_kernel, dim3(total_blocks + nreds), dim3(256), 0, st, jobs, njobs,
-                       total_blocks, reds);
+hipError_t launch_sc_round_group(const ScStep& step, hipStream_t st) {
+  if (step.round_blocks + step.nr)
+    hipLaunchKernelGGL(sc_round_group_kernel, dim3(step.round_blocks + step.nr), dim3(256), 0, st, step);
   return hipGetLastError();
 }
 

@@ -39,9 +39,15 @@ struct ScRoundJob {
   // m1: d = 3 and the second table is the first minus one (not stored: p[1] unused)
   uint32_t d, first_block, nblk, m1;
 };
-// Blocks past total_blocks run the reductions `reds` (one block each).
-hipError_t launch_sc_round_group(const ScRoundJob* jobs, int njobs, uint32_t total_blocks, const RedJob* reds,
-                                 uint32_t nreds, hipStream_t st);
+// One grouped round step, passed by value as the kernel's argument (no upload): jobs j[0..nj)
+// own blocks [0, round_blocks); the blocks after them run the reductions r[0..nr) (one each).
+constexpr int kScMaxJobs = 7, kScMaxReds = 8;
+struct ScStep {
+  ScRoundJob j[kScMaxJobs];
+  RedJob r[kScMaxReds];
+  uint32_t nj, nr, round_blocks, pad;
+};
+hipError_t launch_sc_round_group(const ScStep& step, hipStream_t st);
 
 // The remaining rounds (len <= 2^kScTailLog entries) of one sumcheck, one workgroup per job.
 constexpr int kScTailLog = 10;
@@ -52,9 +58,13 @@ struct ScTailJob {
   ef2* finals;
   uint32_t d, len, m1, pad;  // m1 as in ScRoundJob
 };
-// Blocks past njobs run the reductions `reds`.
-hipError_t launch_sc_tail_group(const ScTailJob* jobs, uint32_t njobs, const RedJob* reds, uint32_t nreds,
-                                hipStream_t st);
+// The tails of every sumcheck (one workgroup each), then the reductions r[0..nr); by value.
+struct ScTail {
+  ScTailJob j[kScMaxJobs];
+  RedJob r[kScMaxReds];
+  uint32_t nj, nr;
+};
+hipError_t launch_sc_tail_group(const ScTail& tail, hipStream_t st);
 
 // One Basefold reduction round over its three channels (bf_round3_kernel, basefold.rs:748-775).
 struct BfRound {
