@@ -464,7 +464,8 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   ef2* a_vec = P.get<ef2>(n);
   ef2* a_copy = P.get<ef2>(n);
   ef2* a_copy2 = P.get<ef2>(n);
-  uint64_t chunks = std::max<uint64_t>(1, std::min<uint64_t>(m, (uint64_t(1) << 21) / n));
+  // row chunks: about 2^19 threads in all (2048 workgroups at n = 2^16), few partials to add
+  uint64_t chunks = std::max<uint64_t>(1, std::min<uint64_t>(m, (uint64_t(1) << 19) / n));
   while (m % chunks) chunks >>= 1;
   ef2* wpart = P.get<ef2>(chunks * n);
   ef2* wpart_g1 = P.get<ef2>(W);

@@ -138,15 +138,15 @@ __global__ void wrows_kernel(const uint64_t* __restrict__ M, uint64_t cols, cons
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= cols) return;
   const uint32_t i0 = blockIdx.y * rows_per_chunk, i1 = i0 + rows_per_chunk;
-  // unreduced sums of the two components, four rows' loads in flight at a time
+  // unreduced sums of the two components, eight rows' loads in flight at a time
   WideAcc a0 = wide_zero(), a1 = wide_zero();
   uint32_t i = i0;
-  for (; i + 4 <= i1; i += 4) {
-    uint64_t x[4];
+  for (; i + 8 <= i1; i += 8) {
+    uint64_t x[8];
 #pragma unroll
-    for (int u = 0; u < 4; u++) x[u] = M[(uint64_t)(i + u) * cols + j];
+    for (int u = 0; u < 8; u++) x[u] = M[(uint64_t)(i + u) * cols + j];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < 8; u++) {
       const ef2 wi = w[i + u];
       wide_mul_add(a0, wi.c0, x[u]);
       wide_mul_add(a1, wi.c1, x[u]);
@@ -345,22 +345,32 @@ __device__ __forceinline__ void vvec_body(const ef2* __restrict__ eqz1, uint64_t
 }
 
 // w[k] = prod_t (1 - r3[t] + r3[t] * g^((2^t k) mod 2n)) (rs_weights, ligesis.rs:553-567);
-// tw[e] = g^e for e < n, and g^(e + n) = -g^e.
+// tw[e] = g^e for e < n, and g^(e + n) = -g^e. Four lanes share k: lane q multiplies the
+// factors t = q mod 4, then two butterfly steps multiply the four partial products.
 __device__ __forceinline__ void rs_weights_body(const ef2* __restrict__ r3, int log2n, uint64_t n,
                                                 const uint64_t* __restrict__ tw, ef2* __restrict__ w,
                                                 uint32_t blk) {
-  const uint64_t k = (uint64_t)blk * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+  const uint64_t gid = (uint64_t)blk * blockDim.x + threadIdx.x;
+  const uint64_t k = gid >> 2;
+  const int q = (int)(gid & 3);
+  const bool live = k < n;  // the same for the four lanes of k; every lane takes the shuffles
   const uint64_t mask = 2 * n - 1;
   ef2 prod = ef_one();
-  for (int t = 0; t < log2n; t++) {
-    const uint64_t e = (k << t) & mask;
-    const uint64_t gp = e < n ? tw[e] : gl_neg(tw[e - n]);
-    const ef2 r = r3[t];
-    const ef2 f = ef_add(ef_one_minus(r), ef_mul_base(r, gp));
-    prod = ef_mul(prod, f);
+  if (live) {
+    for (int t = q; t < log2n; t += 4) {
+      const uint64_t e = (k << t) & mask;
+      const uint64_t gp = e < n ? tw[e] : gl_neg(tw[e - n]);
+      const ef2 r = r3[t];
+      prod = ef_mul(prod, ef_add(ef_one_minus(r), ef_mul_base(r, gp)));
+    }
   }
-  w[k] = prod;
+#pragma unroll
+  for (int off = 1; off < 4; off <<= 1) {
+    const ef2 o{(uint64_t)__shfl_xor((unsigned long long)prod.c0, off),
+                (uint64_t)__shfl_xor((unsigned long long)prod.c1, off)};
+    prod = ef_mul(prod, o);
+  }
+  if (live && q == 0) w[k] = prod;
 }
 
 // The elementwise inputs of the Stage A sumchecks in one launch, block ranges in order: g2 (and
@@ -812,7 +822,7 @@ hipError_t launch_prep_group(PrepJobs a, hipStream_t st) {
   a.b_g2 = (uint32_t)((a.W * 64 + 255) / 256);
   a.b_zero = 256;
   a.b_vv = (uint32_t)((a.W + 255) / 256);
-  a.b_rs = (uint32_t)((a.n + 255) / 256);
+  a.b_rs = (uint32_t)((4 * a.n + 255) / 256);  // four lanes per weight
   const uint32_t b_pl = (uint32_t)((a.pn + 255) / 256);
   hipLaunchKernelGGL(prep_group_kernel, dim3(a.b_g2 + a.b_zero + a.b_vv + a.b_rs + b_pl), dim3(256), 0, st, a);
   return hipGetLastError();
