@@ -139,6 +139,12 @@ struct Writer {
   void ef(const uint64_t* w) { words(w, 2); }
 };
 
+// Split of B_I's 2^Lb-entry eq tables into low (2^klo) and high factors: blocks of the bits
+// rounds take up to 2^klo consecutive pairs sharing the high factor (the larger klo, the fewer
+// blocks and the more pairs per thread), and round tau <= 3 pairs entries 2^(Lb - 1 - tau) apart,
+// which must be a multiple of 2^klo: 8 <= klo <= Lb - 4.
+uint32_t bits_klo(uint32_t Lb) { return std::max<uint32_t>(8, std::min<uint32_t>(12, Lb - 4)); }
+
 struct Sumcheck {
   int d = 0;
   uint32_t nv = 0;
@@ -261,6 +267,46 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     o_wt[c] = put(wt[c]);
     o_eq4[c] = put(eq4[c]);
     o_zp1[c] = put(EfV(zp[c].begin() + (R ? 1 : 0), zp[c].end()));
+  }
+  // Rounds 1..3 of B_I's two sumchecks (binary check, f = 0; batched opening, f = 1) read the
+  // 0/1 bytes: the prefix weights W_tau(y) = prod_t (y_t ? c_t : 1 - c_t) (y_0 the top bit, bound
+  // first), and for every term q of their eq tables the fold of its high factor over the bound
+  // variables, scale_q prod_{t < tau} eq1(pt_q[Lb - 1 - t], c_t), eq1(a, c) = a c + (1 - a)(1 - c).
+  const EfV* bits_ch[2] = {&ch_bin, &ch_bibatch};
+  const EfV* bits_pt[2][3] = {{&alpha1, nullptr, nullptr}, {&r1, &p_bi_hash, &p_bi_recon}};
+  const ef2 bits_scale[2][3] = {{ef_one(), ef_one(), ef_one()}, {gbi[0], gbi[1], gbi[2]}};
+  const size_t bits_offs[2][3] = {{o_alpha1, 0, 0}, {o_r1, o_pbh, o_pbr}};
+  const uint32_t bits_nterms[2] = {1, 3};
+  size_t o_bw[2][4] = {}, o_bs[2][4][3] = {};
+  for (int f = 0; f < 2; f++) {
+    const EfV& c = *bits_ch[f];
+    for (uint32_t tau = 1; tau < 4 && tau < Lb; tau++) {
+      EfV w(size_t(1) << tau);
+      for (size_t y = 0; y < w.size(); y++) {
+        ef2 v = ef_one();
+        for (uint32_t t = 0; t < tau; t++)
+          v = ef_mul(v, ((y >> (tau - 1 - t)) & 1) ? c[t] : ef_sub(ef_one(), c[t]));
+        w[y] = v;
+      }
+      // the kernel looks b up by the 2^tau prefix bits: store every subset sum of w
+      EfV sums(size_t(1) << w.size());
+      for (size_t mm = 0; mm < sums.size(); mm++) {
+        ef2 acc = ef_zero();
+        for (size_t y = 0; y < w.size(); y++)
+          if ((mm >> y) & 1) acc = ef_add(acc, w[y]);
+        sums[mm] = acc;
+      }
+      o_bw[f][tau] = put(sums);
+      for (uint32_t q = 0; q < bits_nterms[f]; q++) {
+        const EfV& pt = *bits_pt[f][q];
+        ef2 v = bits_scale[f][q];
+        for (uint32_t t = 0; t < tau; t++) {
+          const ef2 a = pt[Lb - 1 - t];
+          v = ef_mul(v, ef_add(ef_mul(a, c[t]), ef_mul(ef_sub(ef_one(), a), ef_sub(ef_one(), c[t]))));
+        }
+        o_bs[f][tau][q] = put(EfV{v});
+      }
+    }
   }
 
   // shapes and layouts (the same for every prove of this key)
@@ -397,7 +443,9 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     for (int c = 0; c < 3; c++)
       if (R >= 1) want.push_back({eqr[c][0], R - 1, {{o_zp1[c], nullptr}}, false});  // Basefold rounds
     for (auto& w : want) {
-      const uint32_t klo = w.k / 2, khi = w.k - klo;
+      // B_I's two tables are evaluated from their factors in blocks of 256 consecutive entries,
+      // which must share the high factor: 2^klo >= 256
+      const uint32_t klo = w.factored ? bits_klo(w.k) : w.k / 2, khi = w.k - klo;
       EqTable tb{};
       tb.out = w.out;
       tb.n = uint64_t(1) << w.k;
@@ -419,6 +467,31 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
       factored.push_back(w.factored);
     }
   }
+  // the folded high factors of B_I's eq tables for bits rounds 1..bits_rounds - 1: terms of
+  // k = khi - tau variables (klo = 0) over pt[klo ..), scaled by the folds of the bound ones
+  const uint32_t bits_rounds = std::min<uint32_t>(4, Lb > (uint32_t)kScTailLog ? Lb - kScTailLog : 0);
+  const uint32_t b_klo = bits_klo(Lb), b_khi = Lb - b_klo;
+  struct BitsTerm {
+    int f;
+    uint32_t tau, q;
+  };
+  std::vector<EqTerm> bits_terms;
+  std::vector<BitsTerm> bits_meta;
+  for (int f = 0; f < 2; f++)
+    for (uint32_t tau = 1; tau < bits_rounds; tau++)
+      for (uint32_t q = 0; q < bits_nterms[f]; q++) {
+        EqTerm t{};
+        t.pt = B + bits_offs[f][q] + b_klo;
+        t.scale = B + o_bs[f][tau][q];
+        t.k = b_khi - tau;
+        t.klo = 0;
+        t.lo = (ef2*)(uintptr_t)eq_half_words;
+        eq_half_words += 1;
+        t.hi = (ef2*)(uintptr_t)eq_half_words;
+        eq_half_words += uint64_t(1) << t.k;
+        bits_terms.push_back(t);
+        bits_meta.push_back({f, tau, q});
+      }
   ef2* eq_half = P.get<ef2>(eq_half_words);
   uint32_t eq_term_blocks = 0, eq_tab_blocks = 0;
   {
@@ -442,6 +515,28 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
       eq_tab_blocks += (uint32_t)((tb.n + 255) / 256);
       eq_tabs.push_back(tb);
     }
+  }
+  // eq_fact: [0] big, [1] big2 (round 0), then for f = 0, 1 and tau = 1 .. bits_rounds - 1 the
+  // round-tau factors: the same low halves, the folded high halves
+  std::vector<size_t> bits_fact(8, 0);  // f * 4 + tau -> index into eq_fact
+  if (eq_fact.size() != 2 || eq_fact[0].nterms != 1 || eq_fact[1].nterms != 3)
+    throw Error(1, "prove: B_I's eq tables have one and three terms");
+  for (size_t k = 0; k < bits_terms.size(); k++) {
+    EqTerm& t = bits_terms[k];
+    const BitsTerm& bm = bits_meta[k];
+    t.lo = eq_half + (uintptr_t)t.lo;
+    t.hi = eq_half + (uintptr_t)t.hi;
+    t.first_block = eq_term_blocks;
+    eq_term_blocks += (uint32_t)(((uint64_t(1) << t.k) + 255) / 256);
+    eq_terms.push_back(t);
+    if (bm.q == 0) {
+      EqTable tb = eq_fact[bm.f];
+      tb.out = nullptr;
+      tb.n >>= bm.tau;
+      bits_fact[bm.f * 4 + bm.tau] = eq_fact.size();
+      eq_fact.push_back(tb);
+    }
+    eq_fact[bits_fact[bm.f * 4 + bm.tau]].hi[bm.q] = t.hi;
   }
   const size_t terms_bytes = eq_terms.size() * sizeof(EqTerm), tabs_bytes = eq_tabs.size() * sizeof(EqTable);
   const size_t fact_bytes = eq_fact.size() * sizeof(EqTable);
@@ -558,7 +653,29 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   // Partial sums alternate between two halves of scpart: the launch of step tau writes its half
   // while its extra blocks reduce the other half, written by step tau - 1 (the dot products'
   // partials for step 0); the tail launch reduces the last step's.
-  const size_t part_half = (size_t)7 * 1024 * 4;
+  // blocks of one round of sumcheck i: a bits round covers 256 kpt pairs per block, all inside
+  // one run of 2^klo entries of its eq table's low factor (kpt <= 2^klo / 256)
+  auto round_shape = [&](int i, uint32_t t, uint32_t& nblk, uint32_t& kpt) {
+    const uint64_t half = uint64_t(1) << (sc[i].nv - 1 - t);
+    if ((i == 0 || i == 5) && t < bits_rounds) {
+      kpt = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, half >> 16), (uint64_t(1) << b_klo) >> 8);
+      nblk = (uint32_t)(half / (256 * (uint64_t)kpt));
+    } else {
+      kpt = 0;
+      nblk = sc_round_blocks(half);
+    }
+  };
+  size_t part_half = 0;  // partials of one step, the largest step
+  for (uint32_t tau = 0; tau < steps; tau++) {
+    size_t w = 0;
+    for (int i = 0; i < 7; i++) {
+      if (tau + n_big[i] < steps) continue;
+      uint32_t nb, kp;
+      round_shape(i, tau + n_big[i] - steps, nb, kp);
+      w += (size_t)nb * (sc[i].d + 1);
+    }
+    part_half = std::max(part_half, w);
+  }
   ef2* scpart = P.get<ef2>(2 * part_half);
   std::vector<ScRoundJob> rjobs;
   std::vector<RedJob> reds = {RedJob{part, res + o_y, dot_blocks(n), 1}};
@@ -576,12 +693,17 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
       for (int q = 0; q < 3; q++) j.p[q] = runs[i].p[q];
       j.half = uint64_t(1) << (S.nv - 1 - t);
       j.r = B + runs[i].ch_off + t;
-      if (t == 0 && (i == 0 || i == 5)) {  // B_I's sumchecks start from the 0/1 bytes
+      if ((i == 0 || i == 5) && t < bits_rounds) {  // B_I's first rounds read the 0/1 bytes
+        const int f = i == 0 ? 0 : 1;
         j.bits = bi8;
-        j.eqt = d_eq_fact + (i == 0 ? 0 : 1);
+        j.tau = t;
+        j.eqt = d_eq_fact + (t == 0 ? f : bits_fact[f * 4 + t]);
+        if (t > 0) j.wts = B + o_bw[f][t];
+        if (t + 1 < bits_rounds)  // only the last of them writes the folded tables
+          for (int q = 0; q < 3; q++) j.p[q] = nullptr;
       }
       j.m1 = i == 0;  // the binary check's second table is B_I - 1
-      j.nblk = sc_round_blocks(j.half);
+      round_shape(i, t, j.nblk, j.kpt);
       j.part = scpart + poff;
       poff += (size_t)j.nblk * (S.d + 1);
       reds.push_back(RedJob{j.part, res + S.rounds_at + (size_t)t * (S.d + 1), j.nblk, (uint32_t)S.d + 1});
@@ -616,13 +738,22 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   };
   mark(1, st);
   for (uint32_t tau = 0; tau < steps; tau++) {
-    ScStep step{};
-    step.nj = step_first[tau + 1] - step_first[tau];
-    if (step.nj > (uint32_t)kScMaxJobs) throw Error(1, "prove: too many sumchecks in one launch");
-    for (uint32_t q = 0; q < step.nj; q++) step.j[q] = rjobs[step_first[tau] + q];
-    step.round_blocks = step_blocks[tau];
-    fill_reds(step.r, step.nr, tau);
-    hip_check(launch_sc_round_group(step, st), "sumcheck round step");
+    // bits jobs (B_I's first rounds, all at the same round) go to a launch of their own
+    ScStep bstep{}, step{};
+    uint32_t btau = 0;
+    for (uint32_t q = step_first[tau]; q < step_first[tau + 1]; q++) {
+      ScRoundJob j = rjobs[q];
+      ScStep& dst = j.bits ? bstep : step;
+      if (j.bits) btau = j.tau;
+      if (dst.nj == (uint32_t)kScMaxJobs) throw Error(1, "prove: too many sumchecks in one launch");
+      j.first_block = dst.round_blocks;
+      dst.round_blocks += j.nblk;
+      dst.j[dst.nj++] = j;
+    }
+    ScStep& rs = step.nj ? step : bstep;  // the step's reductions ride on one of the launches
+    fill_reds(rs.r, rs.nr, tau);
+    if (bstep.nj) hip_check(launch_sc_bits_group(bstep, btau, st), "sumcheck bits round step");
+    if (step.nj) hip_check(launch_sc_round_group(step, st), "sumcheck round step");
   }
   ScTail tail{};
   tail.nj = 7;

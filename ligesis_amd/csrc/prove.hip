@@ -117,10 +117,13 @@ __global__ void eq_expand_group_kernel(const EqTable* __restrict__ tabs, int nta
 }
 
 // Entry x of an eq table from its half tables (what eq_expand_group_kernel would store).
+// (constant term indices: a loop over nterms would index the table's pointer arrays at run time
+// and put a copy of the table in scratch memory)
 __device__ __forceinline__ ef2 eq_table_at(const EqTable& t, uint64_t x) {
   const uint64_t bl = x & ((uint64_t(1) << t.klo) - 1), bh = x >> t.klo;
   ef2 v = ef_mul(t.lo[0][bl], t.hi[0][bh]);
-  for (uint32_t q = 1; q < t.nterms; q++) v = ef_add(v, ef_mul(t.lo[q][bl], t.hi[q][bh]));
+  if (t.nterms > 1) v = ef_add(v, ef_mul(t.lo[1][bl], t.hi[1][bh]));
+  if (t.nterms > 2) v = ef_add(v, ef_mul(t.lo[2][bl], t.hi[2][bh]));
   return v;
 }
 
@@ -457,81 +460,199 @@ __device__ __forceinline__ void sc_round_body(ef2* __restrict__ p0, ef2* __restr
   }
 }
 
-// First round of the binary-check sumcheck sum_x eq(x) b(x) (b(x) - 1) (ligesis.rs:405-409)
-// with b read as 0/1 words. Per pair (lo, hi) of the top variable, b(t) = lo + t (hi - lo):
-// b(t) (b(t) - 1) vanishes when lo = hi, and is t (t - 1) otherwise, so the round values are
-// s(0) = s(1) = 0, s(2) = 2 sum (2 E_hi - E_lo), s(3) = 6 sum (3 E_hi - 2 E_lo) over the pairs
-// with lo != hi -- the same field values the general product rule gives, from additions only.
-// The folded bit lo + r (hi - lo) is one of 0, 1, r, 1 - r (the b - 1 table is implicit: the
-// later rounds use b - 1 from b).
-__device__ __forceinline__ void sc_bits3_body(const uint8_t* __restrict__ bits, ef2* __restrict__ b_out,
-                                              const EqTable* __restrict__ eqt, ef2* __restrict__ e, uint64_t half,
-                                              const ef2* __restrict__ r_ptr, ef2* __restrict__ part, uint32_t blk,
-                                              uint32_t nblk, ef2* sh) {
-  const ef2 r = *r_ptr, one = ef_one();
-  const ef2 omr = ef_sub(one, r);
-  const EqTable T = *eqt;
-  ef2 L = ef_zero(), H = ef_zero();
-  for (uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x; i < half; i += (uint64_t)nblk * blockDim.x) {
-    const uint32_t blo = bits[i], bhi = bits[i + half];
-    const ef2 elo = eq_table_at(T, i), ehi = eq_table_at(T, i + half);
-    e[i] = ef_add(elo, ef_mul(r, ef_sub(ehi, elo)));
-    ef2 bf;
-    if (blo == bhi) {
-      bf = blo ? one : ef_zero();
-    } else {
-      bf = bhi ? r : omr;
-      L = ef_add(L, elo);
-      H = ef_add(H, ehi);
-    }
-    b_out[i] = bf;
+// Rounds tau = 0..3 of B_I's two sumchecks, from the 0/1 bytes (ligesis.rs:405-409, 513-522).
+// After tau rounds the first table at x is b(x) = sum_y W(y) bits[y 2^(nv - tau) + x] (y the
+// bound tau-bit prefix), looked up in an LDS copy of the 2^(2^tau) subset sums of W (wts)
+// indexed by the 2^tau bits. The eq table is sum_q lo_q[x mod 2^klo] hi_q[x >> klo] (hi_q already folded
+// over the bound variables): a block covers 256 kpt consecutive pairs inside one 2^klo run, so
+// the high factors of both halves, Ha_q and Hb_q, are the same for the whole block and come out
+// of its sums. Per pair only the low factors L_q multiply:
+//  * binary check, sum eq b (b - 1) (F = 0, one term): with b(t) = b_lo + t db,
+//    b(t) (b(t) - 1) = c0 + t c1 + t^2 c2 (c0 = b_lo (b_lo - 1), c1 = db (2 b_lo - 1),
+//    c2 = db^2) and eq(t) = L H(t), H(t) = Ha + t (Hb - Ha), so s(t) = H(t) (C0 + t C1 + t^2 C2)
+//    with C_k = sum L c_k. In round 0 (bits) the product vanishes unless b_lo != b_hi, where it
+//    is t (t - 1): s = 0, 0, 2 (2 Hb - Ha) U, 6 (3 Hb - 2 Ha) U with U the sum of L over those
+//    pairs;
+//  * batched opening, sum b G (F = 1, up to three terms): s(t) = sum_q H_q(t) (A_q + t (B_q -
+//    A_q)) with A_q = sum L_q b_lo, B_q = sum L_q b_hi; in round 0 A_q, B_q are sums of L_q over
+//    the set bits.
+// The values are the ones the product rule gives entry by entry. With p[0] set (the last such
+// round) both tables are also folded with the round's challenge for the rounds that read tables.
+template <int TAU, int F>
+__device__ __forceinline__ void sc_bits_body(const ScRoundJob& j, uint32_t blk, ef2* wl, ef2* sh) {
+  constexpr int NY = 1 << TAU, NM = 1 << NY;
+  constexpr int D = F ? 2 : 3;
+  if (TAU > 0) {  // wts holds the subset sums (prepared on the host), one coalesced copy
+    for (uint32_t m = threadIdx.x; m < (uint32_t)NM; m += blockDim.x) wl[m] = j.wts[m];
+    __syncthreads();
   }
-  // 2 (2H - L) and 6 (3H - 2L), linear in the block's sums, so partials add up
-  const ef2 h2 = ef_add(H, H), l2 = ef_add(L, L);
-  const ef2 s2 = ef_sub(h2, L);
-  const ef2 s3 = ef_sub(ef_add(h2, H), l2);
-  const ef2 s3x2 = ef_add(s3, s3);
-  ef2 acc[4] = {ef_zero(), ef_zero(), ef_add(s2, s2), ef_add(s3x2, ef_add(s3x2, s3x2))};
-  block_sum<4>(acc, sh);
-  if (threadIdx.x == 0) {
+  // terms: the binary check's eq table has one, the batched opening's three (checked on the
+  // host), so every load below is unconditional
+  constexpr int NT = F ? 3 : 1;
+  const EqTable* __restrict__ T = j.eqt;
+  const uint32_t klo = T->klo;
+  const uint64_t half = j.half, stride = 2 * half, lmask = (uint64_t(1) << klo) - 1;
+  const uint64_t base = (uint64_t)blk * blockDim.x * j.kpt;
+  const uint64_t bh = base >> klo, boff = half >> klo;
+  ef2 Ha[NT], Hb[NT];
+  const ef2* lo_tab[NT];
 #pragma unroll
-    for (int q = 0; q < 4; q++) part[(uint64_t)blk * 4 + q] = acc[q];
+  for (int q = 0; q < NT; q++) {
+    Ha[q] = T->hi[q][bh];
+    Hb[q] = T->hi[q][bh + boff];
+    lo_tab[q] = T->lo[q];
+  }
+  const ef2 r = *j.r, one = ef_one();
+  ef2* __restrict__ bout = j.p[0];
+  ef2* __restrict__ eout = j.p[D - 1];
+  ef2 Hr[NT];  // high factors folded with r (the next round's eq table)
+#pragma unroll
+  for (int q = 0; q < NT; q++) Hr[q] = ef_add(Ha[q], ef_mul(r, ef_sub(Hb[q], Ha[q])));
+  // accumulators: F = 0: [C0, C1, C2] (round 0: [U]); F = 1: [A_q, B_q] per term
+  ef2 acc[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) acc[q] = ef_zero();
+  for (uint32_t k = 0; k < j.kpt; k++) {
+    const uint64_t i = base + (uint64_t)k * blockDim.x + threadIdx.x;
+    ef2 L[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) L[q] = lo_tab[q][i & lmask];
+    ef2 blo, bhi;
+    if (TAU == 0) {
+      const uint32_t xlo = j.bits[i], xhi = j.bits[i + half];
+      if (F == 0) {
+        if (xlo != xhi) acc[0] = ef_add(acc[0], L[0]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < NT; q++) {
+          if (xlo) acc[2 * q] = ef_add(acc[2 * q], L[q]);
+          if (xhi) acc[2 * q + 1] = ef_add(acc[2 * q + 1], L[q]);
+        }
+      }
+      blo = xlo ? one : ef_zero();
+      bhi = xhi ? one : ef_zero();
+    } else {
+      uint32_t mlo = 0, mhi = 0;
+#pragma unroll
+      for (int y = 0; y < NY; y++) {
+        mlo |= (uint32_t)j.bits[(uint64_t)y * stride + i] << y;
+        mhi |= (uint32_t)j.bits[(uint64_t)y * stride + i + half] << y;
+      }
+      blo = wl[mlo];
+      bhi = wl[mhi];
+      const ef2 db = ef_sub(bhi, blo);
+      if (F == 0) {
+        const ef2 c0 = ef_mul(blo, ef_sub(blo, one));
+        const ef2 c1 = ef_mul(db, ef_sub(ef_add(blo, blo), one));
+        const ef2 c2 = ef_mul(db, db);
+        acc[0] = ef_add(acc[0], ef_mul(L[0], c0));
+        acc[1] = ef_add(acc[1], ef_mul(L[0], c1));
+        acc[2] = ef_add(acc[2], ef_mul(L[0], c2));
+      } else {
+#pragma unroll
+        for (int q = 0; q < NT; q++) {
+          acc[2 * q] = ef_add(acc[2 * q], ef_mul(L[q], blo));
+          acc[2 * q + 1] = ef_add(acc[2 * q + 1], ef_mul(L[q], bhi));
+        }
+      }
+    }
+    if (bout) {
+      bout[i] = ef_add(blo, ef_mul(r, ef_sub(bhi, blo)));
+      ef2 e = ef_mul(L[0], Hr[0]);
+#pragma unroll
+      for (int q = 1; q < NT; q++) e = ef_add(e, ef_mul(L[q], Hr[q]));
+      eout[i] = e;
+    }
+  }
+  block_sum<6>(acc, sh);
+  if (threadIdx.x == 0) {
+    ef2 s[4];
+    if (F == 0 && TAU == 0) {
+      const ef2 L = ef_mul(Ha[0], acc[0]), H = ef_mul(Hb[0], acc[0]);
+      const ef2 h2 = ef_add(H, H), l2 = ef_add(L, L);
+      const ef2 s2 = ef_sub(h2, L), s3 = ef_sub(ef_add(h2, H), l2);
+      const ef2 s3x2 = ef_add(s3, s3);
+      s[0] = ef_zero();
+      s[1] = ef_zero();
+      s[2] = ef_add(s2, s2);
+      s[3] = ef_add(s3x2, ef_add(s3x2, s3x2));
+    } else if (F == 0) {
+      const ef2 dH = ef_sub(Hb[0], Ha[0]);
+      ef2 H = Ha[0], tt = ef_zero();  // H(t), t as a field element
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        // C0 + t C1 + t^2 C2
+        const ef2 poly = ef_add(acc[0], ef_mul(tt, ef_add(acc[1], ef_mul(tt, acc[2]))));
+        s[t] = ef_mul(H, poly);
+        H = ef_add(H, dH);
+        tt = ef_add(tt, one);
+      }
+    } else if (TAU == 0) {
+      ef2 A0 = ef_zero(), A1 = ef_zero(), C = ef_zero();
+#pragma unroll
+      for (int q = 0; q < NT; q++) {
+        A0 = ef_add(A0, ef_mul(Ha[q], acc[2 * q]));
+        A1 = ef_add(A1, ef_mul(Hb[q], acc[2 * q + 1]));
+        C = ef_add(C, ef_add(ef_mul(Ha[q], acc[2 * q + 1]), ef_mul(Hb[q], acc[2 * q])));
+      }
+      const ef2 a1x2 = ef_add(A1, A1), cx2 = ef_add(C, C);
+      s[0] = A0;
+      s[1] = A1;
+      s[2] = ef_add(ef_sub(ef_add(a1x2, a1x2), cx2), A0);
+      s[3] = ef_zero();
+    } else {
+#pragma unroll
+      for (int t = 0; t < 3; t++) s[t] = ef_zero();
+      s[3] = ef_zero();
+#pragma unroll
+      for (int q = 0; q < NT; q++) {
+        const ef2 dH = ef_sub(Hb[q], Ha[q]), dA = ef_sub(acc[2 * q + 1], acc[2 * q]);
+        ef2 H = Ha[q], A = acc[2 * q];
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+          s[t] = ef_add(s[t], ef_mul(H, A));
+          H = ef_add(H, dH);
+          A = ef_add(A, dA);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q <= D; q++) j.part[(uint64_t)blk * (D + 1) + q] = s[q];
   }
 }
 
-// First round of sum_x b(x) G(x) with b read as 0/1 words (the batched bit opening,
-// ligesis.rs:513-522): s(0) = sum_{lo = 1} G_lo, s(1) = sum_{hi = 1} G_hi and
-// s(2) = sum (2 hi - lo)(2 G_hi - G_lo) = 4 s(1) - 2 C + s(0) with
-// C = sum_{hi = 1} G_lo + sum_{lo = 1} G_hi.
-__device__ __forceinline__ void sc_bits2_body(const uint8_t* __restrict__ bits, ef2* __restrict__ b_out,
-                                              const EqTable* __restrict__ eqt, ef2* __restrict__ g, uint64_t half,
-                                              const ef2* __restrict__ r_ptr, ef2* __restrict__ part, uint32_t blk,
-                                              uint32_t nblk, ef2* sh) {
-  const ef2 r = *r_ptr, one = ef_one();
-  const ef2 omr = ef_sub(one, r);
-  const EqTable T = *eqt;
-  ef2 A0 = ef_zero(), A1 = ef_zero(), C = ef_zero();
-  for (uint64_t i = (uint64_t)blk * blockDim.x + threadIdx.x; i < half; i += (uint64_t)nblk * blockDim.x) {
-    const uint32_t blo = bits[i], bhi = bits[i + half];
-    const ef2 glo = eq_table_at(T, i), ghi = eq_table_at(T, i + half);
-    g[i] = ef_add(glo, ef_mul(r, ef_sub(ghi, glo)));
-    if (blo) {
-      A0 = ef_add(A0, glo);
-      C = ef_add(C, ghi);
-    }
-    if (bhi) {
-      A1 = ef_add(A1, ghi);
-      C = ef_add(C, glo);
-    }
-    b_out[i] = blo == bhi ? (blo ? one : ef_zero()) : (bhi ? r : omr);
+// The bits rounds of one step (every job in the launch is at round TAU): a kernel of their own,
+// so that each launch carries two small bodies instead of the whole round code.
+template <int TAU>
+__global__ __launch_bounds__(256) void sc_bits_group_kernel(ScStep a) {
+  __shared__ ef2 sh[256 * 4];
+  __shared__ ef2 wl[256];  // subset sums of the prefix weights
+  if (blockIdx.x >= a.round_blocks) {
+    red_job_body(a.r[blockIdx.x - a.round_blocks], sh);
+    return;
   }
-  const ef2 a1x2 = ef_add(A1, A1), cx2 = ef_add(C, C);
-  ef2 acc[3] = {A0, A1, ef_add(ef_sub(ef_add(a1x2, a1x2), cx2), A0)};
-  block_sum<3>(acc, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < 3; q++) part[(uint64_t)blk * 3 + q] = acc[q];
+  int lo = 0, hi = (int)a.nj - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (a.j[mid].first_block <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
   }
+  lo = __builtin_amdgcn_readfirstlane(lo);  // uniform: the job's fields load as scalars
+  const ScRoundJob& j = a.j[lo];
+  const uint32_t blk = blockIdx.x - j.first_block;
+  if (j.d == 3) sc_bits_body<TAU, 0>(j, blk, wl, sh);
+  else sc_bits_body<TAU, 1>(j, blk, wl, sh);
+}
+
+hipError_t launch_sc_bits_group(const ScStep& step, uint32_t tau, hipStream_t st) {
+  const dim3 grid(step.round_blocks + step.nr), blk(256);
+  switch (tau) {
+    case 0: hipLaunchKernelGGL(sc_bits_group_kernel<0>, grid, blk, 0, st, step); break;
+    case 1: hipLaunchKernelGGL(sc_bits_group_kernel<1>, grid, blk, 0, st, step); break;
+    case 2: hipLaunchKernelGGL(sc_bits_group_kernel<2>, grid, blk, 0, st, step); break;
+    default: hipLaunchKernelGGL(sc_bits_group_kernel<3>, grid, blk, 0, st, step); break;
+  }
+  return hipGetLastError();
 }
 
 // One round of several independent sumchecks in one launch (sumchecks run in lockstep): block
@@ -550,12 +671,7 @@ __global__ __launch_bounds__(256) void sc_round_group_kernel(ScStep a) {
   }
   const ScRoundJob& j = a.j[lo];
   const uint32_t blk = blockIdx.x - j.first_block;
-  if (j.bits) {
-    if (j.d == 3)
-      sc_bits3_body(j.bits, j.p[0], j.eqt, j.p[2], j.half, j.r, j.part, blk, j.nblk, sh);
-    else
-      sc_bits2_body(j.bits, j.p[0], j.eqt, j.p[1], j.half, j.r, j.part, blk, j.nblk, sh);
-  } else if (j.d == 3) {
+  if (j.d == 3) {
     if (j.m1) sc_round_body<3, true>(j.p[0], nullptr, j.p[2], j.half, j.r, j.part, blk, j.nblk, sh);
     else sc_round_body<3, false>(j.p[0], j.p[1], j.p[2], j.half, j.r, j.part, blk, j.nblk, sh);
   } else {

@@ -30,14 +30,20 @@ struct ScRoundJob {
   uint64_t half;
   const ef2* r;
   ef2* part;  // nblk x (d + 1) partial sums, reduced into the round's d + 1 values later
-  // bits: first round of a sumcheck whose first table is 0/1 valued, read from these bytes
-  // (2 half entries), and whose last table is the eq table eqt, evaluated from its factors
-  // rather than read; the folded bit table goes to p[0], the folded eq table to p[d - 1]. Null
-  // for an ordinary round.
+  // bits: round tau (< 4) of a sumcheck whose first table is 0/1 valued, read from these bytes
+  // (2^(tau + 1) half entries, before any fold), and whose last table is an eq table evaluated
+  // from the factors eqt (for round tau: its high factor already carries the fold of the tau
+  // bound variables; 2^klo >= 256 kpt), never read from memory. After tau rounds the first table
+  // at x is sum_y W(y) bits[y 2^(nv - tau) + x] over the tau-bit prefixes y, W(y) the products
+  // of c_t or 1 - c_t; wts[m] is the sum of W(y) over the set bits y of m (m < 2^(2^tau)). When p[0] is set the round also folds both tables into p[0]
+  // and p[d - 1] for the rounds that read tables. d = 3: the binary check (one eq term); d = 2:
+  // the batched opening (up to three). Null bits: an ordinary round.
   const uint8_t* bits;
   const EqTable* eqt;
+  const ef2* wts;
   // m1: d = 3 and the second table is the first minus one (not stored: p[1] unused)
-  uint32_t d, first_block, nblk, m1;
+  // kpt: pairs per thread of a bits round (its blocks cover 256 kpt consecutive pairs)
+  uint32_t d, first_block, nblk, m1, tau, kpt;
 };
 // One grouped round step, passed by value as the kernel's argument (no upload): jobs j[0..nj)
 // own blocks [0, round_blocks); the blocks after them run the reductions r[0..nr) (one each).
@@ -47,7 +53,9 @@ struct ScStep {
   RedJob r[kScMaxReds];
   uint32_t nj, nr, round_blocks, pad;
 };
-hipError_t launch_sc_round_group(const ScStep& step, hipStream_t st);
+hipError_t launch_sc_round_group(const ScStep& step, hipStream_t st);  // no bits jobs
+// A step of bits jobs only, all at round tau.
+hipError_t launch_sc_bits_group(const ScStep& step, uint32_t tau, hipStream_t st);
 
 // The remaining rounds (len <= 2^kScTailLog entries) of one sumcheck, one workgroup per job.
 constexpr int kScTailLog = 10;
