@@ -27,18 +27,13 @@ hipStream_t Context::side_stream() {
   return stream2;
 }
 
-hipStream_t Context::third_stream() {
-  if (!stream3) hip_check(hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking), "third stream");
-  return stream3;
-}
-
 void Context::drain() {
-  for (hipStream_t s : {stream, stream2, stream3})
+  for (hipStream_t s : {stream, stream2})
     if (s) (void)hipStreamSynchronize(s);
 }
 
 void Context::destroy_streams() {
-  for (hipStream_t* s : {&stream, &stream2, &stream3})
+  for (hipStream_t* s : {&stream, &stream2})
     if (*s) {
       (void)hipStreamDestroy(*s);
       *s = nullptr;

@@ -569,15 +569,13 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
             "a-bar");
   hip_check(launch_dot(a_vec, eqz2, n, part, st), "y");
 
-  // ---- the Basefold commitment of a-bar (ligesis.rs:384) on a third stream, once a-bar is
-  //      there, so that it runs beside B_I's; Stage B on the second stream waits for it ----
+  // ---- the Basefold commitment of a-bar (ligesis.rs:384) on the second stream, after B_I's,
+  //      once a-bar is there; then Stage B. (A third stream for it gained nothing alone and
+  //      costs hardware queues when several contexts share the device.) ----
   hip_check(hipEventRecord(ctx.ev[9], st), "a-bar ready");
-  hipStream_t s3 = ctx.third_stream();
-  hip_check(hipStreamWaitEvent(s3, ctx.ev[9], 0), "a-bar ready");
-  enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec, true, log_n, li_a, 1, s3);
-  mark(8, s3);
-  hip_check(hipEventRecord(ctx.ev[7], s3), "a-bar committed");
-  hip_check(hipStreamWaitEvent(sb, ctx.ev[7], 0), "a-bar committed");
+  hip_check(hipStreamWaitEvent(sb, ctx.ev[9], 0), "a-bar ready");
+  enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec, true, log_n, li_a, 1, sb);
+  mark(8, sb);
   const DevBasefold* states[3] = {&key.a_state, &bi_state, &av_state};
   const uint64_t* polys[3] = {key.a.ptr, bi, (const uint64_t*)a_vec};
   for (int c = 0; c < 3; c++)

@@ -205,7 +205,6 @@ struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // second stream (default priority) for work that overlaps `stream`
-  hipStream_t stream3 = nullptr;  // third stream (default priority), for a prove's a-bar commitment
   std::string err;
   DevicePool pool;
   std::map<int, DevBuf<uint64_t>> twiddles;
@@ -238,8 +237,7 @@ struct Context {
 
   const uint64_t* twiddle(int logN);
   uint8_t* staging(size_t bytes);
-  hipStream_t side_stream();   // stream2, created on first use
-  hipStream_t third_stream();  // stream3, created on first use
+  hipStream_t side_stream();  // stream2, created on first use
   void drain();                // waits for every stream, ignoring errors (error and teardown paths)
   void destroy_streams();
   hipEvent_t pipe_ev[10] = {};  // cross-stream events of a pipelined commit, created on first use
