@@ -108,6 +108,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="batch workload: number of polynomials")
     ap.add_argument("--requests", type=int, default=96, help="serve workload: number of requests")
     ap.add_argument("--prove-workers", type=int, default=2, help="serve workload: prove worker threads")
+    ap.add_argument("--max-batch", type=int, default=4, help="serve workload: requests per batched commit")
     ap.add_argument("--log-m", type=int, default=8, help="prove workload: log2 rows")
     ap.add_argument("--prove-graph", type=int, default=0,
                     help="prove workload: replay repeated proofs as a captured graph (1) or not (0)")
@@ -180,7 +181,7 @@ def bench_serve(args):
         return lat, time.perf_counter() - t0
 
     # one server for every pass (as in service: its contexts, pools and twiddles stay warm)
-    srv = L.Server(0, hbm_budget=64 << 30, prove_workers=args.prove_workers, max_batch=16)
+    srv = L.Server(0, hbm_budget=64 << 30, prove_workers=args.prove_workers, max_batch=args.max_batch)
 
     def served(arrivals):
         b0 = srv.stats()["batches"]
@@ -253,7 +254,8 @@ def bench_serve(args):
         "dtype": "u64 / F_p^2 (Goldilocks)",
         "data": "synthetic (seeded f per shape in pinned host memory, seeded z and oracle streams)",
         "config": {"workload": f"{nreq} commit-then-open requests, mu uniform in [14,26], "
-                               f"log_m = floor((mu-7)/2), HBM budget 64 GiB, {args.prove_workers} prove workers",
+                               f"log_m = floor((mu-7)/2), HBM budget 64 GiB, {args.prove_workers} prove workers, "
+                               f"batches of up to {args.max_batch}",
                    "mus": mus},
         "saturation": {"served_polys_per_s": nreq / span_sv, "per_request_polys_per_s": nreq / span_pr,
                        "served_p50_ms": pct(lat_sv, 50), "served_p99_ms": pct(lat_sv, 99),
