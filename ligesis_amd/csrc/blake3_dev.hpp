@@ -1,9 +1,9 @@
-// BLAKE3 compression for device code, written from the public BLAKE3 specification
-// (7 rounds of the G mixing function over a 16-word state; message word permutation
-// 2,6,3,10,7,0,4,13,1,11,12,5,9,14,15,8 between rounds). Every message hashed on this path is
-// at most 1024 bytes, i.e. one chunk: its blocks are compressed in order with chunk counter 0,
-// CHUNK_START on the first block and CHUNK_END | ROOT on the last, which yields blake3(msg)
-// as called by utils/src/merkle.rs:10-12.
+// BLAKE3 compression for device code, written from the public BLAKE3 specification (7 rounds
+// of the G mixing function over a 16-word state; message word permutation
+// 2,6,3,10,7,0,4,13,1,11,12,5,9,14,15,8 between rounds; IV = the SHA-256 initial words). Every
+// message hashed on this path is at most 1024 bytes, i.e. one chunk: its blocks are compressed in
+// order with chunk counter 0, CHUNK_START on the first block and CHUNK_END | ROOT on the last,
+// which yields blake3(msg) as called by utils/src/merkle.rs:10-12.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -12,89 +12,73 @@ namespace lgs {
 
 constexpr uint32_t B3_CHUNK_START = 1, B3_CHUNK_END = 2, B3_ROOT = 8;
 
-__host__ __device__ __forceinline__ uint32_t b3_iv(int i) {
-  switch (i) {
-    case 0: return 0x6A09E667u;
-    case 1: return 0xBB67AE85u;
-    case 2: return 0x3C6EF372u;
-    case 3: return 0xA54FF53Au;
-    case 4: return 0x510E527Fu;
-    case 5: return 0x9B05688Cu;
-    case 6: return 0x1F83D9ABu;
-    default: return 0x5BE0CD19u;
-  }
-}
-
 __device__ __forceinline__ uint32_t b3_rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
 
-#define LGS_B3_G(a, b, c, d, x, y)   \
-  do {                               \
-    s[a] = s[a] + s[b] + (x);        \
-    s[d] = b3_rotr(s[d] ^ s[a], 16); \
-    s[c] = s[c] + s[d];              \
-    s[b] = b3_rotr(s[b] ^ s[c], 12); \
-    s[a] = s[a] + s[b] + (y);        \
-    s[d] = b3_rotr(s[d] ^ s[a], 8);  \
-    s[c] = s[c] + s[d];              \
-    s[b] = b3_rotr(s[b] ^ s[c], 7);  \
+// ---- four lanes per compression ----
+// The lanes of a quad (lanes 4k .. 4k + 3 of a wave, c = lane & 3) share one compression: lane c
+// holds column c of the state (s[c], s[4 + c], s[8 + c], s[12 + c]) and runs the column mixing
+// G(c) itself; for the diagonal mixings the rows rotate across the quad (DPP quad permutations),
+// lane c runs G on (s[c], s[4 + (c+1)%4], s[8 + (c+2)%4], s[12 + (c+3)%4]), and the rows rotate
+// back. Every lane holds the whole message block and picks its words by its column.
+
+// word i of the message after r permutations is word b3_sched(r, i) of the block
+__host__ __device__ constexpr int b3_sched(int r, int i) {
+  constexpr int perm[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+  for (int k = 0; k < r; k++) i = perm[i];
+  return i;
+}
+
+__device__ __forceinline__ uint32_t b3_sel4(uint32_t c, uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3) {
+  return c < 2 ? (c == 0 ? v0 : v1) : (c == 2 ? v2 : v3);
+}
+
+// quad permutation: lane i of the quad receives lane p_i (ctrl = p0 | p1 << 2 | p2 << 4 | p3 << 6)
+template <int CTRL>
+__device__ __forceinline__ uint32_t b3_quad(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+#define LGS_B3_G4(x, y)           \
+  do {                            \
+    a = a + b + (x);              \
+    d = b3_rotr(d ^ a, 16);       \
+    cc = cc + d;                  \
+    b = b3_rotr(b ^ cc, 12);      \
+    a = a + b + (y);              \
+    d = b3_rotr(d ^ a, 8);        \
+    cc = cc + d;                  \
+    b = b3_rotr(b ^ cc, 7);       \
   } while (0)
 
-// cv <- first 8 words of compress(cv, m, counter = 0, block_len, flags). The message
-// schedule is applied by renaming (fully unrolled), so no permutation moves are executed.
-__device__ __forceinline__ void b3_compress(uint32_t cv[8], const uint32_t m_in[16],
-                                            uint32_t block_len, uint32_t flags) {
-  uint32_t s[16];
-#pragma unroll
-  for (int i = 0; i < 8; i++) s[i] = cv[i];
-  s[8] = 0x6A09E667u;
-  s[9] = 0xBB67AE85u;
-  s[10] = 0x3C6EF372u;
-  s[11] = 0xA54FF53Au;
-  s[12] = 0;
-  s[13] = 0;
-  s[14] = block_len;
-  s[15] = flags;
-  uint32_t m[16];
-#pragma unroll
-  for (int i = 0; i < 16; i++) m[i] = m_in[i];
+// Lane c's share of cv <- compress(cv, m, counter 0, block_len, flags): h0 = cv[c], h1 = cv[4 + c].
+// All four lanes of the quad must be active.
+__device__ __forceinline__ void b3_compress4(uint32_t& h0, uint32_t& h1, const uint32_t m[16], uint32_t block_len,
+                                             uint32_t flags, uint32_t c) {
+  uint32_t a = h0, b = h1;
+  uint32_t cc = b3_sel4(c, 0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au);
+  uint32_t d = b3_sel4(c, 0u, 0u, block_len, flags);
 #pragma unroll
   for (int r = 0; r < 7; r++) {
-    LGS_B3_G(0, 4, 8, 12, m[0], m[1]);
-    LGS_B3_G(1, 5, 9, 13, m[2], m[3]);
-    LGS_B3_G(2, 6, 10, 14, m[4], m[5]);
-    LGS_B3_G(3, 7, 11, 15, m[6], m[7]);
-    LGS_B3_G(0, 5, 10, 15, m[8], m[9]);
-    LGS_B3_G(1, 6, 11, 12, m[10], m[11]);
-    LGS_B3_G(2, 7, 8, 13, m[12], m[13]);
-    LGS_B3_G(3, 4, 9, 14, m[14], m[15]);
-    if (r < 6) {
-      uint32_t t[16] = {m[2], m[6], m[3], m[10], m[7], m[0], m[4], m[13],
-                        m[1], m[11], m[12], m[5], m[9], m[14], m[15], m[8]};
-#pragma unroll
-      for (int i = 0; i < 16; i++) m[i] = t[i];
-    }
+    LGS_B3_G4(b3_sel4(c, m[b3_sched(r, 0)], m[b3_sched(r, 2)], m[b3_sched(r, 4)], m[b3_sched(r, 6)]),
+              b3_sel4(c, m[b3_sched(r, 1)], m[b3_sched(r, 3)], m[b3_sched(r, 5)], m[b3_sched(r, 7)]));
+    b = b3_quad<0x39>(b);   // [1, 2, 3, 0]
+    cc = b3_quad<0x4E>(cc); // [2, 3, 0, 1]
+    d = b3_quad<0x93>(d);   // [3, 0, 1, 2]
+    LGS_B3_G4(b3_sel4(c, m[b3_sched(r, 8)], m[b3_sched(r, 10)], m[b3_sched(r, 12)], m[b3_sched(r, 14)]),
+              b3_sel4(c, m[b3_sched(r, 9)], m[b3_sched(r, 11)], m[b3_sched(r, 13)], m[b3_sched(r, 15)]));
+    b = b3_quad<0x93>(b);
+    cc = b3_quad<0x4E>(cc);
+    d = b3_quad<0x39>(d);
   }
-#pragma unroll
-  for (int i = 0; i < 8; i++) cv[i] = s[i] ^ s[i + 8];
+  h0 = a ^ cc;
+  h1 = b ^ d;
 }
 
-#undef LGS_B3_G
+#undef LGS_B3_G4
 
-__device__ __forceinline__ void b3_init(uint32_t cv[8]) {
-#pragma unroll
-  for (int i = 0; i < 8; i++) cv[i] = b3_iv(i);
-}
-
-// blake3(left || right) for two 32-byte digests.
-__device__ __forceinline__ void b3_parent(const uint32_t l[8], const uint32_t r[8], uint32_t out[8]) {
-  uint32_t m[16];
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    m[i] = l[i];
-    m[i + 8] = r[i];
-  }
-  b3_init(out);
-  b3_compress(out, m, 64, B3_CHUNK_START | B3_CHUNK_END | B3_ROOT);
+__device__ __forceinline__ void b3_init4(uint32_t& h0, uint32_t& h1, uint32_t c) {
+  h0 = b3_sel4(c, 0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au);
+  h1 = b3_sel4(c, 0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u);
 }
 
 }  // namespace lgs

@@ -5,30 +5,34 @@
 // concatenated, leaves first: layer k starts at sum_{t<k} L >> t, the root is the last node.
 // Leaf counts are powers of two on this path (2n, Basefold domains, folded levels).
 //
-// One workgroup of 2^g threads (g = 6..8, 8 in grouped launches) reduces up to 2^g nodes of a
-// layer by up to g levels in LDS, so a tree over L leaves takes ceil(log2(L) / g) launches; the
+// One workgroup of 4 x 2^g threads (g = 6..8, 8 in grouped launches) reduces up to 2^g nodes of
+// a layer by up to g levels in LDS, so a tree over L leaves takes ceil(log2(L) / g) launches; the
 // first launch also hashes the leaves.
 #include "blake3_dev.hpp"
 #include "kernels.hpp"
 
 namespace lgs {
 
-// Leaf p = words M[r * rs + p * cs] for r < rows (rows * 8 <= 1024 bytes: a single chunk).
-__device__ __forceinline__ void leaf_digest(const uint64_t* __restrict__ M, uint32_t rows,
-                                            uint64_t rs, uint64_t cs, uint64_t p, uint32_t cv[8]) {
-  b3_init(cv);
+// Every compression runs on the four lanes of a quad (b3_compress4): a workgroup of 4 W threads
+// holds W nodes, thread t works on node t / 4 as column t % 4.
+
+// Leaf p = words M[r * rs + p * cs] for r < rows (rows * 8 <= 1024 bytes: a single chunk);
+// lane c of the quad gets h0 = digest word c, h1 = word 4 + c.
+__device__ __forceinline__ void leaf_digest4(const uint64_t* __restrict__ M, uint32_t rows, uint64_t rs, uint64_t cs,
+                                             uint64_t p, uint32_t c, uint32_t& h0, uint32_t& h1) {
+  b3_init4(h0, h1, c);
   const uint32_t nblocks = (rows + 7) / 8;
   for (uint32_t b = 0; b < nblocks; b++) {
     uint32_t m[16];
     const uint32_t wcount = (rows - 8 * b) < 8 ? (rows - 8 * b) : 8;
 #pragma unroll
     for (uint32_t w = 0; w < 8; w++) {
-      uint64_t v = w < wcount ? M[(uint64_t)(8 * b + w) * rs + p * cs] : 0;
+      const uint64_t v = w < wcount ? M[(uint64_t)(8 * b + w) * rs + p * cs] : 0;
       m[2 * w] = (uint32_t)v;
       m[2 * w + 1] = (uint32_t)(v >> 32);
     }
-    uint32_t flags = (b == 0 ? B3_CHUNK_START : 0) | (b + 1 == nblocks ? (B3_CHUNK_END | B3_ROOT) : 0);
-    b3_compress(cv, m, wcount * 8, flags);
+    const uint32_t flags = (b == 0 ? B3_CHUNK_START : 0) | (b + 1 == nblocks ? (B3_CHUNK_END | B3_ROOT) : 0);
+    b3_compress4(h0, h1, m, wcount * 8, flags, c);
   }
 }
 
@@ -36,37 +40,44 @@ __device__ __forceinline__ void merkle_reduce_body(const uint64_t* __restrict__ 
                                                    uint64_t cs, int from_leaves, uint32_t* __restrict__ nodes,
                                                    uint64_t child_off, uint64_t child_count, int levels,
                                                    uint32_t blk, uint32_t* sh) {
-  const uint32_t t = threadIdx.x, width = blockDim.x;
+  const uint32_t node = threadIdx.x >> 2, c = threadIdx.x & 3, width = blockDim.x >> 2;
   const uint64_t base = (uint64_t)blk * width;
   uint32_t local_n = (uint32_t)((child_count - base) < width ? (child_count - base) : width);
-  if (t < local_n) {
-    uint32_t cv[8];
+  if (node < local_n) {  // whole quads: a quad's four lanes share the node
+    uint32_t h0, h1;
+    const uint64_t at = (child_off + base + node) * 8;
     if (from_leaves) {
-      leaf_digest(M, rows, rs, cs, base + t, cv);
-#pragma unroll
-      for (int i = 0; i < 8; i++) nodes[(child_off + base + t) * 8 + i] = cv[i];
+      leaf_digest4(M, rows, rs, cs, base + node, c, h0, h1);
+      nodes[at + c] = h0;
+      nodes[at + 4 + c] = h1;
     } else {
-#pragma unroll
-      for (int i = 0; i < 8; i++) cv[i] = nodes[(child_off + base + t) * 8 + i];
+      h0 = nodes[at + c];
+      h1 = nodes[at + 4 + c];
     }
-#pragma unroll
-    for (int i = 0; i < 8; i++) sh[t * 8 + i] = cv[i];
+    sh[node * 8 + c] = h0;
+    sh[node * 8 + 4 + c] = h1;
   }
   uint64_t off = child_off, count = child_count, blk_base = base;
   for (int lv = 0; lv < levels && local_n > 1; lv++) {
     __syncthreads();
     const uint32_t half = local_n / 2;
-    uint32_t out[8];
-    if (t < half) b3_parent(&sh[(2 * t) * 8], &sh[(2 * t + 1) * 8], out);
+    uint32_t h0 = 0, h1 = 0;
+    if (node < half) {  // parent = blake3(left || right)
+      uint32_t m[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++) m[i] = sh[(2 * node) * 8 + i];
+      b3_init4(h0, h1, c);
+      b3_compress4(h0, h1, m, 64, B3_CHUNK_START | B3_CHUNK_END | B3_ROOT, c);
+    }
     __syncthreads();
     const uint64_t poff = off + count;
     blk_base >>= 1;
-    if (t < half) {
-#pragma unroll
-      for (int i = 0; i < 8; i++) {
-        sh[t * 8 + i] = out[i];
-        nodes[(poff + blk_base + t) * 8 + i] = out[i];
-      }
+    if (node < half) {
+      sh[node * 8 + c] = h0;
+      sh[node * 8 + 4 + c] = h1;
+      const uint64_t at = (poff + blk_base + node) * 8;
+      nodes[at + c] = h0;
+      nodes[at + 4 + c] = h1;
     }
     off = poff;
     count >>= 1;
@@ -74,15 +85,15 @@ __device__ __forceinline__ void merkle_reduce_body(const uint64_t* __restrict__ 
   }
 }
 
-__global__ __launch_bounds__(256) void merkle_reduce_kernel(
+__global__ __launch_bounds__(1024) void merkle_reduce_kernel(
     const uint64_t* __restrict__ M, uint32_t rows, uint64_t rs, uint64_t cs, int from_leaves,
     uint32_t* __restrict__ nodes, uint64_t child_off, uint64_t child_count, int levels) {
   __shared__ uint32_t sh[256 * 8];
   merkle_reduce_body(M, rows, rs, cs, from_leaves, nodes, child_off, child_count, levels, blockIdx.x, sh);
 }
 
-// One reduction round of many trees in one launch (batched commit).
-__global__ __launch_bounds__(256) void merkle_group_kernel(const MerkleJob* __restrict__ jobs, int njobs) {
+// One reduction round of many trees in one launch (batched commit): 256 nodes per workgroup.
+__global__ __launch_bounds__(1024) void merkle_group_kernel(const MerkleJob* __restrict__ jobs, int njobs) {
   __shared__ uint32_t sh[256 * 8];
   int lo = 0, hi = njobs - 1;
   while (lo < hi) {
@@ -96,7 +107,7 @@ __global__ __launch_bounds__(256) void merkle_group_kernel(const MerkleJob* __re
 }
 
 hipError_t launch_merkle_group(const MerkleJob* jobs, int njobs, uint32_t total_blocks, hipStream_t st) {
-  if (total_blocks) hipLaunchKernelGGL(merkle_group_kernel, dim3(total_blocks), dim3(256), 0, st, jobs, njobs);
+  if (total_blocks) hipLaunchKernelGGL(merkle_group_kernel, dim3(total_blocks), dim3(1024), 0, st, jobs, njobs);
   return hipGetLastError();
 }
 
@@ -117,7 +128,7 @@ hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t
   while (first || count > 1) {
     int lv = total_levels - done < lg_block ? total_levels - done : lg_block;
     uint32_t blocks = (uint32_t)((count + width - 1) / width);
-    hipLaunchKernelGGL(merkle_reduce_kernel, dim3(blocks), dim3(width), 0, st, M, rows, rs, cs,
+    hipLaunchKernelGGL(merkle_reduce_kernel, dim3(blocks), dim3(4 * width), 0, st, M, rows, rs, cs,
                        first ? 1 : 0, nodes, off, count, lv);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

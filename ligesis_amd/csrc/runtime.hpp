@@ -140,7 +140,7 @@ struct Tuning {
   int hash_impl = 12;    // MFMA kernel variant (12: asynchronous copies into a 4-slab LDS ring)
   int hash_ksplit = 0;   // slabs of the K (= 64 m) dimension; 0: heuristic
   int ntt_impl = 0;      // row transform variant
-  int merkle_lg = 8;     // log2 threads per Merkle workgroup
+  int merkle_lg = 7;     // log2 nodes per Merkle workgroup (four threads per node)
   int pipeline = 1;      // row chunks whose transform overlaps the hash of the previous chunk (1: off)
 };
 
@@ -218,7 +218,7 @@ struct Context {
                          // 8 / 9 64x128 wave tiles in 4 / 8-wave workgroups, 10 as 0 with A
                          // fragments read ahead, 11 / 12 asynchronous copies into an LDS ring
   int ntt_impl = -1;     // -1 auto; row transform: 0/2/3 radix-8 with 256/512/1024 threads, 1 radix-2
-  int merkle_lg = 0;     // 0 auto; log2 threads per Merkle workgroup = levels per launch (6..8)
+  int merkle_lg = 0;     // 0 auto; log2 nodes per Merkle workgroup = levels per launch (6..8)
   int pipeline = 0;      // 0 auto; 1 off, 2 / 4 / 8 row chunks (transform on stream, hash on stream2)
   // replay repeated proofs of a commitment as a captured graph (off by default: on this ROCm the
   // graph ran the two streams' steps in capture order, slower than issuing them directly)
