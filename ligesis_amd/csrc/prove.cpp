@@ -296,6 +296,16 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
           if ((mm >> y) & 1) acc = ef_add(acc, w[y]);
         sums[mm] = acc;
       }
+      if (f == 0 && tau <= 2) {  // binary check: (c0, c1, c2) for every (lo, hi) bit pattern
+        const size_t nm = sums.size();
+        for (size_t ml = 0; ml < nm; ml++)
+          for (size_t mh = 0; mh < nm; mh++) {
+            const ef2 bl = sums[ml], db = ef_sub(sums[mh], bl);
+            sums.push_back(ef_mul(bl, ef_sub(bl, ef_one())));
+            sums.push_back(ef_mul(db, ef_sub(ef_add(bl, bl), ef_one())));
+            sums.push_back(ef_mul(db, db));
+          }
+      }
       o_bw[f][tau] = put(sums);
       for (uint32_t q = 0; q < bits_nterms[f]; q++) {
         const EfV& pt = *bits_pt[f][q];
@@ -656,7 +666,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   auto round_shape = [&](int i, uint32_t t, uint32_t& nblk, uint32_t& kpt) {
     const uint64_t half = uint64_t(1) << (sc[i].nv - 1 - t);
     if ((i == 0 || i == 5) && t < bits_rounds) {
-      kpt = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, half >> 16), (uint64_t(1) << b_klo) >> 8);
+      kpt = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(1, half >> 17), (uint64_t(1) << b_klo) >> 8);
       nblk = (uint32_t)(half / (256 * (uint64_t)kpt));
     } else {
       kpt = 0;

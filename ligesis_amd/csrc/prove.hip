@@ -482,8 +482,13 @@ template <int TAU, int F>
 __device__ __forceinline__ void sc_bits_body(const ScRoundJob& j, uint32_t blk, ef2* wl, ef2* sh) {
   constexpr int NY = 1 << TAU, NM = 1 << NY;
   constexpr int D = F ? 2 : 3;
-  if (TAU > 0) {  // wts holds the subset sums (prepared on the host), one coalesced copy
-    for (uint32_t m = threadIdx.x; m < (uint32_t)NM; m += blockDim.x) wl[m] = j.wts[m];
+  // binary check, rounds 1 and 2: the quadratic's coefficients (c0, c1, c2) for every pair of
+  // prefix-bit patterns follow the subset sums in wts (host-made), so a pair costs three products
+  constexpr bool CT = F == 0 && (TAU == 1 || TAU == 2);
+  constexpr int NWL = NM + (CT ? 3 * NM * NM : 0);
+  static_assert(NWL <= 1024, "LDS table size");
+  if (TAU > 0) {  // one coalesced copy of the host-made tables
+    for (uint32_t m = threadIdx.x; m < (uint32_t)NWL; m += blockDim.x) wl[m] = j.wts[m];
     __syncthreads();
   }
   // terms: the binary check's eq table has one, the batched opening's three (checked on the
@@ -505,13 +510,18 @@ __device__ __forceinline__ void sc_bits_body(const ScRoundJob& j, uint32_t blk, 
   const ef2 r = *j.r, one = ef_one();
   ef2* __restrict__ bout = j.p[0];
   ef2* __restrict__ eout = j.p[D - 1];
-  ef2 Hr[NT];  // high factors folded with r (the next round's eq table)
+  ef2 Hr[NT];  // high factors folded with r (the next round's eq table; the last bits round)
 #pragma unroll
-  for (int q = 0; q < NT; q++) Hr[q] = ef_add(Ha[q], ef_mul(r, ef_sub(Hb[q], Ha[q])));
+  for (int q = 0; q < NT; q++) Hr[q] = ef_zero();
+  if (bout) {
+#pragma unroll
+    for (int q = 0; q < NT; q++) Hr[q] = ef_add(Ha[q], ef_mul(r, ef_sub(Hb[q], Ha[q])));
+  }
   // accumulators: F = 0: [C0, C1, C2] (round 0: [U]); F = 1: [A_q, B_q] per term
-  ef2 acc[6];
+  constexpr int NA = F ? 2 * NT : (TAU ? 3 : 1);
+  ef2 acc[NA];
 #pragma unroll
-  for (int q = 0; q < 6; q++) acc[q] = ef_zero();
+  for (int q = 0; q < NA; q++) acc[q] = ef_zero();
   for (uint32_t k = 0; k < j.kpt; k++) {
     const uint64_t i = base + (uint64_t)k * blockDim.x + threadIdx.x;
     ef2 L[NT];
@@ -541,7 +551,12 @@ __device__ __forceinline__ void sc_bits_body(const ScRoundJob& j, uint32_t blk, 
       blo = wl[mlo];
       bhi = wl[mhi];
       const ef2 db = ef_sub(bhi, blo);
-      if (F == 0) {
+      if (CT) {
+        const ef2* ce = &wl[NM + 3 * (mlo * NM + mhi)];
+        acc[0] = ef_add(acc[0], ef_mul(L[0], ce[0]));
+        acc[1] = ef_add(acc[1], ef_mul(L[0], ce[1]));
+        acc[2] = ef_add(acc[2], ef_mul(L[0], ce[2]));
+      } else if (F == 0) {
         const ef2 c0 = ef_mul(blo, ef_sub(blo, one));
         const ef2 c1 = ef_mul(db, ef_sub(ef_add(blo, blo), one));
         const ef2 c2 = ef_mul(db, db);
@@ -564,7 +579,7 @@ __device__ __forceinline__ void sc_bits_body(const ScRoundJob& j, uint32_t blk, 
       eout[i] = e;
     }
   }
-  block_sum<6>(acc, sh);
+  block_sum<NA>(acc, sh);
   if (threadIdx.x == 0) {
     ef2 s[4];
     if (F == 0 && TAU == 0) {
@@ -626,7 +641,7 @@ __device__ __forceinline__ void sc_bits_body(const ScRoundJob& j, uint32_t blk, 
 template <int TAU>
 __global__ __launch_bounds__(256) void sc_bits_group_kernel(ScStep a) {
   __shared__ ef2 sh[256 * 4];
-  __shared__ ef2 wl[256];  // subset sums of the prefix weights
+  __shared__ ef2 wl[1024];  // subset sums of the prefix weights (and the binary check's table)
   if (blockIdx.x >= a.round_blocks) {
     red_job_body(a.r[blockIdx.x - a.round_blocks], sh);
     return;
