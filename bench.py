@@ -136,7 +136,7 @@ def bench_serve(args):
     """Served mixed stream: `--requests` commit-then-open requests, mu uniform in [14, 26] with
     log_m = floor((mu - 7) / 2) (the reference's shape window), f in pinned host memory, eval
     points uniform in F_{p^2}^mu. Loads: saturation (everything queued at t = 0: polys/s) and
-    Poisson arrivals at 50 / 70 / 90 / 110 % of the per-request path's capacity (p50 / p99
+    Poisson arrivals at 50 / 70 / 80 / 90 / 100 / 110 % of the per-request path's capacity (p50 / p99
     latency). Each is run through the serving loop and through the plain per-request path
     (commit, then prove, one request at a time in arrival order)."""
     import threading
@@ -221,7 +221,7 @@ def bench_serve(args):
     # load (shared by both paths); "equal p99": the highest swept load at which each path keeps
     # its p99 within the per-request path's p99 at 70 % load.
     loads = []
-    for frac in (0.5, 0.7, 0.9, 1.1):
+    for frac in (0.5, 0.7, 0.8, 0.9, 1.0, 1.1):
         rate = frac * cap
         arrivals = list(np.cumsum(rng.exponential(1.0 / rate, size=nreq)))
         lat_pr_p, span_pr_p = per_request(arrivals)
