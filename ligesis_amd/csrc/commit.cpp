@@ -242,7 +242,8 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
     if (ctx.profile) hip_check(hipEventRecord(ctx.ev[3], sh), "event");
     hip_check(hipEventRecord(ctx.pipe_event(9), sh), "pipeline join");
     hip_check(hipStreamWaitEvent(ctx.stream, ctx.pipe_event(9), 0), "pipeline join");
-    hip_check(launch_merkle(c.h.ptr, 32, two_n, 1, two_n, c.tree.nodes.ptr, ctx.stream, tu.merkle_lg), "merkle");
+    hip_check(launch_merkle(c.h.ptr, 32, two_n, 1, two_n, c.tree.nodes.ptr, ctx.stream, tu.merkle_lg, c.root_host),
+              "merkle");
     if (ctx.profile) hip_check(hipEventRecord(ctx.ev[4], ctx.stream), "event");
     return;
   }
@@ -268,7 +269,8 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
   }
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[3], ctx.stream), "event");
   // column-wise Merkle tree over serialize(H[:, j]) (ligesis.rs:306-313)
-  hip_check(launch_merkle(c.h.ptr, 32, two_n, 1, two_n, c.tree.nodes.ptr, ctx.stream, tu.merkle_lg), "merkle");
+  hip_check(launch_merkle(c.h.ptr, 32, two_n, 1, two_n, c.tree.nodes.ptr, ctx.stream, tu.merkle_lg, c.root_host),
+            "merkle");
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[4], ctx.stream), "event");
 }
 

@@ -81,7 +81,10 @@ hipError_t launch_merkle_group(const MerkleJob* jobs, int njobs, uint32_t total_
 
 // merkle.hip: tree over nleaves (a power of two) leaves, leaf p = M[r*rs + p*cs], r < rows.
 // lg_block: log2 threads per workgroup (6..8) = levels reduced per launch.
+// root_out (optional, device-accessible, e.g. pinned host memory) receives the root's 8 words
+// from the launch that computes it.
 hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs,
-                         uint64_t nleaves, uint32_t* nodes, hipStream_t st, int lg_block = 8);
+                         uint64_t nleaves, uint32_t* nodes, hipStream_t st, int lg_block = 7,
+                         uint32_t* root_out = nullptr);
 
 }  // namespace lgs

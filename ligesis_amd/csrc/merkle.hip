@@ -39,7 +39,7 @@ __device__ __forceinline__ void leaf_digest4(const uint64_t* __restrict__ M, uin
 __device__ __forceinline__ void merkle_reduce_body(const uint64_t* __restrict__ M, uint32_t rows, uint64_t rs,
                                                    uint64_t cs, int from_leaves, uint32_t* __restrict__ nodes,
                                                    uint64_t child_off, uint64_t child_count, int levels,
-                                                   uint32_t blk, uint32_t* sh) {
+                                                   uint32_t blk, uint32_t* sh, uint32_t* root_out = nullptr) {
   const uint32_t node = threadIdx.x >> 2, c = threadIdx.x & 3, width = blockDim.x >> 2;
   const uint64_t base = (uint64_t)blk * width;
   uint32_t local_n = (uint32_t)((child_count - base) < width ? (child_count - base) : width);
@@ -83,13 +83,17 @@ __device__ __forceinline__ void merkle_reduce_body(const uint64_t* __restrict__ 
     count >>= 1;
     local_n = half;
   }
+  if (root_out && count == 1 && node == 0) {  // this launch reached the root (quad of node 0)
+    root_out[c] = sh[c];
+    root_out[4 + c] = sh[4 + c];
+  }
 }
 
 __global__ __launch_bounds__(1024) void merkle_reduce_kernel(
     const uint64_t* __restrict__ M, uint32_t rows, uint64_t rs, uint64_t cs, int from_leaves,
-    uint32_t* __restrict__ nodes, uint64_t child_off, uint64_t child_count, int levels) {
+    uint32_t* __restrict__ nodes, uint64_t child_off, uint64_t child_count, int levels, uint32_t* root_out) {
   __shared__ uint32_t sh[256 * 8];
-  merkle_reduce_body(M, rows, rs, cs, from_leaves, nodes, child_off, child_count, levels, blockIdx.x, sh);
+  merkle_reduce_body(M, rows, rs, cs, from_leaves, nodes, child_off, child_count, levels, blockIdx.x, sh, root_out);
 }
 
 // One reduction round of many trees in one launch (batched commit): 256 nodes per workgroup.
@@ -118,7 +122,7 @@ static int ilog2_u64(uint64_t x) {
 }
 
 hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs,
-                         uint64_t nleaves, uint32_t* nodes, hipStream_t st, int lg_block) {
+                         uint64_t nleaves, uint32_t* nodes, hipStream_t st, int lg_block, uint32_t* root_out) {
   if (lg_block < 6 || lg_block > 8) return hipErrorInvalidValue;
   const uint32_t width = 1u << lg_block;
   uint64_t off = 0, count = nleaves;
@@ -128,8 +132,9 @@ hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t
   while (first || count > 1) {
     int lv = total_levels - done < lg_block ? total_levels - done : lg_block;
     uint32_t blocks = (uint32_t)((count + width - 1) / width);
+    const bool last = total_levels - done <= lg_block;  // this launch writes the root
     hipLaunchKernelGGL(merkle_reduce_kernel, dim3(blocks), dim3(4 * width), 0, st, M, rows, rs, cs,
-                       first ? 1 : 0, nodes, off, count, lv);
+                       first ? 1 : 0, nodes, off, count, lv, last ? root_out : nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     for (int i = 0; i < lv; i++) {

@@ -275,6 +275,9 @@ struct Commitment {
   DevBuf<uint64_t> h;           // 32 x 2n
   DevTree tree;
   uint8_t root[32] = {0};
+  // when set (host-mapped pinned memory), the tree's last launch also writes the root there,
+  // so reading it back needs no copy after the tree
+  uint32_t* root_host = nullptr;
   // device state of the last proof on ctx, replayed as a graph by later proofs (see prove.cpp)
   mutable std::shared_ptr<ProvePlan> plan;
   mutable uint32_t prove_calls = 0;
