@@ -657,7 +657,8 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
     n_big[i] = sc[i].nv > kScTailLog ? sc[i].nv - kScTailLog : 0;  // rounds before the tail
     steps = std::max(steps, n_big[i]);
   }
-  if (n_big[0] == 0 || n_big[5] == 0) throw Error(2, "prove: B_I has fewer than 2^11 entries");
+  if (n_big[0] == 0 || n_big[5] == 0)
+    throw Error(2, "prove: B_I has fewer than 2^" + std::to_string(kScTailLog + 1) + " entries");
   // Partial sums alternate between two halves of scpart: the launch of step tau writes its half
   // while its extra blocks reduce the other half, written by step tau - 1 (the dot products'
   // partials for step 0); the tail launch reduces the last step's.
