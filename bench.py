@@ -514,14 +514,15 @@ def bench_commit(args):
                     12: "hash_mfma_dma_kernel<4>"}[schedule["hash_impl"]])
     hash_tag = f"hash_impl={schedule['hash_impl']} ksplit={schedule['hash_ksplit']} pipeline={schedule['pipeline']}"
     ntt_tag = f"ntt_impl={schedule['ntt_impl']}"
+    ntt_kernel = "ntt_r8_fixed_kernel<13>" if schedule["ntt_impl"] == 4 else "ntt_r8_kernel"
     if avg["hash_ms"] >= avg["ntt_ms"]:
         roof = {"kernel": f"{hash_kernel} (fused bit-decompose + H = A.B)", "bound": "mfma",
                 "achieved": hash_tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                 "frac": hash_tops / INT8_PEAK_TOPS, "traffic": pmc_traffic(hash_kernel, hash_tag)}
     else:
-        roof = {"kernel": "ntt_r8_kernel (row LDE)", "bound": "hbm", "achieved": ntt_gbs,
+        roof = {"kernel": f"{ntt_kernel} (row LDE)", "bound": "hbm", "achieved": ntt_gbs,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ntt_gbs / HBM_PEAK_GBS,
-                "traffic": pmc_traffic("ntt_r8_kernel", ntt_tag)}
+                "traffic": pmc_traffic(ntt_kernel, ntt_tag)}
     out = {
         "metric": "LigeSIS commit Melem/s at mu=24",
         "value": value,
@@ -542,7 +543,7 @@ def bench_commit(args):
         "stages_ms": avg,
         "roofline": roof,
         "ntt_roofline": {"bound": "hbm", "achieved": ntt_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": ntt_gbs / HBM_PEAK_GBS, "traffic": pmc_traffic("ntt_r8_kernel", ntt_tag)},
+                         "frac": ntt_gbs / HBM_PEAK_GBS, "traffic": pmc_traffic(ntt_kernel, ntt_tag)},
         "merkle_gbs": (32.0 * 8 * 2 * n) / (avg["merkle_ms"] / 1e3) / 1e9,
         "hash_rate_gelem_adds_per_s": (32.0 * 64 * m * 2 * n) / (avg["hash_ms"] / 1e3) / 1e9,
         "root": next(iter(roots)).hex(),

@@ -68,7 +68,8 @@ const char* lgs_last_error(const lgs_ctx* ctx);
  * fragment read from LDS one product group ahead, 11 / 12 A digits and F' words copied straight
  * into an LDS ring of 3 / 4 slabs),
  * "ntt_impl" (-1 auto; 0/2/3 radix-8 passes with 256/512/1024 threads per row, 1 radix-2
- * stages), "merkle_lg" (0 auto; 6..8 = log2 nodes per Merkle workgroup = tree levels per
+ * stages, 4 for rows of 2^12 values a 2^13-point kernel with every size fixed at compile time
+ * and per-pass twiddle tables, elsewhere as 3), "merkle_lg" (0 auto; 6..8 = log2 nodes per Merkle workgroup = tree levels per
  * launch), "pipeline" (0 auto; 1 off; 2 / 4 / 8: the rows are encoded in that many chunks and
  * each chunk's K-splits are hashed on a second stream while the next chunk is encoded),
  * "prove_graph" (0 default; 1: the second proof of a commitment on its own context captures

@@ -30,6 +30,7 @@ void ctx_unref(lgs::Context* c) {
   (void)hipSetDevice(c->device);
   c->drain();
   c->twiddles.clear();
+  c->pass_twiddles.clear();
   c->pool.trim();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -156,7 +157,7 @@ int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value) {
     ctx->c.hash_ksplit = (int)value;
   } else if (n == "hash_impl" && value >= -1 && value <= 12) {
     ctx->c.hash_impl = (int)value;
-  } else if (n == "ntt_impl" && value >= -1 && value <= 3) {
+  } else if (n == "ntt_impl" && value >= -1 && value <= 4) {
     ctx->c.ntt_impl = (int)value;
   } else if (n == "merkle_lg" && (value == 0 || (value >= 6 && value <= 8))) {
     ctx->c.merkle_lg = (int)value;

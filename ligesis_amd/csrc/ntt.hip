@@ -299,6 +299,96 @@ __global__ __launch_bounds__(1024) void ntt_r8_kernel(const uint64_t* __restrict
   ntt_r8_row(in + row * rs, es, n_in, logN, tw, 0, out + (row << logN), sm);
 }
 
+// Pass twiddles of a 2^L-point transform whose radix-8 passes start at s = 4 (L = 1 mod 3):
+// for s = 4, 7, ..., L - 3, section s holds tp[off_s + (t - 1) 2^s + j] = w_{2^(s+3)}^(j rev3(t))
+// (t = 1..7, j < 2^s) -- the factors pretwiddle<3> takes from tw -- so that the 64 lanes of a
+// wave (consecutive j) read 512 contiguous bytes per factor.
+__global__ void pass_twiddle_kernel(uint64_t* __restrict__ tp, const uint64_t* __restrict__ tw, int L) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t off = 0;
+  for (int s = 4; L - s >= 3; s += 3) {
+    const uint32_t M = 1u << s;
+    if (i < off + 7 * M) {
+      const uint32_t q = i - off, t = q / M + 1, j = q & (M - 1);
+      const uint32_t r = __brev(t) >> 29;
+      tp[i] = tw_full(tw, (j * r) << (L - s - 3), 1u << (L - 1), 0);
+      return;
+    }
+    off += 7 * M;
+  }
+}
+
+uint32_t pass_twiddle_entries(int L) {
+  uint32_t n = 0;
+  if ((L - 1) % 3 == 0)
+    for (int s = 4; L - s >= 3; s += 3) n += 7u << s;
+  return n;
+}
+
+hipError_t launch_pass_twiddles(uint64_t* tp, const uint64_t* tw, int L, hipStream_t st) {
+  const uint32_t n = pass_twiddle_entries(L);
+  if (n == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(pass_twiddle_kernel, dim3((n + 255) / 256), dim3(256), 0, st, tp, tw, L);
+  return hipGetLastError();
+}
+
+// The row LDE of rows of 2^(L-1) values (row stride rs, unit element stride) zero-padded to
+// 2^L points, one workgroup of 2^(L-3) threads per row: ntt_r8_row's zero-half schedule (stages
+// 2-4 fused with the load, then radix-8 passes from s = 4) with every size a compile-time
+// constant, one butterfly group per thread and pass, twiddles from the pass table tp, and each
+// LDS address one XOR: lds_pos is linear over GF(2), and a group's base and its offsets t 2^s
+// occupy disjoint bits, so lds_pos(base + t 2^s) = lds_pos(base) ^ lds_pos(t 2^s).
+template <int L>
+__global__ __launch_bounds__(1 << (L - 3)) void ntt_r8_fixed_kernel(const uint64_t* __restrict__ in, uint64_t rs,
+                                                                   const uint64_t* __restrict__ tp,
+                                                                   uint64_t* __restrict__ out) {
+  static_assert((L - 1) % 3 == 0 && L >= 10 && L <= LGS_LDS_LOG, "radix-8 passes from s = 4");
+  constexpr uint32_t N = 1u << L, E = N / 16;
+  constexpr int sh = L - 5;
+  extern __shared__ uint64_t sm[];
+  const uint64_t* __restrict__ src = in + (uint64_t)blockIdx.x * rs;
+  uint64_t* __restrict__ dst = out + ((uint64_t)blockIdx.x << L);
+  const uint32_t gi = threadIdx.x;
+  {
+    const uint32_t k = gi & (E - 1), j = gi / E;  // j uniform per half of the workgroup
+    uint64_t x[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) x[t] = src[k + (__brev((uint32_t)t) >> 29) * E];
+    if (j) {
+#pragma unroll
+      for (int t = 1; t < 8; t++) x[t] = gl_mul_pow2(x[t], (156u * (__brev((uint32_t)t) >> 29)) % 192u);
+    }
+    dft8(x);
+    const uint32_t pb = lds_pos(16 * brev_bits(k, L - 4) + j, sh);
+#pragma unroll
+    for (int t = 0; t < 8; t++) sm[pb ^ lds_pos(2 * t, sh)] = x[t];
+  }
+  __syncthreads();
+  uint32_t off = 0;
+#pragma unroll
+  for (int s = 4; s < L; s += 3) {
+    const uint32_t M = 1u << s;
+    const uint32_t j = gi & (M - 1);
+    const uint32_t base = ((gi >> s) << (s + 3)) | j;
+    const uint32_t pb = lds_pos(base, sh);
+    uint64_t x[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) x[t] = sm[pb ^ lds_pos(t << s, sh)];
+#pragma unroll
+    for (int t = 1; t < 8; t++) x[t] = gl_mul(x[t], tp[off + (t - 1) * M + j]);
+    dft8(x);
+    if (s == L - 3) {
+#pragma unroll
+      for (int t = 0; t < 8; t++) dst[base + t * M] = x[t];
+    } else {
+#pragma unroll
+      for (int t = 0; t < 8; t++) sm[pb ^ lds_pos(t << s, sh)] = x[t];
+      __syncthreads();
+    }
+    off += 7 * M;
+  }
+}
+
 // Transforms larger than one LDS tile: block (row, blk) computes the size-2^lt transform of the
 // subsequence x[c + k 2^(logN-lt)], c = rev_{logN-lt}(blk), which is what the first lt stages of
 // an in-place radix-2 DIT leave in positions [blk 2^lt, (blk + 1) 2^lt) of the bit-reversed
@@ -435,8 +525,17 @@ hipError_t launch_fill_field(uint64_t* p, uint64_t n, uint64_t seed, hipStream_t
 }
 
 hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_in, uint32_t rows,
-                      int logN, const uint64_t* tw, uint64_t* out, hipStream_t st, int impl) {
+                      int logN, const uint64_t* tw, uint64_t* out, hipStream_t st, int impl,
+                      const uint64_t* tp) {
   if (rows == 0) return hipSuccess;
+  if (impl == 4) {
+    if (logN == 13 && es == 1 && n_in == (1u << 12) && tp) {
+      hipLaunchKernelGGL(ntt_r8_fixed_kernel<13>, dim3(rows), dim3(1024), ntt_lds_bytes(1u << 13), st, in, rs, tp,
+                         out);
+      return hipGetLastError();
+    }
+    impl = 3;
+  }
   if (impl != 1 && logN >= 1 && logN <= LGS_LDS_LOG) {
     const uint32_t N = 1u << logN;
     const uint32_t want = impl == 2 ? 512 : impl == 3 ? 1024 : 256;

@@ -208,6 +208,7 @@ struct Context {
   std::string err;
   DevicePool pool;
   std::map<int, DevBuf<uint64_t>> twiddles;
+  std::map<int, DevBuf<uint64_t>> pass_twiddles;  // per-pass factor tables (ntt_impl 4)
   // Explicit launch-schedule options (lgs_set_option); unset fields fall back to the process
   // tuning table, then to the built-in defaults (see resolve).
   int hash_variant = 0;  // 0 auto, 1 MFMA, 2 VALU
@@ -236,6 +237,7 @@ struct Context {
   void free_host();  // pinned staging and arena
 
   const uint64_t* twiddle(int logN);
+  const uint64_t* pass_twiddle(int logN);  // null where the transform has no such table
   uint8_t* staging(size_t bytes);
   hipStream_t side_stream();  // stream2, created on first use
   void drain();                // waits for every stream, ignoring errors (error and teardown paths)

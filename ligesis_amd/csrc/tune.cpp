@@ -97,7 +97,7 @@ size_t tune_load(const char* path) {
     const Tuning& t = e.t;
     const bool ok = lm >= 0 && lm <= 20 && ln >= 1 && ln <= 26 && t.hash_variant >= 0 && t.hash_variant <= 2 &&
                     t.hash_impl >= 0 && t.hash_impl <= 12 && t.hash_ksplit >= 0 &&
-                    (t.hash_ksplit & (t.hash_ksplit - 1)) == 0 && t.ntt_impl >= 0 && t.ntt_impl <= 3 &&
+                    (t.hash_ksplit & (t.hash_ksplit - 1)) == 0 && t.ntt_impl >= 0 && t.ntt_impl <= 4 &&
                     t.merkle_lg >= 6 && t.merkle_lg <= 8 &&
                     (t.pipeline == 1 || t.pipeline == 2 || t.pipeline == 4 || t.pipeline == 8);
     if (!ok) throw Error(1, "tune: value out of range on line " + std::to_string(lineno) + " in " + path);
@@ -157,6 +157,7 @@ double autotune(Context& ctx, const Key& key, uint32_t reps, TuneEntry& best) {
       const uint64_t per_row = (uint64_t(1) << logN) / 8;  // radix-8 butterflies per pass
       if (per_row >= 512) cand.push_back(2);
       if (per_row >= 1024) cand.push_back(3);
+      if (logN == 13) cand.push_back(4);  // the compile-time-shaped 2^13 kernel
     }
     double bt = 1e30, base = 1e30;
     int bi = cur.ntt_impl;

@@ -58,6 +58,26 @@ def test_long_rows_match_oracle(gpu_ctx, shape):
         gpu_ctx.set_option("ntt_impl", -1)
 
 
+@pytest.mark.parametrize("shape", [(3, 12), (7, 12), (2, 11)])
+def test_every_row_transform_matches_oracle(gpu_ctx, shape):
+    """Every row-transform variant gives the oracle's F'; ntt_impl 4 (the compile-time 2^13
+    kernel with per-pass twiddle tables) runs at rows of 2^12 values and falls back to 3 elsewhere."""
+    log_m, log_n = shape
+    inp = synth.Inputs(log_m, log_n, seed=60 + log_m + log_n)
+    pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx, commit_only=True)
+    ocd = oracle.CommitData(oracle.ProvingKey(log_m, log_n, inp.a, commit_a=False), inp.f, keep_enc=True)
+    want = ocd.enc()
+    try:
+        for impl in (0, 1, 2, 3, 4):
+            print(f"row transform ntt_impl={impl} shape={shape}", flush=True)
+            gpu_ctx.set_option("ntt_impl", impl)
+            root, cd = Ligesis.commit(pk, inp.f)
+            np.testing.assert_array_equal(cd.encoded(), want)
+            assert root == ocd.root, impl
+    finally:
+        gpu_ctx.set_option("ntt_impl", -1)
+
+
 def test_mfma_and_valu_hash_agree(gpu_ctx):
     inp = synth.Inputs(5, 9, seed=3)
     pk, _ = Ligesis.setup(5, 9, inp.a, ctx=gpu_ctx, commit_only=True)
