@@ -199,11 +199,12 @@ __global__ __launch_bounds__(256) void dot_kernel(const ef2* __restrict__ a, con
 // product at t = q (q <= D) and folds table q (q < D); the folded values are written back in
 // place after a barrier. The round sums do not feed later rounds (every challenge is an input),
 // so each wave keeps per-round partials (a sum over its lanes of one q) and the workgroup adds
-// them up once, after the last round.
+// them up once, after the last round. The challenges are read into LDS with the tables, so no
+// round waits for a global load.
 constexpr int kTailThreads = 1024;
 template <int D>
 __device__ __forceinline__ void sc_tail_body(const ScTailJob& jb, ef2 (*tab)[1 << kScTailLog],
-                                             ef2 (*wpart)[kTailThreads / 64][4]) {
+                                             ef2 (*wpart)[kTailThreads / 64][4], ef2* rch) {
   const uint32_t len = jb.len, tid = threadIdx.x, q = tid & 3, wave = tid >> 6, lane = tid & 63;
   const bool m1 = jb.m1 != 0;  // table 1 = table 0 - 1, not stored
 #pragma unroll
@@ -211,11 +212,12 @@ __device__ __forceinline__ void sc_tail_body(const ScTailJob& jb, ef2 (*tab)[1 <
     if (m1 && j == 1) continue;
     for (uint32_t i = tid; i < len; i += blockDim.x) tab[j][i] = jb.p[j][i];
   }
+  if (len > 1 && tid < 31u - __clz(len)) rch[tid] = jb.chal[tid];  // log2(len) rounds
   __syncthreads();
   uint32_t k = 0;
   for (uint32_t cur = len; cur > 1; cur >>= 1, k++) {
     const uint32_t half = cur / 2;
-    const ef2 r = jb.chal[k];
+    const ef2 r = rch[k];
     ef2 acc = ef_zero();
     ef2 fv[(4 << kScTailLog) / 2 / kTailThreads];  // folded values held across the barrier
 #pragma unroll
@@ -278,15 +280,16 @@ __device__ __forceinline__ void sc_tail_body(const ScTailJob& jb, ef2 (*tab)[1 <
 __global__ __launch_bounds__(kTailThreads) void sc_tail_group_kernel(ScTail a) {
   __shared__ ef2 tab[3][1 << kScTailLog];
   __shared__ ef2 wpart[kScTailLog][kTailThreads / 64][4];
+  __shared__ ef2 rch[kScTailLog];
   if (blockIdx.x >= a.nj) {
     red_job_body(a.r[blockIdx.x - a.nj], &wpart[0][0][0]);
     return;
   }
   const ScTailJob& j = a.j[blockIdx.x];
   if (j.d == 3)
-    sc_tail_body<3>(j, tab, wpart);
+    sc_tail_body<3>(j, tab, wpart, rch);
   else
-    sc_tail_body<2>(j, tab, wpart);
+    sc_tail_body<2>(j, tab, wpart, rch);
 }
 
 hipError_t launch_sc_tail_group(const ScTail& tail, hipStream_t st) {
@@ -825,12 +828,17 @@ __global__ __launch_bounds__(256) void bf_round3_kernel(BfRound a, uint64_t half
 // Per round: thread i folds codeword pair (i, i + len/2) (written back in place and to the
 // level's place in global memory, for the Merkle trees and openings), and for i < len/4 adds
 // m_c[2i] eq_c[i] to channel c's sum and folds m_c and eq_c; reads and writes of m and eq are
-// separated by a barrier. The round sums (outputs only) stay per wave until the end.
+// separated by a barrier. The round sums (outputs only) stay per wave until the end. The
+// challenges and the twiddles of every round (tws[i] = tw[i 2^t0]; round t = t0 + k takes
+// g^((h - i) 2^t) = tws[(h - i) 2^k]) are read into LDS with the tables, so no round waits for a
+// global load.
 __global__ __launch_bounds__(512) void bf_tail_kernel(BfTail a) {
   __shared__ ef2 cw[1 << kBfTailLog];
   __shared__ ef2 m[3][1 << (kBfTailLog - 1)];
   __shared__ ef2 eq[3][1 << (kBfTailLog - 2)];
   __shared__ ef2 wp[kBfTailLog][8][3];
+  __shared__ ef2 rch[32];
+  __shared__ uint64_t tws[1 << (kBfTailLog - 1)];
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int c = 0; c < 3; c++) {  // the last grid round's sums
     if (a.prev[c].nblk) red_job_body(a.prev[c], &wp[0][0][0]);
@@ -838,6 +846,8 @@ __global__ __launch_bounds__(512) void bf_tail_kernel(BfTail a) {
   }
   const uint32_t len0 = 1u << (a.logD - a.t0);
   for (uint32_t i = tid; i < len0; i += blockDim.x) cw[i] = a.cw_in[i];
+  for (uint32_t i = tid; i < len0 / 2; i += blockDim.x) tws[i] = a.tw[(uint64_t)i << a.t0];
+  if (tid < a.R - a.t0) rch[tid] = a.chal[a.t0 + tid];
   for (int c = 0; c < 3; c++) {
     for (uint32_t i = tid; i < len0 / 2; i += blockDim.x) m[c][i] = a.m_in[c][i];
     for (uint32_t i = tid; i < len0 / 4; i += blockDim.x) eq[c][i] = a.eq[c][i];
@@ -846,12 +856,11 @@ __global__ __launch_bounds__(512) void bf_tail_kernel(BfTail a) {
   ef2* lvl = a.lvl_out;
   for (uint32_t t = a.t0; t < a.R; t++) {
     const uint32_t k = t - a.t0, len = len0 >> k, h = len / 2, half = len / 4;
-    const uint32_t lsh = t;  // log2(D / len): the order-len twiddles are every 2^t-th of tw
-    const ef2 r = a.chal[t];
+    const ef2 r = rch[k];
     ef2 cwv = ef_zero(), mv[3], ev[3], acc[3];
     if (tid < h) {  // eq_fold_codeword_at over the subgroup of order len
       const ef2 x = cw[tid], nx = cw[tid + h];
-      const uint64_t ginv = tid == 0 ? 1 : gl_neg(a.tw[(uint64_t)(h - tid) << lsh]);
+      const uint64_t ginv = tid == 0 ? 1 : gl_neg(tws[(h - tid) << k]);
       const ef2 pe = ef_mul_base(ef_add(x, nx), GL_INV2);
       const ef2 po = ef_mul_base(ef_sub(x, nx), gl_mul(ginv, GL_INV2));
       cwv = ef_add(ef_mul(ef_one_minus(r), pe), ef_mul(r, po));
