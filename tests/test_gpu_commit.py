@@ -226,3 +226,32 @@ def test_batched_commit_mixed_shapes_matches_single(gpu_ctx):
     for (r, cd), (er, eh) in zip(res, expect):
         assert r == er
         np.testing.assert_array_equal(cd.digests(), eh)
+
+
+@pytest.mark.parametrize("shape", [(13, 13), (9, 17)])
+def test_full_size_mu26_chain(gpu_ctx, shape):
+    """mu = 26, the largest shapes served: rows of 2^14 and 2^18 points (transforms longer than
+    one LDS tile, at full height), W = 2^19 and 2^15 hash inputs per column. Sampled rows of F'
+    equal the oracle's row encoding, sampled columns of H equal the oracle's subset-sum hash of
+    the bits of F', and the root equals the oracle's Merkle root over the GPU's H."""
+    log_m, log_n = shape
+    inp = synth.Inputs(log_m, log_n, seed=26 + log_m, n_fields=4, n_ints=4)
+    print(f"full-size mu=26 chain check shape={shape}", flush=True)
+    pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx, commit_only=True)
+    root, cd = Ligesis.commit(pk, inp.f)
+    n, m = 1 << log_n, 1 << log_m
+    enc = cd.encoded()
+    h = cd.digests()
+    rng = np.random.default_rng(1)
+    rows = rng.choice(m, 4, replace=False)
+    f2 = inp.f.reshape(m, n)
+    for i in rows:
+        np.testing.assert_array_equal(enc[i], oracle.rs_encode_rows(f2[i], 1, n)[0])
+    cols = rng.choice(2 * n, 3, replace=False)
+    bits = ((enc[:, cols, None] >> np.arange(64, dtype=np.uint64)) & np.uint64(1)).astype(np.uint8)
+    bits = bits.transpose(1, 0, 2).reshape(len(cols), 64 * m)
+    np.testing.assert_array_equal(h[:, cols], oracle.hash_columns(inp.a.reshape(32, 64 * m), bits))
+    leaves = h.T.copy().view(np.uint8).reshape(2 * n, 256)
+    assert oracle.merkle_root(leaves) == root
+    del enc
+    cd.release()
