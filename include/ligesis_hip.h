@@ -124,6 +124,17 @@ int lgs_setup(lgs_ctx* ctx, uint32_t log_m, uint32_t log_n, const uint64_t* a, u
               lgs_key** out);
 /* com_a of the key (ligesis.rs:71); LGS_ERR_STATE for a commit-only key. */
 int lgs_key_com_a(const lgs_key* key, uint8_t com_a[32]);
+
+#define LGS_BASEFOLD_EXT 2u /* lgs_basefold_commit: the polynomial is (c0, c1) extension pairs */
+/* Standalone Basefold commitment root (Basefold::commit_{base,ext}_on_domain,
+ * pcs/src/basefold.rs:341-382; commit_base / commit_ext are log_interleave 6, :309-339): the 2^li
+ * chunks f[j :: 2^li] are zero-padded 2^code_rate-fold and transformed, and leaf p hashes column
+ * p of the codewords into a BLAKE3 Merkle tree. poly holds 2^num_vars canonical base elements,
+ * or pairs with LGS_BASEFOLD_EXT; a device pointer with LGS_INPUT_DEVICE. Limits: num_vars <= 30,
+ * log_interleave <= min(6, num_vars), 1 <= code_rate <= 4. Returns the root only: the prover
+ * state is not kept (the standalone Basefold prove, basefold.rs:384-516, is not provided). */
+int lgs_basefold_commit(lgs_ctx* ctx, const uint64_t* poly, uint32_t num_vars, uint32_t flags,
+                        uint32_t log_interleave, uint32_t code_rate, uint8_t root[32]);
 void lgs_key_release(lgs_key* key);
 
 /* Commit (ligesis.rs:277): f = 2^(log_m + log_n) canonical evaluations, F[i][j] = f[i*n + j].

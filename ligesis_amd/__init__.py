@@ -38,6 +38,7 @@ LGS_ERR_STATE = 7
 
 LGS_SETUP_COMMIT_ONLY = 1
 LGS_INPUT_DEVICE = 1
+LGS_BASEFOLD_EXT = 2
 LGS_READ_ENCODED, LGS_READ_DIGESTS, LGS_READ_TREE = 0, 1, 2
 
 _lib = None
@@ -66,6 +67,8 @@ def lib():
         L.lgs_stage_times.argtypes = [vp, vp, u32]
         L.lgs_setup.argtypes = [vp, u32, u32, vp, u32, pp]
         L.lgs_key_com_a.argtypes = [vp, vp]
+        L.lgs_basefold_commit.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_uint32, vp]
         L.lgs_key_release.argtypes = [vp]
         L.lgs_commit.argtypes = [vp, vp, vp, u64, u32, vp, pp]
         L.lgs_commit_batch.argtypes = [vp, u32, vp, vp, vp, u32, vp, vp]
@@ -344,6 +347,48 @@ class Ligesis:
         ctx.check(rc)
         oracle.fields_idx, oracle.ints_idx = int(used[0]), int(used[1])
         return out[: cap.value].tobytes()
+
+
+class Basefold:
+    """The reference's ``Basefold<Goldilocks, BinomialExtensionField<Goldilocks, 2>>`` commitment
+    (pcs/src/basefold.rs:309-382): the root only, computed on the GPU."""
+
+    LOG_INTERLEAVE = 6  # basefold.rs:16
+
+    @staticmethod
+    def commit_on_domain(poly, code_rate: int, log_interleave: int, ctx: Context | None = None,
+                         device: bool = False) -> bytes:
+        """``commit_{base,ext}_on_domain``: ``poly`` is 2^nv base values, or (2^nv, 2) extension
+        pairs; with ``device=True`` a GPU tensor of those words (``ext`` from its last dimension)."""
+        ctx = ctx or default_context()
+        if device:
+            ext = poly.dim() == 2
+            nv = poly.shape[0].bit_length() - 1
+            ptr, keep = ctypes.c_void_p(poly.data_ptr()), poly
+            if poly.shape[0] != 1 << nv:
+                raise LigesisError(LGS_ERR_ARG, "basefold commit: length must be a power of two")
+        else:
+            a = np.ascontiguousarray(_u64(poly))
+            ext = a.ndim == 2
+            if ext and a.shape[1] != 2:
+                raise LigesisError(LGS_ERR_ARG, "basefold commit: extension values are (c0, c1) pairs")
+            nv = a.shape[0].bit_length() - 1
+            if a.shape[0] != 1 << nv:
+                raise LigesisError(LGS_ERR_ARG, "basefold commit: length must be a power of two")
+            ptr, keep = _ptr(a), a
+        root = np.zeros(32, np.uint8)
+        flags = (LGS_BASEFOLD_EXT if ext else 0) | (LGS_INPUT_DEVICE if device else 0)
+        ctx.check(lib().lgs_basefold_commit(ctx.h, ptr, nv, flags, log_interleave, code_rate, _ptr(root)))
+        del keep
+        return root.tobytes()
+
+    @staticmethod
+    def commit_base(poly, code_rate: int, ctx: Context | None = None) -> bytes:
+        return Basefold.commit_on_domain(poly, code_rate, Basefold.LOG_INTERLEAVE, ctx)
+
+    @staticmethod
+    def commit_ext(poly, code_rate: int, ctx: Context | None = None) -> bytes:
+        return Basefold.commit_on_domain(poly, code_rate, Basefold.LOG_INTERLEAVE, ctx)
 
 
 LGS_PENDING = 8

@@ -265,6 +265,40 @@ int lgs_key_com_a(const lgs_key* key, uint8_t com_a[32]) {
   return LGS_OK;
 }
 
+int lgs_basefold_commit(lgs_ctx* ctx, const uint64_t* poly, uint32_t num_vars, uint32_t flags,
+                        uint32_t log_interleave, uint32_t code_rate, uint8_t root[32]) {
+  if (!ctx || !poly || !root) return LGS_ERR_ARG;
+  return guarded(&ctx->c, [&] {
+    if (num_vars > 30 || log_interleave > 6 || log_interleave > num_vars || code_rate < 1 || code_rate > 4)
+      throw lgs::Error(LGS_ERR_ARG, "basefold commit: num_vars <= 30, log_interleave <= min(6, num_vars), "
+                                    "1 <= code_rate <= 4");
+    lgs::Context& c = ctx->c;
+    (void)hipSetDevice(c.device);
+    const bool ext = (flags & LGS_BASEFOLD_EXT) != 0;
+    const uint64_t words = (uint64_t(1) << num_vars) * (ext ? 2 : 1);
+    try {
+      lgs::DevBuf<uint64_t> owned;
+      const uint64_t* p = poly;
+      if (!(flags & LGS_INPUT_DEVICE)) {
+        owned = lgs::DevBuf<uint64_t>(&c.pool, words);
+        lgs::hip_check(hipMemcpyAsync(owned.ptr, poly, words * 8, hipMemcpyHostToDevice, c.stream), "upload poly");
+        p = owned.ptr;
+      }
+      lgs::DevBasefold st;
+      lgs::enqueue_basefold_commit(c, st, p, ext, num_vars, log_interleave, code_rate);
+      uint8_t* h = c.staging(32);
+      lgs::hip_check(hipMemcpyAsync(h, st.tree.nodes.ptr + (st.tree.node_count() - 1) * 8, 32,
+                                    hipMemcpyDeviceToHost, c.stream),
+                     "read basefold root");
+      lgs::hip_check(hipStreamSynchronize(c.stream), "basefold sync");
+      std::memcpy(root, h, 32);
+    } catch (...) {
+      c.drain();
+      throw;
+    }
+  });
+}
+
 void lgs_key_release(lgs_key* key) {
   if (!key) return;
   lgs::Context* c = key->k.ctx;

@@ -47,7 +47,7 @@ def lib():
             "lgso_merkle_root", "lgso_merkle_open", "lgso_merkle_verify", "lgso_sumcheck_prove",
             "lgso_sumcheck_verify", "lgso_basefold_batch_roundtrip", "lgso_shape_ok",
             "lgso_pk_com_a", "lgso_cd_enc", "lgso_cd_h", "lgso_prove", "lgso_verify",
-            "lgso_proof_reencode",
+            "lgso_proof_reencode", "lgso_basefold_commit",
         ]:
             getattr(L, name).restype = i64
         L.lgso_setup.restype = vp
@@ -63,6 +63,7 @@ def lib():
         L.lgso_verify.argtypes = [u64, u64, vp, vp, vp, vp, u64, vp, u64, vp, u64]
         L.lgso_proof_reencode.argtypes = [vp, u64, vp]
         L.lgso_shape_ok.argtypes = [u64, u64]
+        L.lgso_basefold_commit.argtypes = [vp, u64, i64, u64, u64, vp]
         _lib = L
     return _lib
 
@@ -206,6 +207,19 @@ def sumcheck_verify(claim, rounds, finals, fields) -> bool:
                                     _p(rounds), _p(finals), _p(fields), ctypes.c_uint64(fields.shape[0]))
     assert rc >= 0, err()
     return bool(rc)
+
+
+def basefold_commit(poly, li: int, code_rate: int) -> bytes:
+    """Root of Basefold::commit_{base,ext}_on_domain (basefold.rs:341-382): ``poly`` is 2^nv base
+    values or (2^nv, 2) extension pairs."""
+    a = _u64(poly)
+    ext = a.ndim == 2
+    nv = a.shape[0].bit_length() - 1
+    root = np.zeros(32, np.uint8)
+    flat = np.ascontiguousarray(a).reshape(-1)
+    rc = lib().lgso_basefold_commit(_p(flat), nv, int(ext), li, code_rate, _p(root))
+    assert rc >= 0, err()
+    return root.tobytes()
 
 
 def basefold_batch_roundtrip(bf_polys, ef_polys, points, rounds, code_rate, fields, ints, tamper=0):

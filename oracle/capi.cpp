@@ -171,6 +171,24 @@ int64_t lgso_sumcheck_verify(uint64_t d, uint64_t nv, const uint64_t* claim, con
   });
 }
 
+// ---- basefold: one commitment root (commit_{base,ext}_on_domain, basefold.rs:341-382) ----
+int64_t lgso_basefold_commit(const uint64_t* poly, uint64_t nv, int64_t ext, uint64_t li, uint64_t code_rate,
+                             uint8_t* root) {
+  return guard([&] {
+    const size_t len = size_t(1) << nv;
+    H32 c;
+    if (ext) {
+      BfState<Ef> st;
+      c = commit_on_domain(load_ef(poly, len), code_rate, li, st);
+    } else {
+      BfState<uint64_t> st;
+      c = commit_on_domain(std::vector<uint64_t>(poly, poly + len), code_rate, li, st);
+    }
+    std::copy(c.begin(), c.end(), root);
+    return int64_t(1);
+  });
+}
+
 // ---- basefold (shared-domain batch, mixed base/extension polynomials) ----
 // bf_polys: concatenated base polys of sizes 2^bf_nv[i]; ef_polys likewise (EF pairs).
 // points: concatenated EF points, base polys first. Returns 1 when the oracle's own

@@ -1,0 +1,43 @@
+"""Standalone Basefold commitment on the GPU (pcs/src/basefold.rs:309-382): roots equal the
+oracle's for base and extension polynomials, interleave depths 0..6 (6 = the reference's
+LOG_INTERLEAVE, basefold.rs:16), code rates 1..3, host and device inputs; bad arguments are
+rejected."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from ligesis_amd import Basefold, LigesisError
+from ligesis_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(6, 0, 1, False), (10, 6, 1, False), (12, 4, 2, False), (16, 6, 1, False), (20, 6, 1, False),
+         (9, 3, 1, True), (14, 6, 1, True), (13, 2, 3, False), (18, 5, 2, True)]
+
+
+@pytest.mark.parametrize("nv,li,code_rate,ext", CASES)
+def test_root_matches_oracle(gpu_ctx, nv, li, code_rate, ext):
+    print(f"basefold commit nv={nv} li={li} code_rate={code_rate} ext={ext}", flush=True)
+    poly = synth.ext_elems(nv + 40, 1 << nv) if ext else synth.field_elems(nv + 40, 1 << nv)
+    want = oracle.basefold_commit(poly, li, code_rate)
+    assert Basefold.commit_on_domain(poly, code_rate, li, ctx=gpu_ctx) == want
+    dev = torch.from_numpy(np.ascontiguousarray(poly).view(np.int64)).to("cuda:0")
+    assert Basefold.commit_on_domain(dev, code_rate, li, ctx=gpu_ctx, device=True) == want
+
+
+def test_commit_base_uses_log_interleave_6(gpu_ctx):
+    poly = synth.field_elems(77, 1 << 12)
+    assert Basefold.commit_base(poly, 1, ctx=gpu_ctx) == oracle.basefold_commit(poly, 6, 1)
+
+
+@pytest.mark.parametrize("nv,li,code_rate", [(5, 6, 1), (8, 7, 1), (8, 2, 0), (8, 2, 5), (31, 2, 1)])
+def test_bad_arguments_rejected(gpu_ctx, nv, li, code_rate):
+    poly = np.zeros(1 << min(nv, 8), np.uint64)
+    with pytest.raises(LigesisError):
+        if nv > 8:  # the size limit is checked before the polynomial is read
+            from ligesis_amd import lib, _ptr
+            gpu_ctx.check(lib().lgs_basefold_commit(gpu_ctx.h, _ptr(poly), nv, 0, li, code_rate,
+                                                    _ptr(np.zeros(32, np.uint8))))
+        else:
+            Basefold.commit_on_domain(poly, code_rate, li, ctx=gpu_ctx)

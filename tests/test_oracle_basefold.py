@@ -33,3 +33,29 @@ def test_batch_extension_only_and_base_only():
     assert oracle.basefold_batch_roundtrip(bf, ef, pts, rounds=5, code_rate=1, fields=fields, ints=ints)
     bf, ef, pts, fields, ints = _setup([10], [], 4)
     assert oracle.basefold_batch_roundtrip(bf, ef, pts, rounds=4, code_rate=1, fields=fields, ints=ints)
+
+
+def _restated_commit(poly, li, code_rate):
+    """commit_{base,ext}_on_domain (basefold.rs:341-382) from the separately pinned DFT and Merkle
+    tree: chunk j = poly[j :: 2^li] (MlPoly::split), zero-padded 2^code_rate-fold, transformed;
+    leaf p is the serialized column p of the codewords (chunk order, (c0, c1) for extension)."""
+    import numpy as np
+
+    ext = poly.ndim == 2
+    n = poly.shape[0]
+    length = (n >> li) << code_rate
+    codes = []
+    for j in range(1 << li):
+        c = poly[j :: 1 << li]
+        pad = np.zeros((length, 2) if ext else length, np.uint64)
+        pad[: c.shape[0]] = c
+        codes.append(oracle.dft_ext(pad) if ext else oracle.dft(pad))
+    cols = np.stack(codes, axis=1).reshape(length, -1).astype("<u8")
+    return oracle.merkle_root(cols.view(np.uint8).reshape(length, -1))
+
+
+@pytest.mark.parametrize("nv,li,code_rate,ext", [(6, 0, 1, False), (7, 2, 1, False), (8, 3, 2, False),
+                                                  (6, 1, 1, True), (7, 3, 2, True), (9, 6, 1, False)])
+def test_commit_root_matches_restatement(nv, li, code_rate, ext):
+    poly = synth.ext_elems(nv * 10 + li, 1 << nv) if ext else synth.field_elems(nv * 10 + li, 1 << nv)
+    assert oracle.basefold_commit(poly, li, code_rate) == _restated_commit(poly, li, code_rate)
