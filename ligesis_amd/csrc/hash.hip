@@ -518,6 +518,10 @@ __device__ __forceinline__ void dma_step(const DmaTile& t, uint32_t sl, DmaSlot 
   asm volatile("s_barrier" ::: "memory");
   const uint32_t nx = sl + NB - 1;
   dma_copy_slab(t, nx < t.ns ? nx : t.ns - 1, nxt);
+  // the products issue at raised wave priority, so a SIMD's other wave (copies, barrier) does
+  // not take issue slots from them
+  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_setprio(1);
   v4i b[4];
 #pragma unroll
   for (int nt = 0; nt < 4; nt++) {
@@ -555,6 +559,8 @@ __device__ __forceinline__ void dma_step(const DmaTile& t, uint32_t sl, DmaSlot 
   __builtin_amdgcn_sched_group_barrier(0x008, 9, 0);
   __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
   __builtin_amdgcn_sched_group_barrier(0x008, 15, 0);
+  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_setprio(0);
 }
 
 // One 256 x 128 tile over slabs [split * ns, (split + 1) * ns) with a ring of NB slots (NB = 4
