@@ -73,6 +73,7 @@ def lib():
         L.lgs_commit.argtypes = [vp, vp, vp, u64, u32, vp, pp]
         L.lgs_commit_batch.argtypes = [vp, u32, vp, vp, vp, u32, vp, vp]
         L.lgs_commitment_release.argtypes = [vp]
+        L.lgs_commit_noalloc.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp, vp]
         L.lgs_commitment_read.argtypes = [vp, u32, vp, u64]
         L.lgs_prove.argtypes = [vp, vp, vp, vp, u64, vp, u64, vp, u64, vp,
                                 ctypes.POINTER(ctypes.c_uint64), vp]
@@ -322,6 +323,27 @@ class Ligesis:
         return res
 
     @staticmethod
+    def commit_noalloc(pk: ProvingKey, f, encoded, digests, tree, stream=None, want_root: bool = True):
+        """``lgs_commit_noalloc``: the commit of ``commit`` with f and every output in
+        caller-owned GPU tensors (F' m x 2n words, H 32 x 2n words, the tree (4n - 1) x 32 bytes);
+        the library allocates nothing. ``stream``: a torch stream to enqueue on (None: the
+        context's). Returns the root (waiting for the stream) or None with ``want_root=False``."""
+        m, n = 1 << pk.log_m, 1 << pk.log_n
+        need = {"f": (f, m * n * 8), "encoded": (encoded, m * 2 * n * 8), "digests": (digests, C * 2 * n * 8),
+                "tree": (tree, (4 * n - 1) * 32)}
+        for name, (t, nbytes) in need.items():
+            if not t.is_cuda or not t.is_contiguous() or t.numel() * t.element_size() != nbytes:
+                raise LigesisError(LGS_ERR_ARG, f"commit_noalloc: {name} must be a contiguous GPU tensor "
+                                                f"of {nbytes} bytes")
+        root = np.zeros(32, np.uint8)
+        st = ctypes.c_void_p(stream.cuda_stream) if stream is not None else None
+        pk.ctx.check(lib().lgs_commit_noalloc(pk.ctx.h, pk.h, ctypes.c_void_p(f.data_ptr()), m * n,
+                                              ctypes.c_void_p(encoded.data_ptr()), ctypes.c_void_p(digests.data_ptr()),
+                                              ctypes.c_void_p(tree.data_ptr()), st,
+                                              _ptr(root) if want_root else None))
+        return root.tobytes() if want_root else None
+
+    @staticmethod
     def prove(pk: ProvingKey, data: CommitData, z, oracle: RandomOracle) -> bytes:
         """Eval-prove (ligesis.rs:347): returns the canonical proof bytes and advances the
         oracle's cursors by the draws the proof consumed (the reference's RandomOracle is read
@@ -356,39 +378,84 @@ class Basefold:
     LOG_INTERLEAVE = 6  # basefold.rs:16
 
     @staticmethod
-    def commit_on_domain(poly, code_rate: int, log_interleave: int, ctx: Context | None = None,
-                         device: bool = False) -> bytes:
-        """``commit_{base,ext}_on_domain``: ``poly`` is 2^nv base values, or (2^nv, 2) extension
-        pairs; with ``device=True`` a GPU tensor of those words (``ext`` from its last dimension)."""
-        ctx = ctx or default_context()
+    def _words(poly, ext: bool | None, device: bool):
+        """Validates a polynomial for the C ABI: returns (pointer, num_vars, ext, keepalive).
+        Base-field input is 2^nv canonical words; extension input is (2^nv, 2) (c0, c1) pairs.
+        ``ext`` None takes the field from the shape; True / False demands that field (a 1-D
+        base polynomial given as extension is upcast to (c0, 0) pairs, as MlPoly<EF> holds it)."""
         if device:
-            ext = poly.dim() == 2
-            nv = poly.shape[0].bit_length() - 1
+            import torch
+
+            if poly.dtype not in (torch.int64, torch.uint64):
+                raise LigesisError(LGS_ERR_ARG, "basefold commit: device polynomial must hold 8-byte words")
+            if not poly.is_contiguous():
+                raise LigesisError(LGS_ERR_ARG, "basefold commit: device polynomial must be contiguous")
+            if poly.dim() == 2 and poly.shape[1] != 2:
+                raise LigesisError(LGS_ERR_ARG, "basefold commit: extension values are (c0, c1) pairs")
+            if poly.dim() not in (1, 2):
+                raise LigesisError(LGS_ERR_ARG, "basefold commit: expected 1-D or (N, 2) input")
+            is_ext = poly.dim() == 2
+            if ext is not None and ext != is_ext:
+                if ext and not is_ext:
+                    poly = torch.stack([poly, torch.zeros_like(poly)], dim=1).contiguous()
+                    is_ext = True
+                else:
+                    raise LigesisError(LGS_ERR_ARG, "basefold commit_base: polynomial must be base-field (1-D)")
+            count = poly.shape[0]
             ptr, keep = ctypes.c_void_p(poly.data_ptr()), poly
-            if poly.shape[0] != 1 << nv:
-                raise LigesisError(LGS_ERR_ARG, "basefold commit: length must be a power of two")
         else:
             a = np.ascontiguousarray(_u64(poly))
-            ext = a.ndim == 2
-            if ext and a.shape[1] != 2:
+            if a.ndim == 2 and a.shape[1] != 2:
                 raise LigesisError(LGS_ERR_ARG, "basefold commit: extension values are (c0, c1) pairs")
-            nv = a.shape[0].bit_length() - 1
-            if a.shape[0] != 1 << nv:
-                raise LigesisError(LGS_ERR_ARG, "basefold commit: length must be a power of two")
+            if a.ndim not in (1, 2):
+                raise LigesisError(LGS_ERR_ARG, "basefold commit: expected 1-D or (N, 2) input")
+            is_ext = a.ndim == 2
+            if ext is not None and ext != is_ext:
+                if ext:
+                    a = np.ascontiguousarray(np.stack([a, np.zeros_like(a)], axis=1))
+                    is_ext = True
+                else:
+                    raise LigesisError(LGS_ERR_ARG, "basefold commit_base: polynomial must be base-field (1-D)")
+            count = a.shape[0]
             ptr, keep = _ptr(a), a
+        nv = count.bit_length() - 1
+        if count != 1 << nv:
+            raise LigesisError(LGS_ERR_ARG, "basefold commit: length must be a power of two")
+        return ptr, nv, is_ext, keep
+
+    @staticmethod
+    def commit_on_domain(poly, code_rate: int, log_interleave: int, ctx: Context | None = None,
+                         device: bool = False, ext: bool | None = None) -> bytes:
+        """``commit_{base,ext}_on_domain``: ``poly`` is 2^nv base values, or (2^nv, 2) extension
+        pairs (on the host, or with ``device=True`` a contiguous GPU tensor of 8-byte words)."""
+        ptr, nv, is_ext, keep = Basefold._words(poly, ext, device)
+        ctx = ctx or default_context()
         root = np.zeros(32, np.uint8)
-        flags = (LGS_BASEFOLD_EXT if ext else 0) | (LGS_INPUT_DEVICE if device else 0)
+        flags = (LGS_BASEFOLD_EXT if is_ext else 0) | (LGS_INPUT_DEVICE if device else 0)
         ctx.check(lib().lgs_basefold_commit(ctx.h, ptr, nv, flags, log_interleave, code_rate, _ptr(root)))
         del keep
         return root.tobytes()
 
     @staticmethod
+    def commit_base_on_domain(poly, code_rate: int, log_interleave: int, ctx: Context | None = None,
+                              device: bool = False) -> bytes:
+        return Basefold.commit_on_domain(poly, code_rate, log_interleave, ctx, device, ext=False)
+
+    @staticmethod
+    def commit_ext_on_domain(poly, code_rate: int, log_interleave: int, ctx: Context | None = None,
+                             device: bool = False) -> bytes:
+        return Basefold.commit_on_domain(poly, code_rate, log_interleave, ctx, device, ext=True)
+
+    @staticmethod
     def commit_base(poly, code_rate: int, ctx: Context | None = None) -> bytes:
-        return Basefold.commit_on_domain(poly, code_rate, Basefold.LOG_INTERLEAVE, ctx)
+        """``commit_base`` (basefold.rs:309): a base-field polynomial, log_interleave 6."""
+        return Basefold.commit_on_domain(poly, code_rate, Basefold.LOG_INTERLEAVE, ctx, ext=False)
 
     @staticmethod
     def commit_ext(poly, code_rate: int, ctx: Context | None = None) -> bytes:
-        return Basefold.commit_on_domain(poly, code_rate, Basefold.LOG_INTERLEAVE, ctx)
+        """``commit_ext`` (basefold.rs:330): an extension-field polynomial, log_interleave 6; a 1-D
+        base polynomial is taken as its upcast (c0, 0) pairs."""
+        return Basefold.commit_on_domain(poly, code_rate, Basefold.LOG_INTERLEAVE, ctx, ext=True)
 
 
 LGS_PENDING = 8

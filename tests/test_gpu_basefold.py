@@ -1,7 +1,7 @@
 """Standalone Basefold commitment on the GPU (pcs/src/basefold.rs:309-382): roots equal the
 oracle's for base and extension polynomials, interleave depths 0..6 (6 = the reference's
-LOG_INTERLEAVE, basefold.rs:16), code rates 1..3, host and device inputs; bad arguments are
-rejected."""
+LOG_INTERLEAVE, basefold.rs:16), code rates 1..4 on both transform paths (rows held in LDS, and
+rows longer than one LDS tile), host and device inputs; bad arguments are rejected."""
 import numpy as np
 import pytest
 import torch
@@ -13,7 +13,8 @@ from ligesis_amd import synth
 pytestmark = pytest.mark.gpu
 
 CASES = [(6, 0, 1, False), (10, 6, 1, False), (12, 4, 2, False), (16, 6, 1, False), (20, 6, 1, False),
-         (9, 3, 1, True), (14, 6, 1, True), (13, 2, 3, False), (18, 5, 2, True)]
+         (9, 3, 1, True), (14, 6, 1, True), (13, 2, 3, False), (18, 5, 2, True),
+         (9, 3, 3, False), (10, 6, 4, False), (16, 2, 4, False), (11, 4, 4, True)]
 
 
 @pytest.mark.parametrize("nv,li,code_rate,ext", CASES)
@@ -41,3 +42,26 @@ def test_bad_arguments_rejected(gpu_ctx, nv, li, code_rate):
                                                     _ptr(np.zeros(32, np.uint8))))
         else:
             Basefold.commit_on_domain(poly, code_rate, li, ctx=gpu_ctx)
+
+
+def test_commit_ext_upcasts_a_base_polynomial(gpu_ctx):
+    # commit_ext takes MlPoly<EF>: a base polynomial enters as (c0, 0) pairs (16-byte leaves)
+    poly = synth.field_elems(78, 1 << 10)
+    pairs = np.stack([poly, np.zeros_like(poly)], axis=1)
+    want = oracle.basefold_commit(pairs, 6, 1)
+    assert want != oracle.basefold_commit(poly, 6, 1)
+    assert Basefold.commit_ext(poly, 1, ctx=gpu_ctx) == want
+    assert Basefold.commit_ext(pairs, 1, ctx=gpu_ctx) == want
+
+
+def test_device_input_checks(gpu_ctx):
+    poly = torch.from_numpy(synth.field_elems(79, 1 << 9).view(np.int64)).to("cuda:0")
+    pairs = torch.from_numpy(synth.ext_elems(80, 1 << 9).view(np.int64)).to("cuda:0")
+    with pytest.raises(LigesisError):
+        Basefold.commit_on_domain(pairs[:, 0], 1, 3, ctx=gpu_ctx, device=True)  # strided column
+    with pytest.raises(LigesisError):
+        Basefold.commit_on_domain(poly.to(torch.int32), 1, 3, ctx=gpu_ctx, device=True)
+    with pytest.raises(LigesisError):
+        Basefold.commit_base_on_domain(pairs, 1, 3, ctx=gpu_ctx, device=True)
+    want = oracle.basefold_commit(pairs.cpu().numpy().view(np.uint64), 3, 1)
+    assert Basefold.commit_ext_on_domain(pairs, 1, 3, ctx=gpu_ctx, device=True) == want

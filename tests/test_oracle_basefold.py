@@ -55,7 +55,9 @@ def _restated_commit(poly, li, code_rate):
 
 
 @pytest.mark.parametrize("nv,li,code_rate,ext", [(6, 0, 1, False), (7, 2, 1, False), (8, 3, 2, False),
-                                                  (6, 1, 1, True), (7, 3, 2, True), (9, 6, 1, False)])
+                                                  (6, 1, 1, True), (7, 3, 2, True), (9, 6, 1, False),
+                                                  (8, 3, 3, False), (7, 2, 4, False), (6, 2, 3, True),
+                                                  (7, 4, 4, True)])
 def test_commit_root_matches_restatement(nv, li, code_rate, ext):
     poly = synth.ext_elems(nv * 10 + li, 1 << nv) if ext else synth.field_elems(nv * 10 + li, 1 << nv)
     assert oracle.basefold_commit(poly, li, code_rate) == _restated_commit(poly, li, code_rate)
