@@ -7,7 +7,11 @@ full commit with f already resident in HBM, ending with the 32-byte root on the 
 
 Prints one JSON line: value = 2^24 / (seconds per commit) / 1e6, the roofline of the dominant
 kernel (timed with HIP events on the library's stream inside the timed region), and the CPU
-baseline (the oracle's single-threaded commit on a bounded sample of the same workload).
+baseline (the oracle's single-threaded commit on a bounded sample of the same workload). Beside
+the headline, outside its timed region, the same line carries the rest of the metric: the same
+commit from host memory, the mu = 24 eval-prove time at 2^8 x 2^16 and 2^7 x 2^17, and the
+served mixed stream's p50 / p99 commit+open latency and polys/s (measure_serve); --commit-only
+leaves those out.
 """
 from __future__ import annotations
 
@@ -56,21 +60,50 @@ def settle(step, ms: float = 300.0):
         step()
 
 
-def cpu_baseline(seconds_hint: float = 15.0) -> dict:
-    """Oracle commit (single thread, the reference's loop structure) on 2^9 rows x 2^12 columns:
-    the same row length as the workload and 1/8 of its rows (about 15 s of CPU work). A is then
-    8 MiB, so the sample runs at least as fast per element as the full workload would (a
-    conservative baseline)."""
+def _cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+_CPU_SAMPLE = (9, 12)
+
+
+def _cpu_sample_run() -> float:
+    """Seconds for the oracle's single-threaded commit of the CPU sample (run in a child
+    process pinned to one core)."""
     import oracle
     from ligesis_amd import synth
 
-    lm, ln = 9, 12
+    lm, ln = _CPU_SAMPLE
     inp = synth.Inputs(lm, ln, seed=77, n_fields=4, n_ints=4)
     pk = oracle.ProvingKey(lm, ln, inp.a, commit_a=False)
     t0 = time.perf_counter()
     cd = oracle.CommitData(pk, inp.f)
     dt = time.perf_counter() - t0
     del cd
+    return dt
+
+
+def cpu_baseline(seconds_hint: float = 15.0) -> dict:
+    """Oracle commit (single thread, the reference's loop structure) on 2^9 rows x 2^12 columns:
+    the same row length as the workload and 1/8 of its rows (about 15 s of CPU work). A is then
+    8 MiB, so the sample runs at least as fast per element as the full workload would (a
+    conservative baseline). It runs in a child process pinned to one core (sched_setaffinity,
+    as `taskset -c <core>`), so nothing else of this process shares that core."""
+    import subprocess
+
+    allowed = sorted(os.sched_getaffinity(0))
+    core = allowed[-1]
+    code = ("import os, sys; sys.path.insert(0, %r); os.sched_setaffinity(0, {%d}); "
+            "import bench; print(repr(bench._cpu_sample_run()))" % (ROOT, core))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True)
+    dt = float(r.stdout.strip().splitlines()[-1])
+    lm, ln = _CPU_SAMPLE
     return {
         "value": (1 << (lm + ln)) / dt / 1e6,
         "unit": "Melem/s",
@@ -78,6 +111,10 @@ def cpu_baseline(seconds_hint: float = 15.0) -> dict:
         "kind": "port",
         "sample": f"oracle commit of 2^{lm} x 2^{ln} (mu={lm + ln}, the workload's row length and "
                   f"1/{1 << (LOG_M - lm)} of its rows), {dt:.2f} s, one thread",
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
+        "cpus_allowed": len(allowed),
+        "pinning": f"child process pinned to core {core} with sched_setaffinity (taskset -c {core})",
     }
 
 
@@ -106,7 +143,10 @@ def main():
                     help="commit: the headline metric; prove: mu=24 eval-prove at 2^8 x 2^16; "
                          "batch: 64 polynomials with mu uniform in [16, 24] in one batched commit")
     ap.add_argument("--batch", type=int, default=64, help="batch workload: number of polynomials")
-    ap.add_argument("--requests", type=int, default=96, help="serve workload: number of requests")
+    ap.add_argument("--requests", type=int, default=500, help="serve workload: number of requests")
+    ap.add_argument("--commit-only", action="store_true",
+                    help="commit workload: only the commit metric (no prove, served-stream or "
+                         "host-memory figures)")
     ap.add_argument("--prove-workers", type=int, default=2, help="serve workload: prove worker threads")
     ap.add_argument("--max-batch", type=int, default=4, help="serve workload: requests per batched commit")
     ap.add_argument("--log-m", type=int, default=8, help="prove workload: log2 rows")
@@ -132,13 +172,18 @@ def main():
         bench_commit(args)
 
 
-def bench_serve(args):
-    """Served mixed stream: `--requests` commit-then-open requests, mu uniform in [14, 26] with
+SERVE_LOADS = (0.6, 0.7, 0.8, 0.85, 0.9, 0.95, 1.0, 1.05, 1.1, 1.2)
+
+
+def measure_serve(nreq: int, workers: int, max_batch: int, loads=SERVE_LOADS, seed: int = 7) -> dict:
+    """Served mixed stream: `nreq` commit-then-open requests, mu uniform in [14, 26] with
     log_m = floor((mu - 7) / 2) (the reference's shape window), f in pinned host memory, eval
     points uniform in F_{p^2}^mu. Loads: saturation (everything queued at t = 0: polys/s) and
-    Poisson arrivals at 50 / 70 / 80 / 90 / 100 / 110 % of the per-request path's capacity (p50 / p99
+    Poisson arrivals at the given fractions of the per-request path's capacity (p50 / p99
     latency). Each is run through the serving loop and through the plain per-request path
-    (commit, then prove, one request at a time in arrival order)."""
+    (commit, then prove, one request at a time in arrival order). "Equal p99": the target is the
+    per-request path's p99 at 70 % load; each path's sustained rate is the highest swept rate at
+    which its p99 stays within it."""
     import threading
 
     import torch
@@ -146,8 +191,7 @@ def bench_serve(args):
     import ligesis_amd as L
     from ligesis_amd import synth
 
-    rng = np.random.default_rng(7)
-    nreq = args.requests
+    rng = np.random.default_rng(seed)
     mus = [int(x) for x in rng.integers(14, 27, size=nreq)]
     shapes = sorted(set(mus))
     print(f"bench: served stream of {nreq} requests, mu in [14,26]", file=sys.stderr, flush=True)
@@ -163,6 +207,7 @@ def bench_serve(args):
     reqs = []
     for i, mu in enumerate(mus):
         reqs.append((mu, synth.ext_elems(5000 + i, mu), synth.ext_elems(6000 + i, 256), synth.oracle_ints(7000 + i, 640)))
+    upload_bytes = sum(8 << mu for mu in mus)
 
     def host_f(mu):
         return fs[mu].numpy().view(np.uint64)
@@ -181,7 +226,7 @@ def bench_serve(args):
         return lat, time.perf_counter() - t0
 
     # one server for every pass (as in service: its contexts, pools and twiddles stay warm)
-    srv = L.Server(0, hbm_budget=64 << 30, prove_workers=args.prove_workers, max_batch=args.max_batch)
+    srv = L.Server(0, hbm_budget=64 << 30, prove_workers=workers, max_batch=max_batch)
 
     def served(arrivals):
         b0 = srv.stats()["batches"]
@@ -207,67 +252,81 @@ def bench_serve(args):
         return lat, span, {"batches": srv.stats()["batches"] - b0}
 
     # warm both paths (kernels, twiddles, pools)
-    per_request([0.0] * nreq)
+    warm = min(nreq, 64)
+    per_request([0.0] * warm)
     served([0.0] * nreq)
     zero = [0.0] * nreq
     lat_pr, span_pr = per_request(zero)
     lat_sv, span_sv, st = served(zero)
     cap = nreq / span_pr
+    print(f"bench: saturation: served {nreq / span_sv:.0f} polys/s, per-request {cap:.0f} polys/s",
+          file=sys.stderr, flush=True)
 
     def pct(v, q):
         return float(np.percentile(np.array(v), q))
 
-    # Poisson arrivals at fractions of the per-request path's capacity, one arrival sequence per
-    # load (shared by both paths); "equal p99": the highest swept load at which each path keeps
-    # its p99 within the per-request path's p99 at 70 % load.
-    loads = []
-    for frac in (0.5, 0.7, 0.8, 0.9, 1.0, 1.1):
+    res = []
+    for frac in loads:
         rate = frac * cap
         arrivals = list(np.cumsum(rng.exponential(1.0 / rate, size=nreq)))
         lat_pr_p, span_pr_p = per_request(arrivals)
         lat_sv_p, span_sv_p, st_p = served(arrivals)
-        loads.append({"load": frac, "rate_per_s": rate,
-                      "served_p50_ms": pct(lat_sv_p, 50), "served_p99_ms": pct(lat_sv_p, 99),
-                      "per_request_p50_ms": pct(lat_pr_p, 50), "per_request_p99_ms": pct(lat_pr_p, 99),
-                      "served_polys_per_s": nreq / span_sv_p, "per_request_polys_per_s": nreq / span_pr_p,
-                      "batches": st_p["batches"]})
-        print(f"bench: load {frac}: p99 served {loads[-1]['served_p99_ms']:.1f} ms, "
-              f"per-request {loads[-1]['per_request_p99_ms']:.1f} ms", file=sys.stderr, flush=True)
+        res.append({"load": frac, "rate_per_s": rate,
+                    "served_p50_ms": pct(lat_sv_p, 50), "served_p99_ms": pct(lat_sv_p, 99),
+                    "per_request_p50_ms": pct(lat_pr_p, 50), "per_request_p99_ms": pct(lat_pr_p, 99),
+                    "served_polys_per_s": nreq / span_sv_p, "per_request_polys_per_s": nreq / span_pr_p,
+                    "batches": st_p["batches"]})
+        print(f"bench: load {frac}: p99 served {res[-1]['served_p99_ms']:.1f} ms, "
+              f"per-request {res[-1]['per_request_p99_ms']:.1f} ms", file=sys.stderr, flush=True)
     srv.stop()
-    target = next(l for l in loads if l["load"] == 0.7)["per_request_p99_ms"]
+    ref = [l for l in res if l["load"] == 0.7] or res[:1]
+    target = ref[0]["per_request_p99_ms"]
 
     def best_rate(key):
-        ok = [l["rate_per_s"] for l in loads if l[key] <= target]
+        ok = [l["rate_per_s"] for l in res if l[key] <= target]
         return max(ok) if ok else None
 
     out = {
+        "requests": nreq,
+        "prove_workers": workers,
+        "max_batch": max_batch,
+        "mus": mus,
+        "upload_bytes": upload_bytes,
+        "saturation": {"served_polys_per_s": nreq / span_sv, "per_request_polys_per_s": cap,
+                       "served_p50_ms": pct(lat_sv, 50), "served_p99_ms": pct(lat_sv, 99),
+                       "per_request_p50_ms": pct(lat_pr, 50), "per_request_p99_ms": pct(lat_pr, 99),
+                       "served_upload_gbs": upload_bytes / span_sv / 1e9, "batches": st["batches"]},
+        "poisson": res,
+        "equal_p99": {"p99_target_ms": target, "served_rate_per_s": best_rate("served_p99_ms"),
+                      "per_request_rate_per_s": best_rate("per_request_p99_ms")},
+    }
+    del keys, fs
+    ctx.close()
+    return out
+
+
+def bench_serve(args):
+    """The served mixed stream (measure_serve) as the measured workload."""
+    sv = measure_serve(args.requests, args.prove_workers, args.max_batch)
+    out = {
         "metric": "LigeSIS served commit+open polys/s and p50/p99 latency (mu uniform in [14,26])",
-        "value": nreq / span_sv,
+        "value": sv["saturation"]["served_polys_per_s"],
         "unit": "polys/s",
         "n_gpus": 1,
         "steps": 1,
         "warmup": 1,
-        "ms_per_step": span_sv * 1e3,
+        "ms_per_step": args.requests / sv["saturation"]["served_polys_per_s"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64 / F_p^2 (Goldilocks)",
         "data": "synthetic (seeded f per shape in pinned host memory, seeded z and oracle streams)",
-        "config": {"workload": f"{nreq} commit-then-open requests, mu uniform in [14,26], "
+        "config": {"workload": f"{args.requests} commit-then-open requests, mu uniform in [14,26], "
                                f"log_m = floor((mu-7)/2), HBM budget 64 GiB, {args.prove_workers} prove workers, "
-                               f"batches of up to {args.max_batch}",
-                   "mus": mus},
-        "saturation": {"served_polys_per_s": nreq / span_sv, "per_request_polys_per_s": nreq / span_pr,
-                       "served_p50_ms": pct(lat_sv, 50), "served_p99_ms": pct(lat_sv, 99),
-                       "per_request_p50_ms": pct(lat_pr, 50), "per_request_p99_ms": pct(lat_pr, 99),
-                       "batches": st["batches"]},
-        "poisson": loads,
-        "equal_p99": {"p99_target_ms": target, "served_rate_per_s": best_rate("served_p99_ms"),
-                      "per_request_rate_per_s": best_rate("per_request_p99_ms")},
+                               f"batches of up to {args.max_batch}"},
+        "serve": sv,
         "cpu_baseline": None,
     }
-    del keys, fs
-    ctx.close()
     print(json.dumps(out), flush=True)
 
 
@@ -354,39 +413,64 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_prove(args):
-    """Eval-prove at mu = 24 on a reference-valid shape (2^8 x 2^16 by default; the reference's
-    own bench shape 2^7 x 2^17 with --log-m 7): one step = one full proof (all seven sumchecks
-    and the batched Basefold open) on a resident commitment, ending with the proof bytes on the
-    host."""
+def measure_prove(lm: int, steps: int, warmup: int, prove_graph: int = 0, profile: bool = True) -> dict:
+    """Eval-prove at mu = 24 on a reference-valid shape (2^8 x 2^16; the reference's own bench
+    shape is 2^7 x 2^17): one step = one full proof (all seven sumchecks and the batched Basefold
+    open) on a resident commitment, ending with the proof bytes on the host."""
     import ligesis_amd as L
     from ligesis_amd import synth
 
-    lm = args.log_m
     ln = MU - lm
     print(f"bench: prove mu={MU} shape 2^{lm} x 2^{ln}", file=sys.stderr, flush=True)
     inp = synth.Inputs(lm, ln, seed=5)
     ctx = L.Context(0)
-    ctx.set_option("prove_graph", args.prove_graph)
+    ctx.set_option("prove_graph", prove_graph)
     pk, vk = L.Ligesis.setup(lm, ln, inp.a, ctx=ctx)
     root, cd = L.Ligesis.commit(pk, inp.f)
     settle(lambda: L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints)))
     first = None
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         first = L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints))
     # each step's proof replaces the last one, as a caller that sends proofs on would hold them
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         proof = L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints))
     dt = time.perf_counter() - t0
     assert first is None or proof == first, "prove is not deterministic"
-    ms = dt / args.steps * 1e3
-    # one more proof with host-side timing: how long issuing the work takes vs the whole proof
-    ctx.set_option("profile", 1)
-    L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints))
-    raw = np.zeros(28, np.float64)
-    ctx.check(L.lib().lgs_stage_times(ctx.h, L._ptr(raw), 28))
-    ctx.set_option("profile", 0)
+    ms = dt / steps * 1e3
+    out = {"ms": ms, "log_m": lm, "log_n": ln, "steps": steps, "proof_bytes": len(proof),
+           "proof_sha256": __import__("hashlib").sha256(proof).hexdigest()}
+    if profile:
+        # one more proof with host-side timing: how long issuing the work takes vs the whole proof
+        ctx.set_option("profile", 1)
+        L.Ligesis.prove(pk, cd, inp.z, L.RandomOracle(inp.fields, inp.ints))
+        raw = np.zeros(28, np.float64)
+        ctx.check(L.lib().lgs_stage_times(ctx.h, L._ptr(raw), 28))
+        ctx.set_option("profile", 0)
+        out.update({
+            "host_issue_ms": float(raw[5]),
+            "device_bound_ms": float(raw[6]),
+            "device_phases_ms": {"sumcheck_start": float(raw[7]), "sumcheck_end": float(raw[8]),
+                                 "basefold_open_end": float(raw[9]), "gathered": float(raw[10]),
+                                 "downloaded": float(raw[11]), "side_start": float(raw[12]),
+                                 "bi_committed": float(raw[13]), "abar_committed": float(raw[14]),
+                                 "combined": float(raw[15]), "tail_rounds_start": float(raw[16])},
+            "host_issued_ms": {"first_upload": float(raw[17]), "sumcheck_start": float(raw[18]),
+                               "sumcheck_end": float(raw[19]), "basefold_open_end": float(raw[20]),
+                               "gathered": float(raw[21]), "downloaded": float(raw[22]),
+                               "side_start": float(raw[23]), "bi_committed": float(raw[24]),
+                               "abar_committed": float(raw[25]), "combined": float(raw[26]),
+                               "tail_rounds_start": float(raw[27])},
+        })
+    del cd, pk
+    ctx.close()
+    return out
+
+
+def bench_prove(args):
+    pv = measure_prove(args.log_m, args.steps, args.warmup, args.prove_graph)
+    ms = pv.pop("ms")
+    lm, ln = pv["log_m"], pv["log_n"]
     out = {
         "metric": "LigeSIS eval-prove ms at mu=24",
         "value": ms,
@@ -402,25 +486,11 @@ def bench_prove(args):
         "data": "synthetic (SplitMix64-seeded f, A, z and oracle streams)",
         "config": {"workload": f"LigeSIS eval-prove, mu=24, m x n = 2^{lm} x 2^{ln}, s=128, "
                                "100 queries, proof on a resident commitment", "log_m": lm, "log_n": ln},
-        "host_issue_ms": float(raw[5]),
-        "device_bound_ms": float(raw[6]),
-        "device_phases_ms": {"sumcheck_start": float(raw[7]), "sumcheck_end": float(raw[8]),
-                             "basefold_open_end": float(raw[9]), "gathered": float(raw[10]),
-                             "downloaded": float(raw[11]), "side_start": float(raw[12]),
-                             "bi_committed": float(raw[13]), "abar_committed": float(raw[14]),
-                             "combined": float(raw[15]), "tail_rounds_start": float(raw[16])},
-        "host_issued_ms": {"first_upload": float(raw[17]), "sumcheck_start": float(raw[18]),
-                           "sumcheck_end": float(raw[19]), "basefold_open_end": float(raw[20]),
-                           "gathered": float(raw[21]), "downloaded": float(raw[22]),
-                           "side_start": float(raw[23]), "bi_committed": float(raw[24]),
-                           "abar_committed": float(raw[25]), "combined": float(raw[26]),
-                           "tail_rounds_start": float(raw[27])},
-        "proof_bytes": len(proof),
+        **pv,
     }
-    del cd, pk
-    ctx.close()
     if not args.no_cpu_baseline:
         import oracle
+        from ligesis_amd import synth
 
         slm, sln = 6, 14  # mu = 20 opening shape
         s_inp = synth.Inputs(slm, sln, seed=6)
@@ -435,6 +505,29 @@ def bench_prove(args):
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+
+
+def measure_host_commit(pk, f_host: np.ndarray, steps: int = 5) -> dict:
+    """The same commit with f in host memory (uploaded over PCIe inside the call): pageable
+    numpy memory, and pinned memory. Not the headline value, which starts from HBM."""
+    import torch
+
+    import ligesis_amd as L
+
+    pinned = torch.from_numpy(f_host.view(np.int64)).pin_memory()
+    out = {}
+    for name, src in (("pageable", f_host), ("pinned", pinned.numpy().view(np.uint64))):
+        r, cd = L.Ligesis.commit(pk, src)
+        cd.release()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r, cd = L.Ligesis.commit(pk, src)
+            cd.release()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        out[f"{name}_ms"] = ms
+        out[f"{name}_melem_per_s"] = f_host.size / (ms / 1e3) / 1e6
+    del pinned
+    return out
 
 
 def bench_commit(args):
@@ -548,8 +641,25 @@ def bench_commit(args):
         "hash_rate_gelem_adds_per_s": (32.0 * 64 * m * 2 * n) / (avg["hash_ms"] / 1e3) / 1e9,
         "root": next(iter(roots)).hex(),
     }
-    del pk
+    ctx.set_option("profile", 0)
+    if not args.commit_only:
+        print("bench: commit from host memory (PCIe upload inside the call)", file=sys.stderr, flush=True)
+        out["host_resident_commit"] = measure_host_commit(pk, inp.f)
+    del pk, f_dev
     ctx.close()
+    if not args.commit_only:
+        # the second half of the metric: eval-prove at mu = 24 and the served stream's latency
+        out["prove_mu24"] = [measure_prove(lm, steps=10, warmup=2, profile=False) for lm in (8, 7)]
+        sv = measure_serve(args.requests, args.prove_workers, args.max_batch, loads=(0.7, 0.8, 0.9, 1.0, 1.1))
+        out["serve"] = sv
+        at = next(l for l in sv["poisson"] if l["load"] == 0.8)
+        out["latency"] = {"load": "Poisson arrivals at 0.8 of the per-request capacity "
+                                  f"({at['rate_per_s']:.0f} requests/s), {args.requests} requests, mu in [14,26]",
+                          "served_p50_ms": at["served_p50_ms"], "served_p99_ms": at["served_p99_ms"],
+                          "per_request_p50_ms": at["per_request_p50_ms"],
+                          "per_request_p99_ms": at["per_request_p99_ms"],
+                          "served_polys_per_s_saturated": sv["saturation"]["served_polys_per_s"],
+                          "per_request_polys_per_s_saturated": sv["saturation"]["per_request_polys_per_s"]}
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (oracle, one thread)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()

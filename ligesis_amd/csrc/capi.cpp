@@ -5,20 +5,7 @@
 #include <exception>
 #include <string>
 
-#include "runtime.hpp"
-
-struct lgs_ctx {
-  lgs::Context c;
-};
-struct lgs_key {
-  lgs::Key k;
-};
-struct lgs_commitment {
-  lgs::Commitment c;
-};
-struct lgs_server {
-  lgs::Server* s;
-};
+#include "handles.hpp"
 
 namespace {
 

@@ -210,24 +210,16 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
   c.tree.nodes = DevBuf<uint32_t>(&ctx.pool, (size_t)c.tree.node_count() * 8);
   const uint64_t* tw = ctx.twiddle(logN);
   const uint64_t* tp = tu.ntt_impl == 4 ? ctx.pass_twiddle(logN) : nullptr;
-  if (ctx.profile) hip_check(hipEventRecord(ctx.ev[0], ctx.stream), "event");
-  // H = A . B with B read bitwise from F' (ligesis.rs:292-302)
-  bool mfma_ok = (two_n % 128 == 0) && m >= 2;
-  int variant = tu.hash_variant == 0 ? (mfma_ok ? 1 : 2) : tu.hash_variant;
-  if (variant == 1 && !mfma_ok) throw Error(1, "MFMA hash needs 2n % 128 == 0 and m >= 2");
-  uint32_t ks = 0;
-  if (variant == 1) {
-    uint64_t ntiles = two_n / 128;
-    ks = tu.hash_ksplit ? (uint32_t)tu.hash_ksplit : pow2_at_least((512 + ntiles - 1) / ntiles);
-    if (ks > m / 2) ks = (uint32_t)(m / 2);
-    if (ks == 0 || (m % (2 * ks)) != 0) throw Error(1, "hash_ksplit must divide m / 2");
-  }
+  const HashPlan plan = hash_plan(key, tu);
+  const int variant = plan.variant;
+  const uint32_t ks = plan.ks;
   const uint32_t chunks = (uint32_t)tu.pipeline;
   if (variant == 1 && chunks > 1 && ks % chunks == 0 && chunks <= 8) {
     // Pipelined: row chunk q (the rows of K-splits [q ks/C, (q+1) ks/C)) is encoded on the
     // context stream, then hashed on the second stream while chunk q + 1 is encoded, so the
     // VALU-bound transform and the MFMA-bound hash share the device.
     hipStream_t sh = ctx.side_stream();
+    if (ctx.profile) hip_check(hipEventRecord(ctx.ev[0], ctx.stream), "event");
     const uint64_t rows = m / chunks;
     const uint32_t kc = ks / chunks;
     DevBuf<int32_t> part(&ctx.pool, (size_t)ks * 256 * two_n);
@@ -263,30 +255,71 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
     return;
   }
   ctx.prof_chunks = 0;
+  CommitIO io;
+  io.f = c.f;
+  io.enc = c.enc.ptr;
+  io.h = c.h.ptr;
+  io.nodes = c.tree.nodes.ptr;
+  io.root_host = c.root_host;
+  io.tw = tw;
+  io.tp = tp;
+  // freed when this returns: later allocations on the context reuse it in stream order
+  DevBuf<uint64_t> part(&ctx.pool, (hash_partial_bytes(plan, two_n) + 7) / 8);
+  io.part = part.ptr;
+  enqueue_commit_stages(ctx, key, io, plan, tu, ctx.stream, ctx.profile);
+}
+
+HashPlan hash_plan(const Key& key, const Tuning& tu) {
+  const uint64_t m = uint64_t(1) << key.log_m, two_n = uint64_t(2) << key.log_n;
+  HashPlan hp;
+  const bool mfma_ok = (two_n % 128 == 0) && m >= 2;
+  hp.variant = tu.hash_variant == 0 ? (mfma_ok ? 1 : 2) : tu.hash_variant;
+  if (hp.variant == 1 && !mfma_ok) throw Error(1, "MFMA hash needs 2n % 128 == 0 and m >= 2");
+  hp.impl = tu.hash_impl;
+  if (hp.variant == 1) {
+    const uint64_t ntiles = two_n / 128;
+    hp.ks = tu.hash_ksplit ? (uint32_t)tu.hash_ksplit : pow2_at_least((512 + ntiles - 1) / ntiles);
+    if (hp.ks > m / 2) hp.ks = (uint32_t)(m / 2);
+    if (hp.ks == 0 || (m % (2 * hp.ks)) != 0) throw Error(1, "hash_ksplit must divide m / 2");
+  } else {
+    const uint64_t nb = (two_n + 255) / 256;
+    hp.ks = tu.hash_ksplit ? (uint32_t)tu.hash_ksplit : pow2_at_least((1024 + nb - 1) / nb);
+    if (hp.ks > m) hp.ks = (uint32_t)m;
+    if (hp.ks == 0 || m % hp.ks) throw Error(1, "hash_ksplit must divide m");
+  }
+  return hp;
+}
+
+uint64_t hash_partial_bytes(const HashPlan& hp, uint64_t two_n) {
+  // MFMA kernels that reduce in their epilogue and the VALU kernel leave one word per
+  // (split, row, column); the others eight int32 digit-plane sums per (split, row, column)
+  if (hp.variant == 1 && !hash_impl_reduces(hp.impl)) return (uint64_t)hp.ks * 256 * two_n * 4;
+  return (uint64_t)hp.ks * 32 * two_n * 8;
+}
+
+void enqueue_commit_stages(Context& ctx, const Key& key, const CommitIO& io, const HashPlan& hp, const Tuning& tu,
+                           hipStream_t st, bool profile) {
+  const uint64_t m = uint64_t(1) << key.log_m, n = uint64_t(1) << key.log_n, two_n = 2 * n;
+  const int logN = key.log_n + 1;
+  if (profile) hip_check(hipEventRecord(ctx.ev[0], st), "event");
   // F'[i] = RS(F[i]) (ligesis.rs:289-291)
-  hip_check(launch_ntt(c.f, n, 1, (uint32_t)n, (uint32_t)m, logN, tw, c.enc.ptr, ctx.stream, tu.ntt_impl, tp),
+  hip_check(launch_ntt(io.f, n, 1, (uint32_t)n, (uint32_t)m, logN, io.tw, io.enc, st, tu.ntt_impl, io.tp),
             "row encode");
-  if (ctx.profile) hip_check(hipEventRecord(ctx.ev[1], ctx.stream), "event");
-  if (variant == 1) {
-    DevBuf<int32_t> part(&ctx.pool, (size_t)ks * 256 * two_n);
-    hip_check(launch_hash_mfma(c.enc.ptr, two_n, (uint32_t)m, (uint32_t)two_n, key.dig.ptr, ks, part.ptr,
-                               c.h.ptr, ctx.stream, ctx.profile ? ctx.ev[2] : nullptr, tu.hash_impl),
+  if (profile) hip_check(hipEventRecord(ctx.ev[1], st), "event");
+  // H = A . B with B read bitwise from F' (ligesis.rs:292-302)
+  if (hp.variant == 1) {
+    hip_check(launch_hash_mfma(io.enc, two_n, (uint32_t)m, (uint32_t)two_n, key.dig.ptr, hp.ks, (int32_t*)io.part,
+                               io.h, st, profile ? ctx.ev[2] : nullptr, hp.impl),
               "hash (MFMA)");
   } else {
-    uint64_t nb = (two_n + 255) / 256;
-    uint32_t ks = tu.hash_ksplit ? (uint32_t)tu.hash_ksplit : pow2_at_least((1024 + nb - 1) / nb);
-    if (ks > m) ks = (uint32_t)m;
-    if (m % ks) throw Error(1, "hash_ksplit must divide m");
-    DevBuf<uint64_t> part(&ctx.pool, (size_t)ks * 32 * two_n);
-    hip_check(launch_hash_valu(c.enc.ptr, two_n, (uint32_t)m, (uint32_t)two_n, key.at.ptr, ks, part.ptr,
-                               c.h.ptr, ctx.stream, ctx.profile ? ctx.ev[2] : nullptr),
+    hip_check(launch_hash_valu(io.enc, two_n, (uint32_t)m, (uint32_t)two_n, key.at.ptr, hp.ks, io.part, io.h, st,
+                               profile ? ctx.ev[2] : nullptr),
               "hash (VALU)");
   }
-  if (ctx.profile) hip_check(hipEventRecord(ctx.ev[3], ctx.stream), "event");
+  if (profile) hip_check(hipEventRecord(ctx.ev[3], st), "event");
   // column-wise Merkle tree over serialize(H[:, j]) (ligesis.rs:306-313)
-  hip_check(launch_merkle(c.h.ptr, 32, two_n, 1, two_n, c.tree.nodes.ptr, ctx.stream, tu.merkle_lg, c.root_host),
-            "merkle");
-  if (ctx.profile) hip_check(hipEventRecord(ctx.ev[4], ctx.stream), "event");
+  hip_check(launch_merkle(io.h, 32, two_n, 1, two_n, io.nodes, st, tu.merkle_lg, io.root_host), "merkle");
+  if (profile) hip_check(hipEventRecord(ctx.ev[4], st), "event");
 }
 
 void enqueue_merkle_trees(Context& ctx, const std::vector<MerkleSource>& trees,

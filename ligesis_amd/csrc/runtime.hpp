@@ -297,6 +297,31 @@ struct GroupHold {
 
 // commit.cpp; `force` replaces the resolved schedule (autotune candidates)
 void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* force = nullptr);
+// The hash kernel of a commit: variant (1 MFMA, 2 VALU), MFMA kernel and K-split.
+struct HashPlan {
+  int variant = 1;
+  int impl = 12;
+  uint32_t ks = 1;
+};
+HashPlan hash_plan(const Key& key, const Tuning& tu);
+uint64_t hash_partial_bytes(const HashPlan& hp, uint64_t two_n);  // K-split partials of one commit
+// Device buffers of one commit: f (m x n), F' (m x 2n), H (32 x 2n), the tree's nodes
+// ((4n - 1) x 8 words), the hash partials (hash_partial_bytes), the transform's twiddle table and
+// (ntt_impl 4 only) its pass table; root_host, if set, receives the root from the tree's last launch.
+struct CommitIO {
+  const uint64_t* f = nullptr;
+  uint64_t* enc = nullptr;
+  uint64_t* h = nullptr;
+  uint32_t* nodes = nullptr;
+  uint64_t* part = nullptr;
+  const uint64_t* tw = nullptr;
+  const uint64_t* tp = nullptr;
+  uint32_t* root_host = nullptr;
+};
+// Row transform, hash, finalize and Merkle tree of one commit on stream st, into io's buffers;
+// allocates nothing.
+void enqueue_commit_stages(Context& ctx, const Key& key, const CommitIO& io, const HashPlan& hp, const Tuning& tu,
+                           hipStream_t st, bool profile);
 bool group_eligible(const Key& key);
 void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, Commitment*>>& items,
                           GroupHold& hold);

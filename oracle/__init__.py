@@ -47,7 +47,7 @@ def lib():
             "lgso_merkle_root", "lgso_merkle_open", "lgso_merkle_verify", "lgso_sumcheck_prove",
             "lgso_sumcheck_verify", "lgso_basefold_batch_roundtrip", "lgso_shape_ok",
             "lgso_pk_com_a", "lgso_cd_enc", "lgso_cd_h", "lgso_prove", "lgso_verify",
-            "lgso_proof_reencode", "lgso_basefold_commit",
+            "lgso_proof_reencode", "lgso_basefold_commit", "lgso_basefold_prove", "lgso_basefold_verify",
         ]:
             getattr(L, name).restype = i64
         L.lgso_setup.restype = vp
@@ -64,6 +64,8 @@ def lib():
         L.lgso_proof_reencode.argtypes = [vp, u64, vp]
         L.lgso_shape_ok.argtypes = [u64, u64]
         L.lgso_basefold_commit.argtypes = [vp, u64, i64, u64, u64, vp]
+        L.lgso_basefold_prove.argtypes = [vp, u64, i64, u64, u64, vp, vp, u64, vp, u64, vp, vp, u64, vp]
+        L.lgso_basefold_verify.argtypes = [vp, vp, u64, vp, u64, i64, vp, u64, vp, u64]
         _lib = L
     return _lib
 
@@ -220,6 +222,38 @@ def basefold_commit(poly, li: int, code_rate: int) -> bytes:
     rc = lib().lgso_basefold_commit(_p(flat), nv, int(ext), li, code_rate, _p(root))
     assert rc >= 0, err()
     return root.tobytes()
+
+
+def basefold_prove(poly, li: int, code_rate: int, point, fields, ints):
+    """Basefold::commit_{base,ext}_on_domain then Basefold::prove (basefold.rs:341-516) at
+    ``point`` (nv extension elements). Returns (root, proof bytes, (fields used, ints used))."""
+    a = _u64(poly)
+    ext = a.ndim == 2
+    nv = a.shape[0].bit_length() - 1
+    flat = np.ascontiguousarray(a).reshape(-1)
+    point, fields, ints = _u64(point).reshape(-1, 2), _u64(fields).reshape(-1, 2), _u64(ints)
+    args = (_p(flat), nv, int(ext), li, code_rate, _p(point), _p(fields), fields.shape[0], _p(ints), ints.size)
+    root = np.zeros(32, np.uint8)
+    need = lib().lgso_basefold_prove(*args, _p(root), None, 0, None)
+    if need < 0:
+        raise RuntimeError(err())
+    out = np.zeros(need, np.uint8)
+    used = np.zeros(2, np.uint64)
+    got = lib().lgso_basefold_prove(*args, _p(root), _p(out), need, _p(used))
+    assert got == need, err()
+    return root.tobytes(), out.tobytes(), (int(used[0]), int(used[1]))
+
+
+def basefold_verify(root: bytes, point, proof: bytes, ext: bool, fields, ints) -> bool:
+    """Basefold::verify (basefold.rs:518-642) of a proof in the canonical encoding."""
+    point, fields, ints = _u64(point).reshape(-1, 2), _u64(fields).reshape(-1, 2), _u64(ints)
+    rt = np.frombuffer(root, np.uint8).copy()
+    pf = np.frombuffer(proof, np.uint8).copy() if proof else np.zeros(1, np.uint8)
+    rc = lib().lgso_basefold_verify(_p(rt), _p(point), point.shape[0], _p(pf), len(proof), int(ext),
+                                    _p(fields), fields.shape[0], _p(ints), ints.size)
+    if rc < 0:
+        raise RuntimeError(err())
+    return bool(rc)
 
 
 def basefold_batch_roundtrip(bf_polys, ef_polys, points, rounds, code_rate, fields, ints, tamper=0):

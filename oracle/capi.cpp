@@ -5,6 +5,7 @@
 #include <exception>
 #include <string>
 
+#include "basefold_single.hpp"
 #include "ligesis.hpp"
 
 using namespace lgso;
@@ -186,6 +187,52 @@ int64_t lgso_basefold_commit(const uint64_t* poly, uint64_t nv, int64_t ext, uin
     }
     std::copy(c.begin(), c.end(), root);
     return int64_t(1);
+  });
+}
+
+// ---- basefold, one polynomial (basefold.rs:384-642) ----
+// Commits poly (2^nv base values, or EF pairs with ext) with log_interleave li and code_rate,
+// proves it at `point` (nv EF) from the oracle streams; returns the proof length and writes it
+// when cap suffices; used[0..1] = fields / ints drawn; root receives the commitment.
+int64_t lgso_basefold_prove(const uint64_t* poly, uint64_t nv, int64_t ext, uint64_t li, uint64_t code_rate,
+                            const uint64_t* point, const uint64_t* fields, uint64_t nf, const uint64_t* ints,
+                            uint64_t ni, uint8_t* root, uint8_t* out, uint64_t cap, uint64_t* used) {
+  return guard([&] {
+    const size_t len = size_t(1) << nv;
+    OracleStream o = make_stream(fields, nf, ints, ni);
+    SingleProof pf;
+    H32 c;
+    if (ext) {
+      BfState<Ef> st;
+      c = commit_on_domain(load_ef(poly, len), code_rate, li, st);
+      pf = single_prove(st, load_ef(point, nv), o);
+    } else {
+      BfState<uint64_t> st;
+      c = commit_on_domain(std::vector<uint64_t>(poly, poly + len), code_rate, li, st);
+      pf = single_prove(st, load_ef(point, nv), o);
+    }
+    auto bytes = encode_single_proof(pf);
+    if (root) std::copy(c.begin(), c.end(), root);
+    if (used) {
+      used[0] = o.fi;
+      used[1] = o.ii;
+    }
+    if (out && cap >= bytes.size()) std::memcpy(out, bytes.data(), bytes.size());
+    return (int64_t)bytes.size();
+  });
+}
+
+// 1 when the proof decodes and verifies against root at `point` (np EF), else 0.
+int64_t lgso_basefold_verify(const uint8_t* root, const uint64_t* point, uint64_t np, const uint8_t* proof,
+                             uint64_t plen, int64_t ext, const uint64_t* fields, uint64_t nf, const uint64_t* ints,
+                             uint64_t ni) {
+  return guard([&] {
+    SingleProof pf;
+    if (!decode_single_proof(proof, plen, ext != 0, pf)) return int64_t(0);
+    H32 c;
+    std::copy(root, root + 32, c.begin());
+    OracleStream o = make_stream(fields, nf, ints, ni);
+    return int64_t(single_verify(c, load_ef(point, np), pf, ext != 0, o) ? 1 : 0);
   });
 }
 

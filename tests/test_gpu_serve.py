@@ -50,6 +50,35 @@ def test_served_results_equal_alone(gpu_ctx):
     assert oracle.verify(shape[0], shape[1], vk.com_a, root, r.z, proof, r.fields, r.ints)
 
 
+def test_served_stream_at_full_sizes(gpu_ctx):
+    # the served stream's mu range [14, 26] at its opening shapes log_m = floor((mu - 7) / 2),
+    # two requests each of mu = 24 and 26 among smaller ones, all in flight together
+    mus = [14, 24, 18, 26, 21, 24, 26, 16]
+    keys, reqs = {}, []
+    for i, mu in enumerate(mus):
+        lm = (mu - 7) // 2
+        ln = mu - lm
+        if (lm, ln) not in keys:
+            keys[(lm, ln)] = Ligesis.setup(lm, ln, synth.field_elems(3000 + mu, 32 * (64 << lm)), ctx=gpu_ctx)
+        reqs.append(((lm, ln), synth.Inputs(lm, ln, seed=900 + i, with_a=False)))
+    srv = Server(0, hbm_budget=64 << 30, prove_workers=2, max_batch=4)
+    print("serving mu", mus, flush=True)
+    tickets = [srv.submit(keys[shape][0], r.f, r.z, RandomOracle(r.fields, r.ints)) for shape, r in reqs]
+    got = [srv.poll(t) for t in tickets]
+    srv.stop()
+    for (shape, r), (root, proof, _) in zip(reqs, got):
+        print(f"lone commit + prove shape={shape}", flush=True)
+        pk, vk = keys[shape]
+        eroot, cd = Ligesis.commit(pk, r.f)
+        eproof = Ligesis.prove(pk, cd, r.z, RandomOracle(r.fields, r.ints))
+        cd.release()
+        assert root == eroot
+        assert proof == eproof
+        assert oracle.verify(shape[0], shape[1], vk.com_a, root, r.z, proof, r.fields, r.ints)
+    for pk, _ in keys.values():
+        pk.release()
+
+
 def test_server_budget_and_errors(gpu_ctx):
     inp = synth.Inputs(1, 9, seed=1)
     pk, _ = Ligesis.setup(1, 9, inp.a, ctx=gpu_ctx)

@@ -61,3 +61,88 @@ def _restated_commit(poly, li, code_rate):
 def test_commit_root_matches_restatement(nv, li, code_rate, ext):
     poly = synth.ext_elems(nv * 10 + li, 1 << nv) if ext else synth.field_elems(nv * 10 + li, 1 << nv)
     assert oracle.basefold_commit(poly, li, code_rate) == _restated_commit(poly, li, code_rate)
+
+
+# ---- single-polynomial open: the reference tests basefold.rs:1054-1185, restated ----
+def _single(nv, li, code_rate, ext, seed, point=None):
+    poly = synth.ext_elems(seed, 1 << nv) if ext else synth.field_elems(seed, 1 << nv)
+    if point is None:
+        point = synth.ext_elems(seed + 1, nv)
+    fields = synth.ext_elems(seed + 2, 64)
+    ints = synth.oracle_ints(seed + 3, 128)
+    root, proof, used = oracle.basefold_prove(poly, li, code_rate, point, fields, ints)
+    return poly, point, fields, ints, root, proof, used
+
+
+def _claimed(proof):
+    # BasefoldProof ends with claimed_eval (two words), then num_vars, code_rate, log_interleave
+    import numpy as np
+
+    w = np.frombuffer(proof[-40:], np.uint64)
+    return w[:2], [int(x) for x in w[2:]]
+
+
+@pytest.mark.parametrize("ext", [False, True])
+def test_single_commit_prove_verify_round_trip(ext):
+    # commit_prove_verify_round_trip / commit_ext_round_trip: n = 12, code_rate 2, LOG_INTERLEAVE
+    import numpy as np
+
+    nv, li, cr = 12, 6, 2
+    poly, point, fields, ints, root, proof, used = _single(nv, li, cr, ext, 31 + ext)
+    assert root == oracle.basefold_commit(poly, li, cr)
+    claimed, tail = _claimed(proof)
+    lifted = poly if ext else np.stack([poly, np.zeros_like(poly)], axis=1)
+    np.testing.assert_array_equal(claimed, oracle.mle_eval(lifted, point))
+    assert tail == [nv, cr, li]
+    assert used == (nv - li, (100 + cr - 1) // cr)  # one field per round; query_num ints
+    assert oracle.basefold_verify(root, point, proof, ext, fields, ints)
+
+
+def _tamper(proof, offset):
+    b = bytearray(proof)
+    b[offset] ^= 1
+    return bytes(b)
+
+
+def test_single_tampered_proof_is_rejected():
+    # tampered_proof_is_rejected: reduction_msgs[0].0 changed; also l1, a base leaf value, the
+    # final codeword and the claimed value
+    nv, li, cr = 12, 6, 2
+    poly, point, fields, ints, root, proof, _ = _single(nv, li, cr, False, 41)
+    R = nv - li
+    off_l0, off_l1 = 8, 8 + 16
+    off_final = 8 + 64 * R + 8 + 32 * (R - 1) + 8
+    for off in (off_l0, off_l1, off_final, len(proof) - 40):
+        assert not oracle.basefold_verify(root, point, _tamper(proof, off), False, fields, ints)
+    # a different commitment or point
+    assert not oracle.basefold_verify(bytes(32), point, proof, False, fields, ints)
+    other = point.copy()
+    other[3, 0] ^= 1
+    assert not oracle.basefold_verify(root, other, proof, False, fields, ints)
+
+
+@pytest.mark.parametrize("nv,li,cr,ext", [(8, 0, 1, False), (9, 3, 3, True), (10, 2, 4, False), (7, 6, 1, True),
+                                           (14, 6, 1, False)])
+def test_single_round_trip_shapes(nv, li, cr, ext):
+    import numpy as np
+
+    poly, point, fields, ints, root, proof, used = _single(nv, li, cr, ext, 50 + nv + li)
+    assert oracle.basefold_verify(root, point, proof, ext, fields, ints)
+    lifted = poly if ext else np.stack([poly, np.zeros_like(poly)], axis=1)
+    np.testing.assert_array_equal(_claimed(proof)[0], oracle.mle_eval(lifted, point))
+
+
+def test_single_point_with_zero_and_one_coordinates():
+    # both l_i(0) and l_i(1) are sent, so the opening point may have 0 / 1 coordinates
+    # (basefold.rs:87-90); the batched open cannot take z_i = 0 (basefold.rs:905-909)
+    import numpy as np
+
+    nv = 10
+    point = synth.ext_elems(61, nv)
+    point[7] = [0, 0]
+    point[8] = [1, 0]
+    point[2] = [0, 0]
+    poly, point, fields, ints, root, proof, _ = _single(nv, 4, 1, False, 60, point=point)
+    assert oracle.basefold_verify(root, point, proof, False, fields, ints)
+    lifted = np.stack([poly, np.zeros_like(poly)], axis=1)
+    np.testing.assert_array_equal(_claimed(proof)[0], oracle.mle_eval(lifted, point))
