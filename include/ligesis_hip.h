@@ -148,6 +148,25 @@ int lgs_commit_batch(lgs_ctx* ctx, uint32_t count, const lgs_key* const* keys,
                      const uint64_t* const* fs, const uint64_t* lens, uint32_t flags,
                      uint8_t* roots, lgs_commitment** out);
 void lgs_commitment_release(lgs_commitment* c);
+
+/* Commit without allocating: no counterpart in the reference (its commit, ligesis.rs:277-325,
+ * allocates its outputs); for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that forbid hipMalloc). The commit of lgs_commit, with f and every
+ * output in caller-owned device memory: f = 2^(log_m + log_n) values, encoded = F' (m x 2n
+ * uint64), digests = H (32 x 2n uint64), tree = all Merkle layers ((4n - 1) x 32 bytes, leaves
+ * first, the root last). Scratch (hash K-split partials, twiddle tables) lives in one static
+ * device array of LGS_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; shapes that
+ * need more (rows of more than 2^19 values, or 32 x 2n x 8 bytes of partials above the limit)
+ * return LGS_ERR_ARG. The work is enqueued on `stream` (a hipStream_t; NULL: the context's
+ * stream); with `root` non-NULL the call also copies the root to host memory and waits for that
+ * stream, with `root` NULL it returns at once (the root is then the last 32 bytes of `tree` once
+ * the stream reaches it). Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU (two contexts, or two host threads) race.
+ * Never calls hipMalloc or hipFree. The key must come from the same context. */
+#define LGS_NOALLOC_SCRATCH_BYTES (40ull << 20)
+int lgs_commit_noalloc(lgs_ctx* ctx, const lgs_key* key, const uint64_t* f, uint64_t len, uint64_t* encoded,
+                       uint64_t* digests, uint8_t* tree, void* stream, uint8_t root[32]);
+
 /* Copy a resident stage (LGS_READ_*) to host memory; cap is in bytes. */
 int lgs_commitment_read(const lgs_commitment* c, uint32_t what, void* out, uint64_t cap);
 

@@ -112,6 +112,11 @@ int lgs_init(int device, lgs_ctx** out) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return LGS_ERR_DEVICE;
   if (hipSetDevice(device) != hipSuccess) return LGS_ERR_DEVICE;
+  try {
+    lgs::scratch_init(device);  // device address of the no-allocation entry point's scratch
+  } catch (const lgs::Error&) {
+    return LGS_ERR_DEVICE;
+  }
   auto* ctx = new lgs_ctx();
   ctx->c.owner = ctx;
   ctx->c.device = device;

@@ -360,6 +360,10 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
                    const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,
                    ProofOut out, uint64_t used[2]);
 
+// noalloc.hip: with `device` current, looks up the device address of the no-allocation entry
+// point's scratch array on that device (once per device; throws on failure)
+void scratch_init(int device);
+
 // server.cpp
 struct Server;
 uint64_t request_footprint(uint32_t log_m, uint32_t log_n, bool host_input);
