@@ -139,7 +139,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="commit workload: skip the autotuner (schedule options left at auto then "
                          "take the built-in defaults)")
-    ap.add_argument("--workload", default="commit", choices=["commit", "prove", "batch", "serve"],
+    ap.add_argument("--workload", default="commit", choices=["commit", "prove", "batch", "serve", "basefold"],
                     help="commit: the headline metric; prove: mu=24 eval-prove at 2^8 x 2^16; "
                          "batch: 64 polynomials with mu uniform in [16, 24] in one batched commit")
     ap.add_argument("--batch", type=int, default=64, help="batch workload: number of polynomials")
@@ -168,6 +168,8 @@ def main():
         bench_batch(args)
     elif args.workload == "serve":
         bench_serve(args)
+    elif args.workload == "basefold":
+        bench_basefold(args)
     else:
         bench_commit(args)
 
@@ -504,6 +506,66 @@ def bench_prove(args):
                                          "1/16 of the mu=24 workload's evaluations"}
     else:
         out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def bench_basefold(args):
+    """Standalone Basefold (the reference's pcs/benches/basefold.rs:130-151: commit_base with code
+    rate 1 and log_interleave 6, then prove at a random point) at mu = 22 and 24: commit from HBM,
+    and the eval-prove on the resident prover state ending with the proof bytes on the host."""
+    import torch
+
+    import ligesis_amd as L
+    from ligesis_amd import synth
+
+    ctx = L.Context(0)
+    rows = []
+    for nv in (22, 24):
+        print(f"bench: basefold nv={nv}", file=sys.stderr, flush=True)
+        poly = torch.from_numpy(synth.field_elems(600 + nv, 1 << nv).view(np.int64)).to("cuda:0")
+        point = synth.ext_elems(601 + nv, nv)
+        fields, ints = synth.ext_elems(602 + nv, 64), synth.oracle_ints(603 + nv, 128)
+        torch.cuda.synchronize()
+
+        def commit():
+            st, _ = L.Basefold.commit_base(poly, 1, ctx=ctx, device=True)
+            st.release()
+
+        settle(commit)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            commit()
+        commit_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        st, root = L.Basefold.commit_base(poly, 1, ctx=ctx, device=True)
+        for _ in range(args.warmup):
+            first = L.Basefold.prove(st, point, L.RandomOracle(fields, ints))
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            proof = L.Basefold.prove(st, point, L.RandomOracle(fields, ints))
+        prove_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        assert args.warmup == 0 or proof == first, "basefold prove is not deterministic"
+        st.release()
+        rows.append({"num_vars": nv, "commit_ms": commit_ms, "prove_ms": prove_ms, "proof_bytes": len(proof),
+                     "readme_cpu": {22: "commit ~1 s, eval ~0.85 s", 24: "commit ~5 s, eval ~3.5 s"}[nv]})
+    ctx.close()
+    out = {
+        "metric": "Basefold eval-prove ms (mu=24, code rate 1, log_interleave 6)",
+        "value": rows[-1]["prove_ms"],
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": rows[-1]["prove_ms"],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 / F_p^2 (Goldilocks)",
+        "data": "synthetic (SplitMix64-seeded polynomial, point and oracle streams)",
+        "config": {"workload": "standalone Basefold commit_base + prove at mu=22 and mu=24, code rate 1, "
+                               "log_interleave 6, 100 queries, polynomial resident in HBM"},
+        "rows": rows,
+        "cpu_baseline": None,
+    }
     print(json.dumps(out), flush=True)
 
 

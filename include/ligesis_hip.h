@@ -125,16 +125,33 @@ int lgs_setup(lgs_ctx* ctx, uint32_t log_m, uint32_t log_n, const uint64_t* a, u
 /* com_a of the key (ligesis.rs:71); LGS_ERR_STATE for a commit-only key. */
 int lgs_key_com_a(const lgs_key* key, uint8_t com_a[32]);
 
-#define LGS_BASEFOLD_EXT 2u /* lgs_basefold_commit: the polynomial is (c0, c1) extension pairs */
+#define LGS_BASEFOLD_EXT 2u /* Basefold commitments: the polynomial is (c0, c1) extension pairs */
 /* Standalone Basefold commitment root (Basefold::commit_{base,ext}_on_domain,
  * pcs/src/basefold.rs:341-382; commit_base / commit_ext are log_interleave 6, :309-339): the 2^li
  * chunks f[j :: 2^li] are zero-padded 2^code_rate-fold and transformed, and leaf p hashes column
  * p of the codewords into a BLAKE3 Merkle tree. poly holds 2^num_vars canonical base elements,
  * or pairs with LGS_BASEFOLD_EXT; a device pointer with LGS_INPUT_DEVICE. Limits: num_vars <= 30,
- * log_interleave <= min(6, num_vars), 1 <= code_rate <= 4. Returns the root only: the prover
- * state is not kept (the standalone Basefold prove, basefold.rs:384-516, is not provided). */
+ * log_interleave <= min(6, num_vars), 1 <= code_rate <= 4. Returns the root only; the prover state
+ * is freed (lgs_basefold_commit_state keeps it). */
 int lgs_basefold_commit(lgs_ctx* ctx, const uint64_t* poly, uint32_t num_vars, uint32_t flags,
                         uint32_t log_interleave, uint32_t code_rate, uint8_t root[32]);
+
+/* The same commitment keeping the prover state (BasefoldProverState, basefold.rs:36-54) in HBM:
+ * the interleaved codewords, their Merkle tree and the polynomial (copied from host memory, or
+ * borrowed until release with LGS_INPUT_DEVICE). */
+typedef struct lgs_basefold lgs_basefold;
+int lgs_basefold_commit_state(lgs_ctx* ctx, const uint64_t* poly, uint32_t num_vars, uint32_t flags,
+                              uint32_t log_interleave, uint32_t code_rate, uint8_t root[32], lgs_basefold** out);
+/* Basefold::prove (basefold.rs:384-516): opens the committed polynomial at `point` (point_len =
+ * num_vars extension elements, 2 * point_len uint64). The RandomOracle is its two pre-drawn
+ * streams, read from the start: one field per reduction round (num_vars - log_interleave), then
+ * query_num(code_rate) = ceil(100 / code_rate) integers. The proof is written in the canonical
+ * encoding (field order of BasefoldProof, basefold.rs:84-103) as for lgs_prove; *proof_len and
+ * used[] as there. Verification (basefold.rs:518-642) stays on the CPU with the caller. */
+int lgs_basefold_prove(lgs_ctx* ctx, const lgs_basefold* st, const uint64_t* point, uint64_t point_len,
+                       const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,
+                       uint8_t* proof, uint64_t* proof_len, uint64_t used[2]);
+void lgs_basefold_release(lgs_basefold* st);
 void lgs_key_release(lgs_key* key);
 
 /* Commit (ligesis.rs:277): f = 2^(log_m + log_n) canonical evaluations, F[i][j] = f[i*n + j].

@@ -70,6 +70,9 @@ def lib():
         L.lgs_basefold_commit.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.c_uint32, vp]
         L.lgs_key_release.argtypes = [vp]
+        L.lgs_basefold_commit_state.argtypes = [vp, vp, u32, u32, u32, u32, vp, pp]
+        L.lgs_basefold_prove.argtypes = [vp, vp, vp, u64, vp, u64, vp, u64, vp, ctypes.POINTER(ctypes.c_uint64), vp]
+        L.lgs_basefold_release.argtypes = [vp]
         L.lgs_commit.argtypes = [vp, vp, vp, u64, u32, vp, pp]
         L.lgs_commit_batch.argtypes = [vp, u32, vp, vp, vp, u32, vp, vp]
         L.lgs_commitment_release.argtypes = [vp]
@@ -424,10 +427,11 @@ class Basefold:
         return ptr, nv, is_ext, keep
 
     @staticmethod
-    def commit_on_domain(poly, code_rate: int, log_interleave: int, ctx: Context | None = None,
-                         device: bool = False, ext: bool | None = None) -> bytes:
-        """``commit_{base,ext}_on_domain``: ``poly`` is 2^nv base values, or (2^nv, 2) extension
-        pairs (on the host, or with ``device=True`` a contiguous GPU tensor of 8-byte words)."""
+    def commit_root(poly, code_rate: int, log_interleave: int, ctx: Context | None = None,
+                    device: bool = False, ext: bool | None = None) -> bytes:
+        """The root of ``commit_{base,ext}_on_domain`` only (``lgs_basefold_commit``): ``poly`` is
+        2^nv base values, or (2^nv, 2) extension pairs (on the host, or with ``device=True`` a
+        contiguous GPU tensor of 8-byte words); the prover state is not kept."""
         ptr, nv, is_ext, keep = Basefold._words(poly, ext, device)
         ctx = ctx or default_context()
         root = np.zeros(32, np.uint8)
@@ -437,25 +441,79 @@ class Basefold:
         return root.tobytes()
 
     @staticmethod
+    def _commit(poly, code_rate: int, log_interleave: int, ctx, device: bool, ext):
+        ptr, nv, is_ext, keep = Basefold._words(poly, ext, device)
+        ctx = ctx or default_context()
+        root = np.zeros(32, np.uint8)
+        h = ctypes.c_void_p()
+        flags = (LGS_BASEFOLD_EXT if is_ext else 0) | (LGS_INPUT_DEVICE if device else 0)
+        ctx.check(lib().lgs_basefold_commit_state(ctx.h, ptr, nv, flags, log_interleave, code_rate, _ptr(root),
+                                                  ctypes.byref(h)))
+        state = BasefoldProverState(ctx, h, nv, log_interleave, code_rate, is_ext, keep if device else None)
+        return state, root.tobytes()
+
+    @staticmethod
     def commit_base_on_domain(poly, code_rate: int, log_interleave: int, ctx: Context | None = None,
-                              device: bool = False) -> bytes:
-        return Basefold.commit_on_domain(poly, code_rate, log_interleave, ctx, device, ext=False)
+                              device: bool = False):
+        """``commit_base_on_domain`` (basefold.rs:341-362): returns (prover state, root)."""
+        return Basefold._commit(poly, code_rate, log_interleave, ctx, device, False)
 
     @staticmethod
     def commit_ext_on_domain(poly, code_rate: int, log_interleave: int, ctx: Context | None = None,
-                             device: bool = False) -> bytes:
-        return Basefold.commit_on_domain(poly, code_rate, log_interleave, ctx, device, ext=True)
+                             device: bool = False):
+        """``commit_ext_on_domain`` (basefold.rs:364-382): returns (prover state, root); a 1-D base
+        polynomial is taken as its upcast (c0, 0) pairs."""
+        return Basefold._commit(poly, code_rate, log_interleave, ctx, device, True)
 
     @staticmethod
-    def commit_base(poly, code_rate: int, ctx: Context | None = None) -> bytes:
+    def commit_base(poly, code_rate: int, ctx: Context | None = None, device: bool = False):
         """``commit_base`` (basefold.rs:309): a base-field polynomial, log_interleave 6."""
-        return Basefold.commit_on_domain(poly, code_rate, Basefold.LOG_INTERLEAVE, ctx, ext=False)
+        return Basefold._commit(poly, code_rate, Basefold.LOG_INTERLEAVE, ctx, device, False)
 
     @staticmethod
-    def commit_ext(poly, code_rate: int, ctx: Context | None = None) -> bytes:
-        """``commit_ext`` (basefold.rs:330): an extension-field polynomial, log_interleave 6; a 1-D
-        base polynomial is taken as its upcast (c0, 0) pairs."""
-        return Basefold.commit_on_domain(poly, code_rate, Basefold.LOG_INTERLEAVE, ctx, ext=True)
+    def commit_ext(poly, code_rate: int, ctx: Context | None = None, device: bool = False):
+        """``commit_ext`` (basefold.rs:330): an extension-field polynomial, log_interleave 6."""
+        return Basefold._commit(poly, code_rate, Basefold.LOG_INTERLEAVE, ctx, device, True)
+
+    @staticmethod
+    def prove(state: "BasefoldProverState", point, oracle: RandomOracle) -> bytes:
+        """``Basefold::prove`` (basefold.rs:384-516): opens the committed polynomial at ``point``
+        (num_vars extension elements). Reads the oracle from its start and advances its cursors
+        by the draws used (one field per reduction round, then ceil(100 / code_rate) integers)."""
+        ctx = state.ctx
+        point = _u64(point).reshape(-1, 2)
+        if oracle.fields_idx or oracle.ints_idx:
+            raise LigesisError(LGS_ERR_ARG, "prove reads the oracle streams from the start; restart() it")
+        used = np.zeros(2, np.uint64)
+        args = (ctx.h, state.h, _ptr(point), point.shape[0], _ptr(oracle.fields), oracle.fields.shape[0],
+                _ptr(oracle.ints), oracle.ints.size)
+        out = np.empty(1 << 20, np.uint8)
+        cap = ctypes.c_uint64(out.size)
+        rc = lib().lgs_basefold_prove(*args, _ptr(out), ctypes.byref(cap), _ptr(used))
+        if rc == LGS_ERR_BUFFER:
+            out = np.empty(cap.value, np.uint8)
+            rc = lib().lgs_basefold_prove(*args, _ptr(out), ctypes.byref(cap), _ptr(used))
+        ctx.check(rc)
+        oracle.fields_idx, oracle.ints_idx = int(used[0]), int(used[1])
+        return out[: cap.value].tobytes()
+
+
+class BasefoldProverState:
+    """``BasefoldProverState`` (basefold.rs:36-54) resident in HBM: codewords, Merkle tree and the
+    polynomial (``lgs_basefold_commit_state``)."""
+
+    def __init__(self, ctx: Context, h, num_vars: int, log_interleave: int, code_rate: int, ext: bool, keep):
+        self.ctx, self.h = ctx, h
+        self.num_vars, self.log_interleave, self.code_rate, self.ext = num_vars, log_interleave, code_rate, ext
+        self._keep = keep
+
+    def release(self):
+        if getattr(self, "h", None):
+            lib().lgs_basefold_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.release()
 
 
 LGS_PENDING = 8

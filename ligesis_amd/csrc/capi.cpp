@@ -257,38 +257,98 @@ int lgs_key_com_a(const lgs_key* key, uint8_t com_a[32]) {
   return LGS_OK;
 }
 
+// Commits on the context stream into `st` (uploading a host polynomial into st.poly); waits and
+// writes the root.
+static void basefold_commit_into(lgs::Context& c, const uint64_t* poly, uint32_t num_vars, uint32_t flags,
+                                 uint32_t log_interleave, uint32_t code_rate, lgs::DevBasefold& st, uint8_t root[32]) {
+  if (num_vars > 30 || log_interleave > 6 || log_interleave > num_vars || code_rate < 1 || code_rate > 4)
+    throw lgs::Error(LGS_ERR_ARG, "basefold commit: num_vars <= 30, log_interleave <= min(6, num_vars), "
+                                  "1 <= code_rate <= 4");
+  (void)hipSetDevice(c.device);
+  const bool ext = (flags & LGS_BASEFOLD_EXT) != 0;
+  const uint64_t words = (uint64_t(1) << num_vars) * (ext ? 2 : 1);
+  const uint64_t* p = poly;
+  if (!(flags & LGS_INPUT_DEVICE)) {
+    st.poly = lgs::DevBuf<uint64_t>(&c.pool, words);
+    lgs::hip_check(hipMemcpyAsync(st.poly.ptr, poly, words * 8, hipMemcpyHostToDevice, c.stream), "upload poly");
+    p = st.poly.ptr;
+  }
+  lgs::enqueue_basefold_commit(c, st, p, ext, num_vars, log_interleave, code_rate);
+  uint8_t* h = c.staging(32);
+  lgs::hip_check(hipMemcpyAsync(h, st.tree.nodes.ptr + (st.tree.node_count() - 1) * 8, 32, hipMemcpyDeviceToHost,
+                                c.stream),
+                 "read basefold root");
+  lgs::hip_check(hipStreamSynchronize(c.stream), "basefold sync");
+  std::memcpy(root, h, 32);
+  std::memcpy(st.root, h, 32);
+}
+
 int lgs_basefold_commit(lgs_ctx* ctx, const uint64_t* poly, uint32_t num_vars, uint32_t flags,
                         uint32_t log_interleave, uint32_t code_rate, uint8_t root[32]) {
   if (!ctx || !poly || !root) return LGS_ERR_ARG;
   return guarded(&ctx->c, [&] {
-    if (num_vars > 30 || log_interleave > 6 || log_interleave > num_vars || code_rate < 1 || code_rate > 4)
-      throw lgs::Error(LGS_ERR_ARG, "basefold commit: num_vars <= 30, log_interleave <= min(6, num_vars), "
-                                    "1 <= code_rate <= 4");
-    lgs::Context& c = ctx->c;
-    (void)hipSetDevice(c.device);
-    const bool ext = (flags & LGS_BASEFOLD_EXT) != 0;
-    const uint64_t words = (uint64_t(1) << num_vars) * (ext ? 2 : 1);
+    lgs::DevBasefold st;
     try {
-      lgs::DevBuf<uint64_t> owned;
-      const uint64_t* p = poly;
-      if (!(flags & LGS_INPUT_DEVICE)) {
-        owned = lgs::DevBuf<uint64_t>(&c.pool, words);
-        lgs::hip_check(hipMemcpyAsync(owned.ptr, poly, words * 8, hipMemcpyHostToDevice, c.stream), "upload poly");
-        p = owned.ptr;
-      }
-      lgs::DevBasefold st;
-      lgs::enqueue_basefold_commit(c, st, p, ext, num_vars, log_interleave, code_rate);
-      uint8_t* h = c.staging(32);
-      lgs::hip_check(hipMemcpyAsync(h, st.tree.nodes.ptr + (st.tree.node_count() - 1) * 8, 32,
-                                    hipMemcpyDeviceToHost, c.stream),
-                     "read basefold root");
-      lgs::hip_check(hipStreamSynchronize(c.stream), "basefold sync");
-      std::memcpy(root, h, 32);
+      basefold_commit_into(ctx->c, poly, num_vars, flags, log_interleave, code_rate, st, root);
     } catch (...) {
-      c.drain();
+      ctx->c.drain();
       throw;
     }
   });
+}
+
+int lgs_basefold_commit_state(lgs_ctx* ctx, const uint64_t* poly, uint32_t num_vars, uint32_t flags,
+                              uint32_t log_interleave, uint32_t code_rate, uint8_t root[32], lgs_basefold** out) {
+  if (!ctx || !poly || !root || !out) return LGS_ERR_ARG;
+  *out = nullptr;
+  return guarded(&ctx->c, [&] {
+    auto* b = new lgs_basefold();
+    b->ctx = &ctx->c;
+    try {
+      basefold_commit_into(ctx->c, poly, num_vars, flags, log_interleave, code_rate, b->st, root);
+    } catch (...) {
+      ctx->c.drain();
+      delete b;
+      throw;
+    }
+    ctx_retain(&ctx->c);
+    *out = b;
+  });
+}
+
+int lgs_basefold_prove(lgs_ctx* ctx, const lgs_basefold* st, const uint64_t* point, uint64_t point_len,
+                       const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,
+                       uint8_t* proof, uint64_t* proof_len, uint64_t used[2]) {
+  if (!ctx || !st || !point || !proof_len || (n_fields && !fields) || (n_ints && !ints)) return LGS_ERR_ARG;
+  return guarded(&ctx->c, [&] {
+    if (st->ctx->device != ctx->c.device)
+      throw lgs::Error(LGS_ERR_ARG, "basefold prove: the commitment lives on another device");
+    (void)hipSetDevice(ctx->c.device);
+    if (st->ctx != &ctx->c) lgs::hip_check(hipStreamSynchronize(st->ctx->stream), "basefold commit sync");
+    const uint64_t cap = proof ? *proof_len : 0;
+    uint64_t u[2] = {0, 0}, size = 0;
+    try {
+      size = (uint64_t)lgs::basefold_prove_impl(ctx->c, st->st, point, point_len, fields, n_fields, ints, n_ints,
+                                                lgs::ProofOut{nullptr, proof, cap}, u);
+    } catch (...) {
+      ctx->c.drain();
+      throw;
+    }
+    if (used) {
+      used[0] = u[0];
+      used[1] = u[1];
+    }
+    *proof_len = size;
+    if (cap < size) throw lgs::Error(LGS_ERR_BUFFER, "basefold prove: proof needs " + std::to_string(size) + " bytes");
+  });
+}
+
+void lgs_basefold_release(lgs_basefold* st) {
+  if (!st) return;
+  lgs::Context* c = st->ctx;
+  if (c) (void)hipStreamSynchronize(c->stream);
+  delete st;
+  ctx_unref(c);
 }
 
 void lgs_key_release(lgs_key* key) {

@@ -14,3 +14,7 @@ struct lgs_commitment {
 struct lgs_server {
   lgs::Server* s;
 };
+struct lgs_basefold {
+  lgs::Context* ctx;
+  lgs::DevBasefold st;  // codewords, tree and the (owned or borrowed) polynomial
+};

@@ -920,6 +920,118 @@ hipError_t launch_bf_round3(const BfRound& a, uint64_t half, const ef2* r, hipSt
 }
 
 
+// ---------------- Basefold, one polynomial (basefold.rs:384-516) ----------------
+// One reduction round: partial sums of l0 = sum_i m[2i] eq[i] and l1 = sum_i m[2i+1] eq[i]
+// (basefold.rs:449-456; both are sent), the low-variable fold m'[i] = m[2i] + r (m[2i+1] - m[2i]),
+// the next round's eq table (pair sums), and (row y = 1) the codeword fold of the same round;
+// row y = 2 reduces the previous round's partials.
+__global__ __launch_bounds__(256) void bf1_round_kernel(Bf1Round a, uint64_t half, const ef2* __restrict__ r_ptr) {
+  __shared__ ef2 sh[256 * 4];
+  if (blockIdx.y == 2) {
+    if (blockIdx.x == 0 && a.prev.nblk) red_job_body(a.prev, sh);
+    return;
+  }
+  const ef2 r = *r_ptr;
+  if (blockIdx.y == 1) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.cw_half;
+         i += (uint64_t)gridDim.x * blockDim.x)
+      eq_fold_codeword_at(a.cw_in, a.cw_half, r, a.cw_tw, a.cw_out, i);
+    return;
+  }
+  ef2 acc[2] = {ef_zero(), ef_zero()};
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += (uint64_t)gridDim.x * blockDim.x) {
+    const ef2 e = a.m_in[2 * i], o = a.m_in[2 * i + 1], q = a.eq[i];
+    acc[0] = ef_add(acc[0], ef_mul(e, q));
+    acc[1] = ef_add(acc[1], ef_mul(o, q));
+    a.m_out[i] = ef_add(e, ef_mul(r, ef_sub(o, e)));
+    if (half > 1 && i < half / 2) a.eq_next[i] = ef_add(a.eq[2 * i], a.eq[2 * i + 1]);
+  }
+  block_sum<2>(acc, sh);
+  if (threadIdx.x == 0) {
+    a.part[2 * (uint64_t)blockIdx.x] = acc[0];
+    a.part[2 * (uint64_t)blockIdx.x + 1] = acc[1];
+  }
+}
+
+uint32_t bf1_round_blocks(uint64_t half, uint64_t cw_half) { return bf_round3_blocks(half, cw_half); }
+
+hipError_t launch_bf1_round(const Bf1Round& a, uint64_t half, const ef2* r, hipStream_t st) {
+  hipLaunchKernelGGL(bf1_round_kernel, dim3(bf1_round_blocks(half, a.cw_half), 3), dim3(256), 0, st, a, half, r);
+  return hipGetLastError();
+}
+
+// Rounds t0..R-1 in one workgroup of 512 threads: level t0 (at most 2^kBfTailLog entries), m
+// and eq in LDS. Per round thread i folds codeword pair (i, i + len/2) (kept in LDS and written
+// to the level's place in global memory), and for i < half adds m[2i] eq[i] and m[2i+1] eq[i]
+// to the round's sums and folds m and eq; a barrier separates reads from writes. The twiddles
+// of every round come from tws[j] = tw[j 2^t0] (round t0 + k takes tws[(h - i) 2^k]).
+__global__ __launch_bounds__(512) void bf1_tail_kernel(Bf1Tail a) {
+  __shared__ ef2 cw[1 << kBfTailLog];
+  __shared__ ef2 m[1 << (kBfTailLog - 1)];
+  __shared__ ef2 eq[1 << (kBfTailLog - 2)];
+  __shared__ ef2 wp[kBfTailLog][8][2];
+  __shared__ ef2 rch[kBfTailLog];
+  __shared__ uint64_t tws[1 << (kBfTailLog - 1)];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  if (a.prev.nblk) red_job_body(a.prev, &wp[0][0][0]);
+  __syncthreads();
+  const uint32_t len0 = 1u << (a.logD - a.t0), mlen0 = 1u << (a.R - a.t0);
+  for (uint32_t i = tid; i < len0; i += blockDim.x) cw[i] = a.cw_in[i];
+  for (uint32_t i = tid; i < len0 / 2; i += blockDim.x) tws[i] = a.tw[(uint64_t)i << a.t0];
+  if (tid < a.R - a.t0) rch[tid] = a.chal[a.t0 + tid];
+  for (uint32_t i = tid; i < mlen0; i += blockDim.x) m[i] = a.m_in[i];
+  for (uint32_t i = tid; i < mlen0 / 2; i += blockDim.x) eq[i] = a.eq[i];
+  __syncthreads();
+  ef2* lvl = a.lvl_out;
+  for (uint32_t t = a.t0; t < a.R; t++) {
+    const uint32_t k = t - a.t0, len = len0 >> k, h = len / 2, half = (mlen0 >> k) / 2;
+    const ef2 r = rch[k];
+    ef2 cwv = ef_zero(), mv = ef_zero(), ev = ef_zero(), acc0 = ef_zero(), acc1 = ef_zero();
+    if (tid < h) {
+      const ef2 x = cw[tid], nx = cw[tid + h];
+      const uint64_t ginv = tid == 0 ? 1 : gl_neg(tws[(h - tid) << k]);
+      const ef2 pe = ef_mul_base(ef_add(x, nx), GL_INV2);
+      const ef2 po = ef_mul_base(ef_sub(x, nx), gl_mul(ginv, GL_INV2));
+      cwv = ef_add(ef_mul(ef_one_minus(r), pe), ef_mul(r, po));
+    }
+    if (tid < half) {
+      const ef2 e = m[2 * tid], o = m[2 * tid + 1], q = eq[tid];
+      acc0 = ef_mul(e, q);
+      acc1 = ef_mul(o, q);
+      mv = ef_add(e, ef_mul(r, ef_sub(o, e)));
+      if (tid < half / 2) ev = ef_add(eq[2 * tid], eq[2 * tid + 1]);
+    }
+    __syncthreads();  // every read of this round's tables is done
+    if (tid < h) {
+      cw[tid] = cwv;
+      lvl[tid] = cwv;
+    }
+    if (tid < half) m[tid] = mv;
+    if (tid < half / 2) eq[tid] = ev;
+    acc0 = wave_sum(acc0);
+    acc1 = wave_sum(acc1);
+    if (lane == 0) {
+      wp[k][wave][0] = acc0;
+      wp[k][wave][1] = acc1;
+    }
+    lvl += h;
+    __syncthreads();
+  }
+  const uint32_t rounds = a.R - a.t0, nw = blockDim.x >> 6;
+  if (tid < 2 * rounds) {
+    const uint32_t k = tid / 2, q = tid % 2;
+    ef2 t = ef_zero();
+    for (uint32_t w = 0; w < nw; w++) t = ef_add(t, wp[k][w][q]);
+    a.lpair[2 * (uint64_t)(a.t0 + k) + q] = t;
+  }
+}
+
+hipError_t launch_bf1_tail(const Bf1Tail& a, hipStream_t st) {
+  if (a.t0 < a.R) hipLaunchKernelGGL(bf1_tail_kernel, dim3(1), dim3(512), 0, st, a);
+  return hipGetLastError();
+}
+
+
 // eq_fold_codeword (basefold.rs:201-218): out[i] = (1 - r)(x + nx)/2 + r (x - nx) g^-i / 2 with
 // x = v[i], nx = v[i + h]; g^-i = -g^(h - i) for 0 < i < h, tw[e] = g^e (e < h).
 

@@ -109,6 +109,38 @@ struct BfTail {
 };
 hipError_t launch_bf_tail(const BfTail& a, hipStream_t st);
 
+// One reduction round of the single-polynomial Basefold opening (bf1_round_kernel,
+// basefold.rs:445-467): partial sums of l0 and l1 (two per block, block-major), the fold of m and
+// of the eq table, the codeword fold (cw_out), and the previous round's reduction (prev).
+struct Bf1Round {
+  const ef2* m_in;
+  const ef2* eq;
+  ef2* m_out;
+  ef2* eq_next;
+  ef2* part;
+  RedJob prev;
+  const ef2* cw_in;
+  uint64_t cw_half;
+  const uint64_t* cw_tw;
+  ef2* cw_out;
+};
+uint32_t bf1_round_blocks(uint64_t half, uint64_t cw_half);
+hipError_t launch_bf1_round(const Bf1Round& a, uint64_t half, const ef2* r, hipStream_t st);
+// Rounds t0..R-1 of the single-polynomial opening in one workgroup (bf1_tail_kernel): level t0
+// has D >> t0 <= 2^kBfTailLog entries; lpair[2 t], lpair[2 t + 1] receive round t's l0, l1.
+struct Bf1Tail {
+  const ef2* m_in;  // round t0's m (2^(R - t0) entries) and eq (half of that)
+  const ef2* eq;
+  RedJob prev;        // the last grid round's sums (nblk 0: none)
+  const ef2* cw_in;   // level t0
+  ef2* lvl_out;       // level t0 + 1; each later level follows the previous one
+  ef2* lpair;
+  const ef2* chal;    // round t's challenge at chal[t]
+  const uint64_t* tw; // tw[k] = g^k (k < D / 2), g of order D
+  uint32_t R, t0, logD, pad;
+};
+hipError_t launch_bf1_tail(const Bf1Tail& a, hipStream_t st);
+
 // Up to three Basefold channels combined in one pass (basefold.rs:732-741).
 struct BfChannels {
   int count;

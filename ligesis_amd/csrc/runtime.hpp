@@ -364,6 +364,12 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
 // point's scratch array on that device (once per device; throws on failure)
 void scratch_init(int device);
 
+// basefold.cpp: Basefold::prove (basefold.rs:384-516) of a resident commitment; returns the
+// proof's size in bytes
+int64_t basefold_prove_impl(Context& ctx, const DevBasefold& bf, const uint64_t* point, uint64_t point_len,
+                            const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,
+                            ProofOut out, uint64_t used[2]);
+
 // server.cpp
 struct Server;
 uint64_t request_footprint(uint32_t log_m, uint32_t log_n, bool host_input);
