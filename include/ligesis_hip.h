@@ -200,6 +200,18 @@ int lgs_prove(lgs_ctx* ctx, const lgs_key* key, const lgs_commitment* c, const u
               uint64_t z_len, const uint64_t* fields, uint64_t n_fields, const uint64_t* ints,
               uint64_t n_ints, uint8_t* proof, uint64_t* proof_len, uint64_t used[2]);
 
+/* Several eval-proofs on one randomness oracle (PolyCommitmentScheme::batch_prove, noop.rs:36-40,
+ * over LigeSIS): proof i is lgs_prove of commitment cs[i] (made with keys[i]) at zs[i]
+ * (z_lens[i] extension elements), reading the two streams from where proof i - 1's draws ended,
+ * exactly as `count` lgs_prove calls on one RandomOracle continuing its cursors. The proofs'
+ * device work overlaps on worker contexts owned by ctx (each with its own streams). proofs[i]
+ * receives proof i when proof_lens[i] (in: capacity) suffices; every proof_lens[i] is set to its
+ * size (LGS_ERR_BUFFER if any did not fit). used receives the total draws. */
+int lgs_prove_batch(lgs_ctx* ctx, uint32_t count, const lgs_key* const* keys, const lgs_commitment* const* cs,
+                    const uint64_t* const* zs, const uint64_t* z_lens, const uint64_t* fields, uint64_t n_fields,
+                    const uint64_t* ints, uint64_t n_ints, uint8_t* const* proofs, uint64_t* proof_lens,
+                    uint64_t used[2]);
+
 /* ---- Serving loop (no counterpart in the single-threaded reference; it drives the calls
  * above). Requests are commit-then-open jobs (lgs_commit followed by lgs_prove on the resident
  * commitment) of any shape. One commit worker admits queued requests in arrival order while

@@ -73,6 +73,7 @@ def lib():
         L.lgs_basefold_commit_state.argtypes = [vp, vp, u32, u32, u32, u32, vp, pp]
         L.lgs_basefold_prove.argtypes = [vp, vp, vp, u64, vp, u64, vp, u64, vp, ctypes.POINTER(ctypes.c_uint64), vp]
         L.lgs_basefold_release.argtypes = [vp]
+        L.lgs_prove_batch.argtypes = [vp, u32, vp, vp, vp, vp, vp, u64, vp, u64, vp, vp, vp]
         L.lgs_commit.argtypes = [vp, vp, vp, u64, u32, vp, pp]
         L.lgs_commit_batch.argtypes = [vp, u32, vp, vp, vp, u32, vp, vp]
         L.lgs_commitment_release.argtypes = [vp]
@@ -372,6 +373,61 @@ class Ligesis:
         ctx.check(rc)
         oracle.fields_idx, oracle.ints_idx = int(used[0]), int(used[1])
         return out[: cap.value].tobytes()
+
+    @staticmethod
+    def prove_batch(keys, datas, points, oracle: RandomOracle) -> list:
+        """Consecutive eval-proofs on one RandomOracle (``lgs_prove_batch``): proof i proves
+        ``datas[i]`` (committed with ``keys[i]``) at ``points[i]``, reading the oracle from where
+        proof i - 1 stopped, exactly as sequential ``prove`` calls continuing one oracle's cursors
+        would; their device work overlaps. Starts at the oracle's current cursors and advances
+        them past every proof."""
+        if not (len(keys) == len(datas) == len(points)):
+            raise LigesisError(LGS_ERR_ARG, "prove_batch: keys, datas and points differ in length")
+        return _prove_batch(keys, datas, points, oracle)
+
+
+def prove_draws(log_m: int, log_n: int) -> tuple[int, int]:
+    """(field, integer) draws one eval-proof at 2^log_m x 2^log_n takes from the RandomOracle: the
+    schedule of pcs/src/ligesis.rs:390-530 and basefold.rs:732, 761, 788."""
+    log_w, log_s = 6 + log_m, 7
+    lb = log_w + log_s
+    r = max(11 + log_m, 13 + log_m, log_n) - LOG_INTERLEAVE
+    return (lb + 5 + lb + log_s + 2 * log_w + (log_n + 1) + log_n + 1 + lb + 1 + log_n + 3 + r,
+            4 * NUM_COLS_OPENED + 100)
+
+
+def _prove_batch(keys, datas, points, oracle: RandomOracle) -> list:
+    """``lgs_prove_batch``: consecutive proofs on one oracle, read from its current cursors, which
+    it then advances past every proof's draws."""
+    count = len(keys)
+    if count == 0:
+        return []
+    ctx = keys[0].ctx
+    zs = [_u64(z).reshape(-1, 2) for z in points]
+    fi, ii = oracle.drawn()
+    fields, ints = oracle.fields[fi:], oracle.ints[ii:]
+    # proofs of one key have one size: start from the size the key's last proof needed
+    bufs = [np.empty(getattr(k, "_proof_buf", np.empty(4 << 20, np.uint8)).size, np.uint8) for k in keys]
+    kp = (ctypes.c_void_p * count)(*[k.h.value for k in keys])
+    cp = (ctypes.c_void_p * count)(*[d.h.value for d in datas])
+    zp = (ctypes.c_void_p * count)(*[z.ctypes.data for z in zs])
+    zl = np.array([z.shape[0] for z in zs], np.uint64)
+    lens = np.array([b.size for b in bufs], np.uint64)
+    used = np.zeros(2, np.uint64)
+    fields_c, ints_c = np.ascontiguousarray(fields), np.ascontiguousarray(ints)
+
+    def call():
+        pp = (ctypes.c_void_p * count)(*[b.ctypes.data for b in bufs])
+        return lib().lgs_prove_batch(ctx.h, count, kp, cp, zp, _ptr(zl), _ptr(fields_c), fields_c.shape[0],
+                                     _ptr(ints_c), ints_c.size, pp, _ptr(lens), _ptr(used))
+
+    rc = call()
+    if rc == LGS_ERR_BUFFER:
+        bufs = [np.empty(int(n), np.uint8) for n in lens]
+        rc = call()
+    ctx.check(rc)
+    oracle.fields_idx, oracle.ints_idx = fi + int(used[0]), ii + int(used[1])
+    return [b[: int(n)].tobytes() for b, n in zip(bufs, lens)]
 
 
 class Basefold:

@@ -4,6 +4,8 @@
 #include <cstring>
 #include <exception>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "handles.hpp"
 
@@ -14,19 +16,7 @@ void ctx_retain(lgs::Context* c) { c->refs.fetch_add(1); }
 // Drops one reference; the last one tears the context down.
 void ctx_unref(lgs::Context* c) {
   if (!c || c->refs.fetch_sub(1) != 1) return;
-  (void)hipSetDevice(c->device);
-  c->drain();
-  c->twiddles.clear();
-  c->pass_twiddles.clear();
-  c->pool.trim();
-  for (auto& e : c->ev)
-    if (e) (void)hipEventDestroy(e);
-  for (auto& e : c->pipe_ev)
-    if (e) (void)hipEventDestroy(e);
-  for (auto& e : c->tev)
-    if (e) (void)hipEventDestroy(e);
-  c->free_host();
-  c->destroy_streams();
+  lgs::teardown_context(*c);
   delete (lgs_ctx*)c->owner;
 }
 
@@ -460,6 +450,80 @@ int lgs_prove(lgs_ctx* ctx, const lgs_key* key, const lgs_commitment* c, const u
     }
     *proof_len = size;
     if (cap < size) throw lgs::Error(LGS_ERR_BUFFER, "prove: proof needs " + std::to_string(size) + " bytes");
+  });
+}
+
+int lgs_prove_batch(lgs_ctx* ctx, uint32_t count, const lgs_key* const* keys, const lgs_commitment* const* cs,
+                    const uint64_t* const* zs, const uint64_t* z_lens, const uint64_t* fields, uint64_t n_fields,
+                    const uint64_t* ints, uint64_t n_ints, uint8_t* const* proofs, uint64_t* proof_lens,
+                    uint64_t used[2]) {
+  if (!ctx || (count && (!keys || !cs || !zs || !z_lens || !proof_lens)) || (n_fields && !fields) ||
+      (n_ints && !ints))
+    return LGS_ERR_ARG;
+  return guarded(&ctx->c, [&] {
+    lgs::Context& c = ctx->c;
+    (void)hipSetDevice(c.device);
+    // stream positions: proof i starts where proof i - 1's draws end
+    std::vector<uint64_t> f_off(count + 1, 0), i_off(count + 1, 0);
+    for (uint32_t i = 0; i < count; i++) {
+      if (!keys[i] || !cs[i] || !zs[i]) throw lgs::Error(LGS_ERR_ARG, "prove_batch: null key, commitment or point");
+      uint64_t nf = 0, ni = 0;
+      lgs::prove_draws(keys[i]->k.log_m, keys[i]->k.log_n, nf, ni);
+      f_off[i + 1] = f_off[i] + nf;
+      i_off[i + 1] = i_off[i] + ni;
+    }
+    if (f_off[count] > n_fields) throw lgs::Error(LGS_ERR_ORACLE, "prove_batch: randomness oracle field stream exhausted");
+    if (i_off[count] > n_ints) throw lgs::Error(LGS_ERR_ORACLE, "prove_batch: randomness oracle integer stream exhausted");
+    // worker 0 is this context; the others are its worker contexts, each with its own streams
+    const uint32_t nw = count < 3 ? (count ? count : 1) : 3;
+    std::vector<lgs::Context*> wk{&c};
+    {
+      std::lock_guard<std::mutex> g(c.workers_mu);
+      while (c.workers.size() + 1 < nw) c.workers.push_back(lgs::make_context(c.device));
+      for (uint32_t w = 1; w < nw; w++) wk.push_back(c.workers[w - 1].get());
+    }
+    std::vector<uint64_t> sizes(count, 0);
+    std::vector<int> codes(nw, LGS_OK);
+    std::vector<std::string> msgs(nw);
+    auto run = [&](uint32_t w) {
+      (void)hipSetDevice(c.device);
+      for (uint32_t i = w; i < count && codes[w] == LGS_OK; i += nw) {
+        try {
+          const uint64_t cap = proofs && proofs[i] ? proof_lens[i] : 0;
+          uint64_t u[2];
+          sizes[i] = (uint64_t)lgs::prove_impl(*wk[w], keys[i]->k, cs[i]->c, zs[i], z_lens[i], fields + 2 * f_off[i],
+                                               n_fields - f_off[i], ints + i_off[i], n_ints - i_off[i],
+                                               lgs::ProofOut{nullptr, cap ? proofs[i] : nullptr, cap}, u);
+        } catch (const lgs::Error& e) {
+          codes[w] = e.code;
+          msgs[w] = e.what();
+        } catch (const std::bad_alloc&) {
+          codes[w] = LGS_ERR_NOMEM;
+          msgs[w] = "host allocation failed";
+        } catch (const std::exception& e) {
+          codes[w] = LGS_ERR_ARG;
+          msgs[w] = e.what();
+        }
+      }
+      if (codes[w] != LGS_OK) wk[w]->drain();
+    };
+    std::vector<std::thread> th;
+    for (uint32_t w = 1; w < nw; w++) th.emplace_back(run, w);
+    run(0);
+    for (auto& t : th) t.join();
+    for (uint32_t w = 0; w < nw; w++)
+      if (codes[w] != LGS_OK) throw lgs::Error(codes[w], msgs[w]);
+    bool fit = true;
+    for (uint32_t i = 0; i < count; i++) {
+      const uint64_t cap = proofs && proofs[i] ? proof_lens[i] : 0;
+      fit = fit && cap >= sizes[i];
+      proof_lens[i] = sizes[i];
+    }
+    if (used) {
+      used[0] = f_off[count];
+      used[1] = i_off[count];
+    }
+    if (!fit) throw lgs::Error(LGS_ERR_BUFFER, "prove_batch: a proof buffer is too small (sizes reported)");
   });
 }
 

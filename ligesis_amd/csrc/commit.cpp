@@ -110,6 +110,33 @@ void PinnedArena::release() {
   used_ = 0;
 }
 
+std::unique_ptr<Context> make_context(int device) {
+  std::unique_ptr<Context> c(new Context());
+  c->device = device;
+  hip_check(hipSetDevice(device), "set device");
+  hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
+  for (auto& e : c->ev) hip_check(hipEventCreate(&e), "event");
+  return c;
+}
+
+void teardown_context(Context& c) {
+  (void)hipSetDevice(c.device);
+  for (auto& w : c.workers) teardown_context(*w);
+  c.workers.clear();
+  c.drain();
+  c.twiddles.clear();
+  c.pass_twiddles.clear();
+  c.pool.trim();
+  for (auto& e : c.ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c.pipe_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : c.tev)
+    if (e) (void)hipEventDestroy(e);
+  c.free_host();
+  c.destroy_streams();
+}
+
 void Context::free_host() {
   if (pinned) (void)hipHostFree(pinned);
   pinned = nullptr;

@@ -66,6 +66,15 @@ struct ProvePlan {
   }
 };
 
+void prove_draws(uint32_t log_m, uint32_t log_n, uint64_t& n_fields, uint64_t& n_ints) {
+  // the schedule below: alpha1, alpha2, the binary check, r2, hash, recon, table, RS link,
+  // gamma_bi, B_I batch, gamma_a, a-bar batch, the Basefold coefficients and rounds
+  const uint64_t log_w = 6 + log_m, log_s = 7, Lb = log_w + log_s;
+  n_fields = Lb + 5 + Lb + log_s + log_w + log_w + (log_n + 1) + log_n + 1 + Lb + 1 + log_n + 3 +
+             batch_rounds(log_m, log_n);
+  n_ints = 4 * 128 + 100;  // column samples (ligesis.rs:329-345), then the Basefold queries
+}
+
 int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uint64_t* z, uint64_t z_len,
                    const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,
                    ProofOut out, uint64_t used[2]) {
@@ -109,6 +118,11 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   std::sort(sorted.begin(), sorted.end());
   used[0] = fc;
   used[1] = 4 * s + 100;
+  {
+    uint64_t nf = 0, ni = 0;
+    prove_draws(log_m, log_n, nf, ni);
+    if (nf != used[0] || ni != used[1]) throw Error(1, "prove: transcript length does not match prove_draws");
+  }
 
   EfV zz(z_len);
   for (size_t i = 0; i < z_len; i++) zz[i] = ef2{z[2 * i], z[2 * i + 1]};

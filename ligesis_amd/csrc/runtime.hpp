@@ -251,6 +251,9 @@ struct Context {
   hipEvent_t timing_event(int i);
   // launch schedule of a commit of shape 2^log_m x 2^log_n
   Tuning resolve(uint32_t log_m, uint32_t log_n) const;
+  // contexts of the batched prove's worker threads (lgs_prove_batch), made on first use
+  std::vector<std::unique_ptr<Context>> workers;
+  std::mutex workers_mu;
 };
 
 struct Key {
@@ -295,6 +298,10 @@ struct GroupHold {
   std::vector<std::vector<MerkleJob>> host_merkle;
 };
 
+// commit.cpp: a context with its stream and events on `device`; the release of everything a
+// context holds (its workers' contexts, streams, events, tables, pool and pinned staging)
+std::unique_ptr<Context> make_context(int device);
+void teardown_context(Context& c);
 // commit.cpp; `force` replaces the resolved schedule (autotune candidates)
 void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* force = nullptr);
 // The hash kernel of a commit: variant (1 MFMA, 2 VALU), MFMA kernel and K-split.
@@ -355,6 +362,8 @@ struct ProofOut {
   uint8_t* dst = nullptr;
   uint64_t cap = 0;
 };
+// The numbers of field and integer draws one proof at (log_m, log_n) takes from the oracle.
+void prove_draws(uint32_t log_m, uint32_t log_n, uint64_t& n_fields, uint64_t& n_ints);
 // returns the proof's size in bytes
 int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uint64_t* z, uint64_t z_len,
                    const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,

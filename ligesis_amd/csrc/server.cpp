@@ -281,13 +281,7 @@ Server* server_start(int device, uint64_t budget, uint32_t workers, uint32_t max
   s->max_batch = max_batch ? max_batch : 16;
   if (workers == 0) workers = 2;
   hip_check(hipSetDevice(device), "set device");
-  for (uint32_t i = 0; i <= workers; i++) {
-    std::unique_ptr<Context> c(new Context());
-    c->device = device;
-    hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
-    for (auto& e : c->ev) hip_check(hipEventCreate(&e), "event");
-    s->ctxs.push_back(std::move(c));
-  }
+  for (uint32_t i = 0; i <= workers; i++) s->ctxs.push_back(make_context(device));
   hip_check(hipStreamCreateWithFlags(&s->upload_stream, hipStreamNonBlocking), "stream");
   s->threads.emplace_back(&Server::commit_loop, s.get());
   for (uint32_t i = 1; i <= workers; i++) s->threads.emplace_back(&Server::prove_loop, s.get(), s->ctxs[i].get());
@@ -369,19 +363,7 @@ void server_stop(Server* s) {
     (void)hipStreamSynchronize(s->upload_stream);
     (void)hipStreamDestroy(s->upload_stream);
   }
-  for (auto& c : s->ctxs) {
-    c->drain();
-    c->twiddles.clear();
-    c->pool.trim();
-    for (auto& e : c->ev)
-      if (e) (void)hipEventDestroy(e);
-    for (auto& e : c->pipe_ev)
-      if (e) (void)hipEventDestroy(e);
-    for (auto& e : c->tev)
-      if (e) (void)hipEventDestroy(e);
-    c->free_host();
-    c->destroy_streams();
-  }
+  for (auto& c : s->ctxs) teardown_context(*c);
   delete s;
 }
 

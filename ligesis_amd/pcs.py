@@ -13,22 +13,26 @@ LigeSIS commits base-field ones (pcs/src/ligesis.rs:277) and opens them at exten
   the verifier combines the two claimed values.
 
 The trait's ``commit`` takes no key, so the scheme object carries one proving key (one shape,
-pcs/src/ligesis.rs:239). ``batch_prove`` proves the tasks in order, each component after the
-previous one, all drawing from the caller's oracle: every proof reads the oracle's streams from
-its current cursors and advances them by what it consumed, which is the transcript that
-sequential ``Ligesis::prove`` calls on one ``RandomOracle`` (utils/src/oracle.rs:10-64) give.
-
-Verification is the reference's CPU verifier (pcs/src/ligesis.rs:569-756) and is not part of
-this path; ``batch_verify`` says so instead of accepting.
+pcs/src/ligesis.rs:239). ``batch_prove`` opens every component of every task in task order as
+consecutive proofs on the caller's oracle: proof k reads the oracle's streams from where proof
+k - 1 stopped, which is the transcript that sequential ``Ligesis::prove`` calls on one
+``RandomOracle`` (utils/src/oracle.rs:10-64) give; the proofs' device work overlaps
+(``lgs_prove_batch``). ``batch_verify`` replays the same transcript through a verifier the caller
+supplies (the reference's CPU ``Ligesis::verify``, pcs/src/ligesis.rs:569-756): it checks that
+each claimed value is the combination of its component proofs' values and that every component
+proof verifies, and returns False (it never raises) on any mismatch.
 """
 from __future__ import annotations
 
 import abc
 from dataclasses import dataclass
+from typing import Callable
 
 import numpy as np
 
-from . import CommitData, Ligesis, LigesisError, ProvingKey, RandomOracle, LGS_ERR_ARG
+from . import (CommitData, Ligesis, LigesisError, ProvingKey, RandomOracle, LGS_ERR_ARG, prove_draws)
+
+P = 0xFFFFFFFF00000001
 
 
 class PolyCommitmentScheme(abc.ABC):
@@ -68,12 +72,34 @@ class LigesisPcsData:
             p.release()
 
 
-class LigesisPcs(PolyCommitmentScheme):
-    """LigeSIS as a ``PolyCommitmentScheme`` for polynomials of 2^(log_m + log_n) evaluations."""
+# verifier(log_m, log_n, com_a, root, z, proof, fields, ints) -> bool: the reference's
+# Ligesis::verify with the RandomOracle given as its two streams read from the start
+Verifier = Callable[[int, int, bytes, bytes, np.ndarray, bytes, np.ndarray, np.ndarray], bool]
 
-    def __init__(self, pk: ProvingKey):
+
+def proof_value(proof: bytes) -> tuple[int, int]:
+    """The claimed value y = f(z) of a LigeSIS proof: the first field of LigesisProof
+    (ligesis.rs:99-119), an extension element (c0, c1) in the canonical encoding."""
+    if len(proof) < 16:
+        raise ValueError("proof too short")
+    w = np.frombuffer(proof[:16], np.uint64)
+    return int(w[0]), int(w[1])
+
+
+def _combine(y0, y1):
+    """y0 + X y1 in F_p[X]/(X^2 - 7): (a0 + a1 X) + X (b0 + b1 X) = a0 + 7 b1 + (a1 + b0) X."""
+    return ((y0[0] + 7 * y1[1]) % P, (y0[1] + y1[0]) % P)
+
+
+class LigesisPcs(PolyCommitmentScheme):
+    """LigeSIS as a ``PolyCommitmentScheme`` for polynomials of 2^(log_m + log_n) evaluations.
+    ``com_a`` (the verifying key's A-bar root) and ``verifier`` are what ``batch_verify`` needs."""
+
+    def __init__(self, pk: ProvingKey, com_a: bytes | None = None, verifier: Verifier | None = None):
         self.pk = pk
         self.size = 1 << (pk.log_m + pk.log_n)
+        self.com_a = com_a
+        self.verifier = verifier
 
     def _components(self, poly) -> tuple[list[np.ndarray], bool]:
         a = np.asarray(poly)
@@ -96,25 +122,57 @@ class LigesisPcs(PolyCommitmentScheme):
         return LigesisPcsData(parts, ext), LigesisPcsCommitment(tuple(r for r, _ in res))
 
     def batch_prove(self, tasks, oracle: RandomOracle) -> list:
-        """One list of proof bytes per task (one proof per component), proved in task order
-        from the oracle's current cursors, which it leaves after the last proof's draws."""
+        """One list of proof bytes per task (one proof per component), consecutive on the oracle
+        from its current cursors, which it leaves after the last proof's draws."""
         nvars = self.pk.log_m + self.pk.log_n
         for data, point in tasks:
             if not isinstance(data, LigesisPcsData) or any(not isinstance(p, CommitData) for p in data.parts):
                 raise LigesisError(LGS_ERR_ARG, "pcs batch_prove: prover data must come from this scheme's commit")
             if np.asarray(point).reshape(-1, 2).shape[0] != nvars:
                 raise LigesisError(LGS_ERR_ARG, f"pcs batch_prove: points have {nvars} extension coordinates")
-        out = []
-        for data, point in tasks:
-            proofs = []
-            for cd in data.parts:
-                fi, ii = oracle.drawn()
-                sub = RandomOracle(oracle.fields[fi:], oracle.ints[ii:])
-                proofs.append(Ligesis.prove(self.pk, cd, point, sub))
-                oracle.fields_idx, oracle.ints_idx = fi + sub.fields_idx, ii + sub.ints_idx
-            out.append(proofs)
+        flat = [(cd, point) for data, point in tasks for cd in data.parts]
+        proofs = Ligesis.prove_batch([self.pk] * len(flat), [cd for cd, _ in flat], [z for _, z in flat], oracle)
+        out, k = [], 0
+        for data, _ in tasks:
+            out.append(proofs[k:k + len(data.parts)])
+            k += len(data.parts)
         return out
 
-    def batch_verify(self, claims, proof, oracle) -> bool:
-        raise NotImplementedError("LigeSIS verification is the reference's CPU verifier "
-                                  "(pcs/src/ligesis.rs:569-756); the GPU path proves only")
+    def batch_verify(self, claims, proof, oracle: RandomOracle) -> bool:
+        """``claims[i] = (commitment, point, claimed value)`` against ``proof`` from batch_prove:
+        every component proof must verify on the consecutive transcript (the verifier takes the
+        draws the prover took, ``prove_draws``) and the claimed value must equal the component
+        proofs' values combined (y0 + X y1 for an extension polynomial). Advances the oracle past
+        every proof; returns False on any mismatch or malformed input."""
+        if self.verifier is None or self.com_a is None:
+            return False
+        try:
+            if len(claims) != len(proof):
+                return False
+            nf, ni = prove_draws(self.pk.log_m, self.pk.log_n)
+            nvars = self.pk.log_m + self.pk.log_n
+            fi, ii = oracle.drawn()
+            ok = True
+            for (com, point, value), proofs in zip(claims, proof):
+                roots = tuple(getattr(com, "roots", ()))
+                if len(roots) not in (1, 2) or len(proofs) != len(roots):
+                    return False
+                z = np.ascontiguousarray(np.asarray(point, dtype=np.uint64).reshape(-1, 2))
+                if z.shape[0] != nvars:
+                    return False
+                ys = []
+                for root, pf in zip(roots, proofs):
+                    if fi + nf > oracle.fields.shape[0] or ii + ni > oracle.ints.size:
+                        return False
+                    ok = ok and bool(self.verifier(self.pk.log_m, self.pk.log_n, self.com_a, root, z, pf,
+                                                   oracle.fields[fi:], oracle.ints[ii:]))
+                    ys.append(proof_value(pf))
+                    fi, ii = fi + nf, ii + ni
+                y = ys[0] if len(ys) == 1 else _combine(ys[0], ys[1])
+                v = np.asarray(value, dtype=np.uint64).reshape(-1)
+                want = (int(v[0]), int(v[1]) if v.size > 1 else 0)
+                ok = ok and y == want
+            oracle.fields_idx, oracle.ints_idx = fi, ii
+            return ok
+        except (ValueError, TypeError, IndexError):
+            return False

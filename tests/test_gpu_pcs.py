@@ -1,7 +1,9 @@
 """The PolyCommitmentScheme adapter on the GPU: its commitments are LigeSIS commitments, its
 batched proofs are the proofs sequential eval-proves give on one shared oracle (each accepted by
-the oracle's verifier), and the component openings of an extension-field polynomial combine to
-its evaluation, f(z) = f0(z) + X f1(z)."""
+the oracle's verifier), the component openings of an extension-field polynomial combine to its
+evaluation, f(z) = f0(z) + X f1(z), and batch_verify with the oracle's verifier accepts the
+batch and rejects a wrong claim. Ligesis.prove_batch over keys of different shapes equals
+sequential proves and the oracle's proofs."""
 import numpy as np
 import pytest
 
@@ -32,7 +34,7 @@ def test_batch_prove_matches_sequential_proofs(gpu_ctx, shape):
     print(f"pcs adapter shape={shape}", flush=True)
     inp = synth.Inputs(log_m, log_n, seed=70 + mu)
     pk, vk = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx)
-    pcs = LigesisPcs(pk)
+    pcs = LigesisPcs(pk, com_a=vk.com_a, verifier=oracle.verify)
     f_base = inp.f
     f_ext = np.stack([synth.field_elems(900 + mu, 1 << mu), synth.field_elems(901 + mu, 1 << mu)], axis=1)
     z1, z2 = synth.ext_elems(902 + mu, mu), synth.ext_elems(903 + mu, mu)
@@ -63,5 +65,48 @@ def test_batch_prove_matches_sequential_proofs(gpu_ctx, shape):
     y = [pfmt.decode(p)["y"] for p in proofs[1]]
     want = tuple(int(v) for v in oracle.mle_eval(f_ext, z2))
     assert _ef_add(y[0], _times_x(y[1])) == want
+
+    # batch_verify: the claims are the polynomials' values at their points
+    lifted = np.stack([f_base, np.zeros_like(f_base)], axis=1)
+    y_base = oracle.mle_eval(lifted, z1)
+    claims = [(c1, z1, y_base), (c2, z2, np.array(want, np.uint64))]
+    assert pcs.batch_verify(claims, proofs, RandomOracle(fields, ints))
+    wrong = [(c1, z1, y_base), (c2, z2, np.array([want[0], (want[1] + 1) % P], np.uint64))]
+    assert not pcs.batch_verify(wrong, proofs, RandomOracle(fields, ints))
+    swapped = [claims[1], claims[0]]
+    assert not pcs.batch_verify(swapped, [proofs[1], proofs[0]], RandomOracle(fields, ints))
     d1.release()
     d2.release()
+
+
+def test_prove_batch_mixed_shapes_matches_sequential_and_oracle(gpu_ctx):
+    shapes = [(3, 10), (0, 7), (3, 10), (2, 12), (1, 9)]
+    keys, cds, zs, inps = [], [], [], []
+    made = {}
+    for i, (lm, ln) in enumerate(shapes):
+        inp = synth.Inputs(lm, ln, seed=310 + i)
+        if (lm, ln) not in made:
+            made[(lm, ln)] = (Ligesis.setup(lm, ln, inp.a, ctx=gpu_ctx)[0], inp.a)
+        pk, a = made[(lm, ln)]
+        _, cd = Ligesis.commit(pk, inp.f)
+        keys.append(pk)
+        cds.append(cd)
+        zs.append(inp.z)
+        inps.append((inp, a))
+    fields, ints = synth.ext_elems(330, 4096), synth.oracle_ints(331, 8192)
+    ro = RandomOracle(fields, ints)
+    print("prove_batch of", len(shapes), "proofs", flush=True)
+    proofs = Ligesis.prove_batch(keys, cds, zs, ro)
+    fi = ii = 0
+    for (lm, ln), pk, cd, z, (inp, a), proof in zip(shapes, keys, cds, zs, inps, proofs):
+        ref = RandomOracle(fields[fi:], ints[ii:])
+        assert Ligesis.prove(pk, cd, z, ref) == proof
+        opk = oracle.ProvingKey(lm, ln, a)
+        ocd = oracle.CommitData(opk, inp.f)
+        oproof, oused = oracle.prove(opk, ocd, z, fields[fi:], ints[ii:])
+        assert oproof == proof
+        assert oused == ref.drawn()
+        fi, ii = fi + ref.fields_idx, ii + ref.ints_idx
+    assert ro.drawn() == (fi, ii)
+    for cd in cds:
+        cd.release()
