@@ -245,7 +245,9 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   }
 
   hipStream_t st = ctx.stream;
-  hipStream_t sb = ctx.side_stream();
+  // the second stream; with single_stream_prove the same stream (every step is issued after the
+  // steps it reads, so one stream runs them in a valid order)
+  hipStream_t sb = ctx.single_stream_prove ? st : ctx.side_stream();
   DevicePool* pool = &ctx.pool;
   ctx.upload_reset();  // every earlier call on this context has been synchronized
   // profile mode: device timestamps of the phases (stage_ms[7..16], from the first upload)

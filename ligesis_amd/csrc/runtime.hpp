@@ -225,6 +225,9 @@ struct Context {
   // graph ran the two streams' steps in capture order, slower than issuing them directly)
   bool prove_graph = false;
   bool profile = false;
+  // proofs on this context issue both halves on `stream` (server workers: fewer streams than
+  // the device has hardware queues, so no stream shares a queue with an upload)
+  bool single_stream_prove = false;
   double stage_ms[28] = {0};
   hipEvent_t ev[10] = {};
   uint8_t* pinned = nullptr;  // small pinned staging buffer for roots

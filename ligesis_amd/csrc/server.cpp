@@ -1,12 +1,19 @@
 // Serving loop: commit-then-open requests of mixed sizes, continuously batched.
 //
-// One commit worker admits queued requests in arrival order while their HBM footprint fits the
-// budget, uploads their inputs on its own stream while the previous batch is still committing,
-// commits every admitted request in one batched pass (grouped launches where the shape allows),
-// and hands the resident commitments (f, F', H, Merkle tree) to P prove workers. Each worker
-// proves on its own context and stream, so proofs of one batch run concurrently with each other
-// and with the uploads and commits of the next batches. Memory is returned to the budget when a
-// proof is done; requests that arrive meanwhile join the next batch as soon as they fit.
+// Three kinds of threads share one device:
+//  - the commit thread admits queued requests in arrival order while their HBM footprint fits
+//    the budget (up to max_batch at a time) and enqueues, for each, the upload of f on the upload
+//    stream and its commit on the commit stream. A large f is uploaded in row chunks and each
+//    chunk's rows are transformed and hashed (its K-splits) as soon as the chunk lands, so the
+//    commit ends shortly after the upload. The thread never waits for the device: it records one
+//    event per admitted group and goes back to admitting;
+//  - the finisher thread waits for those events in order, reads the roots (copied into pinned
+//    memory by the commit stream) and hands the resident commitments (f, F', H, Merkle tree) to
+//    the prove workers;
+//  - P prove workers prove on their own contexts, one stream each, so proofs run beside each
+//    other, the uploads and the next commits.
+// Memory returns to the budget when a proof is done; requests that arrive meanwhile are admitted
+// as soon as they fit (continuous batching). Results are identical to lgs_commit + lgs_prove.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -23,14 +30,34 @@ namespace {
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
+
+constexpr uint64_t kChunkMinBytes = uint64_t(64) << 20;   // f of at least this size is uploaded in chunks
+constexpr uint64_t kChunkPartsMax = uint64_t(256) << 20;  // cap on a chunked commit's hash partials
+constexpr uint32_t kRootSlots = 8192;                     // pinned root slots = requests in flight
 }  // namespace
+
+// Row chunks of a served commit of a host-resident f: each chunk's rows are hashed by their own
+// K-splits, so the count divides m / 2 and the partials (one 32 x 2n word plane per split) stay
+// under kChunkPartsMax.
+static uint32_t serve_chunks(uint32_t log_m, uint32_t log_n, bool host_input) {
+  const uint64_t m = uint64_t(1) << log_m, two_n = uint64_t(2) << log_n;
+  const uint64_t fbytes = (m << log_n) * 8;
+  if (!host_input || fbytes < kChunkMinBytes || two_n % 128 != 0 || m < 4) return 1;
+  uint32_t c = fbytes >= (uint64_t(256) << 20) ? 8 : 4;
+  while (c > 1 && ((uint64_t)c * 32 * two_n * 8 > kChunkPartsMax || m % (2 * c) != 0)) c /= 2;
+  return c;
+}
 
 // Conservative device footprint of one commit + open at (log_m, log_n), in bytes.
 uint64_t request_footprint(uint32_t log_m, uint32_t log_n, bool host_input) {
   const uint64_t m = uint64_t(1) << log_m, n = uint64_t(1) << log_n, W = 64 * m, s = 128;
   const uint64_t two_n = 2 * n;
-  uint64_t ks = m >= 1024 ? m / 512 : 1;
-  uint64_t commit = (host_input ? 8 * m * n : 0) + 16 * m * n + 32 * two_n * 8 + 2 * two_n * 32 + ks * 256 * two_n * 4;
+  // hash partials: at most the larger of the built-in K-split (8 int32 planes a split) and the
+  // chunked commit's splits (one word plane a split)
+  const uint64_t ks = m >= 1024 ? m / 512 : 1;
+  const uint64_t parts = std::max<uint64_t>(ks * 256 * two_n * 4,
+                                            (uint64_t)serve_chunks(log_m, log_n, host_input) * 32 * two_n * 8);
+  uint64_t commit = (host_input ? 8 * m * n : 0) + 16 * m * n + 32 * two_n * 8 + 2 * two_n * 32 + parts;
   uint64_t bi = W * s;
   uint64_t D = uint64_t(2) << batch_rounds(log_m, log_n);
   uint64_t prove = bi * (8 + 4 * 16) + 64 * D * 8 + 2 * D * 32 + 4 * D * 16 + 24 * n * 16 + W * 6 * 16 +
@@ -48,10 +75,23 @@ struct Request {
   uint64_t footprint = 0;
   double t_submit = 0, t_committed = 0, t_done = 0;
   std::unique_ptr<Commitment> cm;
+  uint32_t slot = 0;  // pinned root slot
   uint8_t root[32] = {0};
   std::vector<uint8_t> proof;
-  int status = -1;  // -1 pending, else final status
+  int status = -1;        // -1 pending, else final status
+  int enqueue_code = 0;   // set when enqueueing its commit failed (reported after the group's event)
   std::string err;
+};
+
+// An admitted group whose uploads and commits are enqueued; the buffers a commit needs only
+// while it runs (hash partials) and the upload events stay here until the group's event.
+struct Group {
+  std::vector<std::shared_ptr<Request>> reqs;
+  std::vector<DevBuf<uint64_t>> parts;
+  std::vector<hipEvent_t> events;
+  hipEvent_t done = nullptr;
+  int rc = 0;
+  std::string msg;
 };
 
 struct Server {
@@ -59,182 +99,228 @@ struct Server {
   uint64_t budget = 0, reserved = 0;
   uint32_t max_batch = 16;
   std::mutex mu;
-  std::condition_variable cv_commit, cv_prove, cv_done;
+  std::condition_variable cv_commit, cv_finish, cv_prove, cv_done;
   std::deque<std::shared_ptr<Request>> pending, committed;
+  std::deque<std::unique_ptr<Group>> finishing;
   std::map<uint64_t, std::shared_ptr<Request>> all;
   uint64_t next_ticket = 1;
+  uint32_t in_flight = 0;  // admitted requests not yet handed to the prove workers
   bool stopping = false;
-  bool commit_running = true;
+  bool commit_running = true, finish_running = true;
   std::vector<std::unique_ptr<Context>> ctxs;  // [0] commit, [1..] prove
   std::vector<std::thread> threads;
   uint64_t batches = 0, batched_requests = 0;
 
   hipStream_t upload_stream = nullptr;  // host-to-device copies of request inputs
+  uint8_t* roots = nullptr;             // kRootSlots x 32 bytes of pinned memory
+  uint32_t next_slot = 0;
+  std::vector<hipEvent_t> free_events;  // events whose last use has completed (guarded by mu)
 
-  struct Batch;
-  std::unique_ptr<Batch> start_batch(std::vector<std::shared_ptr<Request>>& batch);
-  void commit_batch(Batch& b);
-  void finish_batch(Batch& b);
+  hipEvent_t event();
+  void enqueue_request(Group& g, Request& r);
   void commit_loop();
+  void finish_loop();
   void prove_loop(Context* ctx);
 };
 
-// A batch between admission and hand-over: its inputs go to the device on the upload stream,
-// its commits run on the commit context's stream after them.
-struct Server::Batch {
-  std::vector<std::shared_ptr<Request>> ok;
-  GroupHold hold;
-  hipEvent_t uploaded = nullptr, done = nullptr;
-  int rc = 0;
-  std::string msg;
-  ~Batch() {
-    if (uploaded) (void)hipEventDestroy(uploaded);
-    if (done) (void)hipEventDestroy(done);
-  }
-};
-
-// Uploads of one admitted batch: a commitment per request, f copied on the upload stream.
-std::unique_ptr<Server::Batch> Server::start_batch(std::vector<std::shared_ptr<Request>>& batch) {
-  Context& ctx = *ctxs[0];
-  std::unique_ptr<Batch> b(new Batch());
-  for (auto& r : batch) {
-    try {
-      r->cm.reset(new Commitment());
-      Commitment& c = *r->cm;
-      c.ctx = &ctx;
-      c.key = r->key;
-      if (r->flags & 1u) {
-        c.f = r->f;
-      } else {
-        c.f_owned = DevBuf<uint64_t>(&ctx.pool, r->len);
-        hip_check(hipMemcpyAsync(c.f_owned.ptr, r->f, r->len * 8, hipMemcpyHostToDevice, upload_stream),
-                  "upload f");
-        c.f = c.f_owned.ptr;
-      }
-      b->ok.push_back(r);
-    } catch (const std::exception& e) {
-      r->cm.reset();
-      std::lock_guard<std::mutex> lk(mu);
-      r->status = 4;
-      r->err = e.what();
-      reserved -= r->footprint;
-      cv_done.notify_all();
+hipEvent_t Server::event() {
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!free_events.empty()) {
+      hipEvent_t e = free_events.back();
+      free_events.pop_back();
+      return e;
     }
   }
-  hip_check(hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming), "batch event");
-  hip_check(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "batch event");
-  hip_check(hipEventRecord(b->uploaded, upload_stream), "uploaded");
-  return b;
+  hipEvent_t e = nullptr;
+  hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "server event");
+  return e;
 }
 
-// Commits of a batch whose uploads are enqueued: grouped launches where the shape allows, then
-// the roots to the host.
-void Server::commit_batch(Batch& b) {
+// Upload (if f is in host memory) and commit of one request, enqueued; nothing waits.
+void Server::enqueue_request(Group& g, Request& r) {
   Context& ctx = *ctxs[0];
-  try {
-    hip_check(hipStreamWaitEvent(ctx.stream, b.uploaded, 0), "wait uploads");
-    std::vector<std::pair<const Key*, Commitment*>> grouped, single;
-    for (auto& r : b.ok) (group_eligible(*r->key) ? grouped : single).push_back({r->key, r->cm.get()});
-    enqueue_commit_group(ctx, grouped, b.hold);
-    for (auto& s : single) enqueue_commit(ctx, *s.first, *s.second);
-    for (auto& r : b.ok) {
-      const DevTree& t = r->cm->tree;
-      hip_check(hipMemcpyAsync(r->root, t.nodes.ptr + (t.node_count() - 1) * 8, 32, hipMemcpyDeviceToHost,
-                               ctx.stream),
-                "read root");
-    }
-    hip_check(hipEventRecord(b.done, ctx.stream), "committed");
-  } catch (const std::exception& e) {
-    const Error* le = dynamic_cast<const Error*>(&e);
-    b.rc = le ? le->code : (dynamic_cast<const std::bad_alloc*>(&e) ? 4 : 1);
-    b.msg = e.what();
+  const Key& key = *r.key;
+  r.cm.reset(new Commitment());
+  Commitment& c = *r.cm;
+  c.ctx = &ctx;
+  c.key = r.key;
+  c.log_m = key.log_m;
+  c.log_n = key.log_n;
+  const uint64_t m = uint64_t(1) << key.log_m, n = uint64_t(1) << key.log_n, two_n = 2 * n;
+  const int logN = key.log_n + 1;
+  const bool host = !(r.flags & 1u);
+  Tuning tu = ctx.resolve(key.log_m, key.log_n);
+  tu.pipeline = 1;
+  HashPlan hp = hash_plan(key, tu);
+  const uint32_t chunks = hp.variant == 1 ? serve_chunks(key.log_m, key.log_n, host) : 1;
+  if (chunks > 1) {  // one word plane per split, and whole splits per chunk
+    if (!hash_impl_reduces(hp.impl)) hp.impl = 12;
+    if (hp.ks < chunks) hp.ks = chunks;
   }
+  c.enc = DevBuf<uint64_t>(&ctx.pool, m * two_n);
+  c.h = DevBuf<uint64_t>(&ctx.pool, 32 * two_n);
+  c.tree.leaves = two_n;
+  c.tree.nodes = DevBuf<uint32_t>(&ctx.pool, (size_t)c.tree.node_count() * 8);
+  g.parts.emplace_back(&ctx.pool, (hash_partial_bytes(hp, two_n) + 7) / 8);
+  CommitIO io;
+  io.enc = c.enc.ptr;
+  io.h = c.h.ptr;
+  io.nodes = c.tree.nodes.ptr;
+  io.part = g.parts.back().ptr;
+  io.tw = ctx.twiddle(logN);
+  io.tp = tu.ntt_impl == 4 ? ctx.pass_twiddle(logN) : nullptr;
+  if (!host) {
+    c.f = r.f;
+    io.f = c.f;
+    enqueue_commit_stages(ctx, key, io, hp, tu, ctx.stream, false);
+  } else {
+    c.f_owned = DevBuf<uint64_t>(&ctx.pool, r.len);
+    c.f = c.f_owned.ptr;
+    io.f = c.f;
+    const uint64_t rows = m / chunks;
+    const uint32_t kc = hp.ks / chunks;
+    for (uint32_t q = 0; q < chunks; q++) {
+      const uint64_t off = q * rows * n;
+      hip_check(hipMemcpyAsync(c.f_owned.ptr + off, r.f + off, rows * n * 8, hipMemcpyHostToDevice, upload_stream),
+                "upload f");
+      hipEvent_t e = event();
+      g.events.push_back(e);
+      hip_check(hipEventRecord(e, upload_stream), "uploaded");
+      hip_check(hipStreamWaitEvent(ctx.stream, e, 0), "wait upload");
+      if (chunks == 1) break;
+      // this chunk's rows are encoded and hashed (its K-splits) while the next chunk uploads
+      hip_check(launch_ntt(c.f + off, n, 1, (uint32_t)n, (uint32_t)rows, logN, io.tw, c.enc.ptr + q * rows * two_n,
+                           ctx.stream, tu.ntt_impl, io.tp),
+                "row encode");
+      hip_check(launch_hash_mfma_splits(c.enc.ptr, two_n, (uint32_t)m, (uint32_t)two_n, key.dig.ptr, hp.ks, q * kc, kc,
+                                        (int32_t*)io.part, ctx.stream, hp.impl),
+                "hash (MFMA)");
+    }
+    if (chunks == 1) {
+      enqueue_commit_stages(ctx, key, io, hp, tu, ctx.stream, false);
+    } else {
+      hip_check(launch_hash_mfma_finalize((const int32_t*)io.part, hp.ks, (uint32_t)two_n, c.h.ptr, ctx.stream,
+                                          hp.impl),
+                "hash finalize");
+      hip_check(launch_merkle(c.h.ptr, 32, two_n, 1, two_n, c.tree.nodes.ptr, ctx.stream, tu.merkle_lg), "merkle");
+    }
+  }
+  hip_check(hipMemcpyAsync(roots + 32 * (size_t)r.slot, c.tree.nodes.ptr + (c.tree.node_count() - 1) * 8, 32,
+                           hipMemcpyDeviceToHost, ctx.stream),
+            "read root");
 }
 
-// Waits for a batch's commits and hands its commitments to the prove workers.
-void Server::finish_batch(Batch& b) {
-  Context& ctx = *ctxs[0];
-  if (!b.rc) {
-    const hipError_t e = hipEventSynchronize(b.done);
-    if (e != hipSuccess) {
-      b.rc = 3;
-      b.msg = std::string("commit sync: ") + hipGetErrorString(e);
-    }
-  }
-  if (b.rc) {
-    (void)hipStreamSynchronize(upload_stream);
-    ctx.drain();
-  }
-  std::lock_guard<std::mutex> lk(mu);
-  batches++;
-  batched_requests += b.ok.size();
-  const double t = now_ms();
-  for (auto& r : b.ok) {
-    if (b.rc) {
-      r->status = b.rc;
-      r->err = b.msg;
-      r->cm.reset();
-      reserved -= r->footprint;
-      continue;
-    }
-    std::memcpy(r->cm->root, r->root, 32);
-    r->t_committed = t;
-    committed.push_back(r);
-  }
-  cv_prove.notify_all();
-  cv_done.notify_all();
-  cv_commit.notify_all();
-}
-
-// Admission and commits, pipelined: the uploads of a newly admitted batch are enqueued before the
-// batch in flight is waited for, so they overlap its commit kernels (and the proofs running on
-// the workers' streams); its commits follow once the batch in flight is handed over.
+// Admission and enqueueing; never waits for the device.
 void Server::commit_loop() {
   (void)hipSetDevice(device);
-  std::unique_ptr<Batch> inflight;
   while (true) {
-    std::vector<std::shared_ptr<Request>> batch;
+    std::unique_ptr<Group> g(new Group());
     {
       std::unique_lock<std::mutex> lk(mu);
-      auto admissible = [&] { return !pending.empty() && reserved + pending.front()->footprint <= budget; };
-      if (!inflight) cv_commit.wait(lk, [&] { return stopping || admissible(); });
-      if (stopping && !inflight) {  // queued requests were cancelled by stop; admitted ones finish
+      auto admissible = [&] {
+        return !pending.empty() && reserved + pending.front()->footprint <= budget && in_flight < kRootSlots;
+      };
+      cv_commit.wait(lk, [&] { return stopping || admissible(); });
+      if (stopping) {  // queued requests were cancelled by stop; admitted ones finish
         commit_running = false;
+        cv_finish.notify_all();
+        return;
+      }
+      while (admissible() && g->reqs.size() < max_batch) {
+        auto r = pending.front();
+        pending.pop_front();
+        reserved += r->footprint;
+        in_flight++;
+        r->slot = next_slot;
+        next_slot = (next_slot + 1) % kRootSlots;
+        g->reqs.push_back(r);
+      }
+    }
+    std::vector<std::shared_ptr<Request>> ok;
+    for (auto& r : g->reqs) {
+      try {
+        enqueue_request(*g, *r);
+        ok.push_back(r);
+      } catch (const std::exception& e) {
+        // work already enqueued for this request may still read its buffers, which stay with
+        // it until the group's event; only this request fails
+        const Error* le = dynamic_cast<const Error*>(&e);
+        r->enqueue_code = le ? le->code : (dynamic_cast<const std::bad_alloc*>(&e) ? 4 : 1);
+        r->err = e.what();
+        ok.push_back(r);
+      }
+    }
+    try {
+      g->done = event();
+      hip_check(hipEventRecord(g->done, ctxs[0]->stream), "committed");
+    } catch (const std::exception& e) {
+      g->rc = 3;
+      g->msg = e.what();
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      g->reqs.swap(ok);
+      finishing.push_back(std::move(g));
+    }
+    cv_finish.notify_all();
+  }
+}
+
+// Waits for admitted groups in order and hands their commitments to the prove workers.
+void Server::finish_loop() {
+  (void)hipSetDevice(device);
+  Context& ctx = *ctxs[0];
+  while (true) {
+    std::unique_ptr<Group> g;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_finish.wait(lk, [&] { return !finishing.empty() || !commit_running; });
+      if (finishing.empty()) {  // the commit thread has exited and nothing is left
+        finish_running = false;
         cv_prove.notify_all();
         return;
       }
-      while (!stopping && admissible() && batch.size() < max_batch) {
-        reserved += pending.front()->footprint;
-        batch.push_back(pending.front());
-        pending.pop_front();
+      g = std::move(finishing.front());
+      finishing.pop_front();
+    }
+    if (!g->rc) {
+      const hipError_t e = hipEventSynchronize(g->done);
+      if (e != hipSuccess) {
+        g->rc = 3;
+        g->msg = std::string("commit sync: ") + hipGetErrorString(e);
       }
     }
-    if (batch.empty()) {  // nothing more fits now: hand over the batch in flight
-      finish_batch(*inflight);
-      inflight.reset();
-      continue;
+    if (g->rc) {
+      (void)hipStreamSynchronize(upload_stream);
+      ctx.drain();
     }
-    std::unique_ptr<Batch> next;
-    try {
-      next = start_batch(batch);
-    } catch (const std::exception& e) {  // event creation failed: fail the batch's requests
+    g->parts.clear();  // the commits are done (or the streams drained)
+    {
       std::lock_guard<std::mutex> lk(mu);
-      for (auto& r : batch) {
-        if (r->status >= 0) continue;
-        r->cm.reset();
-        r->status = 3;
-        r->err = e.what();
-        reserved -= r->footprint;
+      for (hipEvent_t e : g->events) free_events.push_back(e);
+      if (g->done) free_events.push_back(g->done);
+      batches++;
+      batched_requests += g->reqs.size();
+      const double t = now_ms();
+      for (auto& r : g->reqs) {
+        in_flight--;
+        if (g->rc || r->enqueue_code) {
+          r->status = r->enqueue_code ? r->enqueue_code : g->rc;
+          if (!r->enqueue_code) r->err = g->msg;
+          r->cm.reset();
+          reserved -= r->footprint;
+          continue;
+        }
+        std::memcpy(r->root, roots + 32 * (size_t)r->slot, 32);
+        std::memcpy(r->cm->root, r->root, 32);
+        r->t_committed = t;
+        committed.push_back(r);
       }
-      cv_done.notify_all();
-      continue;
     }
-    if (inflight) {
-      finish_batch(*inflight);
-      inflight.reset();
-    }
-    commit_batch(*next);
-    inflight = std::move(next);
+    cv_prove.notify_all();
+    cv_done.notify_all();
+    cv_commit.notify_all();
   }
 }
 
@@ -244,8 +330,8 @@ void Server::prove_loop(Context* ctx) {
     std::shared_ptr<Request> r;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv_prove.wait(lk, [&] { return !committed.empty() || !commit_running; });
-      if (committed.empty()) return;  // the commit worker has exited and nothing is left
+      cv_prove.wait(lk, [&] { return !committed.empty() || !finish_running; });
+      if (committed.empty()) return;  // the finisher has exited and nothing is left
       r = committed.front();
       committed.pop_front();
     }
@@ -281,9 +367,14 @@ Server* server_start(int device, uint64_t budget, uint32_t workers, uint32_t max
   s->max_batch = max_batch ? max_batch : 16;
   if (workers == 0) workers = 2;
   hip_check(hipSetDevice(device), "set device");
-  for (uint32_t i = 0; i <= workers; i++) s->ctxs.push_back(make_context(device));
+  for (uint32_t i = 0; i <= workers; i++) {
+    s->ctxs.push_back(make_context(device));
+    s->ctxs.back()->single_stream_prove = true;
+  }
   hip_check(hipStreamCreateWithFlags(&s->upload_stream, hipStreamNonBlocking), "stream");
+  hip_check(hipHostMalloc((void**)&s->roots, 32 * (size_t)kRootSlots, hipHostMallocDefault), "root slots");
   s->threads.emplace_back(&Server::commit_loop, s.get());
+  s->threads.emplace_back(&Server::finish_loop, s.get());
   for (uint32_t i = 1; i <= workers; i++) s->threads.emplace_back(&Server::prove_loop, s.get(), s->ctxs[i].get());
   return s.release();
 }
@@ -355,6 +446,7 @@ void server_stop(Server* s) {
     s->cv_done.notify_all();
   }
   s->cv_commit.notify_all();
+  s->cv_finish.notify_all();
   s->cv_prove.notify_all();
   for (auto& t : s->threads) t.join();
   for (auto& kv : s->all) kv.second->cm.reset();
@@ -364,6 +456,8 @@ void server_stop(Server* s) {
     (void)hipStreamDestroy(s->upload_stream);
   }
   for (auto& c : s->ctxs) teardown_context(*c);
+  for (hipEvent_t e : s->free_events) (void)hipEventDestroy(e);
+  if (s->roots) (void)hipHostFree(s->roots);
   delete s;
 }
 
