@@ -174,7 +174,7 @@ def main():
         bench_commit(args)
 
 
-SERVE_LOADS = (0.6, 0.7, 0.8, 0.85, 0.9, 0.95, 1.0, 1.05, 1.1, 1.2)
+SERVE_LOADS = (0.6, 0.7, 0.8, 0.85, 0.9, 0.95, 1.0, 1.05, 1.1, 1.2, 1.3, 1.4, 1.5)
 
 
 def measure_serve(nreq: int, workers: int, max_batch: int, loads=SERVE_LOADS, seed: int = 7) -> dict:
@@ -712,7 +712,8 @@ def bench_commit(args):
     if not args.commit_only:
         # the second half of the metric: eval-prove at mu = 24 and the served stream's latency
         out["prove_mu24"] = [measure_prove(lm, steps=10, warmup=2, profile=False) for lm in (8, 7)]
-        sv = measure_serve(args.requests, args.prove_workers, args.max_batch, loads=(0.7, 0.8, 0.9, 1.0, 1.1))
+        sv = measure_serve(args.requests, args.prove_workers, args.max_batch,
+                           loads=(0.7, 0.8, 0.9, 1.0, 1.1, 1.2, 1.3, 1.4))
         out["serve"] = sv
         at = next(l for l in sv["poisson"] if l["load"] == 0.8)
         out["latency"] = {"load": "Poisson arrivals at 0.8 of the per-request capacity "

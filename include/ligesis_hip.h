@@ -227,6 +227,23 @@ int lgs_server_start(int device, uint64_t hbm_budget, uint32_t prove_workers, ui
 int lgs_server_submit(lgs_server* srv, const lgs_key* key, const uint64_t* f, uint64_t len, uint32_t flags,
                       const uint64_t* z, uint64_t z_len, const uint64_t* fields, uint64_t n_fields,
                       const uint64_t* ints, uint64_t n_ints, uint64_t* ticket);
+/* Commit-only request: f is committed as by lgs_server_submit, and the commitment (f, F', H and
+ * the tree) then stays resident in HBM, its commit footprint reserved, until lgs_server_release
+ * of this ticket. Polling the ticket returns its root (proof_len 0). The key may be commit-only
+ * (LGS_SETUP_COMMIT_ONLY) if no open of it follows. */
+int lgs_server_submit_commit(lgs_server* srv, const lgs_key* key, const uint64_t* f, uint64_t len, uint32_t flags,
+                             uint64_t* ticket);
+/* Open request (Ligesis::prove, ligesis.rs:347) of the resident commitment of commit-only ticket
+ * `commitment`, at z (z_len extension elements) with the given oracle streams (copied), queued
+ * and admitted like other requests; it waits for the commitment's commit if that is not done.
+ * Polling returns the proof (and the commitment's root). Opens of one commitment may run at once.
+ * LGS_ERR_ARG for an unknown, failed or released commitment. */
+int lgs_server_submit_open(lgs_server* srv, uint64_t commitment, const uint64_t* z, uint64_t z_len,
+                           const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,
+                           uint64_t* ticket);
+/* Releases a resident commitment: its memory returns to the budget once no open of it runs.
+ * Opens submitted earlier still complete; later ones are refused. */
+int lgs_server_release(lgs_server* srv, uint64_t commitment);
 /* Result of a request: LGS_PENDING while running (wait = 0), LGS_ERR_BUFFER with *proof_len set
  * when `proof` is too small (the result is kept), LGS_OK with root, proof and
  * times_ms = {submit->committed, committed->proved, submit->proved} (then the ticket is

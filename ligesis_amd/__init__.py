@@ -588,6 +588,9 @@ class Server:
                                         ctypes.POINTER(ctypes.c_uint64)]
         L.lgs_server_poll.argtypes = [vp, u64, ctypes.c_int, vp, vp, ctypes.POINTER(ctypes.c_uint64), vp]
         L.lgs_server_stats.argtypes = [vp, vp]
+        L.lgs_server_submit_commit.argtypes = [vp, vp, vp, u64, u32, ctypes.POINTER(ctypes.c_uint64)]
+        L.lgs_server_submit_open.argtypes = [vp, u64, vp, u64, vp, u64, vp, u64, ctypes.POINTER(ctypes.c_uint64)]
+        L.lgs_server_release.argtypes = [vp, u64]
         L.lgs_server_stop.argtypes = [vp]
         L.lgs_server_last_error.restype = ctypes.c_char_p
         h = ctypes.c_void_p()
@@ -596,6 +599,7 @@ class Server:
             raise LigesisError(rc, L.lgs_server_last_error().decode())
         self.h = h
         self._keep = {}
+        self._resident_keep = {}  # inputs of resident commitments (a device f is borrowed)
 
     def submit(self, key: ProvingKey, f, z, oracle: RandomOracle, device: bool = False) -> int:
         z = _u64(z).reshape(-1, 2)
@@ -612,6 +616,41 @@ class Server:
             raise LigesisError(rc, lib().lgs_server_last_error().decode())
         self._keep[t.value] = (keep, key)
         return t.value
+
+    def submit_commit(self, key: ProvingKey, f, device: bool = False) -> int:
+        """Commit-only request: the commitment stays resident in the server until ``release``;
+        ``poll`` returns (root, b"", times)."""
+        if device:
+            fptr, flen, keep = ctypes.c_void_p(f.data_ptr()), f.numel(), f
+        else:
+            fa = _u64(f)
+            fptr, flen, keep = _ptr(fa), fa.size, fa
+        t = ctypes.c_uint64(0)
+        rc = lib().lgs_server_submit_commit(self.h, key.h, fptr, flen, LGS_INPUT_DEVICE if device else 0,
+                                            ctypes.byref(t))
+        if rc != LGS_OK:
+            raise LigesisError(rc, lib().lgs_server_last_error().decode())
+        self._keep[t.value] = (keep, key)
+        self._resident_keep[t.value] = (keep, key)
+        return t.value
+
+    def submit_open(self, commitment: int, z, oracle: RandomOracle) -> int:
+        """Open request (eval-prove) of the resident commitment of a commit-only ticket."""
+        z = _u64(z).reshape(-1, 2)
+        t = ctypes.c_uint64(0)
+        rc = lib().lgs_server_submit_open(self.h, commitment, _ptr(z), z.shape[0], _ptr(oracle.fields),
+                                          oracle.fields.shape[0], _ptr(oracle.ints), oracle.ints.size,
+                                          ctypes.byref(t))
+        if rc != LGS_OK:
+            raise LigesisError(rc, lib().lgs_server_last_error().decode())
+        return t.value
+
+    def release(self, commitment: int):
+        """Releases a resident commitment (its memory returns once no open of it runs)."""
+        rc = lib().lgs_server_release(self.h, commitment)
+        if rc != LGS_OK:
+            raise LigesisError(rc, lib().lgs_server_last_error().decode())
+        self._resident_keep.pop(commitment, None)
 
     def poll(self, ticket: int, wait: bool = True):
         """Returns None while running, else (root, proof bytes, {commit_ms, prove_ms, total_ms})."""
@@ -643,6 +682,7 @@ class Server:
             lib().lgs_server_stop(self.h)
             self.h = None
             self._keep.clear()
+            self._resident_keep.clear()
 
     def __del__(self):
         self.stop()

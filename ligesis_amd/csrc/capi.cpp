@@ -593,6 +593,42 @@ int lgs_server_submit(lgs_server* srv, const lgs_key* key, const uint64_t* f, ui
   }
 }
 
+int lgs_server_submit_commit(lgs_server* srv, const lgs_key* key, const uint64_t* f, uint64_t len, uint32_t flags,
+                             uint64_t* ticket) {
+  if (!srv || !key || !f || !ticket) return LGS_ERR_ARG;
+  try {
+    *ticket = lgs::server_submit_commit(srv->s, &key->k, f, len, flags);
+    return LGS_OK;
+  } catch (const lgs::Error& e) {
+    g_server_err = e.what();
+    return e.code;
+  }
+}
+
+int lgs_server_submit_open(lgs_server* srv, uint64_t commitment, const uint64_t* z, uint64_t z_len,
+                           const uint64_t* fields, uint64_t n_fields, const uint64_t* ints, uint64_t n_ints,
+                           uint64_t* ticket) {
+  if (!srv || !z || !ticket || (n_fields && !fields) || (n_ints && !ints)) return LGS_ERR_ARG;
+  try {
+    *ticket = lgs::server_submit_open(srv->s, commitment, z, z_len, fields, n_fields, ints, n_ints);
+    return LGS_OK;
+  } catch (const lgs::Error& e) {
+    g_server_err = e.what();
+    return e.code;
+  }
+}
+
+int lgs_server_release(lgs_server* srv, uint64_t commitment) {
+  if (!srv) return LGS_ERR_ARG;
+  try {
+    lgs::server_release(srv->s, commitment);
+    return LGS_OK;
+  } catch (const lgs::Error& e) {
+    g_server_err = e.what();
+    return e.code;
+  }
+}
+
 int lgs_server_poll(lgs_server* srv, uint64_t ticket, int wait, uint8_t root[32], uint8_t* proof,
                     uint64_t* proof_len, double times_ms[3]) {
   if (!srv || !root || !proof_len) return LGS_ERR_ARG;

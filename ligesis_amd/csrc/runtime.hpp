@@ -389,6 +389,12 @@ Server* server_start(int device, uint64_t budget, uint32_t workers, uint32_t max
 uint64_t server_submit(Server* s, const Key* key, const uint64_t* f, uint64_t len, uint32_t flags,
                        const uint64_t* z, uint64_t z_len, const uint64_t* fields, uint64_t nf,
                        const uint64_t* ints, uint64_t ni);
+// Commit-only request (its commitment stays resident until server_release) and an open of a
+// resident commitment, by the commit-only request's ticket.
+uint64_t server_submit_commit(Server* s, const Key* key, const uint64_t* f, uint64_t len, uint32_t flags);
+uint64_t server_submit_open(Server* s, uint64_t commitment, const uint64_t* z, uint64_t z_len, const uint64_t* fields,
+                            uint64_t nf, const uint64_t* ints, uint64_t ni);
+void server_release(Server* s, uint64_t commitment);
 int server_poll(Server* s, uint64_t ticket, bool wait, uint8_t root[32], uint8_t* proof, uint64_t* proof_len,
                 double times[3], std::string& err);
 void server_stop(Server* s);

@@ -14,6 +14,9 @@
 //    other, the uploads and the next commits.
 // Memory returns to the budget when a proof is done; requests that arrive meanwhile are admitted
 // as soon as they fit (continuous batching). Results are identical to lgs_commit + lgs_prove.
+// Besides commit-then-open requests the loop takes commit-only requests, whose commitment stays
+// resident (f, F', H, tree in HBM) until released, and open requests on such a commitment, queued
+// like the others and proved by the workers (several opens of one commitment may run at once).
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -48,25 +51,35 @@ static uint32_t serve_chunks(uint32_t log_m, uint32_t log_n, bool host_input) {
   return c;
 }
 
-// Conservative device footprint of one commit + open at (log_m, log_n), in bytes.
-uint64_t request_footprint(uint32_t log_m, uint32_t log_n, bool host_input) {
-  const uint64_t m = uint64_t(1) << log_m, n = uint64_t(1) << log_n, W = 64 * m, s = 128;
-  const uint64_t two_n = 2 * n;
-  // hash partials: at most the larger of the built-in K-split (8 int32 planes a split) and the
-  // chunked commit's splits (one word plane a split)
+// Conservative device footprints at (log_m, log_n), in bytes: a commit (with its hash partials:
+// at most the larger of the built-in K-split, 8 int32 planes a split, and the chunked commit's
+// splits, one word plane a split) and one open of it.
+static uint64_t commit_footprint(uint32_t log_m, uint32_t log_n, bool host_input) {
+  const uint64_t m = uint64_t(1) << log_m, n = uint64_t(1) << log_n, two_n = 2 * n;
   const uint64_t ks = m >= 1024 ? m / 512 : 1;
   const uint64_t parts = std::max<uint64_t>(ks * 256 * two_n * 4,
                                             (uint64_t)serve_chunks(log_m, log_n, host_input) * 32 * two_n * 8);
-  uint64_t commit = (host_input ? 8 * m * n : 0) + 16 * m * n + 32 * two_n * 8 + 2 * two_n * 32 + parts;
-  uint64_t bi = W * s;
-  uint64_t D = uint64_t(2) << batch_rounds(log_m, log_n);
-  uint64_t prove = bi * (8 + 4 * 16) + 64 * D * 8 + 2 * D * 32 + 4 * D * 16 + 24 * n * 16 + W * 6 * 16 +
-                   two_n * 64 + (uint64_t(1) << 24);
-  return commit + prove;
+  return (host_input ? 8 * m * n : 0) + 16 * m * n + 32 * two_n * 8 + 2 * two_n * 32 + parts;
 }
+
+static uint64_t open_footprint(uint32_t log_m, uint32_t log_n) {
+  const uint64_t m = uint64_t(1) << log_m, n = uint64_t(1) << log_n, W = 64 * m, s = 128;
+  const uint64_t two_n = 2 * n, bi = W * s;
+  const uint64_t D = uint64_t(2) << batch_rounds(log_m, log_n);
+  return bi * (8 + 4 * 16) + 64 * D * 8 + 2 * D * 32 + 4 * D * 16 + 24 * n * 16 + W * 6 * 16 + two_n * 64 +
+         (uint64_t(1) << 24);
+}
+
+uint64_t request_footprint(uint32_t log_m, uint32_t log_n, bool host_input) {
+  return commit_footprint(log_m, log_n, host_input) + open_footprint(log_m, log_n);
+}
+
+enum Kind { kCommitOpen = 0, kCommitOnly = 1, kOpen = 2 };
 
 struct Request {
   uint64_t ticket = 0;
+  int kind = kCommitOpen;
+  uint64_t of = 0;  // kOpen: the ticket of the resident commitment
   const Key* key = nullptr;
   const uint64_t* f = nullptr;
   uint64_t len = 0;
@@ -74,7 +87,7 @@ struct Request {
   std::vector<uint64_t> z, fields, ints;
   uint64_t footprint = 0;
   double t_submit = 0, t_committed = 0, t_done = 0;
-  std::unique_ptr<Commitment> cm;
+  std::shared_ptr<Commitment> cm;  // kOpen: the resident commitment, shared with other opens
   uint32_t slot = 0;  // pinned root slot
   uint8_t root[32] = {0};
   std::vector<uint8_t> proof;
@@ -94,6 +107,17 @@ struct Group {
   std::string msg;
 };
 
+// A commitment kept in HBM by a commit-only request until released. `footprint` is its commit
+// footprint, returned to the budget once it is released and no open of it runs.
+struct Resident {
+  const Key* key = nullptr;
+  std::shared_ptr<Commitment> cm;  // set when the commit is done
+  uint64_t footprint = 0;
+  uint32_t opens = 0;  // opens admitted and not yet finished
+  bool released = false, failed = false;
+  std::vector<std::shared_ptr<Request>> waiting;  // opens admitted before the commit was done
+};
+
 struct Server {
   int device = 0;
   uint64_t budget = 0, reserved = 0;
@@ -103,6 +127,7 @@ struct Server {
   std::deque<std::shared_ptr<Request>> pending, committed;
   std::deque<std::unique_ptr<Group>> finishing;
   std::map<uint64_t, std::shared_ptr<Request>> all;
+  std::map<uint64_t, Resident> resident;  // by the commit-only request's ticket
   uint64_t next_ticket = 1;
   uint32_t in_flight = 0;  // admitted requests not yet handed to the prove workers
   bool stopping = false;
@@ -117,6 +142,7 @@ struct Server {
   std::vector<hipEvent_t> free_events;  // events whose last use has completed (guarded by mu)
 
   hipEvent_t event();
+  void drop_resident_locked(uint64_t ticket);
   void enqueue_request(Group& g, Request& r);
   void commit_loop();
   void finish_loop();
@@ -135,6 +161,16 @@ hipEvent_t Server::event() {
   hipEvent_t e = nullptr;
   hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "server event");
   return e;
+}
+
+// Frees a released resident commitment once no open of it runs (mu held).
+void Server::drop_resident_locked(uint64_t ticket) {
+  auto it = resident.find(ticket);
+  if (it == resident.end() || !it->second.released || it->second.opens) return;
+  if (!it->second.cm && !it->second.failed) return;  // its commit is still in flight
+  reserved -= it->second.footprint;
+  resident.erase(it);
+  cv_commit.notify_all();
 }
 
 // Upload (if f is in host memory) and commit of one request, enqueued; nothing waits.
@@ -231,11 +267,24 @@ void Server::commit_loop() {
         auto r = pending.front();
         pending.pop_front();
         reserved += r->footprint;
+        if (r->kind == kOpen) {  // no commit: to the provers once its commitment is done
+          Resident& rs = resident.at(r->of);
+          if (rs.cm) {
+            r->cm = rs.cm;
+            r->t_committed = now_ms();
+            committed.push_back(r);
+            cv_prove.notify_all();
+          } else {
+            rs.waiting.push_back(r);
+          }
+          continue;
+        }
         in_flight++;
         r->slot = next_slot;
         next_slot = (next_slot + 1) % kRootSlots;
         g->reqs.push_back(r);
       }
+      if (g->reqs.empty()) continue;
     }
     std::vector<std::shared_ptr<Request>> ok;
     for (auto& r : g->reqs) {
@@ -310,11 +359,42 @@ void Server::finish_loop() {
           if (!r->enqueue_code) r->err = g->msg;
           r->cm.reset();
           reserved -= r->footprint;
+          if (r->kind == kCommitOnly) {  // its waiting opens fail with it
+            Resident& rs = resident.at(r->ticket);
+            for (auto& o : rs.waiting) {
+              o->status = r->status;
+              o->err = "open of a failed commitment: " + r->err;
+              reserved -= o->footprint;
+              rs.opens--;
+            }
+            rs.waiting.clear();
+            rs.failed = true;
+            rs.footprint = 0;
+            rs.released = true;
+            drop_resident_locked(r->ticket);
+          }
           continue;
         }
         std::memcpy(r->root, roots + 32 * (size_t)r->slot, 32);
         std::memcpy(r->cm->root, r->root, 32);
         r->t_committed = t;
+        if (r->kind == kCommitOnly) {  // done: the commitment stays resident for its opens
+          Resident& rs = resident.at(r->ticket);
+          rs.cm = r->cm;
+          rs.footprint = r->footprint;  // reserved until the commitment is released
+          r->footprint = 0;
+          r->cm.reset();
+          r->t_done = t;
+          r->status = 0;
+          for (auto& o : rs.waiting) {
+            o->cm = rs.cm;
+            o->t_committed = t;
+            committed.push_back(o);
+          }
+          rs.waiting.clear();
+          drop_resident_locked(r->ticket);  // released before its commit was done
+          continue;
+        }
         committed.push_back(r);
       }
     }
@@ -348,13 +428,21 @@ void Server::prove_loop(Context* ctx) {
       msg = e.what();
       ctx->drain();
     }
-    r->cm.reset();  // f, F', H and the tree go back to the commit context's pool
+    if (r->kind == kOpen) std::memcpy(r->root, r->cm->root, 32);
+    r->cm.reset();  // f, F', H and the tree go back to the commit context's pool (unless resident)
     std::lock_guard<std::mutex> lk(mu);
     r->proof.swap(proof);
     r->status = rc;
     r->err = msg;
     r->t_done = now_ms();
     reserved -= r->footprint;
+    if (r->kind == kOpen) {
+      auto it = resident.find(r->of);
+      if (it != resident.end()) {
+        it->second.opens--;
+        drop_resident_locked(r->of);
+      }
+    }
     cv_done.notify_all();
     cv_commit.notify_all();
   }
@@ -405,6 +493,65 @@ uint64_t server_submit(Server* s, const Key* key, const uint64_t* f, uint64_t le
   return r->ticket;
 }
 
+uint64_t server_submit_commit(Server* s, const Key* key, const uint64_t* f, uint64_t len, uint32_t flags) {
+  if (len != (uint64_t(1) << (key->log_m + key->log_n))) throw Error(1, "server: polynomial length mismatch");
+  auto r = std::make_shared<Request>();
+  r->kind = kCommitOnly;
+  r->key = key;
+  r->f = f;
+  r->len = len;
+  r->flags = flags;
+  // the whole commit footprint stays reserved while the commitment is resident
+  r->footprint = commit_footprint(key->log_m, key->log_n, !(flags & 1u));
+  if (r->footprint > s->budget) throw Error(4, "server: request needs more HBM than the whole budget");
+  r->t_submit = now_ms();
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stopping) throw Error(7, "server is stopping");
+  r->ticket = s->next_ticket++;
+  s->all[r->ticket] = r;
+  s->resident[r->ticket].key = key;  // its footprint moves to the entry when the commit is done
+  s->pending.push_back(r);
+  s->cv_commit.notify_all();
+  return r->ticket;
+}
+
+uint64_t server_submit_open(Server* s, uint64_t commitment, const uint64_t* z, uint64_t z_len, const uint64_t* fields,
+                            uint64_t nf, const uint64_t* ints, uint64_t ni) {
+  auto r = std::make_shared<Request>();
+  r->kind = kOpen;
+  r->of = commitment;
+  r->z.assign(z, z + 2 * z_len);
+  r->fields.assign(fields, fields + 2 * nf);
+  r->ints.assign(ints, ints + ni);
+  r->t_submit = now_ms();
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stopping) throw Error(7, "server is stopping");
+  auto it = s->resident.find(commitment);
+  if (it == s->resident.end() || it->second.released)
+    throw Error(1, "server: no resident commitment with this ticket (unknown, failed or released)");
+  const Key* key = it->second.key;
+  if (!key->has_state) throw Error(7, "server: opens need a key with the A-bar commitment");
+  if (z_len != key->log_m + key->log_n) throw Error(1, "server: z must have log_m + log_n entries");
+  r->key = key;
+  r->footprint = open_footprint(key->log_m, key->log_n);
+  if (r->footprint > s->budget) throw Error(4, "server: request needs more HBM than the whole budget");
+  it->second.opens++;
+  r->ticket = s->next_ticket++;
+  s->all[r->ticket] = r;
+  s->pending.push_back(r);
+  s->cv_commit.notify_all();
+  return r->ticket;
+}
+
+void server_release(Server* s, uint64_t commitment) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->resident.find(commitment);
+  if (it == s->resident.end() || it->second.released)
+    throw Error(1, "server: no resident commitment with this ticket (unknown or already released)");
+  it->second.released = true;
+  s->drop_resident_locked(commitment);
+}
+
 // 0: done, result copied out and released; 8: still running; 6: proof buffer too small
 // (*proof_len = size needed, result kept); other codes are the request's own failure.
 int server_poll(Server* s, uint64_t ticket, bool wait, uint8_t root[32], uint8_t* proof, uint64_t* proof_len,
@@ -422,9 +569,9 @@ int server_poll(Server* s, uint64_t ticket, bool wait, uint8_t root[32], uint8_t
   }
   const uint64_t cap = *proof_len;
   *proof_len = r->proof.size();
-  if (!proof || cap < r->proof.size()) return 6;
+  if (!r->proof.empty() && (!proof || cap < r->proof.size())) return 6;
   std::memcpy(root, r->root, 32);
-  std::memcpy(proof, r->proof.data(), r->proof.size());
+  if (!r->proof.empty()) std::memcpy(proof, r->proof.data(), r->proof.size());
   if (times) {
     times[0] = r->t_committed - r->t_submit;
     times[1] = r->t_done - r->t_committed;
@@ -451,6 +598,7 @@ void server_stop(Server* s) {
   for (auto& t : s->threads) t.join();
   for (auto& kv : s->all) kv.second->cm.reset();
   s->all.clear();
+  s->resident.clear();
   if (s->upload_stream) {
     (void)hipStreamSynchronize(s->upload_stream);
     (void)hipStreamDestroy(s->upload_stream);

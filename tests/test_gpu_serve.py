@@ -101,3 +101,36 @@ def test_server_budget_and_errors(gpu_ctx):
     assert srv.poll(good)[1] == res[0][1]
     srv.stop()
     _ = np
+
+
+def test_resident_commitment_with_separate_opens(gpu_ctx):
+    # commit-only requests keep their commitment resident; opens at different points, submitted
+    # before and after the commit is done, equal lone proves; a released commitment refuses opens
+    shapes = [(2, 12), (6, 14)]
+    srv = Server(0, hbm_budget=8 << 30, prove_workers=2, max_batch=4)
+    for k, (lm, ln) in enumerate(shapes):
+        print(f"resident commitment shape=({lm},{ln})", flush=True)
+        inp = synth.Inputs(lm, ln, seed=950 + k)
+        pk, vk = Ligesis.setup(lm, ln, inp.a, ctx=gpu_ctx)
+        c = srv.submit_commit(pk, inp.f)
+        pts = [synth.ext_elems(960 + 10 * k + j, lm + ln) for j in range(3)]
+        early = srv.submit_open(c, pts[0], RandomOracle(inp.fields, inp.ints))  # may wait for the commit
+        root, proof, _ = srv.poll(c)
+        assert proof == b""
+        eroot, cd = Ligesis.commit(pk, inp.f)
+        assert root == eroot
+        late = [srv.submit_open(c, p, RandomOracle(inp.fields, inp.ints)) for p in pts[1:]]
+        for t, p in zip([early] + late, pts):
+            r, pf, times = srv.poll(t)
+            assert r == eroot
+            assert pf == Ligesis.prove(pk, cd, p, RandomOracle(inp.fields, inp.ints))
+            assert oracle.verify(lm, ln, vk.com_a, r, p, pf, inp.fields, inp.ints)
+        cd.release()
+        srv.release(c)
+        with pytest.raises(LigesisError):
+            srv.submit_open(c, pts[0], RandomOracle(inp.fields, inp.ints))
+        with pytest.raises(LigesisError):
+            srv.release(c)
+    st = srv.stats()
+    assert st["reserved_bytes"] == 0
+    srv.stop()
