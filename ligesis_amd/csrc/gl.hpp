@@ -72,6 +72,42 @@ LGS_HD uint64_t gl_mul(uint64_t a, uint64_t b) {
   return gl_reduce(lo, hi);
 }
 
+// a * w * 2^-64 mod p, canonical, for a < 2^64 and w < p: the product of a and a factor kept in
+// Montgomery form (w = c 2^64 mod p gives a c). Montgomery reduction of T = hi 2^64 + lo:
+// p^-1 = 1 + 2^32 (mod 2^64), so m = lo (1 + 2^32) mod 2^64 makes T - m p divisible by 2^64, and
+// (T - m p) / 2^64 = hi - k with k = floor(m p / 2^64) = m - (m >> 32) - e, e the carry out of
+// lo + (lo << 32) (e = 1 exactly when the high half of m is below its low half). k < p, so
+// hi - k lies in (-p, 2^64): add p on a borrow, else subtract p once if needed. On 32-bit
+// halves: one add, two two-word subtractions, a conditional -(2^32 - 1) and a compare -- against
+// the 128-bit reduction's two 64-bit corrections, 32 x 32 product and final compare.
+LGS_HD uint64_t gl_mul_mont(uint64_t a, uint64_t w) {
+  uint64_t lo, hi;
+  mul_wide(a, w, lo, hi);
+  unsigned e, c0, c1, d0, d1, f0, f1;
+  const unsigned ml = (unsigned)lo;
+  const unsigned mh = __builtin_addc((unsigned)(lo >> 32), ml, 0u, &e);
+  const unsigned kl = __builtin_subc(ml, mh, e, &c0);
+  const unsigned kh = __builtin_subc(mh, 0u, c0, &c1);
+  unsigned rl = __builtin_subc((unsigned)hi, kl, 0u, &d0);
+  unsigned rh = __builtin_subc((unsigned)(hi >> 32), kh, d0, &d1);
+  // a borrow: r + 2^64 - (2^32 - 1) = hi - k + p, in (0, p)
+  rl = __builtin_subc(rl, d1 ? 0xffffffffu : 0u, 0u, &f0);
+  rh = __builtin_subc(rh, 0u, f0, &f1);
+  return gl_canon(((uint64_t)rh << 32) | rl);
+}
+
+// x * 2^-K mod p for canonical x and 1 <= K <= 32 (K a constant where this is inlined). Since
+// p = 1 (mod 2^32), t = -x mod 2^K makes x + t p divisible by 2^K, and
+// (x + t p) / 2^K = (x + t) / 2^K + t 2^(64-K) - t 2^(32-K): the first two terms occupy disjoint
+// bits, x + t < 2^64, and the subtraction cannot borrow (t 2^(64-K) >= t 2^(32-K)); one
+// canonicalization remains. No 128-bit product or reduction.
+LGS_HD uint64_t gl_mul_inv_pow2(uint64_t x, int K) {
+  const uint64_t t = (0 - x) & ((uint64_t(1) << K) - 1);
+  const uint64_t u = x + t;
+  const uint64_t a = (u >> K) | (t << (64 - K));
+  return gl_canon(a - (t << (32 - K)));
+}
+
 LGS_HD uint64_t gl_pow(uint64_t a, uint64_t e) {
   uint64_t r = 1;
   while (e) {

@@ -57,7 +57,7 @@ hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_i
                       int logN, const uint64_t* tw, uint64_t* out, hipStream_t st, int impl = 0,
                       const uint64_t* tp = nullptr);
 // Entries of the pass twiddle table of a 2^L-point transform (0 where L != 1 mod 3), and its fill
-// from the transform's twiddle table tw.
+// from the transform's twiddle table tw (entries in Montgomery form, c 2^64 mod p).
 uint32_t pass_twiddle_entries(int L);
 hipError_t launch_pass_twiddles(uint64_t* tp, const uint64_t* tw, int L, hipStream_t st);
 

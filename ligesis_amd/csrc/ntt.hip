@@ -78,9 +78,11 @@ __device__ __forceinline__ uint64_t gl_reduce_small(uint64_t lo, uint64_t hi) {
   return gl_canon(r);
 }
 
-// x * 2^e mod p for 0 <= e < 192 (e is a compile-time constant where this is inlined)
+// x * 2^e mod p for 0 <= e < 192 (e is a compile-time constant where this is inlined), by the
+// cheapest exact form: 2^96 = -1 gives 2^e = -2^(e - 96), and for 64 <= e < 96,
+// 2^e = -2^-(96 - e), a division by at most 2^32 (gl_mul_inv_pow2) instead of two reductions.
 __device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x, uint32_t e) {
-  const bool neg = e >= 96;
+  bool neg = e >= 96;
   if (neg) e -= 96;
   uint64_t r;
   if (e == 0) {
@@ -89,9 +91,9 @@ __device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x, uint32_t e) {
     r = gl_reduce_small(x << e, x >> (64 - e));
   } else if (e < 64) {
     r = gl_reduce(x << e, x >> (64 - e));
-  } else {  // x * 2^e = (x * 2^(e - 32)) * 2^32, both factors below 2^64 shifts
-    const uint64_t y = gl_reduce(x << (e - 32), x >> (96 - e));
-    r = gl_reduce_small(y << 32, y >> 32);
+  } else {
+    r = gl_mul_inv_pow2(x, 96 - e);
+    neg = !neg;
   }
   return neg ? gl_neg(r) : r;
 }
@@ -130,15 +132,15 @@ __device__ __forceinline__ void dft8(uint64_t (&x)[8]) {
     x[0] = gl_add(a, b);
     x[4] = gl_sub(a, b);
     a = x[1];
-    b = gl_mul_pow2(x[5], 120);  // w_8
-    x[1] = gl_add(a, b);
-    x[5] = gl_sub(a, b);
+    b = gl_mul_pow2(x[5], 24);  // w_8 = 2^120 = -2^24: the sign swaps the butterfly's outputs
+    x[1] = gl_sub(a, b);
+    x[5] = gl_add(a, b);
     a = x[2];
     b = gl_mul_pow2(x[6], 48);  // w_8^2
     x[2] = gl_add(a, b);
     x[6] = gl_sub(a, b);
     a = x[3];
-    b = gl_mul_pow2(x[7], 168);  // w_8^3
+    b = gl_mul_pow2(x[7], 168);  // w_8^3 = 2^-24
     x[3] = gl_add(a, b);
     x[7] = gl_sub(a, b);
   }
@@ -301,8 +303,9 @@ __global__ __launch_bounds__(1024) void ntt_r8_kernel(const uint64_t* __restrict
 
 // Pass twiddles of a 2^L-point transform whose radix-8 passes start at s = 4 (L = 1 mod 3):
 // for s = 4, 7, ..., L - 3, section s holds tp[off_s + (t - 1) 2^s + j] = w_{2^(s+3)}^(j rev3(t))
-// (t = 1..7, j < 2^s) -- the factors pretwiddle<3> takes from tw -- so that the 64 lanes of a
-// wave (consecutive j) read 512 contiguous bytes per factor.
+// (t = 1..7, j < 2^s) -- the factors pretwiddle<3> takes from tw -- in Montgomery form (times
+// 2^64 mod p, for gl_mul_mont), so that the 64 lanes of a wave (consecutive j) read 512
+// contiguous bytes per factor.
 __global__ void pass_twiddle_kernel(uint64_t* __restrict__ tp, const uint64_t* __restrict__ tw, int L) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t off = 0;
@@ -311,7 +314,7 @@ __global__ void pass_twiddle_kernel(uint64_t* __restrict__ tp, const uint64_t* _
     if (i < off + 7 * M) {
       const uint32_t q = i - off, t = q / M + 1, j = q & (M - 1);
       const uint32_t r = __brev(t) >> 29;
-      tp[i] = tw_full(tw, (j * r) << (L - s - 3), 1u << (L - 1), 0);
+      tp[i] = gl_mul(tw_full(tw, (j * r) << (L - s - 3), 1u << (L - 1), 0), GL_EPS);  // 2^64 = 2^32 - 1
       return;
     }
     off += 7 * M;
@@ -335,7 +338,8 @@ hipError_t launch_pass_twiddles(uint64_t* tp, const uint64_t* tw, int L, hipStre
 // The row LDE of rows of 2^(L-1) values (row stride rs, unit element stride) zero-padded to
 // 2^L points, one workgroup of 2^(L-3) threads per row: ntt_r8_row's zero-half schedule (stages
 // 2-4 fused with the load, then radix-8 passes from s = 4) with every size a compile-time
-// constant, one butterfly group per thread and pass, twiddles from the pass table tp, and each
+// constant, one butterfly group per thread and pass, Montgomery-form twiddles from the pass
+// table tp (one Montgomery reduction per product instead of a 128-bit one), and each
 // LDS address one XOR: lds_pos is linear over GF(2), and a group's base and its offsets t 2^s
 // occupy disjoint bits, so lds_pos(base + t 2^s) = lds_pos(base) ^ lds_pos(t 2^s).
 template <int L>
@@ -375,7 +379,7 @@ __global__ __launch_bounds__(1 << (L - 3)) void ntt_r8_fixed_kernel(const uint64
 #pragma unroll
     for (int t = 0; t < 8; t++) x[t] = sm[pb ^ lds_pos(t << s, sh)];
 #pragma unroll
-    for (int t = 1; t < 8; t++) x[t] = gl_mul(x[t], tp[off + (t - 1) * M + j]);
+    for (int t = 1; t < 8; t++) x[t] = gl_mul_mont(x[t], tp[off + (t - 1) * M + j]);
     dft8(x);
     if (s == L - 3) {
 #pragma unroll

@@ -78,6 +78,31 @@ def test_every_row_transform_matches_oracle(gpu_ctx, shape):
         gpu_ctx.set_option("ntt_impl", -1)
 
 
+def test_row_transform_extreme_values(gpu_ctx):
+    """Rows of boundary values through every row-transform variant at rows of 2^12 (the
+    compile-time kernel with Montgomery-form pass twiddles and shift/division power-of-two
+    factors): p - 1, p - 2, 2^32 - 1, 2^32, 2^63, 1, alternating 0 / p - 1, and a random row."""
+    log_m, log_n = 3, 12
+    p = synth.P
+    n = 1 << log_n
+    rows = [np.full(n, p - 1), np.full(n, p - 2), np.full(n, 0xFFFFFFFF), np.full(n, 1 << 32),
+            np.full(n, 1 << 63), np.ones(n), np.tile([0, p - 1], n // 2),
+            synth.field_elems(91, n)]
+    f = np.concatenate([np.asarray(r, dtype=np.uint64) for r in rows])
+    inp = synth.Inputs(log_m, log_n, seed=92)
+    pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx, commit_only=True)
+    ocd = oracle.CommitData(oracle.ProvingKey(log_m, log_n, inp.a, commit_a=False), f, keep_enc=True)
+    try:
+        for impl in (3, 4):
+            print(f"extreme rows ntt_impl={impl}", flush=True)
+            gpu_ctx.set_option("ntt_impl", impl)
+            root, cd = Ligesis.commit(pk, f)
+            np.testing.assert_array_equal(cd.encoded(), ocd.enc())
+            assert root == ocd.root, impl
+    finally:
+        gpu_ctx.set_option("ntt_impl", -1)
+
+
 def test_mfma_and_valu_hash_agree(gpu_ctx):
     inp = synth.Inputs(5, 9, seed=3)
     pk, _ = Ligesis.setup(5, 9, inp.a, ctx=gpu_ctx, commit_only=True)
