@@ -14,6 +14,7 @@ library or the GPU is missing.
 """
 from __future__ import annotations
 
+import sys as _sys
 import ctypes
 import os
 
@@ -142,7 +143,8 @@ class Context:
             self.h = None
 
     def __del__(self):
-        self.close()
+        if not _sys.is_finalizing():  # module globals may already be gone
+            self.close()
 
 
 TUNE_FIELDS = ("hash_variant", "hash_impl", "hash_ksplit", "ntt_impl", "merkle_lg", "pipeline")
@@ -214,7 +216,8 @@ class ProvingKey:
             self.h = None
 
     def __del__(self):
-        self.release()
+        if not _sys.is_finalizing():  # module globals may already be gone
+            self.release()
 
 
 class CommitData:
@@ -248,7 +251,8 @@ class CommitData:
             self.h = None
 
     def __del__(self):
-        self.release()
+        if not _sys.is_finalizing():  # module globals may already be gone
+            self.release()
 
 
 class RandomOracle:
@@ -569,7 +573,8 @@ class BasefoldProverState:
             self.h = None
 
     def __del__(self):
-        self.release()
+        if not _sys.is_finalizing():  # module globals may already be gone
+            self.release()
 
 
 LGS_PENDING = 8
@@ -685,4 +690,5 @@ class Server:
             self._resident_keep.clear()
 
     def __del__(self):
-        self.stop()
+        if not _sys.is_finalizing():  # module globals may already be gone
+            self.stop()

@@ -168,10 +168,10 @@ struct EqTable {
 };
 hipError_t launch_eq_group(const EqTerm* terms, int nterms, uint32_t term_blocks, const EqTable* tabs, int ntabs,
                            uint32_t tab_blocks, hipStream_t st);
+// out[j] = sum_i w[i] M[i][j] over `rows` rows of `cols` words: `chunks` row ranges summed into
+// part (chunks x cols), then a second launch adds the chunks (copies into out2 / out3 if set).
 hipError_t launch_wrows(const uint64_t* M, uint32_t rows, uint64_t cols, const ef2* w, ef2* part,
                         uint32_t chunks, ef2* out, hipStream_t st, ef2* out2 = nullptr, ef2* out3 = nullptr);
-// Partial sums are reduced by the last block to finish; `counter` is a zeroed device word that
-// the kernel re-arms, so consecutive launches on one stream may share it.
 // Partial sums of sum_i a[i] b[i], one per block (dot_blocks(n) blocks), for a RedJob.
 uint32_t dot_blocks(uint64_t n);
 hipError_t launch_dot(const ef2* a, const ef2* b, uint64_t n, ef2* part, hipStream_t st);

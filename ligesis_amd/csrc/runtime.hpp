@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <cstdlib>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -31,7 +32,25 @@ inline void hip_check(hipError_t e, const char* what) {
 class DevicePool {
  public:
   ~DevicePool() { trim(); }
+  // Test aid: with poison = b (0..255), every block handed out is first filled with byte b,
+  // after the whole device has drained (blocks are reused in stream order, so a block may still
+  // be read by queued work when it is freed). A kernel that reads memory it did not write then
+  // sees the pattern instead of stale or zeroed contents. Slow; set from LGS_POOL_POISON.
+  int poison = poison_from_env();
+  static int poison_from_env() {
+    const char* v = std::getenv("LGS_POOL_POISON");
+    return v && *v ? (std::atoi(v) & 255) : -1;
+  }
   void* alloc(size_t bytes) {
+    void* p = take(bytes);
+    if (poison >= 0) {
+      (void)hipDeviceSynchronize();
+      (void)hipMemset(p, poison, size_class(bytes));
+      (void)hipDeviceSynchronize();
+    }
+    return p;
+  }
+  void* take(size_t bytes) {
     size_t cls = size_class(bytes);
     {
       std::lock_guard<std::mutex> g(mu_);

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import oracle
+from tests import proof_format
 from ligesis_amd import Ligesis, LigesisError, RandomOracle, Server
 from ligesis_amd import synth
 
@@ -66,15 +67,20 @@ def test_served_stream_at_full_sizes(gpu_ctx):
     tickets = [srv.submit(keys[shape][0], r.f, r.z, RandomOracle(r.fields, r.ints)) for shape, r in reqs]
     got = [srv.poll(t) for t in tickets]
     srv.stop()
-    for (shape, r), (root, proof, _) in zip(reqs, got):
-        print(f"lone commit + prove shape={shape}", flush=True)
+    for k, ((shape, r), (root, proof, _)) in enumerate(zip(reqs, got)):
+        print(f"lone commit + prove of request {k} shape={shape}", flush=True)
         pk, vk = keys[shape]
         eroot, cd = Ligesis.commit(pk, r.f)
         eproof = Ligesis.prove(pk, cd, r.z, RandomOracle(r.fields, r.ints))
         cd.release()
+        served_ok = oracle.verify(shape[0], shape[1], vk.com_a, root, r.z, proof, r.fields, r.ints)
+        lone_ok = oracle.verify(shape[0], shape[1], vk.com_a, eroot, r.z, eproof, r.fields, r.ints)
+        if proof != eproof:
+            a, b = proof_format.decode(proof), proof_format.decode(eproof)
+            print(f"request {k}: served proof differs in {[f for f in a if a[f] != b[f]]}", flush=True)
+        assert (served_ok, lone_ok) == (True, True)
         assert root == eroot
         assert proof == eproof
-        assert oracle.verify(shape[0], shape[1], vk.com_a, root, r.z, proof, r.fields, r.ints)
     for pk, _ in keys.values():
         pk.release()
 
