@@ -129,7 +129,8 @@ def main():
                     help="MFMA kernel: -1 auto, 0 pipelined, 1 pipelined/1 slab, 2 basic, 3 256x256 tiles, "
                          "4 loads 2 slabs ahead, 5 as 4 with 256x256 tiles, 6/7 64x64 wave tiles, "
                          "8/9 64x128 wave tiles, 10 as 0 with A fragments read ahead, "
-                         "11/12 asynchronous copies into an LDS ring of 3/4 slabs")
+                         "11/12 asynchronous copies into an LDS ring of 3/4 slabs, 14 as 12 with 8 waves "
+                         "sharing the ring (256-column tiles)")
     ap.add_argument("--ksplit", type=int, default=0, help="hash K-split (0 auto)")
     ap.add_argument("--ntt-impl", type=int, default=-1,
                     help="-1 auto, 0/2/3 radix-8 passes with 256/512/1024 threads per row, 1 radix-2 stages")
@@ -665,8 +666,8 @@ def bench_commit(args):
                     5: "hash_mfma_ring_kernel<4, 2>", 6: "hash_mfma_tile_kernel<4, 4, 2, 2>",
                     7: "hash_mfma_tile_kernel<4, 4, 4, 1>", 8: "hash_mfma_tile_kernel<4, 8, 1, 2>",
                     9: "hash_mfma_tile_kernel<4, 8, 2, 1>",
-                    10: "hash_mfma_pipe_kernel<2, 2, true>", 11: "hash_mfma_dma_kernel<3>",
-                    12: "hash_mfma_dma_kernel<4>"}[schedule["hash_impl"]])
+                    10: "hash_mfma_pipe_kernel<2, 2, true>", 11: "hash_mfma_dma_kernel<3, 2>",
+                    12: "hash_mfma_dma_kernel<4, 2>", 14: "hash_mfma_dma_kernel<4, 4>"}[schedule["hash_impl"]])
     hash_tag = f"hash_impl={schedule['hash_impl']} ksplit={schedule['hash_ksplit']} pipeline={schedule['pipeline']}"
     ntt_tag = f"ntt_impl={schedule['ntt_impl']}"
     ntt_kernel = "ntt_r8_fixed_kernel<13>" if schedule["ntt_impl"] == 4 else "ntt_r8_kernel"

@@ -142,8 +142,8 @@ class Context:
             lib().lgs_shutdown(self.h)
             self.h = None
 
-    def __del__(self):
-        if not _sys.is_finalizing():  # module globals may already be gone
+    def __del__(self, _finalizing=_sys.is_finalizing):  # bound now: module globals go at exit
+        if not _finalizing():
             self.close()
 
 
@@ -215,8 +215,8 @@ class ProvingKey:
             lib().lgs_key_release(self.h)
             self.h = None
 
-    def __del__(self):
-        if not _sys.is_finalizing():  # module globals may already be gone
+    def __del__(self, _finalizing=_sys.is_finalizing):  # bound now: module globals go at exit
+        if not _finalizing():
             self.release()
 
 
@@ -250,8 +250,8 @@ class CommitData:
             lib().lgs_commitment_release(self.h)
             self.h = None
 
-    def __del__(self):
-        if not _sys.is_finalizing():  # module globals may already be gone
+    def __del__(self, _finalizing=_sys.is_finalizing):  # bound now: module globals go at exit
+        if not _finalizing():
             self.release()
 
 
@@ -572,8 +572,8 @@ class BasefoldProverState:
             lib().lgs_basefold_release(self.h)
             self.h = None
 
-    def __del__(self):
-        if not _sys.is_finalizing():  # module globals may already be gone
+    def __del__(self, _finalizing=_sys.is_finalizing):  # bound now: module globals go at exit
+        if not _finalizing():
             self.release()
 
 
@@ -689,6 +689,6 @@ class Server:
             self._keep.clear()
             self._resident_keep.clear()
 
-    def __del__(self):
-        if not _sys.is_finalizing():  # module globals may already be gone
+    def __del__(self, _finalizing=_sys.is_finalizing):  # bound now: module globals go at exit
+        if not _finalizing():
             self.stop()

@@ -417,6 +417,11 @@ void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, 
   std::vector<FinalizeJob> fj;
   uint32_t nblocks = 0, ntiles_total = 0, fblocks = 0;
   int max_logN = 1;
+  // 256-column tiles (8 waves sharing the A ring, one workgroup per CU) when every job has a
+  // multiple of 256 columns, else 128
+  uint32_t tile_cols = 256;
+  for (auto& it : items)
+    if ((uint64_t(2) << it.first->log_n) % 256) tile_cols = 128;
   for (auto& it : items) {
     const Key& key = *it.first;
     Commitment& c = *it.second;
@@ -431,7 +436,7 @@ void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, 
     nj.push_back(NttJob{c.f, n, 1, (uint32_t)n, logN, ctx.twiddle(logN), c.enc.ptr, nblocks});
     nblocks += (uint32_t)m;
     max_logN = logN > max_logN ? logN : max_logN;
-    const uint32_t ntiles = (uint32_t)(two_n / 128);
+    const uint32_t ntiles = (uint32_t)(two_n / tile_cols);
     const Tuning tu = ctx.resolve(key.log_m, key.log_n);
     uint32_t ks = tu.hash_ksplit ? (uint32_t)tu.hash_ksplit : (uint32_t)(m >= 1024 ? m / 512 : 1);
     if (ks > m / 2) ks = (uint32_t)(m / 2);
@@ -455,7 +460,7 @@ void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, 
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[0], ctx.stream), "event");
   hip_check(launch_ntt_group(d_nj, njobs, nblocks, max_logN, ctx.stream), "batched row encode");
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[1], ctx.stream), "event");
-  hip_check(launch_hash_group(d_hj, njobs, ntiles_total, nullptr, 0, ctx.stream), "batched hash");
+  hip_check(launch_hash_group(d_hj, njobs, ntiles_total, nullptr, 0, ctx.stream, (int)tile_cols), "batched hash");
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[2], ctx.stream), "event");
   hip_check(launch_hash_group(nullptr, njobs, 0, d_fj, fblocks, ctx.stream), "batched finalize");
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[3], ctx.stream), "event");

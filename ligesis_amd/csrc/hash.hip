@@ -11,6 +11,9 @@
 //    S_l = D_l . B is an exact int8 x {0,1} product on v_mfma_i32_16x16x64_i8 with int32 sums
 //    (|S_l| <= 128 W < 2^31), and H = sum_l 256^l S_l mod p. M = 8 digits x 32 rows = 256.
 // Both write per-K-split partials that a finalize kernel sums mod p.
+#include <map>
+#include <mutex>
+
 #include "kernels.hpp"
 
 namespace lgs {
@@ -472,14 +475,21 @@ __global__ __launch_bounds__(64 * (16 / MT) * WN, MINB) void hash_mfma_tile_kern
   }
 }
 
-// Asynchronous-copy variant of the 256 x 128 tile (2 x 2 waves of 128 x 64): each slab's A digits
-// (16 KiB) and F' words (128 columns, 1 KiB) go from global memory straight into an LDS ring of NB
-// slots (global_load_lds, no staging registers), NB - 1 slabs ahead of the slab being multiplied.
-// Every wave issues the same five copies per slab (four of A, one of F'), so "this wave's copies
-// of slab s are done" is vmcnt <= 5 (NB - 2) once the copies of the later slabs are issued; a
-// barrier then makes every wave's copies visible. The copy into a slot is issued after the
-// barrier that follows the last reads of its previous slab. Near the end the copies repeat the
-// last slab (unused), so the count stays uniform.
+// Asynchronous-copy variant: a 256 x 64 WN tile (2 x WN waves of 128 x 64; WN = 2 for 128 and
+// WN = 4 for 256 columns). Each slab's A digits (16 KiB, shared by every wave of the workgroup)
+// and F' words (64 WN columns, 512 WN bytes) go from global memory straight into an LDS ring of
+// NB slots (global_load_lds, no staging registers), NB - 1 slabs ahead of the slab being
+// multiplied. Every wave issues the same copies per slab (dma_copies: 16 KiB of A over the
+// workgroup's waves in 1 KiB pieces, and one 256-byte piece of F'), so "this wave's copies of slab
+// s are done" is vmcnt <= copies (NB - 2) once the copies of the later slabs are issued; a barrier
+// then makes every wave's copies visible. The copy into a slot is issued after the barrier that
+// follows the last reads of its previous slab. Near the end the copies repeat the last slab
+// (unused), so the count stays uniform.
+//
+// Copies into LDS must not share a CU with workgroups of other kernels: on this device such
+// copies have corrupted the LDS of co-resident workgroups of other kernels (never this kernel's
+// own results), so every launch of these kernels requests the whole LDS of a CU (dma_lds_bytes).
+// WN = 4 keeps two waves per SIMD in one workgroup per CU; WN = 2 then runs at one.
 #define LGS_AS1 __attribute__((address_space(1)))
 #define LGS_AS3 __attribute__((address_space(3)))
 constexpr int vmcnt_imm(int n) { return (n & 0xF) | (((n >> 4) & 3) << 14) | (7 << 4) | (0xF << 8); }
@@ -501,23 +511,29 @@ struct DmaTile {
   uint32_t ns, tid, lane, wave, wm, wn, g, c, ksh;
 };
 
+// Copies per wave and slab: 16 KiB of A in 1 KiB pieces over the 2 WN waves, plus one of F'.
+template <int WN>
+constexpr int dma_copies() { return 16 / (2 * WN) + 1; }
+
+template <int WN>
 __device__ __forceinline__ void dma_copy_slab(const DmaTile& t, uint32_t sl, DmaSlot d) {
+  constexpr int T = 128 * WN;  // threads
   const v4i* src = t.adig + (t.slab0 + sl) * 1024;
 #pragma unroll
-  for (int q = 0; q < 4; q++)
-    __builtin_amdgcn_global_load_lds((const LGS_AS1 void*)(src + q * 256 + t.tid),
-                                     (LGS_AS3 void*)(d.a + q * 256 + t.wave * 64), 16, 0, 0);
+  for (int q = 0; q < 1024 / T; q++)
+    __builtin_amdgcn_global_load_lds((const LGS_AS1 void*)(src + q * T + t.tid),
+                                     (LGS_AS3 void*)(d.a + q * T + t.wave * 64), 16, 0, 0);
   const uint32_t* bs = (const uint32_t*)(t.enc + (t.slab0 + sl) * t.ld + t.col0) + t.tid;
   __builtin_amdgcn_global_load_lds((const LGS_AS1 void*)bs, (LGS_AS3 void*)(d.b + t.wave * 64), 4, 0, 0);
 }
 
 // Slab sl from slot cur; the copy of slab sl + NB - 1 goes to slot nxt.
-template <int NB>
+template <int NB, int WN>
 __device__ __forceinline__ void dma_step(const DmaTile& t, uint32_t sl, DmaSlot cur, DmaSlot nxt, v4i (&acc)[8][4]) {
-  __builtin_amdgcn_s_waitcnt(vmcnt_imm(5 * (NB - 2)));
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(dma_copies<WN>() * (NB - 2)));
   asm volatile("s_barrier" ::: "memory");
   const uint32_t nx = sl + NB - 1;
-  dma_copy_slab(t, nx < t.ns ? nx : t.ns - 1, nxt);
+  dma_copy_slab<WN>(t, nx < t.ns ? nx : t.ns - 1, nxt);
   // the products issue at raised wave priority, so a SIMD's other wave (copies, barrier) does
   // not take issue slots from them
   __builtin_amdgcn_sched_barrier(0);
@@ -563,14 +579,17 @@ __device__ __forceinline__ void dma_step(const DmaTile& t, uint32_t sl, DmaSlot 
   __builtin_amdgcn_s_setprio(0);
 }
 
-// One 256 x 128 tile over slabs [split * ns, (split + 1) * ns) with a ring of NB slots (NB = 4
+// One 256 x 64 WN tile over slabs [split * ns, (split + 1) * ns) with a ring of NB slots (NB = 4
 // needs ns % 4 == 0).
-template <int NB>
+template <int NB, int WN>
 __device__ __forceinline__ void hash_dma_body(const uint64_t* __restrict__ enc, uint64_t ld,
                                               const v4i* __restrict__ adig, uint32_t ns, uint32_t split,
                                               uint32_t ntile, uint32_t ncols, uint64_t* __restrict__ part,
                                               DmaSlot s0, DmaSlot s1, DmaSlot s2, DmaSlot s3) {
   static_assert(NB == 3 || NB == 4, "ring of 3 or 4 slabs");
+  static_assert(WN == 2 || WN == 4, "2 or 4 waves along N");
+  // the epilogue exchanges through slot wn's A array: with WN = 4 all four slots hold 16 KiB,
+  // also for a 3-slot ring
   DmaTile t;
   t.enc = enc;
   t.ld = ld;
@@ -579,12 +598,12 @@ __device__ __forceinline__ void hash_dma_body(const uint64_t* __restrict__ enc, 
   t.tid = threadIdx.x;
   t.lane = t.tid & 63;
   t.wave = t.tid >> 6;
-  t.wm = t.wave >> 1;
-  t.wn = t.wave & 1;
+  t.wm = t.wave / WN;
+  t.wn = t.wave % WN;
   t.g = t.lane >> 4;
   t.c = t.lane & 15;
   t.ksh = 4 * (t.g & 1);
-  t.col0 = (uint64_t)ntile * 128;
+  t.col0 = (uint64_t)ntile * (64 * WN);
   t.slab0 = (uint64_t)split * ns;
 
   v4i acc[8][4];
@@ -593,31 +612,31 @@ __device__ __forceinline__ void hash_dma_body(const uint64_t* __restrict__ enc, 
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = v4i{0, 0, 0, 0};
 
-  dma_copy_slab(t, 0, s0);
-  dma_copy_slab(t, 1 < ns ? 1 : ns - 1, s1);
-  if (NB == 4) dma_copy_slab(t, 2 < ns ? 2 : ns - 1, s2);
+  dma_copy_slab<WN>(t, 0, s0);
+  dma_copy_slab<WN>(t, 1 < ns ? 1 : ns - 1, s1);
+  if (NB == 4) dma_copy_slab<WN>(t, 2 < ns ? 2 : ns - 1, s2);
   for (uint32_t sl = 0; sl < ns; sl += NB) {
     if (NB == 3) {
-      dma_step<3>(t, sl, s0, s2, acc);
-      if (sl + 1 < ns) dma_step<3>(t, sl + 1, s1, s0, acc);
-      if (sl + 2 < ns) dma_step<3>(t, sl + 2, s2, s1, acc);
+      dma_step<3, WN>(t, sl, s0, s2, acc);
+      if (sl + 1 < ns) dma_step<3, WN>(t, sl + 1, s1, s0, acc);
+      if (sl + 2 < ns) dma_step<3, WN>(t, sl + 2, s2, s1, acc);
     } else {  // one straight-line body
-      dma_step<4>(t, sl, s0, s3, acc);
-      dma_step<4>(t, sl + 1, s1, s0, acc);
-      dma_step<4>(t, sl + 2, s2, s1, acc);
-      dma_step<4>(t, sl + 3, s3, s2, acc);
+      dma_step<4, WN>(t, sl, s0, s3, acc);
+      dma_step<4, WN>(t, sl + 1, s1, s0, acc);
+      dma_step<4, WN>(t, sl + 2, s2, s1, acc);
+      dma_step<4, WN>(t, sl + 3, s3, s2, acc);
     }
   }
   // Epilogue: the split's contribution to H, reduced mod p. Row m = 32 l + r of the tile is digit
   // plane l of hash row r; a lane holds, for r = 16 h + 4 g + v and column nt * 16 + c, the
   // sums of planes l = 4 wm + k (k < 4) in acc[2 k + h][nt][v]. Each wave folds its four planes
   // exactly in 64 bits (|sum_k 256^k S| < 2^55); the upper planes' fold (wm = 1) moves through
-  // LDS (ring slots 0 and 1, free once every copy and read has finished) to the lower planes'
-  // wave, which adds 2^32 times it mod p and writes one word per (split, row, column).
+  // LDS (ring slot wn, free once every copy and read has finished) to the lower planes' wave,
+  // which adds 2^32 times it mod p and writes one word per (split, row, column).
   __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // no copy into LDS outlives the workgroup
   __builtin_amdgcn_s_waitcnt(kLgkmZero);
   asm volatile("s_barrier" ::: "memory");
-  uint64_t* xch = (uint64_t*)(t.wn ? s1.a : s0.a);  // 32 rows x 64 columns
+  uint64_t* xch = (uint64_t*)(t.wn == 0 ? s0.a : t.wn == 1 ? s1.a : t.wn == 2 ? s2.a : s3.a);  // 32 x 64
   // fold first, so the accumulators die here: f[(h * 4 + nt) * 4 + v]
   uint64_t f[32];
 #pragma unroll
@@ -656,25 +675,25 @@ __device__ __forceinline__ void hash_dma_body(const uint64_t* __restrict__ enc, 
   }
 }
 
-template <int NB>
-__global__ __launch_bounds__(256, 2) void hash_mfma_dma_kernel(const uint64_t* __restrict__ enc, uint64_t ld,
-                                                               const v4i* __restrict__ adig, uint32_t ns,
-                                                               uint32_t ntiles, uint32_t ncols,
-                                                               int32_t* __restrict__ part, uint32_t split0) {
+template <int NB, int WN>
+__global__ __launch_bounds__(128 * WN, 2) void hash_mfma_dma_kernel(const uint64_t* __restrict__ enc, uint64_t ld,
+                                                                   const v4i* __restrict__ adig, uint32_t ns,
+                                                                   uint32_t ntiles, uint32_t ncols,
+                                                                   int32_t* __restrict__ part, uint32_t split0) {
   __shared__ v4i la0[1024], la1[1024], la2[1024], la3[NB == 4 ? 1024 : 1];
-  __shared__ uint32_t lb0[256], lb1[256], lb2[256], lb3[NB == 4 ? 256 : 1];
+  __shared__ uint32_t lb0[128 * WN], lb1[128 * WN], lb2[128 * WN], lb3[NB == 4 ? 128 * WN : 1];
   const uint32_t logical = xcd_logical_block();
-  hash_dma_body<NB>(enc, ld, adig, ns, split0 + logical / ntiles, logical % ntiles, ncols, (uint64_t*)part,
-                    DmaSlot{la0, lb0},
-                    DmaSlot{la1, lb1}, DmaSlot{la2, lb2}, DmaSlot{la3, lb3});
+  hash_dma_body<NB, WN>(enc, ld, adig, ns, split0 + logical / ntiles, logical % ntiles, ncols, (uint64_t*)part,
+                        DmaSlot{la0, lb0}, DmaSlot{la1, lb1}, DmaSlot{la2, lb2}, DmaSlot{la3, lb3});
 }
 
 // Tiles of many commitments in one launch (batched commit): logical tile -> job by binary
 // search over the jobs' first tiles; a job whose split has a multiple of 4 slabs takes the 4-slot
-// ring.
-__global__ __launch_bounds__(256, 2) void hash_mfma_dma_group_kernel(const HashJob* __restrict__ jobs, int njobs) {
+// ring. Tiles are 64 WN columns wide.
+template <int WN>
+__global__ __launch_bounds__(128 * WN, 2) void hash_mfma_dma_group_kernel(const HashJob* __restrict__ jobs, int njobs) {
   __shared__ v4i la0[1024], la1[1024], la2[1024], la3[1024];
-  __shared__ uint32_t lb0[256], lb1[256], lb2[256], lb3[256];
+  __shared__ uint32_t lb0[128 * WN], lb1[128 * WN], lb2[128 * WN], lb3[128 * WN];
   const uint32_t logical = xcd_logical_block();
   int lo = 0, hi = njobs - 1;
   while (lo < hi) {
@@ -685,12 +704,12 @@ __global__ __launch_bounds__(256, 2) void hash_mfma_dma_group_kernel(const HashJ
   const HashJob& jb = jobs[lo];
   const uint32_t local = logical - jb.first_block;
   const DmaSlot s0{la0, lb0}, s1{la1, lb1}, s2{la2, lb2}, s3{la3, lb3};
-  if (jb.slabs_per_split % 4 == 0)
-    hash_dma_body<4>(jb.enc, jb.ld, (const v4i*)jb.adig, jb.slabs_per_split, local / jb.ntiles, local % jb.ntiles,
-                     jb.ncols, (uint64_t*)jb.part, s0, s1, s2, s3);
+  if (jb.slabs_per_split % 4 != 0)
+    hash_dma_body<3, WN>(jb.enc, jb.ld, (const v4i*)jb.adig, jb.slabs_per_split, local / jb.ntiles, local % jb.ntiles,
+                         jb.ncols, (uint64_t*)jb.part, s0, s1, s2, s3);
   else
-    hash_dma_body<3>(jb.enc, jb.ld, (const v4i*)jb.adig, jb.slabs_per_split, local / jb.ntiles, local % jb.ntiles,
-                     jb.ncols, (uint64_t*)jb.part, s0, s1, s2, s3);
+    hash_dma_body<4, WN>(jb.enc, jb.ld, (const v4i*)jb.adig, jb.slabs_per_split, local / jb.ntiles,
+                         local % jb.ntiles, jb.ncols, (uint64_t*)jb.part, s0, s1, s2, s3);
 }
 
 // H[r][j] = sum_l 256^l (sum_split part[split][32 l + r][j]) mod p.
@@ -767,6 +786,27 @@ hipError_t launch_hash_valu(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
   return hipGetLastError();
 }
 
+// Dynamic LDS that makes a launch of `kernel` hold all of a CU's LDS: the CU's LDS minus the
+// kernel's static LDS (hipFuncGetAttributes), looked up once per device and kernel.
+static uint32_t full_cu_pad(const void* kernel) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, uint32_t> known;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = known.find({dev, kernel});
+  if (it != known.end()) return it->second;
+  int cu_lds = 0;
+  hipFuncAttributes fa{};
+  uint32_t pad = 0;
+  if (hipDeviceGetAttribute(&cu_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess &&
+      hipFuncGetAttributes(&fa, kernel) == hipSuccess && (size_t)cu_lds > fa.sharedSizeBytes)
+    pad = (uint32_t)(cu_lds - fa.sharedSizeBytes);
+  (void)hipGetLastError();
+  known[{dev, kernel}] = pad;
+  return pad;
+}
+
 hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
                                   const int8_t* dig, uint32_t ksplit, uint32_t split0, uint32_t nsplits,
                                   int32_t* part, hipStream_t st, int impl) {
@@ -789,13 +829,18 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
   } else if (impl == 6 && ncols % 128 == 0) {  // 64 x 64 wave tiles, 8 waves, 256 x 128 workgroup tile
     hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 4, 2, 2>), dim3((ncols / 128) * nsplits), dim3(512), 0, st, enc, ld,
                        ad, sps, ncols / 128, ncols, part, split0);
-  } else if (impl == 11 || impl == 12) {  // asynchronous copies into an LDS ring of 3 / 4 slabs
+  } else if (impl == 14 && ncols % 256 == 0 && sps % 4 == 0) {  // 8 waves sharing the A ring
+    hipLaunchKernelGGL((hash_mfma_dma_kernel<4, 4>), dim3((ncols / 256) * nsplits), dim3(512), full_cu_pad((const void*)hash_mfma_dma_kernel<4, 4>), st,
+                       enc, ld, ad, sps, ncols / 256, ncols, part, split0);
+  } else if (impl == 11 || impl == 12 || impl == 14) {  // asynchronous copies into an LDS ring of 3 / 4 slabs
     if (impl == 11 || sps % 4)
-      hipLaunchKernelGGL((hash_mfma_dma_kernel<3>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps,
-                         ntiles, ncols, part, split0);
+      hipLaunchKernelGGL((hash_mfma_dma_kernel<3, 2>), dim3(ntiles * nsplits), dim3(256),
+                         full_cu_pad((const void*)hash_mfma_dma_kernel<3, 2>), st, enc,
+                         ld, ad, sps, ntiles, ncols, part, split0);
     else
-      hipLaunchKernelGGL((hash_mfma_dma_kernel<4>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps,
-                         ntiles, ncols, part, split0);
+      hipLaunchKernelGGL((hash_mfma_dma_kernel<4, 2>), dim3(ntiles * nsplits), dim3(256),
+                         full_cu_pad((const void*)hash_mfma_dma_kernel<4, 2>), st, enc,
+                         ld, ad, sps, ntiles, ncols, part, split0);
   } else if (impl == 10) {  // as 0, A fragments read one product group ahead
     hipLaunchKernelGGL((hash_mfma_pipe_kernel<2, 2, true>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad,
                        sps, ntiles, ncols, part, split0);
@@ -824,7 +869,7 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
   return hipGetLastError();
 }
 
-bool hash_impl_reduces(int impl) { return impl == 11 || impl == 12; }
+bool hash_impl_reduces(int impl) { return impl == 11 || impl == 12 || impl == 14; }
 
 hipError_t launch_hash_mfma_finalize(const int32_t* part, uint32_t ksplit, uint32_t ncols, uint64_t* h,
                                      hipStream_t st, int impl) {
@@ -865,8 +910,17 @@ __global__ void hash_finalize_group_kernel(const FinalizeJob* __restrict__ jobs,
 }
 
 hipError_t launch_hash_group(const HashJob* jobs, int njobs, uint32_t total_tiles, const FinalizeJob* fjobs,
-                             uint32_t total_fblocks, hipStream_t st) {
-  if (total_tiles) hipLaunchKernelGGL(hash_mfma_dma_group_kernel, dim3(total_tiles), dim3(256), 0, st, jobs, njobs);
+                             uint32_t total_fblocks, hipStream_t st, int tile_cols) {
+  if (total_tiles) {
+    if (tile_cols == 256)
+      hipLaunchKernelGGL((hash_mfma_dma_group_kernel<4>), dim3(total_tiles), dim3(512),
+                         full_cu_pad((const void*)hash_mfma_dma_group_kernel<4>), st, jobs,
+                         njobs);
+    else
+      hipLaunchKernelGGL((hash_mfma_dma_group_kernel<2>), dim3(total_tiles), dim3(256),
+                         full_cu_pad((const void*)hash_mfma_dma_group_kernel<2>), st, jobs,
+                         njobs);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (total_fblocks)

@@ -83,7 +83,7 @@ hipError_t launch_hash_mfma_finalize(const int32_t* part, uint32_t ksplit, uint3
 // Batched launches over job tables that live in device memory.
 hipError_t launch_ntt_group(const NttJob* jobs, int njobs, uint32_t total_blocks, int max_logN, hipStream_t st);
 hipError_t launch_hash_group(const HashJob* jobs, int njobs, uint32_t total_tiles, const FinalizeJob* fjobs,
-                             uint32_t total_fblocks, hipStream_t st);
+                             uint32_t total_fblocks, hipStream_t st, int tile_cols = 128);
 hipError_t launch_merkle_group(const MerkleJob* jobs, int njobs, uint32_t total_blocks, hipStream_t st);
 
 // merkle.hip: tree over nleaves (a power of two) leaves, leaf p = M[r*rs + p*cs], r < rows.

@@ -132,7 +132,7 @@ def test_every_mfma_kernel_matches_oracle(gpu_ctx, shape):
     want = ocd.h()
     gpu_ctx.set_option("hash", 1)
     try:
-        for impl in range(13):
+        for impl in list(range(13)) + [14]:
             for ks in (0, 1, 2, 4):
                 if ks > (1 << log_m) // 2:
                     continue
@@ -179,6 +179,25 @@ def test_commit_batch_matches_single(gpu_ctx):
         expect.append(Ligesis.commit(pk, inp.f)[0])
     res = Ligesis.commit_batch(keys, fs, ctx=gpu_ctx)
     assert [r for r, _ in res] == expect
+
+
+def test_commit_batch_of_wide_shapes_matches_single(gpu_ctx):
+    """Every job with 2n a multiple of 256: the grouped hash runs 256-column tiles (8 waves); jobs
+    whose K-splits have 2 or 4k slabs take the 3- or 4-slot ring."""
+    shapes = [(1, 9), (2, 9), (3, 10), (1, 7), (5, 8), (0, 8)]
+    keys, fs, expect = [], [], []
+    for i, (lm, ln) in enumerate(shapes):
+        inp = synth.Inputs(lm, ln, seed=130 + i, n_fields=4, n_ints=4)
+        pk, _ = Ligesis.setup(lm, ln, inp.a, ctx=gpu_ctx, commit_only=True)
+        root, cd = Ligesis.commit(pk, inp.f)
+        keys.append(pk)
+        fs.append(inp.f)
+        expect.append((root, cd.digests()))
+    print("batched commit of wide shapes", shapes, flush=True)
+    res = Ligesis.commit_batch(keys, fs, ctx=gpu_ctx)
+    for (r, cd), (er, eh) in zip(res, expect):
+        assert r == er
+        np.testing.assert_array_equal(cd.digests(), eh)
 
 
 def test_commit_rejects_bad_length(gpu_ctx):
