@@ -110,12 +110,17 @@ int lgs_init(int device, lgs_ctx** out) {
   auto* ctx = new lgs_ctx();
   ctx->c.owner = ctx;
   ctx->c.device = device;
-  if (hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+  // the second stream right after the first, so the two get different hardware queues (see
+  // make_context)
+  if (hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->c.stream2, hipStreamNonBlocking) != hipSuccess) {
+    lgs::teardown_context(ctx->c);
     delete ctx;
     return LGS_ERR_DEVICE;
   }
   for (auto& e : ctx->c.ev)
     if (hipEventCreate(&e) != hipSuccess) {
+      lgs::teardown_context(ctx->c);
       delete ctx;
       return LGS_ERR_DEVICE;
     }
@@ -479,7 +484,7 @@ int lgs_prove_batch(lgs_ctx* ctx, uint32_t count, const lgs_key* const* keys, co
     std::vector<lgs::Context*> wk{&c};
     {
       std::lock_guard<std::mutex> g(c.workers_mu);
-      while (c.workers.size() + 1 < nw) c.workers.push_back(lgs::make_context(c.device));
+      while (c.workers.size() + 1 < nw) c.workers.push_back(lgs::make_context(c.device, true));
       for (uint32_t w = 1; w < nw; w++) wk.push_back(c.workers[w - 1].get());
     }
     std::vector<uint64_t> sizes(count, 0);

@@ -110,11 +110,12 @@ void PinnedArena::release() {
   used_ = 0;
 }
 
-std::unique_ptr<Context> make_context(int device) {
+std::unique_ptr<Context> make_context(int device, bool two_streams) {
   std::unique_ptr<Context> c(new Context());
   c->device = device;
   hip_check(hipSetDevice(device), "set device");
   hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
+  if (two_streams) (void)c->side_stream();
   for (auto& e : c->ev) hip_check(hipEventCreate(&e), "event");
   return c;
 }

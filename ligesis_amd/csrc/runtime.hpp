@@ -322,7 +322,11 @@ struct GroupHold {
 
 // commit.cpp: a context with its stream and events on `device`; the release of everything a
 // context holds (its workers' contexts, streams, events, tables, pool and pinned staging)
-std::unique_ptr<Context> make_context(int device);
+// A context with its stream; with two_streams also its second stream, created right after the
+// first: HIP maps streams to a few hardware queues in creation order, and a second stream made
+// later (after other contexts' streams) can land on the first one's queue, which serializes the
+// prove's two streams (0.82 ms instead of 0.57 ms at mu = 24).
+std::unique_ptr<Context> make_context(int device, bool two_streams = false);
 void teardown_context(Context& c);
 // commit.cpp; `force` replaces the resolved schedule (autotune candidates)
 void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* force = nullptr);
