@@ -57,6 +57,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        # PyTorch (device tensors) ships its own HIP runtime under the same soname; loaded after
+        # this library, it finds the system runtime already in the process and sees no devices.
+        # Loading it first makes both use one runtime.
+        if "torch" not in _sys.modules:
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
         L = ctypes.CDLL(LIB_PATH)
         vp, u32, u64, i64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64
         pp = ctypes.POINTER(ctypes.c_void_p)
