@@ -43,6 +43,10 @@ enum {
 
 /* lgs_setup flags */
 #define LGS_SETUP_COMMIT_ONLY 1u /* build the hash only: no A-bar commitment, no shape window */
+/* The Z_2^64 subset-sum hash (a paper-level variant the reference leaves out, subset_sum.rs:10-12):
+ * H = A . B with sums mod 2^64 and A any 64-bit words; its commitments are a different scheme
+ * (wire record version 2). Implies LGS_SETUP_COMMIT_ONLY; needs m >= 2 and 2n >= 128. */
+#define LGS_SETUP_RING64 2u
 /* lgs_commit flags */
 #define LGS_INPUT_DEVICE 1u /* f is a device pointer; the commitment borrows it until release */
 
@@ -117,9 +121,10 @@ int lgs_tune_save(lgs_ctx* ctx, const char* path, uint64_t* count);
 int lgs_tune_load(lgs_ctx* ctx, const char* path, uint64_t* count);
 void lgs_tune_clear(void);
 
-/* Setup (ligesis.rs:239): A is the subset-sum matrix, 32 rows x (64 << log_m) canonical
- * uint64, row-major (subset_sum.rs:23-27). Unless LGS_SETUP_COMMIT_ONLY, the shape must pass
- * the reference's shared-domain window and A-bar is Basefold-committed on the GPU. */
+/* Setup (ligesis.rs:239): A is the subset-sum matrix, 32 rows x (64 << log_m) uint64,
+ * row-major (subset_sum.rs:23-27), canonical (< p; LGS_ERR_ARG otherwise) unless LGS_SETUP_RING64.
+ * Unless LGS_SETUP_COMMIT_ONLY, the shape must pass the reference's shared-domain window and
+ * A-bar is Basefold-committed on the GPU. */
 int lgs_setup(lgs_ctx* ctx, uint32_t log_m, uint32_t log_n, const uint64_t* a, uint32_t flags,
               lgs_key** out);
 /* com_a of the key (ligesis.rs:71); LGS_ERR_STATE for a commit-only key. */

@@ -38,6 +38,7 @@ LGS_ERR_BUFFER = 6
 LGS_ERR_STATE = 7
 
 LGS_SETUP_COMMIT_ONLY = 1
+LGS_SETUP_RING64 = 2
 LGS_INPUT_DEVICE = 1
 LGS_BASEFOLD_EXT = 2
 LGS_READ_ENCODED, LGS_READ_DIGESTS, LGS_READ_TREE = 0, 1, 2
@@ -202,16 +203,20 @@ class VerifyingKey:
 class ProvingKey:
     """``ProvingKey`` (ligesis.rs:67-74): the hash matrix A and the A-bar Basefold state, in HBM."""
 
-    def __init__(self, ctx: Context, log_m: int, log_n: int, a: np.ndarray, commit_only: bool = False):
+    def __init__(self, ctx: Context, log_m: int, log_n: int, a: np.ndarray, commit_only: bool = False,
+                 ring64: bool = False):
+        """``ring64``: the Z_2^64 subset-sum hash (sums mod 2^64, any 64-bit A), a commit-only
+        variant the reference leaves out (subset_sum.rs:10-12)."""
         a = _u64(a)
         if a.size != C * (ETA << log_m):
             raise LigesisError(LGS_ERR_ARG, f"A must have {C} x {ETA << log_m} entries")
         h = ctypes.c_void_p()
-        ctx.check(lib().lgs_setup(ctx.h, log_m, log_n, _ptr(a),
-                                  LGS_SETUP_COMMIT_ONLY if commit_only else 0, ctypes.byref(h)))
+        flags = (LGS_SETUP_COMMIT_ONLY if commit_only else 0) | (LGS_SETUP_RING64 if ring64 else 0)
+        ctx.check(lib().lgs_setup(ctx.h, log_m, log_n, _ptr(a), flags, ctypes.byref(h)))
         self.ctx, self.h = ctx, h
         self.log_m, self.log_n = log_m, log_n
-        self.commit_only = commit_only
+        self.commit_only = commit_only or ring64
+        self.ring64 = ring64
 
     def com_a(self) -> bytes:
         out = np.zeros(32, np.uint8)
@@ -285,10 +290,10 @@ class Ligesis:
 
     @staticmethod
     def setup(log_m: int, log_n: int, a: np.ndarray, ctx: Context | None = None,
-              commit_only: bool = False):
+              commit_only: bool = False, ring64: bool = False):
         ctx = ctx or default_context()
-        pk = ProvingKey(ctx, log_m, log_n, a, commit_only=commit_only)
-        vk = VerifyingKey(b"" if commit_only else pk.com_a(), log_m, log_n)
+        pk = ProvingKey(ctx, log_m, log_n, a, commit_only=commit_only, ring64=ring64)
+        vk = VerifyingKey(b"" if pk.commit_only else pk.com_a(), log_m, log_n)
         return pk, vk
 
     @staticmethod

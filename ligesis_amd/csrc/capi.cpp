@@ -222,9 +222,17 @@ int lgs_setup(lgs_ctx* ctx, uint32_t log_m, uint32_t log_n, const uint64_t* a, u
   if (!ctx || !out || !a) return LGS_ERR_ARG;
   *out = nullptr;
   return guarded(&ctx->c, [&] {
-    bool commit_only = (flags & LGS_SETUP_COMMIT_ONLY) != 0;
+    const bool ring64 = (flags & LGS_SETUP_RING64) != 0;
+    const bool commit_only = ring64 || (flags & LGS_SETUP_COMMIT_ONLY) != 0;
     if (log_m > 20 || log_n > 26 || log_n < 1)
       throw lgs::Error(LGS_ERR_ARG, "setup: shape out of supported range");
+    if (ring64 && (log_m < 1 || log_n < 6))
+      throw lgs::Error(LGS_ERR_SHAPE, "setup: the Z_2^64 hash runs on the MFMA kernels (m >= 2, 2n >= 128)");
+    if (!ring64) {  // A is C x 64 m field elements (subset_sum.rs:31-39): canonical words
+      const uint64_t words = uint64_t(32) * (uint64_t(64) << log_m);
+      for (uint64_t i = 0; i < words; i++)
+        if (a[i] >= lgs::GL_P) throw lgs::Error(LGS_ERR_ARG, "setup: A holds a word >= p (not a field element)");
+    }
     if (!commit_only && !lgs::shape_ok(log_m, log_n))
       throw lgs::Error(LGS_ERR_SHAPE,
                        "batched LigeSIS needs the {A-bar, B_I, a-bar} var counts within "
@@ -233,6 +241,7 @@ int lgs_setup(lgs_ctx* ctx, uint32_t log_m, uint32_t log_n, const uint64_t* a, u
     auto* k = new lgs_key();
     k->k.log_m = log_m;
     k->k.log_n = log_n;
+    k->k.ring64 = ring64;
     try {
       lgs::setup_key(ctx->c, k->k, a, !commit_only);
     } catch (...) {

@@ -179,7 +179,7 @@ void setup_key(Context& ctx, Key& key, const uint64_t* host_a, bool commit_a) {
             "upload A");
   key.at = DevBuf<uint64_t>(&ctx.pool, 32 * key.W);
   key.dig = DevBuf<int8_t>(&ctx.pool, 256 * key.W);
-  hip_check(launch_a_layouts(key.a.ptr, key.W, key.at.ptr, key.dig.ptr, ctx.stream), "A layouts");
+  hip_check(launch_a_layouts(key.a.ptr, key.W, key.at.ptr, key.dig.ptr, ctx.stream, key.ring64), "A layouts");
   if (commit_a) {
     uint32_t nv = (uint32_t)ilog2(32 * key.W);
     uint32_t li = shared_li(nv, key.log_m, key.log_n);
@@ -264,7 +264,7 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
       hip_check(hipStreamWaitEvent(sh, ctx.pipe_event(q), 0), "pipeline wait");
       mark(4 * q + 2, sh);
       hip_check(launch_hash_mfma_splits(c.enc.ptr, two_n, (uint32_t)m, (uint32_t)two_n, key.dig.ptr, ks, q * kc, kc,
-                                        part.ptr, sh, tu.hash_impl),
+                                        part.ptr, sh, plan.impl, key.ring64),
                 "hash (MFMA)");
       mark(4 * q + 3, sh);
     }
@@ -273,7 +273,8 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
       hip_check(hipEventRecord(ctx.ev[1], ctx.stream), "event");  // every chunk encoded
       hip_check(hipEventRecord(ctx.ev[2], sh), "event");          // every split hashed
     }
-    hip_check(launch_hash_mfma_finalize(part.ptr, ks, (uint32_t)two_n, c.h.ptr, sh, tu.hash_impl), "hash finalize");
+    hip_check(launch_hash_mfma_finalize(part.ptr, ks, (uint32_t)two_n, c.h.ptr, sh, plan.impl, key.ring64),
+              "hash finalize");
     if (ctx.profile) hip_check(hipEventRecord(ctx.ev[3], sh), "event");
     hip_check(hipEventRecord(ctx.pipe_event(9), sh), "pipeline join");
     hip_check(hipStreamWaitEvent(ctx.stream, ctx.pipe_event(9), 0), "pipeline join");
@@ -302,8 +303,12 @@ HashPlan hash_plan(const Key& key, const Tuning& tu) {
   HashPlan hp;
   const bool mfma_ok = (two_n % 128 == 0) && m >= 2;
   hp.variant = tu.hash_variant == 0 ? (mfma_ok ? 1 : 2) : tu.hash_variant;
-  if (hp.variant == 1 && !mfma_ok) throw Error(1, "MFMA hash needs 2n % 128 == 0 and m >= 2");
   hp.impl = tu.hash_impl;
+  if (key.ring64) {  // the Z_2^64 hash runs on the reducing LDS-copy kernels only
+    hp.variant = 1;
+    if (!hash_impl_reduces(hp.impl)) hp.impl = Tuning{}.hash_impl;
+  }
+  if (hp.variant == 1 && !mfma_ok) throw Error(1, "MFMA hash needs 2n % 128 == 0 and m >= 2");
   if (hp.variant == 1) {
     const uint64_t ntiles = two_n / 128;
     hp.ks = tu.hash_ksplit ? (uint32_t)tu.hash_ksplit : pow2_at_least((512 + ntiles - 1) / ntiles);
@@ -337,7 +342,7 @@ void enqueue_commit_stages(Context& ctx, const Key& key, const CommitIO& io, con
   // H = A . B with B read bitwise from F' (ligesis.rs:292-302)
   if (hp.variant == 1) {
     hip_check(launch_hash_mfma(io.enc, two_n, (uint32_t)m, (uint32_t)two_n, key.dig.ptr, hp.ks, (int32_t*)io.part,
-                               io.h, st, profile ? ctx.ev[2] : nullptr, hp.impl),
+                               io.h, st, profile ? ctx.ev[2] : nullptr, hp.impl, key.ring64),
               "hash (MFMA)");
   } else {
     hip_check(launch_hash_valu(io.enc, two_n, (uint32_t)m, (uint32_t)two_n, key.at.ptr, hp.ks, io.part, io.h, st,
@@ -406,7 +411,7 @@ void enqueue_merkle_trees(Context& ctx, const std::vector<MerkleSource>& trees,
 // job tables in device memory. Results are identical to enqueue_commit per polynomial.
 bool group_eligible(const Key& key) {
   const uint64_t m = uint64_t(1) << key.log_m, two_n = uint64_t(2) << key.log_n;
-  return two_n % 128 == 0 && m >= 2 && key.log_n + 1 <= LGS_LDS_LOG;
+  return !key.ring64 && two_n % 128 == 0 && m >= 2 && key.log_n + 1 <= LGS_LDS_LOG;
 }
 
 void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, Commitment*>>& items,

@@ -58,12 +58,16 @@ __global__ __launch_bounds__(256) void hash_valu_kernel(const uint64_t* __restri
   }
 }
 
+// H = the sum of the K-split partials, mod p, or (ring) mod 2^64 for the Z_2^64 hash.
 __global__ void hash_valu_finalize_kernel(const uint64_t* __restrict__ part, uint32_t ksplit,
-                                          uint32_t ncols, uint64_t* __restrict__ h) {
+                                          uint32_t ncols, uint64_t* __restrict__ h, int ring) {
   uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (uint64_t)32 * ncols) return;
   uint64_t acc = 0;
-  for (uint32_t s = 0; s < ksplit; s++) acc = gl_add(acc, part[(uint64_t)s * 32 * ncols + idx]);
+  for (uint32_t s = 0; s < ksplit; s++) {
+    const uint64_t v = part[(uint64_t)s * 32 * ncols + idx];
+    acc = ring ? acc + v : gl_add(acc, v);
+  }
   h[idx] = acc;
 }
 
@@ -581,7 +585,7 @@ __device__ __forceinline__ void dma_step(const DmaTile& t, uint32_t sl, DmaSlot 
 
 // One 256 x 64 WN tile over slabs [split * ns, (split + 1) * ns) with a ring of NB slots (NB = 4
 // needs ns % 4 == 0).
-template <int NB, int WN>
+template <int NB, int WN, bool RING = false>
 __device__ __forceinline__ void hash_dma_body(const uint64_t* __restrict__ enc, uint64_t ld,
                                               const v4i* __restrict__ adig, uint32_t ns, uint32_t split,
                                               uint32_t ntile, uint32_t ncols, uint64_t* __restrict__ part,
@@ -648,7 +652,8 @@ __device__ __forceinline__ void hash_dma_body(const uint64_t* __restrict__ enc, 
         int64_t x = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++) x += (int64_t)acc[2 * k + h][nt][v] * ((int64_t)1 << (8 * k));
-        f[(h * 4 + nt) * 4 + v] = x >= 0 ? (uint64_t)x : GL_P - (uint64_t)(-x);
+        // mod p, or mod 2^64 for the Z_2^64 hash (two's complement wrap)
+        f[(h * 4 + nt) * 4 + v] = RING ? (uint64_t)x : x >= 0 ? (uint64_t)x : GL_P - (uint64_t)(-x);
       }
   if (t.wm == 1) {
 #pragma unroll
@@ -670,12 +675,13 @@ __device__ __forceinline__ void hash_dma_body(const uint64_t* __restrict__ enc, 
           const uint32_t r = 16 * h + 4 * t.g + v;
           const uint64_t hi = xch[r * 64 + nt * 16 + t.c];
           const uint64_t col = t.col0 + t.wn * 64 + nt * 16 + t.c;
-          part[((uint64_t)split * 32 + r) * ncols + col] = gl_add(f[(h * 4 + nt) * 4 + v], gl_mul(hi, uint64_t(1) << 32));
+          part[((uint64_t)split * 32 + r) * ncols + col] =
+              RING ? f[(h * 4 + nt) * 4 + v] + (hi << 32) : gl_add(f[(h * 4 + nt) * 4 + v], gl_mul(hi, uint64_t(1) << 32));
         }
   }
 }
 
-template <int NB, int WN>
+template <int NB, int WN, bool RING = false>
 __global__ __launch_bounds__(128 * WN, 2) void hash_mfma_dma_kernel(const uint64_t* __restrict__ enc, uint64_t ld,
                                                                    const v4i* __restrict__ adig, uint32_t ns,
                                                                    uint32_t ntiles, uint32_t ncols,
@@ -683,7 +689,7 @@ __global__ __launch_bounds__(128 * WN, 2) void hash_mfma_dma_kernel(const uint64
   __shared__ v4i la0[1024], la1[1024], la2[1024], la3[NB == 4 ? 1024 : 1];
   __shared__ uint32_t lb0[128 * WN], lb1[128 * WN], lb2[128 * WN], lb3[NB == 4 ? 128 * WN : 1];
   const uint32_t logical = xcd_logical_block();
-  hash_dma_body<NB, WN>(enc, ld, adig, ns, split0 + logical / ntiles, logical % ntiles, ncols, (uint64_t*)part,
+  hash_dma_body<NB, WN, RING>(enc, ld, adig, ns, split0 + logical / ntiles, logical % ntiles, ncols, (uint64_t*)part,
                         DmaSlot{la0, lb0}, DmaSlot{la1, lb1}, DmaSlot{la2, lb2}, DmaSlot{la3, lb3});
 }
 
@@ -737,7 +743,10 @@ __global__ void a_transpose_kernel(const uint64_t* __restrict__ a, uint64_t W, u
   at[i * 32 + r] = a[idx];
 }
 
-__global__ void a_digits_kernel(const uint64_t* __restrict__ a, uint64_t W, int8_t* __restrict__ dig) {
+// ring = 0: digits of A or A - p (both = A mod p); ring = 1 (the Z_2^64 hash): digits of the
+// representative of A mod 2^64 in [-0x8080808080808080, 0x7F7F7F7F7F7F7F7F] (the loop's 64-bit
+// arithmetic shift gives exactly that one).
+__global__ void a_digits_kernel(const uint64_t* __restrict__ a, uint64_t W, int8_t* __restrict__ dig, int ring) {
   // one thread per output byte position (slab, mt, lane, byte)
   uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= 256 * W) return;
@@ -751,7 +760,7 @@ __global__ void a_digits_kernel(const uint64_t* __restrict__ a, uint64_t W, int8
   const uint64_t i = 64 * slab + 32 * (g >> 1) + 4 * (g & 1) + s + 8 * t;
   uint64_t v = a[(uint64_t)r * W + i];
   // balanced base-256 digits of v or v - p
-  int64_t x = v <= 0x7F7F7F7F7F7F7F7Full ? (int64_t)v : -(int64_t)(GL_P - v);
+  int64_t x = (ring || v <= 0x7F7F7F7F7F7F7F7Full) ? (int64_t)v : -(int64_t)(GL_P - v);
   int d = 0;
   for (uint32_t q = 0; q <= l; q++) {
     d = (int)(x & 0xff);
@@ -761,13 +770,14 @@ __global__ void a_digits_kernel(const uint64_t* __restrict__ a, uint64_t W, int8
   dig[idx] = (int8_t)d;
 }
 
-hipError_t launch_a_layouts(const uint64_t* a, uint64_t W, uint64_t* at, int8_t* dig, hipStream_t st) {
+hipError_t launch_a_layouts(const uint64_t* a, uint64_t W, uint64_t* at, int8_t* dig, hipStream_t st, bool ring) {
   uint64_t n = 32 * W;
   hipLaunchKernelGGL(a_transpose_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, a, W, at);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || dig == nullptr) return e;
   uint64_t nd = 256 * W;
-  hipLaunchKernelGGL(a_digits_kernel, dim3((uint32_t)((nd + 255) / 256)), dim3(256), 0, st, a, W, dig);
+  hipLaunchKernelGGL(a_digits_kernel, dim3((uint32_t)((nd + 255) / 256)), dim3(256), 0, st, a, W, dig,
+                     ring ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -782,7 +792,7 @@ hipError_t launch_hash_valu(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
   if (mid && (e = hipEventRecord(mid, st)) != hipSuccess) return e;
   uint64_t n = (uint64_t)32 * ncols;
   hipLaunchKernelGGL(hash_valu_finalize_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
-                     part, ksplit, ncols, h);
+                     part, ksplit, ncols, h, 0);
   return hipGetLastError();
 }
 
@@ -809,10 +819,26 @@ static uint32_t full_cu_pad(const void* kernel) {
 
 hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
                                   const int8_t* dig, uint32_t ksplit, uint32_t split0, uint32_t nsplits,
-                                  int32_t* part, hipStream_t st, int impl) {
+                                  int32_t* part, hipStream_t st, int impl, bool ring) {
   const uint32_t ntiles = ncols / HM_NT;
   const uint32_t sps = m_rows / ksplit;
   const v4i* ad = (const v4i*)dig;
+  if (ring) {  // the Z_2^64 hash: the LDS-copy kernels that reduce in their epilogue only
+    if (!hash_impl_reduces(impl)) return hipErrorInvalidValue;
+    if (impl == 14 && ncols % 256 == 0 && sps % 4 == 0)
+      hipLaunchKernelGGL((hash_mfma_dma_kernel<4, 4, true>), dim3((ncols / 256) * nsplits), dim3(512),
+                         full_cu_pad((const void*)hash_mfma_dma_kernel<4, 4, true>), st, enc, ld, ad, sps, ncols / 256,
+                         ncols, part, split0);
+    else if (impl == 11 || sps % 4)
+      hipLaunchKernelGGL((hash_mfma_dma_kernel<3, 2, true>), dim3(ntiles * nsplits), dim3(256),
+                         full_cu_pad((const void*)hash_mfma_dma_kernel<3, 2, true>), st, enc, ld, ad, sps, ntiles,
+                         ncols, part, split0);
+    else
+      hipLaunchKernelGGL((hash_mfma_dma_kernel<4, 2, true>), dim3(ntiles * nsplits), dim3(256),
+                         full_cu_pad((const void*)hash_mfma_dma_kernel<4, 2, true>), st, enc, ld, ad, sps, ntiles,
+                         ncols, part, split0);
+    return hipGetLastError();
+  }
   if (impl == 4 || impl == 5) {
     const int wn = impl == 5 ? 4 : 2;
     if (sps % 2 == 0 && ncols % (64 * wn) == 0) {
@@ -830,8 +856,9 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
     hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 4, 2, 2>), dim3((ncols / 128) * nsplits), dim3(512), 0, st, enc, ld,
                        ad, sps, ncols / 128, ncols, part, split0);
   } else if (impl == 14 && ncols % 256 == 0 && sps % 4 == 0) {  // 8 waves sharing the A ring
-    hipLaunchKernelGGL((hash_mfma_dma_kernel<4, 4>), dim3((ncols / 256) * nsplits), dim3(512), full_cu_pad((const void*)hash_mfma_dma_kernel<4, 4>), st,
-                       enc, ld, ad, sps, ncols / 256, ncols, part, split0);
+    hipLaunchKernelGGL((hash_mfma_dma_kernel<4, 4>), dim3((ncols / 256) * nsplits), dim3(512),
+                       full_cu_pad((const void*)hash_mfma_dma_kernel<4, 4>), st, enc, ld, ad, sps, ncols / 256, ncols,
+                       part, split0);
   } else if (impl == 11 || impl == 12 || impl == 14) {  // asynchronous copies into an LDS ring of 3 / 4 slabs
     if (impl == 11 || sps % 4)
       hipLaunchKernelGGL((hash_mfma_dma_kernel<3, 2>), dim3(ntiles * nsplits), dim3(256),
@@ -872,11 +899,12 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
 bool hash_impl_reduces(int impl) { return impl == 11 || impl == 12 || impl == 14; }
 
 hipError_t launch_hash_mfma_finalize(const int32_t* part, uint32_t ksplit, uint32_t ncols, uint64_t* h,
-                                     hipStream_t st, int impl) {
+                                     hipStream_t st, int impl, bool ring) {
   const uint64_t n = (uint64_t)32 * ncols;
-  if (hash_impl_reduces(impl))  // one word per (split, row, column), already mod p
+  if (ring && !hash_impl_reduces(impl)) return hipErrorInvalidValue;
+  if (hash_impl_reduces(impl))  // one word per (split, row, column), already reduced
     hipLaunchKernelGGL(hash_valu_finalize_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
-                       (const uint64_t*)part, ksplit, ncols, h);
+                       (const uint64_t*)part, ksplit, ncols, h, ring ? 1 : 0);
   else
     hipLaunchKernelGGL(hash_mfma_finalize_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, part, ksplit,
                        ncols, h);
@@ -885,11 +913,11 @@ hipError_t launch_hash_mfma_finalize(const int32_t* part, uint32_t ksplit, uint3
 
 hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
                             const int8_t* dig, uint32_t ksplit, int32_t* part, uint64_t* h,
-                            hipStream_t st, hipEvent_t mid, int impl) {
-  hipError_t e = launch_hash_mfma_splits(enc, ld, m_rows, ncols, dig, ksplit, 0, ksplit, part, st, impl);
+                            hipStream_t st, hipEvent_t mid, int impl, bool ring) {
+  hipError_t e = launch_hash_mfma_splits(enc, ld, m_rows, ncols, dig, ksplit, 0, ksplit, part, st, impl, ring);
   if (e != hipSuccess) return e;
   if (mid && (e = hipEventRecord(mid, st)) != hipSuccess) return e;
-  return launch_hash_mfma_finalize(part, ksplit, ncols, h, st, impl);
+  return launch_hash_mfma_finalize(part, ksplit, ncols, h, st, impl, ring);
 }
 
 __global__ void hash_finalize_group_kernel(const FinalizeJob* __restrict__ jobs, int njobs) {

@@ -286,6 +286,8 @@ struct Key {
   DevBuf<uint64_t> at;   // W x 32 (VALU hash layout)
   DevBuf<int8_t> dig;    // MFMA digit layout
   bool has_state = false;
+  // the Z_2^64 subset-sum hash (commit only): H = A . B with sums mod 2^64, A any 64-bit words
+  bool ring64 = false;
   DevBasefold a_state;
   uint8_t com_a[32] = {0};
 };

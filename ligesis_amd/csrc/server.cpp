@@ -230,14 +230,14 @@ void Server::enqueue_request(Group& g, Request& r) {
                            ctx.stream, tu.ntt_impl, io.tp),
                 "row encode");
       hip_check(launch_hash_mfma_splits(c.enc.ptr, two_n, (uint32_t)m, (uint32_t)two_n, key.dig.ptr, hp.ks, q * kc, kc,
-                                        (int32_t*)io.part, ctx.stream, hp.impl),
+                                        (int32_t*)io.part, ctx.stream, hp.impl, key.ring64),
                 "hash (MFMA)");
     }
     if (chunks == 1) {
       enqueue_commit_stages(ctx, key, io, hp, tu, ctx.stream, false);
     } else {
       hip_check(launch_hash_mfma_finalize((const int32_t*)io.part, hp.ks, (uint32_t)two_n, c.h.ptr, ctx.stream,
-                                          hp.impl),
+                                          hp.impl, key.ring64),
                 "hash finalize");
       hip_check(launch_merkle(c.h.ptr, 32, two_n, 1, two_n, c.tree.nodes.ptr, ctx.stream, tu.merkle_lg), "merkle");
     }

@@ -14,7 +14,10 @@ proof (20 + len bytes)::
 
     b"LGSP" | u16 version | u8 log_m | u8 log_n | u64 len | u32 crc32(payload) | payload[len]
 
-All integers are little-endian. ``version`` is 1. The shape fixes every length inside the payload
+All integers are little-endian. ``version`` is 1 for the reference's scheme and 2 for a
+commitment made with the Z_2^64 subset-sum hash (``ring64`` keys, commit only): the same record,
+but a different hash, so a reader that knows only version 1 refuses it instead of treating the
+root as a version-1 commitment. The shape fixes every length inside a proof's payload
 (pcs/src/ligesis.rs:239-273), so a receiver checks it against its verifying key before parsing.
 The CRC guards against transport damage only; soundness is the verifier's.
 """
@@ -26,6 +29,7 @@ import zlib
 from . import LigesisError, LGS_ERR_ARG
 
 VERSION = 1
+VERSION_RING64 = 2
 COMMIT_MAGIC = b"LGSC"
 PROOF_MAGIC = b"LGSP"
 _COMMIT = struct.Struct("<4sHBB32s")
@@ -37,22 +41,25 @@ def _shape(log_m: int, log_n: int):
         raise LigesisError(LGS_ERR_ARG, f"wire: shape ({log_m}, {log_n}) does not fit the record")
 
 
-def encode_commitment(root: bytes, log_m: int, log_n: int) -> bytes:
+def encode_commitment(root: bytes, log_m: int, log_n: int, ring64: bool = False) -> bytes:
+    """``ring64``: the root of a Z_2^64-hash commitment (record version 2)."""
     _shape(log_m, log_n)
     if len(root) != 32:
         raise LigesisError(LGS_ERR_ARG, "wire: a commitment root is 32 bytes")
-    return _COMMIT.pack(COMMIT_MAGIC, VERSION, log_m, log_n, bytes(root))
+    return _COMMIT.pack(COMMIT_MAGIC, VERSION_RING64 if ring64 else VERSION, log_m, log_n, bytes(root))
 
 
-def decode_commitment(b: bytes):
-    """-> (root, log_m, log_n)"""
+def decode_commitment(b: bytes, ring64: bool = False):
+    """-> (root, log_m, log_n). The record's version must be the one of the expected scheme
+    (``ring64``: version 2, else 1)."""
     if len(b) != _COMMIT.size:
         raise LigesisError(LGS_ERR_ARG, f"wire: a commitment record is {_COMMIT.size} bytes, got {len(b)}")
     magic, ver, log_m, log_n, root = _COMMIT.unpack(b)
     if magic != COMMIT_MAGIC:
         raise LigesisError(LGS_ERR_ARG, "wire: not a commitment record")
-    if ver != VERSION:
-        raise LigesisError(LGS_ERR_ARG, f"wire: commitment record version {ver}, this reader knows {VERSION}")
+    want = VERSION_RING64 if ring64 else VERSION
+    if ver != want:
+        raise LigesisError(LGS_ERR_ARG, f"wire: commitment record version {ver}, expected {want}")
     return root, log_m, log_n
 
 

@@ -271,13 +271,14 @@ def basefold_batch_roundtrip(bf_polys, ef_polys, points, rounds, code_rate, fiel
     return bool(rc)
 
 
-def hash_columns(a, bits):
-    """a: (32, W) u64; bits: (ncols, W) u8 (column-major bit matrix). Returns H (32, ncols)."""
+def hash_columns(a, bits, ring64: bool = False):
+    """a: (32, W) u64; bits: (ncols, W) u8 (column-major bit matrix). Returns H (32, ncols), sums
+    mod p, or mod 2^64 with ``ring64``."""
     a = _u64(a)
     bits = np.ascontiguousarray(bits, dtype=np.uint8)
     ncols, W = bits.shape
     out = np.zeros((32, ncols), np.uint64)
-    lib().lgso_hash_columns(_p(a), ctypes.c_uint64(W), _p(bits), ctypes.c_uint64(ncols), _p(out))
+    lib().lgso_hash_columns(_p(a), ctypes.c_uint64(W), _p(bits), ctypes.c_uint64(ncols), _p(out), int(ring64))
     return out
 
 
@@ -287,10 +288,12 @@ def shape_ok(log_m: int, log_n: int) -> bool:
 
 
 class ProvingKey:
-    def __init__(self, log_m: int, log_n: int, a: np.ndarray, commit_a: bool = True):
+    def __init__(self, log_m: int, log_n: int, a: np.ndarray, commit_a: bool = True, ring64: bool = False):
+        """``ring64``: the Z_2^64 subset-sum hash (commit only; no A-bar commitment)."""
         self.a = _u64(a)
         self.log_m, self.log_n = log_m, log_n
-        self.ptr = lib().lgso_setup(log_m, log_n, _p(self.a), int(commit_a))
+        self.ring64 = ring64
+        self.ptr = lib().lgso_setup(log_m, log_n, _p(self.a), int(commit_a) | (2 if ring64 else 0))
         if not self.ptr:
             raise RuntimeError(err())
 

@@ -290,9 +290,10 @@ int64_t lgso_basefold_batch_roundtrip(const uint64_t* bf_nv, uint64_t nbf, const
 // ---- subset-sum hash ----
 // bits: ncols x width bytes (column-major, one byte per bit); a: C x width; out: C x ncols.
 void lgso_hash_columns(const uint64_t* a, uint64_t width, const uint8_t* bits, uint64_t ncols,
-                       uint64_t* out) {
+                       uint64_t* out, int ring64) {
   ProvingKey pk;
   pk.width = width;
+  pk.ring64 = ring64 != 0;
   for (size_t r = 0; r < C; r++) pk.a.emplace_back(a + r * width, a + (r + 1) * width);
   for (uint64_t j = 0; j < ncols; j++) {
     std::vector<uint8_t> col(bits + j * width, bits + (j + 1) * width);
@@ -304,9 +305,12 @@ void lgso_hash_columns(const uint64_t* a, uint64_t width, const uint8_t* bits, u
 // ---- LigeSIS ----
 int64_t lgso_shape_ok(uint64_t log_m, uint64_t log_n) { return shape_ok(log_m, log_n); }
 
-void* lgso_setup(uint64_t log_m, uint64_t log_n, const uint64_t* a, int commit_a) {
+// mode: bit 0 commit the A-bar Basefold state, bit 1 the Z_2^64 hash (then no A-bar state)
+void* lgso_setup(uint64_t log_m, uint64_t log_n, const uint64_t* a, int mode) {
   try {
-    return new ProvingKey(setup(log_m, log_n, a, commit_a != 0));
+    auto* pk = new ProvingKey(setup(log_m, log_n, a, (mode & 1) != 0 && (mode & 2) == 0));
+    pk->ring64 = (mode & 2) != 0;
+    return pk;
   } catch (const std::exception& e) {
     g_err = e.what();
     return nullptr;

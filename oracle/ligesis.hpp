@@ -10,6 +10,11 @@
 // (ligesis.rs:824-868), y == f(z) by direct evaluation, and the reference's debug-build
 // identities (ligesis.rs:461-471, 477, 492, 501), all checked in tests/test_oracle_ligesis.py.
 //
+// The Z_2^64 commitment (ring64 keys) is a paper-level variant the reference leaves out
+// (subset_sum.rs:10-12): the same commit with H = A . x summed mod 2^64 over any 64-bit A (its
+// words are serialized as they are); restated from that definition and pinned against big-integer
+// sums mod 2^64 in tests/test_oracle_subset_sum.py.
+//
 // Proof bytes: the reference has no serializer (only size_bytes, ligesis.rs:124-138), so the
 // canonical encoding is fixed here: struct fields in declaration order; u64 little-endian for
 // integers and lengths; BF as canonical u64; EF as (c0, c1); [u8;32] raw; Vec<T> = u64 length
@@ -47,6 +52,7 @@ struct ProvingKey {
   BfState<uint64_t> a_state;
   H32 com_a{};
   bool has_state = false;
+  bool ring64 = false;  // the Z_2^64 hash (commit only)
 };
 
 struct CommitData {
@@ -97,16 +103,26 @@ inline std::vector<uint64_t> hash_column(const ProvingKey& pk, const std::vector
   const uint64_t* rows[C];
   for (size_t r = 0; r < C; r++) rows[r] = pk.a[r].data();
   for (size_t i = 0; i < x.size(); i++)
-    if (x[i])
-      for (size_t r = 0; r < C; r++) out[r] = add_noncanon(out[r], rows[r][i]);
+    if (x[i]) {
+      if (pk.ring64)
+        for (size_t r = 0; r < C; r++) out[r] += rows[r][i];  // mod 2^64
+      else
+        for (size_t r = 0; r < C; r++) out[r] = add_noncanon(out[r], rows[r][i]);
+    }
   std::vector<uint64_t> res(C);
-  for (size_t r = 0; r < C; r++) res[r] = canon(out[r]);
+  for (size_t r = 0; r < C; r++) res[r] = pk.ring64 ? out[r] : canon(out[r]);
   return res;
 }
 
-inline std::vector<uint8_t> ser_h_col(const std::vector<std::vector<uint64_t>>& h, size_t j) {
+// A column of H as the leaf bytes: canonical field elements, or (Z_2^64 hash) the raw words.
+inline std::vector<uint8_t> ser_h_col(const std::vector<std::vector<uint64_t>>& h, size_t j, bool ring64 = false) {
   std::vector<uint8_t> b;
-  for (size_t r = 0; r < C; r++) ser_field(b, h[r][j]);
+  for (size_t r = 0; r < C; r++) {
+    if (ring64)
+      put_u64(b, h[r][j]);
+    else
+      ser_field(b, h[r][j]);
+  }
   return b;
 }
 
@@ -133,7 +149,7 @@ inline H32 commit(const ProvingKey& pk, std::vector<uint64_t> f, CommitData& cd,
     for (size_t r = 0; r < C; r++) cd.h[r][j] = d[r];
   }
   std::vector<H32> leaves(two_n);
-  for (size_t j = 0; j < two_n; j++) leaves[j] = blake3(ser_h_col(cd.h, j));
+  for (size_t j = 0; j < two_n; j++) leaves[j] = blake3(ser_h_col(cd.h, j, pk.ring64));
   cd.mt = MerkleTree::from_leaf_hashes(std::move(leaves));
   cd.f = std::move(f);
   return cd.mt.root();

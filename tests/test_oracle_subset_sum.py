@@ -88,3 +88,61 @@ def test_hash_is_linear_over_disjoint_supports():
     h = oracle.hash_columns(a, np.stack([x, y, x | y]))
     for r in range(C):
         assert (int(h[r, 0]) + int(h[r, 1])) % P == int(h[r, 2])
+
+
+# ---- the Z_2^64 hash (ring64): the same sums mod 2^64, over any 64-bit A ---------------------
+def _naive_ring(a: np.ndarray, bits: np.ndarray) -> list[list[int]]:
+    A = [[int(v) for v in row] for row in a]
+    out = [[0] * bits.shape[0] for _ in range(C)]
+    for j in range(bits.shape[0]):
+        sel = [i for i in range(bits.shape[1]) if bits[j, i]]
+        for r in range(C):
+            out[r][j] = sum(A[r][i] for i in sel) % 2**64
+    return out
+
+
+@pytest.mark.parametrize("width,seed", [(128, 11), (1024, 12)])
+def test_ring64_hash_matches_big_integer_sums(width, seed):
+    """Words at and above p (never canonical field elements), 2^64 - 1, 2^63: the sums wrap mod
+    2^64 where the field hash would reduce mod p, so the two differ on such rows."""
+    a = synth.splitmix64(seed, C * width).reshape(C, width)
+    a[0, :] = 2**64 - 1
+    a[1, :] = P
+    a[2, :] = 2**63
+    a[3, :] = 0x7F7F7F7F7F7F7F80  # the balanced-digit boundary of the MFMA kernels
+    a[4, :] = 0x8080808080808080
+    bits = (synth.splitmix64(seed + 1, 6 * width) & 1).astype(np.uint8).reshape(6, width)
+    bits[0, :] = 1
+    got = oracle.hash_columns(a, bits, ring64=True)
+    want = _naive_ring(a, bits)
+    for r in range(C):
+        for j in range(bits.shape[0]):
+            assert int(got[r, j]) == want[r][j], (r, j)
+    assert int(got[0, 0]) == (width * (2**64 - 1)) % 2**64
+    field = oracle.hash_columns(a % P, bits)
+    assert any(int(field[r, 0]) != int(got[r, 0]) for r in range(C))
+
+
+def test_ring64_hash_is_linear_mod_2_64():
+    width = 256
+    a = synth.splitmix64(21, C * width).reshape(C, width)
+    x = (synth.splitmix64(22, width) & 1).astype(np.uint8)
+    y = (synth.splitmix64(23, width) & 1).astype(np.uint8) * (1 - x)  # disjoint support
+    hx, hy, hxy = (oracle.hash_columns(a, b[None, :], ring64=True)[:, 0] for b in (x, y, x + y))
+    for r in range(C):
+        assert (int(hx[r]) + int(hy[r])) % 2**64 == int(hxy[r])
+
+
+def test_ring64_commit_root_is_tree_over_raw_words():
+    """The ring64 commit hashes each H column's 32 raw words (not reduced mod p) into its leaf."""
+    lm, ln = 2, 6
+    inp = synth.Inputs(lm, ln, seed=31, n_fields=4, n_ints=4)
+    a = synth.splitmix64(32, C * (64 << lm))
+    pk = oracle.ProvingKey(lm, ln, a, commit_a=False, ring64=True)
+    cd = oracle.CommitData(pk, inp.f, keep_enc=True)
+    enc = cd.enc()
+    bits = np.array([[(int(enc[i, j]) >> k) & 1 for i in range(1 << lm) for k in range(64)]
+                     for j in range(2 << ln)], np.uint8)
+    np.testing.assert_array_equal(cd.h(), oracle.hash_columns(a.reshape(C, -1), bits, ring64=True))
+    cols = np.frombuffer(np.ascontiguousarray(cd.h().T).astype("<u8").tobytes(), np.uint8).reshape(2 << ln, 256)
+    assert cd.root == oracle.merkle_root(cols)  # leaf j = blake3 of column j's 32 raw words

@@ -64,3 +64,14 @@ def test_encoders_reject_what_does_not_fit():
         wire.encode_commitment(bytes(31), 3, 10)
     with pytest.raises(LigesisError):
         wire.encode_proof(b"", 256, 10)
+
+
+def test_ring64_commitment_record_is_version_2_and_refused_as_version_1():
+    root = bytes(range(32, 64))
+    rec = wire.encode_commitment(root, 4, 9, ring64=True)
+    assert rec == b"LGSC" + b"\x02\x00" + b"\x04\x09" + root
+    assert wire.decode_commitment(rec, ring64=True) == (root, 4, 9)
+    with pytest.raises(LigesisError):
+        wire.decode_commitment(rec)
+    with pytest.raises(LigesisError):
+        wire.decode_commitment(wire.encode_commitment(root, 4, 9), ring64=True)
