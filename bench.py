@@ -705,7 +705,8 @@ def bench_commit(args):
                     7: "hash_mfma_tile_kernel<4, 4, 4, 1>", 8: "hash_mfma_tile_kernel<4, 8, 1, 2>",
                     9: "hash_mfma_tile_kernel<4, 8, 2, 1>",
                     10: "hash_mfma_pipe_kernel<2, 2, true>", 11: "hash_mfma_dma_kernel<3, 2>",
-                    12: "hash_mfma_dma_kernel<4, 2>", 14: "hash_mfma_dma_kernel<4, 4>"}[schedule["hash_impl"]])
+                    12: "hash_mfma_dma_kernel<4, 2>", 14: "hash_mfma_dma_kernel<4, 4>",
+                    15: "hash_mfma_pipe8_kernel<4>", 16: "hash_mfma_pipe8_kernel<8>"}[schedule["hash_impl"]])
     hash_tag = f"hash_impl={schedule['hash_impl']} ksplit={schedule['hash_ksplit']} pipeline={schedule['pipeline']}"
     ntt_tag = f"ntt_impl={schedule['ntt_impl']}"
     ntt_kernel = "ntt_r8_fixed_kernel<13>" if schedule["ntt_impl"] == 4 else "ntt_r8_kernel"

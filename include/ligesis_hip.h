@@ -70,7 +70,10 @@ const char* lgs_last_error(const lgs_ctx* ctx);
  * 4 with 256 x 256 tiles, 6 / 7 64 x 64 wave tiles in 256 x 128 / 256 x 256 workgroup tiles,
  * 8 / 9 64 x 128 wave tiles in 256 x 128 / 256 x 256 workgroup tiles, 10 as 0 with each A
  * fragment read from LDS one product group ahead, 11 / 12 A digits and F' words copied straight
- * into an LDS ring of 3 / 4 slabs),
+ * into an LDS ring of 3 / 4 slabs, 14 as 12 with 8 waves and 256 x 256 tiles sharing one ring,
+ * 15 / 16 as 14 with a 4 / 8-slab ring, each slab's bit expansion, A reads and copies
+ * interleaved with the previous slab's products; shapes a kernel does not fit take the next
+ * one down),
  * "ntt_impl" (-1 auto; 0/2/3 radix-8 passes with 256/512/1024 threads per row, 1 radix-2
  * stages, 4 for rows of 2^12 values a 2^13-point kernel with every size fixed at compile time
  * and per-pass twiddle tables, elsewhere as 3), "merkle_lg" (0 auto; 6..8 = log2 nodes per Merkle workgroup = tree levels per

@@ -515,6 +515,27 @@ struct DmaTile {
   uint32_t ns, tid, lane, wave, wm, wn, g, c, ksh;
 };
 
+template <int WN>
+__device__ __forceinline__ DmaTile make_dma_tile(const uint64_t* enc, uint64_t ld, const v4i* adig, uint32_t ns,
+                                                 uint32_t split, uint32_t ntile) {
+  DmaTile t;
+  t.enc = enc;
+  t.ld = ld;
+  t.adig = adig;
+  t.ns = ns;
+  t.tid = threadIdx.x;
+  t.lane = t.tid & 63;
+  t.wave = t.tid >> 6;
+  t.wm = t.wave / WN;
+  t.wn = t.wave % WN;
+  t.g = t.lane >> 4;
+  t.c = t.lane & 15;
+  t.ksh = 4 * (t.g & 1);
+  t.col0 = (uint64_t)ntile * (64 * WN);
+  t.slab0 = (uint64_t)split * ns;
+  return t;
+}
+
 // Copies per wave and slab: 16 KiB of A in 1 KiB pieces over the 2 WN waves, plus one of F'.
 template <int WN>
 constexpr int dma_copies() { return 16 / (2 * WN) + 1; }
@@ -583,64 +604,19 @@ __device__ __forceinline__ void dma_step(const DmaTile& t, uint32_t sl, DmaSlot 
   __builtin_amdgcn_s_setprio(0);
 }
 
-// One 256 x 64 WN tile over slabs [split * ns, (split + 1) * ns) with a ring of NB slots (NB = 4
-// needs ns % 4 == 0).
-template <int NB, int WN, bool RING = false>
-__device__ __forceinline__ void hash_dma_body(const uint64_t* __restrict__ enc, uint64_t ld,
-                                              const v4i* __restrict__ adig, uint32_t ns, uint32_t split,
-                                              uint32_t ntile, uint32_t ncols, uint64_t* __restrict__ part,
-                                              DmaSlot s0, DmaSlot s1, DmaSlot s2, DmaSlot s3) {
-  static_assert(NB == 3 || NB == 4, "ring of 3 or 4 slabs");
-  static_assert(WN == 2 || WN == 4, "2 or 4 waves along N");
-  // the epilogue exchanges through slot wn's A array: with WN = 4 all four slots hold 16 KiB,
-  // also for a 3-slot ring
-  DmaTile t;
-  t.enc = enc;
-  t.ld = ld;
-  t.adig = adig;
-  t.ns = ns;
-  t.tid = threadIdx.x;
-  t.lane = t.tid & 63;
-  t.wave = t.tid >> 6;
-  t.wm = t.wave / WN;
-  t.wn = t.wave % WN;
-  t.g = t.lane >> 4;
-  t.c = t.lane & 15;
-  t.ksh = 4 * (t.g & 1);
-  t.col0 = (uint64_t)ntile * (64 * WN);
-  t.slab0 = (uint64_t)split * ns;
-
-  v4i acc[8][4];
-#pragma unroll
-  for (int a = 0; a < 8; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) acc[a][b] = v4i{0, 0, 0, 0};
-
-  dma_copy_slab<WN>(t, 0, s0);
-  dma_copy_slab<WN>(t, 1 < ns ? 1 : ns - 1, s1);
-  if (NB == 4) dma_copy_slab<WN>(t, 2 < ns ? 2 : ns - 1, s2);
-  for (uint32_t sl = 0; sl < ns; sl += NB) {
-    if (NB == 3) {
-      dma_step<3, WN>(t, sl, s0, s2, acc);
-      if (sl + 1 < ns) dma_step<3, WN>(t, sl + 1, s1, s0, acc);
-      if (sl + 2 < ns) dma_step<3, WN>(t, sl + 2, s2, s1, acc);
-    } else {  // one straight-line body
-      dma_step<4, WN>(t, sl, s0, s3, acc);
-      dma_step<4, WN>(t, sl + 1, s1, s0, acc);
-      dma_step<4, WN>(t, sl + 2, s2, s1, acc);
-      dma_step<4, WN>(t, sl + 3, s3, s2, acc);
-    }
-  }
-  // Epilogue: the split's contribution to H, reduced mod p. Row m = 32 l + r of the tile is digit
-  // plane l of hash row r; a lane holds, for r = 16 h + 4 g + v and column nt * 16 + c, the
-  // sums of planes l = 4 wm + k (k < 4) in acc[2 k + h][nt][v]. Each wave folds its four planes
-  // exactly in 64 bits (|sum_k 256^k S| < 2^55); the upper planes' fold (wm = 1) moves through
-  // LDS (ring slot wn, free once every copy and read has finished) to the lower planes' wave,
-  // which adds 2^32 times it mod p and writes one word per (split, row, column).
+// Epilogue of a 256 x 64 WN tile: the split's contribution to H, reduced mod p. Row m = 32 l + r of
+// the tile is digit plane l of hash row r; a lane holds, for r = 16 h + 4 g + v and column
+// nt * 16 + c, the sums of planes l = 4 wm + k (k < 4) in acc[2 k + h][nt][v]. Each wave folds its
+// four planes exactly in 64 bits (|sum_k 256^k S| < 2^55); the upper planes' fold (wm = 1) moves
+// through LDS (xch: 32 x 64 words of a ring slot, free once every copy and read has finished) to
+// the lower planes' wave, which adds 2^32 times it mod p and writes one word per (split, row,
+// column).
+template <bool RING>
+__device__ __forceinline__ void dma_epilogue(const DmaTile& t, v4i (&acc)[8][4], uint64_t* xch, uint32_t split,
+                                             uint32_t ncols, uint64_t* __restrict__ part) {
   __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // no copy into LDS outlives the workgroup
   __builtin_amdgcn_s_waitcnt(kLgkmZero);
   asm volatile("s_barrier" ::: "memory");
-  uint64_t* xch = (uint64_t*)(t.wn == 0 ? s0.a : t.wn == 1 ? s1.a : t.wn == 2 ? s2.a : s3.a);  // 32 x 64
   // fold first, so the accumulators die here: f[(h * 4 + nt) * 4 + v]
   uint64_t f[32];
 #pragma unroll
@@ -681,6 +657,44 @@ __device__ __forceinline__ void hash_dma_body(const uint64_t* __restrict__ enc, 
   }
 }
 
+// One 256 x 64 WN tile over slabs [split * ns, (split + 1) * ns) with a ring of NB slots (NB = 4
+// needs ns % 4 == 0).
+template <int NB, int WN, bool RING = false>
+__device__ __forceinline__ void hash_dma_body(const uint64_t* __restrict__ enc, uint64_t ld,
+                                              const v4i* __restrict__ adig, uint32_t ns, uint32_t split,
+                                              uint32_t ntile, uint32_t ncols, uint64_t* __restrict__ part,
+                                              DmaSlot s0, DmaSlot s1, DmaSlot s2, DmaSlot s3) {
+  static_assert(NB == 3 || NB == 4, "ring of 3 or 4 slabs");
+  static_assert(WN == 2 || WN == 4, "2 or 4 waves along N");
+  // the epilogue exchanges through slot wn's A array: with WN = 4 all four slots hold 16 KiB,
+  // also for a 3-slot ring
+  const DmaTile t = make_dma_tile<WN>(enc, ld, adig, ns, split, ntile);
+
+  v4i acc[8][4];
+#pragma unroll
+  for (int a = 0; a < 8; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = v4i{0, 0, 0, 0};
+
+  dma_copy_slab<WN>(t, 0, s0);
+  dma_copy_slab<WN>(t, 1 < ns ? 1 : ns - 1, s1);
+  if (NB == 4) dma_copy_slab<WN>(t, 2 < ns ? 2 : ns - 1, s2);
+  for (uint32_t sl = 0; sl < ns; sl += NB) {
+    if (NB == 3) {
+      dma_step<3, WN>(t, sl, s0, s2, acc);
+      if (sl + 1 < ns) dma_step<3, WN>(t, sl + 1, s1, s0, acc);
+      if (sl + 2 < ns) dma_step<3, WN>(t, sl + 2, s2, s1, acc);
+    } else {  // one straight-line body
+      dma_step<4, WN>(t, sl, s0, s3, acc);
+      dma_step<4, WN>(t, sl + 1, s1, s0, acc);
+      dma_step<4, WN>(t, sl + 2, s2, s1, acc);
+      dma_step<4, WN>(t, sl + 3, s3, s2, acc);
+    }
+  }
+  dma_epilogue<RING>(t, acc, (uint64_t*)(t.wn == 0 ? s0.a : t.wn == 1 ? s1.a : t.wn == 2 ? s2.a : s3.a), split, ncols,
+                     part);
+}
+
 template <int NB, int WN, bool RING = false>
 __global__ __launch_bounds__(128 * WN, 2) void hash_mfma_dma_kernel(const uint64_t* __restrict__ enc, uint64_t ld,
                                                                    const v4i* __restrict__ adig, uint32_t ns,
@@ -719,6 +733,126 @@ __global__ __launch_bounds__(128 * WN, 2) void hash_mfma_dma_group_kernel(const 
 }
 
 // H[r][j] = sum_l 256^l (sum_split part[split][32 l + r][j]) mod p.
+// Software-pipelined variant of the 8-wave tile (256 x 256, WN = 4): the F' bits of slab s + 1 are
+// expanded, and its first A pair read, while slab s's 32 products issue, one expansion instruction
+// after each product, so the matrix pipe is fed without a gap for the expansion or the first LDS
+// reads. The barrier before slab s waits for slab s + 1's copies (vmcnt <= copies (NB - 3)): slab
+// s + 1 is then visible to every wave, and every wave has finished slab s - 1, whose slot takes the
+// copies of slab s + NB - 1. Those copies issue one at a time between slab s's products, so no
+// wave waits behind a burst of copies before its products. ns % NB == 0.
+__device__ __forceinline__ v4i expand_bits(uint32_t y) {
+  return v4i{(int)(y & 0x01010101u), (int)((y >> 1) & 0x01010101u), (int)((y >> 2) & 0x01010101u),
+             (int)((y >> 3) & 0x01010101u)};
+}
+
+// Slab sl (slot cur, B operands bc and A pair 0 in p0, p1); the next slab's B operands and A pair 0
+// (slot nxt) come back in bc, p0, p1; slab sl + NB - 1 (clamped) is copied into slot dst.
+template <int NB>
+__device__ __forceinline__ void dma_pipe_slab(const DmaTile& t, uint32_t sl, DmaSlot cur, DmaSlot nxt, DmaSlot dst,
+                                              v4i (&bc)[4], v4i& p0, v4i& p1, v4i (&acc)[8][4]) {
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(dma_copies<4>() * (NB - 3)));
+  asm volatile("s_barrier" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
+  const uint32_t nx = sl + NB - 1;
+  dma_copy_slab<4>(t, nx < t.ns ? nx : t.ns - 1, dst);
+  uint32_t y[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) y[nt] = nxt.b[2 * (t.wn * 64 + nt * 16 + t.c) + (t.g >> 1)];
+  const v4i* A = cur.a + (t.wm * 8) * 64 + t.lane;
+  const v4i* An = nxt.a + (t.wm * 8) * 64 + t.lane;
+  v4i a0 = p0, a1 = p1, bn[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    v4i n0, n1;
+    if (k < 3) {
+      n0 = A[(2 * k + 2) * 64];
+      n1 = A[(2 * k + 3) * 64];
+    } else {
+      n0 = An[0];
+      n1 = An[64];
+    }
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+      acc[2 * k][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0, bc[nt], acc[2 * k][nt], 0, 0, 0);
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+      acc[2 * k + 1][nt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a1, bc[nt], acc[2 * k + 1][nt], 0, 0, 0);
+    bn[k] = expand_bits(y[k] >> t.ksh);
+    a0 = n0;
+    a1 = n1;
+  }
+  p0 = a0;
+  p1 = a1;
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) bc[nt] = bn[nt];
+  // issue order: B words and A pair 1; four products; then one expansion instruction after each
+  // product. A pair k + 1 is read right after pair k's first product (products 8, 16, 24 for
+  // pairs 2, 3 and the next slab's pair 0): an LDS wait here covers every read in flight, so
+  // a read issued just before the wait for the previous pair would be waited for at once. The
+  // three copies go after products 5, 13, 21.
+  __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+  __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+#pragma unroll
+  for (int i = 4; i < 32; i++) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    if (i % 8 == 0) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+    if (i % 8 == 5) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  // the next slab's pair 0 has had eight products to arrive; waiting here (rather than at the
+  // next slab's first product, behind that slab's new reads) keeps the products back to back
+  __builtin_amdgcn_s_waitcnt(kLgkmZero);
+}
+
+template <int NB, bool RING = false>
+__global__ __launch_bounds__(512, 2) void hash_mfma_pipe8_kernel(const uint64_t* __restrict__ enc, uint64_t ld,
+                                                                const v4i* __restrict__ adig, uint32_t ns,
+                                                                uint32_t ntiles, uint32_t ncols,
+                                                                int32_t* __restrict__ part, uint32_t split0) {
+  static_assert(NB >= 4 && NB <= 8, "4 to 8 slots");
+  __shared__ v4i la0[1024], la1[1024], la2[1024], la3[1024], la4[NB > 4 ? 1024 : 1], la5[NB > 5 ? 1024 : 1],
+      la6[NB > 6 ? 1024 : 1], la7[NB > 7 ? 1024 : 1];
+  __shared__ uint32_t lb0[512], lb1[512], lb2[512], lb3[512], lb4[NB > 4 ? 512 : 1], lb5[NB > 5 ? 512 : 1],
+      lb6[NB > 6 ? 512 : 1], lb7[NB > 7 ? 512 : 1];
+  DmaSlot s[8];
+  s[0] = DmaSlot{la0, lb0};
+  s[1] = DmaSlot{la1, lb1};
+  s[2] = DmaSlot{la2, lb2};
+  s[3] = DmaSlot{la3, lb3};
+  s[4] = DmaSlot{la4, lb4};
+  s[5] = DmaSlot{la5, lb5};
+  s[6] = DmaSlot{la6, lb6};
+  s[7] = DmaSlot{la7, lb7};
+  const uint32_t logical = xcd_logical_block();
+  const uint32_t split = split0 + logical / ntiles;
+  const DmaTile t = make_dma_tile<4>(enc, ld, adig, ns, split, logical % ntiles);
+
+  v4i acc[8][4];
+#pragma unroll
+  for (int a = 0; a < 8; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = v4i{0, 0, 0, 0};
+
+  // slabs 0 .. NB - 2 in flight; wait for slab 0
+#pragma unroll
+  for (int j = 0; j < NB - 1; j++) dma_copy_slab<4>(t, (uint32_t)j < ns ? j : ns - 1, s[j]);
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(dma_copies<4>() * (NB - 2)));
+  asm volatile("s_barrier" ::: "memory");
+  v4i bc[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; nt++) bc[nt] = expand_bits(s[0].b[2 * (t.wn * 64 + nt * 16 + t.c) + (t.g >> 1)] >> t.ksh);
+  v4i p0 = s[0].a[(t.wm * 8) * 64 + t.lane], p1 = s[0].a[(t.wm * 8 + 1) * 64 + t.lane];
+  __builtin_amdgcn_s_waitcnt(kLgkmZero);
+  for (uint32_t sl = 0; sl < ns; sl += NB) {
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+      dma_pipe_slab<NB>(t, sl + j, s[j], s[(j + 1) % NB], s[(j + NB - 1) % NB], bc, p0, p1, acc);
+  }
+  dma_epilogue<RING>(t, acc, (uint64_t*)(t.wn == 0 ? la0 : t.wn == 1 ? la1 : t.wn == 2 ? la2 : la3), split, ncols,
+                    (uint64_t*)part);
+}
+
 __global__ void hash_mfma_finalize_kernel(const int32_t* __restrict__ part, uint32_t ksplit,
                                           uint32_t ncols, uint64_t* __restrict__ h) {
   uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -825,7 +959,15 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
   const v4i* ad = (const v4i*)dig;
   if (ring) {  // the Z_2^64 hash: the LDS-copy kernels that reduce in their epilogue only
     if (!hash_impl_reduces(impl)) return hipErrorInvalidValue;
-    if (impl == 14 && ncols % 256 == 0 && sps % 4 == 0)
+    if (impl >= 16 && ncols % 256 == 0 && sps % 8 == 0)
+      hipLaunchKernelGGL((hash_mfma_pipe8_kernel<8, true>), dim3((ncols / 256) * nsplits), dim3(512),
+                         full_cu_pad((const void*)hash_mfma_pipe8_kernel<8, true>), st, enc, ld, ad, sps, ncols / 256,
+                         ncols, part, split0);
+    else if (impl >= 15 && ncols % 256 == 0 && sps % 4 == 0)
+      hipLaunchKernelGGL((hash_mfma_pipe8_kernel<4, true>), dim3((ncols / 256) * nsplits), dim3(512),
+                         full_cu_pad((const void*)hash_mfma_pipe8_kernel<4, true>), st, enc, ld, ad, sps, ncols / 256,
+                         ncols, part, split0);
+    else if (impl >= 14 && ncols % 256 == 0 && sps % 4 == 0)
       hipLaunchKernelGGL((hash_mfma_dma_kernel<4, 4, true>), dim3((ncols / 256) * nsplits), dim3(512),
                          full_cu_pad((const void*)hash_mfma_dma_kernel<4, 4, true>), st, enc, ld, ad, sps, ncols / 256,
                          ncols, part, split0);
@@ -855,11 +997,19 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
   } else if (impl == 6 && ncols % 128 == 0) {  // 64 x 64 wave tiles, 8 waves, 256 x 128 workgroup tile
     hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 4, 2, 2>), dim3((ncols / 128) * nsplits), dim3(512), 0, st, enc, ld,
                        ad, sps, ncols / 128, ncols, part, split0);
-  } else if (impl == 14 && ncols % 256 == 0 && sps % 4 == 0) {  // 8 waves sharing the A ring
+  } else if (impl >= 16 && ncols % 256 == 0 && sps % 8 == 0) {  // pipelined 8-wave tile, 8 slots
+    hipLaunchKernelGGL((hash_mfma_pipe8_kernel<8>), dim3((ncols / 256) * nsplits), dim3(512),
+                       full_cu_pad((const void*)hash_mfma_pipe8_kernel<8>), st, enc, ld, ad, sps, ncols / 256, ncols,
+                       part, split0);
+  } else if (impl >= 15 && ncols % 256 == 0 && sps % 4 == 0) {  // pipelined, 4 slots
+    hipLaunchKernelGGL((hash_mfma_pipe8_kernel<4>), dim3((ncols / 256) * nsplits), dim3(512),
+                       full_cu_pad((const void*)hash_mfma_pipe8_kernel<4>), st, enc, ld, ad, sps, ncols / 256, ncols,
+                       part, split0);
+  } else if (impl >= 14 && ncols % 256 == 0 && sps % 4 == 0) {  // 8 waves sharing the A ring
     hipLaunchKernelGGL((hash_mfma_dma_kernel<4, 4>), dim3((ncols / 256) * nsplits), dim3(512),
                        full_cu_pad((const void*)hash_mfma_dma_kernel<4, 4>), st, enc, ld, ad, sps, ncols / 256, ncols,
                        part, split0);
-  } else if (impl == 11 || impl == 12 || impl == 14) {  // asynchronous copies into an LDS ring of 3 / 4 slabs
+  } else if (hash_impl_reduces(impl)) {  // asynchronous copies into an LDS ring of 3 / 4 slabs
     if (impl == 11 || sps % 4)
       hipLaunchKernelGGL((hash_mfma_dma_kernel<3, 2>), dim3(ntiles * nsplits), dim3(256),
                          full_cu_pad((const void*)hash_mfma_dma_kernel<3, 2>), st, enc,
@@ -896,7 +1046,7 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
   return hipGetLastError();
 }
 
-bool hash_impl_reduces(int impl) { return impl == 11 || impl == 12 || impl == 14; }
+bool hash_impl_reduces(int impl) { return impl == 11 || impl == 12 || (impl >= 14 && impl <= 16); }
 
 hipError_t launch_hash_mfma_finalize(const int32_t* part, uint32_t ksplit, uint32_t ncols, uint64_t* h,
                                      hipStream_t st, int impl, bool ring) {

@@ -156,7 +156,8 @@ struct DevTree {
 // Launch schedule of one commit shape. Every choice yields the same bytes; only speed differs.
 struct Tuning {
   int hash_variant = 0;  // 1 MFMA, 2 VALU (0: MFMA where eligible)
-  int hash_impl = 14;    // MFMA kernel variant (14: 8 waves sharing a 4-slab LDS ring of copies)
+  int hash_impl = 16;    // MFMA kernel variant (16: 8 waves sharing an 8-slab LDS ring, copies and bit
+                         // expansion interleaved with the products)
   int hash_ksplit = 0;   // slabs of the K (= 64 m) dimension; 0: heuristic
   int ntt_impl = 0;      // row transform variant
   int merkle_lg = 7;     // log2 nodes per Merkle workgroup (four threads per node)
@@ -335,7 +336,7 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
 // The hash kernel of a commit: variant (1 MFMA, 2 VALU), MFMA kernel and K-split.
 struct HashPlan {
   int variant = 1;
-  int impl = 14;
+  int impl = 16;
   uint32_t ks = 1;
 };
 HashPlan hash_plan(const Key& key, const Tuning& tu);

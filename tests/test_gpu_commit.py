@@ -132,7 +132,7 @@ def test_every_mfma_kernel_matches_oracle(gpu_ctx, shape):
     want = ocd.h()
     gpu_ctx.set_option("hash", 1)
     try:
-        for impl in list(range(13)) + [14]:
+        for impl in list(range(13)) + [14, 15, 16]:
             for ks in (0, 1, 2, 4):
                 if ks > (1 << log_m) // 2:
                     continue
