@@ -6,7 +6,9 @@ Goldilocks evaluations and A = 32 x (64 * 2^12) uniform Goldilocks from a seed: 
 full commit with f already resident in HBM, ending with the 32-byte root on the host.
 
 Prints one JSON line: value = 2^24 / (seconds per commit) / 1e6, the roofline of the dominant
-kernel (timed with HIP events on the library's stream inside the timed region), and the CPU
+kernel (timed with HIP events on the library's stream inside the timed region, which records
+only the two events around the hash kernel; the per-stage times in stages_ms come from a
+separate run of as many commits with every stage's events), and the CPU
 baseline (the oracle's single-threaded commit on a bounded sample of the same workload). Beside
 the headline, outside its timed region, the same line carries the rest of the metric, measured
 in a child process (bench_extras): the same commit from host memory, the mu = 24 eval-prove time
@@ -668,15 +670,29 @@ def bench_commit(args):
         cd.release()
         return r
 
-    if tuned is None:  # the autotuner's own warm-up did this otherwise
-        settle(one_commit)
-    ctx.set_option("profile", 1)
+    # also after the autotuner: the commits right after it ran ~5 % slower on every kernel (the
+    # device leaving the power state of its sweep, which includes the slow VALU hash)
+    settle(one_commit)
+    # timed region: HIP events around the hash kernel only (profile level 2); the other stages
+    # are timed afterwards in a separate run of as many commits with every stage's events
+    ctx.set_option("profile", 2)
     roots = set()
     for _ in range(args.warmup):
         roots.add(one_commit())
-    stages = {"ntt_ms": [], "hash_ms": [], "hash_finalize_ms": [], "merkle_ms": [], "commit_ms": []}
+    hash_ms = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r, cd = L.Ligesis.commit(pk, f_dev, device=True)
+        hash_ms.append(ctx.stage_times()["hash_ms"])
+        roots.add(r)
+        cd.release()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt / args.steps * 1e3
+    value = (1 << MU) / (ms / 1e3) / 1e6
+    ctx.set_option("profile", 1)
+    stages = {"ntt_ms": [], "hash_ms": [], "hash_finalize_ms": [], "merkle_ms": [], "commit_ms": []}
     for _ in range(args.steps):
         r, cd = L.Ligesis.commit(pk, f_dev, device=True)
         st = ctx.stage_times()
@@ -684,12 +700,11 @@ def bench_commit(args):
             stages[k].append(st[k])
         roots.add(r)
         cd.release()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    ctx.set_option("profile", 0)
     assert len(roots) == 1, "commit is not deterministic"
-    ms = dt / args.steps * 1e3
-    value = (1 << MU) / (ms / 1e3) / 1e6
     avg = {k: float(np.mean(v)) for k, v in stages.items()}
+    avg["hash_ms_profiled_run"] = avg["hash_ms"]
+    avg["hash_ms"] = float(np.mean(hash_ms))  # the timed region's
 
     m, n = 1 << LOG_M, 1 << LOG_N
     # algorithmic work per launch: hash = 2 * 256 * (64 m) * (2 n) int8 ops (2^40 at m = n = 2^12);

@@ -82,7 +82,9 @@ const char* lgs_last_error(const lgs_ctx* ctx);
  * "prove_graph" (0 default; 1: the second proof of a commitment on its own context captures
  * its launches into a HIP graph, kept with the commitment, and later proofs replay it with their
  * own constants), "profile" (1: time every stage with HIP events; pipelined stages report the
- * summed durations of their launches; proofs are then issued without a graph). An option left
+ * summed durations of their launches; proofs are then issued without a graph; 2: as 1, except
+ * that a plain commit records only the two events around its hash kernel and reports stage [1]
+ * alone). An option left
  * at auto takes the autotuned schedule of the commit's shape when the process tuning table has
  * one (lgs_autotune, lgs_tune_load), else a built-in default. */
 int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value);

@@ -39,6 +39,12 @@ int guarded(lgs::Context* ctx, F&& f) {
 
 void finish_profile(lgs::Context& ctx) {
   if (!ctx.profile) return;
+  if (ctx.profile == 2 && ctx.prof_chunks == 0) {  // the hash kernel only
+    float ms = 0;
+    lgs::hip_check(hipEventElapsedTime(&ms, ctx.ev[1], ctx.ev[2]), "event time");
+    for (int i = 0; i < 5; i++) ctx.stage_ms[i] = i == 1 ? ms : 0;
+    return;
+  }
   for (int i = 0; i < 4; i++) {
     float ms = 0;
     lgs::hip_check(hipEventElapsedTime(&ms, ctx.ev[i], ctx.ev[i + 1]), "event time");
@@ -152,8 +158,8 @@ int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value) {
     ctx->c.pipeline = (int)value;
   } else if (n == "prove_graph" && (value == 0 || value == 1)) {
     ctx->c.prove_graph = value != 0;
-  } else if (n == "profile") {
-    ctx->c.profile = value != 0;
+  } else if (n == "profile" && value >= 0 && value <= 2) {
+    ctx->c.profile = (int)value;
   } else {
     ctx->c.err = "unknown option or value: " + n;
     return LGS_ERR_ARG;
@@ -413,15 +419,16 @@ int lgs_commit_batch(lgs_ctx* ctx, uint32_t count, const lgs_key* const* keys,
       (group ? grouped : single).push_back({&keys[i]->k, &c->c});
     }
     lgs::GroupHold hold;
-    const bool prof = ctx->c.profile;
-    ctx->c.profile = prof && single.empty();
+    const int prof = ctx->c.profile;
+    ctx->c.profile = single.empty() ? (prof ? 1 : 0) : 0;  // the grouped path times every stage
     try {
       lgs::enqueue_commit_group(ctx->c, grouped, hold);
-      ctx->c.profile = false;
+      ctx->c.profile = 0;
       for (auto& s : single) lgs::enqueue_commit(ctx->c, *s.first, *s.second);
       read_root(ctx->c, cs.data(), count);
-      ctx->c.profile = prof;
+      ctx->c.profile = prof ? 1 : 0;
       if (prof && single.empty() && !grouped.empty()) finish_profile(ctx->c);
+      ctx->c.profile = prof;
     } catch (...) {
       ctx->c.profile = prof;
       throw;

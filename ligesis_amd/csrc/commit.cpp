@@ -268,7 +268,7 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
                 "hash (MFMA)");
       mark(4 * q + 3, sh);
     }
-    ctx.prof_chunks = ctx.profile ? (int)chunks : 0;
+    ctx.prof_chunks = ctx.profile ? (int)chunks : 0;  // pipelined: every event at any level
     if (ctx.profile) {
       hip_check(hipEventRecord(ctx.ev[1], ctx.stream), "event");  // every chunk encoded
       hip_check(hipEventRecord(ctx.ev[2], sh), "event");          // every split hashed
@@ -331,10 +331,11 @@ uint64_t hash_partial_bytes(const HashPlan& hp, uint64_t two_n) {
 }
 
 void enqueue_commit_stages(Context& ctx, const Key& key, const CommitIO& io, const HashPlan& hp, const Tuning& tu,
-                           hipStream_t st, bool profile) {
+                           hipStream_t st, int profile) {
   const uint64_t m = uint64_t(1) << key.log_m, n = uint64_t(1) << key.log_n, two_n = 2 * n;
   const int logN = key.log_n + 1;
-  if (profile) hip_check(hipEventRecord(ctx.ev[0], st), "event");
+  const bool every = profile == 1;  // level 2: the events around the hash kernel only
+  if (every) hip_check(hipEventRecord(ctx.ev[0], st), "event");
   // F'[i] = RS(F[i]) (ligesis.rs:289-291)
   hip_check(launch_ntt(io.f, n, 1, (uint32_t)n, (uint32_t)m, logN, io.tw, io.enc, st, tu.ntt_impl, io.tp),
             "row encode");
@@ -349,10 +350,10 @@ void enqueue_commit_stages(Context& ctx, const Key& key, const CommitIO& io, con
                                profile ? ctx.ev[2] : nullptr),
               "hash (VALU)");
   }
-  if (profile) hip_check(hipEventRecord(ctx.ev[3], st), "event");
+  if (every) hip_check(hipEventRecord(ctx.ev[3], st), "event");
   // column-wise Merkle tree over serialize(H[:, j]) (ligesis.rs:306-313)
   hip_check(launch_merkle(io.h, 32, two_n, 1, two_n, io.nodes, st, tu.merkle_lg, io.root_host), "merkle");
-  if (profile) hip_check(hipEventRecord(ctx.ev[4], st), "event");
+  if (every) hip_check(hipEventRecord(ctx.ev[4], st), "event");
 }
 
 void enqueue_merkle_trees(Context& ctx, const std::vector<MerkleSource>& trees,

@@ -244,7 +244,9 @@ struct Context {
   // replay repeated proofs of a commitment as a captured graph (off by default: on this ROCm the
   // graph ran the two streams' steps in capture order, slower than issuing them directly)
   bool prove_graph = false;
-  bool profile = false;
+  // 1: time every stage with HIP events; 2: a plain commit records only the two events around
+  // its hash kernel (stage 1), everything else as 1
+  int profile = 0;
   // proofs on this context issue both halves on `stream` (server workers: fewer streams than
   // the device has hardware queues, so no stream shares a queue with an upload)
   bool single_stream_prove = false;
@@ -357,7 +359,7 @@ struct CommitIO {
 // Row transform, hash, finalize and Merkle tree of one commit on stream st, into io's buffers;
 // allocates nothing.
 void enqueue_commit_stages(Context& ctx, const Key& key, const CommitIO& io, const HashPlan& hp, const Tuning& tu,
-                           hipStream_t st, bool profile);
+                           hipStream_t st, int profile);
 bool group_eligible(const Key& key);
 void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, Commitment*>>& items,
                           GroupHold& hold);

@@ -23,8 +23,8 @@ uint32_t pow2_ceil(uint64_t x) {
 
 struct ProfileOn {
   Context& ctx;
-  bool was;
-  explicit ProfileOn(Context& c) : ctx(c), was(c.profile) { c.profile = true; }
+  int was;
+  explicit ProfileOn(Context& c) : ctx(c), was(c.profile) { c.profile = 1; }
   ~ProfileOn() { ctx.profile = was; }
 };
 
