@@ -721,10 +721,11 @@ def bench_commit(args):
                     9: "hash_mfma_tile_kernel<4, 8, 2, 1>",
                     10: "hash_mfma_pipe_kernel<2, 2, true>", 11: "hash_mfma_dma_kernel<3, 2>",
                     12: "hash_mfma_dma_kernel<4, 2>", 14: "hash_mfma_dma_kernel<4, 4>",
-                    15: "hash_mfma_pipe8_kernel<4>", 16: "hash_mfma_pipe8_kernel<8>"}[schedule["hash_impl"]])
+                    15: "hash_mfma_pipe8_kernel<4>", 16: "hash_mfma_pipe8_kernel<8>",
+                    17: "hash_mfma_pipe8_kernel<8, false, 2>"}[schedule["hash_impl"]])
     hash_tag = f"hash_impl={schedule['hash_impl']} ksplit={schedule['hash_ksplit']} pipeline={schedule['pipeline']}"
     ntt_tag = f"ntt_impl={schedule['ntt_impl']}"
-    ntt_kernel = "ntt_r8_fixed_kernel<13>" if schedule["ntt_impl"] == 4 else "ntt_r8_kernel"
+    ntt_kernel = {4: "ntt_r8_fixed_kernel<13>", 5: "ntt_r16_fixed_kernel<13>"}.get(schedule["ntt_impl"], "ntt_r8_kernel")
     if avg["hash_ms"] >= avg["ntt_ms"]:
         roof = {"kernel": f"{hash_kernel} (fused bit-decompose + H = A.B)", "bound": "mfma",
                 "achieved": hash_tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",

@@ -148,9 +148,9 @@ int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value) {
     ctx->c.hash_variant = (int)value;
   } else if (n == "hash_ksplit" && value >= 0) {
     ctx->c.hash_ksplit = (int)value;
-  } else if (n == "hash_impl" && ((value >= -1 && value <= 12) || (value >= 14 && value <= 16))) {
+  } else if (n == "hash_impl" && ((value >= -1 && value <= 12) || (value >= 14 && value <= 17))) {
     ctx->c.hash_impl = (int)value;
-  } else if (n == "ntt_impl" && value >= -1 && value <= 4) {
+  } else if (n == "ntt_impl" && value >= -1 && value <= 5) {
     ctx->c.ntt_impl = (int)value;
   } else if (n == "merkle_lg" && (value == 0 || (value >= 6 && value <= 8))) {
     ctx->c.merkle_lg = (int)value;

@@ -22,16 +22,17 @@ const uint64_t* Context::twiddle(int logN) {
   return p;
 }
 
-const uint64_t* Context::pass_twiddle(int logN) {
-  auto it = pass_twiddles.find(logN);
+const uint64_t* Context::pass_twiddle(int logN, int impl) {
+  const int key = impl << 8 | logN;
+  auto it = pass_twiddles.find(key);
   if (it != pass_twiddles.end()) return it->second.ptr;
-  const uint32_t n = pass_twiddle_entries(logN);
+  const uint32_t n = pass_twiddle_entries(logN, impl);
   if (n == 0) return nullptr;
   const uint64_t* tw = twiddle(logN);
   DevBuf<uint64_t> t(&pool, n);
-  hip_check(launch_pass_twiddles(t.ptr, tw, logN, stream), "pass twiddles");
+  hip_check(launch_pass_twiddles(t.ptr, tw, logN, impl, stream), "pass twiddles");
   const uint64_t* p = t.ptr;
-  pass_twiddles.emplace(logN, std::move(t));
+  pass_twiddles.emplace(key, std::move(t));
   return p;
 }
 
@@ -237,7 +238,7 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
   c.tree.leaves = two_n;
   c.tree.nodes = DevBuf<uint32_t>(&ctx.pool, (size_t)c.tree.node_count() * 8);
   const uint64_t* tw = ctx.twiddle(logN);
-  const uint64_t* tp = tu.ntt_impl == 4 ? ctx.pass_twiddle(logN) : nullptr;
+  const uint64_t* tp = ctx.pass_twiddle(logN, tu.ntt_impl);
   const HashPlan plan = hash_plan(key, tu);
   const int variant = plan.variant;
   const uint32_t ks = plan.ks;

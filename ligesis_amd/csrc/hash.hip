@@ -746,14 +746,22 @@ __device__ __forceinline__ v4i expand_bits(uint32_t y) {
 }
 
 // Slab sl (slot cur, B operands bc and A pair 0 in p0, p1); the next slab's B operands and A pair 0
-// (slot nxt) come back in bc, p0, p1; slab sl + NB - 1 (clamped) is copied into slot dst.
-template <int NB>
+// (slot nxt) come back in bc, p0, p1; slab sl + NB - KS (clamped) is copied into slot dst.
+// KS slabs share one barrier (SYNC on the first of them). Before slab s = KS u the slabs up to
+// s + KS must be visible (slab s + KS - 1 reads slab s + KS's first operands) and the copies of
+// slabs up to s + NB - KS - 1 have been issued, so slabs s + KS + 1 .. s + NB - KS - 1 may still be
+// in flight: vmcnt <= copies (NB - 2 KS - 1). Slab s + j copies slab s + j + NB - KS into the slot
+// of slab s + j - KS, which every wave finished before this barrier.
+template <int NB, int KS, bool SYNC>
 __device__ __forceinline__ void dma_pipe_slab(const DmaTile& t, uint32_t sl, DmaSlot cur, DmaSlot nxt, DmaSlot dst,
                                               v4i (&bc)[4], v4i& p0, v4i& p1, v4i (&acc)[8][4]) {
-  __builtin_amdgcn_s_waitcnt(vmcnt_imm(dma_copies<4>() * (NB - 3)));
-  asm volatile("s_barrier" ::: "memory");
+  static_assert(NB - 2 * KS - 1 >= 0, "ring too short for KS slabs per barrier");
+  if (SYNC) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(dma_copies<4>() * (NB - 2 * KS - 1)));
+    asm volatile("s_barrier" ::: "memory");
+  }
   __builtin_amdgcn_sched_barrier(0);
-  const uint32_t nx = sl + NB - 1;
+  const uint32_t nx = sl + NB - KS;
   dma_copy_slab<4>(t, nx < t.ns ? nx : t.ns - 1, dst);
   uint32_t y[4];
 #pragma unroll
@@ -805,7 +813,7 @@ __device__ __forceinline__ void dma_pipe_slab(const DmaTile& t, uint32_t sl, Dma
   __builtin_amdgcn_s_waitcnt(kLgkmZero);
 }
 
-template <int NB, bool RING = false>
+template <int NB, bool RING = false, int KS = 1>
 __global__ __launch_bounds__(512, 2) void hash_mfma_pipe8_kernel(const uint64_t* __restrict__ enc, uint64_t ld,
                                                                 const v4i* __restrict__ adig, uint32_t ns,
                                                                 uint32_t ntiles, uint32_t ncols,
@@ -834,10 +842,10 @@ __global__ __launch_bounds__(512, 2) void hash_mfma_pipe8_kernel(const uint64_t*
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = v4i{0, 0, 0, 0};
 
-  // slabs 0 .. NB - 2 in flight; wait for slab 0
+  // slabs 0 .. NB - KS - 1 in flight; wait for slab 0
 #pragma unroll
-  for (int j = 0; j < NB - 1; j++) dma_copy_slab<4>(t, (uint32_t)j < ns ? j : ns - 1, s[j]);
-  __builtin_amdgcn_s_waitcnt(vmcnt_imm(dma_copies<4>() * (NB - 2)));
+  for (int j = 0; j < NB - KS; j++) dma_copy_slab<4>(t, (uint32_t)j < ns ? j : ns - 1, s[j]);
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(dma_copies<4>() * (NB - KS - 1)));
   asm volatile("s_barrier" ::: "memory");
   v4i bc[4];
 #pragma unroll
@@ -846,8 +854,12 @@ __global__ __launch_bounds__(512, 2) void hash_mfma_pipe8_kernel(const uint64_t*
   __builtin_amdgcn_s_waitcnt(kLgkmZero);
   for (uint32_t sl = 0; sl < ns; sl += NB) {
 #pragma unroll
-    for (int j = 0; j < NB; j++)
-      dma_pipe_slab<NB>(t, sl + j, s[j], s[(j + 1) % NB], s[(j + NB - 1) % NB], bc, p0, p1, acc);
+    for (int j = 0; j < NB; j++) {
+      if (j % KS == 0)
+        dma_pipe_slab<NB, KS, true>(t, sl + j, s[j], s[(j + 1) % NB], s[(j + NB - KS) % NB], bc, p0, p1, acc);
+      else
+        dma_pipe_slab<NB, KS, false>(t, sl + j, s[j], s[(j + 1) % NB], s[(j + NB - KS) % NB], bc, p0, p1, acc);
+    }
   }
   dma_epilogue<RING>(t, acc, (uint64_t*)(t.wn == 0 ? la0 : t.wn == 1 ? la1 : t.wn == 2 ? la2 : la3), split, ncols,
                     (uint64_t*)part);
@@ -959,7 +971,11 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
   const v4i* ad = (const v4i*)dig;
   if (ring) {  // the Z_2^64 hash: the LDS-copy kernels that reduce in their epilogue only
     if (!hash_impl_reduces(impl)) return hipErrorInvalidValue;
-    if (impl >= 16 && ncols % 256 == 0 && sps % 8 == 0)
+    if (impl == 17 && ncols % 256 == 0 && sps % 8 == 0)
+      hipLaunchKernelGGL((hash_mfma_pipe8_kernel<8, true, 2>), dim3((ncols / 256) * nsplits), dim3(512),
+                         full_cu_pad((const void*)hash_mfma_pipe8_kernel<8, true, 2>), st, enc, ld, ad, sps,
+                         ncols / 256, ncols, part, split0);
+    else if (impl >= 16 && ncols % 256 == 0 && sps % 8 == 0)
       hipLaunchKernelGGL((hash_mfma_pipe8_kernel<8, true>), dim3((ncols / 256) * nsplits), dim3(512),
                          full_cu_pad((const void*)hash_mfma_pipe8_kernel<8, true>), st, enc, ld, ad, sps, ncols / 256,
                          ncols, part, split0);
@@ -997,6 +1013,10 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
   } else if (impl == 6 && ncols % 128 == 0) {  // 64 x 64 wave tiles, 8 waves, 256 x 128 workgroup tile
     hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 4, 2, 2>), dim3((ncols / 128) * nsplits), dim3(512), 0, st, enc, ld,
                        ad, sps, ncols / 128, ncols, part, split0);
+  } else if (impl == 17 && ncols % 256 == 0 && sps % 8 == 0) {  // 8 slots, one barrier per two slabs
+    hipLaunchKernelGGL((hash_mfma_pipe8_kernel<8, false, 2>), dim3((ncols / 256) * nsplits), dim3(512),
+                       full_cu_pad((const void*)hash_mfma_pipe8_kernel<8, false, 2>), st, enc, ld, ad, sps,
+                       ncols / 256, ncols, part, split0);
   } else if (impl >= 16 && ncols % 256 == 0 && sps % 8 == 0) {  // pipelined 8-wave tile, 8 slots
     hipLaunchKernelGGL((hash_mfma_pipe8_kernel<8>), dim3((ncols / 256) * nsplits), dim3(512),
                        full_cu_pad((const void*)hash_mfma_pipe8_kernel<8>), st, enc, ld, ad, sps, ncols / 256, ncols,
@@ -1046,7 +1066,7 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
   return hipGetLastError();
 }
 
-bool hash_impl_reduces(int impl) { return impl == 11 || impl == 12 || (impl >= 14 && impl <= 16); }
+bool hash_impl_reduces(int impl) { return impl == 11 || impl == 12 || (impl >= 14 && impl <= 17); }
 
 hipError_t launch_hash_mfma_finalize(const int32_t* part, uint32_t ksplit, uint32_t ncols, uint64_t* h,
                                      hipStream_t st, int impl, bool ring) {

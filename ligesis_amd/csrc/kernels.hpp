@@ -51,15 +51,18 @@ hipError_t launch_fill_field(uint64_t* p, uint64_t n, uint64_t seed, hipStream_t
 // impl 0 / 2 / 3: radix-8 passes with the whole row in LDS when 2^logN <= 2^LGS_LDS_LOG, with 256 /
 // 512 / 1024 threads per row; larger rows run radix-8 sub-blocks then global radix-2 passes.
 // impl 1: radix-2 stages, the first LGS_LDS_LOG of them in LDS and the rest as global passes.
-// impl 4: for 2^13-point rows of 2^12 values at unit stride, the compile-time-shaped kernel with
-// the pass twiddle table tp (launch_pass_twiddles); other shapes, or tp null, run impl 3.
+// impl 4 / 5: for 2^13-point rows of 2^12 values at unit stride, the compile-time-shaped radix-8 /
+// radix-16 kernel with the pass twiddle table tp (launch_pass_twiddles for that impl); other
+// shapes, or tp null, run impl 3.
 hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_in, uint32_t rows,
                       int logN, const uint64_t* tw, uint64_t* out, hipStream_t st, int impl = 0,
                       const uint64_t* tp = nullptr);
-// Entries of the pass twiddle table of a 2^L-point transform (0 where L != 1 mod 3), and its fill
-// from the transform's twiddle table tw (entries in Montgomery form, c 2^64 mod p).
-uint32_t pass_twiddle_entries(int L);
-hipError_t launch_pass_twiddles(uint64_t* tp, const uint64_t* tw, int L, hipStream_t st);
+// Entries of the pass twiddle table of a 2^L-point transform for row-transform impl 4 (radix-8
+// passes from s = 4, L = 1 mod 3) or 5 (radix-16 passes from s = 5, L = 1 mod 4), 0 for other
+// impls and shapes, and its fill from the transform's twiddle table tw (entries in Montgomery
+// form, c 2^64 mod p).
+uint32_t pass_twiddle_entries(int L, int impl);
+hipError_t launch_pass_twiddles(uint64_t* tp, const uint64_t* tw, int L, int impl, hipStream_t st);
 
 // hash.hip (`mid`, when not null, is recorded between the main kernel and the finalize).
 // ring: the Z_2^64 subset-sum hash (sums mod 2^64 instead of mod p): A's digits represent A mod

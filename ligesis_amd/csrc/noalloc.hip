@@ -49,12 +49,12 @@ struct NoallocLayout {
   uint64_t tw_off = 0, tp_off = 0, part_off = 0, total = 0;
 };
 
-static NoallocLayout layout(int logN, bool pass_table, const HashPlan& hp, uint64_t two_n) {
+static NoallocLayout layout(int logN, bool pass_table, int impl, const HashPlan& hp, uint64_t two_n) {
   NoallocLayout L;
   const uint64_t tw_bytes = (uint64_t(1) << (logN - 1)) * 8;
   L.tw_off = 0;
   L.tp_off = align256(tw_bytes);
-  const uint64_t tp_bytes = pass_table ? (uint64_t)pass_twiddle_entries(logN) * 8 : 0;
+  const uint64_t tp_bytes = pass_table ? (uint64_t)pass_twiddle_entries(logN, impl) * 8 : 0;
   L.part_off = align256(L.tp_off + tp_bytes);
   L.total = L.part_off + hash_partial_bytes(hp, two_n);
   return L;
@@ -85,12 +85,12 @@ extern "C" int lgs_commit_noalloc(lgs_ctx* ctx, const lgs_key* key, const uint64
     tu.pipeline = 1;  // one stream, no second stream to create
     lgs::HashPlan hp = lgs::hash_plan(k, tu);
     if (hp.variant == 1 && !lgs::hash_impl_reduces(hp.impl)) hp.impl = 16;  // one word per partial
-    const bool pass_table = tu.ntt_impl == 4 && lgs::pass_twiddle_entries(logN) != 0;
-    lgs::NoallocLayout L = layout(logN, pass_table, hp, two_n);
+    const bool pass_table = lgs::pass_twiddle_entries(logN, tu.ntt_impl) != 0;
+    lgs::NoallocLayout L = layout(logN, pass_table, tu.ntt_impl, hp, two_n);
     // fewer K-splits until the partials fit (the split count never changes the result)
     while (L.total > lgs::kScratchBytes && hp.ks > 1) {
       hp.ks /= 2;
-      L = layout(logN, pass_table, hp, two_n);
+      L = layout(logN, pass_table, tu.ntt_impl, hp, two_n);
     }
     if (L.total > lgs::kScratchBytes)
       throw lgs::Error(LGS_ERR_ARG, "commit_noalloc: shape needs " + std::to_string(L.total) +
@@ -106,7 +106,8 @@ extern "C" int lgs_commit_noalloc(lgs_ctx* ctx, const lgs_key* key, const uint64
     io.part = (uint64_t*)(base + L.part_off);
     // the tables are rebuilt on every call: another call on this device may have overwritten them
     lgs::hip_check(lgs::launch_twiddles((uint64_t*)io.tw, logN, st), "twiddles");
-    if (pass_table) lgs::hip_check(lgs::launch_pass_twiddles((uint64_t*)io.tp, io.tw, logN, st), "pass twiddles");
+    if (pass_table)
+      lgs::hip_check(lgs::launch_pass_twiddles((uint64_t*)io.tp, io.tw, logN, tu.ntt_impl, st), "pass twiddles");
     lgs::enqueue_commit_stages(c, k, io, hp, tu, st, false);
     if (root) {
       lgs::hip_check(hipMemcpyAsync(root, tree + (2 * two_n - 2) * 32, 32, hipMemcpyDeviceToHost, st), "read root");

@@ -321,18 +321,11 @@ __global__ void pass_twiddle_kernel(uint64_t* __restrict__ tp, const uint64_t* _
   }
 }
 
-uint32_t pass_twiddle_entries(int L) {
+static uint32_t pass_twiddle8_entries(int L) {
   uint32_t n = 0;
   if ((L - 1) % 3 == 0)
     for (int s = 4; L - s >= 3; s += 3) n += 7u << s;
   return n;
-}
-
-hipError_t launch_pass_twiddles(uint64_t* tp, const uint64_t* tw, int L, hipStream_t st) {
-  const uint32_t n = pass_twiddle_entries(L);
-  if (n == 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(pass_twiddle_kernel, dim3((n + 255) / 256), dim3(256), 0, st, tp, tw, L);
-  return hipGetLastError();
 }
 
 // The row LDE of rows of 2^(L-1) values (row stride rs, unit element stride) zero-padded to
@@ -390,6 +383,161 @@ __global__ __launch_bounds__(1 << (L - 3)) void ntt_r8_fixed_kernel(const uint64
       __syncthreads();
     }
     off += 7 * M;
+  }
+}
+
+// ---------------- radix-16 row transform for rows of 2^(L-1) values (L = 1 mod 4) ----------------
+// b * 2^E folded into the butterfly (a, b) -> (a + b 2^E, a - b 2^E) for a compile-time E < 192:
+// 2^96 = -1 turns E >= 96 into a swap of the two outputs, and 64 <= E < 96 into a division by
+// 2^(96 - E) <= 2^32 with the opposite sign.
+template <uint32_t E>
+__device__ __forceinline__ void bfly_pow2(uint64_t& a, uint64_t& b) {
+  constexpr bool neg0 = E >= 96;
+  constexpr uint32_t e = neg0 ? E - 96 : E;
+  constexpr bool neg = e >= 64 ? !neg0 : neg0;
+  uint64_t c;
+  if constexpr (e == 0) c = b;
+  else if constexpr (e <= 32) c = gl_reduce_small(b << e, b >> (64 - e));
+  else if constexpr (e < 64) c = gl_reduce(b << e, b >> (64 - e));
+  else c = gl_mul_inv_pow2(b, 96 - e);
+  const uint64_t s = gl_add(a, c), d = gl_sub(a, c);
+  a = neg ? d : s;
+  b = neg ? s : d;
+}
+
+// 16-point DFT of values held in bit-reversed order (radix-2 DIT network, w_16 = 2^156 of the
+// order-2^13 generator's powers: w_2^k = 2^(39 2^(6-k) mod 192)); output in natural order.
+__device__ __forceinline__ void dft16(uint64_t (&x)[16]) {
+#pragma unroll
+  for (int u = 0; u < 8; u++) bfly_pow2<0>(x[2 * u], x[2 * u + 1]);
+#pragma unroll
+  for (int h = 0; h < 16; h += 4) {
+    bfly_pow2<0>(x[h], x[h + 2]);
+    bfly_pow2<48>(x[h + 1], x[h + 3]);  // w_4 = 2^48
+  }
+#pragma unroll
+  for (int h = 0; h < 16; h += 8) {
+    bfly_pow2<0>(x[h], x[h + 4]);
+    bfly_pow2<120>(x[h + 1], x[h + 5]);  // w_8 = 2^120
+    bfly_pow2<48>(x[h + 2], x[h + 6]);
+    bfly_pow2<168>(x[h + 3], x[h + 7]);
+  }
+  bfly_pow2<0>(x[0], x[8]);
+  bfly_pow2<156>(x[1], x[9]);  // w_16 = 2^156
+  bfly_pow2<120>(x[2], x[10]);
+  bfly_pow2<84>(x[3], x[11]);
+  bfly_pow2<48>(x[4], x[12]);
+  bfly_pow2<12>(x[5], x[13]);
+  bfly_pow2<168>(x[6], x[14]);
+  bfly_pow2<132>(x[7], x[15]);
+}
+
+// Pass twiddles of a 2^L-point transform whose radix-16 passes start at s = 5 (L = 1 mod 4): for
+// s = 5, 9, ..., L - 4, section s holds tp[off_s + (t - 1) 2^s + j] = w_{2^(s+4)}^(j rev4(t))
+// (t = 1..15, j < 2^s) in Montgomery form.
+__global__ void pass_twiddle16_kernel(uint64_t* __restrict__ tp, const uint64_t* __restrict__ tw, int L) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t off = 0;
+  for (int s = 5; L - s >= 4; s += 4) {
+    const uint32_t M = 1u << s;
+    if (i < off + 15 * M) {
+      const uint32_t q = i - off, t = q / M + 1, j = q & (M - 1);
+      const uint32_t r = __brev(t) >> 28;
+      tp[i] = gl_mul(tw_full(tw, (j * r) << (L - s - 4), 1u << (L - 1), 0), GL_EPS);  // 2^64 = 2^32 - 1
+      return;
+    }
+    off += 15 * M;
+  }
+}
+
+static uint32_t pass_twiddle16_entries(int L) {
+  uint32_t n = 0;
+  if ((L - 1) % 4 == 0)
+    for (int s = 5; L - s >= 4; s += 4) n += 15u << s;
+  return n;
+}
+
+uint32_t pass_twiddle_entries(int L, int impl) {
+  return impl == 4 ? pass_twiddle8_entries(L) : impl == 5 ? pass_twiddle16_entries(L) : 0;
+}
+
+hipError_t launch_pass_twiddles(uint64_t* tp, const uint64_t* tw, int L, int impl, hipStream_t st) {
+  const uint32_t n = pass_twiddle_entries(L, impl);
+  if (n == 0) return hipErrorInvalidValue;
+  if (impl == 4)
+    hipLaunchKernelGGL(pass_twiddle_kernel, dim3((n + 255) / 256), dim3(256), 0, st, tp, tw, L);
+  else
+    hipLaunchKernelGGL(pass_twiddle16_kernel, dim3((n + 255) / 256), dim3(256), 0, st, tp, tw, L);
+  return hipGetLastError();
+}
+
+// The row LDE of rows of 2^(L-1) values zero-padded to 2^L points, one workgroup of 2^(L-4)
+// threads per row, 16 values per thread and pass. Stage 1 pairs each value with one of the zero
+// half (a copy); the first pass does stages 2-5: group (g, j) holds positions 32 g + 2 t + j
+// (t < 16), whose stage-1 values are x[k + rev4(t) N / 32] with k = rev_{L-5}(g), pretwiddled by
+// w_32^(j rev4(t)) = 2^(78 j rev4(t) mod 192); then radix-16 passes from s = 5 with Montgomery-form
+// table twiddles, the last one writing dst. LDS addressing as in ntt_r8_fixed_kernel.
+template <int L>
+__global__ __launch_bounds__(1 << (L - 4)) void ntt_r16_fixed_kernel(const uint64_t* __restrict__ in, uint64_t rs,
+                                                                    const uint64_t* __restrict__ tp,
+                                                                    uint64_t* __restrict__ out) {
+  static_assert((L - 1) % 4 == 0 && L >= 13 && L <= LGS_LDS_LOG, "radix-16 passes from s = 5");
+  constexpr uint32_t N = 1u << L, E = N / 32;
+  constexpr int sh = L - 5;
+  extern __shared__ uint64_t sm[];
+  const uint64_t* __restrict__ src = in + (uint64_t)blockIdx.x * rs;
+  uint64_t* __restrict__ dst = out + ((uint64_t)blockIdx.x << L);
+  const uint32_t gi = threadIdx.x;
+  {
+    const uint32_t k = gi & (E - 1), j = gi / E;  // j uniform per half of the workgroup
+    uint64_t x[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) x[t] = src[k + (__brev((uint32_t)t) >> 28) * E];
+    if (j) {
+      x[1] = gl_mul_pow2(x[1], 78u * 8 % 192);
+      x[2] = gl_mul_pow2(x[2], 78u * 4 % 192);
+      x[3] = gl_mul_pow2(x[3], 78u * 12 % 192);
+      x[4] = gl_mul_pow2(x[4], 78u * 2 % 192);
+      x[5] = gl_mul_pow2(x[5], 78u * 10 % 192);
+      x[6] = gl_mul_pow2(x[6], 78u * 6 % 192);
+      x[7] = gl_mul_pow2(x[7], 78u * 14 % 192);
+      x[8] = gl_mul_pow2(x[8], 78u * 1 % 192);
+      x[9] = gl_mul_pow2(x[9], 78u * 9 % 192);
+      x[10] = gl_mul_pow2(x[10], 78u * 5 % 192);
+      x[11] = gl_mul_pow2(x[11], 78u * 13 % 192);
+      x[12] = gl_mul_pow2(x[12], 78u * 3 % 192);
+      x[13] = gl_mul_pow2(x[13], 78u * 11 % 192);
+      x[14] = gl_mul_pow2(x[14], 78u * 7 % 192);
+      x[15] = gl_mul_pow2(x[15], 78u * 15 % 192);
+    }
+    dft16(x);
+    const uint32_t pb = lds_pos(32 * brev_bits(k, L - 5) + j, sh);
+#pragma unroll
+    for (int t = 0; t < 16; t++) sm[pb ^ lds_pos(2 * t, sh)] = x[t];
+  }
+  __syncthreads();
+  uint32_t off = 0;
+#pragma unroll
+  for (int s = 5; s < L; s += 4) {
+    const uint32_t M = 1u << s;
+    const uint32_t j = gi & (M - 1);
+    const uint32_t base = ((gi >> s) << (s + 4)) | j;
+    const uint32_t pb = lds_pos(base, sh);
+    uint64_t x[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) x[t] = sm[pb ^ lds_pos(t << s, sh)];
+#pragma unroll
+    for (int t = 1; t < 16; t++) x[t] = gl_mul_mont(x[t], tp[off + (t - 1) * M + j]);
+    dft16(x);
+    if (s == L - 4) {
+#pragma unroll
+      for (int t = 0; t < 16; t++) dst[base + t * M] = x[t];
+    } else {
+#pragma unroll
+      for (int t = 0; t < 16; t++) sm[pb ^ lds_pos(t << s, sh)] = x[t];
+      __syncthreads();
+    }
+    off += 15 * M;
   }
 }
 
@@ -532,10 +680,14 @@ hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_i
                       int logN, const uint64_t* tw, uint64_t* out, hipStream_t st, int impl,
                       const uint64_t* tp) {
   if (rows == 0) return hipSuccess;
-  if (impl == 4) {
+  if (impl == 4 || impl == 5) {
     if (logN == 13 && es == 1 && n_in == (1u << 12) && tp) {
-      hipLaunchKernelGGL(ntt_r8_fixed_kernel<13>, dim3(rows), dim3(1024), ntt_lds_bytes(1u << 13), st, in, rs, tp,
-                         out);
+      if (impl == 4)
+        hipLaunchKernelGGL(ntt_r8_fixed_kernel<13>, dim3(rows), dim3(1024), ntt_lds_bytes(1u << 13), st, in, rs, tp,
+                           out);
+      else
+        hipLaunchKernelGGL(ntt_r16_fixed_kernel<13>, dim3(rows), dim3(512), ntt_lds_bytes(1u << 13), st, in, rs, tp,
+                           out);
       return hipGetLastError();
     }
     impl = 3;

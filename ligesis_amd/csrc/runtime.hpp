@@ -228,7 +228,7 @@ struct Context {
   std::string err;
   DevicePool pool;
   std::map<int, DevBuf<uint64_t>> twiddles;
-  std::map<int, DevBuf<uint64_t>> pass_twiddles;  // per-pass factor tables (ntt_impl 4)
+  std::map<int, DevBuf<uint64_t>> pass_twiddles;  // per-pass factor tables (ntt_impl 4, 5), key impl << 8 | logN
   // Explicit launch-schedule options (lgs_set_option); unset fields fall back to the process
   // tuning table, then to the built-in defaults (see resolve).
   int hash_variant = 0;  // 0 auto, 1 MFMA, 2 VALU
@@ -237,7 +237,9 @@ struct Context {
                          // 3 2x4 waves, 4 loads issued 2 slabs ahead, 5 as 4 with 2x4 waves,
                          // 6 / 7 64x64 wave tiles in 8 / 16-wave workgroups,
                          // 8 / 9 64x128 wave tiles in 4 / 8-wave workgroups, 10 as 0 with A
-                         // fragments read ahead, 11 / 12 asynchronous copies into an LDS ring
+                         // fragments read ahead, 11 / 12 asynchronous copies into an LDS ring,
+                         // 14 8 waves sharing a 4-slab ring, 15 / 16 pipelined 8 waves with a 4- /
+                         // 8-slab ring, 17 as 16 with one barrier per two slabs
   int ntt_impl = -1;     // -1 auto; row transform: 0/2/3 radix-8 with 256/512/1024 threads, 1 radix-2
   int merkle_lg = 0;     // 0 auto; log2 nodes per Merkle workgroup = levels per launch (6..8)
   int pipeline = 0;      // 0 auto; 1 off, 2 / 4 / 8 row chunks (transform on stream, hash on stream2)
@@ -262,7 +264,7 @@ struct Context {
   void free_host();  // pinned staging and arena
 
   const uint64_t* twiddle(int logN);
-  const uint64_t* pass_twiddle(int logN);  // null where the transform has no such table
+  const uint64_t* pass_twiddle(int logN, int impl);  // null where the transform has no such table
   uint8_t* staging(size_t bytes);
   hipStream_t side_stream();  // stream2, created on first use
   void drain();                // waits for every stream, ignoring errors (error and teardown paths)

@@ -205,7 +205,7 @@ void Server::enqueue_request(Group& g, Request& r) {
   io.nodes = c.tree.nodes.ptr;
   io.part = g.parts.back().ptr;
   io.tw = ctx.twiddle(logN);
-  io.tp = tu.ntt_impl == 4 ? ctx.pass_twiddle(logN) : nullptr;
+  io.tp = ctx.pass_twiddle(logN, tu.ntt_impl);
   if (!host) {
     c.f = r.f;
     io.f = c.f;

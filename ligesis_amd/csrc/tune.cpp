@@ -96,8 +96,8 @@ size_t tune_load(const char* path) {
       throw Error(1, "tune: malformed line " + std::to_string(lineno) + " in " + path);
     const Tuning& t = e.t;
     const bool ok = lm >= 0 && lm <= 20 && ln >= 1 && ln <= 26 && t.hash_variant >= 0 && t.hash_variant <= 2 &&
-                    ((t.hash_impl >= 0 && t.hash_impl <= 12) || (t.hash_impl >= 14 && t.hash_impl <= 16)) && t.hash_ksplit >= 0 &&
-                    (t.hash_ksplit & (t.hash_ksplit - 1)) == 0 && t.ntt_impl >= 0 && t.ntt_impl <= 4 &&
+                    ((t.hash_impl >= 0 && t.hash_impl <= 12) || (t.hash_impl >= 14 && t.hash_impl <= 17)) && t.hash_ksplit >= 0 &&
+                    (t.hash_ksplit & (t.hash_ksplit - 1)) == 0 && t.ntt_impl >= 0 && t.ntt_impl <= 5 &&
                     t.merkle_lg >= 6 && t.merkle_lg <= 8 &&
                     (t.pipeline == 1 || t.pipeline == 2 || t.pipeline == 4 || t.pipeline == 8);
     if (!ok) throw Error(1, "tune: value out of range on line " + std::to_string(lineno) + " in " + path);
@@ -157,7 +157,10 @@ double autotune(Context& ctx, const Key& key, uint32_t reps, TuneEntry& best) {
       const uint64_t per_row = (uint64_t(1) << logN) / 8;  // radix-8 butterflies per pass
       if (per_row >= 512) cand.push_back(2);
       if (per_row >= 1024) cand.push_back(3);
-      if (logN == 13) cand.push_back(4);  // the compile-time-shaped 2^13 kernel
+      if (logN == 13) {  // the compile-time-shaped 2^13 kernels (radix 8, radix 16)
+        cand.push_back(4);
+        cand.push_back(5);
+      }
     }
     double bt = 1e30, base = 1e30;
     int bi = cur.ntt_impl;
@@ -183,7 +186,7 @@ double autotune(Context& ctx, const Key& key, uint32_t reps, TuneEntry& best) {
       uint32_t h = pow2_ceil((512 + ntiles - 1) / ntiles);
       if (h > m / 2) h = (uint32_t)(m / 2);
       default_ks = h;
-      const int impls[15] = {0, 1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14, 15, 16};
+      const int impls[16] = {0, 1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14, 15, 16, 17};
       for (int impl : impls) {
         if ((impl == 3 || impl == 5 || impl == 7 || impl == 9) && two_n % 256) continue;
         for (uint32_t ks : {h / 2, h, 2 * h}) {

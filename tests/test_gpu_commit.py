@@ -60,15 +60,16 @@ def test_long_rows_match_oracle(gpu_ctx, shape):
 
 @pytest.mark.parametrize("shape", [(3, 12), (7, 12), (2, 11)])
 def test_every_row_transform_matches_oracle(gpu_ctx, shape):
-    """Every row-transform variant gives the oracle's F'; ntt_impl 4 (the compile-time 2^13
-    kernel with per-pass twiddle tables) runs at rows of 2^12 values and falls back to 3 elsewhere."""
+    """Every row-transform variant gives the oracle's F'; ntt_impl 4 and 5 (the compile-time 2^13
+    radix-8 and radix-16 kernels with per-pass twiddle tables) run at rows of 2^12 values and fall
+    back to 3 elsewhere."""
     log_m, log_n = shape
     inp = synth.Inputs(log_m, log_n, seed=60 + log_m + log_n)
     pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx, commit_only=True)
     ocd = oracle.CommitData(oracle.ProvingKey(log_m, log_n, inp.a, commit_a=False), inp.f, keep_enc=True)
     want = ocd.enc()
     try:
-        for impl in (0, 1, 2, 3, 4):
+        for impl in (0, 1, 2, 3, 4, 5):
             print(f"row transform ntt_impl={impl} shape={shape}", flush=True)
             gpu_ctx.set_option("ntt_impl", impl)
             root, cd = Ligesis.commit(pk, inp.f)
@@ -93,7 +94,7 @@ def test_row_transform_extreme_values(gpu_ctx):
     pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx, commit_only=True)
     ocd = oracle.CommitData(oracle.ProvingKey(log_m, log_n, inp.a, commit_a=False), f, keep_enc=True)
     try:
-        for impl in (3, 4):
+        for impl in (3, 4, 5):
             print(f"extreme rows ntt_impl={impl}", flush=True)
             gpu_ctx.set_option("ntt_impl", impl)
             root, cd = Ligesis.commit(pk, f)
@@ -132,7 +133,7 @@ def test_every_mfma_kernel_matches_oracle(gpu_ctx, shape):
     want = ocd.h()
     gpu_ctx.set_option("hash", 1)
     try:
-        for impl in list(range(13)) + [14, 15, 16]:
+        for impl in list(range(13)) + [14, 15, 16, 17]:
             for ks in (0, 1, 2, 4):
                 if ks > (1 << log_m) // 2:
                     continue

@@ -40,7 +40,7 @@ def test_ring64_commit_matches_oracle(gpu_ctx, shape):
     field_root = oracle.CommitData(oracle.ProvingKey(lm, ln, a % P, commit_a=False), inp.f).root
     assert ocd.root != field_root
     try:
-        for impl in (16, 15, 14, 12, 11):
+        for impl in (17, 16, 15, 14, 12, 11):
             for ks in (0, 1, 2, 4):
                 if ks > (1 << lm) // 2:
                     continue

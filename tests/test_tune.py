@@ -44,7 +44,7 @@ def test_load_save_roundtrip(tmp_path):
     "12 12 1 0 8 2 7 1 0.9 extra",    # trailing field
     "12 12 1 13 8 2 7 1 0.9",         # hash_impl out of range
     "12 12 1 0 6 2 7 1 0.9",          # ksplit not a power of two
-    "12 12 1 0 8 5 7 1 0.9",          # ntt_impl out of range
+    "12 12 1 0 8 6 7 1 0.9",          # ntt_impl out of range
     "12 12 1 0 8 2 9 1 0.9",          # merkle_lg out of range
     "12 12 1 0 8 2 7 3 0.9",          # pipeline not 1, 2, 4 or 8
     "21 12 1 0 8 2 7 1 0.9",          # log_m out of range
