@@ -335,16 +335,20 @@ void enqueue_commit_stages(Context& ctx, const Key& key, const CommitIO& io, con
                            hipStream_t st, int profile) {
   const uint64_t m = uint64_t(1) << key.log_m, n = uint64_t(1) << key.log_n, two_n = 2 * n;
   const int logN = key.log_n + 1;
-  const bool every = profile == 1;  // level 2: the events around the hash kernel only
+  // level 1: events between all stages; level 2: the MFMA hash kernel's own start and stop times
+  // (hipExtLaunchKernel), so nothing is added to the stream around it
+  const bool every = profile == 1;
+  const bool kernel_only = profile == 2 && hp.variant == 1;
   if (every) hip_check(hipEventRecord(ctx.ev[0], st), "event");
   // F'[i] = RS(F[i]) (ligesis.rs:289-291)
   hip_check(launch_ntt(io.f, n, 1, (uint32_t)n, (uint32_t)m, logN, io.tw, io.enc, st, tu.ntt_impl, io.tp),
             "row encode");
-  if (profile) hip_check(hipEventRecord(ctx.ev[1], st), "event");
+  if (profile && !kernel_only) hip_check(hipEventRecord(ctx.ev[1], st), "event");
   // H = A . B with B read bitwise from F' (ligesis.rs:292-302)
   if (hp.variant == 1) {
     hip_check(launch_hash_mfma(io.enc, two_n, (uint32_t)m, (uint32_t)two_n, key.dig.ptr, hp.ks, (int32_t*)io.part,
-                               io.h, st, profile ? ctx.ev[2] : nullptr, hp.impl, key.ring64),
+                               io.h, st, every ? ctx.ev[2] : nullptr, hp.impl, key.ring64,
+                               kernel_only ? ctx.ev[1] : nullptr, kernel_only ? ctx.ev[2] : nullptr),
               "hash (MFMA)");
   } else {
     hip_check(launch_hash_valu(io.enc, two_n, (uint32_t)m, (uint32_t)two_n, key.at.ptr, hp.ks, io.part, io.h, st,

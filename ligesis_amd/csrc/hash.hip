@@ -14,6 +14,8 @@
 #include <map>
 #include <mutex>
 
+#include <hip/hip_ext.h>
+
 #include "kernels.hpp"
 
 namespace lgs {
@@ -963,37 +965,48 @@ static uint32_t full_cu_pad(const void* kernel) {
   return pad;
 }
 
+// A kernel launch whose start and stop times go to e0 / e1 when given (hipExtLaunchKernel: the
+// timestamps of the dispatch itself, no marker packets on the stream around it).
+template <typename K, typename... A>
+static void launch_k(K kernel, dim3 grid, dim3 block, uint32_t lds, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                     A... args) {
+  if (e0 || e1)
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, st, e0, e1, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, args...);
+}
+
 hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
                                   const int8_t* dig, uint32_t ksplit, uint32_t split0, uint32_t nsplits,
-                                  int32_t* part, hipStream_t st, int impl, bool ring) {
+                                  int32_t* part, hipStream_t st, int impl, bool ring, hipEvent_t k0, hipEvent_t k1) {
   const uint32_t ntiles = ncols / HM_NT;
   const uint32_t sps = m_rows / ksplit;
   const v4i* ad = (const v4i*)dig;
   if (ring) {  // the Z_2^64 hash: the LDS-copy kernels that reduce in their epilogue only
     if (!hash_impl_reduces(impl)) return hipErrorInvalidValue;
     if (impl == 17 && ncols % 256 == 0 && sps % 8 == 0)
-      hipLaunchKernelGGL((hash_mfma_pipe8_kernel<8, true, 2>), dim3((ncols / 256) * nsplits), dim3(512),
-                         full_cu_pad((const void*)hash_mfma_pipe8_kernel<8, true, 2>), st, enc, ld, ad, sps,
+      launch_k((hash_mfma_pipe8_kernel<8, true, 2>), dim3((ncols / 256) * nsplits), dim3(512),
+                         full_cu_pad((const void*)hash_mfma_pipe8_kernel<8, true, 2>), st, k0, k1, enc, ld, ad, sps,
                          ncols / 256, ncols, part, split0);
     else if (impl >= 16 && ncols % 256 == 0 && sps % 8 == 0)
-      hipLaunchKernelGGL((hash_mfma_pipe8_kernel<8, true>), dim3((ncols / 256) * nsplits), dim3(512),
-                         full_cu_pad((const void*)hash_mfma_pipe8_kernel<8, true>), st, enc, ld, ad, sps, ncols / 256,
+      launch_k((hash_mfma_pipe8_kernel<8, true>), dim3((ncols / 256) * nsplits), dim3(512),
+                         full_cu_pad((const void*)hash_mfma_pipe8_kernel<8, true>), st, k0, k1, enc, ld, ad, sps, ncols / 256,
                          ncols, part, split0);
     else if (impl >= 15 && ncols % 256 == 0 && sps % 4 == 0)
-      hipLaunchKernelGGL((hash_mfma_pipe8_kernel<4, true>), dim3((ncols / 256) * nsplits), dim3(512),
-                         full_cu_pad((const void*)hash_mfma_pipe8_kernel<4, true>), st, enc, ld, ad, sps, ncols / 256,
+      launch_k((hash_mfma_pipe8_kernel<4, true>), dim3((ncols / 256) * nsplits), dim3(512),
+                         full_cu_pad((const void*)hash_mfma_pipe8_kernel<4, true>), st, k0, k1, enc, ld, ad, sps, ncols / 256,
                          ncols, part, split0);
     else if (impl >= 14 && ncols % 256 == 0 && sps % 4 == 0)
-      hipLaunchKernelGGL((hash_mfma_dma_kernel<4, 4, true>), dim3((ncols / 256) * nsplits), dim3(512),
-                         full_cu_pad((const void*)hash_mfma_dma_kernel<4, 4, true>), st, enc, ld, ad, sps, ncols / 256,
+      launch_k((hash_mfma_dma_kernel<4, 4, true>), dim3((ncols / 256) * nsplits), dim3(512),
+                         full_cu_pad((const void*)hash_mfma_dma_kernel<4, 4, true>), st, k0, k1, enc, ld, ad, sps, ncols / 256,
                          ncols, part, split0);
     else if (impl == 11 || sps % 4)
-      hipLaunchKernelGGL((hash_mfma_dma_kernel<3, 2, true>), dim3(ntiles * nsplits), dim3(256),
-                         full_cu_pad((const void*)hash_mfma_dma_kernel<3, 2, true>), st, enc, ld, ad, sps, ntiles,
+      launch_k((hash_mfma_dma_kernel<3, 2, true>), dim3(ntiles * nsplits), dim3(256),
+                         full_cu_pad((const void*)hash_mfma_dma_kernel<3, 2, true>), st, k0, k1, enc, ld, ad, sps, ntiles,
                          ncols, part, split0);
     else
-      hipLaunchKernelGGL((hash_mfma_dma_kernel<4, 2, true>), dim3(ntiles * nsplits), dim3(256),
-                         full_cu_pad((const void*)hash_mfma_dma_kernel<4, 2, true>), st, enc, ld, ad, sps, ntiles,
+      launch_k((hash_mfma_dma_kernel<4, 2, true>), dim3(ntiles * nsplits), dim3(256),
+                         full_cu_pad((const void*)hash_mfma_dma_kernel<4, 2, true>), st, k0, k1, enc, ld, ad, sps, ntiles,
                          ncols, part, split0);
     return hipGetLastError();
   }
@@ -1001,66 +1014,66 @@ hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_
     const int wn = impl == 5 ? 4 : 2;
     if (sps % 2 == 0 && ncols % (64 * wn) == 0) {
       if (impl == 4)
-        hipLaunchKernelGGL((hash_mfma_ring_kernel<2, 2>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps,
+        launch_k((hash_mfma_ring_kernel<2, 2>), dim3(ntiles * nsplits), dim3(256), 0, st, k0, k1, enc, ld, ad, sps,
                            ntiles, ncols, part, split0);
       else
-        hipLaunchKernelGGL((hash_mfma_ring_kernel<4, 2>), dim3((ncols / 256) * nsplits), dim3(512), 0, st, enc, ld,
+        launch_k((hash_mfma_ring_kernel<4, 2>), dim3((ncols / 256) * nsplits), dim3(512), 0, st, k0, k1, enc, ld,
                            ad, sps, ncols / 256, ncols, part, split0);
     } else {
-      hipLaunchKernelGGL((hash_mfma_pipe_kernel<1, 2>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps,
+      launch_k((hash_mfma_pipe_kernel<1, 2>), dim3(ntiles * nsplits), dim3(256), 0, st, k0, k1, enc, ld, ad, sps,
                          ntiles, ncols, part, split0);
     }
   } else if (impl == 6 && ncols % 128 == 0) {  // 64 x 64 wave tiles, 8 waves, 256 x 128 workgroup tile
-    hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 4, 2, 2>), dim3((ncols / 128) * nsplits), dim3(512), 0, st, enc, ld,
+    launch_k((hash_mfma_tile_kernel<4, 4, 2, 2>), dim3((ncols / 128) * nsplits), dim3(512), 0, st, k0, k1, enc, ld,
                        ad, sps, ncols / 128, ncols, part, split0);
   } else if (impl == 17 && ncols % 256 == 0 && sps % 8 == 0) {  // 8 slots, one barrier per two slabs
-    hipLaunchKernelGGL((hash_mfma_pipe8_kernel<8, false, 2>), dim3((ncols / 256) * nsplits), dim3(512),
-                       full_cu_pad((const void*)hash_mfma_pipe8_kernel<8, false, 2>), st, enc, ld, ad, sps,
+    launch_k((hash_mfma_pipe8_kernel<8, false, 2>), dim3((ncols / 256) * nsplits), dim3(512),
+                       full_cu_pad((const void*)hash_mfma_pipe8_kernel<8, false, 2>), st, k0, k1, enc, ld, ad, sps,
                        ncols / 256, ncols, part, split0);
   } else if (impl >= 16 && ncols % 256 == 0 && sps % 8 == 0) {  // pipelined 8-wave tile, 8 slots
-    hipLaunchKernelGGL((hash_mfma_pipe8_kernel<8>), dim3((ncols / 256) * nsplits), dim3(512),
-                       full_cu_pad((const void*)hash_mfma_pipe8_kernel<8>), st, enc, ld, ad, sps, ncols / 256, ncols,
+    launch_k((hash_mfma_pipe8_kernel<8>), dim3((ncols / 256) * nsplits), dim3(512),
+                       full_cu_pad((const void*)hash_mfma_pipe8_kernel<8>), st, k0, k1, enc, ld, ad, sps, ncols / 256, ncols,
                        part, split0);
   } else if (impl >= 15 && ncols % 256 == 0 && sps % 4 == 0) {  // pipelined, 4 slots
-    hipLaunchKernelGGL((hash_mfma_pipe8_kernel<4>), dim3((ncols / 256) * nsplits), dim3(512),
-                       full_cu_pad((const void*)hash_mfma_pipe8_kernel<4>), st, enc, ld, ad, sps, ncols / 256, ncols,
+    launch_k((hash_mfma_pipe8_kernel<4>), dim3((ncols / 256) * nsplits), dim3(512),
+                       full_cu_pad((const void*)hash_mfma_pipe8_kernel<4>), st, k0, k1, enc, ld, ad, sps, ncols / 256, ncols,
                        part, split0);
   } else if (impl >= 14 && ncols % 256 == 0 && sps % 4 == 0) {  // 8 waves sharing the A ring
-    hipLaunchKernelGGL((hash_mfma_dma_kernel<4, 4>), dim3((ncols / 256) * nsplits), dim3(512),
-                       full_cu_pad((const void*)hash_mfma_dma_kernel<4, 4>), st, enc, ld, ad, sps, ncols / 256, ncols,
+    launch_k((hash_mfma_dma_kernel<4, 4>), dim3((ncols / 256) * nsplits), dim3(512),
+                       full_cu_pad((const void*)hash_mfma_dma_kernel<4, 4>), st, k0, k1, enc, ld, ad, sps, ncols / 256, ncols,
                        part, split0);
   } else if (hash_impl_reduces(impl)) {  // asynchronous copies into an LDS ring of 3 / 4 slabs
     if (impl == 11 || sps % 4)
-      hipLaunchKernelGGL((hash_mfma_dma_kernel<3, 2>), dim3(ntiles * nsplits), dim3(256),
-                         full_cu_pad((const void*)hash_mfma_dma_kernel<3, 2>), st, enc,
+      launch_k((hash_mfma_dma_kernel<3, 2>), dim3(ntiles * nsplits), dim3(256),
+                         full_cu_pad((const void*)hash_mfma_dma_kernel<3, 2>), st, k0, k1, enc,
                          ld, ad, sps, ntiles, ncols, part, split0);
     else
-      hipLaunchKernelGGL((hash_mfma_dma_kernel<4, 2>), dim3(ntiles * nsplits), dim3(256),
-                         full_cu_pad((const void*)hash_mfma_dma_kernel<4, 2>), st, enc,
+      launch_k((hash_mfma_dma_kernel<4, 2>), dim3(ntiles * nsplits), dim3(256),
+                         full_cu_pad((const void*)hash_mfma_dma_kernel<4, 2>), st, k0, k1, enc,
                          ld, ad, sps, ntiles, ncols, part, split0);
   } else if (impl == 10) {  // as 0, A fragments read one product group ahead
-    hipLaunchKernelGGL((hash_mfma_pipe_kernel<2, 2, true>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad,
+    launch_k((hash_mfma_pipe_kernel<2, 2, true>), dim3(ntiles * nsplits), dim3(256), 0, st, k0, k1, enc, ld, ad,
                        sps, ntiles, ncols, part, split0);
   } else if (impl == 8 && ncols % 128 == 0) {  // 64 x 128 wave tiles, 4 waves, 256 x 128 workgroup tile
-    hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 8, 1, 2>), dim3((ncols / 128) * nsplits), dim3(256), 0, st, enc, ld,
+    launch_k((hash_mfma_tile_kernel<4, 8, 1, 2>), dim3((ncols / 128) * nsplits), dim3(256), 0, st, k0, k1, enc, ld,
                        ad, sps, ncols / 128, ncols, part, split0);
   } else if (impl == 9 && ncols % 256 == 0) {  // 64 x 128 wave tiles, 8 waves, 256 x 256 workgroup tile
-    hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 8, 2, 1>), dim3((ncols / 256) * nsplits), dim3(512), 0, st, enc, ld,
+    launch_k((hash_mfma_tile_kernel<4, 8, 2, 1>), dim3((ncols / 256) * nsplits), dim3(512), 0, st, k0, k1, enc, ld,
                        ad, sps, ncols / 256, ncols, part, split0);
   } else if (impl == 7 && ncols % 256 == 0) {  // 64 x 64 wave tiles, 16 waves, 256 x 256 workgroup tile
-    hipLaunchKernelGGL((hash_mfma_tile_kernel<4, 4, 4, 1>), dim3((ncols / 256) * nsplits), dim3(1024), 0, st, enc,
+    launch_k((hash_mfma_tile_kernel<4, 4, 4, 1>), dim3((ncols / 256) * nsplits), dim3(1024), 0, st, k0, k1, enc,
                        ld, ad, sps, ncols / 256, ncols, part, split0);
   } else if (impl == 3 && ncols % 256 == 0) {
-    hipLaunchKernelGGL((hash_mfma_pipe_kernel<2, 4>), dim3((ncols / 256) * nsplits), dim3(512), 0, st, enc, ld, ad,
+    launch_k((hash_mfma_pipe_kernel<2, 4>), dim3((ncols / 256) * nsplits), dim3(512), 0, st, k0, k1, enc, ld, ad,
                        sps, ncols / 256, ncols, part, split0);
   } else if (impl == 2) {
-    hipLaunchKernelGGL(hash_mfma_kernel, dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps, ntiles, ncols,
+    launch_k(hash_mfma_kernel, dim3(ntiles * nsplits), dim3(256), 0, st, k0, k1, enc, ld, ad, sps, ntiles, ncols,
                        part, split0);
   } else if (impl == 1) {
-    hipLaunchKernelGGL((hash_mfma_pipe_kernel<1, 2>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps,
+    launch_k((hash_mfma_pipe_kernel<1, 2>), dim3(ntiles * nsplits), dim3(256), 0, st, k0, k1, enc, ld, ad, sps,
                        ntiles, ncols, part, split0);
   } else {
-    hipLaunchKernelGGL((hash_mfma_pipe_kernel<2, 2>), dim3(ntiles * nsplits), dim3(256), 0, st, enc, ld, ad, sps,
+    launch_k((hash_mfma_pipe_kernel<2, 2>), dim3(ntiles * nsplits), dim3(256), 0, st, k0, k1, enc, ld, ad, sps,
                        ntiles, ncols, part, split0);
   }
   return hipGetLastError();
@@ -1083,8 +1096,8 @@ hipError_t launch_hash_mfma_finalize(const int32_t* part, uint32_t ksplit, uint3
 
 hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
                             const int8_t* dig, uint32_t ksplit, int32_t* part, uint64_t* h,
-                            hipStream_t st, hipEvent_t mid, int impl, bool ring) {
-  hipError_t e = launch_hash_mfma_splits(enc, ld, m_rows, ncols, dig, ksplit, 0, ksplit, part, st, impl, ring);
+                            hipStream_t st, hipEvent_t mid, int impl, bool ring, hipEvent_t k0, hipEvent_t k1) {
+  hipError_t e = launch_hash_mfma_splits(enc, ld, m_rows, ncols, dig, ksplit, 0, ksplit, part, st, impl, ring, k0, k1);
   if (e != hipSuccess) return e;
   if (mid && (e = hipEventRecord(mid, st)) != hipSuccess) return e;
   return launch_hash_mfma_finalize(part, ksplit, ncols, h, st, impl, ring);

@@ -64,9 +64,10 @@ hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_i
 uint32_t pass_twiddle_entries(int L, int impl);
 hipError_t launch_pass_twiddles(uint64_t* tp, const uint64_t* tw, int L, int impl, hipStream_t st);
 
-// hash.hip (`mid`, when not null, is recorded between the main kernel and the finalize).
+// hash.hip (`mid`, when not null, is recorded between the main kernel and the finalize; k0 / k1,
+// when not null, receive the main kernel's own start and stop times).
 // ring: the Z_2^64 subset-sum hash (sums mod 2^64 instead of mod p): A's digits represent A mod
-// 2^64, and only the MFMA kernels that reduce in their epilogue (11, 12, 14) take it.
+// 2^64, and only the MFMA kernels that reduce in their epilogue (11, 12, 14-17) take it.
 hipError_t launch_a_layouts(const uint64_t* a, uint64_t W, uint64_t* at, int8_t* dig, hipStream_t st,
                             bool ring = false);
 hipError_t launch_hash_valu(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
@@ -74,12 +75,14 @@ hipError_t launch_hash_valu(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
                             hipStream_t st, hipEvent_t mid);
 hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
                             const int8_t* dig, uint32_t ksplit, int32_t* part, uint64_t* h,
-                            hipStream_t st, hipEvent_t mid, int impl, bool ring = false);
+                            hipStream_t st, hipEvent_t mid, int impl, bool ring = false, hipEvent_t k0 = nullptr,
+                            hipEvent_t k1 = nullptr);
 // The main MFMA kernel for K-splits [split0, split0 + nsplits) only (partials of those splits),
 // and the finalize over all ksplit partials: a commit may issue its splits in several launches.
 hipError_t launch_hash_mfma_splits(const uint64_t* enc, uint64_t ld, uint32_t m_rows, uint32_t ncols,
                                   const int8_t* dig, uint32_t ksplit, uint32_t split0, uint32_t nsplits,
-                                  int32_t* part, hipStream_t st, int impl, bool ring = false);
+                                  int32_t* part, hipStream_t st, int impl, bool ring = false,
+                                  hipEvent_t k0 = nullptr, hipEvent_t k1 = nullptr);
 // Kernels 11 and 12 (and the batched hash) leave one word per (split, row, column), reduced mod p,
 // in the first quarter of the same buffer; the others 8 digit-plane int32 sums.
 bool hash_impl_reduces(int impl);

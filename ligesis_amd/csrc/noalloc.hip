@@ -84,7 +84,7 @@ extern "C" int lgs_commit_noalloc(lgs_ctx* ctx, const lgs_key* key, const uint64
     lgs::Tuning tu = c.resolve(k.log_m, k.log_n);
     tu.pipeline = 1;  // one stream, no second stream to create
     lgs::HashPlan hp = lgs::hash_plan(k, tu);
-    if (hp.variant == 1 && !lgs::hash_impl_reduces(hp.impl)) hp.impl = 16;  // one word per partial
+    if (hp.variant == 1 && !lgs::hash_impl_reduces(hp.impl)) hp.impl = lgs::Tuning{}.hash_impl;  // one word per partial
     const bool pass_table = lgs::pass_twiddle_entries(logN, tu.ntt_impl) != 0;
     lgs::NoallocLayout L = layout(logN, pass_table, tu.ntt_impl, hp, two_n);
     // fewer K-splits until the partials fit (the split count never changes the result)

@@ -156,8 +156,8 @@ struct DevTree {
 // Launch schedule of one commit shape. Every choice yields the same bytes; only speed differs.
 struct Tuning {
   int hash_variant = 0;  // 1 MFMA, 2 VALU (0: MFMA where eligible)
-  int hash_impl = 16;    // MFMA kernel variant (16: 8 waves sharing an 8-slab LDS ring, copies and bit
-                         // expansion interleaved with the products)
+  int hash_impl = 17;    // MFMA kernel variant (17: 8 waves sharing an 8-slab LDS ring, copies and bit
+                         // expansion interleaved with the products, one barrier per two slabs)
   int hash_ksplit = 0;   // slabs of the K (= 64 m) dimension; 0: heuristic
   int ntt_impl = 0;      // row transform variant
   int merkle_lg = 7;     // log2 nodes per Merkle workgroup (four threads per node)

@@ -191,7 +191,7 @@ void Server::enqueue_request(Group& g, Request& r) {
   HashPlan hp = hash_plan(key, tu);
   const uint32_t chunks = hp.variant == 1 ? serve_chunks(key.log_m, key.log_n, host) : 1;
   if (chunks > 1) {  // one word plane per split, and whole splits per chunk
-    if (!hash_impl_reduces(hp.impl)) hp.impl = 16;
+    if (!hash_impl_reduces(hp.impl)) hp.impl = Tuning{}.hash_impl;
     if (hp.ks < chunks) hp.ks = chunks;
   }
   c.enc = DevBuf<uint64_t>(&ctx.pool, m * two_n);
