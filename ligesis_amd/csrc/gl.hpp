@@ -31,6 +31,17 @@ LGS_HD uint64_t gl_add(uint64_t a, uint64_t b) {
   return (c1 | c3) ? (((uint64_t)th << 32) | tl) : (((uint64_t)sh << 32) | sl);
 }
 
+// gl_add as one 64-bit add, two 64-bit compares and a 64-bit add of 0 or 2^32 - 1
+// (v_lshl_add_u64 / v_cmp_lt_u64 on gfx950): s = a + b wraps exactly when the true sum reaches
+// 2^64, and then, as when s >= p, s + 2^32 - 1 = s - p mod 2^64 is the reduced sum. One VALU
+// instruction fewer than the carry chains where the zero high word stays in a register (the row
+// transforms' butterflies).
+LGS_HD uint64_t gl_add64(uint64_t a, uint64_t b) {
+  const uint64_t s = a + b;
+  const uint32_t m = ((s < a) | (s >= GL_P)) ? 0xffffffffu : 0u;
+  return s + (uint64_t)m;
+}
+
 LGS_HD uint64_t gl_sub(uint64_t a, uint64_t b) {
   // a - b, and on a borrow a - b + 2^64 - (2^32 - 1) = a - b + p
   unsigned b0, b1, b2, b3;
@@ -76,10 +87,11 @@ LGS_HD uint64_t gl_mul(uint64_t a, uint64_t b) {
 // Montgomery form (w = c 2^64 mod p gives a c). Montgomery reduction of T = hi 2^64 + lo:
 // p^-1 = 1 + 2^32 (mod 2^64), so m = lo (1 + 2^32) mod 2^64 makes T - m p divisible by 2^64, and
 // (T - m p) / 2^64 = hi - k with k = floor(m p / 2^64) = m - (m >> 32) - e, e the carry out of
-// lo + (lo << 32) (e = 1 exactly when the high half of m is below its low half). k < p, so
-// hi - k lies in (-p, 2^64): add p on a borrow, else subtract p once if needed. On 32-bit
-// halves: one add, two two-word subtractions, a conditional -(2^32 - 1) and a compare -- against
-// the 128-bit reduction's two 64-bit corrections, 32 x 32 product and final compare.
+// lo + (lo << 32) (e = 1 exactly when the high half of m is below its low half). T < 2^64 p
+// gives hi < p, and k < p, so hi - k lies in (-p, p): the result is hi - k, plus p on a borrow,
+// and needs no final compare. On 32-bit halves: one add, two two-word subtractions and a
+// conditional -(2^32 - 1) -- against the 128-bit reduction's two 64-bit corrections, 32 x 32
+// product and final compare.
 LGS_HD uint64_t gl_mul_mont(uint64_t a, uint64_t w) {
   uint64_t lo, hi;
   mul_wide(a, w, lo, hi);
@@ -93,7 +105,7 @@ LGS_HD uint64_t gl_mul_mont(uint64_t a, uint64_t w) {
   // a borrow: r + 2^64 - (2^32 - 1) = hi - k + p, in (0, p)
   rl = __builtin_subc(rl, d1 ? 0xffffffffu : 0u, 0u, &f0);
   rh = __builtin_subc(rh, 0u, f0, &f1);
-  return gl_canon(((uint64_t)rh << 32) | rl);
+  return ((uint64_t)rh << 32) | rl;
 }
 
 // x * 2^-K mod p for canonical x and 1 <= K <= 32 (K a constant where this is inlined). Since
@@ -106,6 +118,31 @@ LGS_HD uint64_t gl_mul_inv_pow2(uint64_t x, int K) {
   const uint64_t u = x + t;
   const uint64_t a = (u >> K) | (t << (64 - K));
   return gl_canon(a - (t << (32 - K)));
+}
+
+// x * 2^e mod p for canonical x and 32 < e < 64 (e a constant where this is inlined), without
+// a 128-bit reduction. With x = x1 2^32 + x0 and 2^(32+e) = 2^(e-32) 2^64 = 2^e - 2^(e-32):
+// x 2^e = s 2^e - x1 2^(e-32), s = x0 + x1 = s0 + c 2^32. Splitting s0 at bit 64 - e,
+// s0 2^e = (s0 mod 2^(64-e)) 2^e + (s0 >> (64-e)) (2^32 - 1), and c 2^(32+e) = c (2^e - 2^(e-32)),
+// so x 2^e = (rotl(s0, e - 32) + c 2^(e-32)) 2^32 - D with D = (s0 >> (64-e)) + (x1 + c) 2^(e-32)
+// < 2^64 (x1 + c cannot wrap: x1 = 2^32 - 1 only with x0 = 0 for x < p). A carry k out of the
+// high word is k 2^64 = k (2^32 - 1); a borrow of the final subtraction adds p. The high word is
+// below 2^32, so the result is canonical without a compare.
+LGS_HD uint64_t gl_mul_pow2_mid(uint64_t x, uint32_t e) {
+  const unsigned x0 = (unsigned)x, x1 = (unsigned)(x >> 32);
+  unsigned c, k, b0, b1, f0, f1;
+  const unsigned s0 = __builtin_addc(x0, x1, 0u, &c);
+  const unsigned xc = x1 + c;
+  const unsigned rot = (s0 << (e - 32)) | (s0 >> (64 - e));
+  const unsigned h = __builtin_addc(rot, c << (e - 32), 0u, &k);
+  const unsigned dlo = (s0 >> (64 - e)) | (xc << (e - 32));
+  const unsigned dhi = xc >> (64 - e);
+  unsigned rl = __builtin_subc(k ? 0xffffffffu : 0u, dlo, 0u, &b0);
+  unsigned rh = __builtin_subc(h, dhi, b0, &b1);
+  // a borrow: r + 2^64 - (2^32 - 1) = r + p
+  rl = __builtin_subc(rl, b1 ? 0xffffffffu : 0u, 0u, &f0);
+  rh = __builtin_subc(rh, 0u, f0, &f1);
+  return ((uint64_t)rh << 32) | rl;
 }
 
 LGS_HD uint64_t gl_pow(uint64_t a, uint64_t e) {

@@ -79,8 +79,9 @@ __device__ __forceinline__ uint64_t gl_reduce_small(uint64_t lo, uint64_t hi) {
 }
 
 // x * 2^e mod p for 0 <= e < 192 (e is a compile-time constant where this is inlined), by the
-// cheapest exact form: 2^96 = -1 gives 2^e = -2^(e - 96), and for 64 <= e < 96,
-// 2^e = -2^-(96 - e), a division by at most 2^32 (gl_mul_inv_pow2) instead of two reductions.
+// cheapest exact form: 2^96 = -1 gives 2^e = -2^(e - 96); for 64 <= e < 96, 2^e = -2^-(96 - e),
+// a division by at most 2^32 (gl_mul_inv_pow2); for 32 < e < 64 a word rotation and one
+// two-word subtraction (gl_mul_pow2_mid); no 128-bit reduction in any case.
 __device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x, uint32_t e) {
   bool neg = e >= 96;
   if (neg) e -= 96;
@@ -90,7 +91,7 @@ __device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x, uint32_t e) {
   } else if (e <= 32) {
     r = gl_reduce_small(x << e, x >> (64 - e));
   } else if (e < 64) {
-    r = gl_reduce(x << e, x >> (64 - e));
+    r = gl_mul_pow2_mid(x, e);
   } else {
     r = gl_mul_inv_pow2(x, 96 - e);
     neg = !neg;
@@ -114,34 +115,34 @@ __device__ __forceinline__ void dft8(uint64_t (&x)[8]) {
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const uint64_t a = x[2 * u], b = x[2 * u + 1];
-    x[2 * u] = ODD_ZERO ? a : gl_add(a, b);
+    x[2 * u] = ODD_ZERO ? a : gl_add64(a, b);
     x[2 * u + 1] = ODD_ZERO ? a : gl_sub(a, b);
   }
 #pragma unroll
   for (int h = 0; h < 8; h += 4) {
     uint64_t a = x[h], b = x[h + 2];
-    x[h] = gl_add(a, b);
+    x[h] = gl_add64(a, b);
     x[h + 2] = gl_sub(a, b);
     a = x[h + 1];
     b = gl_mul_pow2(x[h + 3], 48);  // w_4
-    x[h + 1] = gl_add(a, b);
+    x[h + 1] = gl_add64(a, b);
     x[h + 3] = gl_sub(a, b);
   }
   {
     uint64_t a = x[0], b = x[4];
-    x[0] = gl_add(a, b);
+    x[0] = gl_add64(a, b);
     x[4] = gl_sub(a, b);
     a = x[1];
     b = gl_mul_pow2(x[5], 24);  // w_8 = 2^120 = -2^24: the sign swaps the butterfly's outputs
     x[1] = gl_sub(a, b);
-    x[5] = gl_add(a, b);
+    x[5] = gl_add64(a, b);
     a = x[2];
     b = gl_mul_pow2(x[6], 48);  // w_8^2
-    x[2] = gl_add(a, b);
+    x[2] = gl_add64(a, b);
     x[6] = gl_sub(a, b);
     a = x[3];
     b = gl_mul_pow2(x[7], 168);  // w_8^3 = 2^-24
-    x[3] = gl_add(a, b);
+    x[3] = gl_add64(a, b);
     x[7] = gl_sub(a, b);
   }
 }
@@ -328,13 +329,20 @@ static uint32_t pass_twiddle8_entries(int L) {
   return n;
 }
 
+// LDS word at byte address b (a word's position times 8): the fixed-shape kernels keep a group's
+// base position and its offsets as byte addresses, so an access is one XOR (or none, where the
+// compiler folds an offset into the instruction) instead of an XOR and a shift.
+__device__ __forceinline__ uint64_t& lds_word(uint64_t* sm, uint32_t b) {
+  return *(uint64_t*)((char*)sm + b);
+}
+
 // The row LDE of rows of 2^(L-1) values (row stride rs, unit element stride) zero-padded to
 // 2^L points, one workgroup of 2^(L-3) threads per row: ntt_r8_row's zero-half schedule (stages
 // 2-4 fused with the load, then radix-8 passes from s = 4) with every size a compile-time
 // constant, one butterfly group per thread and pass, Montgomery-form twiddles from the pass
 // table tp (one Montgomery reduction per product instead of a 128-bit one), and each
-// LDS address one XOR: lds_pos is linear over GF(2), and a group's base and its offsets t 2^s
-// occupy disjoint bits, so lds_pos(base + t 2^s) = lds_pos(base) ^ lds_pos(t 2^s).
+// LDS address one XOR of byte addresses: lds_pos is linear over GF(2), and a group's base and its
+// offsets t 2^s occupy disjoint bits, so lds_pos(base + t 2^s) = lds_pos(base) ^ lds_pos(t 2^s).
 template <int L>
 __global__ __launch_bounds__(1 << (L - 3)) void ntt_r8_fixed_kernel(const uint64_t* __restrict__ in, uint64_t rs,
                                                                    const uint64_t* __restrict__ tp,
@@ -356,9 +364,9 @@ __global__ __launch_bounds__(1 << (L - 3)) void ntt_r8_fixed_kernel(const uint64
       for (int t = 1; t < 8; t++) x[t] = gl_mul_pow2(x[t], (156u * (__brev((uint32_t)t) >> 29)) % 192u);
     }
     dft8(x);
-    const uint32_t pb = lds_pos(16 * brev_bits(k, L - 4) + j, sh);
+    const uint32_t pb = lds_pos(16 * brev_bits(k, L - 4) + j, sh) * 8;
 #pragma unroll
-    for (int t = 0; t < 8; t++) sm[pb ^ lds_pos(2 * t, sh)] = x[t];
+    for (int t = 0; t < 8; t++) lds_word(sm, pb ^ (lds_pos(2 * t, sh) * 8)) = x[t];
   }
   __syncthreads();
   uint32_t off = 0;
@@ -367,10 +375,10 @@ __global__ __launch_bounds__(1 << (L - 3)) void ntt_r8_fixed_kernel(const uint64
     const uint32_t M = 1u << s;
     const uint32_t j = gi & (M - 1);
     const uint32_t base = ((gi >> s) << (s + 3)) | j;
-    const uint32_t pb = lds_pos(base, sh);
+    const uint32_t pb = lds_pos(base, sh) * 8;
     uint64_t x[8];
 #pragma unroll
-    for (int t = 0; t < 8; t++) x[t] = sm[pb ^ lds_pos(t << s, sh)];
+    for (int t = 0; t < 8; t++) x[t] = lds_word(sm, pb ^ (lds_pos(t << s, sh) * 8));
 #pragma unroll
     for (int t = 1; t < 8; t++) x[t] = gl_mul_mont(x[t], tp[off + (t - 1) * M + j]);
     dft8(x);
@@ -379,7 +387,7 @@ __global__ __launch_bounds__(1 << (L - 3)) void ntt_r8_fixed_kernel(const uint64
       for (int t = 0; t < 8; t++) dst[base + t * M] = x[t];
     } else {
 #pragma unroll
-      for (int t = 0; t < 8; t++) sm[pb ^ lds_pos(t << s, sh)] = x[t];
+      for (int t = 0; t < 8; t++) lds_word(sm, pb ^ (lds_pos(t << s, sh) * 8)) = x[t];
       __syncthreads();
     }
     off += 7 * M;
@@ -398,9 +406,9 @@ __device__ __forceinline__ void bfly_pow2(uint64_t& a, uint64_t& b) {
   uint64_t c;
   if constexpr (e == 0) c = b;
   else if constexpr (e <= 32) c = gl_reduce_small(b << e, b >> (64 - e));
-  else if constexpr (e < 64) c = gl_reduce(b << e, b >> (64 - e));
+  else if constexpr (e < 64) c = gl_mul_pow2_mid(b, e);
   else c = gl_mul_inv_pow2(b, 96 - e);
-  const uint64_t s = gl_add(a, c), d = gl_sub(a, c);
+  const uint64_t s = gl_add64(a, c), d = gl_sub(a, c);
   a = neg ? d : s;
   b = neg ? s : d;
 }
@@ -511,9 +519,9 @@ __global__ __launch_bounds__(1 << (L - 4)) void ntt_r16_fixed_kernel(const uint6
       x[15] = gl_mul_pow2(x[15], 78u * 15 % 192);
     }
     dft16(x);
-    const uint32_t pb = lds_pos(32 * brev_bits(k, L - 5) + j, sh);
+    const uint32_t pb = lds_pos(32 * brev_bits(k, L - 5) + j, sh) * 8;
 #pragma unroll
-    for (int t = 0; t < 16; t++) sm[pb ^ lds_pos(2 * t, sh)] = x[t];
+    for (int t = 0; t < 16; t++) lds_word(sm, pb ^ (lds_pos(2 * t, sh) * 8)) = x[t];
   }
   __syncthreads();
   uint32_t off = 0;
@@ -522,10 +530,10 @@ __global__ __launch_bounds__(1 << (L - 4)) void ntt_r16_fixed_kernel(const uint6
     const uint32_t M = 1u << s;
     const uint32_t j = gi & (M - 1);
     const uint32_t base = ((gi >> s) << (s + 4)) | j;
-    const uint32_t pb = lds_pos(base, sh);
+    const uint32_t pb = lds_pos(base, sh) * 8;
     uint64_t x[16];
 #pragma unroll
-    for (int t = 0; t < 16; t++) x[t] = sm[pb ^ lds_pos(t << s, sh)];
+    for (int t = 0; t < 16; t++) x[t] = lds_word(sm, pb ^ (lds_pos(t << s, sh) * 8));
 #pragma unroll
     for (int t = 1; t < 16; t++) x[t] = gl_mul_mont(x[t], tp[off + (t - 1) * M + j]);
     dft16(x);
@@ -534,7 +542,7 @@ __global__ __launch_bounds__(1 << (L - 4)) void ntt_r16_fixed_kernel(const uint6
       for (int t = 0; t < 16; t++) dst[base + t * M] = x[t];
     } else {
 #pragma unroll
-      for (int t = 0; t < 16; t++) sm[pb ^ lds_pos(t << s, sh)] = x[t];
+      for (int t = 0; t < 16; t++) lds_word(sm, pb ^ (lds_pos(t << s, sh) * 8)) = x[t];
       __syncthreads();
     }
     off += 15 * M;
