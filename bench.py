@@ -600,10 +600,54 @@ def measure_host_commit(pk, f_host: np.ndarray, steps: int = 5) -> dict:
     return out
 
 
+def measure_graph_commit(pk, f_host: np.ndarray, steps: int = 20) -> dict:
+    """The commit without allocation (lgs_commit_noalloc) into caller-owned buffers, issued
+    directly and replayed from a HIP graph captured on a caller stream: per commit with the root
+    read back every time, and back to back (the roots read after the last one)."""
+    import torch
+
+    import ligesis_amd as L
+
+    m, n = 1 << pk.log_m, 1 << pk.log_n
+    f = torch.from_numpy(f_host.view(np.int64)).to("cuda:0")
+    enc = torch.empty(m * 2 * n, dtype=torch.int64, device="cuda:0")
+    h = torch.empty(32 * 2 * n, dtype=torch.int64, device="cuda:0")
+    tree = torch.empty((4 * n - 1) * 32, dtype=torch.uint8, device="cuda:0")
+    s = torch.cuda.Stream()
+    root = L.Ligesis.commit_noalloc(pk, f, enc, h, tree, stream=s)
+    settle(lambda: L.Ligesis.commit_noalloc(pk, f, enc, h, tree, stream=s))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.Ligesis.commit_noalloc(pk, f, enc, h, tree, stream=s)
+    direct_ms = (time.perf_counter() - t0) / steps * 1e3
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        L.Ligesis.commit_noalloc(pk, f, enc, h, tree, stream=s, want_root=False)
+    g.replay()
+    torch.cuda.synchronize()
+    assert tree[-32:].cpu().numpy().tobytes() == root, "graph replay differs from the direct commit"
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+        got = tree[-32:].cpu()  # the root, waited for
+    replay_ms = (time.perf_counter() - t0) / steps * 1e3
+    assert got.numpy().tobytes() == root
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    back_ms = (time.perf_counter() - t0) / steps * 1e3
+    del g
+    return {"direct_ms": direct_ms, "graph_replay_ms": replay_ms, "graph_back_to_back_ms": back_ms,
+            "graph_back_to_back_melem_per_s": m * n / (back_ms / 1e3) / 1e6}
+
+
 def bench_extras(args):
     """The rest of the commit metric, outside the headline's timed region: the same commit from
-    host memory, the mu = 24 eval-prove at 2^8 x 2^16 and 2^7 x 2^17, and the served mixed
-    stream's latency. One JSON object on stdout."""
+    host memory, the commit without allocation issued directly and from a HIP graph, the mu = 24
+    eval-prove at 2^8 x 2^16 and 2^7 x 2^17, and the served mixed stream's latency. One JSON
+    object on stdout."""
     import torch  # noqa: F401  (before the library: torch's HIP runtime must be loaded first)
 
     import ligesis_amd as L
@@ -616,6 +660,8 @@ def bench_extras(args):
     print("bench: commit from host memory (PCIe upload inside the call)", file=sys.stderr, flush=True)
     settle(lambda: L.Ligesis.commit(pk, inp.f)[1].release())
     out["host_resident_commit"] = measure_host_commit(pk, inp.f)
+    print("bench: commit without allocation, direct and replayed from a HIP graph", file=sys.stderr, flush=True)
+    out["noalloc_commit"] = measure_graph_commit(pk, inp.f)
     del pk
     ctx.close()
     out["prove_mu24"] = [measure_prove(lm, steps=10, warmup=2, profile=False) for lm in (8, 7)]

@@ -65,3 +65,58 @@ def test_noalloc_rejects_shapes_beyond_the_scratch(gpu_ctx):
         Ligesis.commit_noalloc(pk, f, enc, h, tree)
     assert e.value.code == LGS_ERR_ARG
     pk.release()
+
+
+def test_noalloc_captured_in_a_graph(gpu_ctx):
+    """lgs_commit_noalloc allocates nothing and synchronizes nothing without a root, so a caller can
+    capture it into a HIP graph (here torch.cuda.CUDAGraph on a caller stream) and replay it with
+    new polynomials written into the captured input buffer. Each of four replays (inputs f1, f2,
+    f1, f2), with eager commits and synchronizations between them, must equal lgs_commit on the
+    same input."""
+    lm, ln = 8, 10
+    m, n = 1 << lm, 1 << ln
+    inp1 = synth.Inputs(lm, ln, seed=31, n_fields=4, n_ints=4)
+    inp2 = synth.Inputs(lm, ln, seed=32, n_fields=4, n_ints=4)
+    pk, _ = Ligesis.setup(lm, ln, inp1.a, ctx=gpu_ctx, commit_only=True)
+    want = []
+    for inp in (inp1, inp2):
+        root, cd = Ligesis.commit(pk, inp.f)
+        want.append((root, cd.encoded(), cd.digests(), cd.tree()))
+        cd.release()
+    f = torch.empty(m * n, dtype=torch.int64, device="cuda:0")
+    f_src = [torch.from_numpy(inp.f.view(np.int64)).to("cuda:0") for inp in (inp1, inp2)]
+    enc, h, tree = _outputs(lm, ln)
+    s = torch.cuda.Stream()
+    f.copy_(f_src[0])
+    torch.cuda.synchronize()
+    # one eager call on the stream first (it fills the library's per-kernel launch tables)
+    Ligesis.commit_noalloc(pk, f, enc, h, tree, stream=s, want_root=False)
+    s.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        Ligesis.commit_noalloc(pk, f, enc, h, tree, stream=s, want_root=False)
+    other = synth.Inputs(6, 9, seed=33, n_fields=4, n_ints=4)
+    opk, _ = Ligesis.setup(6, 9, other.a, ctx=gpu_ctx, commit_only=True)
+    for rep, which in enumerate((0, 1, 0, 1)):
+        print(f"graph replay {rep} input {which}", flush=True)
+        f.copy_(f_src[which])
+        enc.fill_(-1)
+        h.fill_(-1)
+        tree.fill_(0xA5)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        root, e, d, t = want[which]
+        assert tree[-32:].cpu().numpy().tobytes() == root, rep
+        np.testing.assert_array_equal(enc.cpu().numpy().view(np.uint64).reshape(m, 2 * n), e)
+        np.testing.assert_array_equal(h.cpu().numpy().view(np.uint64).reshape(32, 2 * n), d)
+        assert tree.cpu().numpy().tobytes() == t
+        # eager work between the replays: a commit of another shape through the same scratch
+        e2, h2, t2 = _outputs(6, 9)
+        r2 = Ligesis.commit_noalloc(opk, torch.from_numpy(other.f.view(np.int64)).to("cuda:0"), e2, h2, t2)
+        r3, cd3 = Ligesis.commit(opk, other.f)
+        assert r2 == r3
+        cd3.release()
+    del g
+    opk.release()
+    pk.release()
