@@ -72,19 +72,19 @@ const char* lgs_last_error(const lgs_ctx* ctx);
  * fragment read from LDS one product group ahead, 11 / 12 A digits and F' words copied straight
  * into an LDS ring of 3 / 4 slabs, 14 as 12 with 8 waves and 256 x 256 tiles sharing one ring,
  * 15 / 16 as 14 with a 4 / 8-slab ring, each slab's bit expansion, A reads and copies
- * interleaved with the previous slab's products; shapes a kernel does not fit take the next
- * one down),
+ * interleaved with the previous slab's products, 17 as 16 with one barrier per two slabs;
+ * shapes a kernel does not fit take the next one down),
  * "ntt_impl" (-1 auto; 0/2/3 radix-8 passes with 256/512/1024 threads per row, 1 radix-2
- * stages, 4 for rows of 2^12 values a 2^13-point kernel with every size fixed at compile time
- * and per-pass twiddle tables, elsewhere as 3), "merkle_lg" (0 auto; 6..8 = log2 nodes per Merkle workgroup = tree levels per
+ * stages, 4 / 5 for rows of 2^12 values a 2^13-point radix-8 / radix-16 kernel with every size
+ * fixed at compile time and per-pass twiddle tables, elsewhere as 3), "merkle_lg" (0 auto; 6..8 = log2 nodes per Merkle workgroup = tree levels per
  * launch), "pipeline" (0 auto; 1 off; 2 / 4 / 8: the rows are encoded in that many chunks and
  * each chunk's K-splits are hashed on a second stream while the next chunk is encoded),
  * "prove_graph" (0 default; 1: the second proof of a commitment on its own context captures
  * its launches into a HIP graph, kept with the commitment, and later proofs replay it with their
  * own constants), "profile" (1: time every stage with HIP events; pipelined stages report the
  * summed durations of their launches; proofs are then issued without a graph; 2: as 1, except
- * that a plain commit records only the two events around its hash kernel and reports stage [1]
- * alone). An option left
+ * that a plain commit times only its hash kernel and reports stage [1] alone: the MFMA kernel
+ * from its dispatch's own start and stop times, the VALU hash from events around it). An option left
  * at auto takes the autotuned schedule of the commit's shape when the process tuning table has
  * one (lgs_autotune, lgs_tune_load), else a built-in default. */
 int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value);

@@ -95,3 +95,29 @@ def test_pipelined_commit_matches_oracle(gpu_ctx, shape, pipeline):
     np.testing.assert_array_equal(cd.encoded(), ocd.enc())
     np.testing.assert_array_equal(cd.digests(), ocd.h())
     assert all(v >= 0 for v in st.values()) and st["commit_ms"] > 0
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_hash_kernel_timing_matches_oracle(gpu_ctx, variant):
+    """Profile level 2 (the timed region of the commit benchmark) times only the hash kernel,
+    from the MFMA kernel dispatch's own start and stop times (the VALU hash from events around
+    it); the commit is unchanged and the time is positive, the other stages zero."""
+    log_m, log_n = 4, 10
+    print(f"profile 2 commit hash_variant={variant}", flush=True)
+    inp = synth.Inputs(log_m, log_n, seed=81)
+    pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx, commit_only=True)
+    gpu_ctx.set_option("hash", variant)
+    gpu_ctx.set_option("profile", 2)
+    try:
+        times = []
+        for _ in range(3):
+            root, cd = Ligesis.commit(pk, inp.f)
+            times.append(gpu_ctx.stage_times())
+    finally:
+        gpu_ctx.set_option("hash", 0)
+        gpu_ctx.set_option("profile", 0)
+    ocd = oracle.CommitData(oracle.ProvingKey(log_m, log_n, inp.a, commit_a=False), inp.f, keep_enc=True)
+    assert root == ocd.root
+    np.testing.assert_array_equal(cd.digests(), ocd.h())
+    for st in times:
+        assert st["hash_ms"] > 0 and st["ntt_ms"] == 0 and st["merkle_ms"] == 0
