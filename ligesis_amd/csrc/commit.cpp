@@ -424,11 +424,14 @@ void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, 
                           GroupHold& hold) {
   if (items.empty()) return;
   ctx.upload_reset();  // every earlier call on this context has been synchronized
-  std::vector<NttJob> nj;
+  // rows of 2^12 values (2^13-point transforms) go to the compile-time-shaped kernel, the rest
+  // to the runtime-shaped one, sized by the largest transform among them
+  std::vector<NttJob> nj, nj13;
   std::vector<HashJob> hj;
   std::vector<FinalizeJob> fj;
-  uint32_t nblocks = 0, ntiles_total = 0, fblocks = 0;
+  uint32_t nblocks = 0, nblocks13 = 0, ntiles_total = 0, fblocks = 0;
   int max_logN = 1;
+  bool pipe8 = true;  // every job fits the pipelined 8-wave hash body
   // 256-column tiles (8 waves sharing the A ring, one workgroup per CU) when every job has a
   // multiple of 256 columns, else 128
   uint32_t tile_cols = 256;
@@ -445,9 +448,14 @@ void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, 
     c.h = DevBuf<uint64_t>(&ctx.pool, 32 * two_n);
     c.tree.leaves = two_n;
     c.tree.nodes = DevBuf<uint32_t>(&ctx.pool, (size_t)c.tree.node_count() * 8);
-    nj.push_back(NttJob{c.f, n, 1, (uint32_t)n, logN, ctx.twiddle(logN), c.enc.ptr, nblocks});
-    nblocks += (uint32_t)m;
-    max_logN = logN > max_logN ? logN : max_logN;
+    if (logN == 13) {
+      nj13.push_back(NttJob{c.f, n, 1, (uint32_t)n, logN, ctx.twiddle(logN), c.enc.ptr, nblocks13});
+      nblocks13 += (uint32_t)m;
+    } else {
+      nj.push_back(NttJob{c.f, n, 1, (uint32_t)n, logN, ctx.twiddle(logN), c.enc.ptr, nblocks});
+      nblocks += (uint32_t)m;
+      max_logN = logN > max_logN ? logN : max_logN;
+    }
     const uint32_t ntiles = (uint32_t)(two_n / tile_cols);
     const Tuning tu = ctx.resolve(key.log_m, key.log_n);
     uint32_t ks = tu.hash_ksplit ? (uint32_t)tu.hash_ksplit : (uint32_t)(m >= 1024 ? m / 512 : 1);
@@ -456,6 +464,7 @@ void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, 
     hold.parts.emplace_back(&ctx.pool, (size_t)ks * 256 * two_n);
     hj.push_back(HashJob{c.enc.ptr, two_n, key.dig.ptr, (uint32_t)(m / ks), ntiles, (uint32_t)two_n,
                          hold.parts.back().ptr, ntiles_total});
+    if ((m / ks) % 8) pipe8 = false;
     ntiles_total += ntiles * ks;
     fj.push_back(FinalizeJob{hold.parts.back().ptr, ks, (uint32_t)two_n, c.h.ptr, fblocks});
     fblocks += (uint32_t)((32 * two_n + 255) / 256);
@@ -466,13 +475,18 @@ void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, 
     ctx.upload(hold.tables.back().ptr, src, bytes, ctx.stream);
     return (const void*)hold.tables.back().ptr;
   };
-  const NttJob* d_nj = (const NttJob*)upload(nj.data(), nj.size() * sizeof(NttJob));
+  const NttJob* d_nj = nj.empty() ? nullptr : (const NttJob*)upload(nj.data(), nj.size() * sizeof(NttJob));
+  const NttJob* d_nj13 = nj13.empty() ? nullptr : (const NttJob*)upload(nj13.data(), nj13.size() * sizeof(NttJob));
+  const uint64_t* tp13 = nj13.empty() ? nullptr : ctx.pass_twiddle(13, 4);
   const HashJob* d_hj = (const HashJob*)upload(hj.data(), hj.size() * sizeof(HashJob));
   const FinalizeJob* d_fj = (const FinalizeJob*)upload(fj.data(), fj.size() * sizeof(FinalizeJob));
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[0], ctx.stream), "event");
-  hip_check(launch_ntt_group(d_nj, njobs, nblocks, max_logN, ctx.stream), "batched row encode");
+  if (!nj13.empty())
+    hip_check(launch_ntt_fixed_group(d_nj13, (int)nj13.size(), nblocks13, tp13, ctx.stream), "batched row encode");
+  if (!nj.empty()) hip_check(launch_ntt_group(d_nj, (int)nj.size(), nblocks, max_logN, ctx.stream), "batched row encode");
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[1], ctx.stream), "event");
-  hip_check(launch_hash_group(d_hj, njobs, ntiles_total, nullptr, 0, ctx.stream, (int)tile_cols), "batched hash");
+  hip_check(launch_hash_group(d_hj, njobs, ntiles_total, nullptr, 0, ctx.stream, (int)tile_cols, tile_cols == 256 && pipe8),
+            "batched hash");
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[2], ctx.stream), "event");
   hip_check(launch_hash_group(nullptr, njobs, 0, d_fj, fblocks, ctx.stream), "batched finalize");
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[3], ctx.stream), "event");
@@ -483,6 +497,7 @@ void enqueue_commit_group(Context& ctx, const std::vector<std::pair<const Key*, 
   }
   enqueue_merkle_trees(ctx, trees, hold.host_merkle, hold.tables, ctx.stream);
   if (ctx.profile) hip_check(hipEventRecord(ctx.ev[4], ctx.stream), "event");
+  nj.insert(nj.end(), nj13.begin(), nj13.end());
   hold.host_ntt = std::move(nj);
   hold.host_hash = std::move(hj);
   hold.host_fin = std::move(fj);

@@ -815,28 +815,14 @@ __device__ __forceinline__ void dma_pipe_slab(const DmaTile& t, uint32_t sl, Dma
   __builtin_amdgcn_s_waitcnt(kLgkmZero);
 }
 
-template <int NB, bool RING = false, int KS = 1>
-__global__ __launch_bounds__(512, 2) void hash_mfma_pipe8_kernel(const uint64_t* __restrict__ enc, uint64_t ld,
-                                                                const v4i* __restrict__ adig, uint32_t ns,
-                                                                uint32_t ntiles, uint32_t ncols,
-                                                                int32_t* __restrict__ part, uint32_t split0) {
-  static_assert(NB >= 4 && NB <= 8, "4 to 8 slots");
-  __shared__ v4i la0[1024], la1[1024], la2[1024], la3[1024], la4[NB > 4 ? 1024 : 1], la5[NB > 5 ? 1024 : 1],
-      la6[NB > 6 ? 1024 : 1], la7[NB > 7 ? 1024 : 1];
-  __shared__ uint32_t lb0[512], lb1[512], lb2[512], lb3[512], lb4[NB > 4 ? 512 : 1], lb5[NB > 5 ? 512 : 1],
-      lb6[NB > 6 ? 512 : 1], lb7[NB > 7 ? 512 : 1];
-  DmaSlot s[8];
-  s[0] = DmaSlot{la0, lb0};
-  s[1] = DmaSlot{la1, lb1};
-  s[2] = DmaSlot{la2, lb2};
-  s[3] = DmaSlot{la3, lb3};
-  s[4] = DmaSlot{la4, lb4};
-  s[5] = DmaSlot{la5, lb5};
-  s[6] = DmaSlot{la6, lb6};
-  s[7] = DmaSlot{la7, lb7};
-  const uint32_t logical = xcd_logical_block();
-  const uint32_t split = split0 + logical / ntiles;
-  const DmaTile t = make_dma_tile<4>(enc, ld, adig, ns, split, logical % ntiles);
+// One 256 x 256 tile of the pipelined 8-wave kernel over slabs [split * ns, (split + 1) * ns)
+// (ns % NB == 0), slots in s[0 .. NB).
+template <int NB, bool RING, int KS>
+__device__ __forceinline__ void hash_pipe8_body(const uint64_t* __restrict__ enc, uint64_t ld,
+                                                const v4i* __restrict__ adig, uint32_t ns, uint32_t split,
+                                                uint32_t ntile, uint32_t ncols, int32_t* __restrict__ part,
+                                                const DmaSlot (&s)[8]) {
+  const DmaTile t = make_dma_tile<4>(enc, ld, adig, ns, split, ntile);
 
   v4i acc[8][4];
 #pragma unroll
@@ -863,8 +849,46 @@ __global__ __launch_bounds__(512, 2) void hash_mfma_pipe8_kernel(const uint64_t*
         dma_pipe_slab<NB, KS, false>(t, sl + j, s[j], s[(j + 1) % NB], s[(j + NB - KS) % NB], bc, p0, p1, acc);
     }
   }
-  dma_epilogue<RING>(t, acc, (uint64_t*)(t.wn == 0 ? la0 : t.wn == 1 ? la1 : t.wn == 2 ? la2 : la3), split, ncols,
-                    (uint64_t*)part);
+  dma_epilogue<RING>(t, acc, (uint64_t*)(t.wn == 0 ? s[0].a : t.wn == 1 ? s[1].a : t.wn == 2 ? s[2].a : s[3].a),
+                    split, ncols, (uint64_t*)part);
+}
+
+#define LGS_PIPE8_SLOTS                                                                                              \
+  __shared__ v4i la0[1024], la1[1024], la2[1024], la3[1024], la4[NB > 4 ? 1024 : 1], la5[NB > 5 ? 1024 : 1],      \
+      la6[NB > 6 ? 1024 : 1], la7[NB > 7 ? 1024 : 1];                                                                \
+  __shared__ uint32_t lb0[512], lb1[512], lb2[512], lb3[512], lb4[NB > 4 ? 512 : 1], lb5[NB > 5 ? 512 : 1],        \
+      lb6[NB > 6 ? 512 : 1], lb7[NB > 7 ? 512 : 1];                                                                  \
+  const DmaSlot s[8] = {DmaSlot{la0, lb0}, DmaSlot{la1, lb1}, DmaSlot{la2, lb2}, DmaSlot{la3, lb3},                 \
+                        DmaSlot{la4, lb4}, DmaSlot{la5, lb5}, DmaSlot{la6, lb6}, DmaSlot{la7, lb7}};
+
+template <int NB, bool RING = false, int KS = 1>
+__global__ __launch_bounds__(512, 2) void hash_mfma_pipe8_kernel(const uint64_t* __restrict__ enc, uint64_t ld,
+                                                                const v4i* __restrict__ adig, uint32_t ns,
+                                                                uint32_t ntiles, uint32_t ncols,
+                                                                int32_t* __restrict__ part, uint32_t split0) {
+  static_assert(NB >= 4 && NB <= 8, "4 to 8 slots");
+  LGS_PIPE8_SLOTS
+  const uint32_t logical = xcd_logical_block();
+  hash_pipe8_body<NB, RING, KS>(enc, ld, adig, ns, split0 + logical / ntiles, logical % ntiles, ncols, part, s);
+}
+
+// Tiles of many commitments in one launch (batched commit) with the pipelined 8-wave body: every
+// job has a multiple of 256 columns and of 8 slabs per K-split; logical tile -> job by binary
+// search over the jobs' first tiles.
+__global__ __launch_bounds__(512, 2) void hash_mfma_pipe8_group_kernel(const HashJob* __restrict__ jobs, int njobs) {
+  constexpr int NB = 8;
+  LGS_PIPE8_SLOTS
+  const uint32_t logical = xcd_logical_block();
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].first_block <= logical) lo = mid;
+    else hi = mid - 1;
+  }
+  const HashJob& jb = jobs[lo];
+  const uint32_t local = logical - jb.first_block;
+  hash_pipe8_body<NB, false, 2>(jb.enc, jb.ld, (const v4i*)jb.adig, jb.slabs_per_split, local / jb.ntiles,
+                                local % jb.ntiles, jb.ncols, jb.part, s);
 }
 
 __global__ void hash_mfma_finalize_kernel(const int32_t* __restrict__ part, uint32_t ksplit,
@@ -1121,9 +1145,12 @@ __global__ void hash_finalize_group_kernel(const FinalizeJob* __restrict__ jobs,
 }
 
 hipError_t launch_hash_group(const HashJob* jobs, int njobs, uint32_t total_tiles, const FinalizeJob* fjobs,
-                             uint32_t total_fblocks, hipStream_t st, int tile_cols) {
+                             uint32_t total_fblocks, hipStream_t st, int tile_cols, bool pipe8) {
   if (total_tiles) {
-    if (tile_cols == 256)
+    if (tile_cols == 256 && pipe8)
+      hipLaunchKernelGGL(hash_mfma_pipe8_group_kernel, dim3(total_tiles), dim3(512),
+                         full_cu_pad((const void*)hash_mfma_pipe8_group_kernel), st, jobs, njobs);
+    else if (tile_cols == 256)
       hipLaunchKernelGGL((hash_mfma_dma_group_kernel<4>), dim3(total_tiles), dim3(512),
                          full_cu_pad((const void*)hash_mfma_dma_group_kernel<4>), st, jobs,
                          njobs);

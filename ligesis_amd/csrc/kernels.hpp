@@ -91,8 +91,14 @@ hipError_t launch_hash_mfma_finalize(const int32_t* part, uint32_t ksplit, uint3
 
 // Batched launches over job tables that live in device memory.
 hipError_t launch_ntt_group(const NttJob* jobs, int njobs, uint32_t total_blocks, int max_logN, hipStream_t st);
+// Jobs whose rows hold 2^12 values at unit stride (2^13-point transforms), with the compile-time
+// shaped radix-8 kernel and its pass twiddle table tp (ntt_impl 4's).
+hipError_t launch_ntt_fixed_group(const NttJob* jobs, int njobs, uint32_t total_blocks, const uint64_t* tp,
+                                  hipStream_t st);
+// pipe8: the pipelined 8-wave body (every job with 256-column tiles and a multiple of 8 slabs per
+// K-split), else the 4-slot ring body.
 hipError_t launch_hash_group(const HashJob* jobs, int njobs, uint32_t total_tiles, const FinalizeJob* fjobs,
-                             uint32_t total_fblocks, hipStream_t st, int tile_cols = 128);
+                             uint32_t total_fblocks, hipStream_t st, int tile_cols = 128, bool pipe8 = false);
 hipError_t launch_merkle_group(const MerkleJob* jobs, int njobs, uint32_t total_blocks, hipStream_t st);
 
 // merkle.hip: tree over nleaves (a power of two) leaves, leaf p = M[r*rs + p*cs], r < rows.

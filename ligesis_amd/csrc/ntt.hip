@@ -344,15 +344,11 @@ __device__ __forceinline__ uint64_t& lds_word(uint64_t* sm, uint32_t b) {
 // LDS address one XOR of byte addresses: lds_pos is linear over GF(2), and a group's base and its
 // offsets t 2^s occupy disjoint bits, so lds_pos(base + t 2^s) = lds_pos(base) ^ lds_pos(t 2^s).
 template <int L>
-__global__ __launch_bounds__(1 << (L - 3)) void ntt_r8_fixed_kernel(const uint64_t* __restrict__ in, uint64_t rs,
-                                                                   const uint64_t* __restrict__ tp,
-                                                                   uint64_t* __restrict__ out) {
+__device__ __forceinline__ void ntt_r8_fixed_row(const uint64_t* __restrict__ src, const uint64_t* __restrict__ tp,
+                                                 uint64_t* __restrict__ dst, uint64_t* sm) {
   static_assert((L - 1) % 3 == 0 && L >= 10 && L <= LGS_LDS_LOG, "radix-8 passes from s = 4");
   constexpr uint32_t N = 1u << L, E = N / 16;
   constexpr int sh = L - 5;
-  extern __shared__ uint64_t sm[];
-  const uint64_t* __restrict__ src = in + (uint64_t)blockIdx.x * rs;
-  uint64_t* __restrict__ dst = out + ((uint64_t)blockIdx.x << L);
   const uint32_t gi = threadIdx.x;
   {
     const uint32_t k = gi & (E - 1), j = gi / E;  // j uniform per half of the workgroup
@@ -392,6 +388,31 @@ __global__ __launch_bounds__(1 << (L - 3)) void ntt_r8_fixed_kernel(const uint64
     }
     off += 7 * M;
   }
+}
+
+template <int L>
+__global__ __launch_bounds__(1 << (L - 3)) void ntt_r8_fixed_kernel(const uint64_t* __restrict__ in, uint64_t rs,
+                                                                   const uint64_t* __restrict__ tp,
+                                                                   uint64_t* __restrict__ out) {
+  extern __shared__ uint64_t sm[];
+  ntt_r8_fixed_row<L>(in + (uint64_t)blockIdx.x * rs, tp, out + ((uint64_t)blockIdx.x << L), sm);
+}
+
+// The rows of many commitments whose rows hold 2^(L-1) values at unit stride, in one launch
+// (batched commit): block b is row (b - first_block) of the job whose range holds b.
+template <int L>
+__global__ __launch_bounds__(1 << (L - 3)) void ntt_r8_fixed_group_kernel(const NttJob* __restrict__ jobs, int njobs,
+                                                                         const uint64_t* __restrict__ tp) {
+  extern __shared__ uint64_t sm[];
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].first_block <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const NttJob& jb = jobs[lo];
+  const uint64_t row = blockIdx.x - jb.first_block;
+  ntt_r8_fixed_row<L>(jb.in + row * jb.rs, tp, jb.out + (row << L), sm);
 }
 
 // ---------------- radix-16 row transform for rows of 2^(L-1) values (L = 1 mod 4) ----------------
@@ -581,6 +602,14 @@ __global__ __launch_bounds__(256) void ntt_r8_group_kernel(const NttJob* __restr
   const NttJob& jb = jobs[lo];
   const uint64_t row = blockIdx.x - jb.first_block;
   ntt_r8_row(jb.in + row * jb.rs, jb.es, jb.n_in, jb.logN, jb.tw, 0, jb.out + (row << jb.logN), sm);
+}
+
+hipError_t launch_ntt_fixed_group(const NttJob* jobs, int njobs, uint32_t total_blocks, const uint64_t* tp,
+                                  hipStream_t st) {
+  if (total_blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(ntt_r8_fixed_group_kernel<13>, dim3(total_blocks), dim3(1024), ntt_lds_bytes(1u << 13), st, jobs,
+                     njobs, tp);
+  return hipGetLastError();
 }
 
 hipError_t launch_ntt_group(const NttJob* jobs, int njobs, uint32_t total_blocks, int max_logN, hipStream_t st) {
