@@ -124,12 +124,11 @@ int lgs_init(int device, lgs_ctx** out) {
     delete ctx;
     return LGS_ERR_DEVICE;
   }
-  for (auto& e : ctx->c.ev)
-    if (hipEventCreate(&e) != hipSuccess) {
-      lgs::teardown_context(ctx->c);
-      delete ctx;
-      return LGS_ERR_DEVICE;
-    }
+  if (ctx->c.create_events() != hipSuccess) {
+    lgs::teardown_context(ctx->c);
+    delete ctx;
+    return LGS_ERR_DEVICE;
+  }
   *out = ctx;
   return LGS_OK;
 }

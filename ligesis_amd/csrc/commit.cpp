@@ -60,8 +60,16 @@ hipEvent_t Context::pipe_event(int i) {
 }
 
 hipEvent_t Context::timing_event(int i) {
-  if (!tev[i]) hip_check(hipEventCreate(&tev[i]), "timing event");
+  if (!tev[i]) hip_check(hipEventCreateWithFlags(&tev[i], hipEventDisableSystemFence), "timing event");
   return tev[i];
+}
+
+hipError_t Context::create_events() {
+  for (int i = 0; i < 10; i++) {
+    const hipError_t e = hipEventCreateWithFlags(&ev[i], i < 5 ? hipEventDisableSystemFence : hipEventDefault);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 uint8_t* Context::staging(size_t bytes) {
@@ -117,7 +125,7 @@ std::unique_ptr<Context> make_context(int device, bool two_streams) {
   hip_check(hipSetDevice(device), "set device");
   hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
   if (two_streams) (void)c->side_stream();
-  for (auto& e : c->ev) hip_check(hipEventCreate(&e), "event");
+  hip_check(c->create_events(), "event");
   return c;
 }
 

@@ -253,7 +253,14 @@ struct Context {
   // the device has hardware queues, so no stream shares a queue with an upload)
   bool single_stream_prove = false;
   double stage_ms[28] = {0};
+  // ev[0..4]: stage timestamps of a profiled commit (and the autotuner's); ev[5..9]: the
+  // prove's cross-stream fork / join. Created by create_events.
   hipEvent_t ev[10] = {};
+  // Timing events carry no system-scope fence: recording one with the fence writes back the
+  // L2 caches, which stalled the next kernel's start by 5-7 us after a kernel that wrote tens
+  // of MiB (a visible share of a 0.5 ms commit). They order nothing; the fork / join events
+  // keep the default fence.
+  hipError_t create_events();
   uint8_t* pinned = nullptr;  // small pinned staging buffer for roots
   size_t pinned_size = 0;
   // Pinned staging of the job tables and constants a batched commit or a prove uploads.
