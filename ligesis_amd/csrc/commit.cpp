@@ -352,20 +352,29 @@ void enqueue_commit_stages(Context& ctx, const Key& key, const CommitIO& io, con
   hip_check(launch_ntt(io.f, n, 1, (uint32_t)n, (uint32_t)m, logN, io.tw, io.enc, st, tu.ntt_impl, io.tp),
             "row encode");
   if (profile && !kernel_only) hip_check(hipEventRecord(ctx.ev[1], st), "event");
-  // H = A . B with B read bitwise from F' (ligesis.rs:292-302)
+  // H = A . B with B read bitwise from F' (ligesis.rs:292-302). Kernels that leave one reduced
+  // word per (split, row, column) get the K-split sum fused into the tree's first launch (the
+  // finalize stage then times nothing); the others sum in a finalize kernel of their own.
+  const bool fused = hp.variant == 2 || hash_impl_reduces(hp.impl);
+  uint64_t* h_fin = fused ? nullptr : io.h;
   if (hp.variant == 1) {
     hip_check(launch_hash_mfma(io.enc, two_n, (uint32_t)m, (uint32_t)two_n, key.dig.ptr, hp.ks, (int32_t*)io.part,
-                               io.h, st, every ? ctx.ev[2] : nullptr, hp.impl, key.ring64,
+                               h_fin, st, every ? ctx.ev[2] : nullptr, hp.impl, key.ring64,
                                kernel_only ? ctx.ev[1] : nullptr, kernel_only ? ctx.ev[2] : nullptr),
               "hash (MFMA)");
   } else {
-    hip_check(launch_hash_valu(io.enc, two_n, (uint32_t)m, (uint32_t)two_n, key.at.ptr, hp.ks, io.part, io.h, st,
+    hip_check(launch_hash_valu(io.enc, two_n, (uint32_t)m, (uint32_t)two_n, key.at.ptr, hp.ks, io.part, h_fin, st,
                                profile ? ctx.ev[2] : nullptr),
               "hash (VALU)");
   }
   if (every) hip_check(hipEventRecord(ctx.ev[3], st), "event");
   // column-wise Merkle tree over serialize(H[:, j]) (ligesis.rs:306-313)
-  hip_check(launch_merkle(io.h, 32, two_n, 1, two_n, io.nodes, st, tu.merkle_lg, io.root_host), "merkle");
+  if (fused)
+    hip_check(launch_commit_tree(io.part, hp.ks, (uint32_t)two_n, key.ring64, io.h, io.nodes, st, tu.merkle_lg,
+                                 io.root_host),
+              "hash sum and merkle");
+  else
+    hip_check(launch_merkle(io.h, 32, two_n, 1, two_n, io.nodes, st, tu.merkle_lg, io.root_host), "merkle");
   if (every) hip_check(hipEventRecord(ctx.ev[4], st), "event");
 }
 

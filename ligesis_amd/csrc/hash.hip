@@ -962,6 +962,7 @@ hipError_t launch_hash_valu(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (mid && (e = hipEventRecord(mid, st)) != hipSuccess) return e;
+  if (!h) return hipSuccess;
   uint64_t n = (uint64_t)32 * ncols;
   hipLaunchKernelGGL(hash_valu_finalize_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
                      part, ksplit, ncols, h, 0);
@@ -1124,6 +1125,7 @@ hipError_t launch_hash_mfma(const uint64_t* enc, uint64_t ld, uint32_t m_rows, u
   hipError_t e = launch_hash_mfma_splits(enc, ld, m_rows, ncols, dig, ksplit, 0, ksplit, part, st, impl, ring, k0, k1);
   if (e != hipSuccess) return e;
   if (mid && (e = hipEventRecord(mid, st)) != hipSuccess) return e;
+  if (!h) return hipSuccess;
   return launch_hash_mfma_finalize(part, ksplit, ncols, h, st, impl, ring);
 }
 

@@ -65,7 +65,8 @@ uint32_t pass_twiddle_entries(int L, int impl);
 hipError_t launch_pass_twiddles(uint64_t* tp, const uint64_t* tw, int L, int impl, hipStream_t st);
 
 // hash.hip (`mid`, when not null, is recorded between the main kernel and the finalize; k0 / k1,
-// when not null, receive the main kernel's own start and stop times).
+// when not null, receive the main kernel's own start and stop times; h null: the partials only,
+// no finalize).
 // ring: the Z_2^64 subset-sum hash (sums mod 2^64 instead of mod p): A's digits represent A mod
 // 2^64, and only the MFMA kernels that reduce in their epilogue (11, 12, 14-17) take it.
 hipError_t launch_a_layouts(const uint64_t* a, uint64_t W, uint64_t* at, int8_t* dig, hipStream_t st,
@@ -102,11 +103,16 @@ hipError_t launch_hash_group(const HashJob* jobs, int njobs, uint32_t total_tile
 hipError_t launch_merkle_group(const MerkleJob* jobs, int njobs, uint32_t total_blocks, hipStream_t st);
 
 // merkle.hip: tree over nleaves (a power of two) leaves, leaf p = M[r*rs + p*cs], r < rows.
-// lg_block: log2 threads per workgroup (6..8) = levels reduced per launch.
+// lg_block: log2 nodes per workgroup (6..8; 4 threads per node) = levels reduced per launch.
 // root_out (optional, device-accessible, e.g. pinned host memory) receives the root's 8 words
 // from the launch that computes it.
 hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs,
                          uint64_t nleaves, uint32_t* nodes, hipStream_t st, int lg_block = 7,
                          uint32_t* root_out = nullptr);
+// A commit's finalize and column tree together: H = the sum of the ksplit partials that the
+// reducing hash kernels leave (one word per split, row and column; mod 2^64 when ring), written
+// to h, and the tree over H's ncols columns (a power of two), the root also to root_out.
+hipError_t launch_commit_tree(const uint64_t* part, uint32_t ksplit, uint32_t ncols, bool ring, uint64_t* h,
+                              uint32_t* nodes, hipStream_t st, int lg_block = 7, uint32_t* root_out = nullptr);
 
 }  // namespace lgs

@@ -9,6 +9,7 @@
 // a layer by up to g levels in LDS, so a tree over L leaves takes ceil(log2(L) / g) launches; the
 // first launch also hashes the leaves.
 #include "blake3_dev.hpp"
+#include "gl.hpp"
 #include "kernels.hpp"
 
 namespace lgs {
@@ -36,28 +37,14 @@ __device__ __forceinline__ void leaf_digest4(const uint64_t* __restrict__ M, uin
   }
 }
 
-__device__ __forceinline__ void merkle_reduce_body(const uint64_t* __restrict__ M, uint32_t rows, uint64_t rs,
-                                                   uint64_t cs, int from_leaves, uint32_t* __restrict__ nodes,
-                                                   uint64_t child_off, uint64_t child_count, int levels,
-                                                   uint32_t blk, uint32_t* sh, uint32_t* root_out = nullptr) {
-  const uint32_t node = threadIdx.x >> 2, c = threadIdx.x & 3, width = blockDim.x >> 2;
-  const uint64_t base = (uint64_t)blk * width;
-  uint32_t local_n = (uint32_t)((child_count - base) < width ? (child_count - base) : width);
-  if (node < local_n) {  // whole quads: a quad's four lanes share the node
-    uint32_t h0, h1;
-    const uint64_t at = (child_off + base + node) * 8;
-    if (from_leaves) {
-      leaf_digest4(M, rows, rs, cs, base + node, c, h0, h1);
-      nodes[at + c] = h0;
-      nodes[at + 4 + c] = h1;
-    } else {
-      h0 = nodes[at + c];
-      h1 = nodes[at + 4 + c];
-    }
-    sh[node * 8 + c] = h0;
-    sh[node * 8 + 4 + c] = h1;
-  }
-  uint64_t off = child_off, count = child_count, blk_base = base;
+// The levels of one workgroup's subtree: sh holds the digests of local_n nodes (node t at
+// sh[8 t .. 8 t + 7]) of a layer of `count` nodes stored at nodes[8 off ..], the first of them node
+// blk_base of that layer. Reduces by up to `levels` levels, writing every parent to its layer, and
+// root_out (if set) receives the root from the quad that computes it.
+__device__ __forceinline__ void merkle_block_levels(uint32_t* __restrict__ nodes, uint32_t* sh, uint64_t off,
+                                                    uint64_t count, uint64_t blk_base, uint32_t local_n, int levels,
+                                                    uint32_t* root_out) {
+  const uint32_t node = threadIdx.x >> 2, c = threadIdx.x & 3;
   for (int lv = 0; lv < levels && local_n > 1; lv++) {
     __syncthreads();
     const uint32_t half = local_n / 2;
@@ -87,6 +74,30 @@ __device__ __forceinline__ void merkle_reduce_body(const uint64_t* __restrict__ 
     root_out[c] = sh[c];
     root_out[4 + c] = sh[4 + c];
   }
+}
+
+__device__ __forceinline__ void merkle_reduce_body(const uint64_t* __restrict__ M, uint32_t rows, uint64_t rs,
+                                                   uint64_t cs, int from_leaves, uint32_t* __restrict__ nodes,
+                                                   uint64_t child_off, uint64_t child_count, int levels,
+                                                   uint32_t blk, uint32_t* sh, uint32_t* root_out = nullptr) {
+  const uint32_t node = threadIdx.x >> 2, c = threadIdx.x & 3, width = blockDim.x >> 2;
+  const uint64_t base = (uint64_t)blk * width;
+  uint32_t local_n = (uint32_t)((child_count - base) < width ? (child_count - base) : width);
+  if (node < local_n) {  // whole quads: a quad's four lanes share the node
+    uint32_t h0, h1;
+    const uint64_t at = (child_off + base + node) * 8;
+    if (from_leaves) {
+      leaf_digest4(M, rows, rs, cs, base + node, c, h0, h1);
+      nodes[at + c] = h0;
+      nodes[at + 4 + c] = h1;
+    } else {
+      h0 = nodes[at + c];
+      h1 = nodes[at + 4 + c];
+    }
+    sh[node * 8 + c] = h0;
+    sh[node * 8 + 4 + c] = h1;
+  }
+  merkle_block_levels(nodes, sh, child_off, child_count, base, local_n, levels, root_out);
 }
 
 __global__ __launch_bounds__(1024) void merkle_reduce_kernel(
@@ -121,14 +132,13 @@ static int ilog2_u64(uint64_t x) {
   return l;
 }
 
-hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs,
-                         uint64_t nleaves, uint32_t* nodes, hipStream_t st, int lg_block, uint32_t* root_out) {
-  if (lg_block < 6 || lg_block > 8) return hipErrorInvalidValue;
+// Launches over the layer of `count` nodes at nodes[8 off ..] (or the leaves, from_leaves) until
+// the root, with `done` of the tree's total_levels levels already reduced.
+static hipError_t merkle_rounds(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs, uint32_t* nodes,
+                                hipStream_t st, int lg_block, uint32_t* root_out, uint64_t off, uint64_t count,
+                                int total_levels, int done, bool from_leaves) {
   const uint32_t width = 1u << lg_block;
-  uint64_t off = 0, count = nleaves;
-  int total_levels = ilog2_u64(nleaves);
-  int done = 0;
-  bool first = true;
+  bool first = from_leaves;
   while (first || count > 1) {
     int lv = total_levels - done < lg_block ? total_levels - done : lg_block;
     uint32_t blocks = (uint32_t)((count + width - 1) / width);
@@ -145,6 +155,97 @@ hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t
     first = false;
   }
   return hipSuccess;
+}
+
+hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs,
+                         uint64_t nleaves, uint32_t* nodes, hipStream_t st, int lg_block, uint32_t* root_out) {
+  if (lg_block < 6 || lg_block > 8) return hipErrorInvalidValue;
+  return merkle_rounds(M, rows, rs, cs, nodes, st, lg_block, root_out, 0, nleaves, ilog2_u64(nleaves), 0, true);
+}
+
+// ---------------- K-split sum, leaf digests and subtrees in one launch ----------------
+// The commit's H (32 rows of ncols words) is the sum of its K-split partials part[s][r][col]
+// (mod p, or mod 2^64 for the Z_2^64 hash); its Merkle leaves are the columns of H. Workgroup b
+// takes the 2^LGW columns from b 2^LGW: the quad of column p sums rows 8 k + 2 c and 8 k + 2 c + 1
+// (k < 4) in lane c -- every partial read once, the loads of four splits in flight together --
+// writes them to H and to the column's LDS copy, then hashes the column's four 64-byte blocks
+// from LDS and reduces the workgroup's subtree. One launch instead of a sum kernel, whose H the
+// leaf kernel read back, and a leaf kernel of a quarter as many workgroups.
+constexpr int TREE_LGW = 5;
+constexpr uint32_t TREE_COL_WORDS = 66;  // 64 words of a column + 2: columns start 2 banks apart
+
+template <int LGW>
+__global__ __launch_bounds__(4 << LGW) void commit_tree_kernel(const uint64_t* __restrict__ part, uint32_t ksplit,
+                                                              uint32_t ncols, int ring, uint64_t* __restrict__ h,
+                                                              uint32_t* __restrict__ nodes, uint32_t* root_out) {
+  constexpr uint32_t W = 1u << LGW;
+  __shared__ uint32_t colw[W * TREE_COL_WORDS];
+  __shared__ uint32_t sh[W * 8];
+  const uint32_t q = threadIdx.x >> 2, c = threadIdx.x & 3;
+  const uint64_t base = (uint64_t)blockIdx.x * W, p = base + q;
+  const uint32_t local_n = (uint32_t)((ncols - base) < W ? (ncols - base) : W);
+  if (q < local_n) {
+    uint64_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t s0 = 0; s0 < ksplit; s0 += 4) {
+      uint64_t v[4][8];
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const uint32_t r = 8 * (i >> 1) + 2 * c + (i & 1);
+          v[j][i] = s0 + j < ksplit ? part[((uint64_t)(s0 + j) * 32 + r) * ncols + p] : 0;
+        }
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc[i] = ring ? acc[i] + v[j][i] : gl_add(acc[i], v[j][i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const uint32_t r = 8 * (i >> 1) + 2 * c + (i & 1);
+      h[(uint64_t)r * ncols + p] = acc[i];
+      colw[q * TREE_COL_WORDS + 2 * r] = (uint32_t)acc[i];
+      colw[q * TREE_COL_WORDS + 2 * r + 1] = (uint32_t)(acc[i] >> 32);
+    }
+  }
+  __syncthreads();
+  if (q < local_n) {  // leaf = blake3(serialize(column p)): 32 words = 4 blocks of 64 bytes
+    uint32_t h0, h1;
+    b3_init4(h0, h1, c);
+#pragma unroll
+    for (uint32_t b = 0; b < 4; b++) {
+      uint32_t m[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++) m[i] = colw[q * TREE_COL_WORDS + 16 * b + i];
+      b3_compress4(h0, h1, m, 64, (b == 0 ? B3_CHUNK_START : 0) | (b == 3 ? (B3_CHUNK_END | B3_ROOT) : 0), c);
+    }
+    nodes[p * 8 + c] = h0;
+    nodes[p * 8 + 4 + c] = h1;
+    sh[q * 8 + c] = h0;
+    sh[q * 8 + 4 + c] = h1;
+  }
+  merkle_block_levels(nodes, sh, 0, ncols, base, local_n, LGW, root_out);
+}
+
+hipError_t launch_commit_tree(const uint64_t* part, uint32_t ksplit, uint32_t ncols, bool ring, uint64_t* h,
+                              uint32_t* nodes, hipStream_t st, int lg_block, uint32_t* root_out) {
+  if (lg_block < 6 || lg_block > 8 || ncols == 0 || (ncols & (ncols - 1)) || ksplit == 0)
+    return hipErrorInvalidValue;
+  constexpr uint32_t W = 1u << TREE_LGW;
+  const int total = ilog2_u64(ncols);
+  const bool one = ncols <= W;  // the first launch reaches the root
+  hipLaunchKernelGGL(commit_tree_kernel<TREE_LGW>, dim3((ncols + W - 1) / W), dim3(4 * W), 0, st, part, ksplit,
+                     ncols, ring ? 1 : 0, h, nodes, one ? root_out : nullptr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || one) return e;
+  uint64_t off = 0, count = ncols;
+  for (int i = 0; i < TREE_LGW; i++) {
+    off += count;
+    count >>= 1;
+  }
+  // the rest in workgroups wide enough for all of it where it fits one (2^8 nodes)
+  const int rest = total - TREE_LGW, lg_top = rest > lg_block ? (rest < 8 ? rest : 8) : lg_block;
+  return merkle_rounds(nullptr, 0, 0, 0, nodes, st, lg_top, root_out, off, count, total, TREE_LGW, false);
 }
 
 }  // namespace lgs

@@ -39,6 +39,30 @@ def test_commit_matches_oracle(gpu_ctx, shape, variant):
     gpu_ctx.set_option("hash", 0)
 
 
+@pytest.mark.parametrize("shape", [(1, 3), (2, 4), (3, 5), (4, 11)])
+def test_tree_and_split_sum_small_and_odd_splits(gpu_ctx, shape):
+    """The K-split sum fused into the tree's first launch: trees of 16 and 32 leaves (one
+    workgroup reaches the root), 64 and 4096 leaves (a second launch), every K-split count."""
+    log_m, log_n = shape
+    inp = synth.Inputs(log_m, log_n, seed=90 + log_n)
+    pk, _ = Ligesis.setup(log_m, log_n, inp.a, ctx=gpu_ctx, commit_only=True)
+    ocd = oracle.CommitData(oracle.ProvingKey(log_m, log_n, inp.a, commit_a=False), inp.f, keep_enc=True)
+    variants = (2,) if (2 << log_n) % 128 else (1, 2)
+    try:
+        for variant in variants:
+            splits = [1 << k for k in range(log_m + 1)] if variant == 2 else [1 << k for k in range(log_m)]
+            for ks in splits:
+                print(f"tree shape={shape} hash={variant} ksplit={ks}", flush=True)
+                gpu_ctx.set_option("hash", variant)
+                gpu_ctx.set_option("hash_ksplit", ks)
+                root, cd = Ligesis.commit(pk, inp.f)
+                np.testing.assert_array_equal(cd.digests(), ocd.h())
+                assert root == ocd.root, (variant, ks)
+    finally:
+        gpu_ctx.set_option("hash", 0)
+        gpu_ctx.set_option("hash_ksplit", 0)
+
+
 @pytest.mark.parametrize("shape", [(1, 17), (0, 19)])
 def test_long_rows_match_oracle(gpu_ctx, shape):
     """Rows of 2^18 and 2^20 points: LDS blocks of 2^13, then one and three global radix-2 stages
