@@ -67,6 +67,29 @@ void finish_profile(lgs::Context& ctx) {
   ctx.stage_ms[4] = all;
 }
 
+uint32_t next_root_seq(lgs::Context& ctx) {
+  ctx.root_seq = ctx.root_seq == 0xffffffffu ? 1 : ctx.root_seq + 1;  // never 0 (no completion word)
+  return ctx.root_seq;
+}
+
+// Waits until the commit's tree has stored `seq` into the completion word (after the root, with
+// system-scope release ordering: the commit's last memory access) -- a few microseconds sooner
+// than a stream synchronization, which waits for the kernel's completion signal. The stream is
+// polled now and then so that a failed launch raises its error instead of waiting forever.
+void wait_root(lgs::Context& ctx, const uint32_t* word, uint32_t seq) {
+  for (uint32_t i = 1;; i++) {
+    if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return;
+    if ((i & 4095) == 0) {
+      const hipError_t e = hipStreamQuery(ctx.stream);
+      if (e == hipSuccess) {
+        if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return;
+        throw lgs::Error(LGS_ERR_DEVICE, "commit: the stream finished without the root's completion word");
+      }
+      if (e != hipErrorNotReady) lgs::hip_check(e, "commit");
+    }
+  }
+}
+
 void read_root(lgs::Context& ctx, lgs::Commitment* const* cs, uint32_t count) {
   uint8_t* h = ctx.staging(32 * (size_t)count);
   for (uint32_t i = 0; i < count; i++) {
@@ -377,12 +400,19 @@ int lgs_commit(lgs_ctx* ctx, const lgs_key* key, const uint64_t* f, uint64_t len
     auto* c = new lgs_commitment();
     try {
       prepare_commitment(ctx->c, key->k, f, len, flags, c->c);
-      // the tree's last launch writes the root straight into pinned host memory
-      uint8_t* h = ctx->c.staging(32);
+      // the tree's last launch writes the root straight into pinned host memory, then (unprofiled)
+      // a completion word that is waited for instead of the stream
+      uint8_t* h = ctx->c.staging(64);
       c->c.root_host = (uint32_t*)h;
+      c->c.root_seq = ctx->c.profile ? 0 : next_root_seq(ctx->c);
       lgs::enqueue_commit(ctx->c, key->k, c->c);
       c->c.root_host = nullptr;
-      lgs::hip_check(hipStreamSynchronize(ctx->c.stream), "commit sync");
+      if (c->c.root_seq) {
+        wait_root(ctx->c, (const uint32_t*)h + 8, c->c.root_seq);
+        c->c.device_done = true;
+      } else {
+        lgs::hip_check(hipStreamSynchronize(ctx->c.stream), "commit sync");
+      }
       std::memcpy(c->c.root, h, 32);
       finish_profile(ctx->c);
     } catch (...) {
@@ -550,7 +580,7 @@ int lgs_prove_batch(lgs_ctx* ctx, uint32_t count, const lgs_key* const* keys, co
 void lgs_commitment_release(lgs_commitment* c) {
   if (!c) return;
   lgs::Context* cx = c->c.ctx;
-  if (cx) (void)hipStreamSynchronize(cx->stream);
+  if (cx && !c->c.device_done) (void)hipStreamSynchronize(cx->stream);
   delete c;
   ctx_unref(cx);
 }

@@ -287,7 +287,8 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
     if (ctx.profile) hip_check(hipEventRecord(ctx.ev[3], sh), "event");
     hip_check(hipEventRecord(ctx.pipe_event(9), sh), "pipeline join");
     hip_check(hipStreamWaitEvent(ctx.stream, ctx.pipe_event(9), 0), "pipeline join");
-    hip_check(launch_merkle(c.h.ptr, 32, two_n, 1, two_n, c.tree.nodes.ptr, ctx.stream, tu.merkle_lg, c.root_host),
+    hip_check(launch_merkle(c.h.ptr, 32, two_n, 1, two_n, c.tree.nodes.ptr, ctx.stream, tu.merkle_lg, c.root_host,
+                            c.root_seq),
               "merkle");
     if (ctx.profile) hip_check(hipEventRecord(ctx.ev[4], ctx.stream), "event");
     return;
@@ -299,6 +300,7 @@ void enqueue_commit(Context& ctx, const Key& key, Commitment& c, const Tuning* f
   io.h = c.h.ptr;
   io.nodes = c.tree.nodes.ptr;
   io.root_host = c.root_host;
+  io.root_seq = c.root_seq;
   io.tw = tw;
   io.tp = tp;
   // freed when this returns: later allocations on the context reuse it in stream order
@@ -371,10 +373,11 @@ void enqueue_commit_stages(Context& ctx, const Key& key, const CommitIO& io, con
   // column-wise Merkle tree over serialize(H[:, j]) (ligesis.rs:306-313)
   if (fused)
     hip_check(launch_commit_tree(io.part, hp.ks, (uint32_t)two_n, key.ring64, io.h, io.nodes, st, tu.merkle_lg,
-                                 io.root_host),
+                                 io.root_host, io.root_seq),
               "hash sum and merkle");
   else
-    hip_check(launch_merkle(io.h, 32, two_n, 1, two_n, io.nodes, st, tu.merkle_lg, io.root_host), "merkle");
+    hip_check(launch_merkle(io.h, 32, two_n, 1, two_n, io.nodes, st, tu.merkle_lg, io.root_host, io.root_seq),
+              "merkle");
   if (every) hip_check(hipEventRecord(ctx.ev[4], st), "event");
 }
 

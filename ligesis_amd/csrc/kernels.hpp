@@ -105,14 +105,16 @@ hipError_t launch_merkle_group(const MerkleJob* jobs, int njobs, uint32_t total_
 // merkle.hip: tree over nleaves (a power of two) leaves, leaf p = M[r*rs + p*cs], r < rows.
 // lg_block: log2 nodes per workgroup (6..8; 4 threads per node) = levels reduced per launch.
 // root_out (optional, device-accessible, e.g. pinned host memory) receives the root's 8 words
-// from the launch that computes it.
+// from the launch that computes it, and then, if root_seq is not zero, root_seq in root_out[8]
+// (stored with system-scope release ordering after the root: the tree's last store).
 hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs,
                          uint64_t nleaves, uint32_t* nodes, hipStream_t st, int lg_block = 7,
-                         uint32_t* root_out = nullptr);
+                         uint32_t* root_out = nullptr, uint32_t root_seq = 0);
 // A commit's finalize and column tree together: H = the sum of the ksplit partials that the
 // reducing hash kernels leave (one word per split, row and column; mod 2^64 when ring), written
 // to h, and the tree over H's ncols columns (a power of two), the root also to root_out.
 hipError_t launch_commit_tree(const uint64_t* part, uint32_t ksplit, uint32_t ncols, bool ring, uint64_t* h,
-                              uint32_t* nodes, hipStream_t st, int lg_block = 7, uint32_t* root_out = nullptr);
+                              uint32_t* nodes, hipStream_t st, int lg_block = 7, uint32_t* root_out = nullptr,
+                              uint32_t root_seq = 0);
 
 }  // namespace lgs

@@ -40,10 +40,12 @@ __device__ __forceinline__ void leaf_digest4(const uint64_t* __restrict__ M, uin
 // The levels of one workgroup's subtree: sh holds the digests of local_n nodes (node t at
 // sh[8 t .. 8 t + 7]) of a layer of `count` nodes stored at nodes[8 off ..], the first of them node
 // blk_base of that layer. Reduces by up to `levels` levels, writing every parent to its layer, and
-// root_out (if set) receives the root from the quad that computes it.
+// root_out (if set) receives the root from the quad that computes it, followed by root_seq (if not
+// zero) in root_out[8] once the root is visible to the host: a caller may wait for that word
+// instead of for the stream.
 __device__ __forceinline__ void merkle_block_levels(uint32_t* __restrict__ nodes, uint32_t* sh, uint64_t off,
                                                     uint64_t count, uint64_t blk_base, uint32_t local_n, int levels,
-                                                    uint32_t* root_out) {
+                                                    uint32_t* root_out, uint32_t root_seq = 0) {
   const uint32_t node = threadIdx.x >> 2, c = threadIdx.x & 3;
   for (int lv = 0; lv < levels && local_n > 1; lv++) {
     __syncthreads();
@@ -73,13 +75,18 @@ __device__ __forceinline__ void merkle_block_levels(uint32_t* __restrict__ nodes
   if (root_out && count == 1 && node == 0) {  // this launch reached the root (quad of node 0)
     root_out[c] = sh[c];
     root_out[4 + c] = sh[4 + c];
+    // the root's stores, then (a system-scope release) the completion word: the last store of
+    // the commit's last launch, the nodes of every level having been stored before the level's
+    // barriers
+    if (root_seq && c == 0) __hip_atomic_store(root_out + 8, root_seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
 __device__ __forceinline__ void merkle_reduce_body(const uint64_t* __restrict__ M, uint32_t rows, uint64_t rs,
                                                    uint64_t cs, int from_leaves, uint32_t* __restrict__ nodes,
                                                    uint64_t child_off, uint64_t child_count, int levels,
-                                                   uint32_t blk, uint32_t* sh, uint32_t* root_out = nullptr) {
+                                                   uint32_t blk, uint32_t* sh, uint32_t* root_out = nullptr,
+                                                   uint32_t root_seq = 0) {
   const uint32_t node = threadIdx.x >> 2, c = threadIdx.x & 3, width = blockDim.x >> 2;
   const uint64_t base = (uint64_t)blk * width;
   uint32_t local_n = (uint32_t)((child_count - base) < width ? (child_count - base) : width);
@@ -97,14 +104,16 @@ __device__ __forceinline__ void merkle_reduce_body(const uint64_t* __restrict__ 
     sh[node * 8 + c] = h0;
     sh[node * 8 + 4 + c] = h1;
   }
-  merkle_block_levels(nodes, sh, child_off, child_count, base, local_n, levels, root_out);
+  merkle_block_levels(nodes, sh, child_off, child_count, base, local_n, levels, root_out, root_seq);
 }
 
 __global__ __launch_bounds__(1024) void merkle_reduce_kernel(
     const uint64_t* __restrict__ M, uint32_t rows, uint64_t rs, uint64_t cs, int from_leaves,
-    uint32_t* __restrict__ nodes, uint64_t child_off, uint64_t child_count, int levels, uint32_t* root_out) {
+    uint32_t* __restrict__ nodes, uint64_t child_off, uint64_t child_count, int levels, uint32_t* root_out,
+    uint32_t root_seq) {
   __shared__ uint32_t sh[256 * 8];
-  merkle_reduce_body(M, rows, rs, cs, from_leaves, nodes, child_off, child_count, levels, blockIdx.x, sh, root_out);
+  merkle_reduce_body(M, rows, rs, cs, from_leaves, nodes, child_off, child_count, levels, blockIdx.x, sh, root_out,
+                     root_seq);
 }
 
 // One reduction round of many trees in one launch (batched commit): 256 nodes per workgroup.
@@ -135,8 +144,8 @@ static int ilog2_u64(uint64_t x) {
 // Launches over the layer of `count` nodes at nodes[8 off ..] (or the leaves, from_leaves) until
 // the root, with `done` of the tree's total_levels levels already reduced.
 static hipError_t merkle_rounds(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs, uint32_t* nodes,
-                                hipStream_t st, int lg_block, uint32_t* root_out, uint64_t off, uint64_t count,
-                                int total_levels, int done, bool from_leaves) {
+                                hipStream_t st, int lg_block, uint32_t* root_out, uint32_t root_seq, uint64_t off,
+                                uint64_t count, int total_levels, int done, bool from_leaves) {
   const uint32_t width = 1u << lg_block;
   bool first = from_leaves;
   while (first || count > 1) {
@@ -144,7 +153,7 @@ static hipError_t merkle_rounds(const uint64_t* M, uint32_t rows, uint64_t rs, u
     uint32_t blocks = (uint32_t)((count + width - 1) / width);
     const bool last = total_levels - done <= lg_block;  // this launch writes the root
     hipLaunchKernelGGL(merkle_reduce_kernel, dim3(blocks), dim3(4 * width), 0, st, M, rows, rs, cs,
-                       first ? 1 : 0, nodes, off, count, lv, last ? root_out : nullptr);
+                       first ? 1 : 0, nodes, off, count, lv, last ? root_out : nullptr, root_seq);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     for (int i = 0; i < lv; i++) {
@@ -158,9 +167,11 @@ static hipError_t merkle_rounds(const uint64_t* M, uint32_t rows, uint64_t rs, u
 }
 
 hipError_t launch_merkle(const uint64_t* M, uint32_t rows, uint64_t rs, uint64_t cs,
-                         uint64_t nleaves, uint32_t* nodes, hipStream_t st, int lg_block, uint32_t* root_out) {
+                         uint64_t nleaves, uint32_t* nodes, hipStream_t st, int lg_block, uint32_t* root_out,
+                         uint32_t root_seq) {
   if (lg_block < 6 || lg_block > 8) return hipErrorInvalidValue;
-  return merkle_rounds(M, rows, rs, cs, nodes, st, lg_block, root_out, 0, nleaves, ilog2_u64(nleaves), 0, true);
+  return merkle_rounds(M, rows, rs, cs, nodes, st, lg_block, root_out, root_seq, 0, nleaves, ilog2_u64(nleaves), 0,
+                       true);
 }
 
 // ---------------- K-split sum, leaf digests and subtrees in one launch ----------------
@@ -177,7 +188,8 @@ constexpr uint32_t TREE_COL_WORDS = 66;  // 64 words of a column + 2: columns st
 template <int LGW>
 __global__ __launch_bounds__(4 << LGW) void commit_tree_kernel(const uint64_t* __restrict__ part, uint32_t ksplit,
                                                               uint32_t ncols, int ring, uint64_t* __restrict__ h,
-                                                              uint32_t* __restrict__ nodes, uint32_t* root_out) {
+                                                              uint32_t* __restrict__ nodes, uint32_t* root_out,
+                                                              uint32_t root_seq) {
   constexpr uint32_t W = 1u << LGW;
   __shared__ uint32_t colw[W * TREE_COL_WORDS];
   __shared__ uint32_t sh[W * 8];
@@ -224,18 +236,19 @@ __global__ __launch_bounds__(4 << LGW) void commit_tree_kernel(const uint64_t* _
     sh[q * 8 + c] = h0;
     sh[q * 8 + 4 + c] = h1;
   }
-  merkle_block_levels(nodes, sh, 0, ncols, base, local_n, LGW, root_out);
+  merkle_block_levels(nodes, sh, 0, ncols, base, local_n, LGW, root_out, root_seq);
 }
 
 hipError_t launch_commit_tree(const uint64_t* part, uint32_t ksplit, uint32_t ncols, bool ring, uint64_t* h,
-                              uint32_t* nodes, hipStream_t st, int lg_block, uint32_t* root_out) {
+                              uint32_t* nodes, hipStream_t st, int lg_block, uint32_t* root_out,
+                              uint32_t root_seq) {
   if (lg_block < 6 || lg_block > 8 || ncols == 0 || (ncols & (ncols - 1)) || ksplit == 0)
     return hipErrorInvalidValue;
   constexpr uint32_t W = 1u << TREE_LGW;
   const int total = ilog2_u64(ncols);
   const bool one = ncols <= W;  // the first launch reaches the root
   hipLaunchKernelGGL(commit_tree_kernel<TREE_LGW>, dim3((ncols + W - 1) / W), dim3(4 * W), 0, st, part, ksplit,
-                     ncols, ring ? 1 : 0, h, nodes, one ? root_out : nullptr);
+                     ncols, ring ? 1 : 0, h, nodes, one ? root_out : nullptr, root_seq);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || one) return e;
   uint64_t off = 0, count = ncols;
@@ -245,7 +258,7 @@ hipError_t launch_commit_tree(const uint64_t* part, uint32_t ksplit, uint32_t nc
   }
   // the rest in workgroups wide enough for all of it where it fits one (2^8 nodes)
   const int rest = total - TREE_LGW, lg_top = rest > lg_block ? (rest < 8 ? rest : 8) : lg_block;
-  return merkle_rounds(nullptr, 0, 0, 0, nodes, st, lg_top, root_out, off, count, total, TREE_LGW, false);
+  return merkle_rounds(nullptr, 0, 0, 0, nodes, st, lg_top, root_out, root_seq, off, count, total, TREE_LGW, false);
 }
 
 }  // namespace lgs

@@ -282,6 +282,7 @@ struct Context {
   // 4q + 3); prof_chunks = number of chunks of the last profiled commit (0: not pipelined)
   hipEvent_t tev[48] = {};  // [32..42]: phases of a profiled prove
   int prof_chunks = 0;
+  uint32_t root_seq = 0;  // last completion word handed to a commit's tree (wait_root)
   hipEvent_t timing_event(int i);
   // launch schedule of a commit of shape 2^log_m x 2^log_n
   Tuning resolve(uint32_t log_m, uint32_t log_n) const;
@@ -317,8 +318,13 @@ struct Commitment {
   DevTree tree;
   uint8_t root[32] = {0};
   // when set (host-mapped pinned memory), the tree's last launch also writes the root there,
-  // so reading it back needs no copy after the tree
+  // so reading it back needs no copy after the tree; and root_seq, if not zero, after it in
+  // root_host[8]
   uint32_t* root_host = nullptr;
+  uint32_t root_seq = 0;
+  // the commit's kernels have made their last memory access (its tree wrote the completion word
+  // and nothing was enqueued on the commitment since): release needs no stream wait
+  bool device_done = false;
   // device state of the last proof on ctx, replayed as a graph by later proofs (see prove.cpp)
   mutable std::shared_ptr<ProvePlan> plan;
   mutable uint32_t prove_calls = 0;
@@ -354,7 +360,8 @@ HashPlan hash_plan(const Key& key, const Tuning& tu);
 uint64_t hash_partial_bytes(const HashPlan& hp, uint64_t two_n);  // K-split partials of one commit
 // Device buffers of one commit: f (m x n), F' (m x 2n), H (32 x 2n), the tree's nodes
 // ((4n - 1) x 8 words), the hash partials (hash_partial_bytes), the transform's twiddle table and
-// (ntt_impl 4 only) its pass table; root_host, if set, receives the root from the tree's last launch.
+// (ntt_impl 4 only) its pass table; root_host, if set, receives the root from the tree's last launch
+// (and root_seq, if not zero, in root_host[8] after it).
 struct CommitIO {
   const uint64_t* f = nullptr;
   uint64_t* enc = nullptr;
@@ -364,6 +371,7 @@ struct CommitIO {
   const uint64_t* tw = nullptr;
   const uint64_t* tp = nullptr;
   uint32_t* root_host = nullptr;
+  uint32_t root_seq = 0;
 };
 // Row transform, hash, finalize and Merkle tree of one commit on stream st, into io's buffers;
 // allocates nothing.
