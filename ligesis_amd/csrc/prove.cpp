@@ -489,6 +489,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   for (int c = 0; c < 3; c++)
     if (states[c]->D != D) throw Error(1, "prove: Basefold domains differ");
 
+
   // RS(a-bar) (ligesis.rs:387), both planes zero-padded to 2n, transformed here rather than
   // taken from a-bar's Basefold codeword (the same transform when log_interleave is 0), so that
   // the sumchecks need not wait for the other streams
@@ -669,12 +670,11 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   hip_check(launch_sc_tail_group(tail, st), "sumcheck tails");
   mark(2, st);
 
-
-
   // ---- Stage B: shared-domain batched Basefold open (basefold.rs:679-856), second stream ----
-  // Queued right behind the commitments it opens, before Stage A's work on st, so that the
-  // second stream has its whole share as early as possible; its scratch (partial sums) is its
-  // own.
+  // Its inputs come from sb itself and from st's work before the a-bar event (the eq tables, the
+  // constants); its scratch (partial sums) is its own. Issued after Stage A: Stage A is the
+  // longer chain, and issued first, Stage B (about a quarter of the proof's host issue time)
+  // held back Stage A's start by more than it let Stage B start earlier.
   // l0 partial sums, alternating halves by round parity (each round reduces the previous one's)
   ef2* part_b = P.get<ef2>(2 * 3 * 1024);
   ef2* levels = P.get<ef2>(tot);
@@ -753,6 +753,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   enqueue_merkle_trees(ctx, fsrc, fjobs, ftables, sb, &P.arena);
   mark(3, sb);
   hip_check(hipEventRecord(ctx.ev[6], sb), "join");
+  // the openings read Stage B's levels and trees
   hip_check(hipStreamWaitEvent(st, ctx.ev[6], 0), "join");
   };
 

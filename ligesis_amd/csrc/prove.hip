@@ -10,14 +10,34 @@ namespace lgs {
 __device__ __forceinline__ ef2 ld_ef(const ef2* p, uint64_t i) { return p[i]; }
 __device__ __forceinline__ ef2 ef_one_minus(ef2 a) { return ef2{gl_sub(1, a.c0), gl_neg(a.c1)}; }
 
-// Sum over the 64 lanes of a wave (butterfly: every lane ends with the total).
+// The value of the lane that DPP control CTRL selects, 0 in rows outside ROW_MASK.
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ ef2 ef_dpp(ef2 v) {
+  const uint32_t w0 = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v.c0, CTRL, ROW_MASK, 0xF, false);
+  const uint32_t w1 = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v.c0 >> 32), CTRL, ROW_MASK, 0xF, false);
+  const uint32_t w2 = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v.c1, CTRL, ROW_MASK, 0xF, false);
+  const uint32_t w3 = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v.c1 >> 32), CTRL, ROW_MASK, 0xF, false);
+  return ef2{((uint64_t)w1 << 32) | w0, ((uint64_t)w3 << 32) | w2};
+}
+
+// Sum over the 64 lanes of a wave, returned in every lane; every lane must be active. Row sums
+// by DPP (quad permutations [1,0,3,2] and [2,3,0,1], half-row and row mirrors: every lane of a
+// row of 16 then holds the row's sum), then row 0 into row 1 and row 2 into row 3 (row_bcast15),
+// rows 0-1 into rows 2-3 (row_bcast31): lane 63 holds the total, read back as a scalar. Each step
+// reads a neighbour's registers directly; a shuffle through LDS (ds_bpermute) per step made a
+// reduction several times slower where a kernel waits for it.
 __device__ __forceinline__ ef2 wave_sum(ef2 v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const ef2 o{(uint64_t)__shfl_xor((unsigned long long)v.c0, off), (uint64_t)__shfl_xor((unsigned long long)v.c1, off)};
-    v = ef_add(v, o);
-  }
-  return v;
+  v = ef_add(v, ef_dpp<0xB1>(v));
+  v = ef_add(v, ef_dpp<0x4E>(v));
+  v = ef_add(v, ef_dpp<0x141>(v));
+  v = ef_add(v, ef_dpp<0x140>(v));
+  v = ef_add(v, ef_dpp<0x142, 0xA>(v));
+  v = ef_add(v, ef_dpp<0x143, 0xC>(v));
+  const uint32_t w0 = __builtin_amdgcn_readlane((int)(uint32_t)v.c0, 63);
+  const uint32_t w1 = __builtin_amdgcn_readlane((int)(uint32_t)(v.c0 >> 32), 63);
+  const uint32_t w2 = __builtin_amdgcn_readlane((int)(uint32_t)v.c1, 63);
+  const uint32_t w3 = __builtin_amdgcn_readlane((int)(uint32_t)(v.c1 >> 32), 63);
+  return ef2{((uint64_t)w1 << 32) | w0, ((uint64_t)w3 << 32) | w2};
 }
 
 // Block-wide sum of W extension values per thread (blockDim a multiple of 64, sh at least
@@ -61,6 +81,15 @@ __device__ __forceinline__ void red_job_body(const RedJob& r, ef2* sh) {
       for (uint32_t q = 0; q < 4; q++)
         if (q < wq) r.out[q0 + q] = acc[q];
     }
+  }
+}
+
+// The same reduction by one wave when the partials fit it (nblk <= 64): lane b takes block b's
+// values; no barrier. Every lane of the wave must be active.
+__device__ __forceinline__ void red_job_wave(const RedJob& r, uint32_t lane) {
+  for (uint32_t q = 0; q < r.w; q++) {
+    const ef2 v = wave_sum(lane < r.nblk ? r.part[(uint64_t)lane * r.w + q] : ef_zero());
+    if (lane == 0) r.out[q] = v;
   }
 }
 
@@ -823,79 +852,140 @@ __global__ __launch_bounds__(256) void bf_round3_kernel(BfRound a, uint64_t half
   if (threadIdx.x == 0) a.part[(uint64_t)c * gridDim.x + blockIdx.x] = acc[0];
 }
 
-// The last Basefold reduction rounds t0..R-1 (level t0 has at most 2^kBfTailLog entries) in
-// one workgroup of 512 threads, with the codeword, the three channels' m and eq tables in LDS.
-// Per round: thread i folds codeword pair (i, i + len/2) (written back in place and to the
-// level's place in global memory, for the Merkle trees and openings), and for i < len/4 adds
-// m_c[2i] eq_c[i] to channel c's sum and folds m_c and eq_c; reads and writes of m and eq are
-// separated by a barrier. The round sums (outputs only) stay per wave until the end. The
-// challenges and the twiddles of every round (tws[i] = tw[i 2^t0]; round t = t0 + k takes
-// g^((h - i) 2^t) = tws[(h - i) 2^k]) are read into LDS with the tables, so no round waits for a
-// global load.
+// Offset of round k's sum shares in bf_tail_kernel's share table: rounds j < k hold 3 half_j
+// shares each (half_j = len0 / 2^(j+2)), so sum_{j<k} 3 half_j = 3 (len0 / 2 - len0 / 2^(k+1)).
+__device__ __forceinline__ uint32_t bf_tail_share_off(uint32_t len0, uint32_t k) {
+  return 3 * (len0 / 2 - (len0 >> (k + 1)));
+}
+
+// One round of bf_tail_kernel (level of len entries, h = len / 2 codeword pairs, half = len / 4
+// pairs of each channel's m). The round's work items -- codeword pairs, then each channel's
+// pairs, every kind starting at a multiple of 64 so that a wave takes one kind -- are spread
+// over the workgroup (item x to thread x mod blockDim, up to three items per thread in the
+// first rounds), so a thread's chain of products is one item long once the level is small.
+// Codeword item i folds pair (i, i + h); channel item i stores its share m_c[2i] eq_c[i] of the
+// round's sum into `share`, folds m_c and (i < half / 2) eq_c. Reads before the barrier, writes
+// after it.
+__device__ __forceinline__ void bf_tail_round(uint32_t k, uint32_t len0, uint32_t tid, ef2 r, ef2* cw,
+                                              const uint64_t* tws, ef2 (*m)[1 << (kBfTailLog - 1)],
+                                              ef2 (*eq)[1 << (kBfTailLog - 2)], ef2* lvl, ef2* share) {
+  const uint32_t len = len0 >> k, h = len / 2, half = len / 4;
+  const uint32_t rc = (h + 63) & ~63u, rh = (half + 63) & ~63u, items = rc + 3 * rh;
+  constexpr int kPass = 3;  // items <= 512 + 3 * 256 = 1280 <= 3 * 512
+  ef2 v0[kPass], v1[kPass];
+#pragma unroll
+  for (int p = 0; p < kPass; p++) {
+    const uint32_t x = tid + (uint32_t)p * blockDim.x;
+    v0[p] = ef_zero();
+    v1[p] = ef_zero();
+    if (x >= items) continue;
+    if (x < rc) {
+      if (x < h) {  // eq_fold_codeword_at over the subgroup of order len
+        const ef2 a = cw[x], b = cw[x + h];
+        const uint64_t ginv = x == 0 ? 1 : gl_neg(tws[(h - x) << k]);
+        const ef2 pe = ef_mul_base(ef_add(a, b), GL_INV2);
+        const ef2 po = ef_mul_base(ef_sub(a, b), gl_mul(ginv, GL_INV2));
+        v0[p] = ef_add(ef_mul(ef_one_minus(r), pe), ef_mul(r, po));
+      }
+    } else {
+      const uint32_t y = x - rc, c = y / rh, i = y % rh;
+      if (i < half) {
+        const ef2 e = m[c][2 * i], o = m[c][2 * i + 1];
+        share[c * half + i] = ef_mul(e, eq[c][i]);
+        v0[p] = ef_add(e, ef_mul(r, ef_sub(o, e)));
+        if (i < half / 2) v1[p] = ef_add(eq[c][2 * i], eq[c][2 * i + 1]);
+      }
+    }
+  }
+  __syncthreads();  // every read of this round's tables is done
+#pragma unroll
+  for (int p = 0; p < kPass; p++) {
+    const uint32_t x = tid + (uint32_t)p * blockDim.x;
+    if (x >= items) continue;
+    if (x < rc) {
+      if (x < h) {
+        cw[x] = v0[p];
+        lvl[x] = v0[p];
+      }
+    } else {
+      const uint32_t y = x - rc, c = y / rh, i = y % rh;
+      if (i < half) m[c][i] = v0[p];
+      if (i < half / 2) eq[c][i] = v1[p];
+    }
+  }
+  __syncthreads();
+}
+
+// The last Basefold reduction rounds t0..R-1 (level t0 has len0 <= 2^kBfTailLog entries) in one
+// workgroup of 512 threads, with the codeword, the three channels' m and eq tables in LDS
+// (bf_tail_round per round). The round sums are outputs only (no later round reads them): the
+// rounds store their shares in LDS, and the sums are taken after the last round, in two levels
+// (chunks of 16 shares, then a chunk sum per round and channel) instead of a workgroup reduction
+// inside every round. The challenges and the twiddles of every round (tws[i] = tw[i 2^t0]; round
+// t = t0 + k takes g^((h - i) 2^t) = tws[(h - i) 2^k]) are read into LDS with the tables, so no
+// round waits for a global load. R - t0 <= kBfTailLog (the code rate is at least 1).
 __global__ __launch_bounds__(512) void bf_tail_kernel(BfTail a) {
+  constexpr uint32_t kShares = 3u << (kBfTailLog - 1), kChunk = 16;
   __shared__ ef2 cw[1 << kBfTailLog];
   __shared__ ef2 m[3][1 << (kBfTailLog - 1)];
   __shared__ ef2 eq[3][1 << (kBfTailLog - 2)];
-  __shared__ ef2 wp[kBfTailLog][8][3];
+  __shared__ ef2 share[kShares];
+  __shared__ ef2 csum[kShares / kChunk + 3 * kBfTailLog];
+  __shared__ ef2 wp[8][4];
   __shared__ ef2 rch[32];
   __shared__ uint64_t tws[1 << (kBfTailLog - 1)];
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  for (int c = 0; c < 3; c++) {  // the last grid round's sums
-    if (a.prev[c].nblk) red_job_body(a.prev[c], &wp[0][0][0]);
-    __syncthreads();
+  // the last grid round's sums: a wave per channel where its partials fit one wave (the usual
+  // case: a few blocks per channel), else one channel after the other on the whole block
+  if (a.prev[0].nblk <= 64 && a.prev[1].nblk <= 64 && a.prev[2].nblk <= 64) {
+    if (wave < 3 && a.prev[wave].nblk) red_job_wave(a.prev[wave], lane);
+  } else {
+    for (int c = 0; c < 3; c++) {
+      if (a.prev[c].nblk) red_job_body(a.prev[c], &wp[0][0]);
+      __syncthreads();
+    }
   }
-  const uint32_t len0 = 1u << (a.logD - a.t0);
+  const uint32_t len0 = 1u << (a.logD - a.t0), rounds = a.R - a.t0;
   for (uint32_t i = tid; i < len0; i += blockDim.x) cw[i] = a.cw_in[i];
   for (uint32_t i = tid; i < len0 / 2; i += blockDim.x) tws[i] = a.tw[(uint64_t)i << a.t0];
-  if (tid < a.R - a.t0) rch[tid] = a.chal[a.t0 + tid];
+  if (tid < rounds) rch[tid] = a.chal[a.t0 + tid];
   for (int c = 0; c < 3; c++) {
     for (uint32_t i = tid; i < len0 / 2; i += blockDim.x) m[c][i] = a.m_in[c][i];
     for (uint32_t i = tid; i < len0 / 4; i += blockDim.x) eq[c][i] = a.eq[c][i];
   }
   __syncthreads();
   ef2* lvl = a.lvl_out;
-  for (uint32_t t = a.t0; t < a.R; t++) {
-    const uint32_t k = t - a.t0, len = len0 >> k, h = len / 2, half = len / 4;
-    const ef2 r = rch[k];
-    ef2 cwv = ef_zero(), mv[3], ev[3], acc[3];
-    if (tid < h) {  // eq_fold_codeword_at over the subgroup of order len
-      const ef2 x = cw[tid], nx = cw[tid + h];
-      const uint64_t ginv = tid == 0 ? 1 : gl_neg(tws[(h - tid) << k]);
-      const ef2 pe = ef_mul_base(ef_add(x, nx), GL_INV2);
-      const ef2 po = ef_mul_base(ef_sub(x, nx), gl_mul(ginv, GL_INV2));
-      cwv = ef_add(ef_mul(ef_one_minus(r), pe), ef_mul(r, po));
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      acc[c] = ef_zero();
-      if (tid < half) {
-        const ef2 e = m[c][2 * tid], o = m[c][2 * tid + 1];
-        acc[c] = ef_mul(e, eq[c][tid]);
-        mv[c] = ef_add(e, ef_mul(r, ef_sub(o, e)));
-        if (tid < half / 2) ev[c] = ef_add(eq[c][2 * tid], eq[c][2 * tid + 1]);
-      }
-    }
-    __syncthreads();  // every read of this round's tables is done
-    if (tid < h) {
-      cw[tid] = cwv;
-      lvl[tid] = cwv;
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      if (tid < half) m[c][tid] = mv[c];
-      if (tid < half / 2) eq[c][tid] = ev[c];
-      acc[c] = wave_sum(acc[c]);
-      if (lane == 0) wp[k][wave][c] = acc[c];
-    }
-    lvl += h;
-    __syncthreads();
+  for (uint32_t k = 0; k < rounds; k++) {
+    bf_tail_round(k, len0, tid, rch[k], cw, tws, m, eq, lvl, share + bf_tail_share_off(len0, k));
+    lvl += len0 >> (k + 1);
   }
-  const uint32_t rounds = a.R - a.t0, nw = blockDim.x >> 6;
-  if (tid < 3 * rounds) {
-    const uint32_t k = tid / 3, c = tid % 3;
+  // level 1: thread x sums chunk j of round k, channel c (chunks of kChunk shares)
+  uint32_t x = tid, k = 0, nch = 0, base = 0;
+  for (; k < rounds; k++) {
+    const uint32_t half = len0 >> (k + 2);
+    nch = (half + kChunk - 1) / kChunk;
+    if (x < 3 * nch) break;
+    x -= 3 * nch;
+    base += 3 * nch;
+  }
+  if (k < rounds) {
+    const uint32_t half = len0 >> (k + 2), c = x / nch, j = x % nch;
+    const ef2* sp = share + bf_tail_share_off(len0, k) + c * half + j * kChunk;
+    const uint32_t n = half - j * kChunk < kChunk ? half - j * kChunk : kChunk;
     ef2 t = ef_zero();
-    for (uint32_t w = 0; w < nw; w++) t = ef_add(t, wp[k][w][c]);
-    a.l0[(uint64_t)c * a.R + a.t0 + k] = t;
+    for (uint32_t q = 0; q < n; q++) t = ef_add(t, sp[q]);
+    csum[base + c * nch + j] = t;
+  }
+  __syncthreads();
+  // level 2: thread 3 k + c sums round k's chunk sums of channel c
+  if (tid < 3 * rounds) {
+    const uint32_t kk = tid / 3, c = tid % 3;
+    uint32_t b = 0;
+    for (uint32_t q = 0; q < kk; q++) b += 3 * (((len0 >> (q + 2)) + kChunk - 1) / kChunk);
+    const uint32_t n = ((len0 >> (kk + 2)) + kChunk - 1) / kChunk;
+    ef2 t = ef_zero();
+    for (uint32_t q = 0; q < n; q++) t = ef_add(t, csum[b + c * n + q]);
+    a.l0[(uint64_t)c * a.R + a.t0 + kk] = t;
   }
 }
 
