@@ -253,7 +253,7 @@ __device__ __forceinline__ void sc_tail_body(const ScTailJob& jb, ef2 (*tab)[1 <
     for (uint32_t it = 0; it < (4u << kScTailLog) / 2 / kTailThreads; it++) {
       const uint32_t e = tid + it * kTailThreads, i = e >> 2;
       if (i >= half) break;
-      ef2 y[D];
+      ef2 y[D], flo = ef_zero(), fdiff = ef_zero();
 #pragma unroll
       for (int j = 0; j < D; j++) {
         if (m1 && j == 1) {
@@ -267,8 +267,12 @@ __device__ __forceinline__ void sc_tail_body(const ScTailJob& jb, ef2 (*tab)[1 <
         if (q >= 2) v = ef_add(v, diff);
         if (q == 3) v = ef_add(v, diff);
         y[j] = v;
-        if ((int)q == j) fv[it] = ef_add(lo, ef_mul(r, diff));
+        if ((int)q == j) {  // table q's pair, folded below: one product per thread, not one per table
+          flo = lo;
+          fdiff = diff;
+        }
       }
+      if ((int)q < D) fv[it] = ef_add(flo, ef_mul(r, fdiff));
       if ((int)q <= D) {
         ef2 prod = y[0];
 #pragma unroll
