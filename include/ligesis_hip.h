@@ -92,8 +92,8 @@ int lgs_set_option(lgs_ctx* ctx, const char* name, int64_t value);
 /* Stage times of the last profiled call, in milliseconds. Commit: [0] row NTT, [1] hash main
  * kernel, [2] hash finalize (no kernel between its two events where the K-split sum runs inside
  * the tree's first launch: the default kernels), [3] Merkle tree, [4] whole commit. Prove: [5]
- * host time to issue
- * the work, [6] host time to the proof's download; device timestamps from the first upload:
+ * host time to issue the work, [6] host time to the proof's download; device timestamps from
+ * the first upload:
  * [7] first sumcheck round starts, [8] last sumcheck launch ends, [9] the Basefold open on the
  * second stream ends, [10] openings gathered, [11] proof downloaded, and on the second stream
  * [12] its work starts, [13] B_I's commitment done, [14] a-bar's commitment done, [15] the
