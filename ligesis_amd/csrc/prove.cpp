@@ -278,9 +278,9 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   const ef2* B = (const ef2*)dconst;
   uint32_t* didx = (uint32_t*)(dconst + blob_bytes);
 
-  // ---- the second stream: B_I and its Basefold commitment, then a-bar's, then Stage B ----
-  // B_I (ligesis.rs:402-410) needs only F' and the column indices, and its commitment
-  // (ligesis.rs:411-416) is the longest piece of the second stream, so the stream forks here;
+  // ---- the second stream: B_I, then (issued after st's launches up to the sumchecks) B_I's and
+  // a-bar's Basefold commitments, then Stage B ----
+  // B_I (ligesis.rs:402-410) needs only F' and the column indices, so the stream forks here;
   // only Stage B reads the commitments (and the proof their roots), and Stage B joins st before
   // the openings are gathered. Every twiddle table either stream needs is made on st first.
   for (uint32_t t = 0; t < R; t++) (void)ctx.twiddle(ilog2(D >> t));
@@ -298,8 +298,6 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   ef2* bi_ef2 = P.get<ef2>(nbi / 2);
   hip_check(launch_bits_gather(cm.enc.ptr, two_n, didx, (uint32_t)s, W, bi, bi8, sb), "B_I gather");
   hip_check(hipEventRecord(ctx.ev[8], sb), "B_I ready");
-  enqueue_basefold_commit(ctx, bi_state, bi, false, nv_bi, li_bi, 1, sb);
-  mark(7, sb);
 
   ef2* res = P.get<ef2>(res_n);
   P.res = res;
@@ -477,17 +475,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
             "a-bar");
   hip_check(launch_dot(a_vec, eqz2, n, part, st), "y");
 
-  // ---- the Basefold commitment of a-bar (ligesis.rs:384) on the second stream, after B_I's,
-  //      once a-bar is there; then Stage B. (A third stream for it gained nothing alone and
-  //      costs hardware queues when several contexts share the device.) ----
-  hip_check(hipEventRecord(ctx.ev[9], st), "a-bar ready");
-  hip_check(hipStreamWaitEvent(sb, ctx.ev[9], 0), "a-bar ready");
-  enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec, true, log_n, li_a, 1, sb);
-  mark(8, sb);
-  const DevBasefold* states[3] = {&key.a_state, &bi_state, &av_state};
-  const uint64_t* polys[3] = {key.a.ptr, bi, (const uint64_t*)a_vec};
-  for (int c = 0; c < 3; c++)
-    if (states[c]->D != D) throw Error(1, "prove: Basefold domains differ");
+  hip_check(hipEventRecord(ctx.ev[9], st), "a-bar ready");  // a-bar's Basefold commitment waits for it
 
 
   // RS(a-bar) (ligesis.rs:387), both planes zero-padded to 2n, transformed here rather than
@@ -534,6 +522,22 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   pj.ra = ra;
   hip_check(launch_prep_group(pj, st), "sumcheck inputs");
   hip_check(launch_sel_y4(didx, eqr2, (uint32_t)s, ra, sel, res + o_y4, st), "sel, y4");
+
+  // ---- the Basefold commitments of B_I (ligesis.rs:411-416) and, once a-bar is there, of a-bar
+  //      (ligesis.rs:384) on the second stream. Issued after every launch that the sumchecks
+  //      wait for on st: the host issues a proof more slowly than the device runs it in its
+  //      first part, and st's chain up to the sumchecks is the longer one. (A third stream for
+  //      a-bar's gained nothing alone and costs hardware queues when several contexts share the
+  //      device.) ----
+  enqueue_basefold_commit(ctx, bi_state, bi, false, nv_bi, li_bi, 1, sb);
+  mark(7, sb);
+  hip_check(hipStreamWaitEvent(sb, ctx.ev[9], 0), "a-bar ready");
+  enqueue_basefold_commit(ctx, av_state, (const uint64_t*)a_vec, true, log_n, li_a, 1, sb);
+  mark(8, sb);
+  const DevBasefold* states[3] = {&key.a_state, &bi_state, &av_state};
+  const uint64_t* polys[3] = {key.a.ptr, bi, (const uint64_t*)a_vec};
+  for (int c = 0; c < 3; c++)
+    if (states[c]->D != D) throw Error(1, "prove: Basefold domains differ");
 
   // ---- the seven sumchecks, in lockstep ----
   // Every challenge is fixed before the first round (the oracle streams are inputs), so the
