@@ -665,6 +665,60 @@ __global__ __launch_bounds__(256) void ntt_cols_kernel(uint64_t* __restrict__ da
   }
 }
 
+// The same last K stages for K = 3 or 6 (S = 8 or 64), as radix-8 passes in registers: a
+// workgroup of 256 threads takes J = 256 / (S / 8) consecutive j of one row. Stages 1-3: a thread
+// loads the 8 values of one group g (t = 8 g .. 8 g + 7, runs of J consecutive words across the
+// lanes), twiddles them by w_N^(rev_K(t) j), and runs dft8 on them (they sit in bit-reversed order
+// within the group); for K = 3 that is the result. For K = 6 the groups go through LDS, and stages
+// 4-6 are one more dft8 per (j, group residue i) over positions i + 8 u, pretwiddled by
+// w_64^(i rev3(u)), written straight to position i + 8 u of the natural-order result. One general
+// product per value plus 7/8 of one (K = 6) instead of one per value and per radix-2 stage, and
+// one LDS round trip instead of K barriers.
+template <int K>
+__global__ __launch_bounds__(256) void ntt_cols8_kernel(uint64_t* __restrict__ data, int logN,
+                                                       const uint64_t* __restrict__ tw) {
+  static_assert(K == 3 || K == 6, "one or two radix-8 passes");
+  constexpr uint32_t S = 1u << K, G = S / 8, J = 256 / G;
+  constexpr uint32_t SP = S + 1;  // a column's LDS stride: odd, so the J columns fall in distinct banks
+  __shared__ uint64_t sm[K == 6 ? J * SP : 1];
+  const uint64_t T = uint64_t(1) << (logN - K);
+  const uint32_t jblocks = (uint32_t)(T / J);
+  const uint64_t row = blockIdx.x / jblocks;
+  const uint64_t j0 = (uint64_t)(blockIdx.x % jblocks) * J;
+  uint64_t* d = data + (row << logN);
+  const uint32_t half = 1u << (logN - 1);
+  const uint32_t jj = threadIdx.x % J, g = threadIdx.x / J;
+  const uint64_t j = j0 + jj;
+  uint64_t x[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const uint32_t t = 8 * g + i;
+    const uint64_t v = d[(uint64_t)t * T + j];
+    const uint64_t cj = (uint64_t)brev_bits(t, K) * j;  // < 2^logN
+    x[i] = cj ? gl_mul(v, tw_full(tw, (uint32_t)cj, half, 0)) : v;
+  }
+  dft8(x);
+  if (K == 3) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) d[(uint64_t)i * T + j] = x[i];
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) sm[jj * SP + 8 * g + i] = x[i];
+  __syncthreads();
+  const uint32_t r = g;  // this thread's residue: positions r + 8 u
+#pragma unroll
+  for (int u = 0; u < 8; u++) x[u] = sm[jj * SP + r + 8 * u];
+#pragma unroll
+  for (int u = 1; u < 8; u++) {
+    const uint32_t e = r * (__brev((uint32_t)u) >> 29);  // w_64^(r rev3(u)), e < 64
+    if (e) x[u] = gl_mul(x[u], tw_full(tw, e << (logN - 6), half, 0));
+  }
+  dft8(x);
+#pragma unroll
+  for (int u = 0; u < 8; u++) d[(uint64_t)(r + 8 * u) * T + j] = x[u];
+}
+
 // One global radix-2 stage s (> lt) over every row, in place.
 __global__ __launch_bounds__(256) void ntt_global_stage_kernel(uint64_t* __restrict__ data,
                                                                uint32_t rows, int logN, int s,
@@ -751,9 +805,31 @@ hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_i
     }
     return e;
   }
-  // Longer than one LDS tile: 2^lt-point transforms of strided subsequences in LDS (lt = 10 while
-  // that leaves at most 7 stages, so that few rows still give many workgroups), global radix-2
-  // stages while more than 7 remain, then the last K <= 7 stages in one column pass.
+  // Longer than one LDS tile: 2^lt-point transforms of strided subsequences in LDS, then the last
+  // K stages in one column pass. For 2^14 .. 2^18 points and enough rows to give at least 512
+  // subsequence workgroups, K = 3 or 6: one radix-8 column pass in registers (ntt_cols8_kernel;
+  // at 2^19 points its 2^13-point subsequence transforms measured slower than 2^12-point ones
+  // followed by 7 radix-2 column stages);
+  // otherwise lt = 10 while that leaves at most 7 stages (so that few rows still give many
+  // workgroups), global radix-2 stages while more than 7 remain, and the last K <= 7 stages in
+  // the radix-2 column pass.
+  const int K8 = logN - LGS_LDS_LOG <= 3 ? 3 : 6;
+  if (logN >= 14 && logN <= 18 && ((uint64_t)rows << K8) >= 512) {
+    const int lt8 = logN - K8;
+    const uint32_t T = 1u << lt8, nblk = 1u << K8;
+    const uint32_t bt = T / 8;  // one radix-8 butterfly per thread and pass (1024 threads at 2^13)
+    hipLaunchKernelGGL(ntt_r8_blocks_kernel, dim3(rows * nblk), dim3(bt), ntt_lds_bytes(T), st, in, rs, es, n_in,
+                       logN, lt8, tw, out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (K8 == 3)
+      hipLaunchKernelGGL(ntt_cols8_kernel<3>, dim3((uint32_t)(((uint64_t)rows << (logN - 3)) / 256)), dim3(256), 0,
+                         st, out, logN, tw);
+    else
+      hipLaunchKernelGGL(ntt_cols8_kernel<6>, dim3((uint32_t)(((uint64_t)rows << (logN - 6)) / 32)), dim3(256), 0,
+                         st, out, logN, tw);
+    return hipGetLastError();
+  }
   const int K = logN - 10 < 7 ? logN - 10 : 7;
   const int lt = logN - K > LGS_LDS_LOG ? LGS_LDS_LOG : logN - K;
   const uint32_t T = 1u << lt, nblk = 1u << (logN - lt);
