@@ -665,22 +665,23 @@ __global__ __launch_bounds__(256) void ntt_cols_kernel(uint64_t* __restrict__ da
   }
 }
 
-// The same last K stages for K = 3 or 6 (S = 8 or 64), as radix-8 passes in registers: a
-// workgroup of 256 threads takes J = 256 / (S / 8) consecutive j of one row. Stages 1-3: a thread
-// loads the 8 values of one group g (t = 8 g .. 8 g + 7, runs of J consecutive words across the
-// lanes), twiddles them by w_N^(rev_K(t) j), and runs dft8 on them (they sit in bit-reversed order
-// within the group); for K = 3 that is the result. For K = 6 the groups go through LDS, and stages
-// 4-6 are one more dft8 per (j, group residue i) over positions i + 8 u, pretwiddled by
-// w_64^(i rev3(u)), written straight to position i + 8 u of the natural-order result. One general
-// product per value plus 7/8 of one (K = 6) instead of one per value and per radix-2 stage, and
-// one LDS round trip instead of K barriers.
+// The same last K stages for K = 3, 6 or 7 (S = 8, 64 or 128) as radix-8 / radix-16 passes in
+// registers: a workgroup of 256 threads takes J = 256 / (S / 8) consecutive j of one row.
+// Stages 1-3: a thread loads the 8 values of one group g (t = 8 g .. 8 g + 7, runs of J
+// consecutive words across the lanes), twiddles them by w_N^(rev_K(t) j), and runs dft8 on them
+// (they sit in bit-reversed order within the group); for K = 3 that is the result. Otherwise the
+// groups go through LDS, and stages 4..K are one more DIT pass per (j, group residue r) over
+// positions r + 8 u: a dft8 (K = 6) or dft16 (K = 7) pretwiddled by w_{2^K}^(r rev(u)), written
+// straight to position r + 8 u of the natural-order result. One general product per value plus
+// 7/8 or 15/16 of one, against one per value and per radix-2 stage, and one barrier instead of K.
 template <int K>
 __global__ __launch_bounds__(256) void ntt_cols8_kernel(uint64_t* __restrict__ data, int logN,
                                                        const uint64_t* __restrict__ tw) {
-  static_assert(K == 3 || K == 6, "one or two radix-8 passes");
+  static_assert(K == 3 || K == 6 || K == 7, "radix-8 pass, then none, a radix-8 or a radix-16 pass");
   constexpr uint32_t S = 1u << K, G = S / 8, J = 256 / G;
   constexpr uint32_t SP = S + 1;  // a column's LDS stride: odd, so the J columns fall in distinct banks
-  __shared__ uint64_t sm[K == 6 ? J * SP : 1];
+  constexpr int U = K == 7 ? 16 : 8;  // the second pass's radix
+  __shared__ uint64_t sm[K == 3 ? 1 : J * SP];
   const uint64_t T = uint64_t(1) << (logN - K);
   const uint32_t jblocks = (uint32_t)(T / J);
   const uint64_t row = blockIdx.x / jblocks;
@@ -698,25 +699,29 @@ __global__ __launch_bounds__(256) void ntt_cols8_kernel(uint64_t* __restrict__ d
     x[i] = cj ? gl_mul(v, tw_full(tw, (uint32_t)cj, half, 0)) : v;
   }
   dft8(x);
-  if (K == 3) {
+  if constexpr (K == 3) {
 #pragma unroll
     for (int i = 0; i < 8; i++) d[(uint64_t)i * T + j] = x[i];
-    return;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; i++) sm[jj * SP + 8 * g + i] = x[i];
+    __syncthreads();
+    if (g < 8) {  // K = 7: the 8 residues of a column take half of its 16 threads
+      const uint32_t r = g;  // positions r + 8 u
+      uint64_t y[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) y[u] = sm[jj * SP + r + 8 * u];
+#pragma unroll
+      for (int u = 1; u < U; u++) {
+        const uint32_t e = r * (__brev((uint32_t)u) >> (32 - (K - 3)));  // w_{2^K}^(r rev(u)), e < 2^K
+        if (e) y[u] = gl_mul(y[u], tw_full(tw, e << (logN - K), half, 0));
+      }
+      if constexpr (K == 7) dft16(y);
+      else dft8(y);
+#pragma unroll
+      for (int u = 0; u < U; u++) d[(uint64_t)(r + 8 * u) * T + j] = y[u];
+    }
   }
-#pragma unroll
-  for (int i = 0; i < 8; i++) sm[jj * SP + 8 * g + i] = x[i];
-  __syncthreads();
-  const uint32_t r = g;  // this thread's residue: positions r + 8 u
-#pragma unroll
-  for (int u = 0; u < 8; u++) x[u] = sm[jj * SP + r + 8 * u];
-#pragma unroll
-  for (int u = 1; u < 8; u++) {
-    const uint32_t e = r * (__brev((uint32_t)u) >> 29);  // w_64^(r rev3(u)), e < 64
-    if (e) x[u] = gl_mul(x[u], tw_full(tw, e << (logN - 6), half, 0));
-  }
-  dft8(x);
-#pragma unroll
-  for (int u = 0; u < 8; u++) d[(uint64_t)(r + 8 * u) * T + j] = x[u];
 }
 
 // One global radix-2 stage s (> lt) over every row, in place.
@@ -806,28 +811,29 @@ hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_i
     return e;
   }
   // Longer than one LDS tile: 2^lt-point transforms of strided subsequences in LDS, then the last
-  // K stages in one column pass. For 2^14 .. 2^18 points and enough rows to give at least 512
-  // subsequence workgroups, K = 3 or 6: one radix-8 column pass in registers (ntt_cols8_kernel;
-  // at 2^19 points its 2^13-point subsequence transforms measured slower than 2^12-point ones
-  // followed by 7 radix-2 column stages);
-  // otherwise lt = 10 while that leaves at most 7 stages (so that few rows still give many
-  // workgroups), global radix-2 stages while more than 7 remain, and the last K <= 7 stages in
-  // the radix-2 column pass.
-  const int K8 = logN - LGS_LDS_LOG <= 3 ? 3 : 6;
-  if (logN >= 14 && logN <= 18 && ((uint64_t)rows << K8) >= 512) {
+  // K stages in one column pass. For 2^14 .. 2^19 points and enough rows to give at least 512
+  // subsequence workgroups, K = 3 (2^14 .. 2^16), 6 (2^17, 2^18) or 7 (2^19): one column pass in
+  // registers (ntt_cols8_kernel), the subsequence transforms at most 2^12 points past 2^16 (at
+  // 2^19, 2^13-point ones and 6 column stages measured slower than 2^12 and 7); otherwise
+  // lt = 10 while that leaves at most 7 stages (so that few rows still give many workgroups),
+  // global radix-2 stages while more than 7 remain, and the last K <= 7 stages in the radix-2
+  // column pass.
+  const int K8 = logN <= 16 ? 3 : logN <= 18 ? 6 : 7;
+  if (logN >= 14 && logN <= 19 && ((uint64_t)rows << K8) >= 512) {
     const int lt8 = logN - K8;
     const uint32_t T = 1u << lt8, nblk = 1u << K8;
-    const uint32_t bt = T / 8;  // one radix-8 butterfly per thread and pass (1024 threads at 2^13)
+    const uint32_t bt = T / 8;  // one radix-8 butterfly per thread and pass
     hipLaunchKernelGGL(ntt_r8_blocks_kernel, dim3(rows * nblk), dim3(bt), ntt_lds_bytes(T), st, in, rs, es, n_in,
                        logN, lt8, tw, out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    const uint32_t cblocks = (uint32_t)(((uint64_t)rows << (logN - K8)) / (256 >> (K8 - 3)));
     if (K8 == 3)
-      hipLaunchKernelGGL(ntt_cols8_kernel<3>, dim3((uint32_t)(((uint64_t)rows << (logN - 3)) / 256)), dim3(256), 0,
-                         st, out, logN, tw);
+      hipLaunchKernelGGL(ntt_cols8_kernel<3>, dim3(cblocks), dim3(256), 0, st, out, logN, tw);
+    else if (K8 == 6)
+      hipLaunchKernelGGL(ntt_cols8_kernel<6>, dim3(cblocks), dim3(256), 0, st, out, logN, tw);
     else
-      hipLaunchKernelGGL(ntt_cols8_kernel<6>, dim3((uint32_t)(((uint64_t)rows << (logN - 6)) / 32)), dim3(256), 0,
-                         st, out, logN, tw);
+      hipLaunchKernelGGL(ntt_cols8_kernel<7>, dim3(cblocks), dim3(256), 0, st, out, logN, tw);
     return hipGetLastError();
   }
   const int K = logN - 10 < 7 ? logN - 10 : 7;

@@ -15,8 +15,9 @@ pytestmark = pytest.mark.gpu
 CASES = [(6, 0, 1, False), (10, 6, 1, False), (12, 4, 2, False), (16, 6, 1, False), (20, 6, 1, False),
          (9, 3, 1, True), (14, 6, 1, True), (13, 2, 3, False), (18, 5, 2, True),
          (9, 3, 3, False), (10, 6, 4, False), (16, 2, 4, False), (11, 4, 4, True),
-         # rows of 2^17 and 2^18 points with at least 8 rows: the radix-8 column pass of 6 stages
-         (22, 6, 1, False), (20, 3, 1, True)]
+         # rows of 2^17, 2^18 and 2^19 points over enough rows for the register column pass
+         # (radix-8 over 6 stages, radix-8 then radix-16 over 7)
+         (22, 6, 1, False), (20, 3, 1, True), (21, 3, 1, False)]
 
 
 @pytest.mark.parametrize("nv,li,code_rate,ext", CASES)
