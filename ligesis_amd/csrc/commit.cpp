@@ -172,7 +172,8 @@ void enqueue_basefold_commit(Context& ctx, DevBasefold& st, const uint64_t* poly
   st.poly_ptr = poly;
   // chunk j of the poly (basefold.rs:346, MlPoly::split) is poly[j :: 2^li]; an extension
   // element is two words, so word row rho = 2 j + c walks with stride 2^(li+1).
-  hip_check(launch_ntt(poly, 1, rows, (uint32_t)len_c, rows, logD, ctx.twiddle(logD), st.codes.ptr, s),
+  hip_check(launch_ntt(poly, 1, rows, (uint32_t)len_c, rows, logD, ctx.twiddle(logD), st.codes.ptr, s, 0, nullptr,
+                       logD > LGS_LDS_LOG ? ctx.pass_twiddle(13, 4) : nullptr),
             "basefold encode");
   if (encoded) hip_check(hipEventRecord(encoded, s), "basefold encoded");
   st.tree.leaves = st.D;

@@ -54,9 +54,13 @@ hipError_t launch_fill_field(uint64_t* p, uint64_t n, uint64_t seed, hipStream_t
 // impl 4 / 5: for 2^13-point rows of 2^12 values at unit stride, the compile-time-shaped radix-8 /
 // radix-16 kernel with the pass twiddle table tp (launch_pass_twiddles for that impl); other
 // shapes, or tp null, run impl 3.
+// tp13 (optional): the pass table of a 2^13-point transform (pass_twiddle_entries(13, 4)); with
+// it, rows of 2^16 points whose upper half is zero take their 2^13-point subsequence transforms
+// on the compile-time-shaped kernel (at 2^19 points, 2^13-point subsequences measured slower
+// than 2^12-point ones whatever the kernel).
 hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_in, uint32_t rows,
                       int logN, const uint64_t* tw, uint64_t* out, hipStream_t st, int impl = 0,
-                      const uint64_t* tp = nullptr);
+                      const uint64_t* tp = nullptr, const uint64_t* tp13 = nullptr);
 // Entries of the pass twiddle table of a 2^L-point transform for row-transform impl 4 (radix-8
 // passes from s = 4, L = 1 mod 3) or 5 (radix-16 passes from s = 5, L = 1 mod 4), 0 for other
 // impls and shapes, and its fill from the transform's twiddle table tw (entries in Montgomery

@@ -345,7 +345,7 @@ __device__ __forceinline__ uint64_t& lds_word(uint64_t* sm, uint32_t b) {
 // offsets t 2^s occupy disjoint bits, so lds_pos(base + t 2^s) = lds_pos(base) ^ lds_pos(t 2^s).
 template <int L>
 __device__ __forceinline__ void ntt_r8_fixed_row(const uint64_t* __restrict__ src, const uint64_t* __restrict__ tp,
-                                                 uint64_t* __restrict__ dst, uint64_t* sm) {
+                                                 uint64_t* __restrict__ dst, uint64_t* sm, uint64_t ss = 1) {
   static_assert((L - 1) % 3 == 0 && L >= 10 && L <= LGS_LDS_LOG, "radix-8 passes from s = 4");
   constexpr uint32_t N = 1u << L, E = N / 16;
   constexpr int sh = L - 5;
@@ -354,7 +354,7 @@ __device__ __forceinline__ void ntt_r8_fixed_row(const uint64_t* __restrict__ sr
     const uint32_t k = gi & (E - 1), j = gi / E;  // j uniform per half of the workgroup
     uint64_t x[8];
 #pragma unroll
-    for (int t = 0; t < 8; t++) x[t] = src[k + (__brev((uint32_t)t) >> 29) * E];
+    for (int t = 0; t < 8; t++) x[t] = src[(k + (__brev((uint32_t)t) >> 29) * E) * ss];
     if (j) {
 #pragma unroll
       for (int t = 1; t < 8; t++) x[t] = gl_mul_pow2(x[t], (156u * (__brev((uint32_t)t) >> 29)) % 192u);
@@ -396,6 +396,25 @@ __global__ __launch_bounds__(1 << (L - 3)) void ntt_r8_fixed_kernel(const uint64
                                                                    uint64_t* __restrict__ out) {
   extern __shared__ uint64_t sm[];
   ntt_r8_fixed_row<L>(in + (uint64_t)blockIdx.x * rs, tp, out + ((uint64_t)blockIdx.x << L), sm);
+}
+
+// The first pass of a long row transform (2^logN points, the first 2^(logN - 1) of them values)
+// with 2^L-point subsequence transforms: block b is subsequence c = rev(b mod 2^(logN - L)) of
+// row b / 2^(logN - L), whose 2^(L-1) values sit at stride 2^(logN - L) es; its DFT goes to block
+// b mod 2^(logN - L) of the bit-reversed-block layout that the column pass completes. The
+// compile-time-shaped row kernel with its pass table, in place of the runtime-shaped one.
+template <int L>
+__global__ __launch_bounds__(1 << (L - 3)) void ntt_r8_fixed_sub_kernel(const uint64_t* __restrict__ in, uint64_t rs,
+                                                                       uint64_t es, int logN,
+                                                                       const uint64_t* __restrict__ tp,
+                                                                       uint64_t* __restrict__ out) {
+  extern __shared__ uint64_t sm[];
+  const uint32_t nblk = 1u << (logN - L);
+  const uint64_t row = blockIdx.x / nblk;
+  const uint32_t blk = blockIdx.x % nblk;
+  const uint32_t c = brev_bits(blk, logN - L);
+  ntt_r8_fixed_row<L>(in + row * rs + (uint64_t)c * es, tp, out + (row << logN) + ((uint64_t)blk << L), sm,
+                      (uint64_t)nblk * es);
 }
 
 // The rows of many commitments whose rows hold 2^(L-1) values at unit stride, in one launch
@@ -774,7 +793,7 @@ hipError_t launch_fill_field(uint64_t* p, uint64_t n, uint64_t seed, hipStream_t
 
 hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_in, uint32_t rows,
                       int logN, const uint64_t* tw, uint64_t* out, hipStream_t st, int impl,
-                      const uint64_t* tp) {
+                      const uint64_t* tp, const uint64_t* tp13) {
   if (rows == 0) return hipSuccess;
   if (impl == 4 || impl == 5) {
     if (logN == 13 && es == 1 && n_in == (1u << 12) && tp) {
@@ -823,8 +842,12 @@ hipError_t launch_ntt(const uint64_t* in, uint64_t rs, uint64_t es, uint32_t n_i
     const int lt8 = logN - K8;
     const uint32_t T = 1u << lt8, nblk = 1u << K8;
     const uint32_t bt = T / 8;  // one radix-8 butterfly per thread and pass
-    hipLaunchKernelGGL(ntt_r8_blocks_kernel, dim3(rows * nblk), dim3(bt), ntt_lds_bytes(T), st, in, rs, es, n_in,
-                       logN, lt8, tw, out);
+    if (tp13 && lt8 == 13 && n_in == (1u << (logN - 1)))  // zero upper half: the compile-time kernel
+      hipLaunchKernelGGL(ntt_r8_fixed_sub_kernel<13>, dim3(rows * nblk), dim3(1024), ntt_lds_bytes(T), st, in, rs, es,
+                         logN, tp13, out);
+    else
+      hipLaunchKernelGGL(ntt_r8_blocks_kernel, dim3(rows * nblk), dim3(bt), ntt_lds_bytes(T), st, in, rs, es, n_in,
+                         logN, lt8, tw, out);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const uint32_t cblocks = (uint32_t)(((uint64_t)rows << (logN - K8)) / (256 >> (K8 - 3)));

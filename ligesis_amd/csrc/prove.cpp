@@ -286,6 +286,7 @@ int64_t prove_impl(Context& ctx, const Key& key, const Commitment& cm, const uin
   for (uint32_t t = 0; t < R; t++) (void)ctx.twiddle(ilog2(D >> t));
   (void)ctx.twiddle(ilog2(D));
   (void)ctx.twiddle(logN);
+  (void)ctx.pass_twiddle(13, 4);  // the long Basefold encodings' subsequence transforms
   hip_check(hipEventRecord(ctx.ev[5], st), "fork");
   hip_check(hipStreamWaitEvent(sb, ctx.ev[5], 0), "fork");
   mark(6, sb);

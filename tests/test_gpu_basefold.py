@@ -15,9 +15,11 @@ pytestmark = pytest.mark.gpu
 CASES = [(6, 0, 1, False), (10, 6, 1, False), (12, 4, 2, False), (16, 6, 1, False), (20, 6, 1, False),
          (9, 3, 1, True), (14, 6, 1, True), (13, 2, 3, False), (18, 5, 2, True),
          (9, 3, 3, False), (10, 6, 4, False), (16, 2, 4, False), (11, 4, 4, True),
-         # rows of 2^17, 2^18 and 2^19 points over enough rows for the register column pass
-         # (radix-8 over 6 stages, radix-8 then radix-16 over 7)
-         (22, 6, 1, False), (20, 3, 1, True), (21, 3, 1, False)]
+         # rows of 2^16 - 2^19 points over enough rows for the register column pass (radix-8 over
+         # 3 or 6 stages, radix-8 then radix-16 over 7); at 2^16 with rate 1 the compile-time-shaped
+         # 2^13-point subsequence transforms before it
+         (22, 6, 1, False), (20, 3, 1, True), (21, 3, 1, False), (21, 6, 1, False), (20, 5, 1, True),
+         (19, 3, 2, False)]
 
 
 @pytest.mark.parametrize("nv,li,code_rate,ext", CASES)
