@@ -454,6 +454,16 @@ __global__ __launch_bounds__(256) void sel_y4_kernel(const uint32_t* __restrict_
 // variable is bound first).
 // M1: the second table is the first minus one and is not stored (its pair and fold follow from
 // the first table's).
+// A value that is the same in every lane, moved to scalar registers.
+__device__ __forceinline__ ef2 ef_uniform(ef2 x) {
+  auto u = [](uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+  };
+  return ef2{u(x.c0), u(x.c1)};
+}
+
 template <int D, bool M1>
 __device__ __forceinline__ void sc_round_body(ef2* __restrict__ p0, ef2* __restrict__ p1, ef2* __restrict__ p2,
                                               uint64_t half, const ef2* __restrict__ r_ptr, ef2* __restrict__ part,
@@ -535,23 +545,25 @@ __device__ __forceinline__ void sc_bits_body(const ScRoundJob& j, uint32_t blk, 
   const uint64_t half = j.half, stride = 2 * half, lmask = (uint64_t(1) << klo) - 1;
   const uint64_t base = (uint64_t)blk * blockDim.x * j.kpt;
   const uint64_t bh = base >> klo, boff = half >> klo;
-  ef2 Ha[NT], Hb[NT];
+  // The high factors are the same for the whole block: they are read again after the loop
+  // rather than held in vector registers across it, and their fold with r lives in scalar ones.
   const ef2* lo_tab[NT];
 #pragma unroll
-  for (int q = 0; q < NT; q++) {
-    Ha[q] = T->hi[q][bh];
-    Hb[q] = T->hi[q][bh + boff];
-    lo_tab[q] = T->lo[q];
-  }
-  const ef2 r = *j.r, one = ef_one();
+  for (int q = 0; q < NT; q++) lo_tab[q] = T->lo[q];
+  const ef2 one = ef_one();
   ef2* __restrict__ bout = j.p[0];
   ef2* __restrict__ eout = j.p[D - 1];
   ef2 Hr[NT];  // high factors folded with r (the next round's eq table; the last bits round)
 #pragma unroll
   for (int q = 0; q < NT; q++) Hr[q] = ef_zero();
+  ef2 r = ef_zero();
   if (bout) {
+    r = ef_uniform(*j.r);
 #pragma unroll
-    for (int q = 0; q < NT; q++) Hr[q] = ef_add(Ha[q], ef_mul(r, ef_sub(Hb[q], Ha[q])));
+    for (int q = 0; q < NT; q++) {
+      const ef2 ha = T->hi[q][bh], hb = T->hi[q][bh + boff];
+      Hr[q] = ef_uniform(ef_add(ha, ef_mul(r, ef_sub(hb, ha))));
+    }
   }
   // accumulators: F = 0: [C0, C1, C2] (round 0: [U]); F = 1: [A_q, B_q] per term
   constexpr int NA = F ? 2 * NT : (TAU ? 3 : 1);
@@ -617,6 +629,12 @@ __device__ __forceinline__ void sc_bits_body(const ScRoundJob& j, uint32_t blk, 
   }
   block_sum<NA>(acc, sh);
   if (threadIdx.x == 0) {
+    ef2 Ha[NT], Hb[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+      Ha[q] = T->hi[q][bh];
+      Hb[q] = T->hi[q][bh + boff];
+    }
     ef2 s[4];
     if (F == 0 && TAU == 0) {
       const ef2 L = ef_mul(Ha[0], acc[0]), H = ef_mul(Hb[0], acc[0]);
